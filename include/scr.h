@@ -1,0 +1,227 @@
+/*
+ * scr.h — C ABI of the MI355X compute point-cloud rasteriser (libscr.so).
+ *
+ * Every entry point replaces a piece of the Swift API of
+ * mnmly/Satin-ComputeRasteriser; the reference interface each one stands in
+ * for is cited beside it (paths relative to the reference repository root,
+ * under Sources/SatinComputeRasteriser/ unless stated otherwise).
+ *
+ * Conventions
+ *  - Plain C types only; matrices are float[16], column-major (m[c*4 + r]),
+ *    the same memory layout as simd_float4x4 / Metal float4x4.
+ *  - Every function returns an int status: SCR_OK (0) or a negative SCR_ERR_*
+ *    code; scr_last_error() returns the message of the last failure on a
+ *    context.  Reference preconditions (traps) become error codes.
+ *  - Host arrays passed in are copied before the call returns (the reference
+ *    memcpy's caller data into its pool, ComputeRasteriserPointCloud.swift:646-664);
+ *    a flag selects device-resident inputs where noted.
+ *  - One HIP stream per context.  Render/pack calls enqueue work and return;
+ *    scr_sync and the read functions synchronise.  A context is not
+ *    thread-safe (the reference is driven from one render thread).
+ */
+#ifndef SCR_H
+#define SCR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants: Pipelines/ComputeRasteriserTypes.h:4-9, ComputeRasteriserTypes.swift:4-6 */
+#define SCR_THREADS_PER_GROUP 128
+#define SCR_DEFAULT_POINTS_PER_BATCH 10240 /* computeRasteriserThreadsPerGroup * 80 */
+#define SCR_MAX_POINTS_PER_BATCH 65535     /* uint16 LOD prefix counts, PackedPointCloudFixtures.swift:75 */
+#define SCR_MAX_DEPTH 0xffffffffu
+#define SCR_FILES_RING_SLOTS 6             /* ComputeRasteriserPointCloud.swift:153-157 */
+
+/* ---- status codes */
+#define SCR_OK 0
+#define SCR_ERR_INVALID_ARGUMENT (-1)
+#define SCR_ERR_NO_FREE_SLOTS (-2)      /* addBatches precondition, ComputeRasteriserPointCloud.swift:419 */
+#define SCR_ERR_BATCH_TOO_LARGE (-3)    /* numPoints > slot capacity, :432 */
+#define SCR_ERR_POINTS_PER_BATCH (-4)   /* ppb outside 1..65535 */
+#define SCR_ERR_HIP (-5)
+#define SCR_ERR_OUT_OF_MEMORY (-6)
+#define SCR_ERR_SOURCE_TOO_SMALL (-7)   /* copySlice precondition, :657 */
+#define SCR_ERR_UNSUPPORTED (-8)
+#define SCR_ERR_SLOT_RANGE (-9)         /* removeBatches precondition, :462 */
+#define SCR_ERR_NO_DEVICE (-10)
+
+/* ---- GPU data layout: Pipelines/ComputeRasteriserTypes.h:11-63 (byte-identical) */
+typedef struct scr_raster_batch {
+    int32_t state;               /* 0 = slot not resident */
+    float minX, minY, minZ;
+    float maxX, maxY, maxZ;
+    uint32_t numPoints;
+    uint32_t firstPoint;         /* absolute pool point index = slot * pointsPerBatch */
+    uint32_t fileIndex;
+    uint32_t lodCum01, lodCum23, lodCum45, lodCum67; /* cum[L] = points with level <= L, 2 x u16 per word */
+    uint32_t padding7, padding8;
+} scr_raster_batch; /* 64 B */
+
+typedef struct scr_raster_file {
+    float transform[16];         /* viewProjection * world */
+    float transformFrustum[16];  /* (frustum ?? viewProjection) * world */
+    float world[16];             /* model * file.world */
+    float prevTransform[16];     /* previous viewProjection * world */
+} scr_raster_file; /* 256 B */
+
+typedef struct scr_visible_batch {
+    uint32_t batchIndex;
+    int32_t level;               /* position precision: 0 = 30-bit, 1 = 20-bit, >=2 = 10-bit */
+    float lodThreshold;
+    uint32_t activePoints;       /* LOD survivor prefix */
+} scr_visible_batch; /* 16 B */
+
+typedef struct scr_raster_pixel {
+    uint32_t depth, red, green, blue, count, weight;
+    uint32_t padding[2];
+} scr_raster_pixel; /* 32 B, readback form of the HQS accumulators */
+
+/* ---- configuration: ComputeRasteriserTypes.swift:8-142 */
+#define SCR_MODE_HIGH_QUALITY_AVERAGE 0
+#define SCR_MODE_NEAREST_POINT 1
+#define SCR_POINT_SIZE_SCREEN_SPACE 0
+#define SCR_POINT_SIZE_WORLD_SPACE 1
+
+typedef struct scr_config {
+    int32_t mode;
+    float depthTolerance;
+    float backgroundColor[4];
+    int32_t enableFrustumCulling;
+    int32_t lodBias;
+    int32_t enableCLOD;
+    int32_t enableLODDither;
+    int32_t colorizeChunks;
+    int32_t colorizeOverdraw;
+    int32_t pointSizeMode;
+    float minimumPointSize;
+    float maximumPointSize;
+    float pointSizeScale;
+    int32_t applyTint;
+    int32_t tintAlphaIsCoverage; /* translucent-defocus OIT: not supported, must be 0 */
+    float motionBlur;            /* motion blur: not supported, must be 0 */
+    int32_t reserved[5];
+} scr_config;
+
+/* Reference defaults, ComputeRasteriserTypes.swift:97-119. */
+void scr_config_default(scr_config* cfg);
+
+typedef struct scr_camera {
+    float viewMatrix[16];
+    float projectionMatrix[16];
+} scr_camera;
+
+/* ---- opaque handles */
+typedef struct scr_ctx scr_ctx;
+typedef struct scr_cloud scr_cloud;
+typedef struct scr_frame scr_frame;
+
+/* ---- context */
+int scr_ctx_create(int device, scr_ctx** out);
+int scr_ctx_destroy(scr_ctx* ctx);
+const char* scr_last_error(const scr_ctx* ctx);
+int scr_sync(scr_ctx* ctx);
+/* Version string of the library build (never NULL). */
+const char* scr_version(void);
+
+/* ---- device memory helpers (for callers that keep inputs resident in HBM) */
+int scr_device_alloc(scr_ctx* ctx, size_t bytes, void** out);
+int scr_device_free(scr_ctx* ctx, void* ptr);
+int scr_memcpy_h2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
+int scr_memcpy_d2h(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- point cloud slot pool: ComputeRasteriserPointCloud.swift */
+
+/* init(context:capacity:files:originShift:) — ComputeRasteriserPointCloud.swift:201-217.
+ * files == NULL / nFiles == 0 gives one identity RasterFile. */
+int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch,
+                     const scr_raster_file* files, int nFiles, scr_cloud** out);
+int scr_cloud_destroy(scr_cloud* cloud);
+
+typedef struct scr_cloud_info {
+    int32_t maxResidentBatches;
+    int32_t pointsPerBatch;
+    int32_t residentBatchCount;
+    int32_t freeSlotCount;
+    int64_t pointCount;
+    int32_t nFiles;
+    int32_t hasTints;
+} scr_cloud_info;
+int scr_cloud_get_info(const scr_cloud* cloud, scr_cloud_info* out);
+
+/* addBatches(positionsXYZLow:…:batches:commit:) — ComputeRasteriserPointCloud.swift:410-452.
+ * Each batch i copies numPoints points starting at batches[i].firstPoint of the
+ * source arrays (nSourcePoints long) into a free slot (LIFO), rebases firstPoint
+ * to slot * pointsPerBatch and sets state = 1.  outSlots[i] receives the slot. */
+int scr_cloud_add_batches(scr_cloud* cloud, const uint32_t* xyzLow, const uint32_t* xyzMed,
+                          const uint32_t* xyzHigh, const uint32_t* colors, const uint8_t* levels,
+                          size_t nSourcePoints, const scr_raster_batch* batches, int nBatches,
+                          int commit, int32_t* outSlots);
+/* removeBatches(slots:commit:) — ComputeRasteriserPointCloud.swift:460-473 */
+int scr_cloud_remove_batches(scr_cloud* cloud, const int32_t* slots, int n, int commit);
+/* commitBatchUpdates() — ComputeRasteriserPointCloud.swift:479-481 */
+int scr_cloud_commit(scr_cloud* cloud);
+/* clearAllBatches() — ComputeRasteriserPointCloud.swift:367-377 */
+int scr_cloud_clear(scr_cloud* cloud);
+/* Object.worldMatrix of the cloud node (composed into every file's world). */
+int scr_cloud_set_model(scr_cloud* cloud, const float model[16]);
+/* updateFiles(viewProjection:modelMatrix:frustumTransform:prevViewProjection:)
+ * — ComputeRasteriserPointCloud.swift:487-515.  frustum / prevVP may be NULL. */
+int scr_cloud_update_files(scr_cloud* cloud, const float viewProjection[16], const float model[16],
+                           const float* frustum16, const float* prevViewProjection16);
+/* TintPass output (TintPass.swift:130-184): one float4 per pool point,
+ * nPoints == maxResidentBatches * pointsPerBatch.  onDevice != 0: rgba is a
+ * device pointer.  rgba == NULL drops the tint buffer. */
+int scr_cloud_set_tints(scr_cloud* cloud, const float* rgba, size_t nPoints, int onDevice);
+
+#define SCR_POOL_XYZ_LOW 0
+#define SCR_POOL_XYZ_MED 1
+#define SCR_POOL_XYZ_HIGH 2
+#define SCR_POOL_COLORS 3
+#define SCR_POOL_LEVELS 4
+#define SCR_POOL_BATCHES 5
+/* Read back a pool buffer (whole capacity) into host memory. */
+int scr_cloud_read_pool(scr_cloud* cloud, int which, void* host, size_t bytes);
+/* Read back the visible-batch list of the last render of this cloud
+ * (VisibleBatch records, cull order).  *count receives the visible count. */
+int scr_cloud_read_visible(scr_cloud* cloud, scr_visible_batch* host, int capacity, int* count);
+
+/* ---- frames (per-viewport resources: ComputeRasteriser.swift CachedResources) */
+int scr_frame_create(scr_ctx* ctx, int width, int height, scr_frame** out);
+int scr_frame_destroy(scr_frame* frame);
+
+#define SCR_PLANE_DEPTH 0          /* u32 per pixel, HQS depth */
+#define SCR_PLANE_RED 1            /* u32 sums */
+#define SCR_PLANE_GREEN 2
+#define SCR_PLANE_BLUE 3
+#define SCR_PLANE_COUNT 4
+#define SCR_PLANE_NEAREST_DEPTH 5  /* u32 */
+#define SCR_PLANE_NEAREST_INDEX 6  /* u32, 0xffffffff = none */
+#define SCR_PLANE_RGBA8 7          /* u32 = r | g<<8 | b<<16 | a<<24 (rgba8Unorm texel) */
+#define SCR_PLANE_DEPTH_F32 8      /* float reversed-Z depth texture */
+/* Copy one plane (width*height elements of 4 B) to host memory. */
+int scr_frame_read(scr_frame* frame, int which, void* host, size_t bytes);
+
+/* ---- rendering: ComputeRasteriser.update (:201-268) + encode (:277-296).
+ * HQS: clear → cull(all clouds) → depth(all) → colour(all) → resolve
+ * (encodeHighQualityAverage :322-456).  Nearest: first cloud with resident
+ * batches only (encodeNearestPoint :483-497).  Computes VP = P·V on the host
+ * and calls scr_cloud_update_files for every cloud with its model matrix. */
+int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam,
+               scr_cloud* const* clouds, int nClouds, scr_frame* frame);
+
+/* pickPointIndex(atNDC:in:camera:searchRadius:) — ComputeRasteriser.swift:609-670.
+ * Runs the nearest-index pass for one cloud into `frame`, reads back a band of
+ * rows and returns the nearest non-sentinel pixel's point index within
+ * searchRadius.  *found = 0 when there is none. */
+int scr_pick(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud,
+             scr_frame* frame, float ndcX, float ndcY, int searchRadius,
+             uint32_t* outIndex, int* found);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCR_H */

@@ -1,0 +1,370 @@
+"""ctypes wrapper of the CPU oracle (libscr_oracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and
+by the CPU-baseline leg of bench.py, never by the product path.  The oracle is
+a scalar C restatement of the reference's passes (see scr_oracle.h for the
+arithmetic contract and scr_oracle.c for the file:line citations).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libscr_oracle.so"
+
+RASTER_BATCH_DTYPE = np.dtype(
+    [("state", "<i4"), ("minX", "<f4"), ("minY", "<f4"), ("minZ", "<f4"), ("maxX", "<f4"), ("maxY", "<f4"),
+     ("maxZ", "<f4"), ("numPoints", "<u4"), ("firstPoint", "<u4"), ("fileIndex", "<u4"), ("lodCum01", "<u4"),
+     ("lodCum23", "<u4"), ("lodCum45", "<u4"), ("lodCum67", "<u4"), ("padding7", "<u4"), ("padding8", "<u4")]
+)
+VISIBLE_BATCH_DTYPE = np.dtype(
+    [("batchIndex", "<u4"), ("level", "<i4"), ("lodThreshold", "<f4"), ("activePoints", "<u4")]
+)
+RASTER_FILE_DTYPE = np.dtype(
+    [("transform", "<f4", (4, 4)), ("transformFrustum", "<f4", (4, 4)), ("world", "<f4", (4, 4)),
+     ("prevTransform", "<f4", (4, 4))]
+)
+
+
+class OrcCloud(C.Structure):
+    _fields_ = [
+        ("batches", C.c_void_p), ("nSlots", C.c_int), ("files", C.c_void_p), ("nFiles", C.c_int),
+        ("xyzLow", C.c_void_p), ("xyzMed", C.c_void_p), ("xyzHigh", C.c_void_p), ("colors", C.c_void_p),
+        ("levels", C.c_void_p), ("tints", C.c_void_p),
+    ]
+
+
+class OrcPlanes(C.Structure):
+    _fields_ = [("depth", C.c_void_p), ("red", C.c_void_p), ("green", C.c_void_p), ("blue", C.c_void_p),
+                ("count", C.c_void_p), ("rgba8", C.c_void_p), ("depthF32", C.c_void_p)]
+
+
+class OrcPackOut(C.Structure):
+    _fields_ = [
+        ("batches", C.c_void_p), ("xyzLow", C.c_void_p), ("xyzMed", C.c_void_p), ("xyzHigh", C.c_void_p),
+        ("colors", C.c_void_p), ("levels", C.c_void_p), ("sourceIndices", C.c_void_p),
+        ("orderedPositions", C.c_void_p), ("boundsMin", C.c_float * 3), ("boundsMax", C.c_float * 3),
+    ]
+
+
+_lib = None
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True, env=dict(os.environ))
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        f = C.c_float
+        u32 = C.c_uint32
+        vp = C.c_void_p
+        sig = {
+            "orc_log2f": (f, [f]), "orc_hash_unit": (f, [u32]), "orc_depth_to_uint": (u32, [f]),
+            "orc_uint_to_depth": (f, [u32]), "orc_f2u": (u32, [f]), "orc_f2i": (C.c_int32, [f]),
+            "orc_mat4_mul": (None, [vp, vp, vp]), "orc_mat4_vec": (None, [vp, vp, vp]),
+            "orc_morton30": (u32, [u32, u32, u32]), "orc_morton63": (C.c_uint64, [u32, u32, u32]),
+            "orc_inside_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "orc_precision_level": (C.c_int, [f]), "orc_lod_threshold": (f, [f, C.c_int]),
+            "orc_update_files": (None, [vp, C.c_int, vp, vp, vp, vp, vp]),
+            "orc_cull": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp]),
+            "orc_render_hqs": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.POINTER(OrcPlanes), C.c_int]),
+            "orc_render_nearest": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]),
+            "orc_nearest_cpu_reference": (None, [C.POINTER(OrcCloud), C.c_int, C.c_int, vp, vp, C.c_int, vp, vp]),
+            "orc_pick_search": (C.c_int, [vp, C.c_int, C.c_int, f, f, C.c_int, C.POINTER(u32)]),
+            "orc_pack": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(OrcPackOut)]),
+            "orc_morton_keys": (None, [vp, C.c_int, C.c_size_t, C.c_int, vp]),
+            "orc_lod_levels": (None, [vp, C.c_size_t, vp, vp, C.c_int, C.c_int, vp]),
+            "orc_cube_grid": (None, [C.c_int, f, vp, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float32)
+
+
+# ---------------------------------------------------------------- scalar helpers
+def log2f(x: float) -> float:
+    return lib().orc_log2f(x)
+
+
+def hash_unit(x: int) -> float:
+    return lib().orc_hash_unit(x)
+
+
+def depth_to_uint(z: float) -> int:
+    return lib().orc_depth_to_uint(z)
+
+
+def morton30(x: int, y: int, z: int) -> int:
+    return lib().orc_morton30(x, y, z)
+
+
+def morton63(x: int, y: int, z: int) -> int:
+    return lib().orc_morton63(x, y, z)
+
+
+def mat_mul(a, b) -> np.ndarray:
+    a, b = _f32(a), _f32(b)
+    out = np.zeros((4, 4), np.float32)
+    lib().orc_mat4_mul(a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
+
+def lod_threshold(px: float, bias: int) -> float:
+    return lib().orc_lod_threshold(px, bias)
+
+
+# ---------------------------------------------------------------- fixtures
+def cube_grid(n: int, extent: float = 1.0):
+    """cubeGrid lattice (positions (N,3) float32, colors (N,4) float32)."""
+    n = max(n, 2)
+    pos = np.zeros((n ** 3, 3), np.float32)
+    col = np.zeros((n ** 3, 4), np.float32)
+    lib().orc_cube_grid(n, extent, pos.ctypes.data, col.ctypes.data)
+    return pos, col
+
+
+class Packed:
+    """Output of orc_pack (mirrors PackedPointCloud)."""
+
+    def __init__(self, n: int, nb: int):
+        self.batches = np.zeros(nb, RASTER_BATCH_DTYPE)
+        self.xyzLow = np.zeros(n, np.uint32)
+        self.xyzMed = np.zeros(n, np.uint32)
+        self.xyzHigh = np.zeros(n, np.uint32)
+        self.colors = np.zeros(n, np.uint32)
+        self.levels = np.zeros(n, np.uint8)
+        self.sourceIndices = np.zeros(n, np.uint32)
+        self.orderedPositions = np.zeros((n, 3), np.float32)
+        self.boundsMin = np.zeros(3, np.float32)
+        self.boundsMax = np.zeros(3, np.float32)
+        self.files = identity_files(1)
+
+    @property
+    def pointCount(self) -> int:
+        return int(self.colors.shape[0])
+
+    @property
+    def batchCount(self) -> int:
+        return int(self.batches.shape[0])
+
+
+def identity_files(n: int = 1) -> np.ndarray:
+    f = np.zeros(n, RASTER_FILE_DTYPE)
+    for k in ("transform", "transformFrustum", "world", "prevTransform"):
+        f[k] = np.eye(4, dtype=np.float32)
+    return f
+
+
+def pack(positions, colors, pointsPerBatch: int = 10240, lodLevels: int = 4, coarseVoxelDivisions: int = 64,
+         keyBits: int = 30) -> Packed:
+    """PackedPointCloudFixtures.pack (stable Morton order; keyBits 30 or 63)."""
+    pos = _f32(positions).reshape(-1, 3)
+    col = _f32(colors).reshape(-1, 4)
+    n = pos.shape[0]
+    nb = (n + pointsPerBatch - 1) // pointsPerBatch if n else 0
+    out = Packed(n, nb)
+    o = OrcPackOut()
+    o.batches, o.xyzLow, o.xyzMed, o.xyzHigh = _p(out.batches), _p(out.xyzLow), _p(out.xyzMed), _p(out.xyzHigh)
+    o.colors, o.levels, o.sourceIndices, o.orderedPositions = (_p(out.colors), _p(out.levels),
+                                                               _p(out.sourceIndices), _p(out.orderedPositions))
+    rc = lib().orc_pack(pos.ctypes.data, 3, col.ctypes.data, n, pointsPerBatch, lodLevels, coarseVoxelDivisions,
+                        keyBits, C.byref(o))
+    if rc < 0:
+        raise ValueError(f"orc_pack failed ({rc})")
+    out.boundsMin[:] = list(o.boundsMin)
+    out.boundsMax[:] = list(o.boundsMax)
+    return out
+
+
+def morton_keys(positions, keyBits: int = 30) -> np.ndarray:
+    pos = _f32(positions).reshape(-1, 3)
+    keys = np.zeros(pos.shape[0], np.uint64)
+    lib().orc_morton_keys(pos.ctypes.data, 3, pos.shape[0], keyBits, keys.ctypes.data)
+    return keys
+
+
+def lod_levels(sorted_positions, bmin, bmax, lodLevels: int = 4, coarse: int = 64) -> np.ndarray:
+    sp = _f32(sorted_positions).reshape(-1, 3)
+    out = np.zeros(sp.shape[0], np.uint8)
+    lib().orc_lod_levels(sp.ctypes.data, sp.shape[0], _f32(bmin).ctypes.data, _f32(bmax).ctypes.data, lodLevels,
+                         coarse, out.ctypes.data)
+    return out
+
+
+# ---------------------------------------------------------------- clouds and frames
+class Cloud:
+    """Pool-layout arrays of one cloud, as the rasteriser reads them."""
+
+    def __init__(self, batches, files, xyzLow, xyzMed, xyzHigh, colors, levels, tints=None):
+        self.batches = np.ascontiguousarray(batches, RASTER_BATCH_DTYPE)
+        self.files = np.ascontiguousarray(files, RASTER_FILE_DTYPE)
+        self.xyzLow = np.ascontiguousarray(xyzLow, np.uint32)
+        self.xyzMed = np.ascontiguousarray(xyzMed, np.uint32)
+        self.xyzHigh = np.ascontiguousarray(xyzHigh, np.uint32)
+        self.colors = np.ascontiguousarray(colors, np.uint32)
+        self.levels = np.ascontiguousarray(levels, np.uint8)
+        self.tints = None if tints is None else np.ascontiguousarray(tints, np.float32).reshape(-1, 4)
+        self.model = np.eye(4, dtype=np.float32)
+
+    @staticmethod
+    def from_packed(packed, ppb: Optional[int] = None) -> "Cloud":
+        """The pool a from_packed upload builds: slot i holds batch i at i*ppb."""
+        nb = packed.batchCount
+        if ppb is None:
+            ppb = max(1, int(packed.batches["numPoints"].max())) if nb else 10240
+        cap = max(1, nb) * ppb
+        low = np.zeros(cap, np.uint32)
+        med = np.zeros(cap, np.uint32)
+        high = np.zeros(cap, np.uint32)
+        colors = np.zeros(cap, np.uint32)
+        levels = np.zeros(cap, np.uint8)
+        batches = np.zeros(max(1, nb), RASTER_BATCH_DTYPE)
+        for i in range(nb):
+            b = packed.batches[i].copy()
+            s, n = int(b["firstPoint"]), int(b["numPoints"])
+            d = i * ppb
+            low[d:d + n] = packed.xyzLow[s:s + n]
+            med[d:d + n] = packed.xyzMed[s:s + n]
+            high[d:d + n] = packed.xyzHigh[s:s + n]
+            colors[d:d + n] = packed.colors[s:s + n]
+            levels[d:d + n] = packed.levels[s:s + n]
+            b["firstPoint"] = d
+            b["state"] = 1
+            batches[i] = b
+        return Cloud(batches, packed.files, low, med, high, colors, levels)
+
+    def composed(self, vp) -> "Cloud":
+        """Apply updateFiles(viewProjection, model) to a copy."""
+        files = np.zeros_like(self.files)
+        lib().orc_update_files(self.files.ctypes.data, len(self.files), _f32(vp).ctypes.data,
+                               _f32(self.model).ctypes.data, None, None, files.ctypes.data)
+        c = Cloud(self.batches, files, self.xyzLow, self.xyzMed, self.xyzHigh, self.colors, self.levels, self.tints)
+        c.model = self.model
+        return c
+
+    def c_struct(self) -> OrcCloud:
+        s = OrcCloud()
+        s.batches, s.nSlots = self.batches.ctypes.data, len(self.batches)
+        s.files, s.nFiles = self.files.ctypes.data, len(self.files)
+        s.xyzLow, s.xyzMed, s.xyzHigh = self.xyzLow.ctypes.data, self.xyzMed.ctypes.data, self.xyzHigh.ctypes.data
+        s.colors, s.levels = self.colors.ctypes.data, self.levels.ctypes.data
+        s.tints = None if self.tints is None else self.tints.ctypes.data
+        return s
+
+
+class Config(C.Structure):
+    """scr_config (include/scr.h)."""
+    _fields_ = [
+        ("mode", C.c_int32), ("depthTolerance", C.c_float), ("backgroundColor", C.c_float * 4),
+        ("enableFrustumCulling", C.c_int32), ("lodBias", C.c_int32), ("enableCLOD", C.c_int32),
+        ("enableLODDither", C.c_int32), ("colorizeChunks", C.c_int32), ("colorizeOverdraw", C.c_int32),
+        ("pointSizeMode", C.c_int32), ("minimumPointSize", C.c_float), ("maximumPointSize", C.c_float),
+        ("pointSizeScale", C.c_float), ("applyTint", C.c_int32), ("tintAlphaIsCoverage", C.c_int32),
+        ("motionBlur", C.c_float), ("reserved", C.c_int32 * 5),
+    ]
+
+
+def config_from(cfg) -> Config:
+    """Copy any object with the configuration's field names into a Config."""
+    c = Config()
+    for name, _ in Config._fields_:
+        if name == "reserved" or not hasattr(cfg, name):
+            continue
+        v = getattr(cfg, name)
+        if name == "backgroundColor":
+            for i in range(4):
+                c.backgroundColor[i] = float(v[i])
+        else:
+            setattr(c, name, type(getattr(c, name))(v))
+    return c
+
+
+def cull(cloud: Cloud, cfg, view, proj, width: int, height: int) -> np.ndarray:
+    vp = mat_mul(proj, view)
+    cc = cloud.composed(vp)
+    s = cc.c_struct()
+    out = np.zeros(max(1, len(cc.batches)), VISIBLE_BATCH_DTYPE)
+    c = config_from(cfg)
+    n = lib().orc_cull(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width, height,
+                       out.ctypes.data)
+    return out[:n]
+
+
+def render_hqs(clouds: Sequence[Cloud], cfg, view, proj, width: int, height: int, threads: int = 1) -> dict:
+    """HQS frame: returns dict of planes (H, W)."""
+    vp = mat_mul(proj, view)
+    composed = [c.composed(vp) for c in clouds]
+    structs = (OrcCloud * max(1, len(composed)))(*[c.c_struct() for c in composed])
+    n = width * height
+    planes = {k: np.zeros(n, np.uint32) for k in ("depth", "red", "green", "blue", "count", "rgba8")}
+    planes["depthF32"] = np.zeros(n, np.float32)
+    pl = OrcPlanes(*[planes[k].ctypes.data for k in ("depth", "red", "green", "blue", "count", "rgba8", "depthF32")])
+    c = config_from(cfg)
+    rc = lib().orc_render_hqs(C.addressof(structs), len(composed), C.addressof(c), _f32(view).ctypes.data,
+                              _f32(proj).ctypes.data, width, height, C.byref(pl), threads)
+    if rc != 0:
+        raise RuntimeError("orc_render_hqs failed")
+    return {k: v.reshape(height, width) for k, v in planes.items()}
+
+
+def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int) -> dict:
+    vp = mat_mul(proj, view)
+    cc = cloud.composed(vp)
+    s = cc.c_struct()
+    n = width * height
+    d = np.zeros(n, np.uint32)
+    idx = np.zeros(n, np.uint32)
+    rgba = np.zeros(n, np.uint32)
+    df = np.zeros(n, np.float32)
+    c = config_from(cfg)
+    lib().orc_render_nearest(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width,
+                             height, d.ctypes.data, idx.ctypes.data, rgba.ctypes.data, df.ctypes.data)
+    return {"depth": d.reshape(height, width), "index": idx.reshape(height, width),
+            "rgba8": rgba.reshape(height, width), "depthF32": df.reshape(height, width)}
+
+
+def nearest_cpu_reference(cloud: Cloud, width: int, height: int, view=None, proj=None,
+                          enableFrustumCulling: bool = False) -> dict:
+    view = np.eye(4, dtype=np.float32) if view is None else view
+    proj = np.eye(4, dtype=np.float32) if proj is None else proj
+    vp = mat_mul(proj, view)
+    cc = cloud.composed(vp)
+    s = cc.c_struct()
+    n = max(width, 0) * max(height, 0)
+    d = np.zeros(max(n, 1), np.uint32)
+    idx = np.zeros(max(n, 1), np.uint32)
+    lib().orc_nearest_cpu_reference(C.byref(s), width, height, _f32(view).ctypes.data, _f32(proj).ctypes.data,
+                                    int(enableFrustumCulling), d.ctypes.data, idx.ctypes.data)
+    return {"depths": d[:n], "indices": idx[:n]}
+
+
+def pick_search(indices, width: int, height: int, ndcX: float, ndcY: float, radius: int = 10):
+    ind = np.ascontiguousarray(indices, np.uint32)
+    out = C.c_uint32()
+    found = lib().orc_pick_search(ind.ctypes.data, width, height, ndcX, ndcY, radius, C.byref(out))
+    return int(out.value) if found else None

@@ -1,0 +1,592 @@
+"""MI355X compute point-cloud rasteriser — Python mirror of the reference API.
+
+The product is ``libscr.so`` (hand-written HIP kernels for gfx950 behind the C
+ABI in ``include/scr.h``).  This module is the host-side mirror of the
+reference's Swift interface (``ComputeRasteriser``,
+``ComputeRasteriserPointCloud``, ``ComputeRasteriserConfiguration``,
+``PackedPointCloud``, ``GPUPacker``) over ctypes, so that tests and the
+benchmark read like the reference's own.  There is no CPU fallback: if the
+library is missing or no GPU is present, constructing a ``Context`` raises.
+
+Matrices are numpy ``float32`` arrays of shape (4, 4) indexed ``m[column, row]``
+(the memory layout of ``simd_float4x4``); ``mat()`` converts from the usual
+row-major math notation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import math
+from pathlib import Path
+from typing import Optional, Sequence
+
+import numpy as np
+
+_PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = _PKG_DIR / "libscr.so"
+
+# ---------------------------------------------------------------- constants
+THREADS_PER_GROUP = 128
+DEFAULT_POINTS_PER_BATCH = 10240
+MODE_HIGH_QUALITY_AVERAGE = 0
+MODE_NEAREST_POINT = 1
+POINT_SIZE_SCREEN_SPACE = 0
+POINT_SIZE_WORLD_SPACE = 1
+
+PLANE_DEPTH, PLANE_RED, PLANE_GREEN, PLANE_BLUE, PLANE_COUNT = 0, 1, 2, 3, 4
+PLANE_NEAREST_DEPTH, PLANE_NEAREST_INDEX, PLANE_RGBA8, PLANE_DEPTH_F32 = 5, 6, 7, 8
+POOL_XYZ_LOW, POOL_XYZ_MED, POOL_XYZ_HIGH, POOL_COLORS, POOL_LEVELS, POOL_BATCHES = range(6)
+
+ERRORS = {
+    -1: "SCR_ERR_INVALID_ARGUMENT",
+    -2: "SCR_ERR_NO_FREE_SLOTS",
+    -3: "SCR_ERR_BATCH_TOO_LARGE",
+    -4: "SCR_ERR_POINTS_PER_BATCH",
+    -5: "SCR_ERR_HIP",
+    -6: "SCR_ERR_OUT_OF_MEMORY",
+    -7: "SCR_ERR_SOURCE_TOO_SMALL",
+    -8: "SCR_ERR_UNSUPPORTED",
+    -9: "SCR_ERR_SLOT_RANGE",
+    -10: "SCR_ERR_NO_DEVICE",
+}
+
+# RasterBatch — Pipelines/ComputeRasteriserTypes.h:11-33 (64 B)
+RASTER_BATCH_DTYPE = np.dtype(
+    [
+        ("state", "<i4"),
+        ("minX", "<f4"), ("minY", "<f4"), ("minZ", "<f4"),
+        ("maxX", "<f4"), ("maxY", "<f4"), ("maxZ", "<f4"),
+        ("numPoints", "<u4"),
+        ("firstPoint", "<u4"),
+        ("fileIndex", "<u4"),
+        ("lodCum01", "<u4"), ("lodCum23", "<u4"), ("lodCum45", "<u4"), ("lodCum67", "<u4"),
+        ("padding7", "<u4"), ("padding8", "<u4"),
+    ]
+)
+# VisibleBatch — ComputeRasteriserTypes.h:52-57 (16 B)
+VISIBLE_BATCH_DTYPE = np.dtype(
+    [("batchIndex", "<u4"), ("level", "<i4"), ("lodThreshold", "<f4"), ("activePoints", "<u4")]
+)
+# RasterFile — ComputeRasteriserTypes.h:35-40 (4 x float4x4, 256 B)
+RASTER_FILE_DTYPE = np.dtype(
+    [("transform", "<f4", (4, 4)), ("transformFrustum", "<f4", (4, 4)), ("world", "<f4", (4, 4)),
+     ("prevTransform", "<f4", (4, 4))]
+)
+assert RASTER_BATCH_DTYPE.itemsize == 64 and VISIBLE_BATCH_DTYPE.itemsize == 16
+assert RASTER_FILE_DTYPE.itemsize == 256
+
+
+class ScrError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+# ---------------------------------------------------------------- C structs
+class CConfig(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("depthTolerance", C.c_float), ("backgroundColor", C.c_float * 4),
+        ("enableFrustumCulling", C.c_int32), ("lodBias", C.c_int32), ("enableCLOD", C.c_int32),
+        ("enableLODDither", C.c_int32), ("colorizeChunks", C.c_int32), ("colorizeOverdraw", C.c_int32),
+        ("pointSizeMode", C.c_int32), ("minimumPointSize", C.c_float), ("maximumPointSize", C.c_float),
+        ("pointSizeScale", C.c_float), ("applyTint", C.c_int32), ("tintAlphaIsCoverage", C.c_int32),
+        ("motionBlur", C.c_float), ("reserved", C.c_int32 * 5),
+    ]
+
+
+class CCamera(C.Structure):
+    _fields_ = [("viewMatrix", C.c_float * 16), ("projectionMatrix", C.c_float * 16)]
+
+
+class CCloudInfo(C.Structure):
+    _fields_ = [
+        ("maxResidentBatches", C.c_int32), ("pointsPerBatch", C.c_int32), ("residentBatchCount", C.c_int32),
+        ("freeSlotCount", C.c_int32), ("pointCount", C.c_int64), ("nFiles", C.c_int32), ("hasTints", C.c_int32),
+    ]
+
+
+_VP = C.c_void_p
+_I = C.c_int
+_SZ = C.c_size_t
+_SIGNATURES = {
+    "scr_version": (C.c_char_p, []),
+    "scr_config_default": (None, [C.POINTER(CConfig)]),
+    "scr_ctx_create": (_I, [_I, C.POINTER(_VP)]),
+    "scr_ctx_destroy": (_I, [_VP]),
+    "scr_last_error": (C.c_char_p, [_VP]),
+    "scr_sync": (_I, [_VP]),
+    "scr_device_alloc": (_I, [_VP, _SZ, C.POINTER(_VP)]),
+    "scr_device_free": (_I, [_VP, _VP]),
+    "scr_memcpy_h2d": (_I, [_VP, _VP, _VP, _SZ]),
+    "scr_memcpy_d2h": (_I, [_VP, _VP, _VP, _SZ]),
+    "scr_cloud_create": (_I, [_VP, _I, _I, _VP, _I, C.POINTER(_VP)]),
+    "scr_cloud_destroy": (_I, [_VP]),
+    "scr_cloud_get_info": (_I, [_VP, C.POINTER(CCloudInfo)]),
+    "scr_cloud_add_batches": (_I, [_VP, _VP, _VP, _VP, _VP, _VP, _SZ, _VP, _I, _I, _VP]),
+    "scr_cloud_remove_batches": (_I, [_VP, _VP, _I, _I]),
+    "scr_cloud_commit": (_I, [_VP]),
+    "scr_cloud_clear": (_I, [_VP]),
+    "scr_cloud_set_model": (_I, [_VP, _VP]),
+    "scr_cloud_update_files": (_I, [_VP, _VP, _VP, _VP, _VP]),
+    "scr_cloud_set_tints": (_I, [_VP, _VP, _SZ, _I]),
+    "scr_cloud_read_pool": (_I, [_VP, _I, _VP, _SZ]),
+    "scr_cloud_read_visible": (_I, [_VP, _VP, _I, C.POINTER(_I)]),
+    "scr_frame_create": (_I, [_VP, _I, _I, C.POINTER(_VP)]),
+    "scr_frame_destroy": (_I, [_VP]),
+    "scr_frame_read": (_I, [_VP, _I, _VP, _SZ]),
+    "scr_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _I, _VP]),
+    "scr_pick": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, C.c_float, C.c_float, _I,
+                      C.POINTER(C.c_uint32), C.POINTER(_I)]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def load_library(path: Optional[Path] = None) -> C.CDLL:
+    """Load libscr.so and bind every entry point (raises if it is missing)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[int]:
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------- math helpers
+def mat(rows) -> np.ndarray:
+    """Row-major 4x4 (math notation) → column-major storage m[column, row]."""
+    return np.ascontiguousarray(np.asarray(rows, dtype=np.float64).T, dtype=np.float32)
+
+
+IDENTITY = np.eye(4, dtype=np.float32)
+
+
+def perspective_reverse_z(fovy_degrees: float, aspect: float, near: float, far: float) -> np.ndarray:
+    """Right-handed reverse-Z perspective (view z = -near → ndc 1, z = -far → 0), column-major."""
+    f = 1.0 / math.tan(math.radians(fovy_degrees) * 0.5)
+    return mat([
+        [f / aspect, 0, 0, 0],
+        [0, f, 0, 0],
+        [0, 0, near / (far - near), near * far / (far - near)],
+        [0, 0, -1, 0],
+    ])
+
+
+def look_at(eye, target, up) -> np.ndarray:
+    """Right-handed view matrix looking down -Z, column-major."""
+    eye = np.asarray(eye, np.float64)
+    fwd = np.asarray(target, np.float64) - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(up, np.float64))
+    right /= np.linalg.norm(right)
+    u = np.cross(right, fwd)
+    return mat([
+        [right[0], right[1], right[2], -right @ eye],
+        [u[0], u[1], u[2], -u @ eye],
+        [-fwd[0], -fwd[1], -fwd[2], fwd @ eye],
+        [0, 0, 0, 1],
+    ])
+
+
+@dataclasses.dataclass
+class Camera:
+    """A camera: view and projection matrices (column-major float32 4x4)."""
+    viewMatrix: np.ndarray
+    projectionMatrix: np.ndarray
+
+    @staticmethod
+    def perspective(eye, target, up=(0.0, 1.0, 0.0), fov: float = 60.0, aspect: float = 1.0,
+                    near: float = 0.1, far: float = 100.0) -> "Camera":
+        return Camera(look_at(eye, target, up), perspective_reverse_z(fov, aspect, near, far))
+
+    def to_c(self) -> CCamera:
+        c = CCamera()
+        C.memmove(C.addressof(c.viewMatrix), np.ascontiguousarray(self.viewMatrix, np.float32).ctypes.data, 64)
+        C.memmove(C.addressof(c.projectionMatrix), np.ascontiguousarray(self.projectionMatrix, np.float32).ctypes.data, 64)
+        return c
+
+
+# ---------------------------------------------------------------- configuration
+@dataclasses.dataclass
+class ComputeRasteriserConfiguration:
+    """ComputeRasteriserTypes.swift:26-142 (defaults :97-119)."""
+    mode: int = MODE_HIGH_QUALITY_AVERAGE
+    depthTolerance: float = 0.01
+    backgroundColor: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    enableFrustumCulling: bool = True
+    lodBias: int = 0
+    enableCLOD: bool = True
+    enableLODDither: bool = True
+    colorizeChunks: bool = False
+    colorizeOverdraw: bool = False
+    pointSizeMode: int = POINT_SIZE_SCREEN_SPACE
+    minimumPointSize: float = 1.0
+    maximumPointSize: float = 1.0
+    pointSizeScale: float = 1.0
+    applyTint: bool = False
+    tintAlphaIsCoverage: bool = False
+    motionBlur: float = 0.0
+
+    def to_c(self) -> CConfig:
+        c = CConfig()
+        c.mode = int(self.mode)
+        c.depthTolerance = float(self.depthTolerance)
+        for i in range(4):
+            c.backgroundColor[i] = float(self.backgroundColor[i])
+        c.enableFrustumCulling = int(bool(self.enableFrustumCulling))
+        c.lodBias = int(self.lodBias)
+        c.enableCLOD = int(bool(self.enableCLOD))
+        c.enableLODDither = int(bool(self.enableLODDither))
+        c.colorizeChunks = int(bool(self.colorizeChunks))
+        c.colorizeOverdraw = int(bool(self.colorizeOverdraw))
+        c.pointSizeMode = int(self.pointSizeMode)
+        c.minimumPointSize = float(self.minimumPointSize)
+        c.maximumPointSize = float(self.maximumPointSize)
+        c.pointSizeScale = float(self.pointSizeScale)
+        c.applyTint = int(bool(self.applyTint))
+        c.tintAlphaIsCoverage = int(bool(self.tintAlphaIsCoverage))
+        c.motionBlur = float(self.motionBlur)
+        return c
+
+
+# ---------------------------------------------------------------- packed model
+def identity_files(n: int = 1) -> np.ndarray:
+    f = np.zeros(n, RASTER_FILE_DTYPE)
+    for k in ("transform", "transformFrustum", "world", "prevTransform"):
+        f[k] = IDENTITY
+    return f
+
+
+@dataclasses.dataclass
+class PackedPointCloud:
+    """ComputeRasteriserTypes.swift:269-320 (SoA point arrays + batch table)."""
+    batches: np.ndarray          # RASTER_BATCH_DTYPE
+    files: np.ndarray            # RASTER_FILE_DTYPE
+    xyzLow: np.ndarray           # uint32
+    xyzMed: np.ndarray
+    xyzHigh: np.ndarray
+    colors: np.ndarray           # uint32 RGBA8
+    levels: np.ndarray           # uint8
+    boundsMin: np.ndarray
+    boundsMax: np.ndarray
+    orderedPositions: Optional[np.ndarray] = None
+    sourceIndices: Optional[np.ndarray] = None
+
+    @property
+    def pointCount(self) -> int:
+        return int(self.colors.shape[0])
+
+    @property
+    def batchCount(self) -> int:
+        return int(self.batches.shape[0])
+
+
+def lod_cumulative_counts(batch) -> list:
+    """RasterBatch.lodCumulativeCounts — ComputeRasteriserTypes.swift:196-203."""
+    words = [int(batch["lodCum01"]), int(batch["lodCum23"]), int(batch["lodCum45"]), int(batch["lodCum67"])]
+    out = []
+    for w in words:
+        out += [w & 0xFFFF, (w >> 16) & 0xFFFF]
+    return out
+
+
+def set_lod_cumulative_counts(batch, counts) -> None:
+    """RasterBatch.setLODCumulativeCounts — ComputeRasteriserTypes.swift:214-223."""
+    if len(counts) != 8:
+        raise ValueError(f"expected 8 cumulative level counts, got {len(counts)}")
+    for c in counts:
+        if not 0 <= int(c) <= 65535:
+            raise ValueError(f"cumulative count {c} does not fit uint16")
+    batch["lodCum01"] = int(counts[0]) | (int(counts[1]) << 16)
+    batch["lodCum23"] = int(counts[2]) | (int(counts[3]) << 16)
+    batch["lodCum45"] = int(counts[4]) | (int(counts[5]) << 16)
+    batch["lodCum67"] = int(counts[6]) | (int(counts[7]) << 16)
+
+
+# ---------------------------------------------------------------- runtime objects
+class Context:
+    """One GPU, one HIP stream (scr_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.scr_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise ScrError(rc, "scr_ctx_create failed (no usable GPU?)")
+        self.handle = h
+
+    def check(self, rc: int) -> None:
+        if rc != 0:
+            raise ScrError(rc, (self.lib.scr_last_error(self.handle) or b"").decode())
+
+    def sync(self) -> None:
+        self.check(self.lib.scr_sync(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.scr_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Raw device allocation owned by a Context (inputs kept resident in HBM)."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx.check(ctx.lib.scr_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        ctx.check(ctx.lib.scr_memcpy_h2d(ctx.handle, b.ptr, a.ctypes.data, a.nbytes))
+        return b
+
+    def to_array(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype)
+        self.ctx.check(self.ctx.lib.scr_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx.lib.scr_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class ComputeRasteriserPointCloud:
+    """ComputeRasteriserPointCloud.swift: a slot pool of packed points on the GPU."""
+
+    def __init__(self, ctx: Context, maxResidentBatches: int, pointsPerBatch: int = DEFAULT_POINTS_PER_BATCH,
+                 files: Optional[np.ndarray] = None):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = C.c_void_p()
+        f = None if files is None else np.ascontiguousarray(files, RASTER_FILE_DTYPE)
+        ctx.check(self.lib.scr_cloud_create(ctx.handle, int(maxResidentBatches), int(pointsPerBatch), _ptr(f),
+                                            0 if f is None else len(f), C.byref(h)))
+        self.handle = h
+        self.worldMatrix = IDENTITY.copy()
+
+    @classmethod
+    def from_packed(cls, ctx: Context, packed: PackedPointCloud) -> "ComputeRasteriserPointCloud":
+        """init(context:packed:) — ComputeRasteriserPointCloud.swift:222-236."""
+        nums = packed.batches["numPoints"] if packed.batchCount else np.array([DEFAULT_POINTS_PER_BATCH])
+        ppb = max(1, int(nums.max()))
+        cloud = cls(ctx, max(1, packed.batchCount), ppb, packed.files)
+        if packed.batchCount > 0:
+            cloud.addBatches(packed.xyzLow, packed.xyzMed, packed.xyzHigh, packed.colors, packed.levels,
+                             packed.batches)
+        return cloud
+
+    # --- info
+    def info(self) -> CCloudInfo:
+        i = CCloudInfo()
+        self.ctx.check(self.lib.scr_cloud_get_info(self.handle, C.byref(i)))
+        return i
+
+    @property
+    def capacity(self):
+        i = self.info()
+        return (i.maxResidentBatches, i.pointsPerBatch)
+
+    @property
+    def residentBatchCount(self) -> int:
+        return self.info().residentBatchCount
+
+    @property
+    def pointCount(self) -> int:
+        return self.info().pointCount
+
+    @property
+    def freeSlotCount(self) -> int:
+        return self.info().freeSlotCount
+
+    @property
+    def batchCount(self) -> int:
+        return self.info().maxResidentBatches
+
+    # --- slot pool
+    def addBatches(self, xyzLow, xyzMed, xyzHigh, colors, levels, batches, commit: bool = True) -> list:
+        arrs = [np.ascontiguousarray(a, dt) for a, dt in
+                ((xyzLow, np.uint32), (xyzMed, np.uint32), (xyzHigh, np.uint32), (colors, np.uint32),
+                 (levels, np.uint8))]
+        b = np.ascontiguousarray(batches, RASTER_BATCH_DTYPE)
+        n = min(len(a) for a in arrs)
+        slots = np.zeros(len(b), np.int32)
+        self.ctx.check(self.lib.scr_cloud_add_batches(self.handle, *[a.ctypes.data for a in arrs], n, b.ctypes.data,
+                                                      len(b), int(commit), slots.ctypes.data))
+        return [int(s) for s in slots]
+
+    def removeBatches(self, slots, commit: bool = True) -> None:
+        s = np.ascontiguousarray(slots, np.int32)
+        self.ctx.check(self.lib.scr_cloud_remove_batches(self.handle, s.ctypes.data, len(s), int(commit)))
+
+    def commitBatchUpdates(self) -> None:
+        self.ctx.check(self.lib.scr_cloud_commit(self.handle))
+
+    def clearAllBatches(self) -> None:
+        self.ctx.check(self.lib.scr_cloud_clear(self.handle))
+
+    def updateFiles(self, viewProjection, modelMatrix, frustumTransform=None, prevViewProjection=None) -> None:
+        vp = np.ascontiguousarray(viewProjection, np.float32)
+        m = np.ascontiguousarray(modelMatrix, np.float32)
+        fr = None if frustumTransform is None else np.ascontiguousarray(frustumTransform, np.float32)
+        pv = None if prevViewProjection is None else np.ascontiguousarray(prevViewProjection, np.float32)
+        self.ctx.check(self.lib.scr_cloud_update_files(self.handle, vp.ctypes.data, m.ctypes.data, _ptr(fr), _ptr(pv)))
+
+    def setTints(self, rgba: Optional[np.ndarray]) -> None:
+        """Install the TintPass output (float4 per pool point), or drop it with None."""
+        if rgba is None:
+            self.ctx.check(self.lib.scr_cloud_set_tints(self.handle, None, 0, 0))
+            return
+        t = np.ascontiguousarray(rgba, np.float32).reshape(-1, 4)
+        self.ctx.check(self.lib.scr_cloud_set_tints(self.handle, t.ctypes.data, t.shape[0], 0))
+
+    def setTintsDevice(self, ptr: int, nPoints: int) -> None:
+        self.ctx.check(self.lib.scr_cloud_set_tints(self.handle, ptr, nPoints, 1))
+
+    def readPool(self, which: int) -> np.ndarray:
+        mrb, ppb = self.capacity
+        n = mrb * ppb
+        if which == POOL_BATCHES:
+            out = np.zeros(mrb, RASTER_BATCH_DTYPE)
+        elif which == POOL_LEVELS:
+            out = np.zeros(n, np.uint8)
+        else:
+            out = np.zeros(n, np.uint32)
+        self.ctx.check(self.lib.scr_cloud_read_pool(self.handle, which, out.ctypes.data, out.nbytes))
+        return out
+
+    def readVisible(self) -> np.ndarray:
+        cap = self.batchCount
+        out = np.zeros(cap, VISIBLE_BATCH_DTYPE)
+        n = C.c_int()
+        self.ctx.check(self.lib.scr_cloud_read_visible(self.handle, out.ctypes.data, cap, C.byref(n)))
+        return out[: n.value]
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.scr_cloud_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Frame:
+    """Per-viewport GPU resources (HQS accumulators, nearest winner, output textures)."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx = ctx
+        self.width, self.height = int(width), int(height)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.scr_frame_create(ctx.handle, self.width, self.height, C.byref(h)))
+        self.handle = h
+
+    def read(self, which: int) -> np.ndarray:
+        dt = np.float32 if which == PLANE_DEPTH_F32 else np.uint32
+        out = np.zeros(self.width * self.height, dt)
+        self.ctx.check(self.ctx.lib.scr_frame_read(self.handle, which, out.ctypes.data, out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.ctx.lib.scr_frame_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ComputeRasteriser:
+    """ComputeRasteriser.swift: configuration, clouds, resize, update+encode, pick."""
+
+    def __init__(self, ctx: Context, configuration: Optional[ComputeRasteriserConfiguration] = None):
+        self.ctx = ctx
+        self.configuration = configuration or ComputeRasteriserConfiguration()
+        self.pointClouds: list = []
+        self.frame: Optional[Frame] = None
+
+    def addPointCloud(self, cloud: ComputeRasteriserPointCloud) -> ComputeRasteriserPointCloud:
+        self.pointClouds.append(cloud)
+        return cloud
+
+    def removePointCloud(self, cloud: ComputeRasteriserPointCloud) -> None:
+        self.pointClouds.remove(cloud)
+
+    def resize(self, width: int, height: int) -> None:
+        if self.frame is None or (self.frame.width, self.frame.height) != (int(width), int(height)):
+            self.frame = Frame(self.ctx, width, height)
+
+    def render(self, camera: Camera) -> None:
+        """update(renderContext:camera:viewport:index:) + encode(_:) for one frame."""
+        if self.frame is None:
+            raise RuntimeError("resize() first")
+        for cl in self.pointClouds:
+            m = np.ascontiguousarray(cl.worldMatrix, np.float32)
+            self.ctx.check(self.ctx.lib.scr_cloud_set_model(cl.handle, m.ctypes.data))
+        handles = (C.c_void_p * max(1, len(self.pointClouds)))(*[c.handle.value for c in self.pointClouds])
+        cfg = self.configuration.to_c()
+        cam = camera.to_c()
+        self.ctx.check(self.ctx.lib.scr_render(self.ctx.handle, C.byref(cfg), C.byref(cam), handles,
+                                               len(self.pointClouds), self.frame.handle))
+
+    def read(self, which: int) -> np.ndarray:
+        return self.frame.read(which).reshape(self.frame.height, self.frame.width)
+
+    def outputRGBA8(self) -> np.ndarray:
+        """(H, W, 4) uint8 image (row 0 = top of the viewport)."""
+        return self.read(PLANE_RGBA8).view(np.uint8).reshape(self.frame.height, self.frame.width, 4)
+
+    def pickPointIndex(self, ndc, cloud: ComputeRasteriserPointCloud, camera: Camera,
+                       searchRadius: int = 10) -> Optional[int]:
+        """pickPointIndex(atNDC:in:camera:searchRadius:) — ComputeRasteriser.swift:609-670."""
+        if self.frame is None:
+            return None
+        m = np.ascontiguousarray(cloud.worldMatrix, np.float32)
+        self.ctx.check(self.ctx.lib.scr_cloud_set_model(cloud.handle, m.ctypes.data))
+        cfg = self.configuration.to_c()
+        cam = camera.to_c()
+        out = C.c_uint32()
+        found = C.c_int()
+        self.ctx.check(self.ctx.lib.scr_pick(self.ctx.handle, C.byref(cfg), C.byref(cam), cloud.handle,
+                                             self.frame.handle, float(ndc[0]), float(ndc[1]), int(searchRadius),
+                                             C.byref(out), C.byref(found)))
+        return int(out.value) if found.value else None
+
+
+def version() -> str:
+    return load_library().scr_version().decode()

@@ -1,0 +1,666 @@
+// scr_api.hip — C ABI host runtime: context, point-cloud slot pool, frames,
+// render orchestration and picking.  Reference behaviour is cited per entry
+// point in include/scr.h; the orchestration follows ComputeRasteriser.swift.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "scr_internal.hpp"
+
+namespace scr {
+
+int set_error(scr_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->lastError = msg;
+    return code;
+}
+
+int hip_error(scr_ctx* ctx, hipError_t e, const char* what) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    return set_error(ctx, e == hipErrorOutOfMemory ? SCR_ERR_OUT_OF_MEMORY : SCR_ERR_HIP, buf);
+}
+
+}  // namespace scr
+
+using scr::hip_error;
+using scr::set_error;
+
+#define SCR_HIP(ctx, call)                                   \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_error(ctx, e_, #call); \
+    } while (0)
+
+namespace {
+
+const float kIdentity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template <typename T>
+hipError_t dalloc(T*& p, size_t count) {
+    p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, count * sizeof(T));
+}
+
+void release_cloud(scr_cloud* c) {
+    if (!c) return;
+    dfree(c->dLow);
+    dfree(c->dMed);
+    dfree(c->dHigh);
+    dfree(c->dColors);
+    dfree(c->dLevels);
+    dfree(c->dBatches);
+    dfree(c->dFiles);
+    if (c->tintsOwned) dfree(c->dTints);
+    c->dTints = nullptr;
+    dfree(c->dVisible);
+    dfree(c->dCounter);
+    if (c->hFiles) (void)hipHostFree(c->hFiles);
+    c->hFiles = nullptr;
+    if (c->filesCopied) (void)hipEventDestroy(c->filesCopied);
+    c->filesCopied = nullptr;
+}
+
+void mark_dirty(scr_cloud* c, int slot) {
+    c->dirtyLo = std::min(c->dirtyLo, slot);
+    c->dirtyHi = std::max(c->dirtyHi, slot);
+}
+
+// flushBatchMirror — ComputeRasteriserPointCloud.swift:629-644 (dirty range only)
+int flush_mirror(scr_cloud* c) {
+    if (c->dirtyLo > c->dirtyHi) return SCR_OK;
+    const int lo = std::max(0, c->dirtyLo);
+    const int hi = std::min(c->dirtyHi, c->maxResidentBatches - 1);
+    c->dirtyLo = 0x7fffffff;
+    c->dirtyHi = -1;
+    if (lo > hi) return SCR_OK;
+    SCR_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
+    SCR_HIP(c->ctx, hipMemcpy(c->dBatches + lo, c->mirror.data() + lo, (size_t)(hi - lo + 1) * sizeof(scr_raster_batch),
+                              hipMemcpyHostToDevice));
+    return SCR_OK;
+}
+
+// updateFiles composition (ComputeRasteriserPointCloud.swift:487-515) and its
+// upload through pinned staging; the event keeps the staging untouched until
+// the previous copy has consumed it.
+int upload_files(scr_cloud* c, const float* vp, const float* model, const float* frustum, const float* prevVP) {
+    SCR_HIP(c->ctx, hipEventSynchronize(c->filesCopied));
+    for (size_t i = 0; i < c->files.size(); i++) {
+        scr_raster_file f;
+        float world[16];
+        scr::mat_mul(model, c->files[i].world, world);
+        scr::mat_mul(vp, world, f.transform);
+        scr::mat_mul(frustum ? frustum : vp, world, f.transformFrustum);
+        std::memcpy(f.world, world, sizeof world);
+        scr::mat_mul(prevVP ? prevVP : vp, world, f.prevTransform);
+        c->hFiles[i] = f;
+    }
+    SCR_HIP(c->ctx, hipMemcpyAsync(c->dFiles, c->hFiles, c->files.size() * sizeof(scr_raster_file),
+                                   hipMemcpyHostToDevice, c->ctx->stream));
+    SCR_HIP(c->ctx, hipEventRecord(c->filesCopied, c->ctx->stream));
+    return SCR_OK;
+}
+
+int check_config(scr_ctx* ctx, const scr_config* cfg) {
+    if (!cfg) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "config is NULL");
+    if (cfg->mode != SCR_MODE_HIGH_QUALITY_AVERAGE && cfg->mode != SCR_MODE_NEAREST_POINT)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown render mode");
+    if (cfg->tintAlphaIsCoverage != 0 && cfg->applyTint != 0)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "translucent-defocus (tintAlphaIsCoverage) is not supported");
+    if (cfg->motionBlur > 0.0f) return set_error(ctx, SCR_ERR_UNSUPPORTED, "motion blur is not supported");
+    if (cfg->pointSizeMode != 0 && cfg->pointSizeMode != 1)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown point size mode");
+    return SCR_OK;
+}
+
+scr::PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, int height) {
+    scr::PassParams p;
+    std::memset(&p, 0, sizeof p);
+    std::memcpy(p.view, cam->viewMatrix, sizeof p.view);
+    std::memcpy(p.proj, cam->projectionMatrix, sizeof p.proj);
+    p.width = width;
+    p.height = height;
+    p.pointSizeMode = cfg->pointSizeMode;
+    p.minPointSize = cfg->minimumPointSize;
+    p.maxPointSize = cfg->maximumPointSize;
+    p.pointSizeScale = cfg->pointSizeScale;
+    p.lodDither = cfg->enableLODDither;
+    p.enableFrustumCulling = cfg->enableFrustumCulling;
+    p.lodBias = cfg->lodBias;
+    p.enableCLOD = cfg->enableCLOD;
+    p.depthKeep = 1.0f - cfg->depthTolerance;
+    p.colorizeChunks = cfg->colorizeChunks;
+    p.colorizeOverdraw = cfg->colorizeOverdraw;
+    p.applyTint = cfg->applyTint;
+    const float mn = cfg->minimumPointSize, mx = cfg->maximumPointSize;
+    p.lo = std::fmax(std::fmin(mn, mx), 1.0f);
+    p.hi = std::fmax(std::fmax(mn, mx), p.lo);
+    p.constRadius = (p.lo == p.hi) ? scr::const_radius_for(p.lo) : -1;
+    p.ortho = cam->projectionMatrix[15] == 1.0f;
+    p.focal = ((float)height * 0.5f) * cam->projectionMatrix[5];
+    for (int i = 0; i < 4; i++) p.bg[i] = cfg->backgroundColor[i];
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* scr_version(void) { return "scr 0.1 (gfx950)"; }
+
+void scr_config_default(scr_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->mode = SCR_MODE_HIGH_QUALITY_AVERAGE;
+    c->depthTolerance = 0.01f;
+    c->enableFrustumCulling = 1;
+    c->lodBias = 0;
+    c->enableCLOD = 1;
+    c->enableLODDither = 1;
+    c->pointSizeMode = SCR_POINT_SIZE_SCREEN_SPACE;
+    c->minimumPointSize = 1.0f;
+    c->maximumPointSize = 1.0f;
+    c->pointSizeScale = 1.0f;
+}
+
+// ------------------------------------------------------------------ context
+
+int scr_ctx_create(int device, scr_ctx** out) {
+    if (!out) return SCR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SCR_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = new (std::nothrow) scr_ctx();
+    if (!ctx) return SCR_ERR_OUT_OF_MEMORY;
+    ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return SCR_ERR_HIP;
+    }
+    ctx->numCUs = prop.multiProcessorCount;
+    *out = ctx;
+    return SCR_OK;
+}
+
+int scr_ctx_destroy(scr_ctx* ctx) {
+    if (!ctx) return SCR_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SCR_OK;
+}
+
+const char* scr_last_error(const scr_ctx* ctx) { return ctx ? ctx->lastError.c_str() : "no context"; }
+
+int scr_sync(scr_ctx* ctx) {
+    if (!ctx) return SCR_ERR_INVALID_ARGUMENT;
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SCR_OK;
+}
+
+int scr_device_alloc(scr_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return SCR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (bytes == 0) return SCR_OK;
+    SCR_HIP(ctx, hipMalloc(out, bytes));
+    return SCR_OK;
+}
+
+int scr_device_free(scr_ctx* ctx, void* ptr) {
+    if (!ctx) return SCR_ERR_INVALID_ARGUMENT;
+    if (ptr) {
+        SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SCR_HIP(ctx, hipFree(ptr));
+    }
+    return SCR_OK;
+}
+
+int scr_memcpy_h2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return SCR_ERR_INVALID_ARGUMENT;
+    if (!bytes) return SCR_OK;
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SCR_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SCR_OK;
+}
+
+int scr_memcpy_d2h(scr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return SCR_ERR_INVALID_ARGUMENT;
+    if (!bytes) return SCR_OK;
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    SCR_HIP(ctx, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return SCR_OK;
+}
+
+// ------------------------------------------------------------------ clouds
+
+int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, const scr_raster_file* files,
+                     int nFiles, scr_cloud** out) {
+    if (!ctx || !out) return SCR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (maxResidentBatches <= 0) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "maxResidentBatches must be positive");
+    if (pointsPerBatch <= 0 || pointsPerBatch > SCR_MAX_POINTS_PER_BATCH)
+        return set_error(ctx, SCR_ERR_POINTS_PER_BATCH, "pointsPerBatch must be in 1..65535");
+    if (nFiles < 0 || (nFiles > 0 && !files)) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "bad files table");
+    scr_cloud* c = new (std::nothrow) scr_cloud();
+    if (!c) return SCR_ERR_OUT_OF_MEMORY;
+    c->ctx = ctx;
+    c->maxResidentBatches = maxResidentBatches;
+    c->pointsPerBatch = pointsPerBatch;
+    std::memcpy(c->model, kIdentity, sizeof c->model);
+    if (nFiles == 0) {
+        scr_raster_file f;
+        for (int i = 0; i < 16; i++) f.transform[i] = f.transformFrustum[i] = f.world[i] = f.prevTransform[i] = kIdentity[i];
+        c->files.push_back(f);
+    } else {
+        c->files.assign(files, files + nFiles);
+    }
+    scr_raster_batch empty;
+    std::memset(&empty, 0, sizeof empty);
+    c->mirror.assign((size_t)maxResidentBatches, empty);
+    c->freeSlots.resize((size_t)maxResidentBatches);
+    for (int i = 0; i < maxResidentBatches; i++) c->freeSlots[(size_t)i] = maxResidentBatches - 1 - i;
+    const size_t pts = c->poolPoints();
+    hipError_t e = hipSuccess;
+    (void)hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = dalloc(c->dLow, pts);
+    if (e == hipSuccess) e = dalloc(c->dMed, pts);
+    if (e == hipSuccess) e = dalloc(c->dHigh, pts);
+    if (e == hipSuccess) e = dalloc(c->dColors, pts);
+    if (e == hipSuccess) e = dalloc(c->dLevels, pts);
+    if (e == hipSuccess) e = dalloc(c->dBatches, (size_t)maxResidentBatches);
+    if (e == hipSuccess) e = dalloc(c->dFiles, c->files.size());
+    if (e == hipSuccess) e = dalloc(c->dVisible, (size_t)maxResidentBatches);
+    if (e == hipSuccess) e = dalloc(c->dCounter, 1);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hFiles, c->files.size() * sizeof(scr_raster_file));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->filesCopied, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMemsetAsync(c->dLevels, 0, pts, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->dCounter, 0, sizeof(uint32_t), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->dBatches, c->mirror.data(), c->mirror.size() * sizeof(scr_raster_batch), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->dFiles, c->files.data(), c->files.size() * sizeof(scr_raster_file), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        release_cloud(c);
+        delete c;
+        return hip_error(ctx, e, "scr_cloud_create");
+    }
+    *out = c;
+    return SCR_OK;
+}
+
+int scr_cloud_destroy(scr_cloud* c) {
+    if (!c) return SCR_OK;
+    (void)hipStreamSynchronize(c->ctx->stream);
+    release_cloud(c);
+    delete c;
+    return SCR_OK;
+}
+
+int scr_cloud_get_info(const scr_cloud* c, scr_cloud_info* out) {
+    if (!c || !out) return SCR_ERR_INVALID_ARGUMENT;
+    out->maxResidentBatches = c->maxResidentBatches;
+    out->pointsPerBatch = c->pointsPerBatch;
+    out->residentBatchCount = c->residentBatchCount;
+    out->freeSlotCount = (int32_t)c->freeSlots.size();
+    out->pointCount = c->pointCount;
+    out->nFiles = (int32_t)c->files.size();
+    out->hasTints = c->dTints != nullptr;
+    return SCR_OK;
+}
+
+int scr_cloud_add_batches(scr_cloud* c, const uint32_t* low, const uint32_t* med, const uint32_t* high,
+                          const uint32_t* colors, const uint8_t* levels, size_t nSourcePoints,
+                          const scr_raster_batch* batches, int nBatches, int commit, int32_t* outSlots) {
+    if (!c) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = c->ctx;
+    if (nBatches < 0 || (nBatches > 0 && !batches)) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "bad batches");
+    if ((size_t)nBatches > c->freeSlots.size()) {
+        char buf[128];
+        std::snprintf(buf, sizeof buf, "addBatches: not enough free slots (have %zu, need %d)", c->freeSlots.size(),
+                      nBatches);
+        return set_error(ctx, SCR_ERR_NO_FREE_SLOTS, buf);
+    }
+    if (nBatches == 0) return SCR_OK;
+    // validate everything before mutating (the reference traps part-way)
+    for (int i = 0; i < nBatches; i++) {
+        const scr_raster_batch& b = batches[i];
+        if ((int64_t)b.numPoints > c->pointsPerBatch)
+            return set_error(ctx, SCR_ERR_BATCH_TOO_LARGE, "batch numPoints exceeds slot capacity");
+        if ((uint64_t)b.firstPoint + b.numPoints > nSourcePoints)
+            return set_error(ctx, SCR_ERR_SOURCE_TOO_SMALL, "addBatches: source data too small for batch");
+        if (b.numPoints > 0 && (!low || !med || !high || !colors || !levels))
+            return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "addBatches: NULL source array");
+    }
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int> slots((size_t)nBatches);
+    for (int i = 0; i < nBatches; i++) {
+        slots[(size_t)i] = c->freeSlots.back();
+        c->freeSlots.pop_back();
+    }
+    const size_t ppb = (size_t)c->pointsPerBatch;
+    // copy runs of adjacent slots fed by adjacent full source batches in one go
+    int i = 0;
+    while (i < nBatches) {
+        int j = i;
+        while (j + 1 < nBatches && slots[(size_t)j + 1] == slots[(size_t)j] + 1 &&
+               batches[j].numPoints == ppb &&
+               (uint64_t)batches[j + 1].firstPoint == (uint64_t)batches[j].firstPoint + ppb)
+            j++;
+        const size_t src = batches[i].firstPoint;
+        const size_t dst = (size_t)slots[(size_t)i] * ppb;
+        const size_t n = (size_t)(j - i) * ppb + batches[j].numPoints;
+        if (n > 0) {
+            SCR_HIP(ctx, hipMemcpy(c->dLow + dst, low + src, n * 4, hipMemcpyHostToDevice));
+            SCR_HIP(ctx, hipMemcpy(c->dMed + dst, med + src, n * 4, hipMemcpyHostToDevice));
+            SCR_HIP(ctx, hipMemcpy(c->dHigh + dst, high + src, n * 4, hipMemcpyHostToDevice));
+            SCR_HIP(ctx, hipMemcpy(c->dColors + dst, colors + src, n * 4, hipMemcpyHostToDevice));
+            SCR_HIP(ctx, hipMemcpy(c->dLevels + dst, levels + src, n, hipMemcpyHostToDevice));
+        }
+        i = j + 1;
+    }
+    for (int k = 0; k < nBatches; k++) {
+        const int slot = slots[(size_t)k];
+        scr_raster_batch b = batches[k];
+        b.firstPoint = (uint32_t)((size_t)slot * ppb);
+        b.state = 1;
+        c->mirror[(size_t)slot] = b;
+        mark_dirty(c, slot);
+        c->residentBatchCount += 1;
+        c->pointCount += b.numPoints;
+        if (outSlots) outSlots[k] = slot;
+    }
+    if (commit) return flush_mirror(c);
+    return SCR_OK;
+}
+
+int scr_cloud_remove_batches(scr_cloud* c, const int32_t* slots, int n, int commit) {
+    if (!c) return SCR_ERR_INVALID_ARGUMENT;
+    if (n <= 0) return SCR_OK;
+    if (!slots) return set_error(c->ctx, SCR_ERR_INVALID_ARGUMENT, "slots is NULL");
+    for (int i = 0; i < n; i++)
+        if (slots[i] < 0 || slots[i] >= c->maxResidentBatches)
+            return set_error(c->ctx, SCR_ERR_SLOT_RANGE, "removeBatches: slot out of range");
+    for (int i = 0; i < n; i++) {
+        const int s = slots[i];
+        scr_raster_batch& b = c->mirror[(size_t)s];
+        if (b.state == 0) continue;
+        c->pointCount -= b.numPoints;
+        c->residentBatchCount -= 1;
+        b.state = 0;
+        b.numPoints = 0;
+        mark_dirty(c, s);
+        c->freeSlots.push_back(s);
+    }
+    if (commit) return flush_mirror(c);
+    return SCR_OK;
+}
+
+int scr_cloud_commit(scr_cloud* c) {
+    if (!c) return SCR_ERR_INVALID_ARGUMENT;
+    return flush_mirror(c);
+}
+
+int scr_cloud_clear(scr_cloud* c) {
+    if (!c) return SCR_ERR_INVALID_ARGUMENT;
+    for (auto& b : c->mirror) {
+        b.state = 0;
+        b.numPoints = 0;
+    }
+    c->freeSlots.resize((size_t)c->maxResidentBatches);
+    for (int i = 0; i < c->maxResidentBatches; i++) c->freeSlots[(size_t)i] = c->maxResidentBatches - 1 - i;
+    c->residentBatchCount = 0;
+    c->pointCount = 0;
+    c->dirtyLo = 0;
+    c->dirtyHi = c->maxResidentBatches - 1;
+    return flush_mirror(c);
+}
+
+int scr_cloud_set_model(scr_cloud* c, const float model[16]) {
+    if (!c || !model) return SCR_ERR_INVALID_ARGUMENT;
+    std::memcpy(c->model, model, sizeof c->model);
+    return SCR_OK;
+}
+
+int scr_cloud_update_files(scr_cloud* c, const float vp[16], const float model[16], const float* frustum16,
+                           const float* prevVP16) {
+    if (!c || !vp || !model) return SCR_ERR_INVALID_ARGUMENT;
+    return upload_files(c, vp, model, frustum16, prevVP16);
+}
+
+int scr_cloud_set_tints(scr_cloud* c, const float* rgba, size_t nPoints, int onDevice) {
+    if (!c) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = c->ctx;
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!rgba) {
+        if (c->tintsOwned) dfree(c->dTints);
+        c->dTints = nullptr;
+        c->tintsOwned = false;
+        return SCR_OK;
+    }
+    if (nPoints != c->poolPoints())
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "tint buffer must hold maxResidentBatches * pointsPerBatch");
+    if (!c->dTints || !c->tintsOwned) {
+        c->dTints = nullptr;
+        SCR_HIP(ctx, dalloc(c->dTints, nPoints));
+        c->tintsOwned = true;
+    }
+    SCR_HIP(ctx, hipMemcpy(c->dTints, rgba, nPoints * sizeof(float4),
+                           onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    return SCR_OK;
+}
+
+int scr_cloud_read_pool(scr_cloud* c, int which, void* host, size_t bytes) {
+    if (!c || !host) return SCR_ERR_INVALID_ARGUMENT;
+    const void* src = nullptr;
+    size_t need = 0;
+    const size_t pts = c->poolPoints();
+    switch (which) {
+        case SCR_POOL_XYZ_LOW: src = c->dLow; need = pts * 4; break;
+        case SCR_POOL_XYZ_MED: src = c->dMed; need = pts * 4; break;
+        case SCR_POOL_XYZ_HIGH: src = c->dHigh; need = pts * 4; break;
+        case SCR_POOL_COLORS: src = c->dColors; need = pts * 4; break;
+        case SCR_POOL_LEVELS: src = c->dLevels; need = pts; break;
+        case SCR_POOL_BATCHES: src = c->dBatches; need = (size_t)c->maxResidentBatches * sizeof(scr_raster_batch); break;
+        default: return set_error(c->ctx, SCR_ERR_INVALID_ARGUMENT, "unknown pool buffer");
+    }
+    if (bytes < need) return set_error(c->ctx, SCR_ERR_INVALID_ARGUMENT, "host buffer too small");
+    SCR_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
+    SCR_HIP(c->ctx, hipMemcpy(host, src, need, hipMemcpyDeviceToHost));
+    return SCR_OK;
+}
+
+int scr_cloud_read_visible(scr_cloud* c, scr_visible_batch* host, int capacity, int* count) {
+    if (!c || !count) return SCR_ERR_INVALID_ARGUMENT;
+    uint32_t n = 0;
+    SCR_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
+    SCR_HIP(c->ctx, hipMemcpy(&n, c->dCounter, 4, hipMemcpyDeviceToHost));
+    *count = (int)n;
+    const int k = std::min((int)n, capacity);
+    if (k > 0 && host) SCR_HIP(c->ctx, hipMemcpy(host, c->dVisible, (size_t)k * sizeof(scr_visible_batch), hipMemcpyDeviceToHost));
+    return SCR_OK;
+}
+
+// ------------------------------------------------------------------ frames
+
+int scr_frame_create(scr_ctx* ctx, int width, int height, scr_frame** out) {
+    if (!ctx || !out) return SCR_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (int64_t)1 << 31)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "bad frame size");
+    scr_frame* f = new (std::nothrow) scr_frame();
+    if (!f) return SCR_ERR_OUT_OF_MEMORY;
+    f->ctx = ctx;
+    f->width = width;
+    f->height = height;
+    const size_t n = f->pixels();
+    hipError_t e = dalloc(f->dPlanes, 5 * f->planeStride());
+    if (e == hipSuccess) e = dalloc(f->dNearest, n);
+    if (e == hipSuccess) e = dalloc(f->dRGBA, n);
+    if (e == hipSuccess) e = dalloc(f->dDepthF, n);
+    if (e == hipSuccess) e = scr::launch_fill_u32(ctx->stream, f->dPlanes, 5 * f->planeStride(), 0u);
+    if (e == hipSuccess) e = scr::launch_fill_u64(ctx->stream, f->dNearest, n, 0ull);
+    if (e == hipSuccess) e = scr::launch_fill_u32(ctx->stream, f->dRGBA, n, 0u);
+    if (e == hipSuccess) e = scr::launch_fill_u32(ctx->stream, (uint32_t*)f->dDepthF, n, 0u);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        dfree(f->dPlanes);
+        dfree(f->dNearest);
+        dfree(f->dRGBA);
+        dfree(f->dDepthF);
+        delete f;
+        return hip_error(ctx, e, "scr_frame_create");
+    }
+    *out = f;
+    return SCR_OK;
+}
+
+int scr_frame_destroy(scr_frame* f) {
+    if (!f) return SCR_OK;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    dfree(f->dPlanes);
+    dfree(f->dNearest);
+    dfree(f->dRGBA);
+    dfree(f->dDepthF);
+    delete f;
+    return SCR_OK;
+}
+
+int scr_frame_read(scr_frame* f, int which, void* host, size_t bytes) {
+    if (!f || !host) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = f->ctx;
+    const size_t n = f->pixels();
+    if (bytes < n * 4) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "host buffer too small");
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (which >= SCR_PLANE_DEPTH && which <= SCR_PLANE_COUNT) {
+        SCR_HIP(ctx, hipMemcpy(host, f->plane(which), n * 4, hipMemcpyDeviceToHost));
+    } else if (which == SCR_PLANE_NEAREST_DEPTH || which == SCR_PLANE_NEAREST_INDEX) {
+        std::vector<unsigned long long> tmp(n);
+        SCR_HIP(ctx, hipMemcpy(tmp.data(), f->dNearest, n * 8, hipMemcpyDeviceToHost));
+        uint32_t* o = (uint32_t*)host;
+        for (size_t i = 0; i < n; i++)
+            o[i] = which == SCR_PLANE_NEAREST_DEPTH ? (uint32_t)(tmp[i] >> 32) : ~(uint32_t)tmp[i];
+    } else if (which == SCR_PLANE_RGBA8) {
+        SCR_HIP(ctx, hipMemcpy(host, f->dRGBA, n * 4, hipMemcpyDeviceToHost));
+    } else if (which == SCR_PLANE_DEPTH_F32) {
+        SCR_HIP(ctx, hipMemcpy(host, f->dDepthF, n * 4, hipMemcpyDeviceToHost));
+    } else {
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown plane");
+    }
+    return SCR_OK;
+}
+
+// ------------------------------------------------------------------ render
+
+int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* const* clouds, int nClouds,
+               scr_frame* frame) {
+    if (!ctx || !cam || !frame || nClouds < 0 || (nClouds > 0 && !clouds)) return SCR_ERR_INVALID_ARGUMENT;
+    int rc = check_config(ctx, cfg);
+    if (rc != SCR_OK) return rc;
+    for (int i = 0; i < nClouds; i++)
+        if (!clouds[i] || clouds[i]->ctx != ctx) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "bad cloud handle");
+    const scr::PassParams p = make_params(cfg, cam, frame->width, frame->height);
+    float vp[16];
+    scr::mat_mul(cam->projectionMatrix, cam->viewMatrix, vp);
+    hipStream_t s = ctx->stream;
+
+    if (cfg->mode == SCR_MODE_NEAREST_POINT) {
+        // encodeNearestPoint: first cloud with resident batches only (:483-497)
+        scr_cloud* c = nullptr;
+        for (int i = 0; i < nClouds && !c; i++)
+            if (clouds[i]->residentBatchCount > 0) c = clouds[i];
+        SCR_HIP(ctx, scr::launch_fill_u64(s, frame->dNearest, frame->pixels(), 0ull));
+        const uint32_t* colors = nullptr;
+        if (c) {
+            rc = upload_files(c, vp, c->model, nullptr, nullptr);
+            if (rc != SCR_OK) return rc;
+            const CloudDev d = c->dev();
+            SCR_HIP(ctx, scr::launch_cull(s, d, p));
+            SCR_HIP(ctx, scr::launch_nearest(s, ctx->numCUs, d, p, frame->dNearest));
+            colors = c->dColors;
+        }
+        SCR_HIP(ctx, scr::launch_nearest_resolve(s, p, frame->dNearest, colors, frame->dRGBA, frame->dDepthF));
+        return SCR_OK;
+    }
+
+    // encodeHighQualityAverage (:322-456)
+    SCR_HIP(ctx, scr::launch_fill_u32(s, frame->dPlanes, 5 * frame->planeStride(), 0u));
+    std::vector<scr_cloud*> resident;
+    for (int i = 0; i < nClouds; i++)
+        if (clouds[i]->residentBatchCount > 0) resident.push_back(clouds[i]);
+    for (scr_cloud* c : resident) {
+        rc = upload_files(c, vp, c->model, nullptr, nullptr);
+        if (rc != SCR_OK) return rc;
+    }
+    for (scr_cloud* c : resident) SCR_HIP(ctx, scr::launch_cull(s, c->dev(), p));
+    for (scr_cloud* c : resident) SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
+    for (scr_cloud* c : resident)
+        SCR_HIP(ctx, scr::launch_color(s, ctx->numCUs, c->dev(), p, frame->plane(0), frame->plane(1), frame->plane(2),
+                                       frame->plane(3), frame->plane(4)));
+    SCR_HIP(ctx, scr::launch_resolve(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA, frame->dDepthF));
+    return SCR_OK;
+}
+
+int scr_pick(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud, scr_frame* frame,
+             float ndcX, float ndcY, int searchRadius, uint32_t* outIndex, int* found) {
+    if (!ctx || !cam || !cloud || !frame || !outIndex || !found) return SCR_ERR_INVALID_ARGUMENT;
+    *found = 0;
+    const int W = frame->width, H = frame->height;
+    if (cloud->residentBatchCount <= 0) return SCR_OK;
+    scr_config c2 = *cfg;
+    c2.mode = SCR_MODE_NEAREST_POINT;
+    int rc = check_config(ctx, &c2);
+    if (rc != SCR_OK) return rc;
+    // NDC (y up) → buffer pixel (row 0 at the top), ComputeRasteriser.swift:617-621
+    int px = scr::f2i((ndcX * 0.5f + 0.5f) * (float)W);
+    px = std::min(std::max(px, 0), W - 1);
+    const int pyb = scr::f2i((ndcY * 0.5f + 0.5f) * (float)H);
+    const int py = std::min(std::max(H - 1 - pyb, 0), H - 1);
+    const scr::PassParams p = make_params(&c2, cam, W, H);
+    float vp[16];
+    scr::mat_mul(cam->projectionMatrix, cam->viewMatrix, vp);
+    hipStream_t s = ctx->stream;
+    rc = upload_files(cloud, vp, cloud->model, nullptr, nullptr);
+    if (rc != SCR_OK) return rc;
+    const CloudDev d = cloud->dev();
+    SCR_HIP(ctx, scr::launch_fill_u64(s, frame->dNearest, frame->pixels(), 0ull));
+    SCR_HIP(ctx, scr::launch_cull(s, d, p));
+    SCR_HIP(ctx, scr::launch_nearest(s, ctx->numCUs, d, p, frame->dNearest));
+    const int r = std::max(0, searchRadius);
+    const int y0 = std::max(0, py - r), y1 = std::min(H - 1, py + r);
+    const size_t rows = (size_t)(y1 - y0 + 1);
+    std::vector<unsigned long long> band(rows * (size_t)W);
+    SCR_HIP(ctx, hipStreamSynchronize(s));
+    SCR_HIP(ctx, hipMemcpy(band.data(), frame->dNearest + (size_t)y0 * W, band.size() * 8, hipMemcpyDeviceToHost));
+    long long best = 0x7fffffffffffffffLL;
+    const long long r2 = (long long)r * r;
+    for (int y = y0; y <= y1; y++) {
+        const int dy = y - py;
+        for (int x = std::max(0, px - r); x <= std::min(W - 1, px + r); x++) {
+            const int dx = x - px;
+            const long long dist = (long long)dx * dx + (long long)dy * dy;
+            if (dist > r2 || dist >= best) continue;
+            const uint32_t idx = ~(uint32_t)band[(size_t)(y - y0) * W + x];
+            if (idx != 0xffffffffu) {
+                best = dist;
+                *outIndex = idx;
+                *found = 1;
+            }
+        }
+    }
+    return SCR_OK;
+}
+
+}  // extern "C"

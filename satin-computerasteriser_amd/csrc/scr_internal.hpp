@@ -1,0 +1,117 @@
+// scr_internal.hpp — host-side state behind the C ABI (context, point-cloud
+// slot pool, per-viewport frame buffers) and the kernel launch interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/scr.h"
+#include "scr_math.hpp"
+
+struct scr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int numCUs = 0;
+    std::string lastError;
+    // scratch owned by the context (grown on demand, freed at destroy)
+    void* scratch = nullptr;
+    size_t scratchBytes = 0;
+};
+
+// Device view of one cloud, passed by value to the point kernels.
+struct CloudDev {
+    const scr_raster_batch* batches;
+    const scr_raster_file* files;
+    const uint32_t* xyzLow;
+    const uint32_t* xyzMed;
+    const uint32_t* xyzHigh;
+    const uint32_t* colors;
+    const uint8_t* levels;
+    const float4* tints;           // nullptr: zero stand-in
+    scr_visible_batch* visible;    // cull output
+    uint32_t* counter;             // visible count
+    int nSlots;
+    int nFiles;
+};
+
+struct scr_cloud {
+    scr_ctx* ctx = nullptr;
+    int maxResidentBatches = 0;
+    int pointsPerBatch = 0;
+    std::vector<scr_raster_file> files;  // base table (world per file)
+    float model[16];
+    std::vector<scr_raster_batch> mirror;
+    int dirtyLo = 0x7fffffff, dirtyHi = -1;
+    std::vector<int> freeSlots;  // LIFO: back() is taken first
+    int residentBatchCount = 0;
+    int64_t pointCount = 0;
+    // device buffers
+    uint32_t* dLow = nullptr;
+    uint32_t* dMed = nullptr;
+    uint32_t* dHigh = nullptr;
+    uint32_t* dColors = nullptr;
+    uint8_t* dLevels = nullptr;
+    scr_raster_batch* dBatches = nullptr;
+    scr_raster_file* dFiles = nullptr;
+    scr_raster_file* hFiles = nullptr;  // pinned staging for the composed table
+    hipEvent_t filesCopied = nullptr;
+    float4* dTints = nullptr;
+    bool tintsOwned = false;
+    scr_visible_batch* dVisible = nullptr;
+    uint32_t* dCounter = nullptr;
+
+    size_t poolPoints() const { return (size_t)maxResidentBatches * (size_t)pointsPerBatch; }
+    CloudDev dev() const {
+        CloudDev c;
+        c.batches = dBatches;
+        c.files = dFiles;
+        c.xyzLow = dLow;
+        c.xyzMed = dMed;
+        c.xyzHigh = dHigh;
+        c.colors = dColors;
+        c.levels = dLevels;
+        c.tints = dTints;
+        c.visible = dVisible;
+        c.counter = dCounter;
+        c.nSlots = maxResidentBatches;
+        c.nFiles = (int)files.size();
+        return c;
+    }
+};
+
+struct scr_frame {
+    scr_ctx* ctx = nullptr;
+    int width = 0, height = 0;
+    uint32_t* dPlanes = nullptr;  // depth, red, green, blue, count (SoA, 5 planes of planeStride)
+    unsigned long long* dNearest = nullptr;  // depth << 32 | ~index per pixel
+    uint32_t* dRGBA = nullptr;
+    float* dDepthF = nullptr;
+    size_t pixels() const { return (size_t)width * (size_t)height; }
+    // planes start on 256-byte boundaries so vector stores stay aligned
+    size_t planeStride() const { return (pixels() + 63) & ~(size_t)63; }
+    uint32_t* plane(int i) const { return dPlanes + (size_t)i * planeStride(); }
+};
+
+namespace scr {
+
+int set_error(scr_ctx* ctx, int code, const std::string& msg);
+int hip_error(scr_ctx* ctx, hipError_t e, const char* what);
+
+// Kernel launchers (scr_render.hip).  All enqueue on ctx->stream.
+hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
+hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p);
+hipError_t launch_depth(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, uint32_t* depth);
+hipError_t launch_color(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, const uint32_t* depth,
+                        uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count);
+hipError_t launch_resolve(hipStream_t s, const PassParams& p, const uint32_t* planes, size_t stride, uint32_t* rgba,
+                          float* depthf);
+hipError_t launch_nearest(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p,
+                          unsigned long long* packed);
+hipError_t launch_nearest_resolve(hipStream_t s, const PassParams& p, const unsigned long long* packed,
+                                  const uint32_t* colors, uint32_t* rgba, float* depthf);
+
+}  // namespace scr

@@ -1,0 +1,213 @@
+// scr_math.hpp — device arithmetic of the render passes (gfx950).
+//
+// Implements the shared arithmetic contract of the rasteriser:
+//   * binary32 RNE, no contraction (every .hip file is built with
+//     -ffp-contract=off), source operation order of the Metal passes;
+//   * a / b in the Metal passes is a * (1.0f / b) with the correctly rounded
+//     reciprocal (Metal fast-math reciprocal division made exact);
+//   * sqrt correctly rounded; log2 = scr_log2f (double-precision, + - * / only);
+//   * float→int truncates and saturates, NaN → 0;
+//   * RGBA8 bytes = rint(saturate(x) * 255).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/scr.h"
+
+namespace scr {
+
+constexpr float kTwo32 = 4294967296.0f;
+constexpr float kInvTwo32 = 2.3283064365386963e-10f;  // 2^-32, exact
+constexpr float kInv255 = 1.0f / 255.0f;
+
+__host__ __device__ inline float saturate(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+__host__ __device__ inline float rcp(float x) { return 1.0f / x; }
+
+__host__ __device__ inline uint32_t f2u(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= kTwo32) return 0xffffffffu;
+    return (uint32_t)x;
+}
+
+__host__ __device__ inline int32_t f2i(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 2147483647;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)x;
+}
+
+// Common.metal:10-12 — float(CR_MAX_DEPTH - 1u) is 2^32 in binary32.
+__host__ __device__ inline uint32_t depth_to_uint(float z) { return f2u(saturate(z) * kTwo32); }
+// Common.metal:14-16
+__host__ __device__ inline float uint_to_depth(uint32_t d) { return (float)d * kInvTwo32; }
+
+// Common.metal:190-197 (lowbias32)
+__host__ __device__ inline float hash_unit(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return (float)x * kInvTwo32;
+}
+
+// log2 of the cull pass (Common.metal:185-187): x = 2^e m, m in
+// [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s) with s = (m-1)/(m+1), series to
+// s^21; evaluated in double with + - * / only, rounded once to float.
+__host__ __device__ inline float scr_log2f(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return __builtin_nanf("");
+    if (x == 0.0f) return -__builtin_inff();
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    uint32_t expo = (u >> 23) & 0xffu;
+    if (expo == 0xffu) return x;
+    int e;
+    if (expo == 0u) {
+        const uint32_t w = __builtin_bit_cast(uint32_t, x * 16777216.0f);
+        expo = (w >> 23) & 0xffu;
+        e = (int)expo - 127 - 24;
+        u = w;
+    } else {
+        e = (int)expo - 127;
+    }
+    double m = (double)__builtin_bit_cast(float, (u & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e = e + 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 21.0;
+    p = p * s2 + 1.0 / 19.0;
+    p = p * s2 + 1.0 / 17.0;
+    p = p * s2 + 1.0 / 15.0;
+    p = p * s2 + 1.0 / 13.0;
+    p = p * s2 + 1.0 / 11.0;
+    p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0;
+    p = p * s2 + 1.0 / 5.0;
+    p = p * s2 + 1.0 / 3.0;
+    p = p * s2 + 1.0;
+    const double lnm = 2.0 * s * p;
+    return (float)((double)e + lnm * 1.4426950408889634);
+}
+
+struct f4 {
+    float x, y, z, w;
+};
+
+// Metal float4x4 * float4: ((c0 x + c1 y) + c2 z) + c3 w, column-major m[c*4+r].
+__host__ __device__ inline f4 mul(const float* m, f4 v) {
+    f4 r;
+    r.x = m[0] * v.x;
+    r.y = m[1] * v.x;
+    r.z = m[2] * v.x;
+    r.w = m[3] * v.x;
+    r.x = r.x + m[4] * v.y;
+    r.y = r.y + m[5] * v.y;
+    r.z = r.z + m[6] * v.y;
+    r.w = r.w + m[7] * v.y;
+    r.x = r.x + m[8] * v.z;
+    r.y = r.y + m[9] * v.z;
+    r.z = r.z + m[10] * v.z;
+    r.w = r.w + m[11] * v.z;
+    r.x = r.x + m[12] * v.w;
+    r.y = r.y + m[13] * v.w;
+    r.z = r.z + m[14] * v.w;
+    r.w = r.w + m[15] * v.w;
+    return r;
+}
+
+// Point (w = 1): c3 * 1 is c3 exactly.
+__host__ __device__ inline f4 mul_point(const float* m, float x, float y, float z) {
+    f4 r;
+    r.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12];
+    r.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13];
+    r.z = ((m[2] * x + m[6] * y) + m[10] * z) + m[14];
+    r.w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+    return r;
+}
+
+__host__ __device__ inline void mat_mul(const float* a, const float* b, float* out) {
+    float r[16];
+    for (int c = 0; c < 4; c++) {
+        const f4 col = mul(a, f4{b[c * 4 + 0], b[c * 4 + 1], b[c * 4 + 2], b[c * 4 + 3]});
+        r[c * 4 + 0] = col.x;
+        r[c * 4 + 1] = col.y;
+        r[c * 4 + 2] = col.z;
+        r[c * 4 + 3] = col.w;
+    }
+    for (int i = 0; i < 16; i++) out[i] = r[i];
+}
+
+__host__ __device__ inline float len3(float x, float y, float z) { return sqrtf((x * x + y * y) + z * z); }
+
+// Camera / point-size constants of one render, shared by every point pass.
+struct PassParams {
+    float view[16];
+    float proj[16];
+    int width, height;
+    int pointSizeMode;
+    float minPointSize, maxPointSize, pointSizeScale;
+    int lodDither;
+    int enableFrustumCulling;
+    int lodBias;
+    int enableCLOD;
+    float depthKeep;        // 1 - depthTolerance
+    int colorizeChunks;
+    int colorizeOverdraw;
+    int applyTint;
+    int constRadius;        // >= 0: footprint radius is this constant (lo == hi); -1: per point
+    int ortho;              // proj[3][3] == 1
+    float focal;            // float(height) * 0.5 * proj[1][1]
+    float lo, hi;           // point-size clamp
+    float bg[4];
+};
+
+// pointFootprintRadius — Common.metal:18-61 (world/view transform only when
+// the size depends on the point).
+__device__ inline int footprint_radius(const PassParams& p, const float* world, float x, float y, float z) {
+    if (p.constRadius >= 0) return p.constRadius;
+    const f4 wp = mul_point(world, x, y, z);
+    const f4 v = mul_point(p.view, wp.x, wp.y, wp.z);
+    float ps;
+    if (p.pointSizeMode == 1) {
+        if (p.ortho) {
+            ps = (2.0f * p.pointSizeScale) * p.focal;
+        } else {
+            const float viewZ = fmaxf(-v.z, 0.000001f);
+            ps = ((2.0f * p.pointSizeScale) * p.focal) * rcp(viewZ);
+        }
+    } else {
+        if (p.ortho) {
+            ps = p.pointSizeScale;
+        } else {
+            const float d = fmaxf(len3(v.x, v.y, v.z), 0.000001f);
+            ps = p.pointSizeScale * rcp(d);
+        }
+    }
+    if (!__builtin_isfinite(ps)) ps = p.lo;
+    ps = fminf(fmaxf(ps, p.lo), p.hi);
+    int r = f2i(ceilf(fmaxf(ps - 1.0f, 0.0f) * 0.5f));
+    r = r < 0 ? 0 : r;
+    return r > 16 ? 16 : r;
+}
+
+// Host-side radius for lo == hi (size clamps to lo for every point).
+__host__ inline int const_radius_for(float lo) {
+    int r = f2i(ceilf(fmaxf(lo - 1.0f, 0.0f) * 0.5f));
+    r = r < 0 ? 0 : r;
+    return r > 16 ? 16 : r;
+}
+
+// insidePointFootprint — Common.metal:63-71
+__host__ __device__ inline bool inside_footprint(int ox, int oy, int r) {
+    if (r <= 0) return ox == 0 && oy == 0;
+    const float fx = (float)ox, fy = (float)oy;
+    const float rr = (float)r + 0.5f;
+    return (fx * fx + fy * fy) <= rr * rr;
+}
+
+}  // namespace scr

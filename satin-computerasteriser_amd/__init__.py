@@ -105,6 +105,19 @@ class CCloudInfo(C.Structure):
     ]
 
 
+class CPackOptions(C.Structure):
+    _fields_ = [("keyBits", C.c_int32), ("lodLevels", C.c_int32), ("coarseVoxelDivisions", C.c_int32),
+                ("inputOnDevice", C.c_int32)]
+
+
+class CSceneParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("halfExtent", C.c_float), ("numBoxes", C.c_int32), ("terrainFraction", C.c_float),
+        ("amplitude", C.c_float), ("baseWavelength", C.c_float), ("octaves", C.c_int32), ("boxMinEdge", C.c_float),
+        ("boxMaxEdge", C.c_float), ("colorNoise", C.c_float), ("quantizeColors", C.c_int32),
+    ]
+
+
 _VP = C.c_void_p
 _I = C.c_int
 _SZ = C.c_size_t
@@ -131,6 +144,11 @@ _SIGNATURES = {
     "scr_cloud_set_tints": (_I, [_VP, _VP, _SZ, _I]),
     "scr_cloud_read_pool": (_I, [_VP, _I, _VP, _SZ]),
     "scr_cloud_read_visible": (_I, [_VP, _VP, _I, C.POINTER(_I)]),
+    "scr_pack_options_default": (None, [C.POINTER(CPackOptions)]),
+    "scr_pack": (_I, [_VP, _VP, _VP, _SZ, C.POINTER(CPackOptions), _VP]),
+    "scr_pack_chunk": (_I, [_VP, _VP, _VP, _SZ, C.POINTER(CPackOptions), C.POINTER(C.c_int32)]),
+    "scr_scene_params_default": (None, [C.POINTER(CSceneParams)]),
+    "scr_gen_scene": (_I, [_VP, C.POINTER(CSceneParams), _SZ, _VP, _VP, _I]),
     "scr_frame_create": (_I, [_VP, _I, _I, C.POINTER(_VP)]),
     "scr_frame_destroy": (_I, [_VP]),
     "scr_frame_read": (_I, [_VP, _I, _VP, _SZ]),
@@ -586,6 +604,93 @@ class ComputeRasteriser:
                                              self.frame.handle, float(ndc[0]), float(ndc[1]), int(searchRadius),
                                              C.byref(out), C.byref(found)))
         return int(out.value) if found.value else None
+
+
+# ---------------------------------------------------------------- packer
+def _float4(a: np.ndarray, fill: float) -> np.ndarray:
+    a = np.asarray(a, np.float32)
+    if a.ndim == 2 and a.shape[1] == 4:
+        return np.ascontiguousarray(a)
+    out = np.full((a.shape[0], 4), fill, np.float32)
+    out[:, : a.shape[1]] = a
+    return out
+
+
+class GPUPacker:
+    """GPUPacker.swift: Morton radix-sort packer on the GPU.
+
+    keyBits 30 sorts on the reference's 10-bit-per-axis Morton keys, 63 on
+    21-bit-per-axis keys.  Equal keys keep input order (stable LSD sort).
+    """
+
+    def __init__(self, ctx: Context, lodLevels: int = 4, coarseVoxelDivisions: int = 64, keyBits: int = 30):
+        self.ctx = ctx
+        self.lodLevels = lodLevels
+        self.coarseVoxelDivisions = coarseVoxelDivisions
+        self.keyBits = keyBits
+
+    def _opts(self, on_device: bool) -> CPackOptions:
+        o = CPackOptions()
+        o.keyBits, o.lodLevels, o.coarseVoxelDivisions = self.keyBits, self.lodLevels, self.coarseVoxelDivisions
+        o.inputOnDevice = int(on_device)
+        return o
+
+    def pack(self, cloud: "ComputeRasteriserPointCloud", positions, colors, sourceIndices: bool = False):
+        """replacePackedPointCloud(packer:queue:positions:colors:count:) — GPUPacker.swift:507-519."""
+        pos, col = _float4(positions, 0.0), _float4(colors, 1.0)
+        n = pos.shape[0]
+        src = np.zeros(max(n, 1), np.uint32) if sourceIndices else None
+        o = self._opts(False)
+        self.ctx.check(self.ctx.lib.scr_pack(cloud.handle, pos.ctypes.data, col.ctypes.data, n, C.byref(o),
+                                             _ptr(src)))
+        return src[:n] if sourceIndices else None
+
+    def packDevice(self, cloud: "ComputeRasteriserPointCloud", positions: "DeviceBuffer", colors: "DeviceBuffer",
+                   count: int) -> None:
+        o = self._opts(True)
+        self.ctx.check(self.ctx.lib.scr_pack(cloud.handle, positions.ptr, colors.ptr, int(count), C.byref(o), None))
+
+    def addRawChunksGPU(self, cloud: "ComputeRasteriserPointCloud", chunks) -> list:
+        """addRawChunksGPU(packer:queue:chunks:) — GPUPacker.swift:545-577; returns first slots."""
+        placed = []
+        for positions, colors in chunks:
+            pos, col = _float4(positions, 0.0), _float4(colors, 1.0)
+            if pos.shape[0] == 0:
+                continue
+            first = C.c_int32(-1)
+            o = self._opts(False)
+            rc = self.ctx.lib.scr_pack_chunk(cloud.handle, pos.ctypes.data, col.ctypes.data, pos.shape[0],
+                                             C.byref(o), C.byref(first))
+            if rc == -2:  # no contiguous run free: skipped, as the reference does
+                continue
+            self.ctx.check(rc)
+            placed.append(int(first.value))
+        return placed
+
+
+# ---------------------------------------------------------------- synthetic scenes
+def scene_params(**kw) -> CSceneParams:
+    p = CSceneParams()
+    load_library().scr_scene_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def gen_scene(ctx: Context, count: int, on_device: bool = True, **kw):
+    """Synthetic LiDAR-like scene (float4 positions, float4 colours).
+
+    on_device=True returns two DeviceBuffers (inputs stay in HBM); otherwise
+    two (count, 4) float32 numpy arrays."""
+    p = scene_params(**kw)
+    if on_device:
+        pos, col = DeviceBuffer(ctx, count * 16), DeviceBuffer(ctx, count * 16)
+        ctx.check(ctx.lib.scr_gen_scene(ctx.handle, C.byref(p), count, pos.ptr, col.ptr, 1))
+        return pos, col
+    pos = np.zeros((count, 4), np.float32)
+    col = np.zeros((count, 4), np.float32)
+    ctx.check(ctx.lib.scr_gen_scene(ctx.handle, C.byref(p), count, pos.ctypes.data, col.ctypes.data, 0))
+    return pos, col
 
 
 def version() -> str:

@@ -49,18 +49,12 @@ hipError_t dalloc(T*& p, size_t count) {
     return hipMalloc((void**)&p, count * sizeof(T));
 }
 
+void free_pool(scr_cloud* c);
+
 void release_cloud(scr_cloud* c) {
     if (!c) return;
-    dfree(c->dLow);
-    dfree(c->dMed);
-    dfree(c->dHigh);
-    dfree(c->dColors);
-    dfree(c->dLevels);
-    dfree(c->dBatches);
+    free_pool(c);
     dfree(c->dFiles);
-    if (c->tintsOwned) dfree(c->dTints);
-    c->dTints = nullptr;
-    dfree(c->dVisible);
     dfree(c->dCounter);
     if (c->hFiles) (void)hipHostFree(c->hFiles);
     c->hFiles = nullptr;
@@ -106,6 +100,49 @@ int upload_files(scr_cloud* c, const float* vp, const float* model, const float*
                                    hipMemcpyHostToDevice, c->ctx->stream));
     SCR_HIP(c->ctx, hipEventRecord(c->filesCopied, c->ctx->stream));
     return SCR_OK;
+}
+
+// (Re)allocate every per-slot / per-point device buffer for the current
+// capacity and reset the pool to empty (reallocate + allocateBuffers,
+// ComputeRasteriserPointCloud.swift:519-547).
+hipError_t alloc_pool(scr_cloud* c) {
+    scr_ctx* ctx = c->ctx;
+    const size_t pts = c->poolPoints();
+    const int n = c->maxResidentBatches;
+    scr_raster_batch empty;
+    std::memset(&empty, 0, sizeof empty);
+    c->mirror.assign((size_t)n, empty);
+    c->freeSlots.resize((size_t)n);
+    for (int i = 0; i < n; i++) c->freeSlots[(size_t)i] = n - 1 - i;
+    c->residentBatchCount = 0;
+    c->pointCount = 0;
+    c->dirtyLo = 0x7fffffff;
+    c->dirtyHi = -1;
+    hipError_t e = dalloc(c->dLow, pts);
+    if (e == hipSuccess) e = dalloc(c->dMed, pts);
+    if (e == hipSuccess) e = dalloc(c->dHigh, pts);
+    if (e == hipSuccess) e = dalloc(c->dColors, pts);
+    if (e == hipSuccess) e = dalloc(c->dLevels, pts);
+    if (e == hipSuccess) e = dalloc(c->dBatches, (size_t)n);
+    if (e == hipSuccess) e = dalloc(c->dVisible, (size_t)n);
+    if (e == hipSuccess) e = hipMemsetAsync(c->dLevels, 0, pts, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess)
+        e = hipMemcpy(c->dBatches, c->mirror.data(), c->mirror.size() * sizeof(scr_raster_batch), hipMemcpyHostToDevice);
+    return e;
+}
+
+void free_pool(scr_cloud* c) {
+    dfree(c->dLow);
+    dfree(c->dMed);
+    dfree(c->dHigh);
+    dfree(c->dColors);
+    dfree(c->dLevels);
+    dfree(c->dBatches);
+    dfree(c->dVisible);
+    if (c->tintsOwned) dfree(c->dTints);
+    c->dTints = nullptr;
+    c->tintsOwned = false;
 }
 
 int check_config(scr_ctx* ctx, const scr_config* cfg) {
@@ -266,30 +303,14 @@ int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, c
     } else {
         c->files.assign(files, files + nFiles);
     }
-    scr_raster_batch empty;
-    std::memset(&empty, 0, sizeof empty);
-    c->mirror.assign((size_t)maxResidentBatches, empty);
-    c->freeSlots.resize((size_t)maxResidentBatches);
-    for (int i = 0; i < maxResidentBatches; i++) c->freeSlots[(size_t)i] = maxResidentBatches - 1 - i;
-    const size_t pts = c->poolPoints();
-    hipError_t e = hipSuccess;
     (void)hipSetDevice(ctx->device);
-    if (e == hipSuccess) e = dalloc(c->dLow, pts);
-    if (e == hipSuccess) e = dalloc(c->dMed, pts);
-    if (e == hipSuccess) e = dalloc(c->dHigh, pts);
-    if (e == hipSuccess) e = dalloc(c->dColors, pts);
-    if (e == hipSuccess) e = dalloc(c->dLevels, pts);
-    if (e == hipSuccess) e = dalloc(c->dBatches, (size_t)maxResidentBatches);
+    hipError_t e = alloc_pool(c);
     if (e == hipSuccess) e = dalloc(c->dFiles, c->files.size());
-    if (e == hipSuccess) e = dalloc(c->dVisible, (size_t)maxResidentBatches);
     if (e == hipSuccess) e = dalloc(c->dCounter, 1);
     if (e == hipSuccess) e = hipHostMalloc((void**)&c->hFiles, c->files.size() * sizeof(scr_raster_file));
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->filesCopied, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMemsetAsync(c->dLevels, 0, pts, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->dCounter, 0, sizeof(uint32_t), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess)
-        e = hipMemcpy(c->dBatches, c->mirror.data(), c->mirror.size() * sizeof(scr_raster_batch), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(c->dFiles, c->files.data(), c->files.size() * sizeof(scr_raster_file), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -490,6 +511,153 @@ int scr_cloud_read_visible(scr_cloud* c, scr_visible_batch* host, int capacity, 
     *count = (int)n;
     const int k = std::min((int)n, capacity);
     if (k > 0 && host) SCR_HIP(c->ctx, hipMemcpy(host, c->dVisible, (size_t)k * sizeof(scr_visible_batch), hipMemcpyDeviceToHost));
+    return SCR_OK;
+}
+
+// ------------------------------------------------------------------ GPU packer
+
+void scr_pack_options_default(scr_pack_options* o) {
+    if (!o) return;
+    o->keyBits = 30;
+    o->lodLevels = 4;
+    o->coarseVoxelDivisions = 64;
+    o->inputOnDevice = 0;
+}
+
+namespace {
+
+// Device copies of host inputs (freed by the destructor).
+struct PackInputs {
+    const float4* pos = nullptr;
+    const float4* col = nullptr;
+    float4* ownPos = nullptr;
+    float4* ownCol = nullptr;
+    ~PackInputs() {
+        if (ownPos) (void)hipFree(ownPos);
+        if (ownCol) (void)hipFree(ownCol);
+    }
+};
+
+int stage_inputs(scr_ctx* ctx, const float* p, const float* c, size_t n, bool onDevice, PackInputs& in) {
+    if (onDevice) {
+        in.pos = (const float4*)p;
+        in.col = (const float4*)c;
+        return SCR_OK;
+    }
+    SCR_HIP(ctx, hipMalloc((void**)&in.ownPos, n * sizeof(float4)));
+    SCR_HIP(ctx, hipMalloc((void**)&in.ownCol, n * sizeof(float4)));
+    SCR_HIP(ctx, hipMemcpy(in.ownPos, p, n * sizeof(float4), hipMemcpyHostToDevice));
+    SCR_HIP(ctx, hipMemcpy(in.ownCol, c, n * sizeof(float4), hipMemcpyHostToDevice));
+    in.pos = in.ownPos;
+    in.col = in.ownCol;
+    return SCR_OK;
+}
+
+int check_pack(scr_cloud* c, const float* p, const float* col, size_t count, const scr_pack_options* opt) {
+    if (!c) return SCR_ERR_INVALID_ARGUMENT;
+    if (!opt) return set_error(c->ctx, SCR_ERR_INVALID_ARGUMENT, "pack options are NULL");
+    if (opt->keyBits != 30 && opt->keyBits != 63) return set_error(c->ctx, SCR_ERR_INVALID_ARGUMENT, "keyBits must be 30 or 63");
+    if (count && (!p || !col)) return set_error(c->ctx, SCR_ERR_INVALID_ARGUMENT, "NULL positions/colors");
+    if (count > 0xffffffffull) return set_error(c->ctx, SCR_ERR_INVALID_ARGUMENT, "more than 2^32 - 1 points");
+    return SCR_OK;
+}
+
+scr::PackDest dest_at(scr_cloud* c, int firstSlot) {
+    const size_t base = (size_t)firstSlot * (size_t)c->pointsPerBatch;
+    scr::PackDest d;
+    d.low = c->dLow + base;
+    d.med = c->dMed + base;
+    d.high = c->dHigh + base;
+    d.colors = c->dColors + base;
+    d.levels = c->dLevels + base;
+    d.batches = c->dBatches + firstSlot;
+    d.slotBase = (uint32_t)firstSlot;
+    d.ppb = (uint32_t)c->pointsPerBatch;
+    return d;
+}
+
+}  // namespace
+
+int scr_pack(scr_cloud* c, const float* positions4, const float* colors4, size_t count, const scr_pack_options* opt,
+             uint32_t* sourceIndicesOut) {
+    int rc = check_pack(c, positions4, colors4, count, opt);
+    if (rc != SCR_OK) return rc;
+    scr_ctx* ctx = c->ctx;
+    if (count == 0) return scr_cloud_clear(c);
+    const size_t ppb = (size_t)c->pointsPerBatch;
+    const size_t nb = (count + ppb - 1) / ppb;
+    if (nb > 0x7fffffffull) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "too many batches");
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if ((int)nb > c->maxResidentBatches) {  // grow the pool, keeping pointsPerBatch
+        free_pool(c);
+        c->maxResidentBatches = (int)nb;
+        SCR_HIP(ctx, alloc_pool(c));
+    }
+    // prepareForGPUPack: slots [0, nb) resident, the rest empty
+    for (int s = 0; s < c->maxResidentBatches; s++) {
+        scr_raster_batch& b = c->mirror[(size_t)s];
+        std::memset(&b, 0, sizeof b);
+        if ((size_t)s < nb) {
+            b.state = 1;
+            b.firstPoint = (uint32_t)((size_t)s * ppb);
+            b.numPoints = (uint32_t)std::min(ppb, count - (size_t)s * ppb);
+        }
+    }
+    c->freeSlots.clear();
+    for (int s = c->maxResidentBatches - 1; s >= (int)nb; s--) c->freeSlots.push_back(s);
+    c->residentBatchCount = (int)nb;
+    c->pointCount = (int64_t)count;
+    c->dirtyLo = 0;
+    c->dirtyHi = c->maxResidentBatches - 1;
+    rc = flush_mirror(c);
+    if (rc != SCR_OK) return rc;
+    PackInputs in;
+    rc = stage_inputs(ctx, positions4, colors4, count, opt->inputOnDevice != 0, in);
+    if (rc != SCR_OK) return rc;
+    rc = scr::pack_points(ctx, in.pos, in.col, count, *opt, dest_at(c, 0), sourceIndicesOut);
+    if (rc != SCR_OK) return rc;
+    // adoptGPUBatchBounds (ComputeRasteriserPointCloud.swift:313-320)
+    SCR_HIP(ctx, hipMemcpy(c->mirror.data(), c->dBatches, nb * sizeof(scr_raster_batch), hipMemcpyDeviceToHost));
+    return SCR_OK;
+}
+
+int scr_pack_chunk(scr_cloud* c, const float* positions4, const float* colors4, size_t count,
+                   const scr_pack_options* opt, int32_t* outFirstSlot) {
+    int rc = check_pack(c, positions4, colors4, count, opt);
+    if (rc != SCR_OK) return rc;
+    scr_ctx* ctx = c->ctx;
+    if (outFirstSlot) *outFirstSlot = -1;
+    if (count == 0) return SCR_OK;
+    const size_t ppb = (size_t)c->pointsPerBatch;
+    const int nb = (int)((count + ppb - 1) / ppb);
+    // reserveContiguousSlots — ComputeRasteriserPointCloud.swift:330-347 (first fit)
+    int first = -1;
+    for (int s = 0; s + nb <= c->maxResidentBatches;) {
+        int k = 0;
+        while (k < nb && c->mirror[(size_t)(s + k)].state == 0) k++;
+        if (k == nb) {
+            first = s;
+            break;
+        }
+        s += k + 1;
+    }
+    if (first < 0) return set_error(ctx, SCR_ERR_NO_FREE_SLOTS, "no run of free contiguous slots for the chunk");
+    c->freeSlots.erase(std::remove_if(c->freeSlots.begin(), c->freeSlots.end(),
+                                      [&](int s) { return s >= first && s < first + nb; }),
+                       c->freeSlots.end());
+    PackInputs in;
+    rc = stage_inputs(ctx, positions4, colors4, count, opt->inputOnDevice != 0, in);
+    if (rc == SCR_OK) rc = scr::pack_points(ctx, in.pos, in.col, count, *opt, dest_at(c, first), nullptr);
+    if (rc != SCR_OK) {
+        for (int s = first + nb - 1; s >= first; s--) c->freeSlots.push_back(s);
+        return rc;
+    }
+    // finalizeGPUChunk — ComputeRasteriserPointCloud.swift:351-364
+    SCR_HIP(ctx, hipMemcpy(c->mirror.data() + first, c->dBatches + first, (size_t)nb * sizeof(scr_raster_batch),
+                           hipMemcpyDeviceToHost));
+    c->residentBatchCount += nb;
+    c->pointCount += (int64_t)count;
+    if (outFirstSlot) *outFirstSlot = first;
     return SCR_OK;
 }
 

@@ -114,4 +114,20 @@ hipError_t launch_nearest(hipStream_t s, int numCUs, const CloudDev& c, const Pa
 hipError_t launch_nearest_resolve(hipStream_t s, const PassParams& p, const unsigned long long* packed,
                                   const uint32_t* colors, uint32_t* rgba, float* depthf);
 
+// Pool destination of one pack (device pointers already offset to the slot run).
+struct PackDest {
+    uint32_t* low;
+    uint32_t* med;
+    uint32_t* high;
+    uint32_t* colors;
+    uint8_t* levels;
+    scr_raster_batch* batches;
+    uint32_t slotBase;
+    uint32_t ppb;
+};
+
+// GPU pack of n points (device float4 positions / colours) into dst (scr_pack.hip).
+int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, const scr_pack_options& opt,
+                const PackDest& dst, uint32_t* hostSourceIndices);
+
 }  // namespace scr

@@ -1,0 +1,727 @@
+// scr_pack.hip — GPU packer for gfx950: the GPU counterpart of
+// PackedPointCloudFixtures.pack (PackedPointCloudFixtures.swift:30-167) and
+// GPUPacker.encodePack (GPUPacker.swift:204-392).
+//
+// Stages (all on the context stream):
+//   1 bounds       block min/max → order-preserving u32 atomics   GPUPacker/Shaders.metal:65-105
+//   2 morton       30-bit (10 b/axis, the reference) or 63-bit (21 b/axis) keys, identity payload :109-126
+//   3 radix sort   stable LSD, 8-bit digits: one all-digit histogram pass decides which
+//                  passes are trivial; per pass tile histograms → device-wide exclusive
+//                  scan (digit-major) → stable scatter ranked with wave64 ballots   :134-215
+//   4 gather       Morton-ordered positions                                       :219-235
+//   5 LOD levels   dense voxel grid, atomicMin "first claimer", wave run de-dup   :317-371
+//   6 bucketing    per batch stable 8-way counting sort by level, cum counts      :383-414
+//   7 gather       final order: positions + truncating RGBA8 colours             :219-235
+//   8 batch AABB   one workgroup per batch, full RasterBatch                      :243-284
+//   9 quantize     (p - min) / size (IEEE division, as the CPU pack), 3 x 10-bit words :288-309
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "scr_internal.hpp"
+
+namespace scr {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kChunks = 16;                  // keys per thread per tile
+constexpr int kTile = kThreads * kChunks;    // 4096 keys per tile
+constexpr int kScanSeg = kThreads * 16;      // elements per scan block
+
+__device__ inline uint32_t float_flip(float f) {
+    const uint32_t x = __float_as_uint(f);
+    const uint32_t mask = (uint32_t)(-(int32_t)(x >> 31)) | 0x80000000u;
+    return x ^ mask;
+}
+
+__host__ __device__ inline float float_unflip(uint32_t u) {
+    const uint32_t mask = ((u >> 31) - 1u) | 0x80000000u;
+    return __builtin_bit_cast(float, u ^ mask);
+}
+
+__device__ inline uint32_t spread10(uint32_t v) {
+    uint32_t x = v & 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__device__ inline uint64_t spread21(uint32_t v) {
+    uint64_t x = v & 0x1fffffu;
+    x = (x | (x << 32)) & 0x1f00000000ffffull;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+    x = (x | (x << 8)) & 0x100f00f00f00f00full;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ inline uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+
+// ---------------------------------------------------------------- 1 bounds
+__global__ __launch_bounds__(kThreads) void k_bounds(const float4* __restrict__ pos, size_t n,
+                                                     uint32_t* __restrict__ bounds) {
+    float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+    float mx[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const float4 p = pos[i];
+        mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x); mx[1] = fmaxf(mx[1], p.y); mx[2] = fmaxf(mx[2], p.z);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
+        }
+    if (__lane_id() == 0)
+        for (int a = 0; a < 3; a++) {
+            atomicMin(bounds + a, float_flip(mn[a]));
+            atomicMax(bounds + 3 + a, float_flip(mx[a]));
+        }
+}
+
+__global__ void k_bounds_init(uint32_t* bounds) {
+    if (threadIdx.x < 3) bounds[threadIdx.x] = 0xffffffffu;
+    else if (threadIdx.x < 6) bounds[threadIdx.x] = 0u;
+}
+
+// ---------------------------------------------------------------- 2 morton
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_morton(const float4* __restrict__ pos, size_t n,
+                                                     const uint32_t* __restrict__ bounds, K* __restrict__ keys,
+                                                     uint32_t* __restrict__ idx) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    constexpr bool wide = sizeof(K) == 8;
+    const float qmax = wide ? 2097151.0f : 1023.0f;
+    float mn[3], sc[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = float_unflip(bounds[a]);
+        const float mx = float_unflip(bounds[3 + a]);
+        sc[a] = qmax / fmaxf(mx - mn[a], 0.000001f);
+    }
+    const float4 p = pos[i];
+    const float v[3] = {p.x, p.y, p.z};
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) q[a] = f2u(fminf(fmaxf((v[a] - mn[a]) * sc[a], 0.0f), qmax));
+    if constexpr (wide)
+        keys[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+    else
+        keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    idx[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------- 3 radix sort
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_all_digit_hist(const K* __restrict__ keys, size_t n,
+                                                             uint32_t* __restrict__ ghist) {
+    constexpr int P = sizeof(K);
+    __shared__ uint32_t h[P * 256];
+    for (int i = threadIdx.x; i < P * 256; i += kThreads) h[i] = 0;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kThreads) {
+        const K k = keys[i];
+#pragma unroll
+        for (int p = 0; p < P; p++) atomicAdd(&h[p * 256 + (uint32_t)((k >> (8 * p)) & 0xff)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < P * 256; i += kThreads)
+        if (h[i]) atomicAdd(ghist + i, h[i]);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_tile_hist(const K* __restrict__ keys, size_t n, int shift,
+                                                        uint32_t numTiles, uint32_t* __restrict__ thist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+        const size_t i = base + (size_t)c * kThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(uint32_t)((keys[i] >> shift) & 0xff)], 1u);
+    }
+    __syncthreads();
+    thist[(size_t)threadIdx.x * numTiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl, uint32_t* lds /*>= 4*/) {
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if ((int)__lane_id() >= off) incl += t;
+    }
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 63) lds[w] = incl;
+    __syncthreads();
+    uint32_t waveBase = 0, total = 0;
+    for (int k = 0; k < kThreads / 64; k++) {
+        const uint32_t s = lds[k];
+        if (k < w) waveBase += s;
+        total += s;
+    }
+    __syncthreads();
+    excl = waveBase + incl - v;
+    return total;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __restrict__ in, size_t n,
+                                                          uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lds[4];
+    const size_t base = (size_t)blockIdx.x * kScanSeg + (size_t)threadIdx.x * 16;
+    uint32_t s = 0;
+    for (int j = 0; j < 16; j++)
+        if (base + j < n) s += in[base + j];
+    uint32_t ex;
+    const uint32_t tot = block_exclusive_scan(s, ex, lds);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_sums(uint32_t* __restrict__ sums, uint32_t nseg) {
+    __shared__ uint32_t lds[4];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t c = 0; c < nseg; c += kThreads) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t v = i < nseg ? sums[i] : 0u;
+        uint32_t ex;
+        const uint32_t tot = block_exclusive_scan(v, ex, lds);
+        const uint32_t base = carry;
+        if (i < nseg) sums[i] = base + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = base + tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t* __restrict__ in, size_t n,
+                                                         const uint32_t* __restrict__ sums,
+                                                         uint32_t* __restrict__ out) {
+    __shared__ uint32_t lds[4];
+    const size_t base = (size_t)blockIdx.x * kScanSeg + (size_t)threadIdx.x * 16;
+    uint32_t v[16];
+    uint32_t s = 0;
+    for (int j = 0; j < 16; j++) {
+        v[j] = base + j < n ? in[base + j] : 0u;
+        s += v[j];
+    }
+    uint32_t ex;
+    (void)block_exclusive_scan(s, ex, lds);
+    uint32_t run = sums[blockIdx.x] + ex;
+    for (int j = 0; j < 16; j++)
+        if (base + j < n) {
+            out[base + j] = run;
+            run += v[j];
+        }
+}
+
+// Stable scatter of one tile: chunks of 256 keys in index order; inside a
+// chunk each wave ranks equal digits with 8 ballots; per-wave digit counts in
+// LDS give the cross-wave offsets; a running per-digit cursor carries across
+// chunks.
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                      K* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
+                                                      int shift, uint32_t numTiles,
+                                                      const uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t wcount[kThreads / 64][256];
+    const int tid = threadIdx.x;
+    const int w = tid >> 6;
+    running[tid] = offsets[(size_t)tid * numTiles + blockIdx.x];
+    const size_t base = (size_t)blockIdx.x * kTile;
+    K kr[kChunks];
+    uint32_t vr[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+        const size_t i = base + (size_t)c * kThreads + tid;
+        if (i < n) {
+            kr[c] = kin[i];
+            vr[c] = vin[i];
+        } else {
+            kr[c] = 0;
+            vr[c] = 0;
+        }
+    }
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+        const size_t i = base + (size_t)c * kThreads + tid;
+        const bool valid = i < n;
+        const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
+        for (int j = __lane_id(); j < 256; j += 64) wcount[w][j] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcount[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = running[d] + rank;
+            for (int k = 0; k < w; k++) pos += wcount[k][d];
+            kout[pos] = kr[c];
+            vout[pos] = vr[c];
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int k = 0; k < kThreads / 64; k++) add += wcount[k][tid];
+        running[tid] += add;
+        __syncthreads();  // wcount is cleared again at the top of the next chunk
+    }
+}
+
+// ---------------------------------------------------------------- 4/7 gathers
+__global__ __launch_bounds__(kThreads) void k_gather_pos(const float4* __restrict__ pos,
+                                                         const uint32_t* __restrict__ idx, size_t n,
+                                                         float4* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) out[i] = pos[idx[i]];
+}
+
+// packGather colour conversion: uint(clamp(c, 0, 1) * 255) per channel (truncating)
+__global__ __launch_bounds__(kThreads) void k_gather_final(const float4* __restrict__ pos,
+                                                           const float4* __restrict__ col,
+                                                           const uint32_t* __restrict__ idx, size_t n,
+                                                           float4* __restrict__ outPos,
+                                                           uint32_t* __restrict__ outCol) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = idx[i];
+    outPos[i] = pos[s];
+    const float4 c = col[s];
+    const uint32_t r = f2u(fminf(fmaxf(c.x, 0.0f), 1.0f) * 255.0f);
+    const uint32_t g = f2u(fminf(fmaxf(c.y, 0.0f), 1.0f) * 255.0f);
+    const uint32_t b = f2u(fminf(fmaxf(c.z, 0.0f), 1.0f) * 255.0f);
+    const uint32_t a = f2u(fminf(fmaxf(c.w, 0.0f), 1.0f) * 255.0f);
+    outCol[i] = r | (g << 8) | (b << 16) | (a << 24);
+}
+
+// ---------------------------------------------------------------- 5 LOD
+struct LodGrid {
+    float voxelScale;   // 0.5^level
+    float coarse;
+    uint32_t dx, dy;
+};
+
+__device__ inline void lod_cell(const uint32_t* bounds, const LodGrid& g, float4 p, uint32_t& cx, uint32_t& cy,
+                                uint32_t& cz) {
+    float mn[3], ext[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = float_unflip(bounds[a]);
+        ext[a] = fmaxf(float_unflip(bounds[3 + a]) - mn[a], 0.000001f);
+    }
+    const float longest = fmaxf(ext[0], fmaxf(ext[1], ext[2]));
+    const float voxel = (longest / g.coarse) * g.voxelScale;
+    const float inv = 1.0f / fmaxf(voxel, 0.000001f);
+    int32_t x = f2i(floorf((p.x - mn[0]) * inv));
+    int32_t y = f2i(floorf((p.y - mn[1]) * inv));
+    int32_t z = f2i(floorf((p.z - mn[2]) * inv));
+    x = x < 0 ? 0 : x;
+    y = y < 0 ? 0 : y;
+    z = z < 0 ? 0 : z;
+    cx = min((uint32_t)x, g.dx - 1u);
+    cy = min((uint32_t)y, g.dy - 1u);
+    cz = (uint32_t)z;
+}
+
+__global__ __launch_bounds__(kThreads) void k_lod_init(uint8_t* __restrict__ lv, size_t n, uint8_t maxLevel) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) lv[i] = maxLevel;
+}
+
+__global__ __launch_bounds__(kThreads) void k_lod_claim(const float4* __restrict__ sp, const uint8_t* __restrict__ lv,
+                                                        size_t n, uint8_t maxLevel, const uint32_t* __restrict__ bounds,
+                                                        LodGrid g, uint32_t* __restrict__ grid) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    const bool part = i < n && lv[i] == maxLevel;
+    uint64_t lin = ~0ull;
+    if (part) {
+        uint32_t cx, cy, cz;
+        lod_cell(bounds, g, sp[i], cx, cy, cz);
+        lin = (uint64_t)cx + (uint64_t)g.dx * ((uint64_t)cy + (uint64_t)g.dy * cz);
+    }
+    // the previous point (i - 1, same wave) in the same cell holds a smaller index
+    const uint64_t prev = __shfl_up(lin, 1);
+    const bool dup = __lane_id() > 0 && prev == lin;
+    if (part && !dup) atomicMin(grid + lin, (uint32_t)i);
+}
+
+__global__ __launch_bounds__(kThreads) void k_lod_assign(const float4* __restrict__ sp, uint8_t* __restrict__ lv,
+                                                         size_t n, uint8_t maxLevel, uint8_t level,
+                                                         const uint32_t* __restrict__ bounds, LodGrid g,
+                                                         const uint32_t* __restrict__ grid) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n || lv[i] != maxLevel) return;
+    uint32_t cx, cy, cz;
+    lod_cell(bounds, g, sp[i], cx, cy, cz);
+    const uint64_t lin = (uint64_t)cx + (uint64_t)g.dx * ((uint64_t)cy + (uint64_t)g.dy * cz);
+    if (grid[lin] == (uint32_t)i) lv[i] = level;
+}
+
+// ---------------------------------------------------------------- 6 bucketing
+// One workgroup per batch; stable counting scatter by level (3-bit digit) in
+// index order, cum counts out.
+__global__ __launch_bounds__(kThreads) void k_bucket(const uint8_t* __restrict__ lvIn, const uint32_t* __restrict__ idxIn,
+                                                     size_t n, uint32_t ppb, uint32_t* __restrict__ idxOut,
+                                                     uint8_t* __restrict__ lvOut, uint32_t* __restrict__ cumOut) {
+    __shared__ uint32_t cnt[8];
+    __shared__ uint32_t running[8];
+    __shared__ uint32_t wcount[kThreads / 64][8];
+    const int tid = threadIdx.x;
+    const int w = tid >> 6;
+    const size_t start = (size_t)blockIdx.x * ppb;
+    const size_t end = min(start + (size_t)ppb, n);
+    if (tid < 8) cnt[tid] = 0;
+    __syncthreads();
+    for (size_t i = start + tid; i < end; i += kThreads) atomicAdd(&cnt[lvIn[i] & 7u], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int l = 0; l < 8; l++) {
+            running[l] = acc;
+            acc += cnt[l];
+        }
+        const uint32_t c0 = cnt[0], c1 = c0 + cnt[1], c2 = c1 + cnt[2], c3 = c2 + cnt[3], c4 = c3 + cnt[4],
+                       c5 = c4 + cnt[5], c6 = c5 + cnt[6], c7 = c6 + cnt[7];
+        cumOut[blockIdx.x * 4 + 0] = c0 | (c1 << 16);
+        cumOut[blockIdx.x * 4 + 1] = c2 | (c3 << 16);
+        cumOut[blockIdx.x * 4 + 2] = c4 | (c5 << 16);
+        cumOut[blockIdx.x * 4 + 3] = c6 | (c7 << 16);
+    }
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    for (size_t c = start; c < end; c += kThreads) {
+        const size_t i = c + tid;
+        const bool valid = i < end;
+        const uint8_t lvl = valid ? lvIn[i] : 0;
+        const uint32_t d = lvl & 7u;
+        if (__lane_id() < 8) wcount[w][__lane_id()] = 0;
+        __syncthreads();
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcount[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = running[d] + rank;
+            for (int k = 0; k < w; k++) pos += wcount[k][d];
+            idxOut[start + pos] = idxIn[i];
+            lvOut[start + pos] = lvl;
+        }
+        __syncthreads();
+        if (tid < 8) {
+            uint32_t add = 0;
+            for (int k = 0; k < kThreads / 64; k++) add += wcount[k][tid];
+            running[tid] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- 8 batch AABB
+__global__ __launch_bounds__(kThreads) void k_batch_aabb(const float4* __restrict__ sp, size_t n, uint32_t ppb,
+                                                         uint32_t slotBase, const uint32_t* __restrict__ cum,
+                                                         scr_raster_batch* __restrict__ batches) {
+    __shared__ float smn[4][3], smx[4][3];
+    const size_t start = (size_t)blockIdx.x * ppb;
+    const size_t end = min(start + (size_t)ppb, n);
+    float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+    float mx[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    for (size_t i = start + threadIdx.x; i < end; i += kThreads) {
+        const float4 p = sp[i];
+        mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x); mx[1] = fmaxf(mx[1], p.y); mx[2] = fmaxf(mx[2], p.z);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
+        }
+    const int w = threadIdx.x >> 6;
+    if (__lane_id() == 0)
+        for (int a = 0; a < 3; a++) {
+            smn[w][a] = mn[a];
+            smx[w][a] = mx[a];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kThreads / 64; k++)
+            for (int a = 0; a < 3; a++) {
+                mn[a] = fminf(smn[0][a], smn[k][a]);
+                smn[0][a] = mn[a];
+                mx[a] = fmaxf(smx[0][a], smx[k][a]);
+                smx[0][a] = mx[a];
+            }
+        scr_raster_batch b;
+        b.state = 1;
+        b.minX = smn[0][0]; b.minY = smn[0][1]; b.minZ = smn[0][2];
+        b.maxX = smx[0][0]; b.maxY = smx[0][1]; b.maxZ = smx[0][2];
+        b.numPoints = (uint32_t)(end - start);
+        b.firstPoint = (slotBase + blockIdx.x) * ppb;
+        b.fileIndex = 0;
+        b.lodCum01 = cum[blockIdx.x * 4 + 0];
+        b.lodCum23 = cum[blockIdx.x * 4 + 1];
+        b.lodCum45 = cum[blockIdx.x * 4 + 2];
+        b.lodCum67 = cum[blockIdx.x * 4 + 3];
+        b.padding7 = 0;
+        b.padding8 = 0;
+        batches[blockIdx.x] = b;
+    }
+}
+
+// ---------------------------------------------------------------- 9 quantize
+__global__ __launch_bounds__(kThreads) void k_quantize(const float4* __restrict__ sp, size_t n, uint32_t ppb,
+                                                       const scr_raster_batch* __restrict__ batches,
+                                                       uint32_t* __restrict__ low, uint32_t* __restrict__ med,
+                                                       uint32_t* __restrict__ high) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const scr_raster_batch& b = batches[i / ppb];
+    const float4 p = sp[i];
+    const float mn[3] = {b.minX, b.minY, b.minZ};
+    const float mx[3] = {b.maxX, b.maxY, b.maxZ};
+    const float v[3] = {p.x, p.y, p.z};
+    uint32_t q[3];
+    for (int a = 0; a < 3; a++) {
+        const float size = fmaxf(mx[a] - mn[a], 0.000001f);
+        const float nrm = fminf(fmaxf((v[a] - mn[a]) / size, 0.0f), 0.99999994f);
+        q[a] = f2u(nrm * 1073741824.0f);  // Float(2^30 - 1) rounds to 2^30
+    }
+    low[i] = ((q[0] >> 20) & 1023u) | (((q[1] >> 20) & 1023u) << 10) | (((q[2] >> 20) & 1023u) << 20);
+    med[i] = ((q[0] >> 10) & 1023u) | (((q[1] >> 10) & 1023u) << 10) | (((q[2] >> 10) & 1023u) << 20);
+    high[i] = (q[0] & 1023u) | ((q[1] & 1023u) << 10) | ((q[2] & 1023u) << 20);
+}
+
+unsigned blocks(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
+
+// Bump allocator over the context scratch block.
+struct Arena {
+    char* base;
+    size_t off = 0;
+    template <typename T>
+    T* take(size_t count) {
+        off = (off + 255) & ~(size_t)255;
+        T* p = (T*)(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+}  // namespace
+
+
+static size_t pack_scratch_bytes(size_t n, bool wide, size_t nb, size_t gridCells) {
+    const size_t numTiles = (n + kTile - 1) / kTile;
+    const size_t scanN = 256 * numTiles;
+    const size_t k = wide ? 8 : 4;
+    size_t b = 0;
+    b += 2 * n * k + 2 * n * 4;                 // keys, payload ping-pong
+    b += n * 16;                                // sorted positions
+    b += n;                                     // Morton-order levels
+    b += 2 * scanN * 4 + (scanN / kScanSeg + 2) * 4;  // tile hist, offsets, scan sums
+    b += 8 * 256 * 4 + 64;                      // digit histograms, bounds
+    b += nb * 16;                               // cum counts
+    b += gridCells * 4;
+    b += 32 * 256;                              // alignment slack
+    return b;
+}
+
+template <typename K>
+static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*& vout, size_t n, uint32_t* ghist,
+                      uint32_t* thist, uint32_t* offs, uint32_t* sums, int* passesRun) {
+    hipStream_t s = ctx->stream;
+    constexpr int P = sizeof(K);
+    const uint32_t numTiles = (uint32_t)((n + kTile - 1) / kTile);
+    const size_t scanN = (size_t)256 * numTiles;
+    const uint32_t nseg = (uint32_t)((scanN + kScanSeg - 1) / kScanSeg);
+    hipError_t e = launch_fill_u32(s, ghist, P * 256, 0u);
+    if (e != hipSuccess) return hip_error(ctx, e, "radix hist clear");
+    hipLaunchKernelGGL(k_all_digit_hist<K>, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
+                       dim3(kThreads), 0, s, kin, n, ghist);
+    std::vector<uint32_t> h((size_t)P * 256);
+    e = hipMemcpyAsync(h.data(), ghist, h.size() * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_error(ctx, e, "radix hist");
+    int run = 0;
+    for (int p = 0; p < P; p++) {
+        bool trivial = false;
+        for (int d = 0; d < 256; d++)
+            if (h[(size_t)p * 256 + d] == n) trivial = true;
+        if (trivial) continue;  // every key has the same digit: the pass is the identity
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_tile_hist<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, n, shift, numTiles, thist);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(kThreads), 0, s, thist, scanN, sums);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, s, sums, nseg);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(kThreads), 0, s, thist, scanN, sums, offs);
+        hipLaunchKernelGGL(k_scatter<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift,
+                           numTiles, offs);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        run++;
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_error(ctx, e, "radix sort");
+    if (passesRun) *passesRun = run;
+    return SCR_OK;
+}
+
+// Run the nine stages for n points into `dst`.  pos/col are device float4 arrays.
+int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, const scr_pack_options& opt,
+                const PackDest& dst, uint32_t* hostSourceIndices) {
+    hipStream_t s = ctx->stream;
+    const bool wide = opt.keyBits == 63;
+    const size_t ppb = dst.ppb;
+    const size_t nb = (n + ppb - 1) / ppb;
+    const int lodLevels = std::min(std::max(opt.lodLevels, 1), 8);
+    const int coarse = std::max(opt.coarseVoxelDivisions, 1);
+    const uint8_t maxLevel = (uint8_t)(lodLevels - 1);
+
+    // bounds first (the LOD grid size depends on them)
+    uint32_t* dBounds = nullptr;
+    hipError_t e = hipMalloc((void**)&dBounds, 6 * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_error(ctx, e, "pack bounds alloc");
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, dBounds);
+    hipLaunchKernelGGL(k_bounds, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
+                       dim3(kThreads), 0, s, pos, n, dBounds);
+    uint32_t hb[6];
+    e = hipMemcpyAsync(hb, dBounds, sizeof hb, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipFree(dBounds);
+        return hip_error(ctx, e, "pack bounds");
+    }
+    // dense LOD grid dims at the finest claimed level (cells per axis grow by 2 per level)
+    uint64_t gridCells = 0;
+    LodGrid lg{};
+    uint32_t dims[3] = {1, 1, 1};
+    if (lodLevels > 1) {
+        float mn[3], ext[3];
+        for (int a = 0; a < 3; a++) {
+            mn[a] = float_unflip(hb[a]);
+            ext[a] = std::fmax(float_unflip(hb[3 + a]) - mn[a], 0.000001f);
+        }
+        const float longest = std::fmax(ext[0], std::fmax(ext[1], ext[2]));
+        const float voxel = (longest / (float)coarse) * std::pow(0.5f, (float)(lodLevels - 2));
+        const float inv = 1.0f / std::fmax(voxel, 0.000001f);
+        for (int a = 0; a < 3; a++) dims[a] = (uint32_t)f2u(std::floor(ext[a] * inv)) + 1u;
+        gridCells = (uint64_t)dims[0] * dims[1] * dims[2];
+        if (gridCells > (1ull << 30)) {
+            (void)hipFree(dBounds);
+            return set_error(ctx, SCR_ERR_UNSUPPORTED, "LOD grid exceeds 2^30 cells; lower lodLevels or coarseVoxelDivisions");
+        }
+    }
+    const size_t need = pack_scratch_bytes(n, wide, nb, (size_t)gridCells);
+    if (ctx->scratchBytes < need) {
+        if (ctx->scratch) (void)hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratchBytes = 0;
+        e = hipMalloc(&ctx->scratch, need);
+        if (e != hipSuccess) {
+            (void)hipFree(dBounds);
+            return hip_error(ctx, e, "pack scratch alloc");
+        }
+        ctx->scratchBytes = need;
+    }
+    Arena ar{(char*)ctx->scratch};
+    const size_t numTiles = (n + kTile - 1) / kTile;
+    const size_t scanN = 256 * numTiles;
+    void* keysA = ar.take<char>(n * (wide ? 8 : 4));
+    void* keysB = ar.take<char>(n * (wide ? 8 : 4));
+    uint32_t* idxA = ar.take<uint32_t>(n);
+    uint32_t* idxB = ar.take<uint32_t>(n);
+    float4* sp = ar.take<float4>(n);
+    uint8_t* lvM = ar.take<uint8_t>(n);
+    uint32_t* thist = ar.take<uint32_t>(scanN);
+    uint32_t* offs = ar.take<uint32_t>(scanN);
+    uint32_t* sums = ar.take<uint32_t>(scanN / kScanSeg + 2);
+    uint32_t* ghist = ar.take<uint32_t>(8 * 256);
+    uint32_t* cum = ar.take<uint32_t>(nb * 4);
+    uint32_t* grid = ar.take<uint32_t>((size_t)gridCells);
+
+    // 2 Morton keys
+    uint32_t* sortedIdx = nullptr;
+    int rc = SCR_OK;
+    if (wide) {
+        hipLaunchKernelGGL(k_morton<uint64_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
+                           (uint64_t*)keysA, idxA);
+        uint64_t *kin = (uint64_t*)keysA, *kout = (uint64_t*)keysB;
+        uint32_t *vin = idxA, *vout = idxB;
+        rc = radix_sort<uint64_t>(ctx, kin, vin, kout, vout, n, ghist, thist, offs, sums, nullptr);
+        sortedIdx = vin;
+    } else {
+        hipLaunchKernelGGL(k_morton<uint32_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
+                           (uint32_t*)keysA, idxA);
+        uint32_t *kin = (uint32_t*)keysA, *kout = (uint32_t*)keysB;
+        uint32_t *vin = idxA, *vout = idxB;
+        rc = radix_sort<uint32_t>(ctx, kin, vin, kout, vout, n, ghist, thist, offs, sums, nullptr);
+        sortedIdx = vin;
+    }
+    if (rc != SCR_OK) {
+        (void)hipFree(dBounds);
+        return rc;
+    }
+    uint32_t* permOut = sortedIdx == idxA ? idxB : idxA;
+
+    // 4 gather Morton-order positions
+    hipLaunchKernelGGL(k_gather_pos, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, sortedIdx, n, sp);
+    // 5 LOD levels
+    hipLaunchKernelGGL(k_lod_init, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, lvM, n, maxLevel);
+    for (int level = 0; level < lodLevels - 1; level++) {
+        LodGrid g;
+        g.voxelScale = std::pow(0.5f, (float)level);
+        g.coarse = (float)coarse;
+        // dims of this level's grid (same arithmetic as the device cell index)
+        float mn[3], ext[3];
+        for (int a = 0; a < 3; a++) {
+            mn[a] = float_unflip(hb[a]);
+            ext[a] = std::fmax(float_unflip(hb[3 + a]) - mn[a], 0.000001f);
+        }
+        const float longest = std::fmax(ext[0], std::fmax(ext[1], ext[2]));
+        const float voxel = (longest / g.coarse) * g.voxelScale;
+        const float inv = 1.0f / std::fmax(voxel, 0.000001f);
+        g.dx = (uint32_t)f2u(std::floor(ext[0] * inv)) + 1u;
+        g.dy = (uint32_t)f2u(std::floor(ext[1] * inv)) + 1u;
+        const uint32_t dz = (uint32_t)f2u(std::floor(ext[2] * inv)) + 1u;
+        const size_t cells = (size_t)g.dx * g.dy * dz;
+        e = launch_fill_u32(s, grid, cells, 0xffffffffu);
+        if (e != hipSuccess) break;
+        hipLaunchKernelGGL(k_lod_claim, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel, dBounds,
+                           g, grid);
+        hipLaunchKernelGGL(k_lod_assign, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel,
+                           (uint8_t)level, dBounds, g, grid);
+    }
+    // 6 bucketing → permOut (source indices in final order), final levels
+    hipLaunchKernelGGL(k_bucket, dim3((unsigned)nb), dim3(kThreads), 0, s, lvM, sortedIdx, n, (uint32_t)ppb, permOut,
+                       dst.levels, cum);
+    // 7 final gather (positions into sp, colours into the pool)
+    hipLaunchKernelGGL(k_gather_final, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, col, permOut, n, sp,
+                       dst.colors);
+    // 8 batch AABB + metadata, 9 quantize
+    hipLaunchKernelGGL(k_batch_aabb, dim3((unsigned)nb), dim3(kThreads), 0, s, sp, n, (uint32_t)ppb, dst.slotBase,
+                       cum, dst.batches);
+    hipLaunchKernelGGL(k_quantize, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, n, (uint32_t)ppb, dst.batches,
+                       dst.low, dst.med, dst.high);
+    e = hipGetLastError();
+    if (e == hipSuccess && hostSourceIndices)
+        e = hipMemcpyAsync(hostSourceIndices, permOut, n * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(dBounds);
+    if (e != hipSuccess) return hip_error(ctx, e, "pack");
+    return SCR_OK;
+}
+
+}  // namespace scr

@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""Headline benchmark: HQS frames of a 100M-point cloud at 1920x1080 on one MI355X.
+
+Workload (one "step" = one full HQS frame: clear, cull, depth, colour,
+resolve): the 100M-point LiDAR-like scene (8 km x 8 km value-noise terrain +
+32,000 boxes, seed 0x5C0003), generated on the GPU and packed on the GPU with
+63-bit Morton keys into 10,240-point batches, rendered at 1920x1080 with CLOD
+off (every point drawn), 1-pixel points, from an overview pose that keeps the
+whole scene in view.  Inputs stay resident in HBM; the timed region is K
+frames bracketed by device synchronisation.
+
+value = processed points (all points of all visible batches) per second.
+roofline: the depth pass (algorithmic bytes = per processed point 1 B level
++ 4/8/12 B position by batch precision, + 80 B per visible batch) against the
+MI355X HBM peak; traffic comes from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+summarised in profiles/ when present.
+cpu_baseline: the CPU oracle (scalar C transcription of the same passes,
+multithreaded over batches) on a bounded sample of the same frame's batches.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+PKG_DIR = REPO / "satin-computerasteriser_amd"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    name = "satin_computerasteriser_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, PKG_DIR / "__init__.py",
+                                                  submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def overview_camera(scr, width, height):
+    return scr.Camera(scr.look_at((0.0, -7200.0, 5200.0), (0.0, 300.0, 0.0), (0.0, 0.0, 1.0)),
+                      scr.perspective_reverse_z(55.0, width / height, 10.0, 30000.0))
+
+
+def spline_frame0_camera(scr, width, height):
+    """Frame 0 of the fly-through: 150 m above the terrain, looking ahead and down."""
+    return scr.Camera(scr.look_at((-3000.0, -3000.0, 350.0), (-1500.0, -1200.0, 0.0), (0.0, 0.0, 1.0)),
+                      scr.perspective_reverse_z(60.0, width / height, 1.0, 20000.0))
+
+
+def algorithmic_bytes(vis, per_point_extra=0):
+    """Depth-pass bytes: per processed point 1 B level + 4 B per position word
+    (3/2/1 words for precision level 0/1/>=2), + 80 B per visible batch."""
+    lv = vis["level"].astype(np.int64)
+    words = np.where(lv == 0, 3, np.where(lv == 1, 2, 1))
+    pts = vis["activePoints"].astype(np.int64)
+    return int((pts * (1 + 4 * words + per_point_extra)).sum() + 80 * len(vis)), int(pts.sum())
+
+
+def timed_frames(ctx, ras, cam, steps, warmup):
+    for _ in range(warmup):
+        ras.render(cam)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ras.render(cam)
+    ctx.sync()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def pass_profile(ctx, ras, cam, steps):
+    ctx.profile(True)
+    for _ in range(steps):
+        ras.render(cam)
+    prof = ctx.collect_profile()
+    ctx.profile(False)
+    return {k: v[0] / v[1] for k, v in prof.items()}  # average ms per launch
+
+
+def read_traffic(kernel_tag: str):
+    """HBM bytes per launch of the named kernel from the committed rocprof summary."""
+    f = REPO / "profiles" / "traffic_summary.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        m = re.match(rf"\|\s*{re.escape(kernel_tag)}\s*\|\s*([0-9.eE+]+)\s*\|", line)
+        if m:
+            return float(m.group(1))
+    return None
+
+
+def cpu_baseline(scr, cloud, cfg, cam, width, height, vis, budget_s=12.0):
+    """Oracle on a bounded, evenly spaced sample of the visible batches."""
+    sys.path.insert(0, str(REPO))
+    from oracle import oracle as orc
+
+    threads = min(16, os.cpu_count() or 1)
+    low = cloud.readPool(scr.POOL_XYZ_LOW)
+    med = cloud.readPool(scr.POOL_XYZ_MED)
+    high = cloud.readPool(scr.POOL_XYZ_HIGH)
+    colors = cloud.readPool(scr.POOL_COLORS)
+    levels = cloud.readPool(scr.POOL_LEVELS)
+    batches = cloud.readPool(scr.POOL_BATCHES)
+    vis_idx = np.sort(vis["batchIndex"].astype(np.int64))
+
+    def run(frac):
+        step = max(1, int(round(1.0 / frac)))
+        sel = vis_idx[::step]
+        c = orc.Cloud(batches[sel], scr.identity_files(1), low, med, high, colors, levels)
+        npts = int(batches["numPoints"][sel].astype(np.int64).sum())
+        t0 = time.perf_counter()
+        orc.render_hqs([c], cfg, cam.viewMatrix, cam.projectionMatrix, width, height, threads=threads)
+        return npts, time.perf_counter() - t0, len(sel)
+
+    n0, t0, _ = run(1.0 / 64)
+    frac = min(1.0, max(1.0 / 64, (budget_s / max(t0, 1e-3)) / 64))
+    npts, dt, nb = run(frac) if frac > 1.0 / 64 else (n0, t0, None)
+    return {"value": npts / dt / 1e9, "unit": "Gpts/s", "cores": threads, "kind": "port",
+            "sample": f"{npts} points ({nb or 'every 64th'} of {len(vis_idx)} visible batches, evenly spaced) "
+                      f"of the same frame, HQS {width}x{height}, {threads} threads, {dt:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=100_000_000)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): this build has no multi-GPU "
+              "path; see the sharding notes in DESIGN.md", file=sys.stderr)
+        return 2
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs as a single process on one GPU", file=sys.stderr)
+        return 2
+
+    scr = load_pkg()
+    ctx = scr.Context(0)
+    W, H, N = args.width, args.height, args.points
+
+    log(f"generating {N} points on the GPU")
+    pos, col = scr.gen_scene(ctx, N, on_device=True, seed=0x5C0003, halfExtent=4000.0, numBoxes=32000)
+    cloud = scr.ComputeRasteriserPointCloud(ctx, 1, scr.DEFAULT_POINTS_PER_BATCH)
+    packer = scr.GPUPacker(ctx, keyBits=63)
+    ctx.sync()
+    t0 = time.perf_counter()
+    packer.packDevice(cloud, pos, col, N)
+    ctx.sync()
+    pack_ms = (time.perf_counter() - t0) * 1e3
+    log(f"packed in {pack_ms:.1f} ms ({cloud.residentBatchCount} batches)")
+    pos.free()
+    col.free()
+
+    cfg = scr.ComputeRasteriserConfiguration(enableCLOD=False)
+    cam = overview_camera(scr, W, H)
+    ras = scr.ComputeRasteriser(ctx, cfg)
+    ras.addPointCloud(cloud)
+    ras.resize(W, H)
+
+    log("timing HQS frames")
+    ms = timed_frames(ctx, ras, cam, args.steps, args.warmup)
+    vis = cloud.readVisible()
+    depth_bytes, processed = algorithmic_bytes(vis)
+    prof = pass_profile(ctx, ras, cam, max(3, args.steps // 2))
+    depth_ms = prof.get("depth", float("nan"))
+    achieved = depth_bytes / (depth_ms * 1e-3) / 1e9
+    traffic = read_traffic("k_depth")
+    value = processed / (ms * 1e-3) / 1e9
+    log(f"HQS {ms:.3f} ms/frame, {value:.2f} Gpts/s; passes {prof}")
+
+    extra = {"pack_ms_63bit": round(pack_ms, 2), "pass_ms": {k: round(v, 4) for k, v in prof.items()},
+             "processed_points": processed, "visible_batches": int(len(vis)),
+             "depth_algorithmic_bytes": depth_bytes}
+    if not args.no_extra:
+        cb_bytes, _ = algorithmic_bytes(vis, per_point_extra=4)
+        extra["color_roofline_frac"] = round(cb_bytes / (prof.get("color", float("nan")) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        # CLOD on at the same pose
+        ras.configuration = scr.ComputeRasteriserConfiguration(enableCLOD=True)
+        ms_c = timed_frames(ctx, ras, cam, args.steps, 1)
+        vis_c = cloud.readVisible()
+        _, proc_c = algorithmic_bytes(vis_c)
+        extra["hqs_clod_on"] = {"ms_per_frame": round(ms_c, 4), "processed_points": proc_c,
+                                "Gpts_per_s_of_cloud": round(N / (ms_c * 1e-3) / 1e9, 3)}
+        # nearest-point mode, CLOD off
+        ras.configuration = scr.ComputeRasteriserConfiguration(mode=scr.MODE_NEAREST_POINT, enableCLOD=False)
+        ms_n = timed_frames(ctx, ras, cam, args.steps, 1)
+        extra["nearest"] = {"ms_per_frame": round(ms_n, 4), "Gpts_per_s": round(processed / (ms_n * 1e-3) / 1e9, 3)}
+        # low-altitude fly-through frame 0, CLOD on
+        ras.configuration = scr.ComputeRasteriserConfiguration(enableCLOD=True)
+        cam0 = spline_frame0_camera(scr, W, H)
+        ms_s = timed_frames(ctx, ras, cam0, args.steps, 1)
+        _, proc_s = algorithmic_bytes(cloud.readVisible())
+        extra["flythrough_frame0_clod_on"] = {"ms_per_frame": round(ms_s, 4), "processed_points": proc_s}
+        ras.configuration = cfg
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle) on a bounded sample")
+        ras.render(cam)
+        vis = cloud.readVisible()
+        cpu = cpu_baseline(scr, cloud, cfg, cam, W, H, vis)
+
+    line = {
+        "metric": "Gpts/s HQS 1080p 100M pts",
+        "value": round(value, 4),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u32",
+        "data": "synthetic",
+        "config": {"workload": f"HQS frame, {N} pt LiDAR-like cloud (8 km square, 32k boxes, seed 0x5C0003), "
+                               f"GPU-packed 63-bit Morton, {W}x{H}, CLOD off, 1 px points, overview pose",
+                   "points": N, "width": W, "height": H, "points_per_batch": 10240, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_depth", "depth_ms": round(depth_ms, 4)},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -127,6 +127,20 @@ int scr_sync(scr_ctx* ctx);
 /* Version string of the library build (never NULL). */
 const char* scr_version(void);
 
+/* ---- per-pass timing (HIP events on the context stream; the reference's
+ * equivalent is its labelled encoders under Xcode GPU capture,
+ * ComputeRasteriser.swift:389-442).  While enabled, every kernel the render
+ * and pack paths launch is bracketed by a pair of events. */
+typedef struct scr_pass_time {
+    char name[32];
+    double totalMs;
+    int32_t launches;
+} scr_pass_time;
+int scr_profile_enable(scr_ctx* ctx, int enable);
+/* Synchronise, sum the recorded intervals per pass name, clear the record.
+ * Up to `capacity` entries are written; *count receives the number of names. */
+int scr_profile_collect(scr_ctx* ctx, scr_pass_time* out, int capacity, int* count);
+
 /* ---- device memory helpers (for callers that keep inputs resident in HBM) */
 int scr_device_alloc(scr_ctx* ctx, size_t bytes, void** out);
 int scr_device_free(scr_ctx* ctx, void* ptr);

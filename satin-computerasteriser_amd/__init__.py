@@ -118,6 +118,10 @@ class CSceneParams(C.Structure):
     ]
 
 
+class CPassTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("totalMs", C.c_double), ("launches", C.c_int32)]
+
+
 _VP = C.c_void_p
 _I = C.c_int
 _SZ = C.c_size_t
@@ -128,6 +132,8 @@ _SIGNATURES = {
     "scr_ctx_destroy": (_I, [_VP]),
     "scr_last_error": (C.c_char_p, [_VP]),
     "scr_sync": (_I, [_VP]),
+    "scr_profile_enable": (_I, [_VP, _I]),
+    "scr_profile_collect": (_I, [_VP, _VP, _I, C.POINTER(_I)]),
     "scr_device_alloc": (_I, [_VP, _SZ, C.POINTER(_VP)]),
     "scr_device_free": (_I, [_VP, _VP]),
     "scr_memcpy_h2d": (_I, [_VP, _VP, _VP, _SZ]),
@@ -351,6 +357,17 @@ class Context:
 
     def sync(self) -> None:
         self.check(self.lib.scr_sync(self.handle))
+
+    def profile(self, enable: bool = True) -> None:
+        """Bracket every kernel launch with HIP events (per-pass timing)."""
+        self.check(self.lib.scr_profile_enable(self.handle, int(enable)))
+
+    def collect_profile(self) -> dict:
+        """{pass name: (total ms, launches)} since the last collect."""
+        buf = (CPassTime * 64)()
+        n = C.c_int()
+        self.check(self.lib.scr_profile_collect(self.handle, buf, 64, C.byref(n)))
+        return {buf[i].name.decode(): (buf[i].totalMs, buf[i].launches) for i in range(min(n.value, 64))}
 
     def close(self) -> None:
         if getattr(self, "handle", None):

@@ -21,6 +21,30 @@ int hip_error(scr_ctx* ctx, hipError_t e, const char* what) {
     return set_error(ctx, e == hipErrorOutOfMemory ? SCR_ERR_OUT_OF_MEMORY : SCR_ERR_HIP, buf);
 }
 
+hipEvent_t take_event(scr_ctx* ctx) {
+    if (!ctx->eventPool.empty()) {
+        hipEvent_t e = ctx->eventPool.back();
+        ctx->eventPool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+PassScope::PassScope(scr_ctx* ctx, const char* name) : ctx_(ctx) {
+    if (!ctx->profiling) return;
+    scr_ctx::Interval iv{name, take_event(ctx), take_event(ctx)};
+    if (!iv.begin || !iv.end) return;
+    (void)hipEventRecord(iv.begin, ctx->stream);
+    ctx->intervals.push_back(iv);
+    index_ = (int)ctx->intervals.size() - 1;
+}
+
+PassScope::~PassScope() {
+    if (index_ >= 0) (void)hipEventRecord(ctx_->intervals[(size_t)index_].end, ctx_->stream);
+}
+
 }  // namespace scr
 
 using scr::hip_error;
@@ -180,6 +204,7 @@ scr::PassParams make_params(const scr_config* cfg, const scr_camera* cam, int wi
     p.lo = std::fmax(std::fmin(mn, mx), 1.0f);
     p.hi = std::fmax(std::fmax(mn, mx), p.lo);
     p.constRadius = (p.lo == p.hi) ? scr::const_radius_for(p.lo) : -1;
+    p.maxRadius = scr::const_radius_for(p.hi);
     p.ortho = cam->projectionMatrix[15] == 1.0f;
     p.focal = ((float)height * 0.5f) * cam->projectionMatrix[5];
     for (int i = 0; i < 4; i++) p.bg[i] = cfg->backgroundColor[i];
@@ -229,10 +254,47 @@ int scr_ctx_create(int device, scr_ctx** out) {
     return SCR_OK;
 }
 
+int scr_profile_enable(scr_ctx* ctx, int enable) {
+    if (!ctx) return SCR_ERR_INVALID_ARGUMENT;
+    ctx->profiling = enable != 0;
+    return SCR_OK;
+}
+
+int scr_profile_collect(scr_ctx* ctx, scr_pass_time* out, int capacity, int* count) {
+    if (!ctx || !count) return SCR_ERR_INVALID_ARGUMENT;
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<scr_pass_time> acc;
+    for (auto& iv : ctx->intervals) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, iv.begin, iv.end) != hipSuccess) ms = 0.0f;
+        size_t k = 0;
+        while (k < acc.size() && std::strncmp(acc[k].name, iv.name, sizeof acc[k].name) != 0) k++;
+        if (k == acc.size()) {
+            scr_pass_time t;
+            std::memset(&t, 0, sizeof t);
+            std::snprintf(t.name, sizeof t.name, "%s", iv.name);
+            acc.push_back(t);
+        }
+        acc[k].totalMs += ms;
+        acc[k].launches += 1;
+        ctx->eventPool.push_back(iv.begin);
+        ctx->eventPool.push_back(iv.end);
+    }
+    ctx->intervals.clear();
+    *count = (int)acc.size();
+    for (int i = 0; i < capacity && i < (int)acc.size(); i++) out[i] = acc[(size_t)i];
+    return SCR_OK;
+}
+
 int scr_ctx_destroy(scr_ctx* ctx) {
     if (!ctx) return SCR_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& iv : ctx->intervals) {
+        (void)hipEventDestroy(iv.begin);
+        (void)hipEventDestroy(iv.end);
+    }
+    for (auto e : ctx->eventPool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -749,22 +811,35 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
         scr_cloud* c = nullptr;
         for (int i = 0; i < nClouds && !c; i++)
             if (clouds[i]->residentBatchCount > 0) c = clouds[i];
-        SCR_HIP(ctx, scr::launch_fill_u64(s, frame->dNearest, frame->pixels(), 0ull));
+        {
+            scr::PassScope ps(ctx, "clear_winner");
+            SCR_HIP(ctx, scr::launch_fill_u64(s, frame->dNearest, frame->pixels(), 0ull));
+        }
         const uint32_t* colors = nullptr;
         if (c) {
             rc = upload_files(c, vp, c->model, nullptr, nullptr);
             if (rc != SCR_OK) return rc;
             const CloudDev d = c->dev();
-            SCR_HIP(ctx, scr::launch_cull(s, d, p));
-            SCR_HIP(ctx, scr::launch_nearest(s, ctx->numCUs, d, p, frame->dNearest));
+            {
+                scr::PassScope ps(ctx, "cull");
+                SCR_HIP(ctx, scr::launch_cull(s, d, p));
+            }
+            {
+                scr::PassScope ps(ctx, "nearest");
+                SCR_HIP(ctx, scr::launch_nearest(s, ctx->numCUs, d, p, frame->dNearest));
+            }
             colors = c->dColors;
         }
+        scr::PassScope ps(ctx, "nearest_resolve");
         SCR_HIP(ctx, scr::launch_nearest_resolve(s, p, frame->dNearest, colors, frame->dRGBA, frame->dDepthF));
         return SCR_OK;
     }
 
     // encodeHighQualityAverage (:322-456)
-    SCR_HIP(ctx, scr::launch_fill_u32(s, frame->dPlanes, 5 * frame->planeStride(), 0u));
+    {
+        scr::PassScope ps(ctx, "clear");
+        SCR_HIP(ctx, scr::launch_fill_u32(s, frame->dPlanes, 5 * frame->planeStride(), 0u));
+    }
     std::vector<scr_cloud*> resident;
     for (int i = 0; i < nClouds; i++)
         if (clouds[i]->residentBatchCount > 0) resident.push_back(clouds[i]);
@@ -772,11 +847,20 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
         rc = upload_files(c, vp, c->model, nullptr, nullptr);
         if (rc != SCR_OK) return rc;
     }
-    for (scr_cloud* c : resident) SCR_HIP(ctx, scr::launch_cull(s, c->dev(), p));
-    for (scr_cloud* c : resident) SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
-    for (scr_cloud* c : resident)
+    for (scr_cloud* c : resident) {
+        scr::PassScope ps(ctx, "cull");
+        SCR_HIP(ctx, scr::launch_cull(s, c->dev(), p));
+    }
+    for (scr_cloud* c : resident) {
+        scr::PassScope ps(ctx, "depth");
+        SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
+    }
+    for (scr_cloud* c : resident) {
+        scr::PassScope ps(ctx, "color");
         SCR_HIP(ctx, scr::launch_color(s, ctx->numCUs, c->dev(), p, frame->plane(0), frame->plane(1), frame->plane(2),
                                        frame->plane(3), frame->plane(4)));
+    }
+    scr::PassScope ps(ctx, "resolve");
     SCR_HIP(ctx, scr::launch_resolve(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA, frame->dDepthF));
     return SCR_OK;
 }

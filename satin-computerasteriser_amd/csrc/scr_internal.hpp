@@ -16,6 +16,14 @@ struct scr_ctx {
     hipStream_t stream = nullptr;
     int numCUs = 0;
     std::string lastError;
+    // per-pass timing
+    bool profiling = false;
+    struct Interval {
+        const char* name;
+        hipEvent_t begin, end;
+    };
+    std::vector<Interval> intervals;
+    std::vector<hipEvent_t> eventPool;
     // scratch owned by the context (grown on demand, freed at destroy)
     void* scratch = nullptr;
     size_t scratchBytes = 0;
@@ -96,6 +104,18 @@ struct scr_frame {
 };
 
 namespace scr {
+
+// Brackets the launches issued during its lifetime with events when the
+// context is profiling.
+class PassScope {
+   public:
+    PassScope(scr_ctx* ctx, const char* name);
+    ~PassScope();
+
+   private:
+    scr_ctx* ctx_;
+    int index_ = -1;
+};
 
 int set_error(scr_ctx* ctx, int code, const std::string& msg);
 int hip_error(scr_ctx* ctx, hipError_t e, const char* what);

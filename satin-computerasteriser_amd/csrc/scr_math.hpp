@@ -160,6 +160,7 @@ struct PassParams {
     int colorizeOverdraw;
     int applyTint;
     int constRadius;        // >= 0: footprint radius is this constant (lo == hi); -1: per point
+    int maxRadius;          // upper bound of the footprint radius (radius of hi)
     int ortho;              // proj[3][3] == 1
     float focal;            // float(height) * 0.5 * proj[1][1]
     float lo, hi;           // point-size clamp

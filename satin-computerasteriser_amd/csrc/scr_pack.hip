@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void k_gather_final(const float4* __restr
 struct LodGrid {
     float voxelScale;   // 0.5^level
     float coarse;
-    uint32_t dx, dy;
+    uint32_t dx, dy, dz;
 };
 
 __device__ inline void lod_cell(const uint32_t* bounds, const LodGrid& g, float4 p, uint32_t& cx, uint32_t& cy,
@@ -328,9 +328,11 @@ __device__ inline void lod_cell(const uint32_t* bounds, const LodGrid& g, float4
     x = x < 0 ? 0 : x;
     y = y < 0 ? 0 : y;
     z = z < 0 ? 0 : z;
+    // (p - min) * inv <= extent * inv, so the clamps never bind; they keep
+    // every cell index inside the grid for any input
     cx = min((uint32_t)x, g.dx - 1u);
     cy = min((uint32_t)y, g.dy - 1u);
-    cz = (uint32_t)z;
+    cz = min((uint32_t)z, g.dz - 1u);
 }
 
 __global__ __launch_bounds__(kThreads) void k_lod_init(uint8_t* __restrict__ lv, size_t n, uint8_t maxLevel) {
@@ -695,8 +697,8 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
         const float inv = 1.0f / std::fmax(voxel, 0.000001f);
         g.dx = (uint32_t)f2u(std::floor(ext[0] * inv)) + 1u;
         g.dy = (uint32_t)f2u(std::floor(ext[1] * inv)) + 1u;
-        const uint32_t dz = (uint32_t)f2u(std::floor(ext[2] * inv)) + 1u;
-        const size_t cells = (size_t)g.dx * g.dy * dz;
+        g.dz = (uint32_t)f2u(std::floor(ext[2] * inv)) + 1u;
+        const size_t cells = (size_t)g.dx * g.dy * g.dz;
         e = launch_fill_u32(s, grid, cells, 0xffffffffu);
         if (e != hipSuccess) break;
         hipLaunchKernelGGL(k_lod_claim, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel, dBounds,

@@ -115,9 +115,11 @@ struct BatchCtx {
     float sx, sy, sz;   // batch size * 2^-bits
     int level;
     float lodThreshold;
+    bool keepAll;       // no point of the batch can fail the LOD/dither test
     uint32_t first, active, batchIndex;
-    const float* T;     // file.transform
-    const float* W;     // file.world
+    float T[16];        // file.transform (held in registers: the atomics below
+                        // would otherwise force reloads through memory)
+    const float* W;     // file.world (only read for per-point sizes)
 };
 
 __device__ inline BatchCtx batch_ctx(const CloudDev& c, uint32_t v) {
@@ -126,6 +128,8 @@ __device__ inline BatchCtx batch_ctx(const CloudDev& c, uint32_t v) {
     BatchCtx bc;
     bc.level = vb.level;
     bc.lodThreshold = vb.lodThreshold;
+    // dither is in [0, 1] and (thr - L) + 0.5 decreases with L <= 7
+    bc.keepAll = ((vb.lodThreshold - 7.0f) + 0.5f) > 1.0f;
     bc.active = vb.activePoints;
     bc.batchIndex = vb.batchIndex;
     bc.first = b.firstPoint;
@@ -136,134 +140,386 @@ __device__ inline BatchCtx batch_ctx(const CloudDev& c, uint32_t v) {
     bc.sx = fmaxf(b.maxX - b.minX, 0.000001f) * inv;
     bc.sy = fmaxf(b.maxY - b.minY, 0.000001f) * inv;
     bc.sz = fmaxf(b.maxZ - b.minZ, 0.000001f) * inv;
-    bc.T = c.files[b.fileIndex].transform;
-    bc.W = c.files[b.fileIndex].world;
+    const scr_raster_file* f = c.files + b.fileIndex;
+#pragma unroll
+    for (int i = 0; i < 16; i++) bc.T[i] = f->transform[i];
+    bc.W = f->world;
     return bc;
 }
 
-struct Frag {
-    float x, y, z;   // decoded point
-    float ndcZ;
-    int px, py;      // pixel, y already flipped
+// ---------------------------------------------------------------- chunked draw
+//
+// Each visible batch is processed in chunks of kChunk points: every thread
+// owns kPPT consecutive points (vector loads of the position words), keeps
+// their fragments in registers, and the block reduces the chunk's actual
+// screen rectangle.  When that rectangle is small next to the fragment count,
+// the chunk's fragments meet in an LDS tile (LDS atomics) and the global
+// atomics see one update per touched pixel; otherwise they go straight to the
+// global planes.  Integer max / add are order independent, so both routes
+// give the same planes.
+
+constexpr int kPPT = 8;
+constexpr int kChunk = kDrawThreads * kPPT;
+
+struct ChunkFrags {
+    int px[kPPT];      // -1: no fragment
+    int py[kPPT];      // y-flipped row
+    int r[kPPT];       // footprint radius
+    float z[kPPT];     // ndc z
 };
 
-// dither/LOD skip, decodePoint (Common.metal:73-117), projection, clip and
-// viewport tests, y flip — the shared prologue of every point pass.
+// Loads the position words (and levels) of the thread's kPPT points and runs
+// the front end of every point pass on each of them.
 template <bool kNearestZ>
-__device__ inline bool point_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t pi,
-                                   Frag& fr) {
-    const float pointLevel = (float)((uint32_t)c.levels[pi] & 7u);
-    const float dither = (p.lodDither != 0) ? hash_unit(pi) : 0.5f;
-    if (dither >= (bc.lodThreshold - pointLevel) + 0.5f) return false;
-    const uint32_t lo = c.xyzLow[pi];
-    uint32_t qx, qy, qz;
-    if (bc.level == 0) {
-        const uint32_t me = c.xyzMed[pi], hi = c.xyzHigh[pi];
-        qx = ((lo & 1023u) << 20) | ((me & 1023u) << 10) | (hi & 1023u);
-        qy = (((lo >> 10) & 1023u) << 20) | (((me >> 10) & 1023u) << 10) | ((hi >> 10) & 1023u);
-        qz = (((lo >> 20) & 1023u) << 20) | (((me >> 20) & 1023u) << 10) | ((hi >> 20) & 1023u);
-    } else if (bc.level == 1) {
-        const uint32_t me = c.xyzMed[pi];
-        qx = ((lo & 1023u) << 10) | (me & 1023u);
-        qy = (((lo >> 10) & 1023u) << 10) | ((me >> 10) & 1023u);
-        qz = (((lo >> 20) & 1023u) << 10) | ((me >> 20) & 1023u);
+__device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
+                                   bool vec, ChunkFrags& f) {
+    uint32_t lo[kPPT], me[kPPT], hi[kPPT], lv[kPPT];
+    const uint32_t base = bc.first + li0;
+    const uint32_t n = li0 < bc.active ? min((uint32_t)kPPT, bc.active - li0) : 0u;
+    if (vec && n == (uint32_t)kPPT) {
+        // base is a multiple of kPPT: 32-byte aligned words, 8-byte aligned levels
+        const uint4* pl = reinterpret_cast<const uint4*>(c.xyzLow + base);
+        const uint4 a = pl[0], b = pl[1];
+        lo[0] = a.x; lo[1] = a.y; lo[2] = a.z; lo[3] = a.w; lo[4] = b.x; lo[5] = b.y; lo[6] = b.z; lo[7] = b.w;
+        if (bc.level <= 1) {
+            const uint4* pm = reinterpret_cast<const uint4*>(c.xyzMed + base);
+            const uint4 ma = pm[0], mb = pm[1];
+            me[0] = ma.x; me[1] = ma.y; me[2] = ma.z; me[3] = ma.w; me[4] = mb.x; me[5] = mb.y; me[6] = mb.z; me[7] = mb.w;
+        }
+        if (bc.level == 0) {
+            const uint4* ph = reinterpret_cast<const uint4*>(c.xyzHigh + base);
+            const uint4 ha = ph[0], hb = ph[1];
+            hi[0] = ha.x; hi[1] = ha.y; hi[2] = ha.z; hi[3] = ha.w; hi[4] = hb.x; hi[5] = hb.y; hi[6] = hb.z; hi[7] = hb.w;
+        }
+        if (!bc.keepAll) {
+            const uint2 l8 = *reinterpret_cast<const uint2*>(c.levels + base);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                lv[j] = (l8.x >> (8 * j)) & 0xffu;
+                lv[4 + j] = (l8.y >> (8 * j)) & 0xffu;
+            }
+        }
     } else {
-        qx = lo & 1023u;
-        qy = (lo >> 10) & 1023u;
-        qz = (lo >> 20) & 1023u;
+#pragma unroll
+        for (int j = 0; j < kPPT; j++) {
+            const uint32_t pi = base + (uint32_t)j;
+            const bool ok = (uint32_t)j < n;
+            lo[j] = ok ? c.xyzLow[pi] : 0u;
+            me[j] = (ok && bc.level <= 1) ? c.xyzMed[pi] : 0u;
+            hi[j] = (ok && bc.level == 0) ? c.xyzHigh[pi] : 0u;
+            lv[j] = (ok && !bc.keepAll) ? (uint32_t)c.levels[pi] : 0u;
+        }
     }
-    fr.x = (float)qx * bc.sx + bc.mnx;
-    fr.y = (float)qy * bc.sy + bc.mny;
-    fr.z = (float)qz * bc.sz + bc.mnz;
-    const f4 clip = mul_point(bc.T, fr.x, fr.y, fr.z);
-    if (clip.w <= 0.0f) return false;
-    const float rw = rcp(clip.w);
-    const float nx = clip.x * rw, ny = clip.y * rw, nz = clip.z * rw;
-    if (nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f) return false;
-    if (kNearestZ) {
-        if (nz <= 0.0f || nz > 1.0f) return false;
-    } else {
-        if (nz < 0.0f || nz > 1.0f) return false;
+#pragma unroll
+    for (int j = 0; j < kPPT; j++) {
+        f.px[j] = -1;
+        f.py[j] = 0;
+        f.r[j] = 0;
+        f.z[j] = 0.0f;
+        if ((uint32_t)j >= n) continue;
+        const uint32_t pi = base + (uint32_t)j;
+        if (!bc.keepAll) {
+            const float pointLevel = (float)(lv[j] & 7u);
+            const float dither = (p.lodDither != 0) ? hash_unit(pi) : 0.5f;
+            if (dither >= (bc.lodThreshold - pointLevel) + 0.5f) continue;
+        }
+        uint32_t qx, qy, qz;
+        const uint32_t l = lo[j];
+        if (bc.level == 0) {
+            const uint32_t m = me[j], h = hi[j];
+            qx = ((l & 1023u) << 20) | ((m & 1023u) << 10) | (h & 1023u);
+            qy = (((l >> 10) & 1023u) << 20) | (((m >> 10) & 1023u) << 10) | ((h >> 10) & 1023u);
+            qz = (((l >> 20) & 1023u) << 20) | (((m >> 20) & 1023u) << 10) | ((h >> 20) & 1023u);
+        } else if (bc.level == 1) {
+            const uint32_t m = me[j];
+            qx = ((l & 1023u) << 10) | (m & 1023u);
+            qy = (((l >> 10) & 1023u) << 10) | ((m >> 10) & 1023u);
+            qz = (((l >> 20) & 1023u) << 10) | ((m >> 20) & 1023u);
+        } else {
+            qx = l & 1023u;
+            qy = (l >> 10) & 1023u;
+            qz = (l >> 20) & 1023u;
+        }
+        const float x = (float)qx * bc.sx + bc.mnx;
+        const float y = (float)qy * bc.sy + bc.mny;
+        const float z = (float)qz * bc.sz + bc.mnz;
+        const f4 clip = mul_point(bc.T, x, y, z);
+        if (clip.w <= 0.0f) continue;
+        const float rw = rcp(clip.w);
+        const float nx = clip.x * rw, ny = clip.y * rw, nz = clip.z * rw;
+        if (nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f) continue;
+        if (kNearestZ) {
+            if (nz <= 0.0f || nz > 1.0f) continue;
+        } else {
+            if (nz < 0.0f || nz > 1.0f) continue;
+        }
+        const int ix = f2i((nx * 0.5f + 0.5f) * (float)p.width);
+        const int iy = f2i((ny * 0.5f + 0.5f) * (float)p.height);
+        if (ix < 0 || ix >= p.width || iy < 0 || iy >= p.height) continue;
+        f.px[j] = ix;
+        f.py[j] = p.height - 1 - iy;
+        f.z[j] = nz;
+        f.r[j] = footprint_radius(p, bc.W, x, y, z);
     }
-    const int px = f2i((nx * 0.5f + 0.5f) * (float)p.width);
-    const int py = f2i((ny * 0.5f + 0.5f) * (float)p.height);
-    if (px < 0 || px >= p.width || py < 0 || py >= p.height) return false;
-    fr.px = px;
-    fr.py = p.height - 1 - py;
-    fr.ndcZ = nz;
-    return true;
 }
+
+// Screen rectangle of the chunk's fragments (footprints included, clamped to
+// the screen) and the number of fragment centres.
+struct ChunkBox {
+    int x0, y0, w, h, frags;
+    uint32_t magic;   // i / w == umulhi(i, magic) for i < 2^16
+    bool lds;
+    __device__ inline bool local(int tx, int ty, int& k) const {
+        const int lx = tx - x0, ly = ty - y0;
+        if ((unsigned)lx >= (unsigned)w || (unsigned)ly >= (unsigned)h) return false;
+        k = ly * w + lx;
+        return true;
+    }
+    __device__ inline size_t pixel(int k, int width) const {
+        const int row = (int)__umulhi((uint32_t)k, magic);
+        const int col = k - row * w;
+        return (size_t)(y0 + row) * (size_t)width + (size_t)(x0 + col);
+    }
+};
+
+__device__ inline int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ inline int wave_sum(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block-wide reduction; red must hold 5 * (kDrawThreads / 64) ints.  Ends
+// with a barrier, so the caller may reuse LDS right after.
+__device__ inline ChunkBox chunk_box(const ChunkFrags& f, const PassParams& p, int cap, int* red) {
+    int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1, n = 0;
+#pragma unroll
+    for (int j = 0; j < kPPT; j++) {
+        if (f.px[j] < 0) continue;
+        x0 = min(x0, f.px[j] - f.r[j]);
+        x1 = max(x1, f.px[j] + f.r[j]);
+        y0 = min(y0, f.py[j] - f.r[j]);
+        y1 = max(y1, f.py[j] + f.r[j]);
+        n++;
+    }
+    x0 = wave_min(x0);
+    y0 = wave_min(y0);
+    x1 = wave_max(x1);
+    y1 = wave_max(y1);
+    n = wave_sum(n);
+    const int w = threadIdx.x >> 6;
+    constexpr int kW = kDrawThreads / 64;
+    if (__lane_id() == 0) {
+        red[w] = x0;
+        red[kW + w] = y0;
+        red[2 * kW + w] = x1;
+        red[3 * kW + w] = y1;
+        red[4 * kW + w] = n;
+    }
+    __syncthreads();
+    x0 = red[0];
+    y0 = red[kW];
+    x1 = red[2 * kW];
+    y1 = red[3 * kW];
+    n = red[4 * kW];
+#pragma unroll
+    for (int k = 1; k < kW; k++) {
+        x0 = min(x0, red[k]);
+        y0 = min(y0, red[kW + k]);
+        x1 = max(x1, red[2 * kW + k]);
+        y1 = max(y1, red[3 * kW + k]);
+        n += red[4 * kW + k];
+    }
+    __syncthreads();
+    ChunkBox b;
+    b.frags = n;
+    b.x0 = max(x0, 0);
+    b.y0 = max(y0, 0);
+    x1 = min(x1, p.width - 1);
+    y1 = min(y1, p.height - 1);
+    b.w = x1 >= b.x0 ? x1 - b.x0 + 1 : 0;
+    b.h = y1 >= b.y0 ? y1 - b.y0 + 1 : 0;
+    const long long area = (long long)b.w * b.h;
+    // LDS pays when the tile is small next to the fragments it absorbs
+    b.lds = n > 0 && area > 0 && area <= cap && area <= 8LL * n;
+    b.magic = b.lds ? (uint32_t)(0xffffffffu / (uint32_t)b.w) + 1u : 0u;
+    return b;
+}
+
+__device__ inline bool vec_ok(const BatchCtx& bc) { return (bc.first & (uint32_t)(kPPT - 1)) == 0u; }
 
 // ---------------------------------------------------------------- depth pass
 
+constexpr int kDepthTile = 4096;  // u32 per pixel: 16 KB
+
 __global__ __launch_bounds__(kDrawThreads) void k_depth(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
+    __shared__ uint32_t lds[kDepthTile];
+    __shared__ int red[5 * (kDrawThreads / 64)];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
         const BatchCtx bc = batch_ctx(c, v);
-        for (uint32_t li = threadIdx.x; li < bc.active; li += kDrawThreads) {
-            const uint32_t pi = bc.first + li;
-            Frag fr;
-            if (!point_front<false>(c, p, bc, pi, fr)) continue;
-            const uint32_t d = depth_to_uint(fr.ndcZ);
-            const int r = footprint_radius(p, bc.W, fr.x, fr.y, fr.z);
-            if (r == 0) {
-                atomicMax(depth + (size_t)fr.py * p.width + fr.px, d);
-                continue;
+        const bool vec = vec_ok(bc);
+        for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
+            ChunkFrags f;
+            chunk_front<false>(c, p, bc, cb + threadIdx.x * kPPT, vec, f);
+            const ChunkBox box = chunk_box(f, p, kDepthTile, red);
+            if (box.frags == 0) continue;
+            const int area = box.w * box.h;
+            if (box.lds) {
+                for (int i = threadIdx.x; i < area; i += kDrawThreads) lds[i] = 0u;
+                __syncthreads();
             }
-            for (int oy = -r; oy <= r; oy++)
-                for (int ox = -r; ox <= r; ox++) {
-                    if (!inside_footprint(ox, oy, r)) continue;
-                    const int tx = fr.px + ox, ty = fr.py + oy;
-                    if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                    atomicMax(depth + (size_t)ty * p.width + tx, d);
+#pragma unroll
+            for (int j = 0; j < kPPT; j++) {
+                if (f.px[j] < 0) continue;
+                const uint32_t d = depth_to_uint(f.z[j]);
+                const int r = f.r[j];
+                for (int oy = -r; oy <= r; oy++)
+                    for (int ox = -r; ox <= r; ox++) {
+                        if (!inside_footprint(ox, oy, r)) continue;
+                        const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                        if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                        int k;
+                        if (box.lds && box.local(tx, ty, k)) atomicMax(lds + k, d);
+                        else atomicMax(depth + (size_t)ty * p.width + tx, d);
+                    }
+            }
+            if (box.lds) {
+                __syncthreads();
+                for (int i = threadIdx.x; i < area; i += kDrawThreads) {
+                    const uint32_t d = lds[i];
+                    if (d == 0u) continue;
+                    uint32_t* g = depth + box.pixel(i, p.width);
+                    // depth only grows during the pass: a stale read can only
+                    // cause a redundant atomic, never a missed one
+                    if (d > *g) atomicMax(g, d);
                 }
+                __syncthreads();
+            }
         }
     }
 }
 
 // ---------------------------------------------------------------- colour pass
 
+constexpr int kColorTile = 2048;  // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 40 KB
+
 __global__ __launch_bounds__(kDrawThreads) void k_color(CloudDev c, PassParams p, const uint32_t* __restrict__ depth,
                                                         uint32_t* red, uint32_t* green, uint32_t* blue,
                                                         uint32_t* count) {
+    __shared__ uint32_t ldsDepth[kColorTile];
+    __shared__ unsigned long long ldsRG[kColorTile];
+    __shared__ unsigned long long ldsBC[kColorTile];
+    __shared__ int redbuf[5 * (kDrawThreads / 64)];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
         const BatchCtx bc = batch_ctx(c, v);
-        for (uint32_t li = threadIdx.x; li < bc.active; li += kDrawThreads) {
-            const uint32_t pi = bc.first + li;
-            Frag fr;
-            if (!point_front<false>(c, p, bc, pi, fr)) continue;
-            uint32_t color = c.colors[pi];
-            if (p.colorizeChunks != 0) color = bc.batchIndex * 1234567u;
-            else if (p.colorizeOverdraw != 0) color = 0x00010101u;
-            uint32_t r8 = color & 0xffu, g8 = (color >> 8) & 0xffu, b8 = (color >> 16) & 0xffu;
-            if (p.applyTint != 0) {
-                const float4 t = c.tints ? c.tints[pi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (t.w < 0.0f) continue;  // discard sentinel, ColorPass/Shaders.metal:119
-                const float w = saturate(t.w);
-                const float ox = (float)r8 * kInv255, oy = (float)g8 * kInv255, oz = (float)b8 * kInv255;
-                const float mx = ox + (saturate(t.x) - ox) * w;
-                const float my = oy + (saturate(t.y) - oy) * w;
-                const float mz = oz + (saturate(t.z) - oz) * w;
-                r8 = f2u(saturate(mx) * 255.0f + 0.5f);
-                g8 = f2u(saturate(my) * 255.0f + 0.5f);
-                b8 = f2u(saturate(mz) * 255.0f + 0.5f);
+        const bool vec = vec_ok(bc);
+        for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
+            const uint32_t li0 = cb + threadIdx.x * kPPT;
+            ChunkFrags f;
+            chunk_front<false>(c, p, bc, li0, vec, f);
+            // colour stage (ColorPass/Shaders.metal:95-133): packed r | g << 8 | b << 16
+            uint32_t rgb[kPPT];
+            const uint32_t base = bc.first + li0;
+            if (vec && li0 + (uint32_t)kPPT <= bc.active) {
+                const uint4* pc = reinterpret_cast<const uint4*>(c.colors + base);
+                const uint4 ca = pc[0], cb2 = pc[1];
+                rgb[0] = ca.x; rgb[1] = ca.y; rgb[2] = ca.z; rgb[3] = ca.w;
+                rgb[4] = cb2.x; rgb[5] = cb2.y; rgb[6] = cb2.z; rgb[7] = cb2.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < kPPT; j++) rgb[j] = f.px[j] >= 0 ? c.colors[base + (uint32_t)j] : 0u;
             }
-            const int rad = footprint_radius(p, bc.W, fr.x, fr.y, fr.z);
-            for (int oy = -rad; oy <= rad; oy++)
-                for (int ox = -rad; ox <= rad; ox++) {
-                    if (!inside_footprint(ox, oy, rad)) continue;
-                    const int tx = fr.px + ox, ty = fr.py + oy;
-                    if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                    const size_t pix = (size_t)ty * p.width + tx;
-                    const uint32_t d = depth[pix];
-                    if (d == 0u) continue;
-                    if (!(fr.ndcZ >= uint_to_depth(d) * p.depthKeep)) continue;
-                    atomicAdd(red + pix, r8);
-                    atomicAdd(green + pix, g8);
-                    atomicAdd(blue + pix, b8);
-                    atomicAdd(count + pix, 1u);
+#pragma unroll
+            for (int j = 0; j < kPPT; j++) {
+                const uint32_t pi = base + (uint32_t)j;
+                uint32_t color = rgb[j];
+                rgb[j] = 0u;
+                if (f.px[j] < 0) continue;
+                if (p.colorizeChunks != 0) color = bc.batchIndex * 1234567u;
+                else if (p.colorizeOverdraw != 0) color = 0x00010101u;
+                uint32_t r8 = color & 0xffu, g8 = (color >> 8) & 0xffu, b8 = (color >> 16) & 0xffu;
+                if (p.applyTint != 0) {
+                    const float4 t = c.tints ? c.tints[pi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (t.w < 0.0f) {  // discard sentinel, ColorPass/Shaders.metal:119
+                        f.px[j] = -1;
+                        continue;
+                    }
+                    const float w = saturate(t.w);
+                    const float ox = (float)r8 * kInv255, oy = (float)g8 * kInv255, oz = (float)b8 * kInv255;
+                    const float mx = ox + (saturate(t.x) - ox) * w;
+                    const float my = oy + (saturate(t.y) - oy) * w;
+                    const float mz = oz + (saturate(t.z) - oz) * w;
+                    r8 = f2u(saturate(mx) * 255.0f + 0.5f);
+                    g8 = f2u(saturate(my) * 255.0f + 0.5f);
+                    b8 = f2u(saturate(mz) * 255.0f + 0.5f);
                 }
+                rgb[j] = r8 | (g8 << 8) | (b8 << 16);
+            }
+            const ChunkBox box = chunk_box(f, p, kColorTile, redbuf);
+            if (box.frags == 0) continue;
+            const int area = box.w * box.h;
+            if (box.lds) {
+                for (int i = threadIdx.x; i < area; i += kDrawThreads) {
+                    ldsDepth[i] = depth[box.pixel(i, p.width)];
+                    ldsRG[i] = 0ull;
+                    ldsBC[i] = 0ull;
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int j = 0; j < kPPT; j++) {
+                if (f.px[j] < 0) continue;
+                const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
+                const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
+                const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
+                const int rad = f.r[j];
+                for (int oy = -rad; oy <= rad; oy++)
+                    for (int ox = -rad; ox <= rad; ox++) {
+                        if (!inside_footprint(ox, oy, rad)) continue;
+                        const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                        if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                        int k;
+                        const bool loc = box.lds && box.local(tx, ty, k);
+                        const size_t pix = (size_t)ty * p.width + tx;
+                        const uint32_t d = loc ? ldsDepth[k] : depth[pix];
+                        if (d == 0u) continue;
+                        if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                        if (loc) {
+                            atomicAdd(ldsRG + k, rg);
+                            atomicAdd(ldsBC + k, bn);
+                        } else {
+                            atomicAdd(red + pix, r8);
+                            atomicAdd(green + pix, g8);
+                            atomicAdd(blue + pix, b8);
+                            atomicAdd(count + pix, 1u);
+                        }
+                    }
+            }
+            if (box.lds) {
+                __syncthreads();
+                // per-chunk sums stay below 2^32 (<= 2048 fragments x 255 per
+                // pixel), so the packed halves never carry into each other
+                for (int i = threadIdx.x; i < area; i += kDrawThreads) {
+                    const unsigned long long bn = ldsBC[i];
+                    const uint32_t n = (uint32_t)(bn >> 32);
+                    if (n == 0u) continue;
+                    const unsigned long long rg = ldsRG[i];
+                    const size_t pix = box.pixel(i, p.width);
+                    atomicAdd(red + pix, (uint32_t)rg);
+                    atomicAdd(green + pix, (uint32_t)(rg >> 32));
+                    atomicAdd(blue + pix, (uint32_t)bn);
+                    atomicAdd(count + pix, n);
+                }
+                __syncthreads();
+            }
         }
     }
 }
@@ -297,25 +553,56 @@ __global__ __launch_bounds__(256) void k_resolve(PassParams p, const uint32_t* _
 
 // ---------------------------------------------------------------- nearest
 
+constexpr int kNearestTile = 2048;  // u64 per pixel: 16 KB
+
 __global__ __launch_bounds__(kDrawThreads) void k_nearest(CloudDev c, PassParams p,
                                                           unsigned long long* __restrict__ packed) {
+    __shared__ unsigned long long lds[kNearestTile];
+    __shared__ int red[5 * (kDrawThreads / 64)];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
         const BatchCtx bc = batch_ctx(c, v);
-        for (uint32_t li = threadIdx.x; li < bc.active; li += kDrawThreads) {
-            const uint32_t pi = bc.first + li;
-            Frag fr;
-            if (!point_front<true>(c, p, bc, pi, fr)) continue;
-            const unsigned long long key =
-                ((unsigned long long)depth_to_uint(fr.ndcZ) << 32) | (unsigned long long)(~pi);
-            const int r = footprint_radius(p, bc.W, fr.x, fr.y, fr.z);
-            for (int oy = -r; oy <= r; oy++)
-                for (int ox = -r; ox <= r; ox++) {
-                    if (!inside_footprint(ox, oy, r)) continue;
-                    const int tx = fr.px + ox, ty = fr.py + oy;
-                    if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                    atomicMax(packed + (size_t)ty * p.width + tx, key);
+        const bool vec = vec_ok(bc);
+        for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
+            const uint32_t li0 = cb + threadIdx.x * kPPT;
+            ChunkFrags f;
+            chunk_front<true>(c, p, bc, li0, vec, f);
+            const ChunkBox box = chunk_box(f, p, kNearestTile, red);
+            if (box.frags == 0) continue;
+            const int area = box.w * box.h;
+            if (box.lds) {
+                for (int i = threadIdx.x; i < area; i += kDrawThreads) lds[i] = 0ull;
+                __syncthreads();
+            }
+#pragma unroll
+            for (int j = 0; j < kPPT; j++) {
+                if (f.px[j] < 0) continue;
+                const uint32_t pi = bc.first + li0 + (uint32_t)j;
+                // max of depth << 32 | ~index == the two-pass rule: deepest
+                // reverse-Z depth, then the lowest point index among equal depths
+                const unsigned long long key =
+                    ((unsigned long long)depth_to_uint(f.z[j]) << 32) | (unsigned long long)(~pi);
+                const int r = f.r[j];
+                for (int oy = -r; oy <= r; oy++)
+                    for (int ox = -r; ox <= r; ox++) {
+                        if (!inside_footprint(ox, oy, r)) continue;
+                        const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                        if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                        int k;
+                        if (box.lds && box.local(tx, ty, k)) atomicMax(lds + k, key);
+                        else atomicMax(packed + (size_t)ty * p.width + tx, key);
+                    }
+            }
+            if (box.lds) {
+                __syncthreads();
+                for (int i = threadIdx.x; i < area; i += kDrawThreads) {
+                    const unsigned long long k = lds[i];
+                    if (k == 0ull) continue;
+                    unsigned long long* g = packed + box.pixel(i, p.width);
+                    if (k > *g) atomicMax(g, k);
                 }
+                __syncthreads();
+            }
         }
     }
 }

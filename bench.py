@@ -144,6 +144,9 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (the known-size copy kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): this build has no multi-GPU "
@@ -169,6 +172,15 @@ def main() -> int:
     log(f"packed in {pack_ms:.1f} ms ({cloud.residentBatchCount} batches)")
     pos.free()
     col.free()
+
+    if args.copy_calibration_mb > 0:
+        nbytes = args.copy_calibration_mb << 20
+        a, b = scr.DeviceBuffer(ctx, nbytes), scr.DeviceBuffer(ctx, nbytes)
+        ctx.check(ctx.lib.scr_memcpy_d2d(ctx.handle, b.ptr, a.ptr, nbytes))
+        ctx.sync()
+        a.free()
+        b.free()
+        log(f"copy calibration: {args.copy_calibration_mb} MiB device-to-device")
 
     cfg = scr.ComputeRasteriserConfiguration(enableCLOD=False)
     cam = overview_camera(scr, W, H)

@@ -146,6 +146,8 @@ int scr_device_alloc(scr_ctx* ctx, size_t bytes, void** out);
 int scr_device_free(scr_ctx* ctx, void* ptr);
 int scr_memcpy_h2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
 int scr_memcpy_d2h(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy, enqueued on the context stream. */
+int scr_memcpy_d2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ---- point cloud slot pool: ComputeRasteriserPointCloud.swift */
 

@@ -138,6 +138,7 @@ _SIGNATURES = {
     "scr_device_free": (_I, [_VP, _VP]),
     "scr_memcpy_h2d": (_I, [_VP, _VP, _VP, _SZ]),
     "scr_memcpy_d2h": (_I, [_VP, _VP, _VP, _SZ]),
+    "scr_memcpy_d2d": (_I, [_VP, _VP, _VP, _SZ]),
     "scr_cloud_create": (_I, [_VP, _I, _I, _VP, _I, C.POINTER(_VP)]),
     "scr_cloud_destroy": (_I, [_VP]),
     "scr_cloud_get_info": (_I, [_VP, C.POINTER(CCloudInfo)]),

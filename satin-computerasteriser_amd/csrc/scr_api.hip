@@ -342,6 +342,13 @@ int scr_memcpy_d2h(scr_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return SCR_OK;
 }
 
+int scr_memcpy_d2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return SCR_ERR_INVALID_ARGUMENT;
+    if (!bytes) return SCR_OK;
+    SCR_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return SCR_OK;
+}
+
 // ------------------------------------------------------------------ clouds
 
 int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, const scr_raster_file* files,

@@ -38,10 +38,24 @@ __host__ __device__ inline int32_t f2i(float x) {
     return (int32_t)x;
 }
 
+// Device forms of f2i / f2u: the gfx950 conversions truncate, saturate and
+// map NaN to 0 — exactly the contract above — in one instruction.
+__device__ inline int32_t f2i_hw(float x) {
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ inline uint32_t f2u_hw(float x) {
+    uint32_t r;
+    asm("v_cvt_u32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // Common.metal:10-12 — float(CR_MAX_DEPTH - 1u) is 2^32 in binary32.
 __host__ __device__ inline uint32_t depth_to_uint(float z) { return f2u(saturate(z) * kTwo32); }
 // Common.metal:14-16
 __host__ __device__ inline float uint_to_depth(uint32_t d) { return (float)d * kInvTwo32; }
+__device__ inline uint32_t depth_to_uint_hw(float z) { return f2u_hw(saturate(z) * kTwo32); }
 
 // Common.metal:190-197 (lowbias32)
 __host__ __device__ inline float hash_unit(uint32_t x) {

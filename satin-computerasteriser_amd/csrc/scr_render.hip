@@ -170,7 +170,7 @@ struct ChunkFrags {
 
 // Loads the position words (and levels) of the thread's kPPT points and runs
 // the front end of every point pass on each of them.
-template <bool kNearestZ>
+template <bool kNearestZ, bool kR0>
 __device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                    bool vec, ChunkFrags& f) {
     uint32_t lo[kPPT], me[kPPT], hi[kPPT], lv[kPPT];
@@ -210,18 +210,17 @@ __device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const
             lv[j] = (ok && !bc.keepAll) ? (uint32_t)c.levels[pi] : 0u;
         }
     }
+    // Branch-free per point: every test of the reference folds into `ok`
+    // (each written as the negation of its reject condition, so NaNs behave
+    // as in the Metal source) and the fragment is kept or dropped once.
 #pragma unroll
     for (int j = 0; j < kPPT; j++) {
-        f.px[j] = -1;
-        f.py[j] = 0;
-        f.r[j] = 0;
-        f.z[j] = 0.0f;
-        if ((uint32_t)j >= n) continue;
         const uint32_t pi = base + (uint32_t)j;
+        bool ok = (uint32_t)j < n;
         if (!bc.keepAll) {
             const float pointLevel = (float)(lv[j] & 7u);
             const float dither = (p.lodDither != 0) ? hash_unit(pi) : 0.5f;
-            if (dither >= (bc.lodThreshold - pointLevel) + 0.5f) continue;
+            ok = ok && !(dither >= (bc.lodThreshold - pointLevel) + 0.5f);
         }
         uint32_t qx, qy, qz;
         const uint32_t l = lo[j];
@@ -244,22 +243,19 @@ __device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const
         const float y = (float)qy * bc.sy + bc.mny;
         const float z = (float)qz * bc.sz + bc.mnz;
         const f4 clip = mul_point(bc.T, x, y, z);
-        if (clip.w <= 0.0f) continue;
+        ok = ok && !(clip.w <= 0.0f);
         const float rw = rcp(clip.w);
         const float nx = clip.x * rw, ny = clip.y * rw, nz = clip.z * rw;
-        if (nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f) continue;
-        if (kNearestZ) {
-            if (nz <= 0.0f || nz > 1.0f) continue;
-        } else {
-            if (nz < 0.0f || nz > 1.0f) continue;
-        }
-        const int ix = f2i((nx * 0.5f + 0.5f) * (float)p.width);
-        const int iy = f2i((ny * 0.5f + 0.5f) * (float)p.height);
-        if (ix < 0 || ix >= p.width || iy < 0 || iy >= p.height) continue;
-        f.px[j] = ix;
+        ok = ok && !(nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f);
+        if (kNearestZ) ok = ok && !(nz <= 0.0f || nz > 1.0f);
+        else ok = ok && !(nz < 0.0f || nz > 1.0f);
+        const int ix = f2i_hw((nx * 0.5f + 0.5f) * (float)p.width);
+        const int iy = f2i_hw((ny * 0.5f + 0.5f) * (float)p.height);
+        ok = ok && !(ix < 0 || ix >= p.width || iy < 0 || iy >= p.height);
+        f.px[j] = ok ? ix : -1;
         f.py[j] = p.height - 1 - iy;
         f.z[j] = nz;
-        f.r[j] = footprint_radius(p, bc.W, x, y, z);
+        f.r[j] = (kR0 || !ok) ? 0 : footprint_radius(p, bc.W, x, y, z);
     }
 }
 
@@ -269,12 +265,9 @@ struct ChunkBox {
     int x0, y0, w, h, frags;
     uint32_t magic;   // i / w == umulhi(i, magic) for i < 2^16
     bool lds;
-    __device__ inline bool local(int tx, int ty, int& k) const {
-        const int lx = tx - x0, ly = ty - y0;
-        if ((unsigned)lx >= (unsigned)w || (unsigned)ly >= (unsigned)h) return false;
-        k = ly * w + lx;
-        return true;
-    }
+    // every on-screen footprint pixel of the chunk lies inside the box (it is
+    // the clamped union of the footprints), so no bounds test is needed
+    __device__ inline int local(int tx, int ty) const { return (ty - y0) * w + (tx - x0); }
     __device__ inline size_t pixel(int k, int width) const {
         const int row = (int)__umulhi((uint32_t)k, magic);
         const int col = k - row * w;
@@ -358,6 +351,7 @@ __device__ inline bool vec_ok(const BatchCtx& bc) { return (bc.first & (uint32_t
 
 constexpr int kDepthTile = 4096;  // u32 per pixel: 16 KB
 
+template <bool kR0>
 __global__ __launch_bounds__(kDrawThreads) void k_depth(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
     __shared__ uint32_t lds[kDepthTile];
     __shared__ int red[5 * (kDrawThreads / 64)];
@@ -367,7 +361,7 @@ __global__ __launch_bounds__(kDrawThreads) void k_depth(CloudDev c, PassParams p
         const bool vec = vec_ok(bc);
         for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
             ChunkFrags f;
-            chunk_front<false>(c, p, bc, cb + threadIdx.x * kPPT, vec, f);
+            chunk_front<false, kR0>(c, p, bc, cb + threadIdx.x * kPPT, vec, f);
             const ChunkBox box = chunk_box(f, p, kDepthTile, red);
             if (box.frags == 0) continue;
             const int area = box.w * box.h;
@@ -378,15 +372,19 @@ __global__ __launch_bounds__(kDrawThreads) void k_depth(CloudDev c, PassParams p
 #pragma unroll
             for (int j = 0; j < kPPT; j++) {
                 if (f.px[j] < 0) continue;
-                const uint32_t d = depth_to_uint(f.z[j]);
+                const uint32_t d = depth_to_uint_hw(f.z[j]);
+                if (kR0) {
+                    if (box.lds) atomicMax(lds + box.local(f.px[j], f.py[j]), d);
+                    else atomicMax(depth + (size_t)f.py[j] * p.width + f.px[j], d);
+                    continue;
+                }
                 const int r = f.r[j];
                 for (int oy = -r; oy <= r; oy++)
                     for (int ox = -r; ox <= r; ox++) {
                         if (!inside_footprint(ox, oy, r)) continue;
                         const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                         if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                        int k;
-                        if (box.lds && box.local(tx, ty, k)) atomicMax(lds + k, d);
+                        if (box.lds) atomicMax(lds + box.local(tx, ty), d);
                         else atomicMax(depth + (size_t)ty * p.width + tx, d);
                     }
             }
@@ -410,6 +408,7 @@ __global__ __launch_bounds__(kDrawThreads) void k_depth(CloudDev c, PassParams p
 
 constexpr int kColorTile = 2048;  // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 40 KB
 
+template <bool kR0>
 __global__ __launch_bounds__(kDrawThreads) void k_color(CloudDev c, PassParams p, const uint32_t* __restrict__ depth,
                                                         uint32_t* red, uint32_t* green, uint32_t* blue,
                                                         uint32_t* count) {
@@ -424,7 +423,7 @@ __global__ __launch_bounds__(kDrawThreads) void k_color(CloudDev c, PassParams p
         for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
             const uint32_t li0 = cb + threadIdx.x * kPPT;
             ChunkFrags f;
-            chunk_front<false>(c, p, bc, li0, vec, f);
+            chunk_front<false, kR0>(c, p, bc, li0, vec, f);
             // colour stage (ColorPass/Shaders.metal:95-133): packed r | g << 8 | b << 16
             uint32_t rgb[kPPT];
             const uint32_t base = bc.first + li0;
@@ -480,19 +479,18 @@ __global__ __launch_bounds__(kDrawThreads) void k_color(CloudDev c, PassParams p
                 const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
                 const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
                 const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
-                const int rad = f.r[j];
+                const int rad = kR0 ? 0 : f.r[j];
                 for (int oy = -rad; oy <= rad; oy++)
                     for (int ox = -rad; ox <= rad; ox++) {
-                        if (!inside_footprint(ox, oy, rad)) continue;
+                        if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
                         const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                        if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                        int k;
-                        const bool loc = box.lds && box.local(tx, ty, k);
+                        if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
                         const size_t pix = (size_t)ty * p.width + tx;
-                        const uint32_t d = loc ? ldsDepth[k] : depth[pix];
+                        const int k = box.lds ? box.local(tx, ty) : 0;
+                        const uint32_t d = box.lds ? ldsDepth[k] : depth[pix];
                         if (d == 0u) continue;
                         if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                        if (loc) {
+                        if (box.lds) {
                             atomicAdd(ldsRG + k, rg);
                             atomicAdd(ldsBC + k, bn);
                         } else {
@@ -555,6 +553,7 @@ __global__ __launch_bounds__(256) void k_resolve(PassParams p, const uint32_t* _
 
 constexpr int kNearestTile = 2048;  // u64 per pixel: 16 KB
 
+template <bool kR0>
 __global__ __launch_bounds__(kDrawThreads) void k_nearest(CloudDev c, PassParams p,
                                                           unsigned long long* __restrict__ packed) {
     __shared__ unsigned long long lds[kNearestTile];
@@ -566,7 +565,7 @@ __global__ __launch_bounds__(kDrawThreads) void k_nearest(CloudDev c, PassParams
         for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
             const uint32_t li0 = cb + threadIdx.x * kPPT;
             ChunkFrags f;
-            chunk_front<true>(c, p, bc, li0, vec, f);
+            chunk_front<true, kR0>(c, p, bc, li0, vec, f);
             const ChunkBox box = chunk_box(f, p, kNearestTile, red);
             if (box.frags == 0) continue;
             const int area = box.w * box.h;
@@ -581,15 +580,14 @@ __global__ __launch_bounds__(kDrawThreads) void k_nearest(CloudDev c, PassParams
                 // max of depth << 32 | ~index == the two-pass rule: deepest
                 // reverse-Z depth, then the lowest point index among equal depths
                 const unsigned long long key =
-                    ((unsigned long long)depth_to_uint(f.z[j]) << 32) | (unsigned long long)(~pi);
-                const int r = f.r[j];
+                    ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~pi);
+                const int r = kR0 ? 0 : f.r[j];
                 for (int oy = -r; oy <= r; oy++)
                     for (int ox = -r; ox <= r; ox++) {
-                        if (!inside_footprint(ox, oy, r)) continue;
+                        if (!kR0 && !inside_footprint(ox, oy, r)) continue;
                         const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                        if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                        int k;
-                        if (box.lds && box.local(tx, ty, k)) atomicMax(lds + k, key);
+                        if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
+                        if (box.lds) atomicMax(lds + box.local(tx, ty), key);
                         else atomicMax(packed + (size_t)ty * p.width + tx, key);
                     }
             }
@@ -674,14 +672,21 @@ hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p) {
 }
 
 hipError_t launch_depth(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, uint32_t* depth) {
-    hipLaunchKernelGGL(k_depth, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, depth);
+    if (p.constRadius == 0)
+        hipLaunchKernelGGL(k_depth<true>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, depth);
+    else
+        hipLaunchKernelGGL(k_depth<false>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, depth);
     return hipGetLastError();
 }
 
 hipError_t launch_color(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, const uint32_t* depth,
                         uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count) {
-    hipLaunchKernelGGL(k_color, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, depth, red,
-                       green, blue, count);
+    if (p.constRadius == 0)
+        hipLaunchKernelGGL(k_color<true>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, depth,
+                           red, green, blue, count);
+    else
+        hipLaunchKernelGGL(k_color<false>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, depth,
+                           red, green, blue, count);
     return hipGetLastError();
 }
 
@@ -695,7 +700,10 @@ hipError_t launch_resolve(hipStream_t s, const PassParams& p, const uint32_t* pl
 
 hipError_t launch_nearest(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p,
                           unsigned long long* packed) {
-    hipLaunchKernelGGL(k_nearest, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, packed);
+    if (p.constRadius == 0)
+        hipLaunchKernelGGL(k_nearest<true>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, packed);
+    else
+        hipLaunchKernelGGL(k_nearest<false>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kDrawThreads), 0, s, c, p, packed);
     return hipGetLastError();
 }
 

@@ -103,6 +103,109 @@ def read_traffic(kernel_tag: str):
     return None
 
 
+def cloud_bounds(scr, cloud):
+    b = cloud.readPool(scr.POOL_BATCHES)
+    b = b[b["state"] != 0]
+    lo = np.array([b["minX"].min(), b["minY"].min(), b["minZ"].min()], np.float64)
+    hi = np.array([b["maxX"].max(), b["maxY"].max(), b["maxZ"].max()], np.float64)
+    return (lo + hi) * 0.5, float(np.linalg.norm(hi - lo))
+
+
+MV_SLOTS = (1_000_000, 5_000_000, 20_000_000, 50_000_000)
+
+
+def multiview_bench(scr, ctx, steps, warmup, n_views=64):
+    """64 cameras over 4 resident clouds (1M/5M/20M/50M points of the 2 km
+    scene, seeds 0x5C0003..6); view v renders cloud v mod 4 from an eye
+    uniform on the sphere of 1.5x the cloud's diagonal, at a resolution drawn
+    from {256x256, 512x384, 1024x768, 1920x1080}; screen-space points 1-3 px.
+    Batched (one launch per phase for all views, graph replay, autotuned
+    shapes) against one scr_render per view on the same GPU."""
+    import ctypes as C
+    clouds, centres, diags = [], [], []
+    packer = scr.GPUPacker(ctx, keyBits=63)
+    for i, n in enumerate(MV_SLOTS):
+        pos, col = scr.gen_scene(ctx, n, on_device=True, seed=0x5C0003 + i, halfExtent=1000.0, numBoxes=2000)
+        c = scr.ComputeRasteriserPointCloud(ctx, 1, scr.DEFAULT_POINTS_PER_BATCH)
+        packer.packDevice(c, pos, col, n)
+        ctx.sync()
+        pos.free()
+        col.free()
+        ctr, d = cloud_bounds(scr, c)
+        clouds.append(c)
+        centres.append(ctr)
+        diags.append(d)
+    cams, sizes, which = [], [], []
+    for v in range(n_views):
+        k = v % len(clouds)
+        cc, ss = scr.sphere_views(centres[k], 1.5 * diags[k], 1, 0x5C0004 * 1000 + v)
+        cams += cc
+        sizes += ss
+        which.append(k)
+    cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1.0, maximumPointSize=3.0, pointSizeScale=1500.0)
+    pixels = sum(w * h for w, h in sizes)
+
+    # one launch sequence per view: scr_render with prepared arguments
+    frames = [scr.Frame(ctx, w, h) for w, h in sizes]
+    ccfg = cfg.to_c()
+    ccams = [cam.to_c() for cam in cams]
+    handles = [(C.c_void_p * 1)(clouds[k].handle.value) for k in which]
+    lib = ctx.lib
+
+    def per_view_pass():
+        for v in range(n_views):
+            ctx.check(lib.scr_render(ctx.handle, C.byref(ccfg), C.byref(ccams[v]), handles[v], 1, frames[v].handle))
+
+    processed = 0
+    for v in range(n_views):   # visible work of every view (untimed)
+        ctx.check(lib.scr_render(ctx.handle, C.byref(ccfg), C.byref(ccams[v]), handles[v], 1, frames[v].handle))
+        processed += int(clouds[which[v]].readVisible()["activePoints"].astype(np.int64).sum())
+    for _ in range(warmup):
+        per_view_pass()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        per_view_pass()
+    ctx.sync()
+    ms_per_view_path = (time.perf_counter() - t0) * 1e3 / steps
+    want = [f.read(scr.PLANE_RGBA8) for f in frames]
+    want_count = [f.read(scr.PLANE_COUNT) for f in frames]
+    for f in frames:
+        f.close()
+
+    mv = scr.MultiViewRasteriser(ctx, clouds, sizes, which, cfg)
+    t0 = time.perf_counter()
+    tuning = mv.autotune(cams, reps=2)
+    tune_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        mv.render(cams)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mv.render(cams)
+    ctx.sync()
+    ms_batched = (time.perf_counter() - t0) * 1e3 / steps
+    exact = all(np.array_equal(mv.frames[v].read(scr.PLANE_RGBA8), want[v]) and
+                np.array_equal(mv.frames[v].read(scr.PLANE_COUNT), want_count[v]) for v in range(n_views))
+    ctx.profile(True)
+    mv.render(cams, useGraph=False)
+    prof = {k: round(v[0] / v[1], 4) for k, v in ctx.collect_profile().items()}
+    ctx.profile(False)
+    mv.close()
+    for c in clouds:
+        c.close()
+    return {
+        "views": n_views, "pixels_per_launch": pixels, "processed_points_per_launch": processed,
+        "views_per_s_batched": round(n_views / (ms_batched * 1e-3), 1),
+        "views_per_s_per_view_launches": round(n_views / (ms_per_view_path * 1e-3), 1),
+        "speedup": round(ms_per_view_path / ms_batched, 3),
+        "ms_batched": round(ms_batched, 4), "ms_per_view_launches": round(ms_per_view_path, 4),
+        "Gpts_per_s_batched": round(processed / (ms_batched * 1e-3) / 1e9, 3),
+        "bit_exact_vs_per_view": bool(exact), "tuning": tuning, "autotune_s": round(tune_s, 2),
+        "pass_ms_eager": prof,
+    }
+
+
 def cpu_baseline(scr, cloud, cfg, cam, width, height, vis, budget_s=12.0):
     """Oracle on a bounded, evenly spaced sample of the visible batches."""
     sys.path.insert(0, str(REPO))
@@ -144,6 +247,7 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (the known-size copy kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -223,6 +327,10 @@ def main() -> int:
         _, proc_s = algorithmic_bytes(cloud.readVisible())
         extra["flythrough_frame0_clod_on"] = {"ms_per_frame": round(ms_s, 4), "processed_points": proc_s}
         ras.configuration = cfg
+        if not args.no_multiview:
+            log("batched multi-view (64 views, 4 clouds)")
+            extra["multiview_64"] = multiview_bench(scr, ctx, max(3, args.steps // 4), 1)
+            log(f"multi-view {extra['multiview_64']}")
 
     cpu = None
     if not args.no_cpu_baseline:

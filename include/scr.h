@@ -121,6 +121,9 @@ typedef struct scr_frame scr_frame;
 
 /* ---- context */
 int scr_ctx_create(int device, scr_ctx** out);
+/* Releases the caller's reference: the context (its stream and scratch) is
+ * freed once every cloud, frame and multi-view renderer made on it has been
+ * destroyed as well, so objects may be destroyed in any order. */
 int scr_ctx_destroy(scr_ctx* ctx);
 const char* scr_last_error(const scr_ctx* ctx);
 int scr_sync(scr_ctx* ctx);
@@ -284,6 +287,50 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam,
 int scr_pick(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud,
              scr_frame* frame, float ndcX, float ndcY, int searchRadius,
              uint32_t* outIndex, int* found);
+
+/* ---- batched multi-view rendering.
+ * The reference renders one camera per encode (ComputeRasteriser.update
+ * :201-268 + encode :277-296); a caller with many cameras repeats that call.
+ * A multi-view renderer does the same work for nViews cameras at once: view v
+ * renders cloud clouds[viewCloud[v]] into frames[v], exactly as
+ *     scr_render(ctx, cfg, &cameras[v], &clouds[viewCloud[v]], 1, frames[v])
+ * would (bit-identical planes), but every phase is one launch for all views:
+ * clear, cull over every (view, slot) pair, then persistent draw kernels that
+ * take (visible batch, view) work items from a device queue, then resolve.
+ * The launch sequence can be replayed from a captured HIP graph.  Frames must
+ * be distinct; clouds may be shared by several views. */
+typedef struct scr_multiview scr_multiview;
+
+/* Draw-kernel shape for one phase: threads per workgroup (128, 256 or 512),
+ * points per thread (4, 8 or 16; threads * points <= 4096), and the number of
+ * work items each visible batch is split into (1, 2 or 4). */
+typedef struct scr_mv_shape {
+    int32_t threads;
+    int32_t pointsPerThread;
+    int32_t splits;
+    int32_t reserved;
+} scr_mv_shape;
+
+typedef struct scr_mv_tuning {
+    scr_mv_shape depth;
+    scr_mv_shape color;
+    scr_mv_shape nearest;
+} scr_mv_tuning;
+
+int scr_multiview_create(scr_ctx* ctx, scr_cloud* const* clouds, int nClouds, scr_frame* const* frames,
+                         const int32_t* viewCloud, int nViews, scr_multiview** out);
+int scr_multiview_destroy(scr_multiview* mv);
+/* Renders all views.  useGraph != 0 replays a HIP graph of the launch
+ * sequence (captured on first use and again whenever the config's mode or
+ * footprint class, the tuning or the clouds' capacities change). */
+int scr_multiview_render(scr_multiview* mv, const scr_config* cfg, const scr_camera* cameras, int useGraph);
+int scr_multiview_get_tuning(const scr_multiview* mv, scr_mv_tuning* out);
+int scr_multiview_set_tuning(scr_multiview* mv, const scr_mv_tuning* tuning);
+/* Times every supported shape of each draw phase on these cameras (reps
+ * renders each, best time kept), installs the fastest per phase and returns
+ * it.  Results do not depend on the shape, only the time does. */
+int scr_multiview_autotune(scr_multiview* mv, const scr_config* cfg, const scr_camera* cameras, int reps,
+                           scr_mv_tuning* out);
 
 #ifdef __cplusplus
 }

@@ -14,9 +14,11 @@ row-major math notation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import dataclasses
 import math
+import weakref
 from pathlib import Path
 from typing import Optional, Sequence
 
@@ -122,6 +124,15 @@ class CPassTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("totalMs", C.c_double), ("launches", C.c_int32)]
 
 
+class CMVShape(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("pointsPerThread", C.c_int32), ("splits", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class CMVTuning(C.Structure):
+    _fields_ = [("depth", CMVShape), ("color", CMVShape), ("nearest", CMVShape)]
+
+
 _VP = C.c_void_p
 _I = C.c_int
 _SZ = C.c_size_t
@@ -162,6 +173,12 @@ _SIGNATURES = {
     "scr_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _I, _VP]),
     "scr_pick": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, C.c_float, C.c_float, _I,
                       C.POINTER(C.c_uint32), C.POINTER(_I)]),
+    "scr_multiview_create": (_I, [_VP, _VP, _I, _VP, _VP, _I, C.POINTER(_VP)]),
+    "scr_multiview_destroy": (_I, [_VP]),
+    "scr_multiview_render": (_I, [_VP, C.POINTER(CConfig), _VP, _I]),
+    "scr_multiview_get_tuning": (_I, [_VP, C.POINTER(CMVTuning)]),
+    "scr_multiview_set_tuning": (_I, [_VP, C.POINTER(CMVTuning)]),
+    "scr_multiview_autotune": (_I, [_VP, C.POINTER(CConfig), _VP, _I, C.POINTER(CMVTuning)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -341,6 +358,20 @@ def set_lod_cumulative_counts(batch, counts) -> None:
 
 
 # ---------------------------------------------------------------- runtime objects
+_LIVE_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts() -> None:
+    """Release every context's objects while the interpreter and the HIP
+    runtime are both still intact (renderers first, then frames and clouds)."""
+    for ctx in list(_LIVE_CONTEXTS):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU, one HIP stream (scr_ctx)."""
 
@@ -351,6 +382,12 @@ class Context:
         if rc != 0:
             raise ScrError(rc, "scr_ctx_create failed (no usable GPU?)")
         self.handle = h
+        # objects holding device resources of this context; closed before it
+        self._owned = weakref.WeakSet()
+        _LIVE_CONTEXTS.add(self)
+
+    def _adopt(self, obj) -> None:
+        self._owned.add(obj)
 
     def check(self, rc: int) -> None:
         if rc != 0:
@@ -372,6 +409,10 @@ class Context:
 
     def close(self) -> None:
         if getattr(self, "handle", None):
+            # multi-view renderers refer to frames and clouds: release them first
+            owned = sorted(list(getattr(self, "_owned", ())), key=lambda o: getattr(o, "_close_order", 1))
+            for o in owned:
+                o.close()
             self.lib.scr_ctx_destroy(self.handle)
             self.handle = None
 
@@ -391,6 +432,10 @@ class DeviceBuffer:
         p = C.c_void_p()
         ctx.check(ctx.lib.scr_device_alloc(ctx.handle, self.nbytes, C.byref(p)))
         self.ptr = p.value
+        ctx._adopt(self)
+
+    def close(self) -> None:
+        self.free()
 
     @classmethod
     def from_array(cls, ctx: Context, a: np.ndarray) -> "DeviceBuffer":
@@ -429,6 +474,7 @@ class ComputeRasteriserPointCloud:
                                             0 if f is None else len(f), C.byref(h)))
         self.handle = h
         self.worldMatrix = IDENTITY.copy()
+        ctx._adopt(self)
 
     @classmethod
     def from_packed(cls, ctx: Context, packed: PackedPointCloud) -> "ComputeRasteriserPointCloud":
@@ -548,6 +594,7 @@ class Frame:
         h = C.c_void_p()
         ctx.check(ctx.lib.scr_frame_create(ctx.handle, self.width, self.height, C.byref(h)))
         self.handle = h
+        ctx._adopt(self)
 
     def read(self, which: int) -> np.ndarray:
         dt = np.float32 if which == PLANE_DEPTH_F32 else np.uint32
@@ -622,6 +669,130 @@ class ComputeRasteriser:
                                              self.frame.handle, float(ndc[0]), float(ndc[1]), int(searchRadius),
                                              C.byref(out), C.byref(found)))
         return int(out.value) if found.value else None
+
+
+# ---------------------------------------------------------------- multi-view
+MV_RESOLUTIONS = ((256, 256), (512, 384), (1024, 768), (1920, 1080))
+
+
+def sphere_views(centre, radius: float, count: int, seed: int, resolutions=MV_RESOLUTIONS,
+                 fov: float = 60.0, near_frac: float = 0.01, far_frac: float = 4.0, up=(0.0, 0.0, 1.0)):
+    """``count`` cameras with eyes uniform on the sphere of ``radius`` around
+    ``centre`` looking at it, each with a resolution drawn uniformly from
+    ``resolutions``.  Returns (cameras, sizes).  Deterministic in ``seed``."""
+    rng = np.random.default_rng(seed)
+    centre = np.asarray(centre, np.float64)
+    cams, sizes = [], []
+    for _ in range(count):
+        w, h = resolutions[int(rng.integers(len(resolutions)))]
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = centre + radius * d
+        u = np.asarray(up, np.float64)
+        if abs(float(d @ u)) > 0.99:       # looking along the up axis
+            u = np.array([0.0, 1.0, 0.0]) if abs(u[1]) < 0.5 else np.array([1.0, 0.0, 0.0])
+        cams.append(Camera.perspective(tuple(eye), tuple(centre), tuple(u), fov=fov, aspect=w / h,
+                                       near=radius * near_frac, far=radius * far_frac))
+        sizes.append((w, h))
+    return cams, sizes
+
+
+MV_SHAPES = ((128, 8), (256, 4), (256, 8), (256, 16), (512, 4), (512, 8))
+MV_SPLITS = (1, 2, 4)
+
+
+def _tuning_to_dict(t: CMVTuning) -> dict:
+    return {ph: {"threads": getattr(t, ph).threads, "pointsPerThread": getattr(t, ph).pointsPerThread,
+                 "splits": getattr(t, ph).splits} for ph in ("depth", "color", "nearest")}
+
+
+class MultiViewRasteriser:
+    """Renders many cameras in one batched launch per phase (scr_multiview_*).
+
+    View v renders ``clouds[viewClouds[v]]`` into its own frame of size
+    ``sizes[v]``; the planes are those ComputeRasteriser.render would give
+    that camera alone (order-independent integer accumulation)."""
+
+    def __init__(self, ctx: Context, clouds, sizes, viewClouds,
+                 configuration: Optional[ComputeRasteriserConfiguration] = None):
+        self.ctx = ctx
+        self.clouds = list(clouds)
+        self.frames = [Frame(ctx, w, h) for (w, h) in sizes]
+        self.viewClouds = [int(i) for i in viewClouds]
+        if len(self.viewClouds) != len(self.frames):
+            raise ValueError("one cloud index per view")
+        self.configuration = configuration or ComputeRasteriserConfiguration()
+        ch = (C.c_void_p * len(self.clouds))(*[c.handle.value for c in self.clouds])
+        fh = (C.c_void_p * len(self.frames))(*[f.handle.value for f in self.frames])
+        vc = np.ascontiguousarray(self.viewClouds, np.int32)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.scr_multiview_create(ctx.handle, ch, len(self.clouds), fh, vc.ctypes.data,
+                                               len(self.frames), C.byref(h)))
+        self.handle = h
+        ctx._adopt(self)
+
+    _close_order = 0
+
+    @property
+    def nViews(self) -> int:
+        return len(self.frames)
+
+    def _cams(self, cameras):
+        if len(cameras) != self.nViews:
+            raise ValueError("one camera per view")
+        for cl in self.clouds:
+            m = np.ascontiguousarray(cl.worldMatrix, np.float32)
+            self.ctx.check(self.ctx.lib.scr_cloud_set_model(cl.handle, m.ctypes.data))
+        arr = (CCamera * self.nViews)()
+        for i, cam in enumerate(cameras):
+            arr[i] = cam.to_c()
+        return arr
+
+    def render(self, cameras, useGraph: bool = True) -> None:
+        cams = self._cams(cameras)
+        cfg = self.configuration.to_c()
+        self.ctx.check(self.ctx.lib.scr_multiview_render(self.handle, C.byref(cfg), cams, 1 if useGraph else 0))
+
+    def autotune(self, cameras, reps: int = 2) -> dict:
+        cams = self._cams(cameras)
+        cfg = self.configuration.to_c()
+        t = CMVTuning()
+        self.ctx.check(self.ctx.lib.scr_multiview_autotune(self.handle, C.byref(cfg), cams, int(reps), C.byref(t)))
+        return _tuning_to_dict(t)
+
+    @property
+    def tuning(self) -> dict:
+        t = CMVTuning()
+        self.ctx.check(self.ctx.lib.scr_multiview_get_tuning(self.handle, C.byref(t)))
+        return _tuning_to_dict(t)
+
+    @tuning.setter
+    def tuning(self, d: dict) -> None:
+        t = CMVTuning()
+        self.ctx.check(self.ctx.lib.scr_multiview_get_tuning(self.handle, C.byref(t)))
+        for ph, sh in d.items():
+            dst = getattr(t, ph)
+            dst.threads = int(sh.get("threads", dst.threads))
+            dst.pointsPerThread = int(sh.get("pointsPerThread", dst.pointsPerThread))
+            dst.splits = int(sh.get("splits", dst.splits))
+        self.ctx.check(self.ctx.lib.scr_multiview_set_tuning(self.handle, C.byref(t)))
+
+    def read(self, view: int, which: int) -> np.ndarray:
+        f = self.frames[view]
+        return f.read(which).reshape(f.height, f.width)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.ctx.lib.scr_multiview_destroy(self.handle)
+            self.handle = None
+        for f in getattr(self, "frames", []):
+            f.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- packer

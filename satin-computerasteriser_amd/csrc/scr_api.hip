@@ -21,6 +21,22 @@ int hip_error(scr_ctx* ctx, hipError_t e, const char* what) {
     return set_error(ctx, e == hipErrorOutOfMemory ? SCR_ERR_OUT_OF_MEMORY : SCR_ERR_HIP, buf);
 }
 
+void ctx_retain(scr_ctx* ctx) { ctx->refs++; }
+
+void ctx_release(scr_ctx* ctx) {
+    if (--ctx->refs > 0) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& iv : ctx->intervals) {
+        (void)hipEventDestroy(iv.begin);
+        (void)hipEventDestroy(iv.end);
+    }
+    for (auto e : ctx->eventPool) (void)hipEventDestroy(e);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
 hipEvent_t take_event(scr_ctx* ctx) {
     if (!ctx->eventPool.empty()) {
         hipEvent_t e = ctx->eventPool.back();
@@ -56,6 +72,87 @@ using scr::set_error;
         if (e_ != hipSuccess) return hip_error(ctx, e_, #call); \
     } while (0)
 
+namespace scr {
+
+// flushBatchMirror — ComputeRasteriserPointCloud.swift:629-644 (dirty range only)
+int flush_mirror(scr_cloud* c) {
+    if (c->dirtyLo > c->dirtyHi) return SCR_OK;
+    const int lo = std::max(0, c->dirtyLo);
+    const int hi = std::min(c->dirtyHi, c->maxResidentBatches - 1);
+    c->dirtyLo = 0x7fffffff;
+    c->dirtyHi = -1;
+    if (lo > hi) return SCR_OK;
+    SCR_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
+    SCR_HIP(c->ctx, hipMemcpy(c->dBatches + lo, c->mirror.data() + lo, (size_t)(hi - lo + 1) * sizeof(scr_raster_batch),
+                              hipMemcpyHostToDevice));
+    return SCR_OK;
+}
+
+// updateFiles composition (ComputeRasteriserPointCloud.swift:487-515):
+// world = model·file.world, transform = VP·world, transformFrustum =
+// (frustum ?? VP)·world, prevTransform = (prevVP ?? VP)·world.
+void compose_files(const scr_cloud* c, const float* model, const float* vp, const float* frustum,
+                   const float* prevVP, scr_raster_file* out) {
+    for (size_t i = 0; i < c->files.size(); i++) {
+        scr_raster_file f;
+        float world[16];
+        mat_mul(model, c->files[i].world, world);
+        mat_mul(vp, world, f.transform);
+        mat_mul(frustum ? frustum : vp, world, f.transformFrustum);
+        std::memcpy(f.world, world, sizeof world);
+        mat_mul(prevVP ? prevVP : vp, world, f.prevTransform);
+        out[i] = f;
+    }
+}
+
+int check_config(scr_ctx* ctx, const scr_config* cfg) {
+    if (!cfg) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "config is NULL");
+    if (cfg->mode != SCR_MODE_HIGH_QUALITY_AVERAGE && cfg->mode != SCR_MODE_NEAREST_POINT)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown render mode");
+    if (cfg->tintAlphaIsCoverage != 0 && cfg->applyTint != 0)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "translucent-defocus (tintAlphaIsCoverage) is not supported");
+    if (cfg->motionBlur > 0.0f) return set_error(ctx, SCR_ERR_UNSUPPORTED, "motion blur is not supported");
+    if (cfg->pointSizeMode != 0 && cfg->pointSizeMode != 1)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown point size mode");
+    return SCR_OK;
+}
+
+PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, int height) {
+    PassParams p;
+    std::memset(&p, 0, sizeof p);
+    std::memcpy(p.view, cam->viewMatrix, sizeof p.view);
+    std::memcpy(p.proj, cam->projectionMatrix, sizeof p.proj);
+    p.width = width;
+    p.height = height;
+    p.pointSizeMode = cfg->pointSizeMode;
+    p.minPointSize = cfg->minimumPointSize;
+    p.maxPointSize = cfg->maximumPointSize;
+    p.pointSizeScale = cfg->pointSizeScale;
+    p.lodDither = cfg->enableLODDither;
+    p.enableFrustumCulling = cfg->enableFrustumCulling;
+    p.lodBias = cfg->lodBias;
+    p.enableCLOD = cfg->enableCLOD;
+    p.depthKeep = 1.0f - cfg->depthTolerance;
+    p.colorizeChunks = cfg->colorizeChunks;
+    p.colorizeOverdraw = cfg->colorizeOverdraw;
+    p.applyTint = cfg->applyTint;
+    const float mn = cfg->minimumPointSize, mx = cfg->maximumPointSize;
+    p.lo = std::fmax(std::fmin(mn, mx), 1.0f);
+    p.hi = std::fmax(std::fmax(mn, mx), p.lo);
+    p.constRadius = (p.lo == p.hi) ? const_radius_for(p.lo) : -1;
+    p.maxRadius = const_radius_for(p.hi);
+    p.ortho = cam->projectionMatrix[15] == 1.0f;
+    p.focal = ((float)height * 0.5f) * cam->projectionMatrix[5];
+    for (int i = 0; i < 4; i++) p.bg[i] = cfg->backgroundColor[i];
+    return p;
+}
+
+}  // namespace scr
+
+using scr::check_config;
+using scr::flush_mirror;
+using scr::make_params;
+
 namespace {
 
 const float kIdentity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
@@ -82,8 +179,10 @@ void release_cloud(scr_cloud* c) {
     dfree(c->dCounter);
     if (c->hFiles) (void)hipHostFree(c->hFiles);
     c->hFiles = nullptr;
-    if (c->filesCopied) (void)hipEventDestroy(c->filesCopied);
-    c->filesCopied = nullptr;
+    for (hipEvent_t& e : c->filesCopied) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
 }
 
 void mark_dirty(scr_cloud* c, int slot) {
@@ -91,38 +190,18 @@ void mark_dirty(scr_cloud* c, int slot) {
     c->dirtyHi = std::max(c->dirtyHi, slot);
 }
 
-// flushBatchMirror — ComputeRasteriserPointCloud.swift:629-644 (dirty range only)
-int flush_mirror(scr_cloud* c) {
-    if (c->dirtyLo > c->dirtyHi) return SCR_OK;
-    const int lo = std::max(0, c->dirtyLo);
-    const int hi = std::min(c->dirtyHi, c->maxResidentBatches - 1);
-    c->dirtyLo = 0x7fffffff;
-    c->dirtyHi = -1;
-    if (lo > hi) return SCR_OK;
-    SCR_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
-    SCR_HIP(c->ctx, hipMemcpy(c->dBatches + lo, c->mirror.data() + lo, (size_t)(hi - lo + 1) * sizeof(scr_raster_batch),
-                              hipMemcpyHostToDevice));
-    return SCR_OK;
-}
-
-// updateFiles composition (ComputeRasteriserPointCloud.swift:487-515) and its
-// upload through pinned staging; the event keeps the staging untouched until
-// the previous copy has consumed it.
-int upload_files(scr_cloud* c, const float* vp, const float* model, const float* frustum, const float* prevVP) {
-    SCR_HIP(c->ctx, hipEventSynchronize(c->filesCopied));
-    for (size_t i = 0; i < c->files.size(); i++) {
-        scr_raster_file f;
-        float world[16];
-        scr::mat_mul(model, c->files[i].world, world);
-        scr::mat_mul(vp, world, f.transform);
-        scr::mat_mul(frustum ? frustum : vp, world, f.transformFrustum);
-        std::memcpy(f.world, world, sizeof world);
-        scr::mat_mul(prevVP ? prevVP : vp, world, f.prevTransform);
-        c->hFiles[i] = f;
-    }
-    SCR_HIP(c->ctx, hipMemcpyAsync(c->dFiles, c->hFiles, c->files.size() * sizeof(scr_raster_file),
-                                   hipMemcpyHostToDevice, c->ctx->stream));
-    SCR_HIP(c->ctx, hipEventRecord(c->filesCopied, c->ctx->stream));
+// Upload of the composed table through the pinned ring; the slot's event keeps
+// its staging untouched until the copy kFilesRing uploads ago has consumed it.
+int upload_files(scr_cloud* c, const float* model, const float* vp, const float* frustum, const float* prevVP) {
+    const int k = c->filesNext;
+    c->filesNext = (k + 1) % scr_cloud::kFilesRing;
+    const size_t n = c->files.size();
+    scr_raster_file* stage = c->hFiles + (size_t)k * n;
+    SCR_HIP(c->ctx, hipEventSynchronize(c->filesCopied[k]));
+    scr::compose_files(c, model, vp, frustum, prevVP, stage);
+    SCR_HIP(c->ctx, hipMemcpyAsync(c->dFiles, stage, n * sizeof(scr_raster_file), hipMemcpyHostToDevice,
+                                   c->ctx->stream));
+    SCR_HIP(c->ctx, hipEventRecord(c->filesCopied[k], c->ctx->stream));
     return SCR_OK;
 }
 
@@ -167,48 +246,6 @@ void free_pool(scr_cloud* c) {
     if (c->tintsOwned) dfree(c->dTints);
     c->dTints = nullptr;
     c->tintsOwned = false;
-}
-
-int check_config(scr_ctx* ctx, const scr_config* cfg) {
-    if (!cfg) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "config is NULL");
-    if (cfg->mode != SCR_MODE_HIGH_QUALITY_AVERAGE && cfg->mode != SCR_MODE_NEAREST_POINT)
-        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown render mode");
-    if (cfg->tintAlphaIsCoverage != 0 && cfg->applyTint != 0)
-        return set_error(ctx, SCR_ERR_UNSUPPORTED, "translucent-defocus (tintAlphaIsCoverage) is not supported");
-    if (cfg->motionBlur > 0.0f) return set_error(ctx, SCR_ERR_UNSUPPORTED, "motion blur is not supported");
-    if (cfg->pointSizeMode != 0 && cfg->pointSizeMode != 1)
-        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown point size mode");
-    return SCR_OK;
-}
-
-scr::PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, int height) {
-    scr::PassParams p;
-    std::memset(&p, 0, sizeof p);
-    std::memcpy(p.view, cam->viewMatrix, sizeof p.view);
-    std::memcpy(p.proj, cam->projectionMatrix, sizeof p.proj);
-    p.width = width;
-    p.height = height;
-    p.pointSizeMode = cfg->pointSizeMode;
-    p.minPointSize = cfg->minimumPointSize;
-    p.maxPointSize = cfg->maximumPointSize;
-    p.pointSizeScale = cfg->pointSizeScale;
-    p.lodDither = cfg->enableLODDither;
-    p.enableFrustumCulling = cfg->enableFrustumCulling;
-    p.lodBias = cfg->lodBias;
-    p.enableCLOD = cfg->enableCLOD;
-    p.depthKeep = 1.0f - cfg->depthTolerance;
-    p.colorizeChunks = cfg->colorizeChunks;
-    p.colorizeOverdraw = cfg->colorizeOverdraw;
-    p.applyTint = cfg->applyTint;
-    const float mn = cfg->minimumPointSize, mx = cfg->maximumPointSize;
-    p.lo = std::fmax(std::fmin(mn, mx), 1.0f);
-    p.hi = std::fmax(std::fmax(mn, mx), p.lo);
-    p.constRadius = (p.lo == p.hi) ? scr::const_radius_for(p.lo) : -1;
-    p.maxRadius = scr::const_radius_for(p.hi);
-    p.ortho = cam->projectionMatrix[15] == 1.0f;
-    p.focal = ((float)height * 0.5f) * cam->projectionMatrix[5];
-    for (int i = 0; i < 4; i++) p.bg[i] = cfg->backgroundColor[i];
-    return p;
 }
 
 }  // namespace
@@ -288,16 +325,7 @@ int scr_profile_collect(scr_ctx* ctx, scr_pass_time* out, int capacity, int* cou
 
 int scr_ctx_destroy(scr_ctx* ctx) {
     if (!ctx) return SCR_OK;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto& iv : ctx->intervals) {
-        (void)hipEventDestroy(iv.begin);
-        (void)hipEventDestroy(iv.end);
-    }
-    for (auto e : ctx->eventPool) (void)hipEventDestroy(e);
-    if (ctx->scratch) (void)hipFree(ctx->scratch);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    scr::ctx_release(ctx);
     return SCR_OK;
 }
 
@@ -376,8 +404,10 @@ int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, c
     hipError_t e = alloc_pool(c);
     if (e == hipSuccess) e = dalloc(c->dFiles, c->files.size());
     if (e == hipSuccess) e = dalloc(c->dCounter, 1);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->hFiles, c->files.size() * sizeof(scr_raster_file));
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->filesCopied, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&c->hFiles, scr_cloud::kFilesRing * c->files.size() * sizeof(scr_raster_file));
+    for (int k = 0; k < scr_cloud::kFilesRing && e == hipSuccess; k++)
+        e = hipEventCreateWithFlags(&c->filesCopied[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemsetAsync(c->dCounter, 0, sizeof(uint32_t), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess)
@@ -387,15 +417,18 @@ int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, c
         delete c;
         return hip_error(ctx, e, "scr_cloud_create");
     }
+    scr::ctx_retain(ctx);
     *out = c;
     return SCR_OK;
 }
 
 int scr_cloud_destroy(scr_cloud* c) {
     if (!c) return SCR_OK;
-    (void)hipStreamSynchronize(c->ctx->stream);
+    scr_ctx* ctx = c->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
     release_cloud(c);
     delete c;
+    scr::ctx_release(ctx);
     return SCR_OK;
 }
 
@@ -527,7 +560,7 @@ int scr_cloud_set_model(scr_cloud* c, const float model[16]) {
 int scr_cloud_update_files(scr_cloud* c, const float vp[16], const float model[16], const float* frustum16,
                            const float* prevVP16) {
     if (!c || !vp || !model) return SCR_ERR_INVALID_ARGUMENT;
-    return upload_files(c, vp, model, frustum16, prevVP16);
+    return upload_files(c, model, vp, frustum16, prevVP16);
 }
 
 int scr_cloud_set_tints(scr_cloud* c, const float* rgba, size_t nPoints, int onDevice) {
@@ -760,18 +793,21 @@ int scr_frame_create(scr_ctx* ctx, int width, int height, scr_frame** out) {
         delete f;
         return hip_error(ctx, e, "scr_frame_create");
     }
+    scr::ctx_retain(ctx);
     *out = f;
     return SCR_OK;
 }
 
 int scr_frame_destroy(scr_frame* f) {
     if (!f) return SCR_OK;
-    (void)hipStreamSynchronize(f->ctx->stream);
+    scr_ctx* ctx = f->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
     dfree(f->dPlanes);
     dfree(f->dNearest);
     dfree(f->dRGBA);
     dfree(f->dDepthF);
     delete f;
+    scr::ctx_release(ctx);
     return SCR_OK;
 }
 
@@ -824,7 +860,7 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
         }
         const uint32_t* colors = nullptr;
         if (c) {
-            rc = upload_files(c, vp, c->model, nullptr, nullptr);
+            rc = upload_files(c, c->model, vp, nullptr, nullptr);
             if (rc != SCR_OK) return rc;
             const CloudDev d = c->dev();
             {
@@ -851,7 +887,7 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
     for (int i = 0; i < nClouds; i++)
         if (clouds[i]->residentBatchCount > 0) resident.push_back(clouds[i]);
     for (scr_cloud* c : resident) {
-        rc = upload_files(c, vp, c->model, nullptr, nullptr);
+        rc = upload_files(c, c->model, vp, nullptr, nullptr);
         if (rc != SCR_OK) return rc;
     }
     for (scr_cloud* c : resident) {
@@ -891,7 +927,7 @@ int scr_pick(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_clo
     float vp[16];
     scr::mat_mul(cam->projectionMatrix, cam->viewMatrix, vp);
     hipStream_t s = ctx->stream;
-    rc = upload_files(cloud, vp, cloud->model, nullptr, nullptr);
+    rc = upload_files(cloud, cloud->model, vp, nullptr, nullptr);
     if (rc != SCR_OK) return rc;
     const CloudDev d = cloud->dev();
     SCR_HIP(ctx, scr::launch_fill_u64(s, frame->dNearest, frame->pixels(), 0ull));

@@ -12,6 +12,9 @@
 #include "scr_math.hpp"
 
 struct scr_ctx {
+    // the context lives until scr_ctx_destroy and the destroy call of every
+    // cloud, frame and multi-view renderer made on it (each holds a reference)
+    int refs = 1;
     int device = 0;
     hipStream_t stream = nullptr;
     int numCUs = 0;
@@ -64,8 +67,13 @@ struct scr_cloud {
     uint8_t* dLevels = nullptr;
     scr_raster_batch* dBatches = nullptr;
     scr_raster_file* dFiles = nullptr;
-    scr_raster_file* hFiles = nullptr;  // pinned staging for the composed table
-    hipEvent_t filesCopied = nullptr;
+    // pinned staging for the composed table: a ring of kFilesRing tables (the
+    // reference keeps 6 file buffers in flight, ComputeRasteriserPointCloud.swift:141-146),
+    // so the host composes frame k+6 while earlier copies are still queued
+    static constexpr int kFilesRing = 6;
+    scr_raster_file* hFiles = nullptr;
+    hipEvent_t filesCopied[kFilesRing] = {};
+    int filesNext = 0;
     float4* dTints = nullptr;
     bool tintsOwned = false;
     scr_visible_batch* dVisible = nullptr;
@@ -119,6 +127,18 @@ class PassScope {
 
 int set_error(scr_ctx* ctx, int code, const std::string& msg);
 int hip_error(scr_ctx* ctx, hipError_t e, const char* what);
+void ctx_retain(scr_ctx* ctx);
+void ctx_release(scr_ctx* ctx);   // frees the context with its last reference
+
+// Shared by the single-view and multi-view render paths (scr_api.hip).
+int check_config(scr_ctx* ctx, const scr_config* cfg);
+PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, int height);
+int flush_mirror(scr_cloud* c);
+// updateFiles composition (ComputeRasteriserPointCloud.swift:487-515) of every
+// file of `c` for model matrix `model` and view-projection vp into
+// out[0 .. files.size()).
+void compose_files(const scr_cloud* c, const float* model, const float* vp, const float* frustum,
+                   const float* prevVP, scr_raster_file* out);
 
 // Kernel launchers (scr_render.hip).  All enqueue on ctx->stream.
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);

@@ -49,6 +49,28 @@ def test_cull_visible_set_matches_oracle(ctx, cube64):
     assert vis.tobytes() == ref.tobytes()
 
 
+def test_single_column_chunks_bit_exact(ctx):
+    """Points on a vertical line land in one pixel column, so every chunk's
+    screen box is one pixel wide and goes through the LDS tile (the width-1
+    case of the tile's row/column decode)."""
+    n = 6000
+    y = np.linspace(-0.5, 0.5, n, dtype=np.float32)
+    pos = np.stack([np.zeros(n, np.float32), y, np.sin(y * 7).astype(np.float32) * 0.1], 1)
+    col = np.stack([np.linspace(0, 1, n), np.linspace(1, 0, n), np.full(n, 0.5), np.ones(n)], 1).astype(np.float32)
+    packed = orc.pack(pos, col, pointsPerBatch=4096)
+    cam = scr.Camera.perspective((0.0, 0.0, 3.0), (0.0, 0.0, 0.0), fov=60, aspect=16 / 9)
+    cfg = scr.ComputeRasteriserConfiguration()
+    gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, packed)], cfg, cam, 1280, 720)
+    ref = orc.render_hqs([oracle_cloud(packed)], cfg, cam.viewMatrix, cam.projectionMatrix, 1280, 720)
+    cols = np.nonzero(ref["count"].any(axis=0))[0]
+    assert len(cols) == 1 and ref["count"].sum() > 1000
+    assert_planes_equal(gpu, ref)
+    cfg.mode = scr.MODE_NEAREST_POINT
+    gpu = render_gpu_nearest(ctx, [gpu_cloud(ctx, packed)], cfg, cam, 1280, 720)
+    ref = orc.render_nearest(oracle_cloud(packed), cfg, cam.viewMatrix, cam.projectionMatrix, 1280, 720)
+    assert_planes_equal(gpu, ref)
+
+
 @pytest.fixture(scope="module")
 def jitter():
     pos, col = jittered_cloud(20000, 0x5EED)

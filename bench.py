@@ -206,6 +206,105 @@ def multiview_bench(scr, ctx, steps, warmup, n_views=64):
     }
 
 
+def diff_bench(scr, ctx, steps, fit_steps=200, n_points=5_000_000, n_views=32, size=512):
+    """Differentiable HQS on 5M points of the 2 km scene (seed 0x5C0005,
+    float ground-truth colours), 32 views of 512x512 on a sphere: forward +
+    backward kernel throughput, then an FP32 Adam fit (lr 1e-2) of per-point
+    colours from 0.5 to noisy (sigma 0.02) renders of the ground truth through
+    the torch autograd.Function."""
+    pos, col = scr.gen_scene(ctx, n_points, on_device=False, seed=0x5C0005, halfExtent=1000.0, numBoxes=2000,
+                             quantizeColors=0)
+    cloud = scr.ComputeRasteriserPointCloud(ctx, 1, scr.DEFAULT_POINTS_PER_BATCH)
+    src = scr.GPUPacker(ctx, keyBits=63).pack(cloud, pos, col, sourceIndices=True)
+    batches = cloud.readPool(scr.POOL_BATCHES)
+    npool = cloud.batchCount * cloud.pointsPerBatch
+    gt = np.zeros((npool, 4), np.float32)
+    k = 0
+    for s_ in range(cloud.batchCount):
+        m, first = int(batches["numPoints"][s_]), int(batches["firstPoint"][s_])
+        gt[first:first + m, :3] = col[src[k:k + m], :3]
+        k += m
+    ctr, diag = cloud_bounds(scr, cloud)
+    cams, _ = scr.sphere_views(ctr, 1.2 * diag, n_views, 0x5C0005, resolutions=((size, size),))
+    cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1.0, maximumPointSize=3.0, pointSizeScale=1500.0)
+    r = scr.DifferentiableHQS(ctx, cloud, [(size, size)] * n_views, cfg)
+    del pos, col, src
+
+    # kernel throughput: forward + backward of every view, device-resident buffers
+    params = scr.DeviceBuffer.from_array(ctx, np.full((npool, 4), 0.5, np.float32))
+    img = scr.DeviceBuffer(ctx, size * size * 12)
+    dl = scr.DeviceBuffer.from_array(ctx, np.random.default_rng(5).normal(size=size * size * 3).astype(np.float32))
+    grad = scr.DeviceBuffer(ctx, npool * 32)
+
+    def fwd_bwd():
+        for v, cam in enumerate(cams):
+            r.forward(v, cam, params.ptr, img.ptr)
+            r.backward(v, cam, dl.ptr, grad.ptr)
+
+    processed = 0
+    for v, cam in enumerate(cams):
+        r.forward(v, cam, params.ptr, img.ptr)
+        processed += int(cloud.readVisible()["activePoints"].astype(np.int64).sum())
+    fwd_bwd()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fwd_bwd()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.profile(True)
+    fwd_bwd()
+    prof = {k_: round(v_[0] / v_[1], 4) for k_, v_ in ctx.collect_profile().items()}
+    ctx.profile(False)
+    out = {"points": n_points, "views": n_views, "size": size, "processed_points_per_pass": processed,
+           "ms_fwd_bwd_all_views": round(ms, 4), "Gpts_per_s_fwd_bwd": round(processed / (ms * 1e-3) / 1e9, 3),
+           "pass_ms": prof}
+    params.free(); img.free(); dl.free(); grad.free()
+
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            raise ImportError("torch sees no GPU")
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("scr_torch_diff", PKG_DIR / "torch_diff.py")
+        td = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(td)
+    except ImportError as e:
+        out["fit"] = f"skipped: {e}"
+        r.close(); cloud.close()
+        return out
+    dev = torch.device("cuda:0")
+    render = td.HQSRender(r, cams)
+    with torch.no_grad():
+        gt_t = torch.from_numpy(gt).to(dev)
+        clean = render(gt_t)
+        gen = torch.Generator(device=dev).manual_seed(0x5C0005)
+        target = clean + 0.02 * torch.randn(clean.shape, generator=gen, device=dev)
+    theta = torch.full((npool, 4), 0.5, dtype=torch.float32, device=dev, requires_grad=True)
+    opt = torch.optim.Adam([theta], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    losses = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for it in range(fit_steps):
+        opt.zero_grad()
+        loss = ((render(theta) - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        losses.append(loss.item())
+    torch.cuda.synchronize()
+    fit_s = time.perf_counter() - t0
+    with torch.no_grad():
+        seen = (clean.sum(dim=-1) > 0)
+        rmse_img = float(((render(theta) - clean) ** 2).mean().sqrt())
+    out["fit"] = {"steps": fit_steps, "lr": 1e-2, "loss_first": losses[0], "loss_last": losses[-1],
+                  "loss_at": {str(i): losses[i] for i in (0, 10, 50, 100, 150) if i < len(losses)},
+                  "noise_floor_sigma2": 0.02 ** 2, "image_rmse_vs_clean": rmse_img,
+                  "ms_per_step": round(fit_s * 1e3 / fit_steps, 3), "covered_pixel_frac": float(seen.float().mean())}
+    r.close()
+    cloud.close()
+    return out
+
+
 def cpu_baseline(scr, cloud, cfg, cam, width, height, vis, budget_s=12.0):
     """Oracle on a bounded, evenly spaced sample of the visible batches."""
     sys.path.insert(0, str(REPO))
@@ -248,6 +347,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-mode measurement and fit")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (the known-size copy kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -260,6 +360,13 @@ def main() -> int:
         print("bench.py runs as a single process on one GPU", file=sys.stderr)
         return 2
 
+    if not (args.no_extra or args.no_diff):
+        # torch (for the fit) must own the process's HIP runtime: import it
+        # before the library creates its context
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     scr = load_pkg()
     ctx = scr.Context(0)
     W, H, N = args.width, args.height, args.points
@@ -327,6 +434,10 @@ def main() -> int:
         _, proc_s = algorithmic_bytes(cloud.readVisible())
         extra["flythrough_frame0_clod_on"] = {"ms_per_frame": round(ms_s, 4), "processed_points": proc_s}
         ras.configuration = cfg
+        if not args.no_diff:
+            log("differentiable HQS (5M points, 32 views 512x512) and Adam fit")
+            extra["diff_hqs"] = diff_bench(scr, ctx, max(2, args.steps // 5))
+            log(f"diff {extra['diff_hqs']}")
         if not args.no_multiview:
             log("batched multi-view (64 views, 4 clouds)")
             extra["multiview_64"] = multiview_bench(scr, ctx, max(3, args.steps // 4), 1)

@@ -269,7 +269,11 @@ int scr_frame_destroy(scr_frame* frame);
 #define SCR_PLANE_NEAREST_INDEX 6  /* u32, 0xffffffff = none */
 #define SCR_PLANE_RGBA8 7          /* u32 = r | g<<8 | b<<16 | a<<24 (rgba8Unorm texel) */
 #define SCR_PLANE_DEPTH_F32 8      /* float reversed-Z depth texture */
-/* Copy one plane (width*height elements of 4 B) to host memory. */
+#define SCR_PLANE_DIFF_SUM_R 9     /* int64 per pixel: differentiable-mode fixed-point sums (x 2^16) */
+#define SCR_PLANE_DIFF_SUM_G 10
+#define SCR_PLANE_DIFF_SUM_B 11
+/* Copy one plane (width*height elements of 4 B; 8 B for the DIFF_SUM planes)
+ * to host memory. */
 int scr_frame_read(scr_frame* frame, int which, void* host, size_t bytes);
 
 /* ---- rendering: ComputeRasteriser.update (:201-268) + encode (:277-296).
@@ -287,6 +291,32 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam,
 int scr_pick(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud,
              scr_frame* frame, float ndcX, float ndcY, int searchRadius,
              uint32_t* outIndex, int* found);
+
+/* ---- differentiable HQS.
+ * No reference counterpart: it extends the HQS passes (encodeHighQualityAverage,
+ * ComputeRasteriser.swift:322-456) for fitting per-point colours.  Visibility
+ * is exactly HQS's (cull, depth pass, the colour pass's depth-tolerance test,
+ * ColorPass/Shaders.metal:201-209), so the frame's DEPTH and COUNT planes equal
+ * scr_render's.  The colour of pool point p is params[4p .. 4p+2] (device
+ * float4 per pool point, .w unused), accumulated per pixel as int64
+ * rint(param * 2^16) (SCR_PLANE_DIFF_SUM_*), and
+ *     image[3 * pixel + k] = (float)(sum_k / (2^16 * count))   (FP64 divide)
+ * or backgroundColor[k] where count == 0; image is a device pointer to
+ * 3 * width * height floats, row 0 = top of the viewport.  Integer sums make
+ * the forward deterministic.  mode must be HQS; applyTint / colorize* must be
+ * 0 (SCR_ERR_UNSUPPORTED otherwise). */
+int scr_diff_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud,
+                    const float* params, scr_frame* frame, float* image);
+/* Backward of scr_diff_render for the same cfg / cam / cloud / frame (the
+ * frame's depth and count planes from that forward are read, each point's
+ * fragments are recomputed; nothing per fragment is stored).  For every
+ * point p and channel k:
+ *     grad[4p + k] += Σ over p's kept fragments of dLdImage[3 * pixel + k] / count[pixel]
+ * in FP64 (the fixed-point rounding passes the gradient straight through).
+ * dLdImage: device, 3 floats per pixel; grad: device, 4 doubles per pool
+ * point, accumulated (views add up in call order; no atomics). */
+int scr_diff_backward(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud,
+                      scr_frame* frame, const float* dLdImage, double* grad);
 
 /* ---- batched multi-view rendering.
  * The reference renders one camera per encode (ComputeRasteriser.update

@@ -89,6 +89,12 @@ def lib() -> C.CDLL:
             "orc_morton_keys": (None, [vp, C.c_int, C.c_size_t, C.c_int, vp]),
             "orc_lod_levels": (None, [vp, C.c_size_t, vp, vp, C.c_int, C.c_int, vp]),
             "orc_cube_grid": (None, [C.c_int, f, vp, vp]),
+            "orc_diff_fixed": (C.c_int64, [f]),
+            "orc_diff_pixel": (f, [C.c_int64, u32, f]),
+            "orc_render_diff": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp, vp,
+                                          C.c_int]),
+            "orc_diff_backward": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp,
+                                            vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -346,6 +352,46 @@ def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int) -> di
                              height, d.ctypes.data, idx.ctypes.data, rgba.ctypes.data, df.ctypes.data)
     return {"depth": d.reshape(height, width), "index": idx.reshape(height, width),
             "rgba8": rgba.reshape(height, width), "depthF32": df.reshape(height, width)}
+
+
+def render_diff(cloud: Cloud, cfg, view, proj, width: int, height: int, params, threads: int = 1) -> dict:
+    """Differentiable HQS forward: depth, count (H, W); sums int64 and image
+    float32 (H, W, 3).  params: float32 (poolPoints, 4), rgb used."""
+    vp = mat_mul(proj, view)
+    cc = cloud.composed(vp)
+    s = cc.c_struct()
+    n = width * height
+    prm = np.ascontiguousarray(params, np.float32)
+    d = np.zeros(n, np.uint32)
+    cnt = np.zeros(n, np.uint32)
+    sums = np.zeros(n * 3, np.int64)
+    img = np.zeros(n * 3, np.float32)
+    c = config_from(cfg)
+    rc = lib().orc_render_diff(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width,
+                               height, prm.ctypes.data, d.ctypes.data, cnt.ctypes.data, sums.ctypes.data,
+                               img.ctypes.data, threads)
+    if rc != 0:
+        raise RuntimeError("orc_render_diff failed")
+    return {"depth": d.reshape(height, width), "count": cnt.reshape(height, width),
+            "sums": sums.reshape(height, width, 3), "image": img.reshape(height, width, 3)}
+
+
+def diff_backward(cloud: Cloud, cfg, view, proj, width: int, height: int, depth, count, dLdImage, grad,
+                  mask=None) -> None:
+    """FP64 reference gradient: grad (poolPoints, 4) float64 += per-point sums."""
+    vp = mat_mul(proj, view)
+    cc = cloud.composed(vp)
+    s = cc.c_struct()
+    assert grad.dtype == np.float64 and grad.flags.c_contiguous
+    d = np.ascontiguousarray(depth, np.uint32)
+    cnt = np.ascontiguousarray(count, np.uint32)
+    g = np.ascontiguousarray(dLdImage, np.float32)
+    m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
+    c = config_from(cfg)
+    rc = lib().orc_diff_backward(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width,
+                                 height, d.ctypes.data, cnt.ctypes.data, g.ctypes.data, grad.ctypes.data, _p(m))
+    if rc != 0:
+        raise RuntimeError("orc_diff_backward failed")
 
 
 def nearest_cpu_reference(cloud: Cloud, width: int, height: int, view=None, proj=None,

@@ -983,3 +983,108 @@ void orc_cube_grid(int n, float extent, float* positions3, float* colors4) {
                 colors4[i * 4 + 3] = 1.0f;
             }
 }
+
+/* ------------------------------------------------------------------ differentiable HQS
+ * No reference counterpart: the HQS passes with the per-point colour taken
+ * from float parameters and accumulated as 64-bit fixed point (x 2^16).
+ * Visibility (cull, depth pass, the colour pass's depth-tolerance test) is
+ * the HQS one above, so depth and count equal the HQS planes. */
+int64_t orc_diff_fixed(float v) {
+    float x = v * 65536.0f;
+    if (!(x == x)) return 0;
+    if (x > 9.0e18f) x = 9.0e18f;
+    if (x < -9.0e18f) x = -9.0e18f;
+    return (int64_t)rintf(x);
+}
+
+float orc_diff_pixel(int64_t sum, uint32_t count, float background) {
+    if (count == 0u) return background;
+    return (float)((double)sum / (65536.0 * (double)count));
+}
+
+/* Walks every fragment that passes the colour pass's depth test, calling
+ * visit(ctx, pointIndex, pixel) in the order of the single-threaded passes. */
+typedef void (*orc_frag_visit)(void* ctx, uint32_t pointIndex, size_t pixel);
+
+static int orc_visible_fragments(const orc_cloud* c, const scr_config* cfg, const float* view, const float* proj,
+                                 int width, int height, const uint32_t* depth, const uint8_t* mask,
+                                 orc_frag_visit visit, void* vctx) {
+    scr_visible_batch* vis = (scr_visible_batch*)malloc(sizeof(scr_visible_batch) * (size_t)(c->nSlots > 0 ? c->nSlots : 1));
+    if (!vis) return -1;
+    const int nv = orc_cull(c, cfg, view, proj, width, height, vis);
+    const float keep = 1.0f - cfg->depthTolerance;
+    for (int v = 0; v < nv; v++) {
+        const scr_raster_batch* b = &c->batches[vis[v].batchIndex];
+        const scr_raster_file* f = &c->files[b->fileIndex];
+        for (uint32_t li = 0; li < vis[v].activePoints; li++) {
+            const uint32_t pi = b->firstPoint + li;
+            if (mask && !mask[pi]) continue;
+            orc_frag fr;
+            if (!orc_front(c, b, f, &vis[v], pi, cfg, view, proj, width, height, 0, 1, &fr)) continue;
+            for (int oy = -fr.radius; oy <= fr.radius; oy++)
+                for (int ox = -fr.radius; ox <= fr.radius; ox++) {
+                    if (!orc_inside_footprint(ox, oy, fr.radius)) continue;
+                    const int tx = fr.px + ox, ty = fr.py + oy;
+                    if (tx < 0 || tx >= width || ty < 0 || ty >= height) continue;
+                    const size_t p = (size_t)ty * width + tx;
+                    const uint32_t d = depth[p];
+                    if (d == 0u) continue;
+                    if (!(fr.ndcZ >= orc_uint_to_depth(d) * keep)) continue;
+                    visit(vctx, pi, p);
+                }
+        }
+    }
+    free(vis);
+    return 0;
+}
+
+typedef struct orc_diff_fwd_ctx {
+    const float* params;
+    int64_t* sums;
+} orc_diff_fwd_ctx;
+
+static void orc_diff_fwd_visit(void* vctx, uint32_t pi, size_t p) {
+    orc_diff_fwd_ctx* x = (orc_diff_fwd_ctx*)vctx;
+    for (int k = 0; k < 3; k++)
+        x->sums[p * 3 + k] = (int64_t)((uint64_t)x->sums[p * 3 + k] + (uint64_t)orc_diff_fixed(x->params[(size_t)pi * 4 + k]));
+}
+
+int orc_render_diff(const orc_cloud* c, const scr_config* cfg, const float* view, const float* proj, int width,
+                    int height, const float* params, uint32_t* depth, uint32_t* count, int64_t* sums, float* image,
+                    int nThreads) {
+    const size_t npx = (size_t)width * (size_t)height;
+    uint32_t* tmp = (uint32_t*)malloc(npx * 3 * 4 + 4);
+    if (!tmp) return -1;
+    orc_hqs_planes pl = {depth, tmp, tmp + npx, tmp + 2 * npx, count, NULL, NULL};
+    int rc = orc_render_hqs(c, 1, cfg, view, proj, width, height, &pl, nThreads);
+    free(tmp);
+    if (rc != 0) return rc;
+    memset(sums, 0, npx * 3 * sizeof(int64_t));
+    orc_diff_fwd_ctx x = {params, sums};
+    rc = orc_visible_fragments(c, cfg, view, proj, width, height, depth, NULL, orc_diff_fwd_visit, &x);
+    if (rc != 0) return rc;
+    for (size_t p = 0; p < npx; p++)
+        for (int k = 0; k < 3; k++) image[p * 3 + k] = orc_diff_pixel(sums[p * 3 + k], count[p], cfg->backgroundColor[k]);
+    return 0;
+}
+
+typedef struct orc_diff_bwd_ctx {
+    const uint32_t* count;
+    const float* dLdImage;
+    double* grad;
+} orc_diff_bwd_ctx;
+
+static void orc_diff_bwd_visit(void* vctx, uint32_t pi, size_t p) {
+    orc_diff_bwd_ctx* x = (orc_diff_bwd_ctx*)vctx;
+    const double n = (double)x->count[p];
+    for (int k = 0; k < 3; k++) x->grad[(size_t)pi * 4 + k] += (double)x->dLdImage[p * 3 + k] / n;
+}
+
+/* dL/dparam = sum over the point's visible fragments of dL/dpixel / count
+ * (the fixed-point rounding passes the gradient straight through), FP64. */
+int orc_diff_backward(const orc_cloud* c, const scr_config* cfg, const float* view, const float* proj, int width,
+                      int height, const uint32_t* depth, const uint32_t* count, const float* dLdImage, double* grad,
+                      const uint8_t* mask) {
+    orc_diff_bwd_ctx x = {count, dLdImage, grad};
+    return orc_visible_fragments(c, cfg, view, proj, width, height, depth, mask, orc_diff_bwd_visit, &x);
+}

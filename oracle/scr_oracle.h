@@ -143,6 +143,23 @@ void orc_lod_levels(const float* sortedPositions, size_t count, const float bmin
  * lattice scaled to [-extent/2, extent/2]^3 (extent 1 = reference). */
 void orc_cube_grid(int pointsPerAxis, float extent, float* positions3, float* colors4);
 
+/* ---------------- differentiable HQS (no reference counterpart)
+ * Colour of point p = params[4p .. 4p+2] (float), accumulated per pixel as
+ * int64 fixed point rint(param * 2^16) over the fragments the HQS colour
+ * pass keeps; image = sum / (2^16 * count) rounded to float, background where
+ * count == 0.  depth / count equal the HQS planes.  Buffers: depth, count
+ * W*H; sums, image 3 per pixel (interleaved, row 0 = top). */
+int64_t orc_diff_fixed(float v);
+float orc_diff_pixel(int64_t sum, uint32_t count, float background);
+int orc_render_diff(const orc_cloud* cloud, const scr_config* cfg, const float* view, const float* proj, int width,
+                    int height, const float* params, uint32_t* depth, uint32_t* count, int64_t* sums, float* image,
+                    int nThreads);
+/* grad[4p + k] += sum over p's kept fragments of dLdImage[3 * pixel + k] / count[pixel]
+ * in FP64, for points with mask[p] != 0 (mask NULL: all points). */
+int orc_diff_backward(const orc_cloud* cloud, const scr_config* cfg, const float* view, const float* proj, int width,
+                      int height, const uint32_t* depth, const uint32_t* count, const float* dLdImage,
+                      double* grad, const uint8_t* mask);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@ POINT_SIZE_WORLD_SPACE = 1
 
 PLANE_DEPTH, PLANE_RED, PLANE_GREEN, PLANE_BLUE, PLANE_COUNT = 0, 1, 2, 3, 4
 PLANE_NEAREST_DEPTH, PLANE_NEAREST_INDEX, PLANE_RGBA8, PLANE_DEPTH_F32 = 5, 6, 7, 8
+PLANE_DIFF_SUM_R, PLANE_DIFF_SUM_G, PLANE_DIFF_SUM_B = 9, 10, 11
 POOL_XYZ_LOW, POOL_XYZ_MED, POOL_XYZ_HIGH, POOL_COLORS, POOL_LEVELS, POOL_BATCHES = range(6)
 
 ERRORS = {
@@ -173,6 +174,8 @@ _SIGNATURES = {
     "scr_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _I, _VP]),
     "scr_pick": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, C.c_float, C.c_float, _I,
                       C.POINTER(C.c_uint32), C.POINTER(_I)]),
+    "scr_diff_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, _VP, _VP]),
+    "scr_diff_backward": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, _VP, _VP]),
     "scr_multiview_create": (_I, [_VP, _VP, _I, _VP, _VP, _I, C.POINTER(_VP)]),
     "scr_multiview_destroy": (_I, [_VP]),
     "scr_multiview_render": (_I, [_VP, C.POINTER(CConfig), _VP, _I]),
@@ -514,6 +517,10 @@ class ComputeRasteriserPointCloud:
     def batchCount(self) -> int:
         return self.info().maxResidentBatches
 
+    @property
+    def pointsPerBatch(self) -> int:
+        return self.info().pointsPerBatch
+
     # --- slot pool
     def addBatches(self, xyzLow, xyzMed, xyzHigh, colors, levels, batches, commit: bool = True) -> list:
         arrs = [np.ascontiguousarray(a, dt) for a, dt in
@@ -597,7 +604,8 @@ class Frame:
         ctx._adopt(self)
 
     def read(self, which: int) -> np.ndarray:
-        dt = np.float32 if which == PLANE_DEPTH_F32 else np.uint32
+        dt = np.float32 if which == PLANE_DEPTH_F32 else (
+            np.int64 if PLANE_DIFF_SUM_R <= which <= PLANE_DIFF_SUM_B else np.uint32)
         out = np.zeros(self.width * self.height, dt)
         self.ctx.check(self.ctx.lib.scr_frame_read(self.handle, which, out.ctypes.data, out.nbytes))
         return out
@@ -669,6 +677,55 @@ class ComputeRasteriser:
                                              self.frame.handle, float(ndc[0]), float(ndc[1]), int(searchRadius),
                                              C.byref(out), C.byref(found)))
         return int(out.value) if found.value else None
+
+
+# ---------------------------------------------------------------- differentiable HQS
+class DifferentiableHQS:
+    """Differentiable HQS over one cloud (scr_diff_render / scr_diff_backward).
+
+    One frame per view keeps the forward's depth / count planes for the
+    backward.  Pointers are device addresses: params float4 per pool point,
+    image 3 floats per pixel, dLdImage likewise, grad 4 doubles per pool
+    point (accumulated)."""
+
+    def __init__(self, ctx: Context, cloud: "ComputeRasteriserPointCloud", sizes,
+                 configuration: Optional[ComputeRasteriserConfiguration] = None):
+        self.ctx = ctx
+        self.cloud = cloud
+        self.frames = [Frame(ctx, w, h) for (w, h) in sizes]
+        self.configuration = configuration or ComputeRasteriserConfiguration()
+
+    @property
+    def poolPoints(self) -> int:
+        return self.cloud.batchCount * self.cloud.pointsPerBatch
+
+    def _prep(self, camera: Camera):
+        m = np.ascontiguousarray(self.cloud.worldMatrix, np.float32)
+        self.ctx.check(self.ctx.lib.scr_cloud_set_model(self.cloud.handle, m.ctypes.data))
+        return self.configuration.to_c(), camera.to_c()
+
+    def forward(self, view: int, camera: Camera, params_ptr: int, image_ptr: int) -> None:
+        cfg, cam = self._prep(camera)
+        self.ctx.check(self.ctx.lib.scr_diff_render(self.ctx.handle, C.byref(cfg), C.byref(cam), self.cloud.handle,
+                                                    params_ptr, self.frames[view].handle, image_ptr))
+
+    def backward(self, view: int, camera: Camera, dLdImage_ptr: int, grad_ptr: int) -> None:
+        cfg, cam = self._prep(camera)
+        self.ctx.check(self.ctx.lib.scr_diff_backward(self.ctx.handle, C.byref(cfg), C.byref(cam),
+                                                      self.cloud.handle, self.frames[view].handle, dLdImage_ptr,
+                                                      grad_ptr))
+
+    def read(self, view: int, which: int) -> np.ndarray:
+        f = self.frames[view]
+        return f.read(which).reshape(f.height, f.width)
+
+    def sums(self, view: int) -> np.ndarray:
+        """(H, W, 3) int64 fixed-point sums of the last forward of this view."""
+        return np.stack([self.read(view, PLANE_DIFF_SUM_R + k) for k in range(3)], axis=-1)
+
+    def close(self) -> None:
+        for f in getattr(self, "frames", []):
+            f.close()
 
 
 # ---------------------------------------------------------------- multi-view

@@ -806,6 +806,7 @@ int scr_frame_destroy(scr_frame* f) {
     dfree(f->dNearest);
     dfree(f->dRGBA);
     dfree(f->dDepthF);
+    dfree(f->dSums);
     delete f;
     scr::ctx_release(ctx);
     return SCR_OK;
@@ -829,6 +830,11 @@ int scr_frame_read(scr_frame* f, int which, void* host, size_t bytes) {
         SCR_HIP(ctx, hipMemcpy(host, f->dRGBA, n * 4, hipMemcpyDeviceToHost));
     } else if (which == SCR_PLANE_DEPTH_F32) {
         SCR_HIP(ctx, hipMemcpy(host, f->dDepthF, n * 4, hipMemcpyDeviceToHost));
+    } else if (which >= SCR_PLANE_DIFF_SUM_R && which <= SCR_PLANE_DIFF_SUM_B) {
+        if (bytes < n * 8) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "host buffer too small (8 B per pixel)");
+        if (!f->dSums) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "frame has no differentiable render");
+        SCR_HIP(ctx, hipMemcpy(host, f->dSums + (size_t)(which - SCR_PLANE_DIFF_SUM_R) * f->planeStride(), n * 8,
+                               hipMemcpyDeviceToHost));
     } else {
         return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown plane");
     }
@@ -905,6 +911,92 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
     }
     scr::PassScope ps(ctx, "resolve");
     SCR_HIP(ctx, scr::launch_resolve(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA, frame->dDepthF));
+    return SCR_OK;
+}
+
+// ------------------------------------------------------------------ differentiable HQS
+
+namespace {
+
+int check_diff(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud, scr_frame* frame) {
+    if (!cam || !cloud || !frame) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "NULL camera, cloud or frame");
+    if (cloud->ctx != ctx || frame->ctx != ctx) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "foreign handle");
+    int rc = check_config(ctx, cfg);
+    if (rc != SCR_OK) return rc;
+    if (cfg->mode != SCR_MODE_HIGH_QUALITY_AVERAGE)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "differentiable rendering is an HQS mode");
+    if (cfg->applyTint != 0 || cfg->colorizeChunks != 0 || cfg->colorizeOverdraw != 0)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "differentiable rendering takes its colours from the parameters");
+    return SCR_OK;
+}
+
+}  // namespace
+
+int scr_diff_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud,
+                    const float* params, scr_frame* frame, float* image) {
+    if (!ctx) return SCR_ERR_INVALID_ARGUMENT;
+    int rc = check_diff(ctx, cfg, cam, cloud, frame);
+    if (rc != SCR_OK) return rc;
+    if (!params || !image) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "NULL params or image");
+    if (!frame->dSums) {
+        SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        SCR_HIP(ctx, hipMalloc((void**)&frame->dSums, 3 * frame->planeStride() * sizeof(unsigned long long)));
+    }
+    rc = flush_mirror(cloud);
+    if (rc != SCR_OK) return rc;
+    const scr::PassParams p = make_params(cfg, cam, frame->width, frame->height);
+    float vp[16];
+    scr::mat_mul(cam->projectionMatrix, cam->viewMatrix, vp);
+    hipStream_t s = ctx->stream;
+    {
+        scr::PassScope ps(ctx, "clear");
+        SCR_HIP(ctx, scr::launch_fill_u32(s, frame->dPlanes, 5 * frame->planeStride(), 0u));
+        SCR_HIP(ctx, scr::launch_fill_u64(s, frame->dSums, 3 * frame->planeStride(), 0ull));
+    }
+    if (cloud->residentBatchCount > 0) {
+        rc = upload_files(cloud, cloud->model, vp, nullptr, nullptr);
+        if (rc != SCR_OK) return rc;
+        const CloudDev d = cloud->dev();
+        {
+            scr::PassScope ps(ctx, "cull");
+            SCR_HIP(ctx, scr::launch_cull(s, d, p));
+        }
+        {
+            scr::PassScope ps(ctx, "depth");
+            SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, d, p, frame->plane(0)));
+        }
+        scr::PassScope ps(ctx, "diff_color");
+        SCR_HIP(ctx, scr::launch_diff_color(s, ctx->numCUs, d, p, reinterpret_cast<const float4*>(params),
+                                            frame->plane(0), frame->dSums, frame->planeStride(), frame->plane(4)));
+    }
+    scr::PassScope ps(ctx, "diff_resolve");
+    SCR_HIP(ctx, scr::launch_diff_resolve(s, p, frame->dSums, frame->planeStride(), frame->plane(4), image));
+    return SCR_OK;
+}
+
+int scr_diff_backward(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud,
+                      scr_frame* frame, const float* dLdImage, double* grad) {
+    if (!ctx) return SCR_ERR_INVALID_ARGUMENT;
+    int rc = check_diff(ctx, cfg, cam, cloud, frame);
+    if (rc != SCR_OK) return rc;
+    if (!dLdImage || !grad) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "NULL dLdImage or grad");
+    if (!frame->dSums) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "frame has no differentiable forward");
+    if (cloud->residentBatchCount == 0) return SCR_OK;
+    rc = flush_mirror(cloud);
+    if (rc != SCR_OK) return rc;
+    const scr::PassParams p = make_params(cfg, cam, frame->width, frame->height);
+    float vp[16];
+    scr::mat_mul(cam->projectionMatrix, cam->viewMatrix, vp);
+    hipStream_t s = ctx->stream;
+    rc = upload_files(cloud, cloud->model, vp, nullptr, nullptr);
+    if (rc != SCR_OK) return rc;
+    const CloudDev d = cloud->dev();
+    {
+        scr::PassScope ps(ctx, "cull");
+        SCR_HIP(ctx, scr::launch_cull(s, d, p));
+    }
+    scr::PassScope ps(ctx, "diff_backward");
+    SCR_HIP(ctx, scr::launch_diff_backward(s, ctx->numCUs, d, p, frame->plane(0), frame->plane(4), dLdImage, grad));
     return SCR_OK;
 }
 

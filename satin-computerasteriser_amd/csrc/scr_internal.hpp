@@ -103,6 +103,7 @@ struct scr_frame {
     int width = 0, height = 0;
     uint32_t* dPlanes = nullptr;  // depth, red, green, blue, count (SoA, 5 planes of planeStride)
     unsigned long long* dNearest = nullptr;  // depth << 32 | ~index per pixel
+    unsigned long long* dSums = nullptr;     // differentiable mode: 3 int64 planes of planeStride (on first use)
     uint32_t* dRGBA = nullptr;
     float* dDepthF = nullptr;
     size_t pixels() const { return (size_t)width * (size_t)height; }
@@ -153,6 +154,14 @@ hipError_t launch_nearest(hipStream_t s, int numCUs, const CloudDev& c, const Pa
                           unsigned long long* packed);
 hipError_t launch_nearest_resolve(hipStream_t s, const PassParams& p, const unsigned long long* packed,
                                   const uint32_t* colors, uint32_t* rgba, float* depthf);
+
+// Differentiable HQS (scr_diff.hip).
+hipError_t launch_diff_color(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, const float4* params,
+                             const uint32_t* depth, unsigned long long* sums, size_t stride, uint32_t* count);
+hipError_t launch_diff_resolve(hipStream_t s, const PassParams& p, const unsigned long long* sums, size_t stride,
+                               const uint32_t* count, float* image);
+hipError_t launch_diff_backward(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p,
+                                const uint32_t* depth, const uint32_t* count, const float* dLdImage, double* grad);
 
 // Pool destination of one pack (device pointers already offset to the slot run).
 struct PackDest {

@@ -10,6 +10,16 @@ if str(TESTS) not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path through the C ABI)")
+    # torch ships its own HIP runtime under the same soname as the system one;
+    # whichever loads first serves the process, and torch only finds the GPU
+    # when it is its own.  So when GPU tests may run, torch is imported before
+    # libscr.so is loaded (the torch harness tests need it; the others run on
+    # that runtime too).
+    if "not gpu" not in (config.option.markexpr or ""):
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
 
 
 @pytest.fixture(scope="session")

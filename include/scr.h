@@ -255,6 +255,9 @@ typedef struct scr_scene_params {
 void scr_scene_params_default(scr_scene_params* p);
 int scr_gen_scene(scr_ctx* ctx, const scr_scene_params* params, size_t count, float* positions4,
                   float* colors4, int outputOnDevice);
+/* Terrain height of the scene's heightfield at (x, y) (host computation,
+ * the same function the generator evaluates), for placing cameras. */
+float scr_scene_height(const scr_scene_params* params, float x, float y);
 
 /* ---- frames (per-viewport resources: ComputeRasteriser.swift CachedResources) */
 int scr_frame_create(scr_ctx* ctx, int width, int height, scr_frame** out);

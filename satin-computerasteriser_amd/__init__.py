@@ -168,6 +168,7 @@ _SIGNATURES = {
     "scr_pack_chunk": (_I, [_VP, _VP, _VP, _SZ, C.POINTER(CPackOptions), C.POINTER(C.c_int32)]),
     "scr_scene_params_default": (None, [C.POINTER(CSceneParams)]),
     "scr_gen_scene": (_I, [_VP, C.POINTER(CSceneParams), _SZ, _VP, _VP, _I]),
+    "scr_scene_height": (C.c_float, [C.POINTER(CSceneParams), C.c_float, C.c_float]),
     "scr_frame_create": (_I, [_VP, _I, _I, C.POINTER(_VP)]),
     "scr_frame_destroy": (_I, [_VP]),
     "scr_frame_read": (_I, [_VP, _I, _VP, _SZ]),
@@ -937,6 +938,58 @@ def gen_scene(ctx: Context, count: int, on_device: bool = True, **kw):
     col = np.zeros((count, 4), np.float32)
     ctx.check(ctx.lib.scr_gen_scene(ctx.handle, C.byref(p), count, pos.ctypes.data, col.ctypes.data, 0))
     return pos, col
+
+
+def scene_height(x: float, y: float, **kw) -> float:
+    """Terrain height of the generated scene at (x, y)."""
+    p = scene_params(**kw)
+    return float(load_library().scr_scene_height(C.byref(p), float(x), float(y)))
+
+
+def catmull_rom(points: np.ndarray, frames: int) -> tuple:
+    """Uniform Catmull-Rom spline through ``points`` (K, 3), end points
+    duplicated; ``frames`` samples evenly spaced in the spline parameter.
+    Returns (positions, unit tangents), each (frames, 3) float64."""
+    p = np.asarray(points, np.float64)
+    ext = np.concatenate([p[:1], p, p[-1:]], axis=0)
+    segs = len(p) - 1
+    pos, tan = [], []
+    for f in range(frames):
+        u = f / max(frames - 1, 1) * segs
+        i = min(int(u), segs - 1)
+        t = u - i
+        p0, p1, p2, p3 = ext[i], ext[i + 1], ext[i + 2], ext[i + 3]
+        t2, t3 = t * t, t * t * t
+        pos.append(0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t2 +
+                          (-p0 + 3 * p1 - 3 * p2 + p3) * t3))
+        d = 0.5 * ((-p0 + p2) + 2 * (2 * p0 - 5 * p1 + 4 * p2 - p3) * t + 3 * (-p0 + 3 * p1 - 3 * p2 + p3) * t2)
+        n = np.linalg.norm(d)
+        tan.append(d / n if n > 0 else np.array([1.0, 0.0, 0.0]))
+    return np.array(pos), np.array(tan)
+
+
+def flythrough_cameras(frames: int = 120, pathSeed: int = 0x5C0003, width: int = 2560, height: int = 1440,
+                       controlPoints: int = 8, minAltitude: float = 50.0, maxAltitude: float = 400.0,
+                       fov: float = 60.0, **scene) -> list:
+    """Fly-through over the generated scene (``scene``: scene_params fields):
+    a Catmull-Rom spline through ``controlPoints`` points drawn from
+    ``pathSeed`` at ``minAltitude``..``maxAltitude`` metres above the terrain;
+    each frame looks 200 m ahead along the tangent and 30 m down."""
+    prm = scene_params(**scene)
+    rng = np.random.default_rng(pathSeed)
+    half = float(prm.halfExtent)
+    ctrl = []
+    for _ in range(controlPoints):
+        x, y = rng.uniform(-0.8 * half, 0.8 * half, 2)
+        z = scene_height(x, y, **scene) + rng.uniform(minAltitude, maxAltitude)
+        ctrl.append((x, y, z))
+    pos, tan = catmull_rom(np.array(ctrl), frames)
+    cams = []
+    for p, t in zip(pos, tan):
+        target = p + 200.0 * t + np.array([0.0, 0.0, -30.0])
+        cams.append(Camera.perspective(tuple(p), tuple(target), (0.0, 0.0, 1.0), fov=fov, aspect=width / height,
+                                       near=1.0, far=4.0 * half * 2.0))
+    return cams
 
 
 def version() -> str:

@@ -140,9 +140,14 @@ PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, 
     p.lo = std::fmax(std::fmin(mn, mx), 1.0f);
     p.hi = std::fmax(std::fmax(mn, mx), p.lo);
     p.constRadius = (p.lo == p.hi) ? const_radius_for(p.lo) : -1;
-    p.maxRadius = const_radius_for(p.hi);
     p.ortho = cam->projectionMatrix[15] == 1.0f;
     p.focal = ((float)height * 0.5f) * cam->projectionMatrix[5];
+    if (p.constRadius < 0 && p.ortho) {
+        // orthographic sizes do not depend on the point (Common.metal:42-43, 49-50)
+        const float ps = p.pointSizeMode == 1 ? (2.0f * p.pointSizeScale) * p.focal : p.pointSizeScale;
+        p.constRadius = radius_of_size(ps, p.lo, p.hi);
+    }
+    p.maxRadius = const_radius_for(p.hi);
     for (int i = 0; i < 4; i++) p.bg[i] = cfg->backgroundColor[i];
     return p;
 }

@@ -32,77 +32,89 @@ __device__ inline unsigned long long diff_fixed(float v) {
     return (unsigned long long)(long long)rintf(x);
 }
 
+struct DiffLds {
+    uint32_t depth[kDiffTile];
+    unsigned long long s[3][kDiffTile];
+    uint32_t n[kDiffTile];
+    int red[5 * kMaxWaves];
+};
+
+template <bool kR0>
+__device__ inline void diff_color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
+                                         const float4* __restrict__ params, const uint32_t* __restrict__ depth,
+                                         unsigned long long* sums, size_t stride, uint32_t* count, DiffLds& L) {
+    constexpr int kChunk = kThreads * kPPT;
+    const bool vec = vec_ok<kPPT>(bc);
+    for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
+        const uint32_t li0 = cb + threadIdx.x * kPPT;
+        ChunkFrags<kPPT> f;
+        chunk_front<false, kR0, kPPT>(c, p, bc, li0, vec, f);
+        const ChunkBox box = chunk_box<kThreads, kPPT>(f, p, kDiffTile, L.red);
+        if (box.frags == 0) continue;
+        const int area = box.w * box.h;
+        if (box.lds) {
+            for (int i = threadIdx.x; i < area; i += kThreads) {
+                L.depth[i] = depth[box.pixel(i, p.width)];
+                L.s[0][i] = L.s[1][i] = L.s[2][i] = 0ull;
+                L.n[i] = 0u;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kPPT; j++) {
+            if (f.px[j] < 0) continue;
+            const float4 t = params[bc.first + li0 + (uint32_t)j];
+            const unsigned long long q0 = diff_fixed(t.x), q1 = diff_fixed(t.y), q2 = diff_fixed(t.z);
+            const int rad = kR0 ? 0 : f.r[j];
+            for (int oy = -rad; oy <= rad; oy++)
+                for (int ox = -rad; ox <= rad; ox++) {
+                    if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
+                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
+                    const size_t pix = (size_t)ty * p.width + tx;
+                    const int k = box.lds ? box.local(tx, ty) : 0;
+                    const uint32_t d = box.lds ? L.depth[k] : depth[pix];
+                    if (d == 0u) continue;
+                    if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                    if (box.lds) {
+                        atomicAdd(&L.s[0][k], q0);
+                        atomicAdd(&L.s[1][k], q1);
+                        atomicAdd(&L.s[2][k], q2);
+                        atomicAdd(&L.n[k], 1u);
+                    } else {
+                        atomicAdd(sums + pix, q0);
+                        atomicAdd(sums + stride + pix, q1);
+                        atomicAdd(sums + 2 * stride + pix, q2);
+                        atomicAdd(count + pix, 1u);
+                    }
+                }
+        }
+        if (box.lds) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < area; i += kThreads) {
+                const uint32_t n = L.n[i];
+                if (n == 0u) continue;
+                const size_t pix = box.pixel(i, p.width);
+                atomicAdd(sums + pix, L.s[0][i]);
+                atomicAdd(sums + stride + pix, L.s[1][i]);
+                atomicAdd(sums + 2 * stride + pix, L.s[2][i]);
+                atomicAdd(count + pix, n);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 template <bool kR0>
 __global__ __launch_bounds__(kThreads) void k_diff_color(CloudDev c, PassParams p, const float4* __restrict__ params,
                                                          const uint32_t* __restrict__ depth,
                                                          unsigned long long* sums, size_t stride, uint32_t* count) {
-    __shared__ uint32_t ldsDepth[kDiffTile];
-    __shared__ unsigned long long ldsS[3][kDiffTile];
-    __shared__ uint32_t ldsN[kDiffTile];
-    __shared__ int red[5 * kMaxWaves];
-    constexpr int kChunk = kThreads * kPPT;
+    __shared__ DiffLds L;
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
-        const BatchCtx bc = batch_ctx(c, c.visible[v], 0xffffffffu);
-        const bool vec = vec_ok<kPPT>(bc);
-        for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
-            const uint32_t li0 = cb + threadIdx.x * kPPT;
-            ChunkFrags<kPPT> f;
-            chunk_front<false, kR0, kPPT>(c, p, bc, li0, vec, f);
-            const ChunkBox box = chunk_box<kThreads, kPPT>(f, p, kDiffTile, red);
-            if (box.frags == 0) continue;
-            const int area = box.w * box.h;
-            if (box.lds) {
-                for (int i = threadIdx.x; i < area; i += kThreads) {
-                    ldsDepth[i] = depth[box.pixel(i, p.width)];
-                    ldsS[0][i] = ldsS[1][i] = ldsS[2][i] = 0ull;
-                    ldsN[i] = 0u;
-                }
-                __syncthreads();
-            }
-#pragma unroll
-            for (int j = 0; j < kPPT; j++) {
-                if (f.px[j] < 0) continue;
-                const float4 t = params[bc.first + li0 + (uint32_t)j];
-                const unsigned long long q0 = diff_fixed(t.x), q1 = diff_fixed(t.y), q2 = diff_fixed(t.z);
-                const int rad = kR0 ? 0 : f.r[j];
-                for (int oy = -rad; oy <= rad; oy++)
-                    for (int ox = -rad; ox <= rad; ox++) {
-                        if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
-                        const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                        if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
-                        const size_t pix = (size_t)ty * p.width + tx;
-                        const int k = box.lds ? box.local(tx, ty) : 0;
-                        const uint32_t d = box.lds ? ldsDepth[k] : depth[pix];
-                        if (d == 0u) continue;
-                        if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                        if (box.lds) {
-                            atomicAdd(&ldsS[0][k], q0);
-                            atomicAdd(&ldsS[1][k], q1);
-                            atomicAdd(&ldsS[2][k], q2);
-                            atomicAdd(&ldsN[k], 1u);
-                        } else {
-                            atomicAdd(sums + pix, q0);
-                            atomicAdd(sums + stride + pix, q1);
-                            atomicAdd(sums + 2 * stride + pix, q2);
-                            atomicAdd(count + pix, 1u);
-                        }
-                    }
-            }
-            if (box.lds) {
-                __syncthreads();
-                for (int i = threadIdx.x; i < area; i += kThreads) {
-                    const uint32_t n = ldsN[i];
-                    if (n == 0u) continue;
-                    const size_t pix = box.pixel(i, p.width);
-                    atomicAdd(sums + pix, ldsS[0][i]);
-                    atomicAdd(sums + stride + pix, ldsS[1][i]);
-                    atomicAdd(sums + 2 * stride + pix, ldsS[2][i]);
-                    atomicAdd(count + pix, n);
-                }
-                __syncthreads();
-            }
-        }
+        const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
+        if (!kR0 && bc.rconst == 0) diff_color_chunks<true>(c, p, bc, params, depth, sums, stride, count, L);
+        else diff_color_chunks<kR0>(c, p, bc, params, depth, sums, stride, count, L);
     }
 }
 
@@ -122,48 +134,56 @@ __global__ __launch_bounds__(256) void k_diff_resolve(PassParams p, const unsign
 }
 
 template <bool kR0>
+__device__ inline void diff_backward_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
+                                            const uint32_t* __restrict__ depth, const uint32_t* __restrict__ count,
+                                            const float* __restrict__ dLdImage, double* __restrict__ grad) {
+    constexpr int kChunk = kThreads * kPPT;
+    const bool vec = vec_ok<kPPT>(bc);
+    for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
+        const uint32_t li0 = cb + threadIdx.x * kPPT;
+        ChunkFrags<kPPT> f;
+        chunk_front<false, kR0, kPPT>(c, p, bc, li0, vec, f);
+#pragma unroll
+        for (int j = 0; j < kPPT; j++) {
+            if (f.px[j] < 0) continue;
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+            bool any = false;
+            const int rad = kR0 ? 0 : f.r[j];
+            for (int oy = -rad; oy <= rad; oy++)
+                for (int ox = -rad; ox <= rad; ox++) {
+                    if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
+                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
+                    const size_t pix = (size_t)ty * p.width + tx;
+                    const uint32_t d = depth[pix];
+                    if (d == 0u) continue;
+                    if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                    const double n = (double)count[pix];
+                    g0 += (double)dLdImage[pix * 3 + 0] / n;
+                    g1 += (double)dLdImage[pix * 3 + 1] / n;
+                    g2 += (double)dLdImage[pix * 3 + 2] / n;
+                    any = true;
+                }
+            if (!any) continue;
+            double* gp = grad + (size_t)(bc.first + li0 + (uint32_t)j) * 4;
+            gp[0] += g0;
+            gp[1] += g1;
+            gp[2] += g2;
+        }
+    }
+}
+
+template <bool kR0>
 __global__ __launch_bounds__(kThreads) void k_diff_backward(CloudDev c, PassParams p,
                                                             const uint32_t* __restrict__ depth,
                                                             const uint32_t* __restrict__ count,
                                                             const float* __restrict__ dLdImage,
                                                             double* __restrict__ grad) {
-    constexpr int kChunk = kThreads * kPPT;
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
-        const BatchCtx bc = batch_ctx(c, c.visible[v], 0xffffffffu);
-        const bool vec = vec_ok<kPPT>(bc);
-        for (uint32_t cb = 0; cb < bc.active; cb += kChunk) {
-            const uint32_t li0 = cb + threadIdx.x * kPPT;
-            ChunkFrags<kPPT> f;
-            chunk_front<false, kR0, kPPT>(c, p, bc, li0, vec, f);
-#pragma unroll
-            for (int j = 0; j < kPPT; j++) {
-                if (f.px[j] < 0) continue;
-                double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-                bool any = false;
-                const int rad = kR0 ? 0 : f.r[j];
-                for (int oy = -rad; oy <= rad; oy++)
-                    for (int ox = -rad; ox <= rad; ox++) {
-                        if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
-                        const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                        if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
-                        const size_t pix = (size_t)ty * p.width + tx;
-                        const uint32_t d = depth[pix];
-                        if (d == 0u) continue;
-                        if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                        const double n = (double)count[pix];
-                        g0 += (double)dLdImage[pix * 3 + 0] / n;
-                        g1 += (double)dLdImage[pix * 3 + 1] / n;
-                        g2 += (double)dLdImage[pix * 3 + 2] / n;
-                        any = true;
-                    }
-                if (!any) continue;
-                double* gp = grad + (size_t)(bc.first + li0 + (uint32_t)j) * 4;
-                gp[0] += g0;
-                gp[1] += g1;
-                gp[2] += g2;
-            }
-        }
+        const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
+        if (!kR0 && bc.rconst == 0) diff_backward_chunks<true>(c, p, bc, depth, count, dLdImage, grad);
+        else diff_backward_chunks<kR0>(c, p, bc, depth, count, dLdImage, grad);
     }
 }
 

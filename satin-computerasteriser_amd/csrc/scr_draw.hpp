@@ -118,6 +118,7 @@ struct BatchCtx {
     int level;
     float lodThreshold;
     bool keepAll;       // no point of the batch can fail the LOD/dither test
+    int rconst;         // footprint radius shared by the batch's points, or -1
     uint32_t first, active, batchIndex;   // active: end of the drawn range
     float T[16];        // file.transform (held in registers: the atomics below
                         // would otherwise force reloads through memory)
@@ -125,7 +126,8 @@ struct BatchCtx {
 };
 
 // `hi` bounds the local point indices drawn (at most the active prefix).
-__device__ inline BatchCtx batch_ctx(const CloudDev& c, const scr_visible_batch& vb, uint32_t hi) {
+__device__ inline BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
+                                     uint32_t hi) {
     const scr_raster_batch b = c.batches[vb.batchIndex];
     BatchCtx bc;
     bc.level = vb.level;
@@ -146,6 +148,7 @@ __device__ inline BatchCtx batch_ctx(const CloudDev& c, const scr_visible_batch&
 #pragma unroll
     for (int i = 0; i < 16; i++) bc.T[i] = f->transform[i];
     bc.W = f->world;
+    bc.rconst = batch_const_radius(p, f->world, b);
     return bc;
 }
 
@@ -256,7 +259,7 @@ __device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const
         f.px[j] = ok ? ix : -1;
         f.py[j] = p.height - 1 - iy;
         f.z[j] = nz;
-        f.r[j] = (kR0 || !ok) ? 0 : footprint_radius(p, bc.W, x, y, z);
+        f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, x, y, z));
     }
 }
 
@@ -355,9 +358,8 @@ __device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams&
 // depthPassUpdate — DepthPass/Shaders.metal:21-116, for the local points
 // [lo, hi) of one visible batch (lo a multiple of NT * PPT).
 template <bool kR0, int NT, int PPT>
-__device__ inline void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
-                                   uint32_t hi, uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
-    const BatchCtx bc = batch_ctx(c, vb, hi);
+__device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                    uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
     const bool vec = vec_ok<PPT>(bc);
     constexpr int kChunk = NT * PPT;
     for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
@@ -370,13 +372,22 @@ __device__ inline void depth_batch(const CloudDev& c, const PassParams& p, const
             for (int i = threadIdx.x; i < area; i += NT) lds[i] = 0u;
             __syncthreads();
         }
+        // single-pixel points: a thread's consecutive points often share a
+        // pixel (Morton order); a max the thread has already issued there
+        // makes a smaller one a no-op, so it is skipped
+        int lastK = -1;
+        uint32_t lastD = 0u;
 #pragma unroll
         for (int j = 0; j < PPT; j++) {
             if (f.px[j] < 0) continue;
             const uint32_t d = depth_to_uint_hw(f.z[j]);
             if (kR0) {
-                if (box.lds) atomicMax(lds + box.local(f.px[j], f.py[j]), d);
-                else atomicMax(depth + (size_t)f.py[j] * p.width + f.px[j], d);
+                const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
+                if (k == lastK && d <= lastD) continue;
+                lastD = (k == lastK) ? max(lastD, d) : d;
+                lastK = k;
+                if (box.lds) atomicMax(lds + k, d);
+                else atomicMax(depth + (size_t)k, d);
                 continue;
             }
             const int r = f.r[j];
@@ -413,11 +424,25 @@ struct ColorLds {
     int* red;
 };
 
+// Adds a run's packed sums (r | g << 32, b | count << 32) to LDS tile slot k
+// or to global pixel k.
+__device__ inline void color_flush(bool lds, int k, unsigned long long rg, unsigned long long bn, const ColorLds& L,
+                                   uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count) {
+    if (lds) {
+        atomicAdd(L.rg + k, rg);
+        atomicAdd(L.bn + k, bn);
+    } else {
+        atomicAdd(red + k, (uint32_t)rg);
+        atomicAdd(green + k, (uint32_t)(rg >> 32));
+        atomicAdd(blue + k, (uint32_t)bn);
+        atomicAdd(count + k, (uint32_t)(bn >> 32));
+    }
+}
+
 template <bool kR0, int NT, int PPT>
-__device__ inline void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
-                                   uint32_t hi, const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
-                                   uint32_t* blue, uint32_t* count, const ColorLds& L) {
-    const BatchCtx bc = batch_ctx(c, vb, hi);
+__device__ inline void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                    const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
+                                    uint32_t* blue, uint32_t* count, const ColorLds& L) {
     const bool vec = vec_ok<PPT>(bc);
     constexpr int kChunk = NT * PPT;
     for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
@@ -470,13 +495,33 @@ __device__ inline void color_batch(const CloudDev& c, const PassParams& p, const
             }
             __syncthreads();
         }
+        // single-pixel points: runs of a thread's consecutive fragments on one
+        // pixel (Morton order) are summed in registers and added once
+        int runK = -1;
+        unsigned long long runRG = 0ull, runBN = 0ull;
 #pragma unroll
         for (int j = 0; j < PPT; j++) {
             if (f.px[j] < 0) continue;
             const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
             const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
             const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
-            const int rad = kR0 ? 0 : f.r[j];
+            if (kR0) {
+                const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
+                const uint32_t d = box.lds ? L.depth[k] : depth[k];
+                if (d == 0u) continue;
+                if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                if (k == runK) {
+                    runRG += rg;
+                    runBN += bn;
+                    continue;
+                }
+                if (runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
+                runK = k;
+                runRG = rg;
+                runBN = bn;
+                continue;
+            }
+            const int rad = f.r[j];
             for (int oy = -rad; oy <= rad; oy++)
                 for (int ox = -rad; ox <= rad; ox++) {
                     if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
@@ -498,6 +543,7 @@ __device__ inline void color_batch(const CloudDev& c, const PassParams& p, const
                     }
                 }
         }
+        if (kR0 && runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
         if (box.lds) {
             __syncthreads();
             // per-chunk sums stay below 2^32 (<= kChunk fragments x 255 per
@@ -522,10 +568,8 @@ __device__ inline void color_batch(const CloudDev& c, const PassParams& p, const
 // nearestDepthUpdate + nearestIndexUpdate (NearestDepth/Shaders.metal:14-91,
 // NearestIndex/Shaders.metal:14-94) as one max of depth << 32 | ~index.
 template <bool kR0, int NT, int PPT>
-__device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
-                                     uint32_t hi, unsigned long long* __restrict__ packed, unsigned long long* lds,
-                                     int* red) {
-    const BatchCtx bc = batch_ctx(c, vb, hi);
+__device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                      unsigned long long* __restrict__ packed, unsigned long long* lds, int* red) {
     const bool vec = vec_ok<PPT>(bc);
     constexpr int kChunk = NT * PPT;
     for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
@@ -539,6 +583,8 @@ __device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, con
             for (int i = threadIdx.x; i < area; i += NT) lds[i] = 0ull;
             __syncthreads();
         }
+        int lastK = -1;
+        unsigned long long lastKey = 0ull;
 #pragma unroll
         for (int j = 0; j < PPT; j++) {
             if (f.px[j] < 0) continue;
@@ -547,7 +593,17 @@ __device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, con
             // reverse-Z depth, then the lowest point index among equal depths
             const unsigned long long key =
                 ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~pi);
-            const int r = kR0 ? 0 : f.r[j];
+            if (kR0) {
+                // as in the depth pass: skip a max this thread already exceeded
+                const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
+                if (k == lastK && key <= lastKey) continue;
+                lastKey = (k == lastK) ? (key > lastKey ? key : lastKey) : key;
+                lastK = k;
+                if (box.lds) atomicMax(lds + k, key);
+                else atomicMax(packed + (size_t)k, key);
+                continue;
+            }
+            const int r = f.r[j];
             for (int oy = -r; oy <= r; oy++)
                 for (int ox = -r; ox <= r; ox++) {
                     if (!kR0 && !inside_footprint(ox, oy, r)) continue;
@@ -568,6 +624,35 @@ __device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, con
             __syncthreads();
         }
     }
+}
+
+// ---------------------------------------------------------------- per-batch entry points
+// A batch whose points all have radius 0 runs the single-pixel code.
+
+template <bool kR0, int NT, int PPT>
+__device__ inline void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
+                                   uint32_t hi, uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
+    const BatchCtx bc = batch_ctx(c, p, vb, hi);
+    if (!kR0 && bc.rconst == 0) depth_chunks<true, NT, PPT>(c, p, bc, lo, depth, lds, red);
+    else depth_chunks<kR0, NT, PPT>(c, p, bc, lo, depth, lds, red);
+}
+
+template <bool kR0, int NT, int PPT>
+__device__ inline void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
+                                   uint32_t hi, const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
+                                   uint32_t* blue, uint32_t* count, const ColorLds& L) {
+    const BatchCtx bc = batch_ctx(c, p, vb, hi);
+    if (!kR0 && bc.rconst == 0) color_chunks<true, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
+    else color_chunks<kR0, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
+}
+
+template <bool kR0, int NT, int PPT>
+__device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
+                                     uint32_t hi, unsigned long long* __restrict__ packed, unsigned long long* lds,
+                                     int* red) {
+    const BatchCtx bc = batch_ctx(c, p, vb, hi);
+    if (!kR0 && bc.rconst == 0) nearest_chunks<true, NT, PPT>(c, p, bc, lo, packed, lds, red);
+    else nearest_chunks<kR0, NT, PPT>(c, p, bc, lo, packed, lds, red);
 }
 
 // ---------------------------------------------------------------- resolves

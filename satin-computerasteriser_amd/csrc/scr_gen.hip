@@ -152,16 +152,10 @@ void scr_scene_params_default(scr_scene_params* p) {
     p->quantizeColors = 1;
 }
 
-int scr_gen_scene(scr_ctx* ctx, const scr_scene_params* prm, size_t count, float* positions4, float* colors4,
-                  int outputOnDevice) {
-    using namespace scr;
-    if (!ctx || !prm || (count && (!positions4 || !colors4))) return SCR_ERR_INVALID_ARGUMENT;
-    if (prm->halfExtent <= 0.0f || prm->octaves < 0 || prm->octaves > 16 || prm->numBoxes < 0 ||
-        !(prm->terrainFraction >= 0.0f && prm->terrainFraction <= 1.0f) || prm->boxMaxEdge < prm->boxMinEdge ||
-        prm->baseWavelength <= 0.0f)
-        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "bad scene parameters");
-    if (count == 0) return SCR_OK;
-    GenParams g;
+namespace {
+
+scr::GenParams gen_params(const scr_scene_params* prm, size_t count) {
+    scr::GenParams g;
     g.seed = prm->seed;
     g.half = prm->halfExtent;
     g.amplitude = prm->amplitude;
@@ -172,6 +166,27 @@ int scr_gen_scene(scr_ctx* ctx, const scr_scene_params* prm, size_t count, float
     g.quantize = prm->quantizeColors;
     g.count = count;
     g.nTerrain = prm->numBoxes > 0 ? (uint64_t)std::llround((double)prm->terrainFraction * (double)count) : count;
+    g.nFaces = 0;
+    return g;
+}
+
+}  // namespace
+
+float scr_scene_height(const scr_scene_params* prm, float x, float y) {
+    if (!prm || prm->baseWavelength <= 0.0f || prm->octaves < 0 || prm->octaves > 16) return 0.0f;
+    return scr::terrain_height(gen_params(prm, 0), x, y);
+}
+
+int scr_gen_scene(scr_ctx* ctx, const scr_scene_params* prm, size_t count, float* positions4, float* colors4,
+                  int outputOnDevice) {
+    using namespace scr;
+    if (!ctx || !prm || (count && (!positions4 || !colors4))) return SCR_ERR_INVALID_ARGUMENT;
+    if (prm->halfExtent <= 0.0f || prm->octaves < 0 || prm->octaves > 16 || prm->numBoxes < 0 ||
+        !(prm->terrainFraction >= 0.0f && prm->terrainFraction <= 1.0f) || prm->boxMaxEdge < prm->boxMinEdge ||
+        prm->baseWavelength <= 0.0f)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "bad scene parameters");
+    if (count == 0) return SCR_OK;
+    GenParams g = gen_params(prm, count);
     // boxes and their faces (host, deterministic in the seed)
     std::vector<Face> faces;
     std::vector<double> area;

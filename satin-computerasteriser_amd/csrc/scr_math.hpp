@@ -181,9 +181,20 @@ struct PassParams {
     float bg[4];
 };
 
+// Radius of a point of size ps (the tail of pointFootprintRadius,
+// Common.metal:52-60): non-finite sizes take lo, then clamp to [lo, hi];
+// radius = clamp(int(ceil(max(ps - 1, 0) * 0.5)), 0, 16).  Non-decreasing in ps.
+__host__ __device__ inline int radius_of_size(float ps, float lo, float hi) {
+    if (!__builtin_isfinite(ps)) ps = lo;
+    ps = fminf(fmaxf(ps, lo), hi);
+    int r = f2i(ceilf(fmaxf(ps - 1.0f, 0.0f) * 0.5f));
+    r = r < 0 ? 0 : r;
+    return r > 16 ? 16 : r;
+}
+
 // pointFootprintRadius — Common.metal:18-61 (world/view transform only when
 // the size depends on the point).
-__device__ inline int footprint_radius(const PassParams& p, const float* world, float x, float y, float z) {
+__host__ __device__ inline int footprint_radius(const PassParams& p, const float* world, float x, float y, float z) {
     if (p.constRadius >= 0) return p.constRadius;
     const f4 wp = mul_point(world, x, y, z);
     const f4 v = mul_point(p.view, wp.x, wp.y, wp.z);
@@ -203,19 +214,69 @@ __device__ inline int footprint_radius(const PassParams& p, const float* world, 
             ps = p.pointSizeScale * rcp(d);
         }
     }
-    if (!__builtin_isfinite(ps)) ps = p.lo;
-    ps = fminf(fmaxf(ps, p.lo), p.hi);
-    int r = f2i(ceilf(fmaxf(ps - 1.0f, 0.0f) * 0.5f));
-    r = r < 0 ? 0 : r;
-    return r > 16 ? 16 : r;
+    return radius_of_size(ps, p.lo, p.hi);
+}
+
+// Footprint radius shared by every point of a batch, or -1.
+//
+// In the perspective modes the size is k * rcp(d), d = max(|v|, 1e-6)
+// (screen space) or max(-v.z, 1e-6) (world space), v = view * world * point.
+// Bounding v over the batch box (the box's corners through view * world,
+// widened by 1e-4 of the magnitudes involved, which dwarfs the float error of
+// the per-point products, and d by a further 1e-4 relative) brackets the d
+// every point computes; rcp, a product with k >= 0 and the radius are all
+// non-decreasing or non-increasing in d, so equal radii at the two ends mean
+// that radius for every point of the batch.
+__host__ __device__ inline int batch_const_radius(const PassParams& p, const float* world, const scr_raster_batch& b) {
+    if (p.constRadius >= 0) return p.constRadius;
+    if (p.ortho) return -1;
+    const float k = p.pointSizeMode == 1 ? (2.0f * p.pointSizeScale) * p.focal : p.pointSizeScale;
+    if (!(k >= 0.0f) || !__builtin_isfinite(k)) return -1;
+    float m[16];
+    mat_mul(p.view, world, m);
+    const float bmin[3] = {b.minX, b.minY, b.minZ}, bmax[3] = {b.maxX, b.maxY, b.maxZ};
+    float lo[3], hi[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const float t = m[12 + i];
+        float l = t, h = t, mag = fabsf(t);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const float a = m[j * 4 + i] * bmin[j], c = m[j * 4 + i] * bmax[j];
+            l += fminf(a, c);
+            h += fmaxf(a, c);
+            mag += fmaxf(fabsf(a), fabsf(c));
+        }
+        const float margin = 1e-4f * mag;
+        lo[i] = l - margin;
+        hi[i] = h + margin;
+    }
+    float dmin, dmax;
+    if (p.pointSizeMode == 1) {
+        dmin = -hi[2];
+        dmax = -lo[2];
+    } else {
+        float n2 = 0.0f, f2 = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float near = fmaxf(0.0f, fmaxf(lo[i], -hi[i]));
+            const float far = fmaxf(fabsf(lo[i]), fabsf(hi[i]));
+            n2 += near * near;
+            f2 += far * far;
+        }
+        dmin = sqrtf(n2);
+        dmax = sqrtf(f2);
+    }
+    dmin = fmaxf(dmin * (1.0f - 1e-4f), 0.000001f);
+    dmax = fmaxf(dmax * (1.0f + 1e-4f), 0.000001f);
+    if (!__builtin_isfinite(dmin) || !__builtin_isfinite(dmax)) return -1;
+    const int rNear = radius_of_size(k * rcp(dmin), p.lo, p.hi);
+    const int rFar = radius_of_size(k * rcp(dmax), p.lo, p.hi);
+    return rNear == rFar ? rNear : -1;
 }
 
 // Host-side radius for lo == hi (size clamps to lo for every point).
-__host__ inline int const_radius_for(float lo) {
-    int r = f2i(ceilf(fmaxf(lo - 1.0f, 0.0f) * 0.5f));
-    r = r < 0 ? 0 : r;
-    return r > 16 ? 16 : r;
-}
+__host__ inline int const_radius_for(float lo) { return radius_of_size(lo, lo, lo); }
 
 // insidePointFootprint — Common.metal:63-71
 __host__ __device__ inline bool inside_footprint(int ox, int oy, int r) {

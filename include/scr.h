@@ -335,13 +335,15 @@ int scr_diff_backward(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
 typedef struct scr_multiview scr_multiview;
 
 /* Draw-kernel shape for one phase: threads per workgroup (128, 256 or 512),
- * points per thread (4, 8 or 16; threads * points <= 4096), and the number of
- * work items each visible batch is split into (1, 2 or 4). */
+ * points per thread (4, 8 or 16; threads * points <= 4096), the number of
+ * work items each visible batch is split into (1, 2 or 4), and whether each
+ * wave draws its own chunk with a private LDS tile (1) or the workgroup
+ * shares one chunk and tile (0). */
 typedef struct scr_mv_shape {
     int32_t threads;
     int32_t pointsPerThread;
     int32_t splits;
-    int32_t reserved;
+    int32_t waveTiles;
 } scr_mv_shape;
 
 typedef struct scr_mv_tuning {

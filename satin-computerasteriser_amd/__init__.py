@@ -127,7 +127,7 @@ class CPassTime(C.Structure):
 
 class CMVShape(C.Structure):
     _fields_ = [("threads", C.c_int32), ("pointsPerThread", C.c_int32), ("splits", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("waveTiles", C.c_int32)]
 
 
 class CMVTuning(C.Structure):
@@ -761,7 +761,8 @@ MV_SPLITS = (1, 2, 4)
 
 def _tuning_to_dict(t: CMVTuning) -> dict:
     return {ph: {"threads": getattr(t, ph).threads, "pointsPerThread": getattr(t, ph).pointsPerThread,
-                 "splits": getattr(t, ph).splits} for ph in ("depth", "color", "nearest")}
+                 "splits": getattr(t, ph).splits, "waveTiles": getattr(t, ph).waveTiles}
+            for ph in ("depth", "color", "nearest")}
 
 
 class MultiViewRasteriser:
@@ -833,6 +834,7 @@ class MultiViewRasteriser:
             dst.threads = int(sh.get("threads", dst.threads))
             dst.pointsPerThread = int(sh.get("pointsPerThread", dst.pointsPerThread))
             dst.splits = int(sh.get("splits", dst.splits))
+            dst.waveTiles = int(sh.get("waveTiles", dst.waveTiles))
         self.ctx.check(self.ctx.lib.scr_multiview_set_tuning(self.handle, C.byref(t)))
 
     def read(self, view: int, which: int) -> np.ndarray:

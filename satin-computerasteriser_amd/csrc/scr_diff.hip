@@ -49,7 +49,7 @@ __device__ inline void diff_color_chunks(const CloudDev& c, const PassParams& p,
         const uint32_t li0 = cb + threadIdx.x * kPPT;
         ChunkFrags<kPPT> f;
         chunk_front<false, kR0, kPPT>(c, p, bc, li0, vec, f);
-        const ChunkBox box = chunk_box<kThreads, kPPT>(f, p, kDiffTile, L.red);
+        const ChunkBox box = chunk_box<false, kThreads, kPPT>(f, p, kDiffTile, L.red);
         if (box.frags == 0) continue;
         const int area = box.w * box.h;
         if (box.lds) {

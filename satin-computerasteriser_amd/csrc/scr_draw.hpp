@@ -180,40 +180,50 @@ __device__ inline void load_words(const uint32_t* src, uint32_t base, uint32_t* 
     }
 }
 
-// Loads the position words (and levels) of the thread's PPT points and runs
-// the front end of every point pass on each of them: dither/LOD skip,
-// decodePoint (Common.metal:73-117), projection, clip and viewport tests,
-// y flip, footprint radius.
-template <bool kNearestZ, bool kR0, int PPT>
-__device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
-                                   bool vec, ChunkFrags<PPT>& f) {
-    uint32_t lo[PPT], me[PPT], hi[PPT], lv[PPT];
+// The raw words of a thread's PPT points: positions at the batch precision,
+// levels packed four to a word.
+template <int PPT>
+struct ChunkRaw {
+    uint32_t lo[PPT], me[PPT], hi[PPT], lv4[(PPT + 3) / 4];
+};
+
+// Issues the loads of the thread's PPT points starting at local index li0.
+template <int PPT>
+__device__ inline void chunk_fetch(const CloudDev& c, const BatchCtx& bc, uint32_t li0, bool vec, ChunkRaw<PPT>& r) {
     const uint32_t base = bc.first + li0;
     const uint32_t n = li0 < bc.active ? min((uint32_t)PPT, bc.active - li0) : 0u;
     if (vec && n == (uint32_t)PPT) {
         // base is a multiple of PPT: 16-byte aligned words, PPT-byte aligned levels
-        load_words<PPT>(c.xyzLow, base, lo);
-        if (bc.level <= 1) load_words<PPT>(c.xyzMed, base, me);
-        if (bc.level == 0) load_words<PPT>(c.xyzHigh, base, hi);
+        load_words<PPT>(c.xyzLow, base, r.lo);
+        if (bc.level <= 1) load_words<PPT>(c.xyzMed, base, r.me);
+        if (bc.level == 0) load_words<PPT>(c.xyzHigh, base, r.hi);
         if (!bc.keepAll) {
 #pragma unroll
-            for (int q = 0; q < PPT / 4; q++) {
-                const uint32_t l4 = reinterpret_cast<const uint32_t*>(c.levels + base)[q];
-#pragma unroll
-                for (int j = 0; j < 4; j++) lv[4 * q + j] = (l4 >> (8 * j)) & 0xffu;
-            }
+            for (int q = 0; q < PPT / 4; q++) r.lv4[q] = reinterpret_cast<const uint32_t*>(c.levels + base)[q];
         }
     } else {
+#pragma unroll
+        for (int q = 0; q < (PPT + 3) / 4; q++) r.lv4[q] = 0u;
 #pragma unroll
         for (int j = 0; j < PPT; j++) {
             const uint32_t pi = base + (uint32_t)j;
             const bool ok = (uint32_t)j < n;
-            lo[j] = ok ? c.xyzLow[pi] : 0u;
-            me[j] = (ok && bc.level <= 1) ? c.xyzMed[pi] : 0u;
-            hi[j] = (ok && bc.level == 0) ? c.xyzHigh[pi] : 0u;
-            lv[j] = (ok && !bc.keepAll) ? (uint32_t)c.levels[pi] : 0u;
+            r.lo[j] = ok ? c.xyzLow[pi] : 0u;
+            r.me[j] = (ok && bc.level <= 1) ? c.xyzMed[pi] : 0u;
+            r.hi[j] = (ok && bc.level == 0) ? c.xyzHigh[pi] : 0u;
+            if (ok && !bc.keepAll) r.lv4[j / 4] |= (uint32_t)c.levels[pi] << (8 * (j % 4));
         }
     }
+}
+
+// Runs the front end of every point pass on the fetched points: dither/LOD
+// skip, decodePoint (Common.metal:73-117), projection, clip and viewport
+// tests, y flip, footprint radius.
+template <bool kNearestZ, bool kR0, int PPT>
+__device__ inline void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
+                                    const ChunkRaw<PPT>& r, ChunkFrags<PPT>& f) {
+    const uint32_t base = bc.first + li0;
+    const uint32_t n = li0 < bc.active ? min((uint32_t)PPT, bc.active - li0) : 0u;
     // Branch-free per point: every test of the reference folds into `ok`
     // (each written as the negation of its reject condition, so NaNs behave
     // as in the Metal source) and the fragment is kept or dropped once.
@@ -222,19 +232,19 @@ __device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const
         const uint32_t pi = base + (uint32_t)j;
         bool ok = (uint32_t)j < n;
         if (!bc.keepAll) {
-            const float pointLevel = (float)(lv[j] & 7u);
+            const float pointLevel = (float)((r.lv4[j / 4] >> (8 * (j % 4))) & 7u);
             const float dither = (p.lodDither != 0) ? hash_unit(pi) : 0.5f;
             ok = ok && !(dither >= (bc.lodThreshold - pointLevel) + 0.5f);
         }
         uint32_t qx, qy, qz;
-        const uint32_t l = lo[j];
+        const uint32_t l = r.lo[j];
         if (bc.level == 0) {
-            const uint32_t m = me[j], h = hi[j];
+            const uint32_t m = r.me[j], h = r.hi[j];
             qx = ((l & 1023u) << 20) | ((m & 1023u) << 10) | (h & 1023u);
             qy = (((l >> 10) & 1023u) << 20) | (((m >> 10) & 1023u) << 10) | ((h >> 10) & 1023u);
             qz = (((l >> 20) & 1023u) << 20) | (((m >> 20) & 1023u) << 10) | ((h >> 20) & 1023u);
         } else if (bc.level == 1) {
-            const uint32_t m = me[j];
+            const uint32_t m = r.me[j];
             qx = ((l & 1023u) << 10) | (m & 1023u);
             qy = (((l >> 10) & 1023u) << 10) | ((m >> 10) & 1023u);
             qz = (((l >> 20) & 1023u) << 10) | ((m >> 20) & 1023u);
@@ -261,6 +271,14 @@ __device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const
         f.z[j] = nz;
         f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, x, y, z));
     }
+}
+
+template <bool kNearestZ, bool kR0, int PPT>
+__device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
+                                   bool vec, ChunkFrags<PPT>& f) {
+    ChunkRaw<PPT> r;
+    chunk_fetch<PPT>(c, bc, li0, vec, r);
+    chunk_decode<kNearestZ, kR0, PPT>(c, p, bc, li0, r, f);
 }
 
 // ---------------------------------------------------------------- chunk rectangle
@@ -296,9 +314,30 @@ __device__ inline int wave_sum(int v) {
     return v;
 }
 
-// Block-wide reduction; red holds 5 * kMaxWaves ints.  Ends with a barrier,
-// so the caller may reuse LDS right after.
-template <int NT, int PPT>
+// The threads that share one chunk and its LDS tile: the whole workgroup
+// (block barriers), or each wave on its own (a workgroup then works on NT/64
+// chunks at once, with a private slice of the tile each and no block
+// barriers; a wave's LDS operations complete in issue order, so a fence that
+// keeps the compiler from reordering them is all the synchronisation needed).
+template <bool kWave, int NT>
+struct Group {
+    static constexpr int kSize = kWave ? 64 : NT;
+    static constexpr int kCount = kWave ? NT / 64 : 1;
+    __device__ static int rank() { return kWave ? (int)__lane_id() : (int)threadIdx.x; }
+    __device__ static int index() { return kWave ? (int)(threadIdx.x >> 6) : 0; }
+    __device__ static void sync() {
+        if (kWave) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            __syncthreads();
+        }
+    }
+};
+
+// Rectangle reduction over the group; red holds 5 * kMaxWaves ints (block
+// groups only).  Ends with a barrier, so the caller may reuse LDS right after.
+template <bool kWave, int NT, int PPT>
 __device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams& p, int cap, int* red) {
     int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1, n = 0;
 #pragma unroll
@@ -315,30 +354,32 @@ __device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams&
     x1 = wave_max(x1);
     y1 = wave_max(y1);
     n = wave_sum(n);
-    constexpr int kWaves = NT / 64;
+    constexpr int kWaves = kWave ? 1 : NT / 64;
     const int w = threadIdx.x >> 6;
-    if (__lane_id() == 0) {
+    if (!kWave && __lane_id() == 0) {
         red[w] = x0;
         red[kWaves + w] = y0;
         red[2 * kWaves + w] = x1;
         red[3 * kWaves + w] = y1;
         red[4 * kWaves + w] = n;
     }
-    __syncthreads();
-    x0 = red[0];
-    y0 = red[kWaves];
-    x1 = red[2 * kWaves];
-    y1 = red[3 * kWaves];
-    n = red[4 * kWaves];
+    if (!kWave) {
+        __syncthreads();
+        x0 = red[0];
+        y0 = red[kWaves];
+        x1 = red[2 * kWaves];
+        y1 = red[3 * kWaves];
+        n = red[4 * kWaves];
 #pragma unroll
-    for (int k = 1; k < kWaves; k++) {
-        x0 = min(x0, red[k]);
-        y0 = min(y0, red[kWaves + k]);
-        x1 = max(x1, red[2 * kWaves + k]);
-        y1 = max(y1, red[3 * kWaves + k]);
-        n += red[4 * kWaves + k];
+        for (int k = 1; k < kWaves; k++) {
+            x0 = min(x0, red[k]);
+            y0 = min(y0, red[kWaves + k]);
+            x1 = max(x1, red[2 * kWaves + k]);
+            y1 = max(y1, red[3 * kWaves + k]);
+            n += red[4 * kWaves + k];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     ChunkBox b;
     b.frags = n;
     b.x0 = max(x0, 0);
@@ -356,21 +397,24 @@ __device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams&
 
 // ---------------------------------------------------------------- depth pass body
 // depthPassUpdate — DepthPass/Shaders.metal:21-116, for the local points
-// [lo, hi) of one visible batch (lo a multiple of NT * PPT).
-template <bool kR0, int NT, int PPT>
+// [lo, hi) of one visible batch (lo a multiple of NT * PPT).  The group's
+// chunks start at lo + group * group size * PPT and step by NT * PPT.
+template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
-                                    uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
+                                    uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red) {
+    using G = Group<kWave, NT>;
+    constexpr int kCap = kDepthTile / G::kCount;
+    uint32_t* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
-    constexpr int kChunk = NT * PPT;
-    for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
+    for (uint32_t cb = lo + (uint32_t)(G::index() * G::kSize * PPT); cb < bc.active; cb += NT * PPT) {
         ChunkFrags<PPT> f;
-        chunk_front<false, kR0, PPT>(c, p, bc, cb + threadIdx.x * PPT, vec, f);
-        const ChunkBox box = chunk_box<NT, PPT>(f, p, kDepthTile, red);
+        chunk_front<false, kR0, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
+        const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
         if (box.frags == 0) continue;
         const int area = box.w * box.h;
         if (box.lds) {
-            for (int i = threadIdx.x; i < area; i += NT) lds[i] = 0u;
-            __syncthreads();
+            for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0u;
+            G::sync();
         }
         // single-pixel points: a thread's consecutive points often share a
         // pixel (Morton order); a max the thread has already issued there
@@ -401,8 +445,8 @@ __device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, cons
                 }
         }
         if (box.lds) {
-            __syncthreads();
-            for (int i = threadIdx.x; i < area; i += NT) {
+            G::sync();
+            for (int i = G::rank(); i < area; i += G::kSize) {
                 const uint32_t d = lds[i];
                 if (d == 0u) continue;
                 uint32_t* g = depth + box.pixel(i, p.width);
@@ -410,7 +454,7 @@ __device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, cons
                 // cause a redundant atomic, never a missed one
                 if (d > *g) atomicMax(g, d);
             }
-            __syncthreads();
+            G::sync();
         }
     }
 }
@@ -439,14 +483,17 @@ __device__ inline void color_flush(bool lds, int k, unsigned long long rg, unsig
     }
 }
 
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
-                                    uint32_t* blue, uint32_t* count, const ColorLds& L) {
+                                    uint32_t* blue, uint32_t* count, const ColorLds& LAll) {
+    using G = Group<kWave, NT>;
+    constexpr int kCap = kColorTile / G::kCount;
+    const int slice = G::index() * kCap;
+    const ColorLds L{LAll.depth + slice, LAll.rg + slice, LAll.bn + slice, LAll.red};
     const bool vec = vec_ok<PPT>(bc);
-    constexpr int kChunk = NT * PPT;
-    for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
-        const uint32_t li0 = cb + threadIdx.x * PPT;
+    for (uint32_t cb = lo + (uint32_t)(G::index() * G::kSize * PPT); cb < bc.active; cb += NT * PPT) {
+        const uint32_t li0 = cb + G::rank() * PPT;
         ChunkFrags<PPT> f;
         chunk_front<false, kR0, PPT>(c, p, bc, li0, vec, f);
         // colour stage (ColorPass/Shaders.metal:95-133): packed r | g << 8 | b << 16
@@ -484,16 +531,16 @@ __device__ inline void color_chunks(const CloudDev& c, const PassParams& p, cons
             }
             rgb[j] = r8 | (g8 << 8) | (b8 << 16);
         }
-        const ChunkBox box = chunk_box<NT, PPT>(f, p, kColorTile, L.red);
+        const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
         if (box.frags == 0) continue;
         const int area = box.w * box.h;
         if (box.lds) {
-            for (int i = threadIdx.x; i < area; i += NT) {
+            for (int i = G::rank(); i < area; i += G::kSize) {
                 L.depth[i] = depth[box.pixel(i, p.width)];
                 L.rg[i] = 0ull;
                 L.bn[i] = 0ull;
             }
-            __syncthreads();
+            G::sync();
         }
         // single-pixel points: runs of a thread's consecutive fragments on one
         // pixel (Morton order) are summed in registers and added once
@@ -545,10 +592,10 @@ __device__ inline void color_chunks(const CloudDev& c, const PassParams& p, cons
         }
         if (kR0 && runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
         if (box.lds) {
-            __syncthreads();
-            // per-chunk sums stay below 2^32 (<= kChunk fragments x 255 per
+            G::sync();
+            // per-chunk sums stay below 2^32 (<= NT * PPT fragments x 255 per
             // pixel), so the packed halves never carry into each other
-            for (int i = threadIdx.x; i < area; i += NT) {
+            for (int i = G::rank(); i < area; i += G::kSize) {
                 const unsigned long long bn = L.bn[i];
                 const uint32_t n = (uint32_t)(bn >> 32);
                 if (n == 0u) continue;
@@ -559,7 +606,7 @@ __device__ inline void color_chunks(const CloudDev& c, const PassParams& p, cons
                 atomicAdd(blue + pix, (uint32_t)bn);
                 atomicAdd(count + pix, n);
             }
-            __syncthreads();
+            G::sync();
         }
     }
 }
@@ -567,21 +614,23 @@ __device__ inline void color_chunks(const CloudDev& c, const PassParams& p, cons
 // ---------------------------------------------------------------- nearest pass body
 // nearestDepthUpdate + nearestIndexUpdate (NearestDepth/Shaders.metal:14-91,
 // NearestIndex/Shaders.metal:14-94) as one max of depth << 32 | ~index.
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
-                                      unsigned long long* __restrict__ packed, unsigned long long* lds, int* red) {
+                                      unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red) {
+    using G = Group<kWave, NT>;
+    constexpr int kCap = kNearestTile / G::kCount;
+    unsigned long long* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
-    constexpr int kChunk = NT * PPT;
-    for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
-        const uint32_t li0 = cb + threadIdx.x * PPT;
+    for (uint32_t cb = lo + (uint32_t)(G::index() * G::kSize * PPT); cb < bc.active; cb += NT * PPT) {
+        const uint32_t li0 = cb + G::rank() * PPT;
         ChunkFrags<PPT> f;
         chunk_front<true, kR0, PPT>(c, p, bc, li0, vec, f);
-        const ChunkBox box = chunk_box<NT, PPT>(f, p, kNearestTile, red);
+        const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
         if (box.frags == 0) continue;
         const int area = box.w * box.h;
         if (box.lds) {
-            for (int i = threadIdx.x; i < area; i += NT) lds[i] = 0ull;
-            __syncthreads();
+            for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0ull;
+            G::sync();
         }
         int lastK = -1;
         unsigned long long lastKey = 0ull;
@@ -614,14 +663,14 @@ __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, co
                 }
         }
         if (box.lds) {
-            __syncthreads();
-            for (int i = threadIdx.x; i < area; i += NT) {
+            G::sync();
+            for (int i = G::rank(); i < area; i += G::kSize) {
                 const unsigned long long k = lds[i];
                 if (k == 0ull) continue;
                 unsigned long long* g = packed + box.pixel(i, p.width);
                 if (k > *g) atomicMax(g, k);
             }
-            __syncthreads();
+            G::sync();
         }
     }
 }
@@ -629,30 +678,30 @@ __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, co
 // ---------------------------------------------------------------- per-batch entry points
 // A batch whose points all have radius 0 runs the single-pixel code.
 
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                    uint32_t hi, uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
-    if (!kR0 && bc.rconst == 0) depth_chunks<true, NT, PPT>(c, p, bc, lo, depth, lds, red);
-    else depth_chunks<kR0, NT, PPT>(c, p, bc, lo, depth, lds, red);
+    if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT>(c, p, bc, lo, depth, lds, red);
+    else depth_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, depth, lds, red);
 }
 
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                    uint32_t hi, const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                    uint32_t* blue, uint32_t* count, const ColorLds& L) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
-    if (!kR0 && bc.rconst == 0) color_chunks<true, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
-    else color_chunks<kR0, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
+    if (!kR0 && bc.rconst == 0) color_chunks<true, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
+    else color_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
 }
 
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                      uint32_t hi, unsigned long long* __restrict__ packed, unsigned long long* lds,
                                      int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
-    if (!kR0 && bc.rconst == 0) nearest_chunks<true, NT, PPT>(c, p, bc, lo, packed, lds, red);
-    else nearest_chunks<kR0, NT, PPT>(c, p, bc, lo, packed, lds, red);
+    if (!kR0 && bc.rconst == 0) nearest_chunks<true, kWave, NT, PPT>(c, p, bc, lo, packed, lds, red);
+    else nearest_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, packed, lds, red);
 }
 
 // ---------------------------------------------------------------- resolves

@@ -113,18 +113,18 @@ __device__ inline void drain(const MVView* __restrict__ views, const MVItem* __r
     }
 }
 
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT) void k_mv_depth(const MVView* __restrict__ views, const MVItem* __restrict__ items,
                                                  uint32_t nUnits, int splits, uint32_t* heads) {
     __shared__ uint32_t lds[kDepthTile];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
                    [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       depth_batch<kR0, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, lds, red);
+                       depth_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, lds, red);
                    });
 }
 
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
                                                  uint32_t nUnits, int splits, uint32_t* heads) {
     __shared__ uint32_t ldsDepth[kColorTile];
@@ -135,12 +135,12 @@ __global__ __launch_bounds__(NT) void k_mv_color(const MVView* __restrict__ view
     drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
                    [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
                        const size_t s = (size_t)V.stride;
-                       color_batch<kR0, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
+                       color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
                                                  V.planes + 3 * s, V.planes + 4 * s, L);
                    });
 }
 
-template <bool kR0, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT) void k_mv_nearest(const MVView* __restrict__ views,
                                                    const MVItem* __restrict__ items, uint32_t nUnits, int splits,
                                                    uint32_t* heads) {
@@ -148,7 +148,7 @@ __global__ __launch_bounds__(NT) void k_mv_nearest(const MVView* __restrict__ vi
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
                    [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       nearest_batch<kR0, NT, PPT>(V.c, V.p, vb, lo, hi, V.nearest, lds, red);
+                       nearest_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.nearest, lds, red);
                    });
 }
 
@@ -174,15 +174,15 @@ using DrawKernel = void (*)(const MVView*, const MVItem*, uint32_t, int, uint32_
 
 struct Variant {
     int threads, ppt;
-    DrawKernel depth[2], color[2], nearest[2];  // [kR0]
+    DrawKernel depth[2][2], color[2][2], nearest[2][2];  // [kR0][kWave]
 };
 
-#define SCR_MV_VARIANT(NT, PPT)                                                                        \
-    Variant {                                                                                          \
-        NT, PPT, {k_mv_depth<false, NT, PPT>, k_mv_depth<true, NT, PPT>},                              \
-            {k_mv_color<false, NT, PPT>, k_mv_color<true, NT, PPT>}, {                                 \
-            k_mv_nearest<false, NT, PPT>, k_mv_nearest<true, NT, PPT>                                  \
-        }                                                                                              \
+#define SCR_MV_KERNELS(K, NT, PPT) \
+    {{K<false, false, NT, PPT>, K<false, true, NT, PPT>}, {K<true, false, NT, PPT>, K<true, true, NT, PPT>}}
+#define SCR_MV_VARIANT(NT, PPT)                                                                     \
+    Variant {                                                                                       \
+        NT, PPT, SCR_MV_KERNELS(k_mv_depth, NT, PPT), SCR_MV_KERNELS(k_mv_color, NT, PPT),          \
+            SCR_MV_KERNELS(k_mv_nearest, NT, PPT)                                                   \
     }
 
 const Variant kVariants[] = {
@@ -199,14 +199,15 @@ const Variant* find_variant(const scr_mv_shape& s) {
 }
 
 bool valid_shape(const scr_mv_shape& s) {
-    return find_variant(s) && (s.splits == 1 || s.splits == 2 || s.splits == 4);
+    return find_variant(s) && (s.splits == 1 || s.splits == 2 || s.splits == 4) &&
+           (s.waveTiles == 0 || s.waveTiles == 1);
 }
 
 enum Phase { kDepth = 0, kColor = 1, kNearest = 2 };
 
-DrawKernel kernel_of(const Variant& v, Phase ph, bool r0) {
-    const int k = r0 ? 1 : 0;
-    return ph == kDepth ? v.depth[k] : (ph == kColor ? v.color[k] : v.nearest[k]);
+DrawKernel kernel_of(const Variant& v, Phase ph, bool r0, bool wave) {
+    const int k = r0 ? 1 : 0, w = wave ? 1 : 0;
+    return ph == kDepth ? v.depth[k][w] : (ph == kColor ? v.color[k][w] : v.nearest[k][w]);
 }
 
 unsigned grid_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
@@ -341,7 +342,7 @@ int enqueue(scr_multiview* mv, int mode, bool r0, uint32_t nItems, int timedPhas
                                                                                             : mv->tuning.nearest);
         const scr::Variant* var = scr::find_variant(sh);
         if (!var) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unsupported draw shape");
-        const scr::DrawKernel kern = scr::kernel_of(*var, ph, r0);
+        const scr::DrawKernel kern = scr::kernel_of(*var, ph, r0, sh.waveTiles != 0);
         const uint32_t nUnits = nItems * (uint32_t)sh.splits;
         unsigned grid = 1;
         int rc = resident_grid(mv, kern, var->threads, nUnits, &grid);
@@ -439,8 +440,8 @@ int stage_params(scr_multiview* mv, const scr_config* cfg, const scr_camera* cam
 int check_tuning(scr_ctx* ctx, const scr_mv_tuning& t) {
     if (!scr::valid_shape(t.depth) || !scr::valid_shape(t.color) || !scr::valid_shape(t.nearest))
         return set_error(ctx, SCR_ERR_INVALID_ARGUMENT,
-                         "draw shape must be threads/pointsPerThread in {128/8, 256/4, 256/8, 256/16, 512/4, 512/8} "
-                         "and splits in {1, 2, 4}");
+                         "draw shape must be threads/pointsPerThread in {128/8, 256/4, 256/8, 256/16, 512/4, 512/8}, "
+                         "splits in {1, 2, 4} and waveTiles in {0, 1}");
     return SCR_OK;
 }
 
@@ -582,8 +583,9 @@ int scr_multiview_autotune(scr_multiview* mv, const scr_config* cfg, const scr_c
         scr_mv_shape bestShape = *slot;
         float bestMs = 1e30f;
         for (int vi = 0; vi < scr::kNumVariants && rc == SCR_OK; vi++)
+            for (int wave = 0; wave < 2; wave++)
             for (int split : scr::kSplits) {
-                *slot = scr_mv_shape{scr::kVariants[vi].threads, scr::kVariants[vi].ppt, split, 0};
+                *slot = scr_mv_shape{scr::kVariants[vi].threads, scr::kVariants[vi].ppt, split, wave};
                 float ms = 1e30f;
                 for (int r = 0; r < reps && rc == SCR_OK; r++) {
                     uint32_t nItems = 0;

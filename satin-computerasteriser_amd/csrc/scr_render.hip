@@ -21,6 +21,7 @@ using namespace draw;
 
 constexpr int kThreads = 256;   // draw workgroup
 constexpr int kPPT = 8;         // points per thread
+constexpr bool kWaveTiles = false;  // chunks and LDS tiles shared by the workgroup (measured faster than per-wave tiles)
 
 __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -40,7 +41,7 @@ __global__ __launch_bounds__(kThreads) void k_depth(CloudDev c, PassParams p, ui
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        depth_batch<kR0, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, depth, lds, red);
+        depth_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, depth, lds, red);
 }
 
 template <bool kR0>
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(kThreads) void k_color(CloudDev c, PassParams p, co
     const ColorLds L{ldsDepth, ldsRG, ldsBC, redbuf};
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        color_batch<kR0, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, depth, red, green, blue, count, L);
+        color_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, depth, red, green, blue, count, L);
 }
 
 __global__ __launch_bounds__(256) void k_resolve(PassParams p, const uint32_t* __restrict__ planes, size_t stride,
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kThreads) void k_nearest(CloudDev c, PassParams p,
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        nearest_batch<kR0, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, packed, lds, red);
+        nearest_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, packed, lds, red);
 }
 
 __global__ __launch_bounds__(256) void k_nearest_resolve(PassParams p, const unsigned long long* __restrict__ packed,

@@ -133,10 +133,11 @@ def test_multiview_every_shape_gives_the_same_planes(ctx, scene):
             mv = scr.MultiViewRasteriser(ctx, scene[0], sizes, which, cfg)
             for threads, ppt in scr.MV_SHAPES:
                 for splits in scr.MV_SPLITS:
-                    sh = {"threads": threads, "pointsPerThread": ppt, "splits": splits}
-                    mv.tuning = {"depth": sh, "color": sh, "nearest": sh}
-                    mv.render(cams, useGraph=(splits == 2))
-                    _assert_same(_batched(mv, planes), want)
+                    for wave in (0, 1):
+                        sh = {"threads": threads, "pointsPerThread": ppt, "splits": splits, "waveTiles": wave}
+                        mv.tuning = {"depth": sh, "color": sh, "nearest": sh}
+                        mv.render(cams, useGraph=(splits == 2))
+                        _assert_same(_batched(mv, planes), want)
             mv.close()
 
 
@@ -148,7 +149,7 @@ def test_multiview_autotune_installs_a_valid_shape(ctx, scene):
     assert t == mv.tuning
     for ph in ("depth", "color"):
         assert (t[ph]["threads"], t[ph]["pointsPerThread"]) in scr.MV_SHAPES
-        assert t[ph]["splits"] in scr.MV_SPLITS
+        assert t[ph]["splits"] in scr.MV_SPLITS and t[ph]["waveTiles"] in (0, 1)
     want = _per_view(ctx, scene, cfg, cams, sizes, which, HQS_PLANES)
     mv.render(cams)
     _assert_same(_batched(mv, HQS_PLANES), want)

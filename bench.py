@@ -251,39 +251,45 @@ def diff_bench(scr, ctx, steps, fit_steps=200, n_points=5_000_000, n_views=32, s
     ctr, diag = cloud_bounds(scr, cloud)
     cams, _ = scr.sphere_views(ctr, 1.2 * diag, n_views, 0x5C0005, resolutions=((size, size),))
     cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1.0, maximumPointSize=3.0, pointSizeScale=1500.0)
-    r = scr.DifferentiableHQS(ctx, cloud, [(size, size)] * n_views, cfg)
+    r = scr.DifferentiableHQS(ctx, cloud, [(size, size)] * n_views, cfg, batched=True)
+    r1 = scr.DifferentiableHQS(ctx, cloud, [(size, size)] * n_views, cfg)
     del pos, col, src
 
-    # kernel throughput: forward + backward of every view, device-resident buffers
+    # kernel throughput: forward + backward of every view, device-resident buffers,
+    # batched (one launch per phase for all views) and one view at a time
     params = scr.DeviceBuffer.from_array(ctx, np.full((npool, 4), 0.5, np.float32))
-    img = scr.DeviceBuffer(ctx, size * size * 12)
+    imgs = [scr.DeviceBuffer(ctx, size * size * 12) for _ in range(n_views)]
     dl = scr.DeviceBuffer.from_array(ctx, np.random.default_rng(5).normal(size=size * size * 3).astype(np.float32))
     grad = scr.DeviceBuffer(ctx, npool * 32)
 
-    def fwd_bwd():
-        for v, cam in enumerate(cams):
-            r.forward(v, cam, params.ptr, img.ptr)
-            r.backward(v, cam, dl.ptr, grad.ptr)
+    def fwd_bwd(rr):
+        rr.forward_all(cams, params.ptr, [b.ptr for b in imgs])
+        rr.backward_all(cams, [dl.ptr] * n_views, grad.ptr)
 
     processed = 0
     for v, cam in enumerate(cams):
-        r.forward(v, cam, params.ptr, img.ptr)
+        r1.forward(v, cam, params.ptr, imgs[v].ptr)
         processed += int(cloud.readVisible()["activePoints"].astype(np.int64).sum())
-    fwd_bwd()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        fwd_bwd()
-    ctx.sync()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    ctx.profile(True)
-    fwd_bwd()
-    prof = {k_: round(v_[0] / v_[1], 4) for k_, v_ in ctx.collect_profile().items()}
-    ctx.profile(False)
-    out = {"points": n_points, "views": n_views, "size": size, "processed_points_per_pass": processed,
-           "ms_fwd_bwd_all_views": round(ms, 4), "Gpts_per_s_fwd_bwd": round(processed / (ms * 1e-3) / 1e9, 3),
-           "pass_ms": prof}
-    params.free(); img.free(); dl.free(); grad.free()
+    out = {"points": n_points, "views": n_views, "size": size, "processed_points_per_pass": processed}
+    for name, rr in (("batched", r), ("per_view", r1)):
+        fwd_bwd(rr)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fwd_bwd(rr)
+        ctx.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.profile(True)
+        fwd_bwd(rr)
+        prof = {k_: round(v_[0] / v_[1], 4) for k_, v_ in ctx.collect_profile().items()}
+        ctx.profile(False)
+        out[name] = {"ms_fwd_bwd_all_views": round(ms, 4), "Gpts_per_s_fwd_bwd": round(processed / (ms * 1e-3) / 1e9, 3),
+                     "pass_ms": prof}
+    out["Gpts_per_s_fwd_bwd"] = out["batched"]["Gpts_per_s_fwd_bwd"]
+    params.free(); dl.free(); grad.free()
+    for b in imgs:
+        b.free()
+    r1.close()
 
     try:
         import torch

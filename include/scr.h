@@ -366,6 +366,18 @@ int scr_multiview_set_tuning(scr_multiview* mv, const scr_mv_tuning* tuning);
  * it.  Results do not depend on the shape, only the time does. */
 int scr_multiview_autotune(scr_multiview* mv, const scr_config* cfg, const scr_camera* cameras, int reps,
                            scr_mv_tuning* out);
+/* Differentiable HQS (see scr_diff_render) for every view at once; the
+ * multi-view renderer must have exactly one cloud.  images[v]: device, 3
+ * floats per pixel of frame v.  Results equal per-view scr_diff_render calls
+ * bit for bit. */
+int scr_multiview_diff_render(scr_multiview* mv, const scr_config* cfg, const scr_camera* cameras,
+                              const float* params, float* const* images);
+/* Backward of scr_multiview_diff_render for the same cameras: grad (device,
+ * 4 doubles per pool point) += the gradients of all views.  Each point's
+ * views are added in view order by one thread, so the result equals per-view
+ * scr_diff_backward calls made in view order, bit for bit. */
+int scr_multiview_diff_backward(scr_multiview* mv, const scr_config* cfg, const scr_camera* cameras,
+                                const float* const* dLdImages, double* grad);
 
 #ifdef __cplusplus
 }

@@ -183,6 +183,8 @@ _SIGNATURES = {
     "scr_multiview_get_tuning": (_I, [_VP, C.POINTER(CMVTuning)]),
     "scr_multiview_set_tuning": (_I, [_VP, C.POINTER(CMVTuning)]),
     "scr_multiview_autotune": (_I, [_VP, C.POINTER(CConfig), _VP, _I, C.POINTER(CMVTuning)]),
+    "scr_multiview_diff_render": (_I, [_VP, C.POINTER(CConfig), _VP, _VP, _VP]),
+    "scr_multiview_diff_backward": (_I, [_VP, C.POINTER(CConfig), _VP, _VP, _VP]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -685,16 +687,27 @@ class DifferentiableHQS:
     """Differentiable HQS over one cloud (scr_diff_render / scr_diff_backward).
 
     One frame per view keeps the forward's depth / count planes for the
-    backward.  Pointers are device addresses: params float4 per pool point,
-    image 3 floats per pixel, dLdImage likewise, grad 4 doubles per pool
-    point (accumulated)."""
+    backward.  batched=True renders and differentiates all views with one
+    launch per phase (scr_multiview_diff_*); the results are the same.
+    Pointers are device addresses: params float4 per pool point, images 3
+    floats per pixel, dLdImage likewise, grad 4 doubles per pool point
+    (accumulated)."""
 
     def __init__(self, ctx: Context, cloud: "ComputeRasteriserPointCloud", sizes,
-                 configuration: Optional[ComputeRasteriserConfiguration] = None):
+                 configuration: Optional[ComputeRasteriserConfiguration] = None, batched: bool = False):
         self.ctx = ctx
         self.cloud = cloud
-        self.frames = [Frame(ctx, w, h) for (w, h) in sizes]
         self.configuration = configuration or ComputeRasteriserConfiguration()
+        self._mv = None
+        if batched:
+            self._mv = MultiViewRasteriser(ctx, [cloud], sizes, [0] * len(sizes), self.configuration)
+            self.frames = self._mv.frames
+        else:
+            self.frames = [Frame(ctx, w, h) for (w, h) in sizes]
+
+    @property
+    def batched(self) -> bool:
+        return self._mv is not None
 
     @property
     def poolPoints(self) -> int:
@@ -716,6 +729,23 @@ class DifferentiableHQS:
                                                       self.cloud.handle, self.frames[view].handle, dLdImage_ptr,
                                                       grad_ptr))
 
+    def forward_all(self, cameras, params_ptr: int, image_ptrs) -> None:
+        if self._mv is not None:
+            self._mv.configuration = self.configuration
+            self._mv.diffRender(cameras, params_ptr, image_ptrs)
+        else:
+            for v, cam in enumerate(cameras):
+                self.forward(v, cam, params_ptr, image_ptrs[v])
+
+    def backward_all(self, cameras, dLdImage_ptrs, grad_ptr: int) -> None:
+        """Views add to grad in view order."""
+        if self._mv is not None:
+            self._mv.configuration = self.configuration
+            self._mv.diffBackward(cameras, dLdImage_ptrs, grad_ptr)
+        else:
+            for v, cam in enumerate(cameras):
+                self.backward(v, cam, dLdImage_ptrs[v], grad_ptr)
+
     def read(self, view: int, which: int) -> np.ndarray:
         f = self.frames[view]
         return f.read(which).reshape(f.height, f.width)
@@ -725,8 +755,12 @@ class DifferentiableHQS:
         return np.stack([self.read(view, PLANE_DIFF_SUM_R + k) for k in range(3)], axis=-1)
 
     def close(self) -> None:
-        for f in getattr(self, "frames", []):
-            f.close()
+        if self._mv is not None:
+            self._mv.close()
+            self._mv = None
+        else:
+            for f in getattr(self, "frames", []):
+                f.close()
 
 
 # ---------------------------------------------------------------- multi-view
@@ -840,6 +874,20 @@ class MultiViewRasteriser:
     def read(self, view: int, which: int) -> np.ndarray:
         f = self.frames[view]
         return f.read(which).reshape(f.height, f.width)
+
+    def diffRender(self, cameras, params_ptr: int, image_ptrs) -> None:
+        """scr_multiview_diff_render: image_ptrs[v] = device address of view v's image."""
+        cams = self._cams(cameras)
+        cfg = self.configuration.to_c()
+        imgs = (C.c_void_p * self.nViews)(*[int(p) for p in image_ptrs])
+        self.ctx.check(self.ctx.lib.scr_multiview_diff_render(self.handle, C.byref(cfg), cams, params_ptr, imgs))
+
+    def diffBackward(self, cameras, dLdImage_ptrs, grad_ptr: int) -> None:
+        """scr_multiview_diff_backward: grad (4 doubles per pool point) += all views' gradients."""
+        cams = self._cams(cameras)
+        cfg = self.configuration.to_c()
+        dls = (C.c_void_p * self.nViews)(*[int(p) for p in dLdImage_ptrs])
+        self.ctx.check(self.ctx.lib.scr_multiview_diff_backward(self.handle, C.byref(cfg), cams, dls, grad_ptr))
 
     def close(self) -> None:
         if getattr(self, "handle", None):

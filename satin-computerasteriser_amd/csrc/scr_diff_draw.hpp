@@ -1,0 +1,158 @@
+// scr_diff_draw.hpp — device code of the differentiable HQS passes, shared by
+// the single-view kernels (scr_diff.hip) and the batched views of
+// scr_multiview.hip.
+//
+// Colour of pool point p = params[p].xyz, accumulated per pixel as int64
+// rint(param * 2^16) over the fragments the HQS colour pass keeps
+// (ColorPass/Shaders.metal:201-214 visibility).  The backward recomputes each
+// point's fragments and adds dL/dimage / count per fragment in FP64; one
+// thread owns a point, so a point's sum needs no atomics.
+#pragma once
+
+#include "scr_draw.hpp"
+
+namespace scr {
+namespace draw {
+
+constexpr int kDiffThreads = 256;  // diff kernels: workgroup size
+constexpr int kDiffPPT = 8;        // points per thread
+constexpr int kDiffTile = 1024;  // depth u32 + 3 sums u64 + count u32 = 32 B per pixel: 32 KB
+
+__device__ inline unsigned long long diff_fixed(float v) {
+    float x = v * 65536.0f;
+    if (!(x == x)) return 0ull;
+    x = fminf(fmaxf(x, -9.0e18f), 9.0e18f);
+    return (unsigned long long)(long long)rintf(x);
+}
+
+struct DiffLds {
+    uint32_t depth[kDiffTile];
+    unsigned long long s[3][kDiffTile];
+    uint32_t n[kDiffTile];
+    int red[5 * kMaxWaves];
+};
+
+// Local points [lo, bc.active) of one visible batch (lo a multiple of the chunk).
+template <bool kR0>
+__device__ inline void diff_color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                         const float4* __restrict__ params, const uint32_t* __restrict__ depth,
+                                         unsigned long long* sums, size_t stride, uint32_t* count, DiffLds& L) {
+    constexpr int kChunk = kDiffThreads * kDiffPPT;
+    const bool vec = vec_ok<kDiffPPT>(bc);
+    for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
+        const uint32_t li0 = cb + threadIdx.x * kDiffPPT;
+        ChunkFrags<kDiffPPT> f;
+        chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+        const ChunkBox box = chunk_box<false, kDiffThreads, kDiffPPT>(f, p, kDiffTile, L.red);
+        if (box.frags == 0) continue;
+        const int area = box.w * box.h;
+        if (box.lds) {
+            for (int i = threadIdx.x; i < area; i += kDiffThreads) {
+                L.depth[i] = depth[box.pixel(i, p.width)];
+                L.s[0][i] = L.s[1][i] = L.s[2][i] = 0ull;
+                L.n[i] = 0u;
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kDiffPPT; j++) {
+            if (f.px[j] < 0) continue;
+            const float4 t = params[bc.first + li0 + (uint32_t)j];
+            const unsigned long long q0 = diff_fixed(t.x), q1 = diff_fixed(t.y), q2 = diff_fixed(t.z);
+            const int rad = kR0 ? 0 : f.r[j];
+            for (int oy = -rad; oy <= rad; oy++)
+                for (int ox = -rad; ox <= rad; ox++) {
+                    if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
+                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
+                    const size_t pix = (size_t)ty * p.width + tx;
+                    const int k = box.lds ? box.local(tx, ty) : 0;
+                    const uint32_t d = box.lds ? L.depth[k] : depth[pix];
+                    if (d == 0u) continue;
+                    if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                    if (box.lds) {
+                        atomicAdd(&L.s[0][k], q0);
+                        atomicAdd(&L.s[1][k], q1);
+                        atomicAdd(&L.s[2][k], q2);
+                        atomicAdd(&L.n[k], 1u);
+                    } else {
+                        atomicAdd(sums + pix, q0);
+                        atomicAdd(sums + stride + pix, q1);
+                        atomicAdd(sums + 2 * stride + pix, q2);
+                        atomicAdd(count + pix, 1u);
+                    }
+                }
+        }
+        if (box.lds) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < area; i += kDiffThreads) {
+                const uint32_t n = L.n[i];
+                if (n == 0u) continue;
+                const size_t pix = box.pixel(i, p.width);
+                atomicAdd(sums + pix, L.s[0][i]);
+                atomicAdd(sums + stride + pix, L.s[1][i]);
+                atomicAdd(sums + 2 * stride + pix, L.s[2][i]);
+                atomicAdd(count + pix, n);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Local points [lo, bc.active) of one visible batch (lo a multiple of the chunk).
+template <bool kR0>
+__device__ inline void diff_backward_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                            const uint32_t* __restrict__ depth, const uint32_t* __restrict__ count,
+                                            const float* __restrict__ dLdImage, double* __restrict__ grad) {
+    constexpr int kChunk = kDiffThreads * kDiffPPT;
+    const bool vec = vec_ok<kDiffPPT>(bc);
+    for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
+        const uint32_t li0 = cb + threadIdx.x * kDiffPPT;
+        ChunkFrags<kDiffPPT> f;
+        chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+#pragma unroll
+        for (int j = 0; j < kDiffPPT; j++) {
+            if (f.px[j] < 0) continue;
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+            bool any = false;
+            const int rad = kR0 ? 0 : f.r[j];
+            for (int oy = -rad; oy <= rad; oy++)
+                for (int ox = -rad; ox <= rad; ox++) {
+                    if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
+                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
+                    const size_t pix = (size_t)ty * p.width + tx;
+                    const uint32_t d = depth[pix];
+                    if (d == 0u) continue;
+                    if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                    const double n = (double)count[pix];
+                    g0 += (double)dLdImage[pix * 3 + 0] / n;
+                    g1 += (double)dLdImage[pix * 3 + 1] / n;
+                    g2 += (double)dLdImage[pix * 3 + 2] / n;
+                    any = true;
+                }
+            if (!any) continue;
+            double* gp = grad + (size_t)(bc.first + li0 + (uint32_t)j) * 4;
+            gp[0] += g0;
+            gp[1] += g1;
+            gp[2] += g2;
+        }
+    }
+}
+
+// Pixel i of the differentiable image: sum / (2^16 * count) in FP64, or the
+// background where count is 0.
+__device__ inline void diff_resolve_pixel(const PassParams& p, const unsigned long long* __restrict__ sums,
+                                          size_t stride, const uint32_t* __restrict__ count, size_t i,
+                                          float* __restrict__ image) {
+    const uint32_t cnt = count[i];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float v = p.bg[k];
+        if (cnt != 0u) v = (float)((double)(long long)sums[(size_t)k * stride + i] / (65536.0 * (double)cnt));
+        image[i * 3 + (size_t)k] = v;
+    }
+}
+
+}  // namespace draw
+}  // namespace scr

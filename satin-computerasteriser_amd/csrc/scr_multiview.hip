@@ -21,7 +21,7 @@
 #include <cstring>
 #include <new>
 
-#include "scr_draw.hpp"
+#include "scr_diff_draw.hpp"
 
 namespace scr {
 namespace {
@@ -37,6 +37,10 @@ struct MVView {
     float* depthf;
     unsigned long long stride;
     uint32_t itemBase, itemStride;  // item of slot s = itemBase + s * itemStride
+    // differentiable mode
+    unsigned long long* sums;       // 3 int64 planes, stride apart
+    float* image;                   // 3 floats per pixel
+    const float* dLdImage;          // 3 floats per pixel
 };
 
 struct MVItem {
@@ -46,14 +50,16 @@ struct MVItem {
 };
 static_assert(sizeof(MVItem) == 32, "MVItem is two 16-byte words");
 
-constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, spare
+constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
 
-__global__ __launch_bounds__(256) void k_mv_clear(const MVView* __restrict__ views, int nearestMode,
+enum ClearMode { kClearHQS = 0, kClearNearest = 1, kClearDiff = 2 };
+
+__global__ __launch_bounds__(256) void k_mv_clear(const MVView* __restrict__ views, int mode,
                                                   uint32_t* __restrict__ heads) {
     const MVView& V = views[blockIdx.y];
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < kQueues) heads[threadIdx.x] = 0u;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (nearestMode) {
+    if (mode == kClearNearest) {
         const size_t n = (size_t)V.p.width * (size_t)V.p.height;
         const size_t j = i * 2;
         if (j + 2 <= n) {
@@ -66,6 +72,9 @@ __global__ __launch_bounds__(256) void k_mv_clear(const MVView* __restrict__ vie
         const size_t n = 5 * (size_t)V.stride;
         const size_t j = i * 4;
         if (j < n) *reinterpret_cast<uint4*>(V.planes + j) = make_uint4(0u, 0u, 0u, 0u);
+        // differentiable mode: 3 int64 planes, two per thread
+        if (mode == kClearDiff && i * 2 < 3 * (size_t)V.stride)
+            *reinterpret_cast<ulonglong2*>(V.sums + i * 2) = make_ulonglong2(0ull, 0ull);
     }
 }
 
@@ -166,6 +175,61 @@ __global__ __launch_bounds__(256) void k_mv_nearest_resolve(const MVView* __rest
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     nearest_resolve_pixel(V.p, V.nearest, V.c.colors, i, V.rgba, V.depthf);
+}
+
+// ---------------------------------------------------------------- differentiable views
+
+template <bool kR0>
+__global__ __launch_bounds__(kDiffThreads) void k_mv_diff_color(const MVView* __restrict__ views,
+                                                                const MVItem* __restrict__ items, uint32_t nUnits,
+                                                                const float4* __restrict__ params, uint32_t* heads) {
+    __shared__ DiffLds L;
+    drain<kDiffThreads, kDiffPPT>(views, items, nUnits, 1, heads + 3,
+                                  [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
+                                      const BatchCtx bc = batch_ctx(V.c, V.p, vb, hi);
+                                      const size_t s = (size_t)V.stride;
+                                      if (!kR0 && bc.rconst == 0)
+                                          diff_color_chunks<true>(V.c, V.p, bc, lo, params, V.planes, V.sums, s,
+                                                                  V.planes + 4 * s, L);
+                                      else
+                                          diff_color_chunks<kR0>(V.c, V.p, bc, lo, params, V.planes, V.sums, s,
+                                                                 V.planes + 4 * s, L);
+                                  });
+}
+
+__global__ __launch_bounds__(256) void k_mv_diff_resolve(const MVView* __restrict__ views) {
+    const MVView& V = views[blockIdx.y];
+    const size_t n = (size_t)V.p.width * (size_t)V.p.height;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    diff_resolve_pixel(V.p, V.sums, (size_t)V.stride, V.planes + 4 * (size_t)V.stride, i, V.image);
+}
+
+// Backward over the views of one cloud.  A work unit is one chunk of one
+// slot (perSlot chunks cover a slot's capacity); it visits that slot's item of every view in view order (the items of
+// a slot are adjacent in the table), so each point's gradient is added by one
+// thread, view after view, in the order separate per-view calls would add it.
+template <bool kR0>
+__global__ __launch_bounds__(kDiffThreads) void k_mv_diff_backward(const MVView* __restrict__ views,
+                                                                   const MVItem* __restrict__ items, int nSlots,
+                                                                   int perSlot, int nViews, double* __restrict__ grad) {
+    constexpr uint32_t kChunk = kDiffThreads * kDiffPPT;
+    const int nUnits = nSlots * perSlot;
+    for (int u = blockIdx.x; u < nUnits; u += gridDim.x) {
+        const int slot = u / perSlot, chunk = u - slot * perSlot;
+        const uint32_t lo = (uint32_t)chunk * kChunk;
+        for (int k = 0; k < nViews; k++) {
+            const MVItem it = items[(size_t)slot * (size_t)nViews + (size_t)k];
+            if (it.vb.activePoints <= lo) continue;
+            const MVView& V = views[__builtin_amdgcn_readfirstlane(it.view)];
+            const BatchCtx bc = batch_ctx(V.c, V.p, it.vb, lo + kChunk);
+            const size_t s = (size_t)V.stride;
+            if (!kR0 && bc.rconst == 0)
+                diff_backward_chunks<true>(V.c, V.p, bc, lo, V.planes, V.planes + 4 * s, V.dLdImage, grad);
+            else
+                diff_backward_chunks<kR0>(V.c, V.p, bc, lo, V.planes, V.planes + 4 * s, V.dLdImage, grad);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- draw variants
@@ -325,7 +389,8 @@ int enqueue(scr_multiview* mv, int mode, bool r0, uint32_t nItems, int timedPhas
         const size_t perThread = nearest ? 2 : 4;
         const size_t n = nearest ? mv->maxPixels : mv->maxWords;
         hipLaunchKernelGGL(scr::k_mv_clear, dim3(std::max(1u, scr::grid_for(n, 256 * (unsigned)perThread)), nv),
-                           dim3(256), 0, s, views, nearest ? 1 : 0, mv->dHeads);
+                           dim3(256), 0, s, views, nearest ? (int)scr::kClearNearest : (int)scr::kClearHQS,
+                           mv->dHeads);
         SCR_HIP(ctx, hipGetLastError());
     }
     if (maxSlots > 0) {
@@ -371,7 +436,8 @@ int enqueue(scr_multiview* mv, int mode, bool r0, uint32_t nItems, int timedPhas
 
 // Composes the per-view parameter table on the host and queues its upload;
 // returns the item count.
-int stage_params(scr_multiview* mv, const scr_config* cfg, const scr_camera* cams, uint32_t* nItemsOut, bool* r0) {
+int stage_params(scr_multiview* mv, const scr_config* cfg, const scr_camera* cams, uint32_t* nItemsOut, bool* r0,
+                 float* const* images = nullptr, const float* const* dLdImages = nullptr) {
     scr_ctx* ctx = mv->ctx;
     for (scr_cloud* c : mv->clouds) {
         int rc = scr::flush_mirror(c);
@@ -429,6 +495,9 @@ int stage_params(scr_multiview* mv, const scr_config* cfg, const scr_camera* cam
         V.stride = f->planeStride();
         V.itemBase = base[ci] + rank[ci]++;
         V.itemStride = nvOf[ci];
+        V.sums = f->dSums;
+        V.image = images ? images[v] : nullptr;
+        V.dLdImage = dLdImages ? dLdImages[v] : nullptr;
     }
     SCR_HIP(ctx, hipMemcpyAsync(mv->dParams, stage, mv->paramsBytes, hipMemcpyHostToDevice, ctx->stream));
     SCR_HIP(ctx, hipEventRecord(mv->staged[k], ctx->stream));
@@ -477,6 +546,30 @@ int render_impl(scr_multiview* mv, const scr_config* cfg, const scr_camera* cams
     }
     SCR_HIP(ctx, hipGraphLaunch(mv->graphExec, ctx->stream));
     return SCR_OK;
+}
+
+// Differentiable views: every view must render the multi-view's one cloud.
+int check_diff_views(scr_multiview* mv, const scr_config* cfg) {
+    scr_ctx* ctx = mv->ctx;
+    int rc = scr::check_config(ctx, cfg);
+    if (rc != SCR_OK) return rc;
+    if (cfg->mode != SCR_MODE_HIGH_QUALITY_AVERAGE || cfg->applyTint != 0 || cfg->colorizeChunks != 0 ||
+        cfg->colorizeOverdraw != 0)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "differentiable rendering is HQS with colours from the parameters");
+    if (mv->clouds.size() != 1)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "differentiable views render one cloud");
+    for (scr_frame* f : mv->frames)
+        if (!f->dSums) {
+            SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            SCR_HIP(ctx, hipMalloc((void**)&f->dSums, 3 * f->planeStride() * sizeof(unsigned long long)));
+        }
+    return SCR_OK;
+}
+
+unsigned diff_grid(scr_multiview* mv, scr::DrawKernel k, uint32_t nUnits) {
+    unsigned grid = 1;
+    if (resident_grid(mv, k, scr::kDiffThreads, nUnits, &grid) != SCR_OK) grid = (unsigned)mv->ctx->numCUs * 4u;
+    return std::max(1u, grid);
 }
 
 }  // namespace
@@ -617,6 +710,111 @@ int scr_multiview_autotune(scr_multiview* mv, const scr_config* cfg, const scr_c
     mv->tuning = best;
     drop_graph(mv);
     if (out) *out = best;
+    return SCR_OK;
+}
+
+int scr_multiview_diff_render(scr_multiview* mv, const scr_config* cfg, const scr_camera* cameras,
+                              const float* params, float* const* images) {
+    if (!mv || !cameras || !params || !images) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = mv->ctx;
+    for (int v = 0; v < mv->nViews; v++)
+        if (!images[v]) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "NULL image pointer");
+    int rc = check_diff_views(mv, cfg);
+    if (rc != SCR_OK) return rc;
+    uint32_t nItems = 0;
+    bool r0 = false;
+    rc = stage_params(mv, cfg, cameras, &nItems, &r0, images, nullptr);
+    if (rc != SCR_OK) return rc;
+    hipStream_t s = ctx->stream;
+    const MVView* views = reinterpret_cast<const MVView*>(mv->dParams);
+    const unsigned nv = (unsigned)mv->nViews;
+    const int maxSlots = mv->clouds[0]->maxResidentBatches;
+    {
+        scr::PassScope ps(ctx, "mv_clear");
+        // threads: 4 plane words or 2 sums each; the 3 sum planes need the most
+        const size_t maxStride = mv->maxWords / 5;
+        const size_t threads = std::max(mv->maxWords / 4, 3 * maxStride / 2);
+        hipLaunchKernelGGL(scr::k_mv_clear, dim3(std::max(1u, scr::grid_for(threads, 256)), nv), dim3(256), 0, s,
+                           views, (int)scr::kClearDiff, mv->dHeads);
+        SCR_HIP(ctx, hipGetLastError());
+    }
+    if (nItems == 0 || maxSlots == 0) return SCR_OK;
+    {
+        scr::PassScope ps(ctx, "mv_cull");
+        hipLaunchKernelGGL(scr::k_mv_cull, dim3(scr::grid_for((size_t)maxSlots, 256), nv), dim3(256), 0, s, views,
+                           mv->dItems);
+        SCR_HIP(ctx, hipGetLastError());
+    }
+    {
+        const scr_mv_shape& sh = mv->tuning.depth;
+        const scr::Variant* var = scr::find_variant(sh);
+        const scr::DrawKernel kern = scr::kernel_of(*var, scr::kDepth, r0, sh.waveTiles != 0);
+        const uint32_t nUnits = nItems * (uint32_t)sh.splits;
+        unsigned grid = 1;
+        rc = resident_grid(mv, kern, var->threads, nUnits, &grid);
+        if (rc != SCR_OK) return rc;
+        scr::PassScope ps(ctx, "mv_depth");
+        hipLaunchKernelGGL(kern, dim3(grid), dim3((unsigned)var->threads), 0, s, views, mv->dItems, nUnits, sh.splits,
+                           mv->dHeads);
+        SCR_HIP(ctx, hipGetLastError());
+    }
+    {
+        const scr::DrawKernel kern = reinterpret_cast<scr::DrawKernel>(
+            r0 ? (void*)scr::k_mv_diff_color<true> : (void*)scr::k_mv_diff_color<false>);
+        const unsigned grid = diff_grid(mv, kern, nItems);
+        scr::PassScope ps(ctx, "mv_diff_color");
+        if (r0)
+            hipLaunchKernelGGL(scr::k_mv_diff_color<true>, dim3(grid), dim3(scr::kDiffThreads), 0, s, views,
+                               mv->dItems, nItems, reinterpret_cast<const float4*>(params), mv->dHeads);
+        else
+            hipLaunchKernelGGL(scr::k_mv_diff_color<false>, dim3(grid), dim3(scr::kDiffThreads), 0, s, views,
+                               mv->dItems, nItems, reinterpret_cast<const float4*>(params), mv->dHeads);
+        SCR_HIP(ctx, hipGetLastError());
+    }
+    scr::PassScope ps(ctx, "mv_diff_resolve");
+    hipLaunchKernelGGL(scr::k_mv_diff_resolve, dim3(std::max(1u, scr::grid_for(mv->maxPixels, 256)), nv), dim3(256),
+                       0, s, views);
+    SCR_HIP(ctx, hipGetLastError());
+    return SCR_OK;
+}
+
+int scr_multiview_diff_backward(scr_multiview* mv, const scr_config* cfg, const scr_camera* cameras,
+                                const float* const* dLdImages, double* grad) {
+    if (!mv || !cameras || !dLdImages || !grad) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = mv->ctx;
+    for (int v = 0; v < mv->nViews; v++)
+        if (!dLdImages[v]) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "NULL dLdImage pointer");
+    int rc = check_diff_views(mv, cfg);
+    if (rc != SCR_OK) return rc;
+    uint32_t nItems = 0;
+    bool r0 = false;
+    rc = stage_params(mv, cfg, cameras, &nItems, &r0, nullptr, dLdImages);
+    if (rc != SCR_OK) return rc;
+    scr_cloud* cloud = mv->clouds[0];
+    const int maxSlots = cloud->maxResidentBatches;
+    if (nItems == 0 || maxSlots == 0 || cloud->residentBatchCount == 0) return SCR_OK;
+    hipStream_t s = ctx->stream;
+    const MVView* views = reinterpret_cast<const MVView*>(mv->dParams);
+    {
+        scr::PassScope ps(ctx, "mv_cull");
+        hipLaunchKernelGGL(scr::k_mv_cull, dim3(scr::grid_for((size_t)maxSlots, 256), (unsigned)mv->nViews),
+                           dim3(256), 0, s, views, mv->dItems);
+        SCR_HIP(ctx, hipGetLastError());
+    }
+    constexpr int kChunk = scr::kDiffThreads * scr::kDiffPPT;
+    const int perSlot = (cloud->pointsPerBatch + kChunk - 1) / kChunk;
+    const uint32_t nUnits = (uint32_t)maxSlots * (uint32_t)perSlot;
+    const scr::DrawKernel kern = reinterpret_cast<scr::DrawKernel>(
+        r0 ? (void*)scr::k_mv_diff_backward<true> : (void*)scr::k_mv_diff_backward<false>);
+    const unsigned grid = diff_grid(mv, kern, nUnits);
+    scr::PassScope ps(ctx, "mv_diff_backward");
+    if (r0)
+        hipLaunchKernelGGL(scr::k_mv_diff_backward<true>, dim3(grid), dim3(scr::kDiffThreads), 0, s, views,
+                           mv->dItems, maxSlots, perSlot, mv->nViews, grad);
+    else
+        hipLaunchKernelGGL(scr::k_mv_diff_backward<false>, dim3(grid), dim3(scr::kDiffThreads), 0, s, views,
+                           mv->dItems, maxSlots, perSlot, mv->nViews, grad);
+    SCR_HIP(ctx, hipGetLastError());
     return SCR_OK;
 }
 

@@ -38,8 +38,7 @@ class DiffHQSFunction(torch.autograd.Function):
             raise ValueError("all views of one call share a resolution")
         images = torch.empty((len(cameras), h, w, 3), dtype=torch.float32, device=params.device)
         torch.cuda.synchronize(params.device)
-        for v, cam in enumerate(cameras):
-            renderer.forward(v, cam, params.data_ptr(), images[v].data_ptr())
+        renderer.forward_all(cameras, params.data_ptr(), [images[v].data_ptr() for v in range(len(cameras))])
         renderer.ctx.sync()
         ctx.renderer = renderer
         ctx.cameras = cameras
@@ -51,8 +50,7 @@ class DiffHQSFunction(torch.autograd.Function):
         g = grad_images.contiguous().to(torch.float32)
         grad = torch.zeros((ctx.n, 4), dtype=torch.float64, device=g.device)
         torch.cuda.synchronize(g.device)
-        for v, cam in enumerate(ctx.cameras):
-            ctx.renderer.backward(v, cam, g[v].data_ptr(), grad.data_ptr())
+        ctx.renderer.backward_all(ctx.cameras, [g[v].data_ptr() for v in range(g.shape[0])], grad.data_ptr())
         ctx.renderer.ctx.sync()
         return grad.to(torch.float32), None, None
 

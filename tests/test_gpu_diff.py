@@ -105,6 +105,44 @@ def test_diff_backward_matches_fp64_reference(ctx, diffscene):
     r.close()
 
 
+@pytest.mark.parametrize("kw", CFGS)
+def test_batched_views_equal_per_view_calls(ctx, diffscene, kw):
+    """One launch per phase for all views gives the per-view images, planes
+    and FP64 gradients bit for bit."""
+    cloud, ocloud, gt, cams = diffscene
+    cfg = scr.ComputeRasteriserConfiguration(**kw)
+    sizes = [(W, H)] * len(cams)
+    one = scr.DifferentiableHQS(ctx, cloud, sizes, cfg)
+    many = scr.DifferentiableHQS(ctx, cloud, sizes, cfg, batched=True)
+    npool = one.poolPoints
+    params = gt.copy()
+    params[::3, :3] *= np.float32(0.8)
+    dparams = _dev(ctx, params)
+    nv = len(cams)
+    imgs = [scr.DeviceBuffer(ctx, W * H * 12) for _ in range(2 * nv)]
+    one.forward_all(cams, dparams.ptr, [b.ptr for b in imgs[:nv]])
+    many.forward_all(cams, dparams.ptr, [b.ptr for b in imgs[nv:]])
+    for v in range(nv):
+        a = imgs[v].to_array(np.uint32, W * H * 3)
+        b = imgs[nv + v].to_array(np.uint32, W * H * 3)
+        assert np.array_equal(a, b), v
+        assert np.array_equal(one.sums(v), many.sums(v))
+        for pl in (scr.PLANE_DEPTH, scr.PLANE_COUNT):
+            assert np.array_equal(one.read(v, pl), many.read(v, pl))
+    rng = np.random.default_rng(9)
+    dls = [_dev(ctx, rng.normal(size=(H, W, 3)).astype(np.float32)) for _ in range(nv)]
+    ga = _dev(ctx, np.zeros((npool, 4), np.float64))
+    gb = _dev(ctx, np.zeros((npool, 4), np.float64))
+    one.backward_all(cams, [d.ptr for d in dls], ga.ptr)
+    many.backward_all(cams, [d.ptr for d in dls], gb.ptr)
+    a = ga.to_array(np.float64, npool * 4)
+    b = gb.to_array(np.float64, npool * 4)
+    assert np.count_nonzero(a) > 3000
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    one.close()
+    many.close()
+
+
 def _torch_setup(ctx, diffscene, cfgkw):
     from _pkg import PKG_DIR
     import importlib.util
@@ -113,7 +151,7 @@ def _torch_setup(ctx, diffscene, cfgkw):
     spec.loader.exec_module(td)
     cloud, ocloud, gt, cams = diffscene
     cfg = scr.ComputeRasteriserConfiguration(**cfgkw)
-    r = scr.DifferentiableHQS(ctx, cloud, [(W, H)] * len(cams), cfg)
+    r = scr.DifferentiableHQS(ctx, cloud, [(W, H)] * len(cams), cfg, batched=True)
     return td, r, cfg
 
 
@@ -235,7 +273,7 @@ def test_config5_full_size_gradient_checks(ctx):
     cams, _ = scr.sphere_views((lo + hi) / 2, 1.2 * float(np.linalg.norm(hi - lo)), nv, 0x5C0005,
                                resolutions=((size, size),))
     cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1.0, maximumPointSize=3.0, pointSizeScale=1500.0)
-    r = scr.DifferentiableHQS(ctx, cloud, [(size, size)] * nv, cfg)
+    r = scr.DifferentiableHQS(ctx, cloud, [(size, size)] * nv, cfg, batched=True)
     dev = torch.device("cuda:0")
     render = td.HQSRender(r, cams)
     with torch.no_grad():

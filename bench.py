@@ -121,9 +121,9 @@ def read_traffic(kernel_tag: str):
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
-        m = re.match(rf"\|\s*{re.escape(kernel_tag)}\s*\|\s*([0-9.eE+]+)\s*\|", line)
+        m = re.match(rf"\|\s*{re.escape(kernel_tag)}(<[^|]*>)?\s*\|\s*([0-9.eE+]+)\s*\|", line)
         if m:
-            return float(m.group(1))
+            return float(m.group(2))
     return None
 
 

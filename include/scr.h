@@ -314,7 +314,7 @@ int scr_diff_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, 
  * frame's depth and count planes from that forward are read, each point's
  * fragments are recomputed; nothing per fragment is stored).  For every
  * point p and channel k:
- *     grad[4p + k] += Σ over p's kept fragments of dLdImage[3 * pixel + k] / count[pixel]
+ *     grad[4p + k] += Σ over p's kept fragments of dLdImage[3 * pixel + k] · (1 / count[pixel])
  * in FP64 (the fixed-point rounding passes the gradient straight through).
  * dLdImage: device, 3 floats per pixel; grad: device, 4 doubles per pool
  * point, accumulated (views add up in call order; no atomics). */

@@ -1076,11 +1076,11 @@ typedef struct orc_diff_bwd_ctx {
 
 static void orc_diff_bwd_visit(void* vctx, uint32_t pi, size_t p) {
     orc_diff_bwd_ctx* x = (orc_diff_bwd_ctx*)vctx;
-    const double n = (double)x->count[p];
-    for (int k = 0; k < 3; k++) x->grad[(size_t)pi * 4 + k] += (double)x->dLdImage[p * 3 + k] / n;
+    const double inv = 1.0 / (double)x->count[p];
+    for (int k = 0; k < 3; k++) x->grad[(size_t)pi * 4 + k] += (double)x->dLdImage[p * 3 + k] * inv;
 }
 
-/* dL/dparam = sum over the point's visible fragments of dL/dpixel / count
+/* dL/dparam = sum over the point's visible fragments of dL/dpixel x (1 / count)
  * (the fixed-point rounding passes the gradient straight through), FP64. */
 int orc_diff_backward(const orc_cloud* c, const scr_config* cfg, const float* view, const float* proj, int width,
                       int height, const uint32_t* depth, const uint32_t* count, const float* dLdImage, double* grad,

@@ -154,7 +154,7 @@ float orc_diff_pixel(int64_t sum, uint32_t count, float background);
 int orc_render_diff(const orc_cloud* cloud, const scr_config* cfg, const float* view, const float* proj, int width,
                     int height, const float* params, uint32_t* depth, uint32_t* count, int64_t* sums, float* image,
                     int nThreads);
-/* grad[4p + k] += sum over p's kept fragments of dLdImage[3 * pixel + k] / count[pixel]
+/* grad[4p + k] += sum over p's kept fragments of dLdImage[3 * pixel + k] * (1 / count[pixel])
  * in FP64, for points with mask[p] != 0 (mask NULL: all points). */
 int orc_diff_backward(const orc_cloud* cloud, const scr_config* cfg, const float* view, const float* proj, int width,
                       int height, const uint32_t* depth, const uint32_t* count, const float* dLdImage,

@@ -6,7 +6,7 @@
 //             sum_k(pixel) = Σ rint(param_k · 2^16) over the fragments the HQS
 //             colour pass keeps (ColorPass/Shaders.metal:201-214 visibility);
 //             image = sum / (2^16 · count) in FP64, rounded to float
-//   backward  cull → k_diff_backward: dL/dparam_k(p) += Σ_fragments dL/dimage_k / count
+//   backward  cull → k_diff_backward: dL/dparam_k(p) += Σ_fragments dL/dimage_k · (1 / count)
 //             (rounding passes the gradient straight through), FP64, one thread
 //             per point (no atomics, so the result does not depend on timing)
 //

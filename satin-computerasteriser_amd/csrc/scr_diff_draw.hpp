@@ -5,7 +5,7 @@
 // Colour of pool point p = params[p].xyz, accumulated per pixel as int64
 // rint(param * 2^16) over the fragments the HQS colour pass keeps
 // (ColorPass/Shaders.metal:201-214 visibility).  The backward recomputes each
-// point's fragments and adds dL/dimage / count per fragment in FP64; one
+// point's fragments and adds dL/dimage x (1 / count) per fragment in FP64; one
 // thread owns a point, so a point's sum needs no atomics.
 #pragma once
 
@@ -125,10 +125,10 @@ __device__ inline void diff_backward_chunks(const CloudDev& c, const PassParams&
                     const uint32_t d = depth[pix];
                     if (d == 0u) continue;
                     if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                    const double n = (double)count[pix];
-                    g0 += (double)dLdImage[pix * 3 + 0] / n;
-                    g1 += (double)dLdImage[pix * 3 + 1] / n;
-                    g2 += (double)dLdImage[pix * 3 + 2] / n;
+                    const double inv = 1.0 / (double)count[pix];
+                    g0 += (double)dLdImage[pix * 3 + 0] * inv;
+                    g1 += (double)dLdImage[pix * 3 + 1] * inv;
+                    g2 += (double)dLdImage[pix * 3 + 2] * inv;
                     any = true;
                 }
             if (!any) continue;

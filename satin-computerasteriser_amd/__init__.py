@@ -191,7 +191,9 @@ _lib: Optional[C.CDLL] = None
 
 
 def load_library(path: Optional[Path] = None) -> C.CDLL:
-    """Load libscr.so and bind every entry point (raises if it is missing)."""
+    """Load libscr.so (or the library at ``path``, which then serves every
+    object created afterwards) and bind every entry point; raises if it is
+    missing."""
     global _lib
     if _lib is not None and path is None:
         return _lib
@@ -203,8 +205,7 @@ def load_library(path: Optional[Path] = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if path is None:
-        _lib = lib
+    _lib = lib
     return lib
 
 

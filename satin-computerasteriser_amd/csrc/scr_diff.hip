@@ -62,7 +62,9 @@ __global__ __launch_bounds__(kThreads) void k_diff_backward(CloudDev c, PassPara
 unsigned grid_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
 unsigned draw_grid(int numCUs, int nSlots) {
-    const int g = numCUs * 8;
+    // about one workgroup per visible batch up to 24 per CU: more workgroups
+    // than are resident keep the tail of the pass balanced (measured)
+    const int g = numCUs * 24;
     return (unsigned)(nSlots < g ? (nSlots > 0 ? nSlots : 1) : g);
 }
 

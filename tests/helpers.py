@@ -2,6 +2,7 @@
 import numpy as np
 
 from _pkg import orc, scr
+from fixtures import jittered_cloud
 
 PLANES = {
     "depth": scr.PLANE_DEPTH, "red": scr.PLANE_RED, "green": scr.PLANE_GREEN, "blue": scr.PLANE_BLUE,
@@ -59,3 +60,26 @@ def assert_planes_equal(gpu, ref, keys=None):
         bad = np.argwhere(g != r)
         assert bad.size == 0, f"plane {k}: {len(bad)} pixels differ, first at {bad[:5].tolist()} " \
                               f"gpu={g[tuple(bad[0])] if len(bad) else None} ref={r[tuple(bad[0])] if len(bad) else None}"
+
+
+def multi_file_packed():
+    """Batches spread over three RasterFiles with different world matrices
+    (rotation + translation, uniform scale), as tiled datasets use them."""
+    pos, col = jittered_cloud(6000, 0x99)
+    packed = orc.pack(pos, col, pointsPerBatch=500)
+    files = scr.identity_files(3)
+    a = np.deg2rad(30.0)
+    rot = np.eye(4, dtype=np.float32)
+    rot[0, 0], rot[0, 1], rot[1, 0], rot[1, 1] = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)  # [column][row]
+    rot[3, :3] = (3.0, -1.0, 0.5)
+    files[1]["world"] = rot
+    sc = np.eye(4, dtype=np.float32) * np.float32(0.5)
+    sc[3] = (-2.0, 2.0, 1.0, 1.0)
+    files[2]["world"] = sc
+    packed.files = files
+    packed.batches["fileIndex"] = np.arange(packed.batchCount) % 3
+    return packed
+
+
+MODEL = np.eye(4, dtype=np.float32)
+MODEL[3, :3] = (0.5, 1.0, -0.25)   # translation (column 3)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from _pkg import scr
+from helpers import MODEL, multi_file_packed
 
 pytestmark = pytest.mark.gpu
 
@@ -172,6 +173,28 @@ def test_multiview_empty_cloud_and_shared_cloud(ctx, scene):
     want = _per_view(ctx, (clouds,), cfg, cams, sizes, which, HQS_PLANES)
     _assert_same(got, want)
     mv.close()
+
+
+def test_multiview_multi_file_clouds_with_model_matrices(ctx, scene):
+    """Per-view composed file tables: clouds with several RasterFiles and a
+    model matrix render the same batched as per view."""
+    packed = multi_file_packed()
+    tiled = scr.ComputeRasteriserPointCloud.from_packed(ctx, packed)
+    tiled.worldMatrix = MODEL
+    clouds = [tiled, scene[0][1]]
+    cams, sizes, _ = _views(scene, 5, seed=8, resolutions=((256, 256), (512, 384)))
+    cams[0] = scr.Camera.perspective((12.0, 9.0, 13.0), (0, 0, 0), fov=55, aspect=1.0)
+    cams[2] = scr.Camera.perspective((-10.0, 8.0, 11.0), (0, 0, 0), fov=60, aspect=4 / 3)
+    which = [0, 1, 0, 1, 0]
+    cams[4] = scr.Camera.perspective((9.0, -12.0, 7.0), (0, 0, 0), fov=50, aspect=1.0)
+    for kw in (dict(), dict(minimumPointSize=1, maximumPointSize=4, pointSizeScale=30.0)):
+        cfg = scr.ComputeRasteriserConfiguration(**kw)
+        want = _per_view(ctx, (clouds,), cfg, cams, sizes, which, HQS_PLANES)
+        assert _drawn([want[0], want[2], want[4]]) > 500
+        mv = scr.MultiViewRasteriser(ctx, clouds, sizes, which, cfg)
+        mv.render(cams)
+        _assert_same(_batched(mv, HQS_PLANES), want)
+        mv.close()
 
 
 def test_multiview_argument_errors(ctx, scene):

@@ -5,8 +5,8 @@ import pytest
 
 from _pkg import orc, scr
 from fixtures import jittered_cloud, solid_cube
-from helpers import (assert_planes_equal, config1_camera, gpu_cloud, oracle_cloud, render_gpu_hqs,
-                     render_gpu_nearest)
+from helpers import (MODEL, assert_planes_equal, config1_camera, gpu_cloud, multi_file_packed, oracle_cloud,
+                     render_gpu_hqs, render_gpu_nearest)
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +69,21 @@ def test_single_column_chunks_bit_exact(ctx):
     gpu = render_gpu_nearest(ctx, [gpu_cloud(ctx, packed)], cfg, cam, 1280, 720)
     ref = orc.render_nearest(oracle_cloud(packed), cfg, cam.viewMatrix, cam.projectionMatrix, 1280, 720)
     assert_planes_equal(gpu, ref)
+
+
+def test_multiple_files_and_model_matrix(ctx):
+    packed = multi_file_packed()
+    cam = scr.Camera.perspective((12.0, 9.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    for kw in (dict(), dict(minimumPointSize=1, maximumPointSize=4, pointSizeScale=30.0)):
+        cfg = scr.ComputeRasteriserConfiguration(**kw)
+        gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, packed, MODEL)], cfg, cam, 200, 150)
+        ref = orc.render_hqs([oracle_cloud(packed, MODEL)], cfg, cam.viewMatrix, cam.projectionMatrix, 200, 150)
+        assert ref["count"].sum() > 1000
+        assert_planes_equal(gpu, ref)
+        cfg.mode = scr.MODE_NEAREST_POINT
+        gpu = render_gpu_nearest(ctx, [gpu_cloud(ctx, packed, MODEL)], cfg, cam, 200, 150)
+        ref = orc.render_nearest(oracle_cloud(packed, MODEL), cfg, cam.viewMatrix, cam.projectionMatrix, 200, 150)
+        assert_planes_equal(gpu, ref)
 
 
 @pytest.fixture(scope="module")

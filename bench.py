@@ -20,6 +20,8 @@ multithreaded over batches) on a bounded sample of the same frame's batches.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import importlib.util
 import json
 import os
@@ -51,6 +53,19 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+@contextlib.contextmanager
+def quiet_host():
+    """Timed regions run with Python's cyclic collector paused (collected just
+    before): a collection inside a region stalls the host thread that feeds
+    the GPU and shows up as several milliseconds of device idle (measured)."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
+
+
 def overview_camera(scr, width, height):
     return scr.Camera(scr.look_at((0.0, -7200.0, 5200.0), (0.0, 300.0, 0.0), (0.0, 0.0, 1.0)),
                       scr.perspective_reverse_z(55.0, width / height, 10.0, 30000.0))
@@ -73,11 +88,12 @@ def flythrough_bench(scr, ctx, cloud, frames=120, width=2560, height=1440):
         ctx.sync()
         per_frame.append((time.perf_counter() - t0) * 1e3)
         processed.append(int(cloud.readVisible()["activePoints"].astype(np.int64).sum()))
-    t0 = time.perf_counter()
-    for cam in cams:
-        ras.render(cam)
-    ctx.sync()
-    total_ms = (time.perf_counter() - t0) * 1e3
+    with quiet_host():
+        t0 = time.perf_counter()
+        for cam in cams:
+            ras.render(cam)
+        ctx.sync()
+        total_ms = (time.perf_counter() - t0) * 1e3
     ras.frame.close()
     return {"frames": frames, "width": width, "height": height, "ms_per_frame_mean": round(total_ms / frames, 4),
             "ms_per_frame_synced_min_max": [round(min(per_frame), 4), round(max(per_frame), 4)],
@@ -99,11 +115,12 @@ def timed_frames(ctx, ras, cam, steps, warmup):
     for _ in range(warmup):
         ras.render(cam)
     ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ras.render(cam)
-    ctx.sync()
-    return (time.perf_counter() - t0) * 1e3 / steps
+    with quiet_host():
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ras.render(cam)
+        ctx.sync()
+        return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def pass_profile(ctx, ras, cam, steps):
@@ -187,11 +204,12 @@ def multiview_bench(scr, ctx, steps, warmup, n_views=64):
     for _ in range(warmup):
         per_view_pass()
     ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        per_view_pass()
-    ctx.sync()
-    ms_per_view_path = (time.perf_counter() - t0) * 1e3 / steps
+    with quiet_host():
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            per_view_pass()
+        ctx.sync()
+        ms_per_view_path = (time.perf_counter() - t0) * 1e3 / steps
     want = [f.read(scr.PLANE_RGBA8) for f in frames]
     want_count = [f.read(scr.PLANE_COUNT) for f in frames]
     for f in frames:
@@ -201,14 +219,15 @@ def multiview_bench(scr, ctx, steps, warmup, n_views=64):
     t0 = time.perf_counter()
     tuning = mv.autotune(cams, reps=2)
     tune_s = time.perf_counter() - t0
-    for _ in range(warmup):
+    for _ in range(max(warmup, 1)):   # the first render captures the graph
         mv.render(cams)
     ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        mv.render(cams)
-    ctx.sync()
-    ms_batched = (time.perf_counter() - t0) * 1e3 / steps
+    with quiet_host():
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            mv.render(cams)
+        ctx.sync()
+        ms_batched = (time.perf_counter() - t0) * 1e3 / steps
     exact = all(np.array_equal(mv.frames[v].read(scr.PLANE_RGBA8), want[v]) and
                 np.array_equal(mv.frames[v].read(scr.PLANE_COUNT), want_count[v]) for v in range(n_views))
     ctx.profile(True)
@@ -274,11 +293,12 @@ def diff_bench(scr, ctx, steps, fit_steps=200, n_points=5_000_000, n_views=32, s
     for name, rr in (("batched", r), ("per_view", r1)):
         fwd_bwd(rr)
         ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fwd_bwd(rr)
-        ctx.sync()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
+        with quiet_host():
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fwd_bwd(rr)
+            ctx.sync()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
         ctx.profile(True)
         fwd_bwd(rr)
         prof = {k_: round(v_[0] / v_[1], 4) for k_, v_ in ctx.collect_profile().items()}

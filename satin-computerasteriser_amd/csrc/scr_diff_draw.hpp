@@ -32,68 +32,107 @@ struct DiffLds {
     int red[5 * kMaxWaves];
 };
 
-// Local points [lo, bc.active) of one visible batch (lo a multiple of the chunk).
+// A chunk's fixed-point sums (the routes of depth_frags: kCheck tests each
+// fragment against a batch rectangle, otherwise the box is the chunk's own).
+template <bool kR0, bool kCheck>
+__device__ __forceinline__ void diff_frags(const ChunkFrags<kDiffPPT>& f, uint32_t base, const ChunkBox& box,
+                                           const PassParams& p, const float4* __restrict__ params,
+                                           const uint32_t* __restrict__ depth, unsigned long long* sums,
+                                           size_t stride, uint32_t* count, DiffLds& L) {
+#pragma unroll
+    for (int j = 0; j < kDiffPPT; j++) {
+        if (f.px[j] < 0) continue;
+        const float4 t = params[base + (uint32_t)j];
+        const unsigned long long q0 = diff_fixed(t.x), q1 = diff_fixed(t.y), q2 = diff_fixed(t.z);
+        const int rad = kR0 ? 0 : f.r[j];
+        for (int oy = -rad; oy <= rad; oy++)
+            for (int ox = -rad; ox <= rad; ox++) {
+                if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
+                const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
+                const size_t pix = (size_t)ty * p.width + tx;
+                const bool inTile = box.lds && (!kCheck || box_holds(box, tx, ty));
+                const int k = inTile ? box.local(tx, ty) : 0;
+                const uint32_t d = inTile ? L.depth[k] : depth[pix];
+                if (d == 0u) continue;
+                if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                if (inTile) {
+                    atomicAdd(&L.s[0][k], q0);
+                    atomicAdd(&L.s[1][k], q1);
+                    atomicAdd(&L.s[2][k], q2);
+                    atomicAdd(&L.n[k], 1u);
+                } else {
+                    atomicAdd(sums + pix, q0);
+                    atomicAdd(sums + stride + pix, q1);
+                    atomicAdd(sums + 2 * stride + pix, q2);
+                    atomicAdd(count + pix, 1u);
+                }
+            }
+    }
+}
+
+__device__ __forceinline__ void diff_stage(const ChunkBox& box, const PassParams& p,
+                                           const uint32_t* __restrict__ depth, DiffLds& L) {
+    const int area = box.w * box.h;
+    for (int i = threadIdx.x; i < area; i += kDiffThreads) {
+        L.depth[i] = depth[box.pixel(i, p.width)];
+        L.s[0][i] = L.s[1][i] = L.s[2][i] = 0ull;
+        L.n[i] = 0u;
+    }
+}
+
+__device__ __forceinline__ void diff_tile_flush(const ChunkBox& box, const PassParams& p, unsigned long long* sums,
+                                                size_t stride, uint32_t* count, const DiffLds& L) {
+    const int area = box.w * box.h;
+    for (int i = threadIdx.x; i < area; i += kDiffThreads) {
+        const uint32_t n = L.n[i];
+        if (n == 0u) continue;
+        const size_t pix = box.pixel(i, p.width);
+        atomicAdd(sums + pix, L.s[0][i]);
+        atomicAdd(sums + stride + pix, L.s[1][i]);
+        atomicAdd(sums + 2 * stride + pix, L.s[2][i]);
+        atomicAdd(count + pix, n);
+    }
+}
+
+// Local points [lo, bc.active) of one visible batch (lo a multiple of the
+// chunk): one tile for the whole range when the batch rectangle fits it,
+// else one per chunk.
 template <bool kR0>
 __device__ inline void diff_color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                          const float4* __restrict__ params, const uint32_t* __restrict__ depth,
                                          unsigned long long* sums, size_t stride, uint32_t* count, DiffLds& L) {
     constexpr int kChunk = kDiffThreads * kDiffPPT;
     const bool vec = vec_ok<kDiffPPT>(bc);
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? bc.active - lo : 0u, kDiffTile);
+    if (seg.lds) {
+        diff_stage(seg, p, depth, L);
+        __syncthreads();
+        for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
+            const uint32_t li0 = cb + threadIdx.x * kDiffPPT;
+            ChunkFrags<kDiffPPT> f;
+            chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+            diff_frags<kR0, true>(f, bc.first + li0, seg, p, params, depth, sums, stride, count, L);
+        }
+        __syncthreads();
+        diff_tile_flush(seg, p, sums, stride, count, L);
+        __syncthreads();
+        return;
+    }
     for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
         const uint32_t li0 = cb + threadIdx.x * kDiffPPT;
         ChunkFrags<kDiffPPT> f;
         chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
         const ChunkBox box = chunk_box<false, kDiffThreads, kDiffPPT>(f, p, kDiffTile, L.red);
         if (box.frags == 0) continue;
-        const int area = box.w * box.h;
         if (box.lds) {
-            for (int i = threadIdx.x; i < area; i += kDiffThreads) {
-                L.depth[i] = depth[box.pixel(i, p.width)];
-                L.s[0][i] = L.s[1][i] = L.s[2][i] = 0ull;
-                L.n[i] = 0u;
-            }
+            diff_stage(box, p, depth, L);
             __syncthreads();
         }
-#pragma unroll
-        for (int j = 0; j < kDiffPPT; j++) {
-            if (f.px[j] < 0) continue;
-            const float4 t = params[bc.first + li0 + (uint32_t)j];
-            const unsigned long long q0 = diff_fixed(t.x), q1 = diff_fixed(t.y), q2 = diff_fixed(t.z);
-            const int rad = kR0 ? 0 : f.r[j];
-            for (int oy = -rad; oy <= rad; oy++)
-                for (int ox = -rad; ox <= rad; ox++) {
-                    if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
-                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
-                    const size_t pix = (size_t)ty * p.width + tx;
-                    const int k = box.lds ? box.local(tx, ty) : 0;
-                    const uint32_t d = box.lds ? L.depth[k] : depth[pix];
-                    if (d == 0u) continue;
-                    if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                    if (box.lds) {
-                        atomicAdd(&L.s[0][k], q0);
-                        atomicAdd(&L.s[1][k], q1);
-                        atomicAdd(&L.s[2][k], q2);
-                        atomicAdd(&L.n[k], 1u);
-                    } else {
-                        atomicAdd(sums + pix, q0);
-                        atomicAdd(sums + stride + pix, q1);
-                        atomicAdd(sums + 2 * stride + pix, q2);
-                        atomicAdd(count + pix, 1u);
-                    }
-                }
-        }
+        diff_frags<kR0, false>(f, bc.first + li0, box, p, params, depth, sums, stride, count, L);
         if (box.lds) {
             __syncthreads();
-            for (int i = threadIdx.x; i < area; i += kDiffThreads) {
-                const uint32_t n = L.n[i];
-                if (n == 0u) continue;
-                const size_t pix = box.pixel(i, p.width);
-                atomicAdd(sums + pix, L.s[0][i]);
-                atomicAdd(sums + stride + pix, L.s[1][i]);
-                atomicAdd(sums + 2 * stride + pix, L.s[2][i]);
-                atomicAdd(count + pix, n);
-            }
+            diff_tile_flush(box, p, sums, stride, count, L);
             __syncthreads();
         }
     }

@@ -3,13 +3,14 @@
 // front end, and the per-batch bodies of the depth, colour and nearest passes.
 //
 // A visible batch is drawn in chunks of NT * PPT points: every
-// thread owns PPT consecutive points (vector loads of the position words),
-// keeps their fragments in registers, and the block reduces the chunk's
-// actual screen rectangle.  When that rectangle is small next to the fragment
-// count, the chunk's fragments meet in an LDS tile (LDS atomics) and the global
-// atomics see one update per touched pixel; otherwise they go straight to the
-// global planes.  Integer max / add are order independent, so both routes give
-// the same planes.
+// thread owns PPT consecutive points (vector loads of the position words) and
+// keeps their fragments in registers.  When the batch's projected box fits an
+// LDS tile, all of its chunks meet in that one tile (LDS atomics), flushed
+// once with one global update per touched pixel.  Otherwise the group reduces
+// each chunk's actual screen rectangle and, when that is small next to the
+// fragment count, uses a tile per chunk; the rest go straight to the global
+// planes.  Integer max / add are order independent, so every route gives the
+// same planes.
 #pragma once
 
 #include "scr_internal.hpp"
@@ -22,6 +23,11 @@ constexpr int kMaxWaves = 8;        // NT <= 512
 constexpr int kDepthTile = 4096;    // u32 per pixel: 16 KB
 constexpr int kColorTile = 2048;    // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 40 KB
 constexpr int kNearestTile = 2048;  // u64 per pixel: 16 KB
+
+// Waves per SIMD the depth and nearest kernels are compiled for: with up to
+// 8 points per thread, 4 (at most 128 VGPRs) hide memory latency better than
+// the 3 an unconstrained allocation leaves (measured); 2 for 16 points.
+constexpr int draw_waves(int ppt) { return ppt <= 8 ? 4 : 2; }
 
 // ---------------------------------------------------------------- cull
 
@@ -395,10 +401,128 @@ __device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams&
     return b;
 }
 
+// Screen rectangle of a whole batch: the batch box [min, min + size] (every
+// decoded position lies in it) through file.transform, one corner per lane of
+// each 8-lane group, widened by a pixel for the float error of the per-point
+// arithmetic and by the largest footprint radius, clamped to the screen.
+// `lds` is set when the rectangle fits `cap` pixels, is small next to the
+// `points` the group draws from it, and every corner lies in front of the
+// camera.  The draw bodies test each fragment against it, so a fragment
+// outside still reaches the planes: the rectangle only decides where the
+// group's fragments meet.  Called by every lane of the wave.
+__device__ inline ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, uint32_t points, int cap) {
+    const int corner = (int)(__lane_id() & 7u);
+    // sx * steps is the batch size (power-of-two scaling, exact)
+    const float steps = bc.level == 0 ? 1073741824.0f : (bc.level == 1 ? 1048576.0f : 1024.0f);
+    const float x = (corner & 1) ? bc.mnx + bc.sx * steps : bc.mnx;
+    const float y = (corner & 2) ? bc.mny + bc.sy * steps : bc.mny;
+    const float z = (corner & 4) ? bc.mnz + bc.sz * steps : bc.mnz;
+    const f4 clip = mul_point(bc.T, x, y, z);
+    const float rw = 1.0f / clip.w;
+    // screen x and the unflipped row of the corner, kept finite and far from
+    // the int range (a corner off the screen only widens the clamp below)
+    const float lim = 65536.0f;
+    float sx = ((clip.x * rw) * 0.5f + 0.5f) * (float)p.width;
+    float sy = ((clip.y * rw) * 0.5f + 0.5f) * (float)p.height;
+    int ok = (clip.w > 0.0f && sx == sx && sy == sy) ? 1 : 0;
+    sx = fminf(fmaxf(sx, -lim), lim);
+    sy = fminf(fmaxf(sy, -lim), lim);
+    float x0 = sx, x1 = sx, y0 = sy, y1 = sy;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        x0 = fminf(x0, __shfl_xor(x0, o));
+        x1 = fmaxf(x1, __shfl_xor(x1, o));
+        y0 = fminf(y0, __shfl_xor(y0, o));
+        y1 = fmaxf(y1, __shfl_xor(y1, o));
+        ok &= __shfl_xor(ok, o);
+    }
+    const int r = (bc.rconst >= 0 ? bc.rconst : p.maxRadius) + 1;
+    const int ix0 = __builtin_amdgcn_readfirstlane((int)floorf(x0)) - r;
+    const int ix1 = __builtin_amdgcn_readfirstlane((int)floorf(x1)) + r;
+    const int iy0 = __builtin_amdgcn_readfirstlane((int)floorf(y0)) - r;
+    const int iy1 = __builtin_amdgcn_readfirstlane((int)floorf(y1)) + r;
+    ChunkBox b;
+    b.frags = 0;
+    b.x0 = max(ix0, 0);
+    const int bx1 = min(ix1, p.width - 1);
+    // rows are flipped: py = height - 1 - iy
+    b.y0 = max(p.height - 1 - iy1, 0);
+    const int by1 = min(p.height - 1 - iy0, p.height - 1);
+    b.w = bx1 >= b.x0 ? bx1 - b.x0 + 1 : 0;
+    b.h = by1 >= b.y0 ? by1 - b.y0 + 1 : 0;
+    const long long area = (long long)b.w * b.h;
+    b.lds = __builtin_amdgcn_readfirstlane(ok) != 0 && area <= cap && area <= 8LL * points;
+    b.magic = (b.lds && b.w > 1) ? 0xffffffffu / (uint32_t)b.w + 1u : 0u;
+    return b;
+}
+
+// Whether (x, y) lies in the rectangle.
+__device__ inline bool box_holds(const ChunkBox& b, int x, int y) {
+    return (uint32_t)(x - b.x0) < (uint32_t)b.w && (uint32_t)(y - b.y0) < (uint32_t)b.h;
+}
+
 // ---------------------------------------------------------------- depth pass body
 // depthPassUpdate — DepthPass/Shaders.metal:21-116, for the local points
 // [lo, hi) of one visible batch (lo a multiple of NT * PPT).  The group's
 // chunks start at lo + group * group size * PPT and step by NT * PPT.
+//
+// Two ways to meet in LDS: when the batch's screen rectangle fits the tile
+// (segment_box), every chunk of the group adds to one tile that is flushed
+// once; otherwise each chunk reduces its own rectangle and flushes it.
+
+// A chunk's depth maxima.  With box.lds, fragments inside the box go to the
+// LDS tile (kCheck: each is tested, the box being a bound of the batch; else
+// the box is the chunk's own and holds them all) and the rest to the plane.
+template <bool kR0, bool kCheck, int PPT>
+__device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const ChunkBox& box, const PassParams& p, uint32_t* lds,
+                                   uint32_t* __restrict__ depth) {
+    // single-pixel points: a thread's consecutive points often share a
+    // pixel (Morton order); a max the thread has already issued there
+    // makes a smaller one a no-op, so it is skipped
+    int lastK = -1;
+    uint32_t lastD = 0u;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        if (f.px[j] < 0) continue;
+        const uint32_t d = depth_to_uint_hw(f.z[j]);
+        if (kR0) {
+            if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
+                atomicMax(depth + (size_t)f.py[j] * p.width + f.px[j], d);
+                continue;
+            }
+            const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
+            if (k == lastK && d <= lastD) continue;
+            lastD = (k == lastK) ? max(lastD, d) : d;
+            lastK = k;
+            if (box.lds) atomicMax(lds + k, d);
+            else atomicMax(depth + (size_t)k, d);
+            continue;
+        }
+        const int r = f.r[j];
+        for (int oy = -r; oy <= r; oy++)
+            for (int ox = -r; ox <= r; ox++) {
+                if (!inside_footprint(ox, oy, r)) continue;
+                const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                if (box.lds && (!kCheck || box_holds(box, tx, ty))) atomicMax(lds + box.local(tx, ty), d);
+                else atomicMax(depth + (size_t)ty * p.width + tx, d);
+            }
+    }
+}
+
+// Flushes a tile to the depth plane.  No read-before-max: waiting for the
+// read costs more than the atomics it would save (measured on the 100M frame).
+template <typename G>
+__device__ __forceinline__ void depth_flush(const ChunkBox& box, const PassParams& p, const uint32_t* lds,
+                                   uint32_t* __restrict__ depth) {
+    const int area = box.w * box.h;
+    for (int i = G::rank(); i < area; i += G::kSize) {
+        const uint32_t d = lds[i];
+        if (d == 0u) continue;
+        atomicMax(depth + box.pixel(i, p.width), d);
+    }
+}
+
 template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red) {
@@ -406,53 +530,36 @@ __device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, cons
     constexpr int kCap = kDepthTile / G::kCount;
     uint32_t* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
-    for (uint32_t cb = lo + (uint32_t)(G::index() * G::kSize * PPT); cb < bc.active; cb += NT * PPT) {
+    const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    if (seg.lds) {
+        const int area = seg.w * seg.h;
+        for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0u;
+        G::sync();
+        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+            ChunkFrags<PPT> f;
+            chunk_front<false, kR0, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
+            depth_frags<kR0, true, PPT>(f, seg, p, lds, depth);
+        }
+        G::sync();
+        depth_flush<G>(seg, p, lds, depth);
+        G::sync();
+        return;
+    }
+    for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         ChunkFrags<PPT> f;
         chunk_front<false, kR0, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
         if (box.frags == 0) continue;
-        const int area = box.w * box.h;
         if (box.lds) {
+            const int area = box.w * box.h;
             for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0u;
             G::sync();
         }
-        // single-pixel points: a thread's consecutive points often share a
-        // pixel (Morton order); a max the thread has already issued there
-        // makes a smaller one a no-op, so it is skipped
-        int lastK = -1;
-        uint32_t lastD = 0u;
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            if (f.px[j] < 0) continue;
-            const uint32_t d = depth_to_uint_hw(f.z[j]);
-            if (kR0) {
-                const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
-                if (k == lastK && d <= lastD) continue;
-                lastD = (k == lastK) ? max(lastD, d) : d;
-                lastK = k;
-                if (box.lds) atomicMax(lds + k, d);
-                else atomicMax(depth + (size_t)k, d);
-                continue;
-            }
-            const int r = f.r[j];
-            for (int oy = -r; oy <= r; oy++)
-                for (int ox = -r; ox <= r; ox++) {
-                    if (!inside_footprint(ox, oy, r)) continue;
-                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                    if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                    if (box.lds) atomicMax(lds + box.local(tx, ty), d);
-                    else atomicMax(depth + (size_t)ty * p.width + tx, d);
-                }
-        }
+        depth_frags<kR0, false, PPT>(f, box, p, lds, depth);
         if (box.lds) {
             G::sync();
-            for (int i = G::rank(); i < area; i += G::kSize) {
-                const uint32_t d = lds[i];
-                if (d == 0u) continue;
-                // no read-before-max: waiting for the read costs more than the
-                // atomics it would save (measured on the 100M frame)
-                atomicMax(depth + box.pixel(i, p.width), d);
-            }
+            depth_flush<G>(box, p, lds, depth);
             G::sync();
         }
     }
@@ -467,9 +574,21 @@ struct ColorLds {
     int* red;
 };
 
+// Built from arguments rather than a constant initializer: a constant
+// aggregate of LDS addresses would become a global the back end cannot emit.
+__device__ __forceinline__ ColorLds color_lds(uint32_t* depth, unsigned long long* rg, unsigned long long* bn,
+                                              int* red) {
+    ColorLds L;
+    L.depth = depth;
+    L.rg = rg;
+    L.bn = bn;
+    L.red = red;
+    return L;
+}
+
 // Adds a run's packed sums (r | g << 32, b | count << 32) to LDS tile slot k
 // or to global pixel k.
-__device__ inline void color_flush(bool lds, int k, unsigned long long rg, unsigned long long bn, const ColorLds& L,
+__device__ __forceinline__ void color_flush(bool lds, int k, unsigned long long rg, unsigned long long bn, ColorLds L,
                                    uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count) {
     if (lds) {
         atomicAdd(L.rg + k, rg);
@@ -482,129 +601,188 @@ __device__ inline void color_flush(bool lds, int k, unsigned long long rg, unsig
     }
 }
 
+// The colour stage (ColorPass/Shaders.metal:95-133) of a chunk: packed
+// r | g << 8 | b << 16 per fragment; a tint discard drops the fragment.
+template <int PPT>
+__device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0, bool vec,
+                                    ChunkFrags<PPT>& f, uint32_t* rgb) {
+    const uint32_t base = bc.first + li0;
+    if (vec && li0 + (uint32_t)PPT <= bc.active) {
+        load_words<PPT>(c.colors, base, rgb);
+    } else {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) rgb[j] = f.px[j] >= 0 ? c.colors[base + (uint32_t)j] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        const uint32_t pi = base + (uint32_t)j;
+        uint32_t color = rgb[j];
+        rgb[j] = 0u;
+        if (f.px[j] < 0) continue;
+        if (p.colorizeChunks != 0) color = bc.batchIndex * 1234567u;
+        else if (p.colorizeOverdraw != 0) color = 0x00010101u;
+        uint32_t r8 = color & 0xffu, g8 = (color >> 8) & 0xffu, b8 = (color >> 16) & 0xffu;
+        if (p.applyTint != 0) {
+            const float4 t = c.tints ? c.tints[pi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (t.w < 0.0f) {  // discard sentinel, ColorPass/Shaders.metal:119
+                f.px[j] = -1;
+                continue;
+            }
+            const float w = saturate(t.w);
+            const float ox = (float)r8 * kInv255, oy = (float)g8 * kInv255, oz = (float)b8 * kInv255;
+            const float mx = ox + (saturate(t.x) - ox) * w;
+            const float my = oy + (saturate(t.y) - oy) * w;
+            const float mz = oz + (saturate(t.z) - oz) * w;
+            r8 = f2u(saturate(mx) * 255.0f + 0.5f);
+            g8 = f2u(saturate(my) * 255.0f + 0.5f);
+            b8 = f2u(saturate(mz) * 255.0f + 0.5f);
+        }
+        rgb[j] = r8 | (g8 << 8) | (b8 << 16);
+    }
+}
+
+// A chunk's colour sums (same routes as depth_frags).  Single-pixel points:
+// runs of a thread's consecutive fragments on one tile pixel (Morton order)
+// are summed in registers and added once.
+template <bool kR0, bool kCheck, int PPT>
+__device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint32_t* rgb, const ChunkBox& box,
+                                   const PassParams& p, const uint32_t* __restrict__ depth, uint32_t* red,
+                                   uint32_t* green, uint32_t* blue, uint32_t* count, ColorLds L) {
+    int runK = -1;
+    unsigned long long runRG = 0ull, runBN = 0ull;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        if (f.px[j] < 0) continue;
+        const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
+        const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
+        const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
+        if (kR0) {
+            if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
+                const size_t pix = (size_t)f.py[j] * p.width + f.px[j];
+                const uint32_t d = depth[pix];
+                if (d == 0u || !(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                atomicAdd(red + pix, r8);
+                atomicAdd(green + pix, g8);
+                atomicAdd(blue + pix, b8);
+                atomicAdd(count + pix, 1u);
+                continue;
+            }
+            const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
+            const uint32_t d = box.lds ? L.depth[k] : depth[k];
+            if (d == 0u) continue;
+            if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+            if (k == runK) {
+                runRG += rg;
+                runBN += bn;
+                continue;
+            }
+            if (runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
+            runK = k;
+            runRG = rg;
+            runBN = bn;
+            continue;
+        }
+        const int rad = f.r[j];
+        for (int oy = -rad; oy <= rad; oy++)
+            for (int ox = -rad; ox <= rad; ox++) {
+                if (!inside_footprint(ox, oy, rad)) continue;
+                const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                const size_t pix = (size_t)ty * p.width + tx;
+                const bool inTile = box.lds && (!kCheck || box_holds(box, tx, ty));
+                const int k = inTile ? box.local(tx, ty) : 0;
+                const uint32_t d = inTile ? L.depth[k] : depth[pix];
+                if (d == 0u) continue;
+                if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                if (inTile) {
+                    atomicAdd(L.rg + k, rg);
+                    atomicAdd(L.bn + k, bn);
+                } else {
+                    atomicAdd(red + pix, r8);
+                    atomicAdd(green + pix, g8);
+                    atomicAdd(blue + pix, b8);
+                    atomicAdd(count + pix, 1u);
+                }
+            }
+    }
+    if (kR0 && runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
+}
+
+// Stages the tile's depths and clears its sums.
+template <typename G>
+__device__ __forceinline__ void color_stage(const ChunkBox& box, const PassParams& p, const uint32_t* __restrict__ depth,
+                                   ColorLds L) {
+    const int area = box.w * box.h;
+    for (int i = G::rank(); i < area; i += G::kSize) {
+        L.depth[i] = depth[box.pixel(i, p.width)];
+        L.rg[i] = 0ull;
+        L.bn[i] = 0ull;
+    }
+}
+
+// Adds a tile's sums to the planes.  A tile holds one batch, so a pixel's
+// sums stay below 2^32 (at most 65,535 points per batch, each adding at most
+// 255 once), and the packed halves never carry into each other.
+template <typename G>
+__device__ __forceinline__ void color_tile_flush(const ChunkBox& box, const PassParams& p, ColorLds L, uint32_t* red,
+                                        uint32_t* green, uint32_t* blue, uint32_t* count) {
+    const int area = box.w * box.h;
+    for (int i = G::rank(); i < area; i += G::kSize) {
+        const unsigned long long bn = L.bn[i];
+        const uint32_t n = (uint32_t)(bn >> 32);
+        if (n == 0u) continue;
+        const unsigned long long rg = L.rg[i];
+        const size_t pix = box.pixel(i, p.width);
+        atomicAdd(red + pix, (uint32_t)rg);
+        atomicAdd(green + pix, (uint32_t)(rg >> 32));
+        atomicAdd(blue + pix, (uint32_t)bn);
+        atomicAdd(count + pix, n);
+    }
+}
+
 template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
-                                    uint32_t* blue, uint32_t* count, const ColorLds& LAll) {
+                                    uint32_t* blue, uint32_t* count, ColorLds LAll) {
     using G = Group<kWave, NT>;
     constexpr int kCap = kColorTile / G::kCount;
     const int slice = G::index() * kCap;
-    const ColorLds L{LAll.depth + slice, LAll.rg + slice, LAll.bn + slice, LAll.red};
+    const ColorLds L = color_lds(LAll.depth + slice, LAll.rg + slice, LAll.bn + slice, LAll.red);
     const bool vec = vec_ok<PPT>(bc);
-    for (uint32_t cb = lo + (uint32_t)(G::index() * G::kSize * PPT); cb < bc.active; cb += NT * PPT) {
+    const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    if (seg.lds) {
+        color_stage<G>(seg, p, depth, L);
+        G::sync();
+        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+            const uint32_t li0 = cb + G::rank() * PPT;
+            ChunkFrags<PPT> f;
+            chunk_front<false, kR0, PPT>(c, p, bc, li0, vec, f);
+            uint32_t rgb[PPT];
+            chunk_colors<PPT>(c, p, bc, li0, vec, f, rgb);
+            color_frags<kR0, true, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
+        }
+        G::sync();
+        color_tile_flush<G>(seg, p, L, red, green, blue, count);
+        G::sync();
+        return;
+    }
+    for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         const uint32_t li0 = cb + G::rank() * PPT;
         ChunkFrags<PPT> f;
         chunk_front<false, kR0, PPT>(c, p, bc, li0, vec, f);
-        // colour stage (ColorPass/Shaders.metal:95-133): packed r | g << 8 | b << 16
         uint32_t rgb[PPT];
-        const uint32_t base = bc.first + li0;
-        if (vec && li0 + (uint32_t)PPT <= bc.active) {
-            load_words<PPT>(c.colors, base, rgb);
-        } else {
-#pragma unroll
-            for (int j = 0; j < PPT; j++) rgb[j] = f.px[j] >= 0 ? c.colors[base + (uint32_t)j] : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            const uint32_t pi = base + (uint32_t)j;
-            uint32_t color = rgb[j];
-            rgb[j] = 0u;
-            if (f.px[j] < 0) continue;
-            if (p.colorizeChunks != 0) color = bc.batchIndex * 1234567u;
-            else if (p.colorizeOverdraw != 0) color = 0x00010101u;
-            uint32_t r8 = color & 0xffu, g8 = (color >> 8) & 0xffu, b8 = (color >> 16) & 0xffu;
-            if (p.applyTint != 0) {
-                const float4 t = c.tints ? c.tints[pi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                if (t.w < 0.0f) {  // discard sentinel, ColorPass/Shaders.metal:119
-                    f.px[j] = -1;
-                    continue;
-                }
-                const float w = saturate(t.w);
-                const float ox = (float)r8 * kInv255, oy = (float)g8 * kInv255, oz = (float)b8 * kInv255;
-                const float mx = ox + (saturate(t.x) - ox) * w;
-                const float my = oy + (saturate(t.y) - oy) * w;
-                const float mz = oz + (saturate(t.z) - oz) * w;
-                r8 = f2u(saturate(mx) * 255.0f + 0.5f);
-                g8 = f2u(saturate(my) * 255.0f + 0.5f);
-                b8 = f2u(saturate(mz) * 255.0f + 0.5f);
-            }
-            rgb[j] = r8 | (g8 << 8) | (b8 << 16);
-        }
+        chunk_colors<PPT>(c, p, bc, li0, vec, f, rgb);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
         if (box.frags == 0) continue;
-        const int area = box.w * box.h;
         if (box.lds) {
-            for (int i = G::rank(); i < area; i += G::kSize) {
-                L.depth[i] = depth[box.pixel(i, p.width)];
-                L.rg[i] = 0ull;
-                L.bn[i] = 0ull;
-            }
+            color_stage<G>(box, p, depth, L);
             G::sync();
         }
-        // single-pixel points: runs of a thread's consecutive fragments on one
-        // pixel (Morton order) are summed in registers and added once
-        int runK = -1;
-        unsigned long long runRG = 0ull, runBN = 0ull;
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            if (f.px[j] < 0) continue;
-            const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
-            const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
-            const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
-            if (kR0) {
-                const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
-                const uint32_t d = box.lds ? L.depth[k] : depth[k];
-                if (d == 0u) continue;
-                if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                if (k == runK) {
-                    runRG += rg;
-                    runBN += bn;
-                    continue;
-                }
-                if (runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
-                runK = k;
-                runRG = rg;
-                runBN = bn;
-                continue;
-            }
-            const int rad = f.r[j];
-            for (int oy = -rad; oy <= rad; oy++)
-                for (int ox = -rad; ox <= rad; ox++) {
-                    if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
-                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
-                    const size_t pix = (size_t)ty * p.width + tx;
-                    const int k = box.lds ? box.local(tx, ty) : 0;
-                    const uint32_t d = box.lds ? L.depth[k] : depth[pix];
-                    if (d == 0u) continue;
-                    if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                    if (box.lds) {
-                        atomicAdd(L.rg + k, rg);
-                        atomicAdd(L.bn + k, bn);
-                    } else {
-                        atomicAdd(red + pix, r8);
-                        atomicAdd(green + pix, g8);
-                        atomicAdd(blue + pix, b8);
-                        atomicAdd(count + pix, 1u);
-                    }
-                }
-        }
-        if (kR0 && runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
+        color_frags<kR0, false, PPT>(f, rgb, box, p, depth, red, green, blue, count, L);
         if (box.lds) {
             G::sync();
-            // per-chunk sums stay below 2^32 (<= NT * PPT fragments x 255 per
-            // pixel), so the packed halves never carry into each other
-            for (int i = G::rank(); i < area; i += G::kSize) {
-                const unsigned long long bn = L.bn[i];
-                const uint32_t n = (uint32_t)(bn >> 32);
-                if (n == 0u) continue;
-                const unsigned long long rg = L.rg[i];
-                const size_t pix = box.pixel(i, p.width);
-                atomicAdd(red + pix, (uint32_t)rg);
-                atomicAdd(green + pix, (uint32_t)(rg >> 32));
-                atomicAdd(blue + pix, (uint32_t)bn);
-                atomicAdd(count + pix, n);
-            }
+            color_tile_flush<G>(box, p, L, red, green, blue, count);
             G::sync();
         }
     }
@@ -613,6 +791,57 @@ __device__ inline void color_chunks(const CloudDev& c, const PassParams& p, cons
 // ---------------------------------------------------------------- nearest pass body
 // nearestDepthUpdate + nearestIndexUpdate (NearestDepth/Shaders.metal:14-91,
 // NearestIndex/Shaders.metal:14-94) as one max of depth << 32 | ~index.
+template <bool kR0, bool kCheck, int PPT>
+__device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t base, const ChunkBox& box,
+                                     const PassParams& p, unsigned long long* lds,
+                                     unsigned long long* __restrict__ packed) {
+    int lastK = -1;
+    unsigned long long lastKey = 0ull;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        if (f.px[j] < 0) continue;
+        const uint32_t pi = base + (uint32_t)j;
+        // max of depth << 32 | ~index == the two-pass rule: deepest
+        // reverse-Z depth, then the lowest point index among equal depths
+        const unsigned long long key =
+            ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~pi);
+        if (kR0) {
+            if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
+                atomicMax(packed + (size_t)f.py[j] * p.width + f.px[j], key);
+                continue;
+            }
+            // as in the depth pass: skip a max this thread already exceeded
+            const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
+            if (k == lastK && key <= lastKey) continue;
+            lastKey = (k == lastK) ? (key > lastKey ? key : lastKey) : key;
+            lastK = k;
+            if (box.lds) atomicMax(lds + k, key);
+            else atomicMax(packed + (size_t)k, key);
+            continue;
+        }
+        const int r = f.r[j];
+        for (int oy = -r; oy <= r; oy++)
+            for (int ox = -r; ox <= r; ox++) {
+                if (!inside_footprint(ox, oy, r)) continue;
+                const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                if (box.lds && (!kCheck || box_holds(box, tx, ty))) atomicMax(lds + box.local(tx, ty), key);
+                else atomicMax(packed + (size_t)ty * p.width + tx, key);
+            }
+    }
+}
+
+template <typename G>
+__device__ __forceinline__ void nearest_flush(const ChunkBox& box, const PassParams& p, const unsigned long long* lds,
+                                     unsigned long long* __restrict__ packed) {
+    const int area = box.w * box.h;
+    for (int i = G::rank(); i < area; i += G::kSize) {
+        const unsigned long long k = lds[i];
+        if (k == 0ull) continue;
+        atomicMax(packed + box.pixel(i, p.width), k);
+    }
+}
+
 template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                       unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red) {
@@ -620,54 +849,38 @@ __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, co
     constexpr int kCap = kNearestTile / G::kCount;
     unsigned long long* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
-    for (uint32_t cb = lo + (uint32_t)(G::index() * G::kSize * PPT); cb < bc.active; cb += NT * PPT) {
+    const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    if (seg.lds) {
+        const int area = seg.w * seg.h;
+        for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0ull;
+        G::sync();
+        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+            const uint32_t li0 = cb + G::rank() * PPT;
+            ChunkFrags<PPT> f;
+            chunk_front<true, kR0, PPT>(c, p, bc, li0, vec, f);
+            nearest_frags<kR0, true, PPT>(f, bc.first + li0, seg, p, lds, packed);
+        }
+        G::sync();
+        nearest_flush<G>(seg, p, lds, packed);
+        G::sync();
+        return;
+    }
+    for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         const uint32_t li0 = cb + G::rank() * PPT;
         ChunkFrags<PPT> f;
         chunk_front<true, kR0, PPT>(c, p, bc, li0, vec, f);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
         if (box.frags == 0) continue;
-        const int area = box.w * box.h;
         if (box.lds) {
+            const int area = box.w * box.h;
             for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0ull;
             G::sync();
         }
-        int lastK = -1;
-        unsigned long long lastKey = 0ull;
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            if (f.px[j] < 0) continue;
-            const uint32_t pi = bc.first + li0 + (uint32_t)j;
-            // max of depth << 32 | ~index == the two-pass rule: deepest
-            // reverse-Z depth, then the lowest point index among equal depths
-            const unsigned long long key =
-                ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~pi);
-            if (kR0) {
-                // as in the depth pass: skip a max this thread already exceeded
-                const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
-                if (k == lastK && key <= lastKey) continue;
-                lastKey = (k == lastK) ? (key > lastKey ? key : lastKey) : key;
-                lastK = k;
-                if (box.lds) atomicMax(lds + k, key);
-                else atomicMax(packed + (size_t)k, key);
-                continue;
-            }
-            const int r = f.r[j];
-            for (int oy = -r; oy <= r; oy++)
-                for (int ox = -r; ox <= r; ox++) {
-                    if (!kR0 && !inside_footprint(ox, oy, r)) continue;
-                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
-                    if (box.lds) atomicMax(lds + box.local(tx, ty), key);
-                    else atomicMax(packed + (size_t)ty * p.width + tx, key);
-                }
-        }
+        nearest_frags<kR0, false, PPT>(f, bc.first + li0, box, p, lds, packed);
         if (box.lds) {
             G::sync();
-            for (int i = G::rank(); i < area; i += G::kSize) {
-                const unsigned long long k = lds[i];
-                if (k == 0ull) continue;
-                atomicMax(packed + box.pixel(i, p.width), k);
-            }
+            nearest_flush<G>(box, p, lds, packed);
             G::sync();
         }
     }
@@ -687,7 +900,7 @@ __device__ inline void depth_batch(const CloudDev& c, const PassParams& p, const
 template <bool kR0, bool kWave, int NT, int PPT>
 __device__ inline void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                    uint32_t hi, const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
-                                   uint32_t* blue, uint32_t* count, const ColorLds& L) {
+                                   uint32_t* blue, uint32_t* count, ColorLds L) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
     if (!kR0 && bc.rconst == 0) color_chunks<true, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
     else color_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);

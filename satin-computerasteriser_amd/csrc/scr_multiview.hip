@@ -123,7 +123,7 @@ __device__ inline void drain(const MVView* __restrict__ views, const MVItem* __r
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__global__ __launch_bounds__(NT) void k_mv_depth(const MVView* __restrict__ views, const MVItem* __restrict__ items,
+__global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* __restrict__ views, const MVItem* __restrict__ items,
                                                  uint32_t nUnits, int splits, uint32_t* heads) {
     __shared__ uint32_t lds[kDepthTile];
     __shared__ int red[5 * kMaxWaves];
@@ -140,9 +140,9 @@ __global__ __launch_bounds__(NT) void k_mv_color(const MVView* __restrict__ view
     __shared__ unsigned long long ldsRG[kColorTile];
     __shared__ unsigned long long ldsBC[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
-    const ColorLds L{ldsDepth, ldsRG, ldsBC, redbuf};
     drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
                    [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
+                       const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
                        const size_t s = (size_t)V.stride;
                        color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
                                                  V.planes + 3 * s, V.planes + 4 * s, L);
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(NT) void k_mv_color(const MVView* __restrict__ view
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__global__ __launch_bounds__(NT) void k_mv_nearest(const MVView* __restrict__ views,
+__global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView* __restrict__ views,
                                                    const MVItem* __restrict__ items, uint32_t nUnits, int splits,
                                                    uint32_t* heads) {
     __shared__ unsigned long long lds[kNearestTile];

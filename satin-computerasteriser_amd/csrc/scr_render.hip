@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
 // b, b + grid, ... up to the count the cull kernel left on the device.
 
 template <bool kR0>
-__global__ __launch_bounds__(kThreads) void k_depth(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
+__global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_depth(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
     __shared__ uint32_t lds[kDepthTile];
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kThreads) void k_color(CloudDev c, PassParams p, co
     __shared__ unsigned long long ldsRG[kColorTile];
     __shared__ unsigned long long ldsBC[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
-    const ColorLds L{ldsDepth, ldsRG, ldsBC, redbuf};
+    const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
         color_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, depth, red, green, blue, count, L);
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void k_resolve(PassParams p, const uint32_t* _
 }
 
 template <bool kR0>
-__global__ __launch_bounds__(kThreads) void k_nearest(CloudDev c, PassParams p,
+__global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_nearest(CloudDev c, PassParams p,
                                                       unsigned long long* __restrict__ packed) {
     __shared__ unsigned long long lds[kNearestTile];
     __shared__ int red[5 * kMaxWaves];

@@ -75,6 +75,8 @@ def lib() -> C.CDLL:
             "orc_log2f": (f, [f]), "orc_hash_unit": (f, [u32]), "orc_depth_to_uint": (u32, [f]),
             "orc_uint_to_depth": (f, [u32]), "orc_f2u": (u32, [f]), "orc_f2i": (C.c_int32, [f]),
             "orc_mat4_mul": (None, [vp, vp, vp]), "orc_mat4_vec": (None, [vp, vp, vp]),
+            "orc_transform_point": (None, [vp, vp, vp]),
+            "orc_decode_point": (None, [u32, vp, vp, vp, vp, C.c_int, C.c_int, vp]),
             "orc_morton30": (u32, [u32, u32, u32]), "orc_morton63": (C.c_uint64, [u32, u32, u32]),
             "orc_inside_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int]),
             "orc_precision_level": (C.c_int, [f]), "orc_lod_threshold": (f, [f, C.c_int]),
@@ -137,6 +139,26 @@ def mat_mul(a, b) -> np.ndarray:
     a, b = _f32(a), _f32(b)
     out = np.zeros((4, 4), np.float32)
     lib().orc_mat4_mul(a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
+
+def transform_point(m, p) -> np.ndarray:
+    """m (column-major 4x4) applied to the point (p, 1) as the render passes do."""
+    m = _f32(m).reshape(-1)
+    p = _f32(p).reshape(-1)
+    out = np.zeros(4, np.float32)
+    lib().orc_transform_point(m.ctypes.data, p.ctypes.data, out.ctypes.data)
+    return out
+
+
+def decode_point(batch: np.ndarray, words, level: int, fused: bool = True) -> np.ndarray:
+    """decodePoint of the single point whose (low, med, high) words are given,
+    under the batch record ``batch`` (one RASTER_BATCH_DTYPE row; firstPoint ignored)."""
+    b = np.array(batch, dtype=RASTER_BATCH_DTYPE).reshape(1)
+    w = [np.array([x], dtype=np.uint32) for x in words]
+    out = np.zeros(3, np.float32)
+    lib().orc_decode_point(0, b.ctypes.data, w[0].ctypes.data, w[1].ctypes.data, w[2].ctypes.data, level,
+                           1 if fused else 0, out.ctypes.data)
     return out
 
 

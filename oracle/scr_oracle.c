@@ -114,6 +114,15 @@ void orc_mat4_vec(const float* m, const float* v, float* out) {
     memcpy(out, r, sizeof r);
 }
 
+/* The per-point transform of the render passes (C1): a point (w = 1)
+ * through m, column sums in source order with every product-then-add fused,
+ * fma(c2, z, fma(c1, y, c0 x)) + c3. */
+void orc_transform_point(const float* m, const float p[3], float out[4]) {
+    float r[4];
+    for (int i = 0; i < 4; i++) r[i] = fmaf(m[2 * 4 + i], p[2], fmaf(m[1 * 4 + i], p[1], m[0 * 4 + i] * p[0])) + m[3 * 4 + i];
+    memcpy(out, r, sizeof r);
+}
+
 void orc_mat4_mul(const float* a, const float* b, float* out) {
     float r[16];
     for (int c = 0; c < 4; c++) orc_mat4_vec(a, b + c * 4, r + c * 4);
@@ -158,11 +167,9 @@ int orc_footprint_radius(const float p[3], const scr_raster_file* file, const fl
     const float mn = cfg->minimumPointSize, mx = cfg->maximumPointSize;
     const float lo = fmaxf(fminf(mn, mx), 1.0f);
     const float hi = fmaxf(fmaxf(mn, mx), lo);
-    const float p4[4] = {p[0], p[1], p[2], 1.0f};
     float wp[4], vp[4];
-    orc_mat4_vec(file->world, p4, wp);
-    const float w4[4] = {wp[0], wp[1], wp[2], 1.0f};
-    orc_mat4_vec(view, w4, vp);
+    orc_transform_point(file->world, p, wp);
+    orc_transform_point(view, wp, vp);
     const int ortho = (proj[3 * 4 + 3] == 1.0f);
     float ps;
     if (cfg->pointSizeMode == 1) {
@@ -197,9 +204,11 @@ int orc_inside_footprint(int ox, int oy, int radius) {
     return (x * x + y * y) <= r * r;
 }
 
-/* decodePoint — Common.metal:73-117 */
+/* decodePoint — Common.metal:73-117 (fused: q * s + min as one fmaf, the
+ * render passes' contract C1) and NearestPointCPUReference.swift:87-117
+ * (fused = 0: Swift rounds the product and the sum separately). */
 void orc_decode_point(uint32_t pi, const scr_raster_batch* b, const uint32_t* low,
-                      const uint32_t* med, const uint32_t* high, int level, float out[3]) {
+                      const uint32_t* med, const uint32_t* high, int level, int fused, float out[3]) {
     const float mn[3] = {b->minX, b->minY, b->minZ};
     const float mx[3] = {b->maxX, b->maxY, b->maxZ};
     float size[3];
@@ -226,7 +235,8 @@ void orc_decode_point(uint32_t pi, const scr_raster_batch* b, const uint32_t* lo
         steps = 1024.0f;
     }
     const float invSteps = 1.0f / steps; /* power of two: exact */
-    for (int a = 0; a < 3; a++) out[a] = (float)q[a] * (size[a] * invSteps) + mn[a];
+    for (int a = 0; a < 3; a++)
+        out[a] = fused ? fmaf((float)q[a], size[a] * invSteps, mn[a]) : (float)q[a] * (size[a] * invSteps) + mn[a];
 }
 
 /* makePlane / intersectsFrustum — Common.metal:119-153 */
@@ -391,10 +401,9 @@ static int orc_front(const orc_cloud* c, const scr_raster_batch* b, const scr_ra
     const float pointLevel = (float)((uint32_t)c->levels[pointIndex] & 7u);
     const float dither = (cfg->enableLODDither != 0) ? orc_hash_unit(pointIndex) : 0.5f;
     if (dither >= (vb->lodThreshold - pointLevel) + 0.5f) return 0;
-    orc_decode_point(pointIndex, b, c->xyzLow, c->xyzMed, c->xyzHigh, vb->level, fr->point);
-    const float p4[4] = {fr->point[0], fr->point[1], fr->point[2], 1.0f};
+    orc_decode_point(pointIndex, b, c->xyzLow, c->xyzMed, c->xyzHigh, vb->level, 1, fr->point);
     float clip[4];
-    orc_mat4_vec(f->transform, p4, clip);
+    orc_transform_point(f->transform, fr->point, clip);
     if (clip[3] <= 0.0f) return 0;
     const float rw = orc_rcp(clip[3]);
     const float nx = clip[0] * rw, ny = clip[1] * rw, nz = clip[2] * rw;
@@ -404,8 +413,8 @@ static int orc_front(const orc_cloud* c, const scr_raster_batch* b, const scr_ra
     } else {
         if (nz < 0.0f || nz > 1.0f) return 0;
     }
-    int px = orc_f2i((nx * 0.5f + 0.5f) * (float)width);
-    int py = orc_f2i((ny * 0.5f + 0.5f) * (float)height);
+    int px = orc_f2i(fmaf(nx, 0.5f, 0.5f) * (float)width);
+    int py = orc_f2i(fmaf(ny, 0.5f, 0.5f) * (float)height);
     if (px < 0 || px >= width || py < 0 || py >= height) return 0;
     py = height - 1 - py;
     fr->ndcZ = nz;
@@ -706,7 +715,7 @@ void orc_nearest_cpu_reference(const orc_cloud* c, int width, int height, const 
         for (uint32_t li = 0; li < b->numPoints; li++) {
             const uint32_t pi = b->firstPoint + li;
             float pt[3], clip[4];
-            orc_decode_point(pi, b, c->xyzLow, c->xyzMed, c->xyzHigh, level, pt);
+            orc_decode_point(pi, b, c->xyzLow, c->xyzMed, c->xyzHigh, level, 0, pt);
             const float p4[4] = {pt[0], pt[1], pt[2], 1.0f};
             orc_mat4_vec(f->transform, p4, clip);
             if (clip[3] <= 0.0f) continue;

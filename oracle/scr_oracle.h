@@ -48,13 +48,14 @@ uint32_t orc_f2u(float x);
 int32_t orc_f2i(float x);
 void orc_mat4_mul(const float* a, const float* b, float* out);       /* out = a * b */
 void orc_mat4_vec(const float* m, const float* v4, float* out4);     /* out = m * v */
+void orc_transform_point(const float* m, const float p[3], float out4[4]); /* m * (p, 1), fused */
 uint32_t orc_morton30(uint32_t qx, uint32_t qy, uint32_t qz);
 uint64_t orc_morton63(uint32_t qx, uint32_t qy, uint32_t qz);
 int orc_footprint_radius(const float p[3], const scr_raster_file* file, const float* view,
                          const float* proj, int width, int height, const scr_config* cfg);
 int orc_inside_footprint(int ox, int oy, int radius);
 void orc_decode_point(uint32_t pointIndex, const scr_raster_batch* b, const uint32_t* low,
-                      const uint32_t* med, const uint32_t* high, int level, float out[3]);
+                      const uint32_t* med, const uint32_t* high, int level, int fused, float out[3]);
 int orc_intersects_frustum(const float* m, const float mn[3], const float mx[3]);
 float orc_pixel_size_on_screen(const scr_raster_batch* b, const scr_raster_file* f,
                                const float* view, const float* proj, int width, int height);

@@ -259,9 +259,9 @@ __device__ inline void chunk_decode(const CloudDev& c, const PassParams& p, cons
             qy = (l >> 10) & 1023u;
             qz = (l >> 20) & 1023u;
         }
-        const float x = (float)qx * bc.sx + bc.mnx;
-        const float y = (float)qy * bc.sy + bc.mny;
-        const float z = (float)qz * bc.sz + bc.mnz;
+        const float x = fmaf((float)qx, bc.sx, bc.mnx);
+        const float y = fmaf((float)qy, bc.sy, bc.mny);
+        const float z = fmaf((float)qz, bc.sz, bc.mnz);
         const f4 clip = mul_point(bc.T, x, y, z);
         ok = ok && !(clip.w <= 0.0f);
         const float rw = rcp(clip.w);
@@ -269,8 +269,8 @@ __device__ inline void chunk_decode(const CloudDev& c, const PassParams& p, cons
         ok = ok && !(nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f);
         if (kNearestZ) ok = ok && !(nz <= 0.0f || nz > 1.0f);
         else ok = ok && !(nz < 0.0f || nz > 1.0f);
-        const int ix = f2i_hw((nx * 0.5f + 0.5f) * (float)p.width);
-        const int iy = f2i_hw((ny * 0.5f + 0.5f) * (float)p.height);
+        const int ix = f2i_hw(fmaf(nx, 0.5f, 0.5f) * (float)p.width);
+        const int iy = f2i_hw(fmaf(ny, 0.5f, 0.5f) * (float)p.height);
         ok = ok && !(ix < 0 || ix >= p.width || iy < 0 || iy >= p.height);
         f.px[j] = ok ? ix : -1;
         f.py[j] = p.height - 1 - iy;
@@ -422,8 +422,8 @@ __device__ inline ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, 
     // screen x and the unflipped row of the corner, kept finite and far from
     // the int range (a corner off the screen only widens the clamp below)
     const float lim = 65536.0f;
-    float sx = ((clip.x * rw) * 0.5f + 0.5f) * (float)p.width;
-    float sy = ((clip.y * rw) * 0.5f + 0.5f) * (float)p.height;
+    float sx = fmaf(clip.x * rw, 0.5f, 0.5f) * (float)p.width;
+    float sy = fmaf(clip.y * rw, 0.5f, 0.5f) * (float)p.height;
     int ok = (clip.w > 0.0f && sx == sx && sy == sy) ? 1 : 0;
     sx = fminf(fmaxf(sx, -lim), lim);
     sy = fminf(fmaxf(sy, -lim), lim);

@@ -1,8 +1,11 @@
 // scr_math.hpp — device arithmetic of the render passes (gfx950).
 //
 // Implements the shared arithmetic contract of the rasteriser:
-//   * binary32 RNE, no contraction (every .hip file is built with
-//     -ffp-contract=off), source operation order of the Metal passes;
+//   * binary32 RNE, source operation order of the Metal passes; no implicit
+//     contraction (every .hip file is built with -ffp-contract=off), and
+//     explicit fmaf exactly where the per-point front end multiplies then
+//     adds: decodePoint's q * s + min, the point transform (mul_point) and
+//     the viewport's ndc * 0.5 + 0.5;
 //   * a / b in the Metal passes is a * (1.0f / b) with the correctly rounded
 //     reciprocal (Metal fast-math reciprocal division made exact);
 //   * sqrt correctly rounded; log2 = scr_log2f (double-precision, + - * / only);
@@ -134,13 +137,15 @@ __host__ __device__ inline f4 mul(const float* m, f4 v) {
     return r;
 }
 
-// Point (w = 1): c3 * 1 is c3 exactly.
+// A point (w = 1) through m, the per-point transform of the draw passes:
+// the column sums in source order with each product-then-add fused,
+// fma(c2, z, fma(c1, y, c0 * x)) + c3 (c3 * 1 is c3 exactly).
 __host__ __device__ inline f4 mul_point(const float* m, float x, float y, float z) {
     f4 r;
-    r.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12];
-    r.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13];
-    r.z = ((m[2] * x + m[6] * y) + m[10] * z) + m[14];
-    r.w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15];
+    r.x = fmaf(m[8], z, fmaf(m[4], y, m[0] * x)) + m[12];
+    r.y = fmaf(m[9], z, fmaf(m[5], y, m[1] * x)) + m[13];
+    r.z = fmaf(m[10], z, fmaf(m[6], y, m[2] * x)) + m[14];
+    r.w = fmaf(m[11], z, fmaf(m[7], y, m[3] * x)) + m[15];
     return r;
 }
 

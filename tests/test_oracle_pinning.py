@@ -2,6 +2,7 @@
 algorithms and brute force (so that a dropped term, wrong shift or
 transposed operand in the oracle fails here)."""
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -173,3 +174,71 @@ def test_nearest_two_pass_matches_packed_max_rule():
     rgba = out["rgba8"]
     assert np.all((rgba[covered] & 0x00FFFFFF) == (cloud.colors[idx[covered]] & 0x00FFFFFF))
     assert np.all((rgba[covered] >> 24) == 255)
+
+
+def _round_f32(v: Fraction) -> np.float32:
+    """The binary32 nearest to the exact rational v (ties to even; normal range)."""
+    if v == 0:
+        return np.float32(0.0)
+    sign, a = (-1, -v) if v < 0 else (1, v)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if a < Fraction(2) ** e:
+        e -= 1
+    m = a / Fraction(2) ** (e - 23)          # in [2^23, 2^24)
+    n = m.numerator // m.denominator
+    rem = m - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return np.float32(sign * float(Fraction(n) * Fraction(2) ** (e - 23)))
+
+
+def _fma(a, b, c) -> np.float32:
+    return _round_f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def test_point_transform_is_fused_in_source_order():
+    # fma(c2, z, fma(c1, y, c0 x)) + c3 per row, each fma rounded once (exact
+    # rationals); the unfused chain differs often enough that it would fail
+    rng = np.random.default_rng(0x5EED)
+    differ = 0
+    for _ in range(1000):
+        m = (rng.uniform(0.5, 2.0, 16) * rng.choice([-1.0, 1.0], 16)).astype(np.float32)
+        p = (rng.uniform(0.5, 2.0, 3) * rng.choice([-1.0, 1.0], 3)).astype(np.float32)
+        got = orc.transform_point(m, p)
+        for i in range(4):
+            a = np.float32(m[i] * p[0])
+            a = _fma(m[4 + i], p[1], a)
+            a = _fma(m[8 + i], p[2], a)
+            a = np.float32(a + m[12 + i])
+            assert got[i] == a, (m, p, i)
+            u = np.float32(np.float32(np.float32(m[i] * p[0]) + np.float32(m[4 + i] * p[1])) +
+                           np.float32(m[8 + i] * p[2]))
+            differ += int(np.float32(u + m[12 + i]) != a)
+    assert differ > 200
+
+
+def test_decode_point_is_fused_and_the_swift_reference_is_not():
+    # decodePoint of the passes: fma(float(q), size * 2^-bits, min); the
+    # NearestPointCPUReference restatement rounds the product first
+    rng = np.random.default_rng(0xDEC0)
+    b = np.zeros(1, dtype=orc.RASTER_BATCH_DTYPE)
+    differ = 0
+    for _ in range(1000):
+        mn = rng.uniform(-3000.0, 3000.0, 3).astype(np.float32)
+        size = rng.uniform(1.0, 500.0, 3).astype(np.float32)
+        b["minX"], b["minY"], b["minZ"] = mn
+        b["maxX"], b["maxY"], b["maxZ"] = mn + size
+        level = int(rng.integers(0, 3))
+        words = [int(w) for w in rng.integers(0, 1 << 30, 3)]
+        got = orc.decode_point(b[0], words, level)
+        ref = orc.decode_point(b[0], words, level, fused=False)
+        bits = (30, 20, 10)[level]
+        for a in range(3):
+            lo, me, hi = ((w >> (10 * a)) & 1023 for w in words)
+            q = ((lo << 20) | (me << 10) | hi) if level == 0 else (((lo << 10) | me) if level == 1 else lo)
+            ext = np.float32(b["max" + "XYZ"[a]][0] - b["min" + "XYZ"[a]][0])
+            s = np.float32(max(ext, np.float32(1e-6)) * np.float32(2.0 ** -bits))
+            assert got[a] == _fma(np.float32(q), s, mn[a]), (a, q, s, mn[a])
+            assert ref[a] == np.float32(np.float32(np.float32(q) * s) + mn[a])
+            differ += int(got[a] != ref[a])
+    assert differ > 100

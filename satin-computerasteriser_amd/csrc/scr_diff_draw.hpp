@@ -42,7 +42,7 @@ __device__ __forceinline__ void diff_frags(const ChunkFrags<kDiffPPT>& f, uint32
 #pragma unroll
     for (int j = 0; j < kDiffPPT; j++) {
         if (f.px[j] < 0) continue;
-        const float4 t = params[base + (uint32_t)j];
+        const float4 t = gload_rec(params + base + (uint32_t)j);
         const unsigned long long q0 = diff_fixed(t.x), q1 = diff_fixed(t.y), q2 = diff_fixed(t.z);
         const int rad = kR0 ? 0 : f.r[j];
         for (int oy = -rad; oy <= rad; oy++)
@@ -53,19 +53,19 @@ __device__ __forceinline__ void diff_frags(const ChunkFrags<kDiffPPT>& f, uint32
                 const size_t pix = (size_t)ty * p.width + tx;
                 const bool inTile = box.lds && (!kCheck || box_holds(box, tx, ty));
                 const int k = inTile ? box.local(tx, ty) : 0;
-                const uint32_t d = inTile ? L.depth[k] : depth[pix];
+                const uint32_t d = inTile ? lds_load(L.depth + k) : gload(depth + pix);
                 if (d == 0u) continue;
                 if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
                 if (inTile) {
-                    atomicAdd(&L.s[0][k], q0);
-                    atomicAdd(&L.s[1][k], q1);
-                    atomicAdd(&L.s[2][k], q2);
-                    atomicAdd(&L.n[k], 1u);
+                    lds_add(&L.s[0][k], q0);
+                    lds_add(&L.s[1][k], q1);
+                    lds_add(&L.s[2][k], q2);
+                    lds_add(&L.n[k], 1u);
                 } else {
-                    atomicAdd(sums + pix, q0);
-                    atomicAdd(sums + stride + pix, q1);
-                    atomicAdd(sums + 2 * stride + pix, q2);
-                    atomicAdd(count + pix, 1u);
+                    gadd(sums + pix, q0);
+                    gadd(sums + stride + pix, q1);
+                    gadd(sums + 2 * stride + pix, q2);
+                    gadd(count + pix, 1u);
                 }
             }
     }
@@ -75,9 +75,11 @@ __device__ __forceinline__ void diff_stage(const ChunkBox& box, const PassParams
                                            const uint32_t* __restrict__ depth, DiffLds& L) {
     const int area = box.w * box.h;
     for (int i = threadIdx.x; i < area; i += kDiffThreads) {
-        L.depth[i] = depth[box.pixel(i, p.width)];
-        L.s[0][i] = L.s[1][i] = L.s[2][i] = 0ull;
-        L.n[i] = 0u;
+        lds_store(&L.depth[i], gload(depth + box.pixel(i, p.width)));
+        lds_store(&L.s[0][i], 0ull);
+        lds_store(&L.s[1][i], 0ull);
+        lds_store(&L.s[2][i], 0ull);
+        lds_store(&L.n[i], 0u);
     }
 }
 
@@ -85,13 +87,13 @@ __device__ __forceinline__ void diff_tile_flush(const ChunkBox& box, const PassP
                                                 size_t stride, uint32_t* count, const DiffLds& L) {
     const int area = box.w * box.h;
     for (int i = threadIdx.x; i < area; i += kDiffThreads) {
-        const uint32_t n = L.n[i];
+        const uint32_t n = lds_load(&L.n[i]);
         if (n == 0u) continue;
         const size_t pix = box.pixel(i, p.width);
-        atomicAdd(sums + pix, L.s[0][i]);
-        atomicAdd(sums + stride + pix, L.s[1][i]);
-        atomicAdd(sums + 2 * stride + pix, L.s[2][i]);
-        atomicAdd(count + pix, n);
+        gadd(sums + pix, lds_load(&L.s[0][i]));
+        gadd(sums + stride + pix, lds_load(&L.s[1][i]));
+        gadd(sums + 2 * stride + pix, lds_load(&L.s[2][i]));
+        gadd(count + pix, n);
     }
 }
 
@@ -161,17 +163,17 @@ __device__ inline void diff_backward_chunks(const CloudDev& c, const PassParams&
                     const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                     if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
                     const size_t pix = (size_t)ty * p.width + tx;
-                    const uint32_t d = depth[pix];
+                    const uint32_t d = gload(depth + pix);
                     if (d == 0u) continue;
                     if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                    const double inv = 1.0 / (double)count[pix];
-                    g0 += (double)dLdImage[pix * 3 + 0] * inv;
-                    g1 += (double)dLdImage[pix * 3 + 1] * inv;
-                    g2 += (double)dLdImage[pix * 3 + 2] * inv;
+                    const double inv = 1.0 / (double)gload(count + pix);
+                    g0 += (double)gload(dLdImage + pix * 3 + 0) * inv;
+                    g1 += (double)gload(dLdImage + pix * 3 + 1) * inv;
+                    g2 += (double)gload(dLdImage + pix * 3 + 2) * inv;
                     any = true;
                 }
             if (!any) continue;
-            double* gp = grad + (size_t)(bc.first + li0 + (uint32_t)j) * 4;
+            SCR_GLOBAL double* gp = gptr(grad + (size_t)(bc.first + li0 + (uint32_t)j) * 4);
             gp[0] += g0;
             gp[1] += g1;
             gp[2] += g2;

@@ -23,6 +23,58 @@ constexpr int kMaxWaves = 8;        // NT <= 512
 constexpr int kDepthTile = 4096;    // u32 per pixel: 16 KB
 constexpr int kColorTile = 2048;    // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 40 KB
 constexpr int kNearestTile = 2048;  // u64 per pixel: 16 KB
+constexpr int kSpare = 64;          // words past a tile: one per lane, the target of no-op updates
+
+// LDS tiles are read and updated through pointers typed as LDS: code that
+// picks between an LDS and a global pointer would otherwise compile to flat
+// instructions, which are slower than the ds_* forms.
+#define SCR_LDS __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ SCR_LDS T* lds_ptr(T* p) { return (SCR_LDS T*)p; }
+template <typename T>
+__device__ __forceinline__ T lds_load(const T* p) { return *lds_ptr(const_cast<T*>(p)); }
+template <typename T>
+__device__ __forceinline__ void lds_store(T* p, T v) { *lds_ptr(p) = v; }
+template <typename T>
+__device__ __forceinline__ void lds_max(T* p, T v) {
+    __hip_atomic_fetch_max(lds_ptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T v) {
+    __hip_atomic_fetch_add(lds_ptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// The same for device memory: pointers that reach the draw bodies through
+// memory (the multi-view tables) are not known to be global, and would
+// otherwise be read and updated with flat instructions.
+#define SCR_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ SCR_GLOBAL T* gptr(T* p) { return (SCR_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ const SCR_GLOBAL T* gptr(const T* p) { return (const SCR_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) { return *gptr(p); }
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// A 16-byte-aligned record of whole 16-byte words (vector types and structs
+// have no constructor from a global reference, so they go through words).
+template <typename T>
+__device__ __forceinline__ T gload_rec(const T* p) {
+    static_assert(sizeof(T) % 16 == 0, "record of 16-byte words");
+    u32x4 w[sizeof(T) / 16];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 16); i++) w[i] = ((const SCR_GLOBAL u32x4*)p)[i];
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
+}
+template <typename T>
+__device__ __forceinline__ void gmax(T* p, T v) {
+    __hip_atomic_fetch_max(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void gadd(T* p, T v) {
+    __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Waves per SIMD the depth and nearest kernels are compiled for: with up to
 // 8 points per thread, 4 (at most 128 VGPRs) hide memory latency better than
@@ -134,7 +186,7 @@ struct BatchCtx {
 // `hi` bounds the local point indices drawn (at most the active prefix).
 __device__ inline BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
                                      uint32_t hi) {
-    const scr_raster_batch b = c.batches[vb.batchIndex];
+    const scr_raster_batch b = gload_rec(c.batches + vb.batchIndex);
     BatchCtx bc;
     bc.level = vb.level;
     bc.lodThreshold = vb.lodThreshold;
@@ -152,7 +204,7 @@ __device__ inline BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, con
     bc.sz = fmaxf(b.maxZ - b.minZ, 0.000001f) * inv;
     const scr_raster_file* f = c.files + b.fileIndex;
 #pragma unroll
-    for (int i = 0; i < 16; i++) bc.T[i] = f->transform[i];
+    for (int i = 0; i < 16; i++) bc.T[i] = gload(f->transform + i);
     bc.W = f->world;
     bc.rconst = batch_const_radius(p, f->world, b);
     return bc;
@@ -175,10 +227,10 @@ struct ChunkFrags {
 
 template <int PPT>
 __device__ inline void load_words(const uint32_t* src, uint32_t base, uint32_t* out) {
-    const uint4* p = reinterpret_cast<const uint4*>(src + base);
+    const SCR_GLOBAL u32x4* p = (const SCR_GLOBAL u32x4*)(src + base);
 #pragma unroll
     for (int q = 0; q < PPT / 4; q++) {
-        const uint4 v = p[q];
+        const u32x4 v = p[q];
         out[4 * q + 0] = v.x;
         out[4 * q + 1] = v.y;
         out[4 * q + 2] = v.z;
@@ -205,7 +257,7 @@ __device__ inline void chunk_fetch(const CloudDev& c, const BatchCtx& bc, uint32
         if (bc.level == 0) load_words<PPT>(c.xyzHigh, base, r.hi);
         if (!bc.keepAll) {
 #pragma unroll
-            for (int q = 0; q < PPT / 4; q++) r.lv4[q] = reinterpret_cast<const uint32_t*>(c.levels + base)[q];
+            for (int q = 0; q < PPT / 4; q++) r.lv4[q] = gload(reinterpret_cast<const uint32_t*>(c.levels + base) + q);
         }
     } else {
 #pragma unroll
@@ -214,10 +266,10 @@ __device__ inline void chunk_fetch(const CloudDev& c, const BatchCtx& bc, uint32
         for (int j = 0; j < PPT; j++) {
             const uint32_t pi = base + (uint32_t)j;
             const bool ok = (uint32_t)j < n;
-            r.lo[j] = ok ? c.xyzLow[pi] : 0u;
-            r.me[j] = (ok && bc.level <= 1) ? c.xyzMed[pi] : 0u;
-            r.hi[j] = (ok && bc.level == 0) ? c.xyzHigh[pi] : 0u;
-            if (ok && !bc.keepAll) r.lv4[j / 4] |= (uint32_t)c.levels[pi] << (8 * (j % 4));
+            r.lo[j] = ok ? gload(c.xyzLow + pi) : 0u;
+            r.me[j] = (ok && bc.level <= 1) ? gload(c.xyzMed + pi) : 0u;
+            r.hi[j] = (ok && bc.level == 0) ? gload(c.xyzHigh + pi) : 0u;
+            if (ok && !bc.keepAll) r.lv4[j / 4] |= (uint32_t)gload(c.levels + pi) << (8 * (j % 4));
         }
     }
 }
@@ -363,26 +415,26 @@ __device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams&
     constexpr int kWaves = kWave ? 1 : NT / 64;
     const int w = threadIdx.x >> 6;
     if (!kWave && __lane_id() == 0) {
-        red[w] = x0;
-        red[kWaves + w] = y0;
-        red[2 * kWaves + w] = x1;
-        red[3 * kWaves + w] = y1;
-        red[4 * kWaves + w] = n;
+        lds_store(red + w, x0);
+        lds_store(red + kWaves + w, y0);
+        lds_store(red + 2 * kWaves + w, x1);
+        lds_store(red + 3 * kWaves + w, y1);
+        lds_store(red + 4 * kWaves + w, n);
     }
     if (!kWave) {
         __syncthreads();
-        x0 = red[0];
-        y0 = red[kWaves];
-        x1 = red[2 * kWaves];
-        y1 = red[3 * kWaves];
-        n = red[4 * kWaves];
+        x0 = lds_load(red);
+        y0 = lds_load(red + kWaves);
+        x1 = lds_load(red + 2 * kWaves);
+        y1 = lds_load(red + 3 * kWaves);
+        n = lds_load(red + 4 * kWaves);
 #pragma unroll
         for (int k = 1; k < kWaves; k++) {
-            x0 = min(x0, red[k]);
-            y0 = min(y0, red[kWaves + k]);
-            x1 = max(x1, red[2 * kWaves + k]);
-            y1 = max(y1, red[3 * kWaves + k]);
-            n += red[4 * kWaves + k];
+            x0 = min(x0, lds_load(red + k));
+            y0 = min(y0, lds_load(red + kWaves + k));
+            x1 = max(x1, lds_load(red + 2 * kWaves + k));
+            y1 = max(y1, lds_load(red + 3 * kWaves + k));
+            n += lds_load(red + 4 * kWaves + k);
         }
         __syncthreads();
     }
@@ -487,15 +539,15 @@ __device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const Chun
         const uint32_t d = depth_to_uint_hw(f.z[j]);
         if (kR0) {
             if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
-                atomicMax(depth + (size_t)f.py[j] * p.width + f.px[j], d);
+                gmax(depth + (size_t)f.py[j] * p.width + f.px[j], d);
                 continue;
             }
             const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
             if (k == lastK && d <= lastD) continue;
             lastD = (k == lastK) ? max(lastD, d) : d;
             lastK = k;
-            if (box.lds) atomicMax(lds + k, d);
-            else atomicMax(depth + (size_t)k, d);
+            if (box.lds) lds_max(lds + k, d);
+            else gmax(depth + (size_t)k, d);
             continue;
         }
         const int r = f.r[j];
@@ -504,9 +556,42 @@ __device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const Chun
                 if (!inside_footprint(ox, oy, r)) continue;
                 const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                 if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                if (box.lds && (!kCheck || box_holds(box, tx, ty))) atomicMax(lds + box.local(tx, ty), d);
-                else atomicMax(depth + (size_t)ty * p.width + tx, d);
+                if (box.lds && (!kCheck || box_holds(box, tx, ty))) lds_max(lds + box.local(tx, ty), d);
+                else gmax(depth + (size_t)ty * p.width + tx, d);
             }
+    }
+}
+
+// depth_frags for single-pixel points meeting in a batch tile, with no
+// per-fragment branches: a fragment that is invalid, outside the box or
+// already exceeded by this thread aims a zero max (a no-op) at the lane's
+// spare word past the tile; fragments outside the box (none in practice, the
+// box bounding the batch) take one wave-uniform branch to the plane.
+template <int PPT>
+__device__ __forceinline__ void depth_frags_tile(const ChunkFrags<PPT>& f, const ChunkBox& box,
+                                                 const PassParams& p, uint32_t* lds, uint32_t* spare,
+                                                 uint32_t* __restrict__ depth) {
+    int lastK = -1;
+    uint32_t lastD = 0u;
+    bool outside = false;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        const uint32_t d = depth_to_uint_hw(f.z[j]);
+        const bool valid = f.px[j] >= 0;
+        const bool in = valid && box_holds(box, f.px[j], f.py[j]);
+        outside = outside || (valid && !in);
+        const int k = box.local(f.px[j], f.py[j]);
+        const bool same = k == lastK;
+        const bool fresh = in && !(same && d <= lastD);
+        lastD = in ? (same ? max(lastD, d) : d) : lastD;
+        lastK = in ? k : lastK;
+        lds_max(fresh ? lds + k : spare, fresh ? d : 0u);
+    }
+    if (__ballot(outside) != 0ull) {
+#pragma unroll
+        for (int j = 0; j < PPT; j++)
+            if (f.px[j] >= 0 && !box_holds(box, f.px[j], f.py[j]))
+                gmax(depth + (size_t)f.py[j] * p.width + f.px[j], depth_to_uint_hw(f.z[j]));
     }
 }
 
@@ -517,9 +602,9 @@ __device__ __forceinline__ void depth_flush(const ChunkBox& box, const PassParam
                                    uint32_t* __restrict__ depth) {
     const int area = box.w * box.h;
     for (int i = G::rank(); i < area; i += G::kSize) {
-        const uint32_t d = lds[i];
+        const uint32_t d = lds_load(lds + i);
         if (d == 0u) continue;
-        atomicMax(depth + box.pixel(i, p.width), d);
+        gmax(depth + box.pixel(i, p.width), d);
     }
 }
 
@@ -534,12 +619,14 @@ __device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, cons
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.lds) {
         const int area = seg.w * seg.h;
-        for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0u;
+        for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
         G::sync();
+        uint32_t* spare = ldsAll + kDepthTile + __lane_id();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             ChunkFrags<PPT> f;
             chunk_front<false, kR0, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
-            depth_frags<kR0, true, PPT>(f, seg, p, lds, depth);
+            if (kR0) depth_frags_tile<PPT>(f, seg, p, lds, spare, depth);
+            else depth_frags<kR0, true, PPT>(f, seg, p, lds, depth);
         }
         G::sync();
         depth_flush<G>(seg, p, lds, depth);
@@ -553,7 +640,7 @@ __device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, cons
         if (box.frags == 0) continue;
         if (box.lds) {
             const int area = box.w * box.h;
-            for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0u;
+            for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
             G::sync();
         }
         depth_frags<kR0, false, PPT>(f, box, p, lds, depth);
@@ -591,13 +678,13 @@ __device__ __forceinline__ ColorLds color_lds(uint32_t* depth, unsigned long lon
 __device__ __forceinline__ void color_flush(bool lds, int k, unsigned long long rg, unsigned long long bn, ColorLds L,
                                    uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count) {
     if (lds) {
-        atomicAdd(L.rg + k, rg);
-        atomicAdd(L.bn + k, bn);
+        lds_add(L.rg + k, rg);
+        lds_add(L.bn + k, bn);
     } else {
-        atomicAdd(red + k, (uint32_t)rg);
-        atomicAdd(green + k, (uint32_t)(rg >> 32));
-        atomicAdd(blue + k, (uint32_t)bn);
-        atomicAdd(count + k, (uint32_t)(bn >> 32));
+        gadd(red + k, (uint32_t)rg);
+        gadd(green + k, (uint32_t)(rg >> 32));
+        gadd(blue + k, (uint32_t)bn);
+        gadd(count + k, (uint32_t)(bn >> 32));
     }
 }
 
@@ -611,7 +698,7 @@ __device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams
         load_words<PPT>(c.colors, base, rgb);
     } else {
 #pragma unroll
-        for (int j = 0; j < PPT; j++) rgb[j] = f.px[j] >= 0 ? c.colors[base + (uint32_t)j] : 0u;
+        for (int j = 0; j < PPT; j++) rgb[j] = f.px[j] >= 0 ? gload(c.colors + base + (uint32_t)j) : 0u;
     }
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
@@ -623,7 +710,7 @@ __device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams
         else if (p.colorizeOverdraw != 0) color = 0x00010101u;
         uint32_t r8 = color & 0xffu, g8 = (color >> 8) & 0xffu, b8 = (color >> 16) & 0xffu;
         if (p.applyTint != 0) {
-            const float4 t = c.tints ? c.tints[pi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float4 t = c.tints ? gload_rec(c.tints + pi) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (t.w < 0.0f) {  // discard sentinel, ColorPass/Shaders.metal:119
                 f.px[j] = -1;
                 continue;
@@ -659,16 +746,16 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
         if (kR0) {
             if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
                 const size_t pix = (size_t)f.py[j] * p.width + f.px[j];
-                const uint32_t d = depth[pix];
+                const uint32_t d = gload(depth + pix);
                 if (d == 0u || !(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                atomicAdd(red + pix, r8);
-                atomicAdd(green + pix, g8);
-                atomicAdd(blue + pix, b8);
-                atomicAdd(count + pix, 1u);
+                gadd(red + pix, r8);
+                gadd(green + pix, g8);
+                gadd(blue + pix, b8);
+                gadd(count + pix, 1u);
                 continue;
             }
             const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
-            const uint32_t d = box.lds ? L.depth[k] : depth[k];
+            const uint32_t d = box.lds ? lds_load(L.depth + k) : gload(depth + k);
             if (d == 0u) continue;
             if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
             if (k == runK) {
@@ -691,17 +778,17 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
                 const size_t pix = (size_t)ty * p.width + tx;
                 const bool inTile = box.lds && (!kCheck || box_holds(box, tx, ty));
                 const int k = inTile ? box.local(tx, ty) : 0;
-                const uint32_t d = inTile ? L.depth[k] : depth[pix];
+                const uint32_t d = inTile ? lds_load(L.depth + k) : gload(depth + pix);
                 if (d == 0u) continue;
                 if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
                 if (inTile) {
-                    atomicAdd(L.rg + k, rg);
-                    atomicAdd(L.bn + k, bn);
+                    lds_add(L.rg + k, rg);
+                    lds_add(L.bn + k, bn);
                 } else {
-                    atomicAdd(red + pix, r8);
-                    atomicAdd(green + pix, g8);
-                    atomicAdd(blue + pix, b8);
-                    atomicAdd(count + pix, 1u);
+                    gadd(red + pix, r8);
+                    gadd(green + pix, g8);
+                    gadd(blue + pix, b8);
+                    gadd(count + pix, 1u);
                 }
             }
     }
@@ -714,9 +801,9 @@ __device__ __forceinline__ void color_stage(const ChunkBox& box, const PassParam
                                    ColorLds L) {
     const int area = box.w * box.h;
     for (int i = G::rank(); i < area; i += G::kSize) {
-        L.depth[i] = depth[box.pixel(i, p.width)];
-        L.rg[i] = 0ull;
-        L.bn[i] = 0ull;
+        lds_store(L.depth + i, gload(depth + box.pixel(i, p.width)));
+        lds_store(L.rg + i, 0ull);
+        lds_store(L.bn + i, 0ull);
     }
 }
 
@@ -728,15 +815,15 @@ __device__ __forceinline__ void color_tile_flush(const ChunkBox& box, const Pass
                                         uint32_t* green, uint32_t* blue, uint32_t* count) {
     const int area = box.w * box.h;
     for (int i = G::rank(); i < area; i += G::kSize) {
-        const unsigned long long bn = L.bn[i];
+        const unsigned long long bn = lds_load(L.bn + i);
         const uint32_t n = (uint32_t)(bn >> 32);
         if (n == 0u) continue;
-        const unsigned long long rg = L.rg[i];
+        const unsigned long long rg = lds_load(L.rg + i);
         const size_t pix = box.pixel(i, p.width);
-        atomicAdd(red + pix, (uint32_t)rg);
-        atomicAdd(green + pix, (uint32_t)(rg >> 32));
-        atomicAdd(blue + pix, (uint32_t)bn);
-        atomicAdd(count + pix, n);
+        gadd(red + pix, (uint32_t)rg);
+        gadd(green + pix, (uint32_t)(rg >> 32));
+        gadd(blue + pix, (uint32_t)bn);
+        gadd(count + pix, n);
     }
 }
 
@@ -807,7 +894,7 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
             ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~pi);
         if (kR0) {
             if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
-                atomicMax(packed + (size_t)f.py[j] * p.width + f.px[j], key);
+                gmax(packed + (size_t)f.py[j] * p.width + f.px[j], key);
                 continue;
             }
             // as in the depth pass: skip a max this thread already exceeded
@@ -815,8 +902,8 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
             if (k == lastK && key <= lastKey) continue;
             lastKey = (k == lastK) ? (key > lastKey ? key : lastKey) : key;
             lastK = k;
-            if (box.lds) atomicMax(lds + k, key);
-            else atomicMax(packed + (size_t)k, key);
+            if (box.lds) lds_max(lds + k, key);
+            else gmax(packed + (size_t)k, key);
             continue;
         }
         const int r = f.r[j];
@@ -825,8 +912,8 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
                 if (!inside_footprint(ox, oy, r)) continue;
                 const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                 if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                if (box.lds && (!kCheck || box_holds(box, tx, ty))) atomicMax(lds + box.local(tx, ty), key);
-                else atomicMax(packed + (size_t)ty * p.width + tx, key);
+                if (box.lds && (!kCheck || box_holds(box, tx, ty))) lds_max(lds + box.local(tx, ty), key);
+                else gmax(packed + (size_t)ty * p.width + tx, key);
             }
     }
 }
@@ -836,9 +923,9 @@ __device__ __forceinline__ void nearest_flush(const ChunkBox& box, const PassPar
                                      unsigned long long* __restrict__ packed) {
     const int area = box.w * box.h;
     for (int i = G::rank(); i < area; i += G::kSize) {
-        const unsigned long long k = lds[i];
+        const unsigned long long k = lds_load(lds + i);
         if (k == 0ull) continue;
-        atomicMax(packed + box.pixel(i, p.width), k);
+        gmax(packed + box.pixel(i, p.width), k);
     }
 }
 
@@ -853,7 +940,7 @@ __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, co
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.lds) {
         const int area = seg.w * seg.h;
-        for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0ull;
+        for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
         G::sync();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
@@ -874,7 +961,7 @@ __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, co
         if (box.frags == 0) continue;
         if (box.lds) {
             const int area = box.w * box.h;
-            for (int i = G::rank(); i < area; i += G::kSize) lds[i] = 0ull;
+            for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
             G::sync();
         }
         nearest_frags<kR0, false, PPT>(f, bc.first + li0, box, p, lds, packed);

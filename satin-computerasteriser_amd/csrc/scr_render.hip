@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
 
 template <bool kR0>
 __global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_depth(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
-    __shared__ uint32_t lds[kDepthTile];
+    __shared__ uint32_t lds[kDepthTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)

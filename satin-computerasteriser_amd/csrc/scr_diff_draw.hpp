@@ -18,7 +18,7 @@ constexpr int kDiffThreads = 256;  // diff kernels: workgroup size
 constexpr int kDiffPPT = 8;        // points per thread
 constexpr int kDiffTile = 1024;  // depth u32 + 3 sums u64 + count u32 = 32 B per pixel: 32 KB
 
-__device__ inline unsigned long long diff_fixed(float v) {
+__device__ __forceinline__ unsigned long long diff_fixed(float v) {
     float x = v * 65536.0f;
     if (!(x == x)) return 0ull;
     x = fminf(fmaxf(x, -9.0e18f), 9.0e18f);
@@ -101,7 +101,7 @@ __device__ __forceinline__ void diff_tile_flush(const ChunkBox& box, const PassP
 // chunk): one tile for the whole range when the batch rectangle fits it,
 // else one per chunk.
 template <bool kR0>
-__device__ inline void diff_color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+__device__ __forceinline__ void diff_color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                          const float4* __restrict__ params, const uint32_t* __restrict__ depth,
                                          unsigned long long* sums, size_t stride, uint32_t* count, DiffLds& L) {
     constexpr int kChunk = kDiffThreads * kDiffPPT;
@@ -142,7 +142,7 @@ __device__ inline void diff_color_chunks(const CloudDev& c, const PassParams& p,
 
 // Local points [lo, bc.active) of one visible batch (lo a multiple of the chunk).
 template <bool kR0>
-__device__ inline void diff_backward_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+__device__ __forceinline__ void diff_backward_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                             const uint32_t* __restrict__ depth, const uint32_t* __restrict__ count,
                                             const float* __restrict__ dLdImage, double* __restrict__ grad) {
     constexpr int kChunk = kDiffThreads * kDiffPPT;
@@ -183,7 +183,7 @@ __device__ inline void diff_backward_chunks(const CloudDev& c, const PassParams&
 
 // Pixel i of the differentiable image: sum / (2^16 * count) in FP64, or the
 // background where count is 0.
-__device__ inline void diff_resolve_pixel(const PassParams& p, const unsigned long long* __restrict__ sums,
+__device__ __forceinline__ void diff_resolve_pixel(const PassParams& p, const unsigned long long* __restrict__ sums,
                                           size_t stride, const uint32_t* __restrict__ count, size_t i,
                                           float* __restrict__ image) {
     const uint32_t cnt = count[i];

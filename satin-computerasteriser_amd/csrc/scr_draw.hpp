@@ -84,7 +84,7 @@ constexpr int draw_waves(int ppt) { return ppt <= 8 ? 4 : 2; }
 // ---------------------------------------------------------------- cull
 
 // makePlane + intersectsFrustum — Common.metal:119-153
-__device__ inline bool intersects_frustum(const float* m, const scr_raster_batch& b) {
+__device__ __forceinline__ bool intersects_frustum(const float* m, const scr_raster_batch& b) {
 #define M(c, r) m[(c) * 4 + (r)]
     const float pl[6][4] = {
         {M(0, 3) - M(0, 0), M(1, 3) - M(1, 0), M(2, 3) - M(2, 0), M(3, 3) - M(3, 0)},
@@ -111,7 +111,7 @@ __device__ inline bool intersects_frustum(const float* m, const scr_raster_batch
 }
 
 // pixelSizeOnScreen — Common.metal:155-172
-__device__ inline float pixel_size_on_screen(const scr_raster_batch& b, const float* world, const PassParams& p) {
+__device__ __forceinline__ float pixel_size_on_screen(const scr_raster_batch& b, const float* world, const PassParams& p) {
     const float cx = (b.minX + b.maxX) * 0.5f;
     const float cy = (b.minY + b.maxY) * 0.5f;
     const float cz = (b.minZ + b.maxZ) * 0.5f;
@@ -137,7 +137,7 @@ __device__ inline float pixel_size_on_screen(const scr_raster_batch& b, const fl
 
 // The decision of cullUpdate (Cull/Shaders.metal:19-83) for slot `gid`;
 // true when the batch is drawn, with its VisibleBatch record in `vb`.
-__device__ inline bool cull_slot(const CloudDev& c, const PassParams& p, int gid, scr_visible_batch& vb) {
+__device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p, int gid, scr_visible_batch& vb) {
     const scr_raster_batch b = c.batches[gid];
     if (b.state == 0) return false;  // residency gate, Cull/Shaders.metal:34-36
     if ((int)b.fileIndex >= c.nFiles) return false;
@@ -184,7 +184,7 @@ struct BatchCtx {
 };
 
 // `hi` bounds the local point indices drawn (at most the active prefix).
-__device__ inline BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
+__device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
                                      uint32_t hi) {
     const scr_raster_batch b = gload_rec(c.batches + vb.batchIndex);
     BatchCtx bc;
@@ -211,7 +211,7 @@ __device__ inline BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, con
 }
 
 template <int PPT>
-__device__ inline bool vec_ok(const BatchCtx& bc) {
+__device__ __forceinline__ bool vec_ok(const BatchCtx& bc) {
     return (bc.first & (uint32_t)(PPT - 1)) == 0u;
 }
 
@@ -226,7 +226,7 @@ struct ChunkFrags {
 };
 
 template <int PPT>
-__device__ inline void load_words(const uint32_t* src, uint32_t base, uint32_t* out) {
+__device__ __forceinline__ void load_words(const uint32_t* src, uint32_t base, uint32_t* out) {
     const SCR_GLOBAL u32x4* p = (const SCR_GLOBAL u32x4*)(src + base);
 #pragma unroll
     for (int q = 0; q < PPT / 4; q++) {
@@ -247,7 +247,7 @@ struct ChunkRaw {
 
 // Issues the loads of the thread's PPT points starting at local index li0.
 template <int PPT>
-__device__ inline void chunk_fetch(const CloudDev& c, const BatchCtx& bc, uint32_t li0, bool vec, ChunkRaw<PPT>& r) {
+__device__ __forceinline__ void chunk_fetch(const CloudDev& c, const BatchCtx& bc, uint32_t li0, bool vec, ChunkRaw<PPT>& r) {
     const uint32_t base = bc.first + li0;
     const uint32_t n = li0 < bc.active ? min((uint32_t)PPT, bc.active - li0) : 0u;
     if (vec && n == (uint32_t)PPT) {
@@ -278,7 +278,7 @@ __device__ inline void chunk_fetch(const CloudDev& c, const BatchCtx& bc, uint32
 // skip, decodePoint (Common.metal:73-117), projection, clip and viewport
 // tests, y flip, footprint radius.
 template <bool kNearestZ, bool kR0, int PPT>
-__device__ inline void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
+__device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                     const ChunkRaw<PPT>& r, ChunkFrags<PPT>& f) {
     const uint32_t base = bc.first + li0;
     const uint32_t n = li0 < bc.active ? min((uint32_t)PPT, bc.active - li0) : 0u;
@@ -332,7 +332,7 @@ __device__ inline void chunk_decode(const CloudDev& c, const PassParams& p, cons
 }
 
 template <bool kNearestZ, bool kR0, int PPT>
-__device__ inline void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
+__device__ __forceinline__ void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                    bool vec, ChunkFrags<PPT>& f) {
     ChunkRaw<PPT> r;
     chunk_fetch<PPT>(c, bc, li0, vec, r);
@@ -359,15 +359,15 @@ struct ChunkBox {
     }
 };
 
-__device__ inline int wave_min(int v) {
+__device__ __forceinline__ int wave_min(int v) {
     for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
     return v;
 }
-__device__ inline int wave_max(int v) {
+__device__ __forceinline__ int wave_max(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
     return v;
 }
-__device__ inline int wave_sum(int v) {
+__device__ __forceinline__ int wave_sum(int v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
@@ -396,7 +396,7 @@ struct Group {
 // Rectangle reduction over the group; red holds 5 * kMaxWaves ints (block
 // groups only).  Ends with a barrier, so the caller may reuse LDS right after.
 template <bool kWave, int NT, int PPT>
-__device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams& p, int cap, int* red) {
+__device__ __forceinline__ ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams& p, int cap, int* red) {
     int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1, n = 0;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
@@ -462,7 +462,7 @@ __device__ inline ChunkBox chunk_box(const ChunkFrags<PPT>& f, const PassParams&
 // camera.  The draw bodies test each fragment against it, so a fragment
 // outside still reaches the planes: the rectangle only decides where the
 // group's fragments meet.  Called by every lane of the wave.
-__device__ inline ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, uint32_t points, int cap) {
+__device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, uint32_t points, int cap) {
     const int corner = (int)(__lane_id() & 7u);
     // sx * steps is the batch size (power-of-two scaling, exact)
     const float steps = bc.level == 0 ? 1073741824.0f : (bc.level == 1 ? 1048576.0f : 1024.0f);
@@ -509,7 +509,7 @@ __device__ inline ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, 
 }
 
 // Whether (x, y) lies in the rectangle.
-__device__ inline bool box_holds(const ChunkBox& b, int x, int y) {
+__device__ __forceinline__ bool box_holds(const ChunkBox& b, int x, int y) {
     return (uint32_t)(x - b.x0) < (uint32_t)b.w && (uint32_t)(y - b.y0) < (uint32_t)b.h;
 }
 
@@ -609,7 +609,7 @@ __device__ __forceinline__ void depth_flush(const ChunkBox& box, const PassParam
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__device__ inline void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+__device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red) {
     using G = Group<kWave, NT>;
     constexpr int kCap = kDepthTile / G::kCount;
@@ -828,7 +828,7 @@ __device__ __forceinline__ void color_tile_flush(const ChunkBox& box, const Pass
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__device__ inline void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+__device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                     uint32_t* blue, uint32_t* count, ColorLds LAll) {
     using G = Group<kWave, NT>;
@@ -930,7 +930,7 @@ __device__ __forceinline__ void nearest_flush(const ChunkBox& box, const PassPar
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+__device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                       unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red) {
     using G = Group<kWave, NT>;
     constexpr int kCap = kNearestTile / G::kCount;
@@ -977,7 +977,7 @@ __device__ inline void nearest_chunks(const CloudDev& c, const PassParams& p, co
 // A batch whose points all have radius 0 runs the single-pixel code.
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__device__ inline void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
+__device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                    uint32_t hi, uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
     if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT>(c, p, bc, lo, depth, lds, red);
@@ -985,7 +985,7 @@ __device__ inline void depth_batch(const CloudDev& c, const PassParams& p, const
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__device__ inline void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
+__device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                    uint32_t hi, const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                    uint32_t* blue, uint32_t* count, ColorLds L) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
@@ -994,7 +994,7 @@ __device__ inline void color_batch(const CloudDev& c, const PassParams& p, const
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
+__device__ __forceinline__ void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                      uint32_t hi, unsigned long long* __restrict__ packed, unsigned long long* lds,
                                      int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
@@ -1004,14 +1004,14 @@ __device__ inline void nearest_batch(const CloudDev& c, const PassParams& p, con
 
 // ---------------------------------------------------------------- resolves
 
-__device__ inline uint32_t unorm8(float x) { return f2u(rintf(saturate(x) * 255.0f)); }
+__device__ __forceinline__ uint32_t unorm8(float x) { return f2u(rintf(saturate(x) * 255.0f)); }
 
-__device__ inline uint32_t pack_rgba(float r, float g, float b, float a) {
+__device__ __forceinline__ uint32_t pack_rgba(float r, float g, float b, float a) {
     return unorm8(r) | (unorm8(g) << 8) | (unorm8(b) << 16) | (unorm8(a) << 24);
 }
 
 // resolveUpdate (count-average path) — Resolve/Shaders.metal:13-57
-__device__ inline void resolve_pixel(const PassParams& p, const uint32_t* __restrict__ planes, size_t stride, size_t i,
+__device__ __forceinline__ void resolve_pixel(const PassParams& p, const uint32_t* __restrict__ planes, size_t stride, size_t i,
                                      uint32_t* __restrict__ rgba, float* __restrict__ depthf) {
     const uint32_t cnt = planes[4 * stride + i];
     if (cnt == 0u) {
@@ -1028,7 +1028,7 @@ __device__ inline void resolve_pixel(const PassParams& p, const uint32_t* __rest
 }
 
 // nearestResolveUpdate — NearestResolve/Shaders.metal:8-38
-__device__ inline void nearest_resolve_pixel(const PassParams& p, const unsigned long long* __restrict__ packed,
+__device__ __forceinline__ void nearest_resolve_pixel(const PassParams& p, const unsigned long long* __restrict__ packed,
                                              const uint32_t* __restrict__ colors, size_t i, uint32_t* __restrict__ rgba,
                                              float* __restrict__ depthf) {
     const unsigned long long k = packed[i];

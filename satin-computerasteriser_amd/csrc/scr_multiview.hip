@@ -21,36 +21,23 @@
 #include <cstring>
 #include <new>
 
-#include "scr_diff_draw.hpp"
+#include "scr_mv.hpp"
 
 namespace scr {
+using mv::DrawKernel;
+using mv::kDepth;
+using mv::kColor;
+using mv::kNearest;
+using mv::kQueues;
+using mv::MVItem;
+using mv::MVView;
+using mv::Phase;
+
 namespace {
 
 using namespace draw;
+using namespace mv;
 
-struct MVView {
-    PassParams p;
-    CloudDev c;                     // files -> this view's composed table
-    uint32_t* planes;               // depth, red, green, blue, count (stride apart)
-    unsigned long long* nearest;
-    uint32_t* rgba;
-    float* depthf;
-    unsigned long long stride;
-    uint32_t itemBase, itemStride;  // item of slot s = itemBase + s * itemStride
-    // differentiable mode
-    unsigned long long* sums;       // 3 int64 planes, stride apart
-    float* image;                   // 3 floats per pixel
-    const float* dLdImage;          // 3 floats per pixel
-};
-
-struct MVItem {
-    uint32_t view;
-    uint32_t pad[3];
-    scr_visible_batch vb;           // activePoints == 0: culled
-};
-static_assert(sizeof(MVItem) == 32, "MVItem is two 16-byte words");
-
-constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
 
 enum ClearMode { kClearHQS = 0, kClearNearest = 1, kClearDiff = 2 };
 
@@ -92,73 +79,6 @@ __global__ __launch_bounds__(256) void k_mv_cull(const MVView* __restrict__ view
         it.vb.activePoints = 0u;
     }
     items[V.itemBase + (uint32_t)slot * V.itemStride] = it;
-}
-
-// Takes work units from heads[queue] until the table is drained.  A unit is
-// (item, part); part k of S draws the k-th of S equal, chunk-aligned pieces of
-// the item's active prefix.  Every workgroup leaves the loop once the counter
-// passes nUnits, so the grid drains.
-template <int NT, int PPT, typename Body>
-__device__ inline void drain(const MVView* __restrict__ views, const MVItem* __restrict__ items, uint32_t nUnits,
-                             int splits, uint32_t* head, Body body) {
-    __shared__ uint32_t sUnit;
-    constexpr uint32_t kChunk = NT * PPT;
-    for (;;) {
-        if (threadIdx.x == 0) sUnit = atomicAdd(head, 1u);
-        __syncthreads();
-        const uint32_t u = __builtin_amdgcn_readfirstlane(sUnit);
-        __syncthreads();
-        if (u >= nUnits) break;
-        const uint32_t item = u / (uint32_t)splits, part = u - item * (uint32_t)splits;
-        const MVItem it = items[item];
-        const uint32_t active = it.vb.activePoints;
-        if (active == 0u) continue;
-        const uint32_t chunks = (active + kChunk - 1) / kChunk;
-        const uint32_t per = (chunks + (uint32_t)splits - 1) / (uint32_t)splits;
-        const uint32_t lo = part * per * kChunk;
-        if (lo >= active) continue;
-        const uint32_t hi = min(active, lo + per * kChunk);
-        body(views[__builtin_amdgcn_readfirstlane(it.view)], it.vb, lo, hi);
-    }
-}
-
-template <bool kR0, bool kWave, int NT, int PPT>
-__global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* __restrict__ views, const MVItem* __restrict__ items,
-                                                 uint32_t nUnits, int splits, uint32_t* heads) {
-    __shared__ uint32_t lds[kDepthTile + kSpare];
-    __shared__ int red[5 * kMaxWaves];
-    drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
-                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       depth_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, lds, red);
-                   });
-}
-
-template <bool kR0, bool kWave, int NT, int PPT>
-__global__ __launch_bounds__(NT) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
-                                                 uint32_t nUnits, int splits, uint32_t* heads) {
-    __shared__ uint32_t ldsDepth[kColorTile];
-    __shared__ unsigned long long ldsRG[kColorTile];
-    __shared__ unsigned long long ldsBC[kColorTile];
-    __shared__ int redbuf[5 * kMaxWaves];
-    drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
-                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
-                       const size_t s = (size_t)V.stride;
-                       color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
-                                                 V.planes + 3 * s, V.planes + 4 * s, L);
-                   });
-}
-
-template <bool kR0, bool kWave, int NT, int PPT>
-__global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView* __restrict__ views,
-                                                   const MVItem* __restrict__ items, uint32_t nUnits, int splits,
-                                                   uint32_t* heads) {
-    __shared__ unsigned long long lds[kNearestTile];
-    __shared__ int red[5 * kMaxWaves];
-    drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
-                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       nearest_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.nearest, lds, red);
-                   });
 }
 
 __global__ __launch_bounds__(256) void k_mv_resolve(const MVView* __restrict__ views) {
@@ -234,31 +154,35 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_backward(const MVView*
 
 // ---------------------------------------------------------------- draw variants
 
-using DrawKernel = void (*)(const MVView*, const MVItem*, uint32_t, int, uint32_t*);
-
 struct Variant {
     int threads, ppt;
     DrawKernel depth[2][2], color[2][2], nearest[2][2];  // [kR0][kWave]
 };
 
-#define SCR_MV_KERNELS(K, NT, PPT) \
-    {{K<false, false, NT, PPT>, K<false, true, NT, PPT>}, {K<true, false, NT, PPT>, K<true, true, NT, PPT>}}
-#define SCR_MV_VARIANT(NT, PPT)                                                                     \
-    Variant {                                                                                       \
-        NT, PPT, SCR_MV_KERNELS(k_mv_depth, NT, PPT), SCR_MV_KERNELS(k_mv_color, NT, PPT),          \
-            SCR_MV_KERNELS(k_mv_nearest, NT, PPT)                                                   \
-    }
-
-const Variant kVariants[] = {
-    SCR_MV_VARIANT(128, 8),  SCR_MV_VARIANT(256, 4), SCR_MV_VARIANT(256, 8),
-    SCR_MV_VARIANT(256, 16), SCR_MV_VARIANT(512, 4), SCR_MV_VARIANT(512, 8),
-};
-constexpr int kNumVariants = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+// The shapes with their kernels from the phase translation units.
+const Variant* variants() {
+    static const Variant* table = [] {
+        static Variant t[kNumShapes];
+        for (int i = 0; i < kNumShapes; i++) {
+            t[i].threads = kShapes[i].threads;
+            t[i].ppt = kShapes[i].ppt;
+            for (int r = 0; r < 2; r++)
+                for (int w = 0; w < 2; w++) {
+                    t[i].depth[r][w] = depth_kernels()[i][r][w];
+                    t[i].color[r][w] = color_kernels()[i][r][w];
+                    t[i].nearest[r][w] = nearest_kernels()[i][r][w];
+                }
+        }
+        return t;
+    }();
+    return table;
+}
+constexpr int kNumVariants = kNumShapes;
 const int kSplits[] = {1, 2, 4};
 
 const Variant* find_variant(const scr_mv_shape& s) {
-    for (const Variant& v : kVariants)
-        if (v.threads == s.threads && v.ppt == s.pointsPerThread) return &v;
+    for (int i = 0; i < kNumVariants; i++)
+        if (variants()[i].threads == s.threads && variants()[i].ppt == s.pointsPerThread) return variants() + i;
     return nullptr;
 }
 
@@ -267,7 +191,6 @@ bool valid_shape(const scr_mv_shape& s) {
            (s.waveTiles == 0 || s.waveTiles == 1);
 }
 
-enum Phase { kDepth = 0, kColor = 1, kNearest = 2 };
 
 DrawKernel kernel_of(const Variant& v, Phase ph, bool r0, bool wave) {
     const int k = r0 ? 1 : 0, w = wave ? 1 : 0;
@@ -678,7 +601,7 @@ int scr_multiview_autotune(scr_multiview* mv, const scr_config* cfg, const scr_c
         for (int vi = 0; vi < scr::kNumVariants && rc == SCR_OK; vi++)
             for (int wave = 0; wave < 2; wave++)
             for (int split : scr::kSplits) {
-                *slot = scr_mv_shape{scr::kVariants[vi].threads, scr::kVariants[vi].ppt, split, wave};
+                *slot = scr_mv_shape{scr::variants()[vi].threads, scr::variants()[vi].ppt, split, wave};
                 float ms = 1e30f;
                 for (int r = 0; r < reps && rc == SCR_OK; r++) {
                     uint32_t nItems = 0;

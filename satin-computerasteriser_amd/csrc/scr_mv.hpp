@@ -1,0 +1,131 @@
+// scr_mv.hpp — device side of batched multi-view rendering shared by the
+// draw-kernel translation units (scr_mv_depth.hip, scr_mv_color.hip,
+// scr_mv_nearest.hip) and the host side (scr_multiview.hip): the per-view
+// record, the work-item table, the work-queue drain loop and the draw kernel
+// templates.  Each draw phase is compiled in its own translation unit, which
+// instantiates every shape of it and exports the table of kernels.
+#pragma once
+
+#include "scr_diff_draw.hpp"
+
+namespace scr {
+namespace mv {
+
+using namespace draw;
+
+struct MVView {
+    PassParams p;
+    CloudDev c;                     // files -> this view's composed table
+    uint32_t* planes;               // depth, red, green, blue, count (stride apart)
+    unsigned long long* nearest;
+    uint32_t* rgba;
+    float* depthf;
+    unsigned long long stride;
+    uint32_t itemBase, itemStride;  // item of slot s = itemBase + s * itemStride
+    // differentiable mode
+    unsigned long long* sums;       // 3 int64 planes, stride apart
+    float* image;                   // 3 floats per pixel
+    const float* dLdImage;          // 3 floats per pixel
+};
+
+struct MVItem {
+    uint32_t view;
+    uint32_t pad[3];
+    scr_visible_batch vb;           // activePoints == 0: culled
+};
+static_assert(sizeof(MVItem) == 32, "MVItem is two 16-byte words");
+
+constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
+
+// Takes work units from heads[queue] until the table is drained.  A unit is
+// (item, part); part k of S draws the k-th of S equal, chunk-aligned pieces of
+// the item's active prefix.  Every workgroup leaves the loop once the counter
+// passes nUnits, so the grid drains.
+template <int NT, int PPT, typename Body>
+__device__ inline void drain(const MVView* __restrict__ views, const MVItem* __restrict__ items, uint32_t nUnits,
+                             int splits, uint32_t* head, Body body) {
+    __shared__ uint32_t sUnit;
+    constexpr uint32_t kChunk = NT * PPT;
+    for (;;) {
+        if (threadIdx.x == 0) sUnit = atomicAdd(head, 1u);
+        __syncthreads();
+        const uint32_t u = __builtin_amdgcn_readfirstlane(sUnit);
+        __syncthreads();
+        if (u >= nUnits) break;
+        const uint32_t item = u / (uint32_t)splits, part = u - item * (uint32_t)splits;
+        const MVItem it = items[item];
+        const uint32_t active = it.vb.activePoints;
+        if (active == 0u) continue;
+        const uint32_t chunks = (active + kChunk - 1) / kChunk;
+        const uint32_t per = (chunks + (uint32_t)splits - 1) / (uint32_t)splits;
+        const uint32_t lo = part * per * kChunk;
+        if (lo >= active) continue;
+        const uint32_t hi = min(active, lo + per * kChunk);
+        body(views[__builtin_amdgcn_readfirstlane(it.view)], it.vb, lo, hi);
+    }
+}
+
+template <bool kR0, bool kWave, int NT, int PPT>
+__global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* __restrict__ views, const MVItem* __restrict__ items,
+                                                 uint32_t nUnits, int splits, uint32_t* heads) {
+    __shared__ uint32_t lds[kDepthTile + kSpare];
+    __shared__ int red[5 * kMaxWaves];
+    drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
+                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
+                       depth_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, lds, red);
+                   });
+}
+
+template <bool kR0, bool kWave, int NT, int PPT>
+__global__ __launch_bounds__(NT) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
+                                                 uint32_t nUnits, int splits, uint32_t* heads) {
+    __shared__ uint32_t ldsDepth[kColorTile];
+    __shared__ unsigned long long ldsRG[kColorTile];
+    __shared__ unsigned long long ldsBC[kColorTile];
+    __shared__ int redbuf[5 * kMaxWaves];
+    drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
+                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
+                       const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
+                       const size_t s = (size_t)V.stride;
+                       color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
+                                                 V.planes + 3 * s, V.planes + 4 * s, L);
+                   });
+}
+
+template <bool kR0, bool kWave, int NT, int PPT>
+__global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView* __restrict__ views,
+                                                   const MVItem* __restrict__ items, uint32_t nUnits, int splits,
+                                                   uint32_t* heads) {
+    __shared__ unsigned long long lds[kNearestTile];
+    __shared__ int red[5 * kMaxWaves];
+    drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
+                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
+                       nearest_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.nearest, lds, red);
+                   });
+}
+
+using DrawKernel = void (*)(const MVView*, const MVItem*, uint32_t, int, uint32_t*);
+
+// Draw shapes (threads x points per thread) the autotuner chooses from.
+struct Shape {
+    int threads, ppt;
+};
+constexpr Shape kShapes[] = {{128, 8}, {256, 4}, {256, 8}, {256, 16}, {512, 4}, {512, 8}};
+constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+
+// kernels[shape][kR0][kWave] of one phase, in kShapes order
+#define SCR_MV_FOUR(K, NT, PPT) \
+    {{K<false, false, NT, PPT>, K<false, true, NT, PPT>}, {K<true, false, NT, PPT>, K<true, true, NT, PPT>}}
+#define SCR_MV_TABLE(K)                                                                                     \
+    {SCR_MV_FOUR(K, 128, 8), SCR_MV_FOUR(K, 256, 4), SCR_MV_FOUR(K, 256, 8), SCR_MV_FOUR(K, 256, 16),      \
+     SCR_MV_FOUR(K, 512, 4), SCR_MV_FOUR(K, 512, 8)}
+
+enum Phase { kDepth = 0, kColor = 1, kNearest = 2 };
+
+using KernelTable = DrawKernel[kNumShapes][2][2];
+const KernelTable& depth_kernels();    // scr_mv_depth.hip
+const KernelTable& color_kernels();    // scr_mv_color.hip
+const KernelTable& nearest_kernels();  // scr_mv_nearest.hip
+
+}  // namespace mv
+}  // namespace scr

@@ -233,6 +233,19 @@ def perspective_reverse_z(fovy_degrees: float, aspect: float, near: float, far: 
     ])
 
 
+def orthographic_reverse_z(height: float, aspect: float, near: float, far: float) -> np.ndarray:
+    """Right-handed reverse-Z orthographic projection of a view volume
+    `height` high and `height * aspect` wide (view z = -near → ndc 1,
+    z = -far → 0; w = 1), column-major."""
+    h = 2.0 / height
+    return mat([
+        [h / aspect, 0, 0, 0],
+        [0, h, 0, 0],
+        [0, 0, 1.0 / (far - near), far / (far - near)],
+        [0, 0, 0, 1],
+    ])
+
+
 def look_at(eye, target, up) -> np.ndarray:
     """Right-handed view matrix looking down -Z, column-major."""
     eye = np.asarray(eye, np.float64)
@@ -259,6 +272,11 @@ class Camera:
     def perspective(eye, target, up=(0.0, 1.0, 0.0), fov: float = 60.0, aspect: float = 1.0,
                     near: float = 0.1, far: float = 100.0) -> "Camera":
         return Camera(look_at(eye, target, up), perspective_reverse_z(fov, aspect, near, far))
+
+    @staticmethod
+    def orthographic(eye, target, up=(0.0, 1.0, 0.0), height: float = 2.0, aspect: float = 1.0,
+                     near: float = 0.1, far: float = 100.0) -> "Camera":
+        return Camera(look_at(eye, target, up), orthographic_reverse_z(height, aspect, near, far))
 
     def to_c(self) -> CCamera:
         c = CCamera()

@@ -91,6 +91,25 @@ def test_multiview_hqs_variable_size_clod_matches_per_view(ctx, scene):
     mv.close()
 
 
+def test_multiview_orthographic_and_perspective_views(ctx, scene):
+    # views of both projections in one launch: w = 1 for the orthographic ones
+    cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1.0, maximumPointSize=3.0, pointSizeScale=2.0)
+    cams, sizes, which = _views(scene, 6, seed=7)
+    clouds, centres, diags = scene
+    for v in (1, 4):
+        k = which[v]
+        eye = centres[k] + np.array([0.6, 0.5, 0.7]) * diags[k]
+        w, h = sizes[v]
+        cams[v] = scr.Camera.orthographic(eye, centres[k], (0.0, 0.0, 1.0), height=0.8 * diags[k], aspect=w / h,
+                                          near=0.05 * diags[k], far=4.0 * diags[k])
+    mv = scr.MultiViewRasteriser(ctx, clouds, sizes, which, cfg)
+    want = _per_view(ctx, scene, cfg, cams, sizes, which, HQS_PLANES)
+    assert all(int(np.count_nonzero(want[v]["count"])) > 100 for v in (1, 4))
+    mv.render(cams)
+    _assert_same(_batched(mv, HQS_PLANES), want)
+    mv.close()
+
+
 def test_multiview_hqs_one_pixel_graph_replays(ctx, scene):
     """Graph replays, with other eager work and synchronisation between them,
     each reproduce the eager result; changing cameras between replays works

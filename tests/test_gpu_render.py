@@ -114,6 +114,30 @@ def test_point_sizes_hqs_and_nearest(ctx, jitter, sizes):
     assert_planes_equal(gpu, ref)
 
 
+ORTHO_SIZE_CASES = [
+    dict(),
+    dict(minimumPointSize=1, maximumPointSize=4, pointSizeScale=3.0),
+    dict(pointSizeMode=scr.POINT_SIZE_WORLD_SPACE, minimumPointSize=1, maximumPointSize=6, pointSizeScale=0.1),
+]
+
+
+@pytest.mark.parametrize("sizes", ORTHO_SIZE_CASES)
+def test_orthographic_hqs_and_nearest(ctx, jitter, sizes):
+    # an orthographic projection (w = 1 everywhere): constant footprint radius
+    # in both size modes (Common.metal:42-43, 49-50)
+    cam = scr.Camera.orthographic((11.0, 7.0, 13.0), (0, 0, 0), height=14.0, aspect=4 / 3, near=1.0, far=40.0)
+    cfg = scr.ComputeRasteriserConfiguration(**sizes)
+    gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, jitter)], cfg, cam, 320, 240)
+    ref = orc.render_hqs([oracle_cloud(jitter)], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert ref["count"].sum() > 1000
+    assert_planes_equal(gpu, ref)
+    cfg.mode = scr.MODE_NEAREST_POINT
+    gpu = render_gpu_nearest(ctx, [gpu_cloud(ctx, jitter)], cfg, cam, 320, 240)
+    ref = orc.render_nearest(oracle_cloud(jitter), cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert (ref["depth"] > 0).sum() > 1000
+    assert_planes_equal(gpu, ref)
+
+
 def test_tints_mix_and_discard(ctx, jitter):
     cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
     cfg = scr.ComputeRasteriserConfiguration(applyTint=True, minimumPointSize=1, maximumPointSize=2,

@@ -688,18 +688,27 @@ __device__ __forceinline__ void color_flush(bool lds, int k, unsigned long long 
     }
 }
 
-// The colour stage (ColorPass/Shaders.metal:95-133) of a chunk: packed
-// r | g << 8 | b << 16 per fragment; a tint discard drops the fragment.
+// Issues the colour loads of the thread's PPT points (with the position
+// loads, so that both are in flight while the front end waits for the first).
 template <int PPT>
-__device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0, bool vec,
-                                    ChunkFrags<PPT>& f, uint32_t* rgb) {
+__device__ __forceinline__ void color_fetch(const CloudDev& c, const BatchCtx& bc, uint32_t li0, bool vec,
+                                            uint32_t* rgb) {
     const uint32_t base = bc.first + li0;
-    if (vec && li0 + (uint32_t)PPT <= bc.active) {
+    const uint32_t n = li0 < bc.active ? min((uint32_t)PPT, bc.active - li0) : 0u;
+    if (vec && n == (uint32_t)PPT) {
         load_words<PPT>(c.colors, base, rgb);
     } else {
 #pragma unroll
-        for (int j = 0; j < PPT; j++) rgb[j] = f.px[j] >= 0 ? gload(c.colors + base + (uint32_t)j) : 0u;
+        for (int j = 0; j < PPT; j++) rgb[j] = (uint32_t)j < n ? gload(c.colors + base + (uint32_t)j) : 0u;
     }
+}
+
+// The colour stage (ColorPass/Shaders.metal:95-133) of a chunk: packed
+// r | g << 8 | b << 16 per fragment; a tint discard drops the fragment.
+template <int PPT>
+__device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
+                                             ChunkFrags<PPT>& f, uint32_t* rgb) {
+    const uint32_t base = bc.first + li0;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const uint32_t pi = base + (uint32_t)j;
@@ -843,10 +852,13 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
         G::sync();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
-            ChunkFrags<PPT> f;
-            chunk_front<false, kR0, PPT>(c, p, bc, li0, vec, f);
+            ChunkRaw<PPT> raw;
+            chunk_fetch<PPT>(c, bc, li0, vec, raw);
             uint32_t rgb[PPT];
-            chunk_colors<PPT>(c, p, bc, li0, vec, f, rgb);
+            color_fetch<PPT>(c, bc, li0, vec, rgb);
+            ChunkFrags<PPT> f;
+            chunk_decode<false, kR0, PPT>(c, p, bc, li0, raw, f);
+            chunk_colors<PPT>(c, p, bc, li0, f, rgb);
             color_frags<kR0, true, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
         }
         G::sync();
@@ -856,10 +868,13 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
     }
     for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         const uint32_t li0 = cb + G::rank() * PPT;
-        ChunkFrags<PPT> f;
-        chunk_front<false, kR0, PPT>(c, p, bc, li0, vec, f);
+        ChunkRaw<PPT> raw;
+        chunk_fetch<PPT>(c, bc, li0, vec, raw);
         uint32_t rgb[PPT];
-        chunk_colors<PPT>(c, p, bc, li0, vec, f, rgb);
+        color_fetch<PPT>(c, bc, li0, vec, rgb);
+        ChunkFrags<PPT> f;
+        chunk_decode<false, kR0, PPT>(c, p, bc, li0, raw, f);
+        chunk_colors<PPT>(c, p, bc, li0, f, rgb);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
         if (box.frags == 0) continue;
         if (box.lds) {

@@ -140,7 +140,69 @@ __device__ __forceinline__ void diff_color_chunks(const CloudDev& c, const PassP
     }
 }
 
-// Local points [lo, bc.active) of one visible batch (lo a multiple of the chunk).
+// The gradient of the thread's PPT points starting at local index li0 under
+// one view, added to acc (loaded from grad when a point is first touched;
+// touched marks the points to write back): per fragment that passes the
+// colour pass's depth test, dL/dpixel x (1 / count) in FP64, summed over the
+// point's footprint and then added to the point's running sum.
+template <bool kR0>
+__device__ __forceinline__ void diff_backward_acc(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
+                                                  uint32_t li0, bool vec, const uint32_t* __restrict__ depth,
+                                                  const uint32_t* __restrict__ count,
+                                                  const float* __restrict__ dLdImage, const double* grad,
+                                                  double (&acc)[kDiffPPT][3], bool (&touched)[kDiffPPT]) {
+    ChunkFrags<kDiffPPT> f;
+    chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+#pragma unroll
+    for (int j = 0; j < kDiffPPT; j++) {
+        if (f.px[j] < 0) continue;
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        bool any = false;
+        const int rad = kR0 ? 0 : f.r[j];
+        for (int oy = -rad; oy <= rad; oy++)
+            for (int ox = -rad; ox <= rad; ox++) {
+                if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
+                const int tx = f.px[j] + ox, ty = f.py[j] + oy;
+                if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
+                const size_t pix = (size_t)ty * p.width + tx;
+                const uint32_t d = gload(depth + pix);
+                if (d == 0u) continue;
+                if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                const double inv = 1.0 / (double)gload(count + pix);
+                g0 += (double)gload(dLdImage + pix * 3 + 0) * inv;
+                g1 += (double)gload(dLdImage + pix * 3 + 1) * inv;
+                g2 += (double)gload(dLdImage + pix * 3 + 2) * inv;
+                any = true;
+            }
+        if (!any) continue;
+        if (!touched[j]) {
+            const double* gp = grad + (size_t)(bc.first + li0 + (uint32_t)j) * 4;
+            acc[j][0] = gload(gp);
+            acc[j][1] = gload(gp + 1);
+            acc[j][2] = gload(gp + 2);
+            touched[j] = true;
+        }
+        acc[j][0] += g0;
+        acc[j][1] += g1;
+        acc[j][2] += g2;
+    }
+}
+
+// Writes back the touched points' sums.
+__device__ __forceinline__ void diff_backward_store(uint32_t first, const double (&acc)[kDiffPPT][3],
+                                                    const bool (&touched)[kDiffPPT], double* grad) {
+#pragma unroll
+    for (int j = 0; j < kDiffPPT; j++) {
+        if (!touched[j]) continue;
+        SCR_GLOBAL double* gp = gptr(grad + (size_t)(first + (uint32_t)j) * 4);
+        gp[0] = acc[j][0];
+        gp[1] = acc[j][1];
+        gp[2] = acc[j][2];
+    }
+}
+
+// Local points [lo, bc.active) of one visible batch (lo a multiple of the
+// chunk) under one view: each point's gradient gets this view's sum added.
 template <bool kR0>
 __device__ __forceinline__ void diff_backward_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                             const uint32_t* __restrict__ depth, const uint32_t* __restrict__ count,
@@ -149,35 +211,12 @@ __device__ __forceinline__ void diff_backward_chunks(const CloudDev& c, const Pa
     const bool vec = vec_ok<kDiffPPT>(bc);
     for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
         const uint32_t li0 = cb + threadIdx.x * kDiffPPT;
-        ChunkFrags<kDiffPPT> f;
-        chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+        double acc[kDiffPPT][3];
+        bool touched[kDiffPPT];
 #pragma unroll
-        for (int j = 0; j < kDiffPPT; j++) {
-            if (f.px[j] < 0) continue;
-            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
-            bool any = false;
-            const int rad = kR0 ? 0 : f.r[j];
-            for (int oy = -rad; oy <= rad; oy++)
-                for (int ox = -rad; ox <= rad; ox++) {
-                    if (!kR0 && !inside_footprint(ox, oy, rad)) continue;
-                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
-                    if (!kR0 && (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height)) continue;
-                    const size_t pix = (size_t)ty * p.width + tx;
-                    const uint32_t d = gload(depth + pix);
-                    if (d == 0u) continue;
-                    if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                    const double inv = 1.0 / (double)gload(count + pix);
-                    g0 += (double)gload(dLdImage + pix * 3 + 0) * inv;
-                    g1 += (double)gload(dLdImage + pix * 3 + 1) * inv;
-                    g2 += (double)gload(dLdImage + pix * 3 + 2) * inv;
-                    any = true;
-                }
-            if (!any) continue;
-            SCR_GLOBAL double* gp = gptr(grad + (size_t)(bc.first + li0 + (uint32_t)j) * 4);
-            gp[0] += g0;
-            gp[1] += g1;
-            gp[2] += g2;
-        }
+        for (int j = 0; j < kDiffPPT; j++) touched[j] = false;
+        diff_backward_acc<kR0>(c, p, bc, li0, vec, depth, count, dLdImage, grad, acc, touched);
+        diff_backward_store(bc.first + li0, acc, touched, grad);
     }
 }
 

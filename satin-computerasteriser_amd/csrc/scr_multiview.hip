@@ -126,9 +126,12 @@ __global__ __launch_bounds__(256) void k_mv_diff_resolve(const MVView* __restric
 }
 
 // Backward over the views of one cloud.  A work unit is one chunk of one
-// slot (perSlot chunks cover a slot's capacity); it visits that slot's item of every view in view order (the items of
-// a slot are adjacent in the table), so each point's gradient is added by one
-// thread, view after view, in the order separate per-view calls would add it.
+// slot (perSlot chunks cover a slot's capacity); it visits that slot's item
+// of every view in view order (the items of a slot are adjacent in the
+// table), each thread adding its points' per-view sums to running sums in
+// registers, loaded from and stored to the gradient once: the additions
+// happen in the order separate per-view calls make them, so the result is
+// the same bit for bit, with one gradient read and write per point.
 template <bool kR0>
 __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_backward(const MVView* __restrict__ views,
                                                                    const MVItem* __restrict__ items, int nSlots,
@@ -138,17 +141,28 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_backward(const MVView*
     for (int u = blockIdx.x; u < nUnits; u += gridDim.x) {
         const int slot = u / perSlot, chunk = u - slot * perSlot;
         const uint32_t lo = (uint32_t)chunk * kChunk;
+        const uint32_t li0 = lo + threadIdx.x * kDiffPPT;
+        double acc[kDiffPPT][3];
+        bool touched[kDiffPPT];
+#pragma unroll
+        for (int j = 0; j < kDiffPPT; j++) touched[j] = false;
+        uint32_t first = 0u;
         for (int k = 0; k < nViews; k++) {
             const MVItem it = items[(size_t)slot * (size_t)nViews + (size_t)k];
             if (it.vb.activePoints <= lo) continue;
             const MVView& V = views[__builtin_amdgcn_readfirstlane(it.view)];
             const BatchCtx bc = batch_ctx(V.c, V.p, it.vb, lo + kChunk);
+            first = bc.first;
             const size_t s = (size_t)V.stride;
+            const bool vec = vec_ok<kDiffPPT>(bc);
             if (!kR0 && bc.rconst == 0)
-                diff_backward_chunks<true>(V.c, V.p, bc, lo, V.planes, V.planes + 4 * s, V.dLdImage, grad);
+                diff_backward_acc<true>(V.c, V.p, bc, li0, vec, V.planes, V.planes + 4 * s, V.dLdImage, grad, acc,
+                                        touched);
             else
-                diff_backward_chunks<kR0>(V.c, V.p, bc, lo, V.planes, V.planes + 4 * s, V.dLdImage, grad);
+                diff_backward_acc<kR0>(V.c, V.p, bc, li0, vec, V.planes, V.planes + 4 * s, V.dLdImage, grad, acc,
+                                       touched);
         }
+        diff_backward_store(first + li0, acc, touched, grad);
     }
 }
 

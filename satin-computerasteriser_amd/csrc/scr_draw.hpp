@@ -349,6 +349,7 @@ struct ChunkBox {
     // exhaustively); w == 1 has no 32-bit magic and divides by itself
     uint32_t magic;
     bool lds;
+    float zmax;  // (segment_box) above every point's ndc z; +inf when unknown
     // every on-screen footprint pixel of the chunk lies inside the box (it is
     // the clamped union of the footprints), so no bounds test is needed
     __device__ inline int local(int tx, int ty) const { return (ty - y0) * w + (tx - x0); }
@@ -439,6 +440,7 @@ __device__ __forceinline__ ChunkBox chunk_box(const ChunkFrags<PPT>& f, const Pa
         __syncthreads();
     }
     ChunkBox b;
+    b.zmax = __builtin_inff();
     b.frags = n;
     b.x0 = max(x0, 0);
     b.y0 = max(y0, 0);
@@ -479,13 +481,16 @@ __device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const Batch
     int ok = (clip.w > 0.0f && sx == sx && sy == sy) ? 1 : 0;
     sx = fminf(fmaxf(sx, -lim), lim);
     sy = fminf(fmaxf(sy, -lim), lim);
-    float x0 = sx, x1 = sx, y0 = sy, y1 = sy;
+    // ndc z = clip.z / clip.w is linear-fractional with w > 0 over the box,
+    // so its largest value is at a corner
+    float x0 = sx, x1 = sx, y0 = sy, y1 = sy, z1 = clip.z * rw;
 #pragma unroll
     for (int o = 1; o < 8; o <<= 1) {
         x0 = fminf(x0, __shfl_xor(x0, o));
         x1 = fmaxf(x1, __shfl_xor(x1, o));
         y0 = fminf(y0, __shfl_xor(y0, o));
         y1 = fmaxf(y1, __shfl_xor(y1, o));
+        z1 = fmaxf(z1, __shfl_xor(z1, o));
         ok &= __shfl_xor(ok, o);
     }
     const int r = (bc.rconst >= 0 ? bc.rconst : p.maxRadius) + 1;
@@ -494,6 +499,12 @@ __device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const Batch
     const int iy0 = __builtin_amdgcn_readfirstlane((int)floorf(y0)) - r;
     const int iy1 = __builtin_amdgcn_readfirstlane((int)floorf(y1)) + r;
     ChunkBox b;
+    // the per-point z differs from the corner bound by a few ulps of values
+    // near [0, 1]; 1e-4 covers it many times over
+    const int zb = __builtin_amdgcn_readfirstlane(__float_as_int(z1 + 1e-4f));
+    b.zmax = (__builtin_amdgcn_readfirstlane(ok) != 0 && __int_as_float(zb) == __int_as_float(zb))
+                 ? __int_as_float(zb)
+                 : __builtin_inff();
     b.frags = 0;
     b.x0 = max(ix0, 0);
     const int bx1 = min(ix1, p.width - 1);
@@ -806,14 +817,36 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
 
 // Stages the tile's depths and clears its sums.
 template <typename G>
-__device__ __forceinline__ void color_stage(const ChunkBox& box, const PassParams& p, const uint32_t* __restrict__ depth,
+__device__ __forceinline__ float color_stage(const ChunkBox& box, const PassParams& p, const uint32_t* __restrict__ depth,
                                    ColorLds L) {
+    // also returns the smallest acceptance threshold uint_to_depth(d) *
+    // depthKeep of the thread's pixels (+inf for pixels with no depth, where
+    // no fragment is accepted)
     const int area = box.w * box.h;
+    float tmin = __builtin_inff();
     for (int i = G::rank(); i < area; i += G::kSize) {
-        lds_store(L.depth + i, gload(depth + box.pixel(i, p.width)));
+        const uint32_t d = gload(depth + box.pixel(i, p.width));
+        lds_store(L.depth + i, d);
         lds_store(L.rg + i, 0ull);
         lds_store(L.bn + i, 0ull);
+        if (d != 0u) tmin = fminf(tmin, uint_to_depth(d) * p.depthKeep);
     }
+    return tmin;
+}
+
+// Smallest v over the group, in every lane; ends with a barrier.
+template <bool kWave, int NT>
+__device__ __forceinline__ float group_minf(float v, int* red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    if (kWave) return v;
+    constexpr int kWaves = NT / 64;
+    if (__lane_id() == 0) lds_store(red + (threadIdx.x >> 6), __float_as_int(v));
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kWaves; k++) v = fminf(v, __int_as_float(lds_load(red + k)));
+    __syncthreads();
+    return v;
 }
 
 // Adds a tile's sums to the planes.  A tile holds one batch, so a pixel's
@@ -848,8 +881,11 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.lds) {
-        color_stage<G>(seg, p, depth, L);
+        const float tmin = group_minf<kWave, NT>(color_stage<G>(seg, p, depth, L), L.red);
         G::sync();
+        // every point of the batch lies behind every accepted depth of its
+        // rectangle: no fragment can pass the test, nothing to add
+        if (tmin > seg.zmax) return;
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
             ChunkRaw<PPT> raw;

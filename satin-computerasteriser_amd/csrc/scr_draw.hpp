@@ -1,5 +1,5 @@
 // scr_draw.hpp — device code shared by the single-view (scr_render.hip) and
-// multi-view (scr_multiview.hip) draw kernels: the cull decision, the point
+// multi-view (scr_mv.hpp, scr_mv_*.hip) draw kernels: the cull decision, the point
 // front end, and the per-batch bodies of the depth, colour and nearest passes.
 //
 // A visible batch is drawn in chunks of NT * PPT points: every

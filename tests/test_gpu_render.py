@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from _pkg import orc, scr
-from fixtures import jittered_cloud, solid_cube
+from fixtures import jittered_cloud, lattice, solid_cube
 from helpers import (MODEL, assert_planes_equal, config1_camera, gpu_cloud, multi_file_packed, oracle_cloud,
                      render_gpu_hqs, render_gpu_nearest)
 
@@ -196,6 +196,28 @@ def test_multi_cloud_occlusion_independent_of_order(ctx):
     texel = outs[0]["rgba8"][32, 32]
     r, b = (texel & 0xFF) / 255, ((texel >> 16) & 0xFF) / 255
     assert r > 0.6 and b < 0.2
+
+
+def test_batches_hidden_behind_a_wall(ctx):
+    # a dense wall of points in front of a cube: the cube's batches lie behind
+    # every accepted depth of their rectangles, so the colour pass skips them
+    # whole; the planes must still equal the oracle's, which draws every point
+    g = np.linspace(-1.5, 1.5, 600, dtype=np.float32)
+    wx, wy = np.meshgrid(g, g)
+    wall = np.stack([wx.ravel(), wy.ravel(), np.full(wx.size, 0.5, np.float32)], axis=1)
+    cube, _ = lattice(24, extent=1.0)
+    cube = cube + np.array([0.0, 0.0, -1.5], np.float32)
+    pos = np.concatenate([wall, cube]).astype(np.float32)
+    col = np.concatenate([np.tile([[0.2, 0.8, 0.2, 1.0]], (len(wall), 1)),
+                          np.tile([[0.9, 0.1, 0.1, 1.0]], (len(cube), 1))]).astype(np.float32)
+    packed = orc.pack(pos, col, pointsPerBatch=1024)
+    cam = scr.Camera.perspective((0.0, 0.0, 5.0), (0, 0, 0), fov=30, aspect=1.0, near=0.5, far=20.0)
+    cfg = scr.ComputeRasteriserConfiguration(enableCLOD=False, depthTolerance=0.01)
+    gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, packed)], cfg, cam, 256, 256)
+    ref = orc.render_hqs([oracle_cloud(packed)], cfg, cam.viewMatrix, cam.projectionMatrix, 256, 256)
+    assert (ref["count"] > 0).all()
+    assert ref["red"].sum() == 51 * ref["count"].sum()   # only wall points (red byte 51) are accepted
+    assert_planes_equal(gpu, ref)
 
 
 def test_pick_round_trips_through_source_indices(ctx):

@@ -14,6 +14,9 @@ roofline: the depth pass (algorithmic bytes = per processed point 1 B level
 + 4/8/12 B position by batch precision, + 80 B per visible batch) against the
 MI355X HBM peak; traffic comes from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs
 summarised in profiles/ when present.
+extra.atomic_roofline: the depth and colour passes' global atomics (counts per
+launch from the rocprofv3 TCC_ATOMIC summary in profiles/) per second against
+the device's measured atomic rate for the same access pattern.
 cpu_baseline: the CPU oracle (scalar C transcription of the same passes,
 multithreaded over batches) on a bounded sample of the same frame's batches.
 """
@@ -142,6 +145,40 @@ def read_traffic(kernel_tag: str):
         if m:
             return float(m.group(2))
     return None
+
+
+def read_counter(kernel_tag: str, counter: str):
+    """Per-dispatch value of a rocprof counter for the named kernel from the
+    committed counter summary (the table whose header names the counter)."""
+    f = REPO / "profiles" / "hqs_100m_counters.md"
+    if not f.exists():
+        return None
+    col = None
+    for line in f.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if counter in cells:
+            col = cells.index(counter)
+            continue
+        if col is not None and cells and re.fullmatch(rf"{re.escape(kernel_tag)}(<[^>]*>)?", cells[0]):
+            try:
+                return float(cells[col])
+            except (ValueError, IndexError):
+                return None
+    return None
+
+
+def atomic_roofline(ctx, kernel_tag: str, ms: float, add: bool):
+    """Global atomics per second of a pass (the TCC_ATOMIC count per launch
+    from the committed counter summary over the pass time measured here)
+    against the device's measured rate for the flushes' pattern."""
+    count = read_counter(kernel_tag, "TCC_ATOMIC_sum")
+    peak = ctx.measure_atomic_rate(add=add) / 1e9
+    if count is None or not (ms > 0):
+        return {"unit": "Gatomic/s", "peak": round(peak, 2), "achieved": None, "frac": None, "atomics": count}
+    achieved = count / (ms * 1e-3) / 1e9
+    return {"bound": "atomics", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "Gatomic/s",
+            "frac": round(achieved / peak, 4) if peak > 0 else None, "atomics": count,
+            "op": "atomicAdd" if add else "atomicMax"}
 
 
 def cloud_bounds(scr, cloud):
@@ -472,6 +509,9 @@ def main() -> int:
     if not args.no_extra:
         cb_bytes, _ = algorithmic_bytes(vis, per_point_extra=4)
         extra["color_roofline_frac"] = round(cb_bytes / (prof.get("color", float("nan")) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        extra["atomic_roofline"] = {"k_depth": atomic_roofline(ctx, "k_depth", depth_ms, add=False),
+                                    "k_color": atomic_roofline(ctx, "k_color", prof.get("color", float("nan")),
+                                                               add=True)}
         # CLOD on at the same pose
         ras.configuration = scr.ComputeRasteriserConfiguration(enableCLOD=True)
         ms_c = timed_frames(ctx, ras, cam, args.steps, 1)

@@ -152,6 +152,13 @@ int scr_memcpy_d2h(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Device-to-device copy, enqueued on the context stream. */
 int scr_memcpy_d2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
 
+/* ---- device measurement (no counterpart in the reference): the global
+ * atomic rate the draw passes' tile flushes can reach — waves updating 64
+ * consecutive u32 words at pseudo-random spots of an 8 MB plane, atomicMax
+ * (add = 0) or atomicAdd (add = 1) — in atomics per second, timed with HIP
+ * events on the context stream.  Synchronous; allocates its own buffer. */
+int scr_measure_atomic_rate(scr_ctx* ctx, int add, double* atomicsPerSecond);
+
 /* ---- point cloud slot pool: ComputeRasteriserPointCloud.swift */
 
 /* init(context:capacity:files:originShift:) — ComputeRasteriserPointCloud.swift:201-217.

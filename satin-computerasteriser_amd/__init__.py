@@ -151,6 +151,7 @@ _SIGNATURES = {
     "scr_memcpy_h2d": (_I, [_VP, _VP, _VP, _SZ]),
     "scr_memcpy_d2h": (_I, [_VP, _VP, _VP, _SZ]),
     "scr_memcpy_d2d": (_I, [_VP, _VP, _VP, _SZ]),
+    "scr_measure_atomic_rate": (_I, [_VP, _I, C.POINTER(C.c_double)]),
     "scr_cloud_create": (_I, [_VP, _I, _I, _VP, _I, C.POINTER(_VP)]),
     "scr_cloud_destroy": (_I, [_VP]),
     "scr_cloud_get_info": (_I, [_VP, C.POINTER(CCloudInfo)]),
@@ -420,6 +421,13 @@ class Context:
 
     def sync(self) -> None:
         self.check(self.lib.scr_sync(self.handle))
+
+    def measure_atomic_rate(self, add: bool = False) -> float:
+        """Global atomics per second (atomicMax, or atomicAdd with add=True) in
+        the pattern of the draw passes' tile flushes."""
+        out = C.c_double(0.0)
+        self.check(self.lib.scr_measure_atomic_rate(self.handle, 1 if add else 0, C.byref(out)))
+        return float(out.value)
 
     def profile(self, enable: bool = True) -> None:
         """Bracket every kernel launch with HIP events (per-pass timing)."""

@@ -382,6 +382,32 @@ int scr_memcpy_d2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return SCR_OK;
 }
 
+int scr_measure_atomic_rate(scr_ctx* ctx, int add, double* atomicsPerSecond) {
+    if (!ctx || !atomicsPerSecond) return SCR_ERR_INVALID_ARGUMENT;
+    *atomicsPerSecond = 0.0;
+    constexpr uint32_t kSpots = 1u << 15;  // 32 K spots x 64 words: an 8 MB plane
+    uint32_t* buf = nullptr;
+    SCR_HIP(ctx, hipMalloc(&buf, (size_t)kSpots * 64u * sizeof(uint32_t)));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipError_t e = scr::launch_fill_u32(ctx->stream, buf, (size_t)kSpots * 64u, 0u);
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    uint64_t n = 0;
+    if (e == hipSuccess) e = scr::launch_atomic_rate(ctx->stream, ctx->numCUs, buf, kSpots, add, 64, &n);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(t0, ctx->stream);
+    if (e == hipSuccess) e = scr::launch_atomic_rate(ctx->stream, ctx->numCUs, buf, kSpots, add, 512, &n);
+    if (e == hipSuccess) e = hipEventRecord(t1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(t1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_error(ctx, e, "atomic rate");
+    *atomicsPerSecond = ms > 0.0f ? (double)n / ((double)ms * 1e-3) : 0.0;
+    return SCR_OK;
+}
+
 // ------------------------------------------------------------------ clouds
 
 int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, const scr_raster_file* files,

@@ -102,6 +102,23 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* __restrict
     for (size_t j = i; j < n && j < i + 2; j++) p[j] = v;
 }
 
+// Global atomic throughput as the draw passes' tile flushes use it: each wave
+// updates 64 consecutive words (one per lane) at a pseudo-random 256-byte
+// aligned spot of the buffer, `iters` times.
+__global__ __launch_bounds__(256) void k_atomic_rate(uint32_t* __restrict__ buf, uint32_t spots, int add, int iters) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t lane = __lane_id();
+    uint32_t h = wave * 0x9E3779B9u + 0x7F4A7C15u;
+    for (int i = 0; i < iters; i++) {
+        h ^= h >> 16;
+        h *= 0x7FEB352Du;
+        h ^= h >> 15;
+        uint32_t* p = buf + (size_t)(h % spots) * 64u + lane;
+        if (add) atomicAdd(p, 1u);
+        else atomicMax(p, h);
+    }
+}
+
 unsigned grid_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
 unsigned draw_grid(int numCUs, int nSlots) {
@@ -122,6 +139,14 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 512)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t spots, int add, int iters,
+                              uint64_t* atomics) {
+    const unsigned grid = (unsigned)numCUs * 16u;
+    hipLaunchKernelGGL(k_atomic_rate, dim3(grid), dim3(256), 0, s, buf, spots, add, iters);
+    *atomics = (uint64_t)grid * 256u * (uint64_t)iters;
     return hipGetLastError();
 }
 

@@ -18,6 +18,14 @@ def cube64():
     return orc.pack(pos, col, pointsPerBatch=10240, lodLevels=4, coarseVoxelDivisions=64, keyBits=30)
 
 
+def test_measure_atomic_rate(ctx):
+    # the atomic rate the bench reports the passes against: finite, and well
+    # above a billion updates per second on any current GPU
+    for add in (False, True):
+        r = ctx.measure_atomic_rate(add=add)
+        assert np.isfinite(r) and r > 1e9
+
+
 @pytest.mark.parametrize("clod", [True, False])
 @pytest.mark.parametrize("dither", [True, False])
 def test_config1_hqs_bit_exact(ctx, cube64, clod, dither):

@@ -352,7 +352,10 @@ struct ChunkBox {
     float zmax;  // (segment_box) above every point's ndc z; +inf when unknown
     // every on-screen footprint pixel of the chunk lies inside the box (it is
     // the clamped union of the footprints), so no bounds test is needed
-    __device__ inline int local(int tx, int ty) const { return (ty - y0) * w + (tx - x0); }
+    // a box holds at most 2^24 pixels per side: one 24-bit multiply-add
+    __device__ inline int local(int tx, int ty) const {
+        return (int)__umul24((uint32_t)(ty - y0), (uint32_t)w) + (tx - x0);
+    }
     __device__ inline size_t pixel(int k, int width) const {
         const int row = w == 1 ? k : (int)__umulhi((uint32_t)k, magic);
         const int col = k - row * w;
@@ -521,7 +524,8 @@ __device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const Batch
 
 // Whether (x, y) lies in the rectangle.
 __device__ __forceinline__ bool box_holds(const ChunkBox& b, int x, int y) {
-    return (uint32_t)(x - b.x0) < (uint32_t)b.w && (uint32_t)(y - b.y0) < (uint32_t)b.h;
+    // both compares, no short circuit: no branch per fragment
+    return ((uint32_t)(x - b.x0) < (uint32_t)b.w) & ((uint32_t)(y - b.y0) < (uint32_t)b.h);
 }
 
 // ---------------------------------------------------------------- depth pass body
@@ -589,11 +593,11 @@ __device__ __forceinline__ void depth_frags_tile(const ChunkFrags<PPT>& f, const
     for (int j = 0; j < PPT; j++) {
         const uint32_t d = depth_to_uint_hw(f.z[j]);
         const bool valid = f.px[j] >= 0;
-        const bool in = valid && box_holds(box, f.px[j], f.py[j]);
-        outside = outside || (valid && !in);
+        const bool in = valid & box_holds(box, f.px[j], f.py[j]);
+        outside = outside | (valid & !in);
         const int k = box.local(f.px[j], f.py[j]);
         const bool same = k == lastK;
-        const bool fresh = in && !(same && d <= lastD);
+        const bool fresh = in & !(same & (d <= lastD));
         lastD = in ? (same ? max(lastD, d) : d) : lastD;
         lastK = in ? k : lastK;
         lds_max(fresh ? lds + k : spare, fresh ? d : 0u);

@@ -819,6 +819,61 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
     if (kR0 && runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
 }
 
+// color_frags for single-pixel points meeting in a batch tile: one branch
+// per fragment (accepted or not) instead of one per test; fragments outside
+// the box (none in practice) take one wave-uniform branch to the planes.
+template <int PPT>
+__device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const uint32_t* rgb, const ChunkBox& box,
+                                                 const PassParams& p, const uint32_t* __restrict__ depth,
+                                                 uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count,
+                                                 ColorLds L) {
+    int runK = -1;
+    unsigned long long runRG = 0ull, runBN = 0ull;
+    bool outside = false;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        const bool valid = f.px[j] >= 0;
+        const bool in = valid & box_holds(box, f.px[j], f.py[j]);
+        outside = outside | (valid & !in);
+        const int k = in ? box.local(f.px[j], f.py[j]) : 0;
+        const uint32_t d = lds_load(L.depth + k);
+        const bool pass = in & (d != 0u) & (f.z[j] >= uint_to_depth(d) * p.depthKeep);
+        if (!pass) continue;
+        const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
+        const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
+        const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
+        if (k == runK) {
+            runRG += rg;
+            runBN += bn;
+            continue;
+        }
+        if (runK >= 0) {
+            lds_add(L.rg + runK, runRG);
+            lds_add(L.bn + runK, runBN);
+        }
+        runK = k;
+        runRG = rg;
+        runBN = bn;
+    }
+    if (runK >= 0) {
+        lds_add(L.rg + runK, runRG);
+        lds_add(L.bn + runK, runBN);
+    }
+    if (__ballot(outside) != 0ull) {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) {
+            if (f.px[j] < 0 || box_holds(box, f.px[j], f.py[j])) continue;
+            const size_t pix = (size_t)f.py[j] * p.width + f.px[j];
+            const uint32_t d = gload(depth + pix);
+            if (d == 0u || !(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+            gadd(red + pix, rgb[j] & 0xffu);
+            gadd(green + pix, (rgb[j] >> 8) & 0xffu);
+            gadd(blue + pix, (rgb[j] >> 16) & 0xffu);
+            gadd(count + pix, 1u);
+        }
+    }
+}
+
 // Stages the tile's depths and clears its sums.
 template <typename G>
 __device__ __forceinline__ float color_stage(const ChunkBox& box, const PassParams& p, const uint32_t* __restrict__ depth,
@@ -899,7 +954,8 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
             ChunkFrags<PPT> f;
             chunk_decode<false, kR0, PPT>(c, p, bc, li0, raw, f);
             chunk_colors<PPT>(c, p, bc, li0, f, rgb);
-            color_frags<kR0, true, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
+            if (kR0) color_frags_tile<PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
+            else color_frags<kR0, true, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
         }
         G::sync();
         color_tile_flush<G>(seg, p, L, red, green, blue, count);

@@ -1029,6 +1029,43 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
     }
 }
 
+// nearest_frags for single-pixel points meeting in a batch tile, with no
+// per-fragment branch (as depth_frags_tile): a fragment that is invalid,
+// outside the box or already exceeded by this thread aims a zero max at the
+// lane's spare word.
+template <int PPT>
+__device__ __forceinline__ void nearest_frags_tile(const ChunkFrags<PPT>& f, uint32_t base, const ChunkBox& box,
+                                                   const PassParams& p, unsigned long long* lds,
+                                                   unsigned long long* spare,
+                                                   unsigned long long* __restrict__ packed) {
+    int lastK = -1;
+    unsigned long long lastKey = 0ull;
+    bool outside = false;
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        // max of depth << 32 | ~index == the two-pass rule: deepest
+        // reverse-Z depth, then the lowest point index among equal depths
+        const unsigned long long key =
+            ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~(base + (uint32_t)j));
+        const bool valid = f.px[j] >= 0;
+        const bool in = valid & box_holds(box, f.px[j], f.py[j]);
+        outside = outside | (valid & !in);
+        const int k = box.local(f.px[j], f.py[j]);
+        const bool same = k == lastK;
+        const bool fresh = in & !(same & (key <= lastKey));
+        lastKey = in ? (same ? (key > lastKey ? key : lastKey) : key) : lastKey;
+        lastK = in ? k : lastK;
+        lds_max(fresh ? lds + k : spare, fresh ? key : 0ull);
+    }
+    if (__ballot(outside) != 0ull) {
+#pragma unroll
+        for (int j = 0; j < PPT; j++)
+            if (f.px[j] >= 0 && !box_holds(box, f.px[j], f.py[j]))
+                gmax(packed + (size_t)f.py[j] * p.width + f.px[j],
+                     ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~(base + (uint32_t)j)));
+    }
+}
+
 template <typename G>
 __device__ __forceinline__ void nearest_flush(const ChunkBox& box, const PassParams& p, const unsigned long long* lds,
                                      unsigned long long* __restrict__ packed) {
@@ -1053,11 +1090,13 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
         const int area = seg.w * seg.h;
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
         G::sync();
+        unsigned long long* spare = ldsAll + kNearestTile + __lane_id();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
             ChunkFrags<PPT> f;
             chunk_front<true, kR0, PPT>(c, p, bc, li0, vec, f);
-            nearest_frags<kR0, true, PPT>(f, bc.first + li0, seg, p, lds, packed);
+            if (kR0) nearest_frags_tile<PPT>(f, bc.first + li0, seg, p, lds, spare, packed);
+            else nearest_frags<kR0, true, PPT>(f, bc.first + li0, seg, p, lds, packed);
         }
         G::sync();
         nearest_flush<G>(seg, p, lds, packed);

@@ -96,7 +96,7 @@ template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView* __restrict__ views,
                                                    const MVItem* __restrict__ items, uint32_t nUnits, int splits,
                                                    uint32_t* heads) {
-    __shared__ unsigned long long lds[kNearestTile];
+    __shared__ unsigned long long lds[kNearestTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
                    [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {

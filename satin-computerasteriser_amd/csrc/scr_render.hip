@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_resolve(PassParams p, const uint32_t* _
 template <bool kR0>
 __global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_nearest(CloudDev c, PassParams p,
                                                       unsigned long long* __restrict__ packed) {
-    __shared__ unsigned long long lds[kNearestTile];
+    __shared__ unsigned long long lds[kNearestTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)

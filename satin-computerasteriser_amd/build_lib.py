@@ -42,21 +42,40 @@ def _hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def _compile(src: Path, hipcc: str) -> Path:
-    obj = BUILD / (src.stem + ".o")
-    cmd = [hipcc, *COMMON_FLAGS, "-c", str(src), "-o", str(obj)]
+# scr_mv_draw.hip holds the batched draw kernel of one (phase, shape) pair
+# and is compiled once per pair: 3 phases x 6 shapes
+MV_DRAW = "scr_mv_draw.hip"
+MV_UNITS = [(ph, sh) for ph in range(3) for sh in range(6)]
+
+
+def _compile(unit, hipcc: str) -> Path:
+    src, defines, stem = unit
+    obj = BUILD / (stem + ".o")
+    cmd = [hipcc, *COMMON_FLAGS, *defines, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr}")
+        raise RuntimeError(f"hipcc failed for {stem}:\n{res.stderr}")
     return obj
+
+
+def _units():
+    units = []
+    for src in sorted(CSRC.glob("*.hip")):
+        if src.name == MV_DRAW:
+            units += [(src, [f"-DSCR_MV_PHASE={ph}", f"-DSCR_MV_SHAPE={sh}"], f"{src.stem}_{ph}_{sh}")
+                      for ph, sh in MV_UNITS]
+        else:
+            units.append((src, [], src.stem))
+    # longest first: the multi-view draw units, then the rest
+    return sorted(units, key=lambda u: 0 if u[0].name == MV_DRAW else 1)
 
 
 def build(verbose: bool = False) -> Path:
     hipcc = _hipcc()
     BUILD.mkdir(exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
-        objs = list(ex.map(lambda s: _compile(s, hipcc), sources))
+    units = _units()
+    with cf.ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, 16, len(units))) as ex:
+        objs = list(ex.map(lambda u: _compile(u, hipcc), units))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -64,7 +83,7 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
     if verbose:
-        print(f"built {LIB} from {len(sources)} sources", file=sys.stderr)
+        print(f"built {LIB} from {len(units)} objects", file=sys.stderr)
     return LIB
 
 

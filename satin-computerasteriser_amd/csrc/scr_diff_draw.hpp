@@ -100,20 +100,20 @@ __device__ __forceinline__ void diff_tile_flush(const ChunkBox& box, const PassP
 // Local points [lo, bc.active) of one visible batch (lo a multiple of the
 // chunk): one tile for the whole range when the batch rectangle fits it,
 // else one per chunk.
-template <bool kR0>
-__device__ __forceinline__ void diff_color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+template <bool kR0, int kLevel>
+__device__ __forceinline__ void diff_color_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                          const float4* __restrict__ params, const uint32_t* __restrict__ depth,
-                                         unsigned long long* sums, size_t stride, uint32_t* count, DiffLds& L) {
+                                         unsigned long long* sums, size_t stride, uint32_t* count, DiffLds& L,
+                                              const ChunkBox& seg) {
     constexpr int kChunk = kDiffThreads * kDiffPPT;
     const bool vec = vec_ok<kDiffPPT>(bc);
-    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? bc.active - lo : 0u, kDiffTile);
     if (seg.lds) {
         diff_stage(seg, p, depth, L);
         __syncthreads();
         for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
             const uint32_t li0 = cb + threadIdx.x * kDiffPPT;
             ChunkFrags<kDiffPPT> f;
-            chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+            chunk_front<false, kR0, kLevel, kDiffPPT>(c, p, bc, li0, vec, f);
             diff_frags<kR0, true>(f, bc.first + li0, seg, p, params, depth, sums, stride, count, L);
         }
         __syncthreads();
@@ -124,7 +124,7 @@ __device__ __forceinline__ void diff_color_chunks(const CloudDev& c, const PassP
     for (uint32_t cb = lo; cb < bc.active; cb += kChunk) {
         const uint32_t li0 = cb + threadIdx.x * kDiffPPT;
         ChunkFrags<kDiffPPT> f;
-        chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+        chunk_front<false, kR0, kLevel, kDiffPPT>(c, p, bc, li0, vec, f);
         const ChunkBox box = chunk_box<false, kDiffThreads, kDiffPPT>(f, p, kDiffTile, L.red);
         if (box.frags == 0) continue;
         if (box.lds) {
@@ -140,6 +140,16 @@ __device__ __forceinline__ void diff_color_chunks(const CloudDev& c, const PassP
     }
 }
 
+template <bool kR0>
+__device__ __forceinline__ void diff_color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                         const float4* __restrict__ params, const uint32_t* __restrict__ depth,
+                                         unsigned long long* sums, size_t stride, uint32_t* count, DiffLds& L) {
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? bc.active - lo : 0u, kDiffTile);
+    if (bc.level == 0) diff_color_chunks_level<kR0, 0>(c, p, bc, lo, params, depth, sums, stride, count, L, seg);
+    else if (bc.level == 1) diff_color_chunks_level<kR0, 1>(c, p, bc, lo, params, depth, sums, stride, count, L, seg);
+    else diff_color_chunks_level<kR0, 2>(c, p, bc, lo, params, depth, sums, stride, count, L, seg);
+}
+
 // The gradient of the thread's PPT points starting at local index li0 under
 // one view, added to acc (loaded from grad when a point is first touched;
 // touched marks the points to write back): per fragment that passes the
@@ -152,7 +162,9 @@ __device__ __forceinline__ void diff_backward_acc(const CloudDev& c, const PassP
                                                   const float* __restrict__ dLdImage, const double* grad,
                                                   double (&acc)[kDiffPPT][3], bool (&touched)[kDiffPPT]) {
     ChunkFrags<kDiffPPT> f;
-    chunk_front<false, kR0, kDiffPPT>(c, p, bc, li0, vec, f);
+    if (bc.level == 0) chunk_front<false, kR0, 0, kDiffPPT>(c, p, bc, li0, vec, f);
+    else if (bc.level == 1) chunk_front<false, kR0, 1, kDiffPPT>(c, p, bc, li0, vec, f);
+    else chunk_front<false, kR0, 2, kDiffPPT>(c, p, bc, li0, vec, f);
 #pragma unroll
     for (int j = 0; j < kDiffPPT; j++) {
         if (f.px[j] < 0) continue;

@@ -246,15 +246,15 @@ struct ChunkRaw {
 };
 
 // Issues the loads of the thread's PPT points starting at local index li0.
-template <int PPT>
+template <int kLevel, int PPT>
 __device__ __forceinline__ void chunk_fetch(const CloudDev& c, const BatchCtx& bc, uint32_t li0, bool vec, ChunkRaw<PPT>& r) {
     const uint32_t base = bc.first + li0;
     const uint32_t n = li0 < bc.active ? min((uint32_t)PPT, bc.active - li0) : 0u;
     if (vec && n == (uint32_t)PPT) {
         // base is a multiple of PPT: 16-byte aligned words, PPT-byte aligned levels
         load_words<PPT>(c.xyzLow, base, r.lo);
-        if (bc.level <= 1) load_words<PPT>(c.xyzMed, base, r.me);
-        if (bc.level == 0) load_words<PPT>(c.xyzHigh, base, r.hi);
+        if (kLevel <= 1) load_words<PPT>(c.xyzMed, base, r.me);
+        if (kLevel == 0) load_words<PPT>(c.xyzHigh, base, r.hi);
         if (!bc.keepAll) {
 #pragma unroll
             for (int q = 0; q < PPT / 4; q++) r.lv4[q] = gload(reinterpret_cast<const uint32_t*>(c.levels + base) + q);
@@ -267,8 +267,8 @@ __device__ __forceinline__ void chunk_fetch(const CloudDev& c, const BatchCtx& b
             const uint32_t pi = base + (uint32_t)j;
             const bool ok = (uint32_t)j < n;
             r.lo[j] = ok ? gload(c.xyzLow + pi) : 0u;
-            r.me[j] = (ok && bc.level <= 1) ? gload(c.xyzMed + pi) : 0u;
-            r.hi[j] = (ok && bc.level == 0) ? gload(c.xyzHigh + pi) : 0u;
+            r.me[j] = (ok && kLevel <= 1) ? gload(c.xyzMed + pi) : 0u;
+            r.hi[j] = (ok && kLevel == 0) ? gload(c.xyzHigh + pi) : 0u;
             if (ok && !bc.keepAll) r.lv4[j / 4] |= (uint32_t)gload(c.levels + pi) << (8 * (j % 4));
         }
     }
@@ -277,7 +277,7 @@ __device__ __forceinline__ void chunk_fetch(const CloudDev& c, const BatchCtx& b
 // Runs the front end of every point pass on the fetched points: dither/LOD
 // skip, decodePoint (Common.metal:73-117), projection, clip and viewport
 // tests, y flip, footprint radius.
-template <bool kNearestZ, bool kR0, int PPT>
+template <bool kNearestZ, bool kR0, int kLevel, int PPT>
 __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                     const ChunkRaw<PPT>& r, ChunkFrags<PPT>& f) {
     const uint32_t base = bc.first + li0;
@@ -296,12 +296,12 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         }
         uint32_t qx, qy, qz;
         const uint32_t l = r.lo[j];
-        if (bc.level == 0) {
+        if (kLevel == 0) {
             const uint32_t m = r.me[j], h = r.hi[j];
             qx = ((l & 1023u) << 20) | ((m & 1023u) << 10) | (h & 1023u);
             qy = (((l >> 10) & 1023u) << 20) | (((m >> 10) & 1023u) << 10) | ((h >> 10) & 1023u);
             qz = (((l >> 20) & 1023u) << 20) | (((m >> 20) & 1023u) << 10) | ((h >> 20) & 1023u);
-        } else if (bc.level == 1) {
+        } else if (kLevel == 1) {
             const uint32_t m = r.me[j];
             qx = ((l & 1023u) << 10) | (m & 1023u);
             qy = (((l >> 10) & 1023u) << 10) | ((m >> 10) & 1023u);
@@ -331,12 +331,12 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
     }
 }
 
-template <bool kNearestZ, bool kR0, int PPT>
+template <bool kNearestZ, bool kR0, int kLevel, int PPT>
 __device__ __forceinline__ void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                    bool vec, ChunkFrags<PPT>& f) {
     ChunkRaw<PPT> r;
-    chunk_fetch<PPT>(c, bc, li0, vec, r);
-    chunk_decode<kNearestZ, kR0, PPT>(c, p, bc, li0, r, f);
+    chunk_fetch<kLevel, PPT>(c, bc, li0, vec, r);
+    chunk_decode<kNearestZ, kR0, kLevel, PPT>(c, p, bc, li0, r, f);
 }
 
 // ---------------------------------------------------------------- chunk rectangle
@@ -623,15 +623,15 @@ __device__ __forceinline__ void depth_flush(const ChunkBox& box, const PassParam
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
-__device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
-                                    uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red) {
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel>
+__device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                    uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red,
+        const ChunkBox& seg) {
     using G = Group<kWave, NT>;
     constexpr int kCap = kDepthTile / G::kCount;
     uint32_t* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
-    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.lds) {
         const int area = seg.w * seg.h;
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
@@ -639,7 +639,7 @@ __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams
         uint32_t* spare = ldsAll + kDepthTile + __lane_id();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             ChunkFrags<PPT> f;
-            chunk_front<false, kR0, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
+            chunk_front<false, kR0, kLevel, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
             if (kR0) depth_frags_tile<PPT>(f, seg, p, lds, spare, depth);
             else depth_frags<kR0, true, PPT>(f, seg, p, lds, depth);
         }
@@ -650,7 +650,7 @@ __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams
     }
     for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         ChunkFrags<PPT> f;
-        chunk_front<false, kR0, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
+        chunk_front<false, kR0, kLevel, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
         if (box.frags == 0) continue;
         if (box.lds) {
@@ -665,6 +665,18 @@ __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams
             G::sync();
         }
     }
+}
+
+template <bool kR0, bool kWave, int NT, int PPT>
+__device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                    uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red) {
+    using G = Group<kWave, NT>;
+    constexpr int kCap = kDepthTile / G::kCount;
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    // one straight-line copy of the body per position precision
+    if (bc.level == 0) depth_chunks_level<kR0, kWave, NT, PPT, 0>(c, p, bc, lo, depth, ldsAll, red, seg);
+    else if (bc.level == 1) depth_chunks_level<kR0, kWave, NT, PPT, 1>(c, p, bc, lo, depth, ldsAll, red, seg);
+    else depth_chunks_level<kR0, kWave, NT, PPT, 2>(c, p, bc, lo, depth, ldsAll, red, seg);
 }
 
 // ---------------------------------------------------------------- colour pass body
@@ -928,17 +940,17 @@ __device__ __forceinline__ void color_tile_flush(const ChunkBox& box, const Pass
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
-__device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel>
+__device__ __forceinline__ void color_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
-                                    uint32_t* blue, uint32_t* count, ColorLds LAll) {
+                                    uint32_t* blue, uint32_t* count, ColorLds LAll,
+        const ChunkBox& seg) {
     using G = Group<kWave, NT>;
     constexpr int kCap = kColorTile / G::kCount;
     const int slice = G::index() * kCap;
     const ColorLds L = color_lds(LAll.depth + slice, LAll.rg + slice, LAll.bn + slice, LAll.red);
     const bool vec = vec_ok<PPT>(bc);
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
-    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.lds) {
         const float tmin = group_minf<kWave, NT>(color_stage<G>(seg, p, depth, L), L.red);
         G::sync();
@@ -948,11 +960,11 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
             ChunkRaw<PPT> raw;
-            chunk_fetch<PPT>(c, bc, li0, vec, raw);
+            chunk_fetch<kLevel, PPT>(c, bc, li0, vec, raw);
             uint32_t rgb[PPT];
             color_fetch<PPT>(c, bc, li0, vec, rgb);
             ChunkFrags<PPT> f;
-            chunk_decode<false, kR0, PPT>(c, p, bc, li0, raw, f);
+            chunk_decode<false, kR0, kLevel, PPT>(c, p, bc, li0, raw, f);
             chunk_colors<PPT>(c, p, bc, li0, f, rgb);
             if (kR0) color_frags_tile<PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
             else color_frags<kR0, true, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
@@ -965,11 +977,11 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
     for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         const uint32_t li0 = cb + G::rank() * PPT;
         ChunkRaw<PPT> raw;
-        chunk_fetch<PPT>(c, bc, li0, vec, raw);
+        chunk_fetch<kLevel, PPT>(c, bc, li0, vec, raw);
         uint32_t rgb[PPT];
         color_fetch<PPT>(c, bc, li0, vec, rgb);
         ChunkFrags<PPT> f;
-        chunk_decode<false, kR0, PPT>(c, p, bc, li0, raw, f);
+        chunk_decode<false, kR0, kLevel, PPT>(c, p, bc, li0, raw, f);
         chunk_colors<PPT>(c, p, bc, li0, f, rgb);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
         if (box.frags == 0) continue;
@@ -984,6 +996,19 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
             G::sync();
         }
     }
+}
+
+template <bool kR0, bool kWave, int NT, int PPT>
+__device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                    const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
+                                    uint32_t* blue, uint32_t* count, ColorLds LAll) {
+    using G = Group<kWave, NT>;
+    constexpr int kCap = kColorTile / G::kCount;
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    // one straight-line copy of the body per position precision
+    if (bc.level == 0) color_chunks_level<kR0, kWave, NT, PPT, 0>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    else if (bc.level == 1) color_chunks_level<kR0, kWave, NT, PPT, 1>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    else color_chunks_level<kR0, kWave, NT, PPT, 2>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
 }
 
 // ---------------------------------------------------------------- nearest pass body
@@ -1077,15 +1102,15 @@ __device__ __forceinline__ void nearest_flush(const ChunkBox& box, const PassPar
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
-__device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
-                                      unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red) {
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel>
+__device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                      unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red,
+        const ChunkBox& seg) {
     using G = Group<kWave, NT>;
     constexpr int kCap = kNearestTile / G::kCount;
     unsigned long long* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
-    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.lds) {
         const int area = seg.w * seg.h;
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
@@ -1094,7 +1119,7 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
             ChunkFrags<PPT> f;
-            chunk_front<true, kR0, PPT>(c, p, bc, li0, vec, f);
+            chunk_front<true, kR0, kLevel, PPT>(c, p, bc, li0, vec, f);
             if (kR0) nearest_frags_tile<PPT>(f, bc.first + li0, seg, p, lds, spare, packed);
             else nearest_frags<kR0, true, PPT>(f, bc.first + li0, seg, p, lds, packed);
         }
@@ -1106,7 +1131,7 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
     for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         const uint32_t li0 = cb + G::rank() * PPT;
         ChunkFrags<PPT> f;
-        chunk_front<true, kR0, PPT>(c, p, bc, li0, vec, f);
+        chunk_front<true, kR0, kLevel, PPT>(c, p, bc, li0, vec, f);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
         if (box.frags == 0) continue;
         if (box.lds) {
@@ -1121,6 +1146,18 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
             G::sync();
         }
     }
+}
+
+template <bool kR0, bool kWave, int NT, int PPT>
+__device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
+                                      unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red) {
+    using G = Group<kWave, NT>;
+    constexpr int kCap = kNearestTile / G::kCount;
+    const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    // one straight-line copy of the body per position precision
+    if (bc.level == 0) nearest_chunks_level<kR0, kWave, NT, PPT, 0>(c, p, bc, lo, packed, ldsAll, red, seg);
+    else if (bc.level == 1) nearest_chunks_level<kR0, kWave, NT, PPT, 1>(c, p, bc, lo, packed, ldsAll, red, seg);
+    else nearest_chunks_level<kR0, kWave, NT, PPT, 2>(c, p, bc, lo, packed, ldsAll, red, seg);
 }
 
 // ---------------------------------------------------------------- per-batch entry points

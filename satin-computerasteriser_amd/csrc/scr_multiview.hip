@@ -168,6 +168,17 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_backward(const MVView*
 
 // ---------------------------------------------------------------- draw variants
 
+// The kernel set of one (phase, shape), from its own object file.
+const ShapeKernels& shape_kernels(Phase ph, int shape) {
+#define SCR_MV_ROW(P) \
+    {&SCR_MV_ACCESSOR(P, 0), &SCR_MV_ACCESSOR(P, 1), &SCR_MV_ACCESSOR(P, 2), &SCR_MV_ACCESSOR(P, 3), \
+     &SCR_MV_ACCESSOR(P, 4), &SCR_MV_ACCESSOR(P, 5)}
+    using Accessor = const ShapeKernels& (*)();
+    static const Accessor table[3][kNumShapes] = {SCR_MV_ROW(0), SCR_MV_ROW(1), SCR_MV_ROW(2)};
+#undef SCR_MV_ROW
+    return table[(int)ph][shape]();
+}
+
 struct Variant {
     int threads, ppt;
     DrawKernel depth[2][2], color[2][2], nearest[2][2];  // [kR0][kWave]
@@ -182,9 +193,9 @@ const Variant* variants() {
             t[i].ppt = kShapes[i].ppt;
             for (int r = 0; r < 2; r++)
                 for (int w = 0; w < 2; w++) {
-                    t[i].depth[r][w] = depth_kernels()[i][r][w];
-                    t[i].color[r][w] = color_kernels()[i][r][w];
-                    t[i].nearest[r][w] = nearest_kernels()[i][r][w];
+                    t[i].depth[r][w] = shape_kernels(kDepth, i)[r][w];
+                    t[i].color[r][w] = shape_kernels(kColor, i)[r][w];
+                    t[i].nearest[r][w] = shape_kernels(kNearest, i)[r][w];
                 }
         }
         return t;

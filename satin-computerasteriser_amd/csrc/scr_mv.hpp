@@ -113,19 +113,24 @@ struct Shape {
 constexpr Shape kShapes[] = {{128, 8}, {256, 4}, {256, 8}, {256, 16}, {512, 4}, {512, 8}};
 constexpr int kNumShapes = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
 
-// kernels[shape][kR0][kWave] of one phase, in kShapes order
-#define SCR_MV_FOUR(K, NT, PPT) \
-    {{K<false, false, NT, PPT>, K<false, true, NT, PPT>}, {K<true, false, NT, PPT>, K<true, true, NT, PPT>}}
-#define SCR_MV_TABLE(K)                                                                                     \
-    {SCR_MV_FOUR(K, 128, 8), SCR_MV_FOUR(K, 256, 4), SCR_MV_FOUR(K, 256, 8), SCR_MV_FOUR(K, 256, 16),      \
-     SCR_MV_FOUR(K, 512, 4), SCR_MV_FOUR(K, 512, 8)}
-
 enum Phase { kDepth = 0, kColor = 1, kNearest = 2 };
 
-using KernelTable = DrawKernel[kNumShapes][2][2];
-const KernelTable& depth_kernels();    // scr_mv_depth.hip
-const KernelTable& color_kernels();    // scr_mv_color.hip
-const KernelTable& nearest_kernels();  // scr_mv_nearest.hip
+// kernels[kR0][kWave] of one (phase, shape).  scr_mv_draw.hip is compiled
+// once per pair (SCR_MV_PHASE, SCR_MV_SHAPE), so the library builds the
+// 18 kernel sets in parallel; each object defines its pair's accessor.
+using ShapeKernels = DrawKernel[2][2];
+#define SCR_MV_ACCESSOR(P, S) shape_kernels_##P##_##S
+#define SCR_MV_DECLARE(P) \
+    const ShapeKernels& SCR_MV_ACCESSOR(P, 0)(); \
+    const ShapeKernels& SCR_MV_ACCESSOR(P, 1)(); \
+    const ShapeKernels& SCR_MV_ACCESSOR(P, 2)(); \
+    const ShapeKernels& SCR_MV_ACCESSOR(P, 3)(); \
+    const ShapeKernels& SCR_MV_ACCESSOR(P, 4)(); \
+    const ShapeKernels& SCR_MV_ACCESSOR(P, 5)();
+SCR_MV_DECLARE(0)
+SCR_MV_DECLARE(1)
+SCR_MV_DECLARE(2)
+static_assert(kNumShapes == 6, "one accessor per shape");
 
 }  // namespace mv
 }  // namespace scr

@@ -735,6 +735,13 @@ __device__ __forceinline__ void color_fetch(const CloudDev& c, const BatchCtx& b
 template <int PPT>
 __device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                              ChunkFrags<PPT>& f, uint32_t* rgb) {
+    if (p.colorizeChunks == 0 && p.colorizeOverdraw == 0 && p.applyTint == 0) {
+        // plain colours (one uniform test per chunk): the point word's RGB
+        // bytes; the words of points without a fragment are never read
+#pragma unroll
+        for (int j = 0; j < PPT; j++) rgb[j] &= 0x00ffffffu;
+        return;
+    }
     const uint32_t base = bc.first + li0;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {

@@ -323,7 +323,9 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         else ok = ok && !(nz < 0.0f || nz > 1.0f);
         const int ix = f2i_hw(fmaf(nx, 0.5f, 0.5f) * (float)p.width);
         const int iy = f2i_hw(fmaf(ny, 0.5f, 0.5f) * (float)p.height);
-        ok = ok && !(ix < 0 || ix >= p.width || iy < 0 || iy >= p.height);
+        // ix, iy >= 0 whenever the tests above pass: fma(n, 0.5, 0.5) of
+        // n in [-1, 1] lies in [0, 1], and a NaN converts to 0
+        ok = ok && !(ix >= p.width || iy >= p.height);
         f.px[j] = ok ? ix : -1;
         f.py[j] = p.height - 1 - iy;
         f.z[j] = nz;
@@ -579,8 +581,8 @@ __device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const Chun
 
 // depth_frags for single-pixel points meeting in a batch tile, with no
 // per-fragment branches: a fragment that is invalid, outside the box or
-// already exceeded by this thread aims a zero max (a no-op) at the lane's
-// spare word past the tile; fragments outside the box (none in practice, the
+// already exceeded by this thread aims its max at the lane's spare word past
+// the tile, which nothing reads; fragments outside the box (none in practice, the
 // box bounding the batch) take one wave-uniform branch to the plane.
 template <int PPT>
 __device__ __forceinline__ void depth_frags_tile(const ChunkFrags<PPT>& f, const ChunkBox& box,
@@ -592,15 +594,15 @@ __device__ __forceinline__ void depth_frags_tile(const ChunkFrags<PPT>& f, const
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const uint32_t d = depth_to_uint_hw(f.z[j]);
-        const bool valid = f.px[j] >= 0;
-        const bool in = valid & box_holds(box, f.px[j], f.py[j]);
-        outside = outside | (valid & !in);
+        // box.x0 >= 0 and box.w <= the tile: the sentinel px = -1 is never held
+        const bool in = box_holds(box, f.px[j], f.py[j]);
+        outside = outside | ((f.px[j] >= 0) & !in);
         const int k = box.local(f.px[j], f.py[j]);
-        const bool same = k == lastK;
-        const bool fresh = in & !(same & (d <= lastD));
-        lastD = in ? (same ? max(lastD, d) : d) : lastD;
-        lastK = in ? k : lastK;
-        lds_max(fresh ? lds + k : spare, fresh ? d : 0u);
+        const bool fresh = in & !((k == lastK) & (d <= lastD));
+        lastD = fresh ? d : lastD;
+        lastK = fresh ? k : lastK;
+        // the spare words are never read, so any value may land there
+        lds_max(fresh ? lds + k : spare, d);
     }
     if (__ballot(outside) != 0ull) {
 #pragma unroll
@@ -1063,8 +1065,8 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
 
 // nearest_frags for single-pixel points meeting in a batch tile, with no
 // per-fragment branch (as depth_frags_tile): a fragment that is invalid,
-// outside the box or already exceeded by this thread aims a zero max at the
-// lane's spare word.
+// outside the box or already exceeded by this thread aims its max at the
+// lane's spare word, which nothing reads.
 template <int PPT>
 __device__ __forceinline__ void nearest_frags_tile(const ChunkFrags<PPT>& f, uint32_t base, const ChunkBox& box,
                                                    const PassParams& p, unsigned long long* lds,
@@ -1079,15 +1081,15 @@ __device__ __forceinline__ void nearest_frags_tile(const ChunkFrags<PPT>& f, uin
         // reverse-Z depth, then the lowest point index among equal depths
         const unsigned long long key =
             ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~(base + (uint32_t)j));
-        const bool valid = f.px[j] >= 0;
-        const bool in = valid & box_holds(box, f.px[j], f.py[j]);
-        outside = outside | (valid & !in);
+        // box.x0 >= 0 and box.w <= the tile: the sentinel px = -1 is never held
+        const bool in = box_holds(box, f.px[j], f.py[j]);
+        outside = outside | ((f.px[j] >= 0) & !in);
         const int k = box.local(f.px[j], f.py[j]);
-        const bool same = k == lastK;
-        const bool fresh = in & !(same & (key <= lastKey));
-        lastKey = in ? (same ? (key > lastKey ? key : lastKey) : key) : lastKey;
-        lastK = in ? k : lastK;
-        lds_max(fresh ? lds + k : spare, fresh ? key : 0ull);
+        const bool fresh = in & !((k == lastK) & (key <= lastKey));
+        lastKey = fresh ? key : lastKey;
+        lastK = fresh ? k : lastK;
+        // the spare words are never read, so any value may land there
+        lds_max(fresh ? lds + k : spare, key);
     }
     if (__ballot(outside) != 0ull) {
 #pragma unroll

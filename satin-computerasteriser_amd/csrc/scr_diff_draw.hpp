@@ -41,7 +41,7 @@ __device__ __forceinline__ void diff_frags(const ChunkFrags<kDiffPPT>& f, uint32
                                            size_t stride, uint32_t* count, DiffLds& L) {
 #pragma unroll
     for (int j = 0; j < kDiffPPT; j++) {
-        if (f.px[j] < 0) continue;
+        if (!f.ok[j]) continue;
         const float4 t = gload_rec(params + base + (uint32_t)j);
         const unsigned long long q0 = diff_fixed(t.x), q1 = diff_fixed(t.y), q2 = diff_fixed(t.z);
         const int rad = kR0 ? 0 : f.r[j];
@@ -167,7 +167,7 @@ __device__ __forceinline__ void diff_backward_acc(const CloudDev& c, const PassP
     else chunk_front<false, kR0, 2, kDiffPPT>(c, p, bc, li0, vec, f);
 #pragma unroll
     for (int j = 0; j < kDiffPPT; j++) {
-        if (f.px[j] < 0) continue;
+        if (!f.ok[j]) continue;
         double g0 = 0.0, g1 = 0.0, g2 = 0.0;
         bool any = false;
         const int rad = kR0 ? 0 : f.r[j];

@@ -219,7 +219,8 @@ __device__ __forceinline__ bool vec_ok(const BatchCtx& bc) {
 
 template <int PPT>
 struct ChunkFrags {
-    int px[PPT];      // -1: no fragment
+    bool ok[PPT];     // the point has a fragment (px, py, z, r are defined only then)
+    int px[PPT];
     int py[PPT];      // y-flipped row
     int r[PPT];       // footprint radius
     float z[PPT];     // ndc z
@@ -326,7 +327,8 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         // ix, iy >= 0 whenever the tests above pass: fma(n, 0.5, 0.5) of
         // n in [-1, 1] lies in [0, 1], and a NaN converts to 0
         ok = ok && !(ix >= p.width || iy >= p.height);
-        f.px[j] = ok ? ix : -1;
+        f.ok[j] = ok;
+        f.px[j] = ix;
         f.py[j] = p.height - 1 - iy;
         f.z[j] = nz;
         f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, x, y, z));
@@ -406,7 +408,7 @@ __device__ __forceinline__ ChunkBox chunk_box(const ChunkFrags<PPT>& f, const Pa
     int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -1, y1 = -1, n = 0;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        if (f.px[j] < 0) continue;
+        if (!f.ok[j]) continue;
         x0 = min(x0, f.px[j] - f.r[j]);
         x1 = max(x1, f.px[j] + f.r[j]);
         y0 = min(y0, f.py[j] - f.r[j]);
@@ -552,7 +554,7 @@ __device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const Chun
     uint32_t lastD = 0u;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        if (f.px[j] < 0) continue;
+        if (!f.ok[j]) continue;
         const uint32_t d = depth_to_uint_hw(f.z[j]);
         if (kR0) {
             if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
@@ -594,9 +596,8 @@ __device__ __forceinline__ void depth_frags_tile(const ChunkFrags<PPT>& f, const
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const uint32_t d = depth_to_uint_hw(f.z[j]);
-        // box.x0 >= 0 and box.w <= the tile: the sentinel px = -1 is never held
-        const bool in = box_holds(box, f.px[j], f.py[j]);
-        outside = outside | ((f.px[j] >= 0) & !in);
+        const bool in = f.ok[j] & box_holds(box, f.px[j], f.py[j]);
+        outside = outside | (f.ok[j] & !in);
         const int k = box.local(f.px[j], f.py[j]);
         const bool fresh = in & !((k == lastK) & (d <= lastD));
         lastD = fresh ? d : lastD;
@@ -607,7 +608,7 @@ __device__ __forceinline__ void depth_frags_tile(const ChunkFrags<PPT>& f, const
     if (__ballot(outside) != 0ull) {
 #pragma unroll
         for (int j = 0; j < PPT; j++)
-            if (f.px[j] >= 0 && !box_holds(box, f.px[j], f.py[j]))
+            if (f.ok[j] && !box_holds(box, f.px[j], f.py[j]))
                 gmax(depth + (size_t)f.py[j] * p.width + f.px[j], depth_to_uint_hw(f.z[j]));
     }
 }
@@ -750,14 +751,14 @@ __device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams
         const uint32_t pi = base + (uint32_t)j;
         uint32_t color = rgb[j];
         rgb[j] = 0u;
-        if (f.px[j] < 0) continue;
+        if (!f.ok[j]) continue;
         if (p.colorizeChunks != 0) color = bc.batchIndex * 1234567u;
         else if (p.colorizeOverdraw != 0) color = 0x00010101u;
         uint32_t r8 = color & 0xffu, g8 = (color >> 8) & 0xffu, b8 = (color >> 16) & 0xffu;
         if (p.applyTint != 0) {
             const float4 t = c.tints ? gload_rec(c.tints + pi) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (t.w < 0.0f) {  // discard sentinel, ColorPass/Shaders.metal:119
-                f.px[j] = -1;
+                f.ok[j] = false;
                 continue;
             }
             const float w = saturate(t.w);
@@ -784,7 +785,7 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
     unsigned long long runRG = 0ull, runBN = 0ull;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        if (f.px[j] < 0) continue;
+        if (!f.ok[j]) continue;
         const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
         const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
         const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
@@ -853,7 +854,7 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
     bool outside = false;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        const bool valid = f.px[j] >= 0;
+        const bool valid = f.ok[j];
         const bool in = valid & box_holds(box, f.px[j], f.py[j]);
         outside = outside | (valid & !in);
         const int k = in ? box.local(f.px[j], f.py[j]) : 0;
@@ -883,7 +884,7 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
     if (__ballot(outside) != 0ull) {
 #pragma unroll
         for (int j = 0; j < PPT; j++) {
-            if (f.px[j] < 0 || box_holds(box, f.px[j], f.py[j])) continue;
+            if (!f.ok[j] || box_holds(box, f.px[j], f.py[j])) continue;
             const size_t pix = (size_t)f.py[j] * p.width + f.px[j];
             const uint32_t d = gload(depth + pix);
             if (d == 0u || !(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
@@ -1031,7 +1032,7 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
     unsigned long long lastKey = 0ull;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        if (f.px[j] < 0) continue;
+        if (!f.ok[j]) continue;
         const uint32_t pi = base + (uint32_t)j;
         // max of depth << 32 | ~index == the two-pass rule: deepest
         // reverse-Z depth, then the lowest point index among equal depths
@@ -1081,9 +1082,8 @@ __device__ __forceinline__ void nearest_frags_tile(const ChunkFrags<PPT>& f, uin
         // reverse-Z depth, then the lowest point index among equal depths
         const unsigned long long key =
             ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~(base + (uint32_t)j));
-        // box.x0 >= 0 and box.w <= the tile: the sentinel px = -1 is never held
-        const bool in = box_holds(box, f.px[j], f.py[j]);
-        outside = outside | ((f.px[j] >= 0) & !in);
+        const bool in = f.ok[j] & box_holds(box, f.px[j], f.py[j]);
+        outside = outside | (f.ok[j] & !in);
         const int k = box.local(f.px[j], f.py[j]);
         const bool fresh = in & !((k == lastK) & (key <= lastKey));
         lastKey = fresh ? key : lastKey;
@@ -1094,7 +1094,7 @@ __device__ __forceinline__ void nearest_frags_tile(const ChunkFrags<PPT>& f, uin
     if (__ballot(outside) != 0ull) {
 #pragma unroll
         for (int j = 0; j < PPT; j++)
-            if (f.px[j] >= 0 && !box_holds(box, f.px[j], f.py[j]))
+            if (f.ok[j] && !box_holds(box, f.px[j], f.py[j]))
                 gmax(packed + (size_t)f.py[j] * p.width + f.px[j],
                      ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~(base + (uint32_t)j)));
     }

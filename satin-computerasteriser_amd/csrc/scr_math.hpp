@@ -58,7 +58,10 @@ __device__ inline uint32_t f2u_hw(float x) {
 __host__ __device__ inline uint32_t depth_to_uint(float z) { return f2u(saturate(z) * kTwo32); }
 // Common.metal:14-16
 __host__ __device__ inline float uint_to_depth(uint32_t d) { return (float)d * kInvTwo32; }
-__device__ inline uint32_t depth_to_uint_hw(float z) { return f2u_hw(saturate(z) * kTwo32); }
+// depth_to_uint on the device: scaling by 2^32 is exact, and the saturating
+// conversion already sends z < 0, -0 and NaN to 0 and z > 1 to 2^32 - 1, so
+// the clamp of depth_to_uint changes no result and is left out.
+__device__ inline uint32_t depth_to_uint_hw(float z) { return f2u_hw(z * kTwo32); }
 
 // Common.metal:190-197 (lowbias32)
 __host__ __device__ inline float hash_unit(uint32_t x) {

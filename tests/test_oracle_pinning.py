@@ -68,6 +68,29 @@ def test_depth_to_uint_closed_forms():
     assert orc.depth_to_uint(float(z)) == int(np.float64(z) * 2 ** 32)
 
 
+def _cvt_u32(y: np.float32) -> int:
+    # float -> u32 under contract C4: truncate, saturate, NaN -> 0
+    if y != y or y <= 0:
+        return 0
+    if y >= np.float32(2.0 ** 32):
+        return 0xFFFFFFFF
+    return int(y)
+
+
+def test_depth_to_uint_needs_no_clamp_before_a_saturating_conversion():
+    # the device form converts z * 2^32 without clamping z to [0, 1] first
+    rng = np.random.default_rng(7)
+    bits = rng.integers(0, 2 ** 32, size=20000, dtype=np.uint64).astype(np.uint32)
+    zs = list(bits.view(np.float32))
+    zs += [np.float32(v) for v in (0.0, -0.0, 1.0, -1.0, 1.0000001, 0.99999994, 2.0 ** -149, -2.0 ** -149,
+                                   2.0 ** -126, 2.0 ** 96, 2.0 ** 127, float("inf"), float("-inf"), float("nan"))]
+    zs += list(rng.random(5000, dtype=np.float32))
+    two32 = np.float32(2.0 ** 32)
+    with np.errstate(over="ignore", invalid="ignore"):
+        for z in zs:
+            assert orc.depth_to_uint(float(z)) == _cvt_u32(np.float32(z) * two32), z
+
+
 def _interleave(x, y, z, bits):
     k = 0
     for i in range(bits):

@@ -159,6 +159,14 @@ int scr_memcpy_d2d(scr_ctx* ctx, void* dst, const void* src, size_t bytes);
  * events on the context stream.  Synchronous; allocates its own buffer. */
 int scr_measure_atomic_rate(scr_ctx* ctx, int add, double* atomicsPerSecond);
 
+/* ---- device self-check (no counterpart in the reference): compares the
+ * three-instruction reciprocal the draw passes use for batches whose clip w
+ * is proven to lie in [2^-125, 2^125] with the correctly rounded 1.0f / w,
+ * bit for bit, on every float in [2^-126, 2^126).  Writes the number of
+ * mismatches and the bit pattern of the smallest mismatching float
+ * (0xffffffff when none).  Synchronous; allocates its own counters. */
+int scr_check_fast_reciprocal(scr_ctx* ctx, uint64_t* mismatches, uint32_t* firstMismatch);
+
 /* ---- point cloud slot pool: ComputeRasteriserPointCloud.swift */
 
 /* init(context:capacity:files:originShift:) — ComputeRasteriserPointCloud.swift:201-217.

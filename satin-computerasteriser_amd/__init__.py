@@ -152,6 +152,7 @@ _SIGNATURES = {
     "scr_memcpy_d2h": (_I, [_VP, _VP, _VP, _SZ]),
     "scr_memcpy_d2d": (_I, [_VP, _VP, _VP, _SZ]),
     "scr_measure_atomic_rate": (_I, [_VP, _I, C.POINTER(C.c_double)]),
+    "scr_check_fast_reciprocal": (_I, [_VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "scr_cloud_create": (_I, [_VP, _I, _I, _VP, _I, C.POINTER(_VP)]),
     "scr_cloud_destroy": (_I, [_VP]),
     "scr_cloud_get_info": (_I, [_VP, C.POINTER(CCloudInfo)]),
@@ -428,6 +429,13 @@ class Context:
         out = C.c_double(0.0)
         self.check(self.lib.scr_measure_atomic_rate(self.handle, 1 if add else 0, C.byref(out)))
         return float(out.value)
+
+    def check_fast_reciprocal(self) -> tuple:
+        """(mismatches, smallest mismatching float's bits) of the draw passes'
+        three-instruction reciprocal against 1.0f / w over [2^-126, 2^126)."""
+        n, first = C.c_uint64(0), C.c_uint32(0)
+        self.check(self.lib.scr_check_fast_reciprocal(self.handle, C.byref(n), C.byref(first)))
+        return int(n.value), int(first.value)
 
     def profile(self, enable: bool = True) -> None:
         """Bracket every kernel launch with HIP events (per-pass timing)."""

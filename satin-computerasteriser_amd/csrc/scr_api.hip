@@ -408,6 +408,28 @@ int scr_measure_atomic_rate(scr_ctx* ctx, int add, double* atomicsPerSecond) {
     return SCR_OK;
 }
 
+int scr_check_fast_reciprocal(scr_ctx* ctx, uint64_t* mismatches, uint32_t* firstMismatch) {
+    if (!ctx || !mismatches || !firstMismatch) return SCR_ERR_INVALID_ARGUMENT;
+    *mismatches = 0;
+    *firstMismatch = 0xffffffffu;
+    unsigned long long* d = nullptr;
+    SCR_HIP(ctx, hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    uint32_t* first = reinterpret_cast<uint32_t*>(d + 1);
+    hipError_t e = scr::launch_fill_u64(ctx->stream, d, 1, 0ull);
+    if (e == hipSuccess) e = scr::launch_fill_u32(ctx->stream, first, 1, 0xffffffffu);
+    if (e == hipSuccess) e = scr::launch_check_fast_rcp(ctx->stream, ctx->numCUs, d, first);
+    unsigned long long h[2] = {0ull, 0ull};
+    if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_error(ctx, e, "fast reciprocal check");
+    *mismatches = (uint64_t)h[0];
+    uint32_t f;
+    memcpy(&f, &h[1], sizeof(f));
+    *firstMismatch = f;
+    return SCR_OK;
+}
+
 // ------------------------------------------------------------------ clouds
 
 int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, const scr_raster_file* files,

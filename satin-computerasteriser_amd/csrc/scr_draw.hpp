@@ -177,11 +177,43 @@ struct BatchCtx {
     float lodThreshold;
     bool keepAll;       // no point of the batch can fail the LOD/dither test
     int rconst;         // footprint radius shared by the batch's points, or -1
+    bool fastRcp;       // every point's clip w lies in [2^-125, 2^125] (rcp_fast is exact)
     uint32_t first, active, batchIndex;   // active: end of the drawn range
     float T[16];        // file.transform (held in registers: the atomics below
                         // would otherwise force reloads through memory)
     const float* W;     // file.world (only read for per-point sizes)
 };
+
+// Whether the float clip w of every point of the batch lies in
+// [2^-125, 2^125], where rcp_fast equals 1.0f / w.  The decoded positions lie
+// in the box [min, min + size] (fma(q, s, min) is monotone in q < 2^bits), w
+// is affine in the position, so its exact value over the box lies between
+// its corner values; mul_point rounds four times, each time by at most 2^-24
+// of a partial sum bounded by A = |m3 x| + |m7 y| + |m11 z| + |m15| (largest
+// at a corner), so the float w is within 2^-22 A of the exact one; the test
+// allows 2^-20 A.  Corner values in double (products of floats are exact),
+// one corner per lane of each 8-lane group.  Called by every lane of the wave.
+__device__ __forceinline__ bool clip_w_in_fast_range(const BatchCtx& bc) {
+    const int corner = (int)(__lane_id() & 7u);
+    const float steps = bc.level == 0 ? 1073741824.0f : (bc.level == 1 ? 1048576.0f : 1024.0f);
+    const double x = (corner & 1) ? (double)(bc.mnx + bc.sx * steps) : (double)bc.mnx;
+    const double y = (corner & 2) ? (double)(bc.mny + bc.sy * steps) : (double)bc.mny;
+    const double z = (corner & 4) ? (double)(bc.mnz + bc.sz * steps) : (double)bc.mnz;
+    const double tx = (double)bc.T[3] * x, ty = (double)bc.T[7] * y, tz = (double)bc.T[11] * z;
+    double w = ((tx + ty) + tz) + (double)bc.T[15];
+    double wmax = w;
+    double a = ((fabs(tx) + fabs(ty)) + fabs(tz)) + fabs((double)bc.T[15]);
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        w = fmin(w, __shfl_xor(w, o));
+        wmax = fmax(wmax, __shfl_xor(wmax, o));
+        a = fmax(a, __shfl_xor(a, o));
+    }
+    const double e = a * 0x1p-20;
+    // NaN or infinite corners fail both comparisons
+    const int ok = (w - e >= 0x1p-125 && wmax + e <= 0x1p125) ? 1 : 0;
+    return __builtin_amdgcn_readfirstlane(ok) != 0;
+}
 
 // `hi` bounds the local point indices drawn (at most the active prefix).
 __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
@@ -207,6 +239,7 @@ __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParam
     for (int i = 0; i < 16; i++) bc.T[i] = gload(f->transform + i);
     bc.W = f->world;
     bc.rconst = batch_const_radius(p, f->world, b);
+    bc.fastRcp = clip_w_in_fast_range(bc);
     return bc;
 }
 
@@ -286,14 +319,17 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
     // Branch-free per point: every test of the reference folds into `ok`
     // (each written as the negation of its reject condition, so NaNs behave
     // as in the Metal source) and the fragment is kept or dropped once.
+    f4 clip[PPT];
+    float wx[PPT], wy[PPT], wz[PPT];
+    bool keep[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const uint32_t pi = base + (uint32_t)j;
-        bool ok = (uint32_t)j < n;
+        keep[j] = (uint32_t)j < n;
         if (!bc.keepAll) {
             const float pointLevel = (float)((r.lv4[j / 4] >> (8 * (j % 4))) & 7u);
             const float dither = (p.lodDither != 0) ? hash_unit(pi) : 0.5f;
-            ok = ok && !(dither >= (bc.lodThreshold - pointLevel) + 0.5f);
+            keep[j] = keep[j] && !(dither >= (bc.lodThreshold - pointLevel) + 0.5f);
         }
         uint32_t qx, qy, qz;
         const uint32_t l = r.lo[j];
@@ -312,13 +348,25 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
             qy = (l >> 10) & 1023u;
             qz = (l >> 20) & 1023u;
         }
-        const float x = fmaf((float)qx, bc.sx, bc.mnx);
-        const float y = fmaf((float)qy, bc.sy, bc.mny);
-        const float z = fmaf((float)qz, bc.sz, bc.mnz);
-        const f4 clip = mul_point(bc.T, x, y, z);
-        ok = ok && !(clip.w <= 0.0f);
-        const float rw = rcp(clip.w);
-        const float nx = clip.x * rw, ny = clip.y * rw, nz = clip.z * rw;
+        wx[j] = fmaf((float)qx, bc.sx, bc.mnx);
+        wy[j] = fmaf((float)qy, bc.sy, bc.mny);
+        wz[j] = fmaf((float)qz, bc.sz, bc.mnz);
+        clip[j] = mul_point(bc.T, wx[j], wy[j], wz[j]);
+        keep[j] = keep[j] && !(clip[j].w <= 0.0f);
+    }
+    // 1 / w, one wave-uniform branch per chunk
+    float rw[PPT];
+    if (bc.fastRcp) {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) rw[j] = rcp_fast(clip[j].w);
+    } else {
+#pragma unroll
+        for (int j = 0; j < PPT; j++) rw[j] = rcp(clip[j].w);
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        bool ok = keep[j];
+        const float nx = clip[j].x * rw[j], ny = clip[j].y * rw[j], nz = clip[j].z * rw[j];
         ok = ok && !(nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f);
         if (kNearestZ) ok = ok && !(nz <= 0.0f || nz > 1.0f);
         else ok = ok && !(nz < 0.0f || nz > 1.0f);
@@ -331,7 +379,7 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         f.px[j] = ix;
         f.py[j] = p.height - 1 - iy;
         f.z[j] = nz;
-        f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, x, y, z));
+        f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, wx[j], wy[j], wz[j]));
     }
 }
 

@@ -145,6 +145,7 @@ void compose_files(const scr_cloud* c, const float* model, const float* vp, cons
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
 hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p);
+hipError_t launch_check_fast_rcp(hipStream_t s, int numCUs, unsigned long long* mismatches, uint32_t* first);
 hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t spots, int add, int iters,
                               uint64_t* atomics);
 hipError_t launch_depth(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, uint32_t* depth);

@@ -28,6 +28,17 @@ __host__ __device__ inline float saturate(float x) { return fminf(fmaxf(x, 0.0f)
 
 __host__ __device__ inline float rcp(float x) { return 1.0f / x; }
 
+// The same correctly rounded 1 / x for 2^-126 <= x < 2^126, in three
+// instructions: the hardware reciprocal (within 1 ulp) refined by one
+// Newton step, e = 1 - x * r exact through the fused multiply-add (the
+// whole range is compared with 1.0f / x, bit for bit, on the device by
+// scr_check_fast_reciprocal).  Callers prove the range first.
+__device__ inline float rcp_fast(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    const float e = fmaf(-x, r, 1.0f);
+    return fmaf(e, r, r);
+}
+
 __host__ __device__ inline uint32_t f2u(float x) {
     if (!(x > 0.0f)) return 0u;
     if (x >= kTwo32) return 0xffffffffu;

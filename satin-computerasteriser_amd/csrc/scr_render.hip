@@ -142,6 +142,31 @@ hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsig
     return hipGetLastError();
 }
 
+// Every float in [2^-126, 2^126): rcp_fast against the correctly rounded
+// reciprocal, bit for bit.
+__global__ __launch_bounds__(256) void k_check_fast_rcp(unsigned long long* __restrict__ mismatches,
+                                                        uint32_t* __restrict__ first) {
+    constexpr uint64_t kLo = 0x00800000u, kHi = 0x7e800000u;
+    uint32_t bad = 0u, lowest = 0xffffffffu;
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = kLo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < kHi; i += step) {
+        const float w = __uint_as_float((uint32_t)i);
+        if (__float_as_uint(rcp_fast(w)) != __float_as_uint(rcp(w))) {
+            bad++;
+            lowest = min(lowest, (uint32_t)i);
+        }
+    }
+    if (bad != 0u) {
+        atomicAdd(mismatches, (unsigned long long)bad);
+        atomicMin(first, lowest);
+    }
+}
+
+hipError_t launch_check_fast_rcp(hipStream_t s, int numCUs, unsigned long long* mismatches, uint32_t* first) {
+    hipLaunchKernelGGL(k_check_fast_rcp, dim3((unsigned)numCUs * 8u), dim3(256), 0, s, mismatches, first);
+    return hipGetLastError();
+}
+
 hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t spots, int add, int iters,
                               uint64_t* atomics) {
     const unsigned grid = (unsigned)numCUs * 16u;

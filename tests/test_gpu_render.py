@@ -26,6 +26,13 @@ def test_measure_atomic_rate(ctx):
         assert np.isfinite(r) and r > 1e9
 
 
+def test_fast_reciprocal_is_correctly_rounded_on_its_range(ctx):
+    # batches whose clip w is proven inside [2^-125, 2^125] take the
+    # three-instruction reciprocal: it must equal 1.0f / w on every float there
+    mismatches, first = ctx.check_fast_reciprocal()
+    assert mismatches == 0, hex(first)
+
+
 @pytest.mark.parametrize("clod", [True, False])
 @pytest.mark.parametrize("dither", [True, False])
 def test_config1_hqs_bit_exact(ctx, cube64, clod, dither):

@@ -21,7 +21,7 @@ namespace draw {
 // Draw workgroups hold NT threads (a multiple of 64) that own PPT points each.
 constexpr int kMaxWaves = 8;        // NT <= 512
 constexpr int kDepthTile = 4096;    // u32 per pixel: 16 KB
-constexpr int kColorTile = 2048;    // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 40 KB
+constexpr int kColorTile = 2016;    // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 39.4 KB, four workgroups per CU
 constexpr int kNearestTile = 2048;  // u64 per pixel: 16 KB
 constexpr int kSpare = 64;          // words past a tile: one per lane, the target of no-op updates
 

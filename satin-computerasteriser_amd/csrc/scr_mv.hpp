@@ -77,7 +77,7 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* 
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__global__ __launch_bounds__(NT, PPT <= 8 ? 3 : 2) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
+__global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
                                                  uint32_t nUnits, int splits, uint32_t* heads) {
     __shared__ uint32_t ldsDepth[kColorTile];
     __shared__ unsigned long long ldsRG[kColorTile];

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_depth(CloudDev c
 }
 
 template <bool kR0>
-__global__ __launch_bounds__(kThreads, 3) void k_color(CloudDev c, PassParams p, const uint32_t* __restrict__ depth,
+__global__ __launch_bounds__(kThreads, kR0 ? 4 : 3) void k_color(CloudDev c, PassParams p, const uint32_t* __restrict__ depth,
                                                     uint32_t* red, uint32_t* green, uint32_t* blue,
                                                     uint32_t* count) {
     __shared__ uint32_t ldsDepth[kColorTile];

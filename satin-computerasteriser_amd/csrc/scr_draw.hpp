@@ -733,7 +733,7 @@ __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams
 // ---------------------------------------------------------------- colour pass body
 // colorPassUpdate (opaque path) — ColorPass/Shaders.metal:30-219, one visible batch.
 struct ColorLds {
-    uint32_t* depth;
+    uint32_t* depth;   // per tile pixel: color_threshold of its depth, as float bits
     unsigned long long* rg;
     unsigned long long* bn;
     int* red;
@@ -749,6 +749,14 @@ __device__ __forceinline__ ColorLds color_lds(uint32_t* depth, unsigned long lon
     L.bn = bn;
     L.red = red;
     return L;
+}
+
+// The colour test accepts a fragment at ndc z when z >= this: the pixel's
+// depth times depthKeep, or +inf (nothing accepted) where no point reached
+// the pixel.  Same value and comparison as `d != 0 && z >= uint_to_depth(d)
+// * depthKeep` for every z, NaN included.
+__device__ __forceinline__ float color_threshold(uint32_t d, const PassParams& p) {
+    return d != 0u ? uint_to_depth(d) * p.depthKeep : __builtin_inff();
 }
 
 // Adds a run's packed sums (r | g << 32, b | count << 32) to LDS tile slot k
@@ -849,9 +857,8 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
                 continue;
             }
             const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
-            const uint32_t d = box.lds ? lds_load(L.depth + k) : gload(depth + k);
-            if (d == 0u) continue;
-            if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+            const float thr = box.lds ? __uint_as_float(lds_load(L.depth + k)) : color_threshold(gload(depth + k), p);
+            if (!(f.z[j] >= thr)) continue;
             if (k == runK) {
                 runRG += rg;
                 runBN += bn;
@@ -872,9 +879,9 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
                 const size_t pix = (size_t)ty * p.width + tx;
                 const bool inTile = box.lds && (!kCheck || box_holds(box, tx, ty));
                 const int k = inTile ? box.local(tx, ty) : 0;
-                const uint32_t d = inTile ? lds_load(L.depth + k) : gload(depth + pix);
-                if (d == 0u) continue;
-                if (!(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
+                const float thr =
+                    inTile ? __uint_as_float(lds_load(L.depth + k)) : color_threshold(gload(depth + pix), p);
+                if (!(f.z[j] >= thr)) continue;
                 if (inTile) {
                     lds_add(L.rg + k, rg);
                     lds_add(L.bn + k, bn);
@@ -906,8 +913,7 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
         const bool in = valid & box_holds(box, f.px[j], f.py[j]);
         outside = outside | (valid & !in);
         const int k = in ? box.local(f.px[j], f.py[j]) : 0;
-        const uint32_t d = lds_load(L.depth + k);
-        const bool pass = in & (d != 0u) & (f.z[j] >= uint_to_depth(d) * p.depthKeep);
+        const bool pass = in & (f.z[j] >= __uint_as_float(lds_load(L.depth + k)));
         if (!pass) continue;
         const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
         const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
@@ -944,21 +950,20 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
     }
 }
 
-// Stages the tile's depths and clears its sums.
+// Stages the tile's acceptance thresholds and clears its sums.
 template <typename G>
 __device__ __forceinline__ float color_stage(const ChunkBox& box, const PassParams& p, const uint32_t* __restrict__ depth,
                                    ColorLds L) {
-    // also returns the smallest acceptance threshold uint_to_depth(d) *
-    // depthKeep of the thread's pixels (+inf for pixels with no depth, where
-    // no fragment is accepted)
+    // stages each pixel's color_threshold; also returns the smallest of the
+    // thread's pixels (+inf where no pixel has a depth)
     const int area = box.w * box.h;
     float tmin = __builtin_inff();
     for (int i = G::rank(); i < area; i += G::kSize) {
-        const uint32_t d = gload(depth + box.pixel(i, p.width));
-        lds_store(L.depth + i, d);
+        const float thr = color_threshold(gload(depth + box.pixel(i, p.width)), p);
+        lds_store(L.depth + i, __float_as_uint(thr));
         lds_store(L.rg + i, 0ull);
         lds_store(L.bn + i, 0ull);
-        if (d != 0u) tmin = fminf(tmin, uint_to_depth(d) * p.depthKeep);
+        tmin = fminf(tmin, thr);
     }
     return tmin;
 }

@@ -48,6 +48,14 @@ extern "C" {
 #define SCR_ERR_UNSUPPORTED (-8)
 #define SCR_ERR_SLOT_RANGE (-9)         /* removeBatches precondition, :462 */
 #define SCR_ERR_NO_DEVICE (-10)
+/* PLYPointCloudLoaderError (PLYPointCloudLoader.swift:4-11) */
+#define SCR_ERR_PLY_INVALID_HEADER (-11)
+#define SCR_ERR_PLY_UNSUPPORTED_FORMAT (-12)
+#define SCR_ERR_PLY_MISSING_VERTEX_ELEMENT (-13)
+#define SCR_ERR_PLY_MISSING_PROPERTY (-14)
+#define SCR_ERR_PLY_UNSUPPORTED_PROPERTY (-15)
+#define SCR_ERR_PLY_TRUNCATED_BODY (-16)
+#define SCR_ERR_PLY_INVALID_ASCII_VALUE (-17)
 
 /* ---- GPU data layout: Pipelines/ComputeRasteriserTypes.h:11-63 (byte-identical) */
 typedef struct scr_raster_batch {
@@ -247,6 +255,20 @@ int scr_pack(scr_cloud* cloud, const float* positions4, const float* colors4, si
  * adds the batches to the resident set.  *outFirstSlot receives the run start. */
 int scr_pack_chunk(scr_cloud* cloud, const float* positions4, const float* colors4, size_t count,
                    const scr_pack_options* opt, int32_t* outFirstSlot);
+
+/* ---- PLY input: PLYPointCloudLoader.parse (PLYPointCloudLoader.swift:38-333)
+ * up to the point where the reference hands its arrays to the CPU pack.
+ * Parses an ASCII or binary little-endian PLY image in host memory into
+ * float4 positions (x, y, z, 0) and float4 colours (r, g, b, 1 — channels the
+ * file lacks stay 1; integer channels divided by their type's maximum, all
+ * clamped to [0, 1]), the arrays scr_pack takes.  With positions4 and colors4
+ * NULL it validates the whole image and returns the vertex count only; with
+ * both set (capacity >= the count, 4 floats per vertex each) it also writes
+ * the vertices.  Errors are the SCR_ERR_PLY_* codes above, with the
+ * reference's message (property name, vertex line) in msg when given.  Host
+ * only: needs no context or device. */
+int scr_ply_parse(const void* data, size_t bytes, size_t capacity, float* positions4, float* colors4,
+                  size_t* vertexCount, char* msg, size_t msgBytes);
 
 /* ---- synthetic scene generator (benchmark and test input; no reference
  * counterpart).  A LiDAR-like scene: terrainFraction of the points on a

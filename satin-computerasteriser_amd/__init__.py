@@ -51,6 +51,13 @@ ERRORS = {
     -8: "SCR_ERR_UNSUPPORTED",
     -9: "SCR_ERR_SLOT_RANGE",
     -10: "SCR_ERR_NO_DEVICE",
+    -11: "SCR_ERR_PLY_INVALID_HEADER",
+    -12: "SCR_ERR_PLY_UNSUPPORTED_FORMAT",
+    -13: "SCR_ERR_PLY_MISSING_VERTEX_ELEMENT",
+    -14: "SCR_ERR_PLY_MISSING_PROPERTY",
+    -15: "SCR_ERR_PLY_UNSUPPORTED_PROPERTY",
+    -16: "SCR_ERR_PLY_TRUNCATED_BODY",
+    -17: "SCR_ERR_PLY_INVALID_ASCII_VALUE",
 }
 
 # RasterBatch — Pipelines/ComputeRasteriserTypes.h:11-33 (64 B)
@@ -153,6 +160,7 @@ _SIGNATURES = {
     "scr_memcpy_d2d": (_I, [_VP, _VP, _VP, _SZ]),
     "scr_measure_atomic_rate": (_I, [_VP, _I, C.POINTER(C.c_double)]),
     "scr_check_fast_reciprocal": (_I, [_VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
+    "scr_ply_parse": (_I, [_VP, _SZ, _SZ, _VP, _VP, C.POINTER(_SZ), C.c_char_p, _SZ]),
     "scr_cloud_create": (_I, [_VP, _I, _I, _VP, _I, C.POINTER(_VP)]),
     "scr_cloud_destroy": (_I, [_VP]),
     "scr_cloud_get_info": (_I, [_VP, C.POINTER(CCloudInfo)]),
@@ -998,6 +1006,54 @@ class GPUPacker:
             self.ctx.check(rc)
             placed.append(int(first.value))
         return placed
+
+
+# ---------------------------------------------------------------- PLY input
+class PLYPointCloudLoaderError(ScrError):
+    """PLYPointCloudLoaderError (PLYPointCloudLoader.swift:4-31): ``code`` is
+    the SCR_ERR_PLY_* value, the message the reference's description."""
+
+
+class PLYPointCloudLoader:
+    """PLYPointCloudLoader (PLYPointCloudLoader.swift:33-54), parsed natively
+    by ``scr_ply_parse``.  ``parse``/``load`` return the loader's intermediate
+    arrays — float32 (n, 4) positions (x, y, z, 0) and colours (r, g, b, 1) —
+    and ``packInto`` hands them to the GPU packer, where the reference calls
+    its CPU pack.  Needs no GPU for parsing."""
+
+    @staticmethod
+    def parse(data) -> tuple:
+        lib = load_library()
+        buf = np.frombuffer(data, np.uint8) if not isinstance(data, np.ndarray) else data.view(np.uint8).ravel()
+        ptr = buf.ctypes.data if buf.size else None
+        n = C.c_size_t(0)
+        msg = C.create_string_buffer(256)
+        rc = lib.scr_ply_parse(ptr, buf.size, 0, None, None, C.byref(n), msg, 256)
+        if rc != 0:
+            raise PLYPointCloudLoaderError(rc, msg.value.decode())
+        pos = np.empty((n.value, 4), np.float32)
+        col = np.empty((n.value, 4), np.float32)
+        rc = lib.scr_ply_parse(ptr, buf.size, n.value, pos.ctypes.data, col.ctypes.data, C.byref(n), msg, 256)
+        if rc != 0:
+            raise PLYPointCloudLoaderError(rc, msg.value.decode())
+        return pos, col
+
+    @staticmethod
+    def load(path) -> tuple:
+        """load(url:) — the file is memory-mapped, as the reference's Data(contentsOf:options: .mappedIfSafe)."""
+        size = Path(path).stat().st_size
+        data = np.memmap(path, np.uint8, mode="r") if size else np.zeros(0, np.uint8)
+        return PLYPointCloudLoader.parse(data)
+
+    @staticmethod
+    def packInto(packer: "GPUPacker", cloud: "ComputeRasteriserPointCloud", source) -> int:
+        """Parses ``source`` (bytes, or a path) and packs it into ``cloud``; returns the point count."""
+        if isinstance(source, (bytes, bytearray, memoryview, np.ndarray)):
+            pos, col = PLYPointCloudLoader.parse(source)
+        else:
+            pos, col = PLYPointCloudLoader.load(source)
+        packer.pack(cloud, pos, col)
+        return pos.shape[0]
 
 
 # ---------------------------------------------------------------- synthetic scenes

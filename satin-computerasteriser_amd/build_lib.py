@@ -60,7 +60,8 @@ def _compile(unit, hipcc: str) -> Path:
 
 def _units():
     units = []
-    for src in sorted(CSRC.glob("*.hip")):
+    # *.hip: kernels and their host code; *.cpp: host-only code (PLY input)
+    for src in sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp"))):
         if src.name == MV_DRAW:
             units += [(src, [f"-DSCR_MV_PHASE={ph}", f"-DSCR_MV_SHAPE={sh}"], f"{src.stem}_{ph}_{sh}")
                       for ph, sh in MV_UNITS]

@@ -164,6 +164,32 @@ def test_gpu_pack_equals_cpu_pack(ctx, case, bits):
     _assert_pack_equal(cloud, ref)
 
 
+def test_ply_input_packs_like_the_cpu_pack(ctx):
+    # PLYPointCloudLoader hands its arrays to the pack: the reference's own
+    # case (LayoutTests.swift:44-67) and a binary file of a jittered cloud
+    ply = ("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\nproperty float z\n"
+           "property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n"
+           "0 0 0 255 0 0\n1 0 0 0 255 0\n0 1 0 0 0 255").encode()
+    p, c = jittered_cloud(6000, 0x91F)
+    rec = np.zeros(p.shape[0], [("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("r", "u1"), ("g", "u1"), ("b", "u1")])
+    rec["x"], rec["y"], rec["z"] = p[:, 0], p[:, 1], p[:, 2]
+    for k, ch in enumerate("rgb"):
+        rec[ch] = np.rint(np.clip(c[:, k], 0, 1) * 255).astype(np.uint8)
+    head = (f"ply\nformat binary_little_endian 1.0\nelement vertex {p.shape[0]}\nproperty float x\n"
+            "property float y\nproperty float z\nproperty uchar r\nproperty uchar g\nproperty uchar b\nend_header\n")
+    binary = head.encode() + rec.tobytes()
+    for data, ppb in ((ply, 2), (binary, 777)):
+        pos, col = scr.PLYPointCloudLoader.parse(data)
+        ref = orc.pack(pos[:, :3], col, pointsPerBatch=ppb)
+        cloud = scr.ComputeRasteriserPointCloud(ctx, 1, ppb)
+        n = scr.PLYPointCloudLoader.packInto(scr.GPUPacker(ctx), cloud, data)
+        assert n == ref.pointCount and cloud.residentBatchCount == ref.batchCount
+        _assert_pack_equal(cloud, ref)
+        if data is ply:
+            rgb = set(int(v) & 0x00FFFFFF for v in cloud.readPool(scr.POOL_COLORS)[:3])
+            assert rgb == {0x000000FF, 0x0000FF00, 0x00FF0000}
+
+
 def test_gpu_pack_generated_scene_both_key_widths(ctx):
     pos, col = scr.gen_scene(ctx, 300_000, on_device=False, seed=0x5C0002)
     for bits in (30, 63):

@@ -219,6 +219,31 @@ int scr_cloud_update_files(scr_cloud* cloud, const float viewProjection[16], con
  * device pointer.  rgba == NULL drops the tint buffer. */
 int scr_cloud_set_tints(scr_cloud* cloud, const float* rgba, size_t nPoints, int onDevice);
 
+/* Device view of a cloud's pool for user kernels — what the TintPass /
+ * DisplacementPass preamble binds for user Metal kernels (TintPass.swift:215-338,
+ * buffers SCR_TINT_BUF_*).  Device pointers; tints is one float4 per pool
+ * point (NULL unless requested); stream is the context's hipStream_t, on which
+ * a user kernel runs in order with renders.  The helpers that read it are in
+ * scr_device.h. */
+typedef struct scr_cloud_view {
+    const scr_raster_batch* batches; /* `slots` entries */
+    const uint32_t* xyzLow;
+    const uint32_t* xyzMed;
+    const uint32_t* xyzHigh;
+    const uint32_t* colors;          /* packed RGBA8 */
+    const uint8_t* levels;           /* LOD level per point */
+    float* tints;                    /* float4 per pool point: TintPass output */
+    uint32_t pointsPerBatch;
+    uint32_t slots;                  /* maxResidentBatches */
+    void* stream;
+} scr_cloud_view;
+/* Fills *out after committing pending batch updates (commitBatchUpdates).
+ * withTints != 0 also makes sure the cloud has a tint buffer (a new one holds
+ * zeros; one installed with scr_cloud_set_tints is returned as is), for a
+ * user kernel to write and a render with applyTint to read (TintPass.swift:
+ * 130-184).  The pointers stay valid until the pool is resized or destroyed. */
+int scr_cloud_device_view(scr_cloud* cloud, int withTints, scr_cloud_view* out);
+
 #define SCR_POOL_XYZ_LOW 0
 #define SCR_POOL_XYZ_MED 1
 #define SCR_POOL_XYZ_HIGH 2

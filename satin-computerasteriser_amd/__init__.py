@@ -115,6 +115,13 @@ class CCloudInfo(C.Structure):
     ]
 
 
+class CCloudView(C.Structure):
+    """scr_cloud_view: device pointers of a cloud's pool for user kernels."""
+    _fields_ = [("batches", C.c_void_p), ("xyzLow", C.c_void_p), ("xyzMed", C.c_void_p), ("xyzHigh", C.c_void_p),
+                ("colors", C.c_void_p), ("levels", C.c_void_p), ("tints", C.c_void_p),
+                ("pointsPerBatch", C.c_uint32), ("slots", C.c_uint32), ("stream", C.c_void_p)]
+
+
 class CPackOptions(C.Structure):
     _fields_ = [("keyBits", C.c_int32), ("lodLevels", C.c_int32), ("coarseVoxelDivisions", C.c_int32),
                 ("inputOnDevice", C.c_int32)]
@@ -171,6 +178,7 @@ _SIGNATURES = {
     "scr_cloud_set_model": (_I, [_VP, _VP]),
     "scr_cloud_update_files": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "scr_cloud_set_tints": (_I, [_VP, _VP, _SZ, _I]),
+    "scr_cloud_device_view": (_I, [_VP, _I, _VP]),
     "scr_cloud_read_pool": (_I, [_VP, _I, _VP, _SZ]),
     "scr_cloud_read_visible": (_I, [_VP, _VP, _I, C.POINTER(_I)]),
     "scr_pack_options_default": (None, [C.POINTER(CPackOptions)]),
@@ -606,6 +614,14 @@ class ComputeRasteriserPointCloud:
 
     def setTintsDevice(self, ptr: int, nPoints: int) -> None:
         self.ctx.check(self.lib.scr_cloud_set_tints(self.handle, ptr, nPoints, 1))
+
+    def deviceView(self, withTints: bool = False) -> CCloudView:
+        """The pool's device pointers and the context stream for a user HIP
+        kernel (scr_device.h) — what TintPass binds for its user kernel
+        (TintPass.swift:215-338); withTints makes sure a tint buffer exists."""
+        v = CCloudView()
+        self.ctx.check(self.lib.scr_cloud_device_view(self.handle, int(bool(withTints)), C.byref(v)))
+        return v
 
     def readPool(self, which: int) -> np.ndarray:
         mrb, ppb = self.capacity

@@ -638,6 +638,30 @@ int scr_cloud_set_tints(scr_cloud* c, const float* rgba, size_t nPoints, int onD
     return SCR_OK;
 }
 
+int scr_cloud_device_view(scr_cloud* c, int withTints, scr_cloud_view* out) {
+    if (!c || !out) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = c->ctx;
+    const int rc = flush_mirror(c);
+    if (rc != SCR_OK) return rc;
+    if (withTints && !c->dTints) {
+        SCR_HIP(ctx, dalloc(c->dTints, c->poolPoints()));
+        c->tintsOwned = true;
+        SCR_HIP(ctx, scr::launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(c->dTints), c->poolPoints() * 4,
+                                          0u));
+    }
+    out->batches = c->dBatches;
+    out->xyzLow = c->dLow;
+    out->xyzMed = c->dMed;
+    out->xyzHigh = c->dHigh;
+    out->colors = c->dColors;
+    out->levels = c->dLevels;
+    out->tints = reinterpret_cast<float*>(c->dTints);
+    out->pointsPerBatch = (uint32_t)c->pointsPerBatch;
+    out->slots = (uint32_t)c->maxResidentBatches;
+    out->stream = ctx->stream;
+    return SCR_OK;
+}
+
 int scr_cloud_read_pool(scr_cloud* c, int which, void* host, size_t bytes) {
     if (!c || !host) return SCR_ERR_INVALID_ARGUMENT;
     const void* src = nullptr;

@@ -340,6 +340,13 @@ int scr_frame_destroy(scr_frame* frame);
 /* Copy one plane (width*height elements of 4 B; 8 B for the DIFF_SUM planes)
  * to host memory. */
 int scr_frame_read(scr_frame* frame, int which, void* host, size_t bytes);
+/* HoleFillProcessor.update(_:iterations:) (HoleFillProcessor.swift:22-45,
+ * HoleFill/Shaders.metal:10-48): `iterations` passes of the 3 x 3
+ * neighbour-average expansion over the frame's RGBA8 plane, in place (through
+ * a scratch plane the frame allocates on first use), enqueued on the context
+ * stream after the frame's last render.  A texel with alpha >= 0.5 is kept;
+ * an empty one takes the mean colour of its valid neighbours with alpha 1. */
+int scr_frame_hole_fill(scr_frame* frame, int iterations);
 
 /* ---- rendering: ComputeRasteriser.update (:201-268) + encode (:277-296).
  * HQS: clear → cull(all clouds) → depth(all) → colour(all) → resolve

@@ -436,3 +436,44 @@ def pick_search(indices, width: int, height: int, ndcX: float, ndcY: float, radi
     out = C.c_uint32()
     found = lib().orc_pick_search(ind.ctypes.data, width, height, ndcX, ndcY, radius, C.byref(out))
     return int(out.value) if found else None
+
+
+# ---------------------------------------------------------------- post
+def hole_fill(rgba8: np.ndarray, iterations: int = 1) -> np.ndarray:
+    """holeFillUpdate (HoleFill/Shaders.metal:10-48), ``iterations`` passes over
+    an (H, W) uint32 RGBA8 image: a texel with alpha byte >= 128 is kept; an
+    empty one takes the mean of its valid 8-neighbours' colours (bytes read as
+    byte * fl(1/255), summed in the shader's row-major neighbour order in
+    binary32, scaled by the correctly rounded 1/count) as rint(saturate(x) *
+    255) bytes with alpha 255, or is kept when none is valid."""
+    img = np.ascontiguousarray(rgba8, np.uint32)
+    h, w = img.shape
+    f32 = np.float32
+    k = f32(1.0) / f32(255.0)
+    for _ in range(iterations):
+        valid = (img >> 24) >= 128
+        s = np.zeros((h, w, 3), f32)
+        n = np.zeros((h, w), f32)
+        pad = np.zeros((h + 2, w + 2), np.uint32)
+        pad[1:-1, 1:-1] = img
+        vpad = np.zeros((h + 2, w + 2), bool)
+        vpad[1:-1, 1:-1] = valid
+        for oy in (-1, 0, 1):
+            for ox in (-1, 0, 1):
+                if ox == 0 and oy == 0:
+                    continue
+                q = pad[1 + oy:1 + oy + h, 1 + ox:1 + ox + w]
+                ok = vpad[1 + oy:1 + oy + h, 1 + ox:1 + ox + w]
+                for ch in range(3):
+                    v = ((q >> (8 * ch)) & 0xFF).astype(f32) * k
+                    s[:, :, ch] = np.where(ok, s[:, :, ch] + v, s[:, :, ch]).astype(f32)
+                n = np.where(ok, n + f32(1), n).astype(f32)
+        # texels without a valid neighbour (n = 0) are kept below; their
+        # values here are discarded
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = (f32(1) / n).astype(f32)
+            b = [np.rint(np.clip((s[:, :, ch] * r).astype(f32), 0, 1).astype(f32) * f32(255)).astype(np.uint32)
+                 for ch in range(3)]
+        filled = b[0] | (b[1] << 8) | (b[2] << 16) | np.uint32(0xFF000000)
+        img = np.where(valid | (n == 0), img, filled).astype(np.uint32)
+    return img

@@ -189,6 +189,7 @@ _SIGNATURES = {
     "scr_scene_height": (C.c_float, [C.POINTER(CSceneParams), C.c_float, C.c_float]),
     "scr_frame_create": (_I, [_VP, _I, _I, C.POINTER(_VP)]),
     "scr_frame_destroy": (_I, [_VP]),
+    "scr_frame_hole_fill": (_I, [_VP, _I]),
     "scr_frame_read": (_I, [_VP, _I, _VP, _SZ]),
     "scr_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _I, _VP]),
     "scr_pick": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, C.c_float, C.c_float, _I,
@@ -664,6 +665,10 @@ class Frame:
         ctx.check(ctx.lib.scr_frame_create(ctx.handle, self.width, self.height, C.byref(h)))
         self.handle = h
         ctx._adopt(self)
+
+    def holeFill(self, iterations: int = 3) -> None:
+        """HoleFillProcessor.update(_:iterations:) over the RGBA8 output, in place."""
+        self.ctx.check(self.ctx.lib.scr_frame_hole_fill(self.handle, int(iterations)))
 
     def read(self, which: int) -> np.ndarray:
         dt = np.float32 if which == PLANE_DEPTH_F32 else (

@@ -882,10 +882,30 @@ int scr_frame_destroy(scr_frame* f) {
     dfree(f->dPlanes);
     dfree(f->dNearest);
     dfree(f->dRGBA);
+    dfree(f->dRGBAScratch);
     dfree(f->dDepthF);
     dfree(f->dSums);
     delete f;
     scr::ctx_release(ctx);
+    return SCR_OK;
+}
+
+int scr_frame_hole_fill(scr_frame* f, int iterations) {
+    if (!f) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = f->ctx;
+    if (iterations < 0) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+    if (iterations == 0) return SCR_OK;
+    if (!f->dRGBAScratch) SCR_HIP(ctx, dalloc(f->dRGBAScratch, f->pixels()));
+    scr::PassScope scope(ctx, "hole_fill");
+    // passes alternate plane -> scratch -> plane; an odd count ends with a copy back
+    uint32_t* src = f->dRGBA;
+    uint32_t* dst = f->dRGBAScratch;
+    for (int k = 0; k < iterations; k++) {
+        SCR_HIP(ctx, scr::launch_hole_fill(ctx->stream, src, dst, f->width, f->height));
+        std::swap(src, dst);
+    }
+    if (src != f->dRGBA)
+        SCR_HIP(ctx, hipMemcpyAsync(f->dRGBA, src, f->pixels() * 4, hipMemcpyDeviceToDevice, ctx->stream));
     return SCR_OK;
 }
 

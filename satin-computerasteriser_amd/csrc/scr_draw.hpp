@@ -1276,6 +1276,32 @@ __device__ __forceinline__ void resolve_pixel(const PassParams& p, const uint32_
     depthf[i] = uint_to_depth(planes[i]);
 }
 
+// holeFillUpdate — HoleFill/Shaders.metal:10-48, one pixel of an RGBA8
+// plane: a texel with alpha >= 0.5 (byte >= 128) passes through; an empty one
+// takes the mean of its valid 8-neighbours' colours (each byte read as
+// byte * (1/255), summed in the shader's row-major order, divided as
+// sum * rcp(count)) with alpha 1, or stays as it is when none is valid.
+__device__ __forceinline__ uint32_t hole_fill_pixel(const uint32_t* __restrict__ in, int w, int h, int x, int y) {
+    const uint32_t c = in[(size_t)y * w + x];
+    if ((c >> 24) >= 128u) return c;
+    float sr = 0.0f, sg = 0.0f, sb = 0.0f, n = 0.0f;
+    for (int oy = -1; oy <= 1; oy++)
+        for (int ox = -1; ox <= 1; ox++) {
+            if (ox == 0 && oy == 0) continue;
+            const int sx = x + ox, sy = y + oy;
+            if (sx < 0 || sx >= w || sy < 0 || sy >= h) continue;
+            const uint32_t q = in[(size_t)sy * w + sx];
+            if ((q >> 24) < 128u) continue;
+            sr += (float)(q & 0xffu) * kInv255;
+            sg += (float)((q >> 8) & 0xffu) * kInv255;
+            sb += (float)((q >> 16) & 0xffu) * kInv255;
+            n += 1.0f;
+        }
+    if (!(n > 0.0f)) return c;
+    const float r = rcp(n);
+    return pack_rgba(sr * r, sg * r, sb * r, 1.0f);
+}
+
 // nearestResolveUpdate — NearestResolve/Shaders.metal:8-38
 __device__ __forceinline__ void nearest_resolve_pixel(const PassParams& p, const unsigned long long* __restrict__ packed,
                                              const uint32_t* __restrict__ colors, size_t i, uint32_t* __restrict__ rgba,

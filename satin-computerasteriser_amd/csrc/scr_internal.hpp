@@ -105,6 +105,7 @@ struct scr_frame {
     unsigned long long* dNearest = nullptr;  // depth << 32 | ~index per pixel
     unsigned long long* dSums = nullptr;     // differentiable mode: 3 int64 planes of planeStride (on first use)
     uint32_t* dRGBA = nullptr;
+    uint32_t* dRGBAScratch = nullptr;  // hole fill ping-pong plane (on first use)
     float* dDepthF = nullptr;
     size_t pixels() const { return (size_t)width * (size_t)height; }
     // planes start on 256-byte boundaries so vector stores stay aligned
@@ -145,6 +146,7 @@ void compose_files(const scr_cloud* c, const float* model, const float* vp, cons
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
 hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p);
+hipError_t launch_hole_fill(hipStream_t s, const uint32_t* in, uint32_t* out, int w, int h);
 hipError_t launch_check_fast_rcp(hipStream_t s, int numCUs, unsigned long long* mismatches, uint32_t* first);
 hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t spots, int add, int iters,
                               uint64_t* atomics);

@@ -142,6 +142,22 @@ hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsig
     return hipGetLastError();
 }
 
+// One hole-fill pass over a width x height RGBA8 plane, 16 x 16 pixels per
+// workgroup (HoleFillProcessor.swift:37-44).
+__global__ __launch_bounds__(256) void k_hole_fill(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                   int w, int h) {
+    const int x = (int)(blockIdx.x * 16u + (threadIdx.x & 15u));
+    const int y = (int)(blockIdx.y * 16u + (threadIdx.x >> 4));
+    if (x >= w || y >= h) return;
+    out[(size_t)y * w + x] = draw::hole_fill_pixel(in, w, h, x, y);
+}
+
+hipError_t launch_hole_fill(hipStream_t s, const uint32_t* in, uint32_t* out, int w, int h) {
+    const dim3 grid((unsigned)(w + 15) / 16u, (unsigned)(h + 15) / 16u);
+    hipLaunchKernelGGL(k_hole_fill, grid, dim3(256), 0, s, in, out, w, h);
+    return hipGetLastError();
+}
+
 // Every float in [2^-126, 2^126): rcp_fast against the correctly rounded
 // reciprocal, bit for bit.
 __global__ __launch_bounds__(256) void k_check_fast_rcp(unsigned long long* __restrict__ mismatches,

@@ -284,3 +284,27 @@ def test_empty_and_all_culled_frames(ctx, cube64):
     assert len(cloud.readVisible()) == 0
     assert out["count"].sum() == 0
     assert np.all(out["rgba8"] == bg["rgba8"][0, 0])
+
+
+@pytest.mark.parametrize("iterations", [1, 2, 3])
+def test_hole_fill_on_a_sparse_frame(ctx, iterations):
+    # a sparse cloud leaves holes between its 1-px points; the hole fill of
+    # the frame's RGBA8 output must equal the oracle's on the same image
+    pos, col = jittered_cloud(2500, 0x401E)
+    packed = orc.pack(pos, col, pointsPerBatch=500)
+    cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    ras = scr.ComputeRasteriser(ctx, scr.ComputeRasteriserConfiguration())
+    ras.addPointCloud(gpu_cloud(ctx, packed))
+    ras.resize(160, 120)
+    ras.render(cam)
+    before = ras.read(scr.PLANE_RGBA8)
+    holes = (before >> 24) < 128
+    assert 0.2 < holes.mean() < 0.98
+    ras.frame.holeFill(iterations)
+    after = ras.read(scr.PLANE_RGBA8)
+    want = orc.hole_fill(before, iterations)
+    assert (want != before).sum() > 100
+    bad = np.argwhere(after != want)
+    assert bad.size == 0, (bad[:4], after[tuple(bad[0])], want[tuple(bad[0])])
+    ras.frame.holeFill(0)
+    assert np.array_equal(ras.read(scr.PLANE_RGBA8), after)

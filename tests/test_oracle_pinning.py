@@ -265,3 +265,59 @@ def test_decode_point_is_fused_and_the_swift_reference_is_not():
             assert ref[a] == np.float32(np.float32(np.float32(q) * s) + mn[a])
             differ += int(got[a] != ref[a])
     assert differ > 100
+
+
+def _hole_fill_loops(img, iterations):
+    """holeFillUpdate restated with scalar binary32 loops (tiny images only)."""
+    f32 = np.float32
+    k = f32(1) / f32(255)
+    h, w = img.shape
+    cur = img.copy()
+    for _ in range(iterations):
+        nxt = cur.copy()
+        for y in range(h):
+            for x in range(w):
+                c = int(cur[y, x])
+                if (c >> 24) >= 128:
+                    continue
+                s = [f32(0), f32(0), f32(0)]
+                n = f32(0)
+                for oy in (-1, 0, 1):
+                    for ox in (-1, 0, 1):
+                        if (ox, oy) == (0, 0) or not (0 <= x + ox < w and 0 <= y + oy < h):
+                            continue
+                        q = int(cur[y + oy, x + ox])
+                        if (q >> 24) < 128:
+                            continue
+                        for ch in range(3):
+                            s[ch] = f32(s[ch] + f32((q >> (8 * ch)) & 0xFF) * k)
+                        n = f32(n + f32(1))
+                if n > 0:
+                    r = f32(f32(1) / n)
+                    b = [int(np.rint(f32(min(max(f32(s[ch] * r), f32(0)), f32(1)) * f32(255)))) for ch in range(3)]
+                    nxt[y, x] = b[0] | (b[1] << 8) | (b[2] << 16) | 0xFF000000
+        cur = nxt
+    return cur
+
+
+def test_hole_fill_closed_forms_and_loops():
+    # an empty texel ringed by one colour takes that colour exactly; a texel
+    # with one valid neighbour copies it; one with none stays; valid texels
+    # (alpha byte >= 128, i.e. alpha >= 0.5) never change
+    img = np.zeros((5, 5), np.uint32)
+    img[1:4, 1:4] = 0xFF336699
+    img[2, 2] = 0x00000000
+    out = orc.hole_fill(img, 1)
+    assert out[2, 2] == 0xFF336699 and out[1, 1] == 0xFF336699
+    img = np.zeros((3, 4), np.uint32)
+    img[0, 0] = 0x80FF0010      # alpha 128: valid
+    img[2, 3] = 0x7F00FF00      # alpha 127: empty
+    out = orc.hole_fill(img, 1)
+    assert out[0, 0] == 0x80FF0010 and out[1, 1] == 0xFFFF0010 and out[2, 3] == 0x7F00FF00
+    assert out[2, 0] == 0 and out[0, 3] == 0
+    rng = np.random.default_rng(3)
+    for shape in ((1, 1), (1, 7), (6, 5), (9, 9)):
+        img = rng.integers(0, 2 ** 32, shape, dtype=np.uint64).astype(np.uint32)
+        img[rng.random(shape) < 0.6] &= 0x00FFFFFF          # mostly holes
+        for it in (1, 2, 3):
+            assert np.array_equal(orc.hole_fill(img, it), _hole_fill_loops(img, it)), (shape, it)

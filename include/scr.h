@@ -111,7 +111,10 @@ typedef struct scr_config {
     int32_t applyTint;
     int32_t tintAlphaIsCoverage; /* translucent-defocus OIT: not supported, must be 0 */
     float motionBlur;            /* motion blur: not supported, must be 0 */
-    int32_t reserved[5];
+    int32_t holeFillIterations;  /* hole-fill passes after the resolve (<= 0: none) */
+    int32_t motionBlurSamples;
+    float motionBlurMaxSpread;
+    int32_t reserved[2];
 } scr_config;
 
 /* Reference defaults, ComputeRasteriserTypes.swift:97-119. */

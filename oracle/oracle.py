@@ -313,7 +313,8 @@ class Config(C.Structure):
         ("enableLODDither", C.c_int32), ("colorizeChunks", C.c_int32), ("colorizeOverdraw", C.c_int32),
         ("pointSizeMode", C.c_int32), ("minimumPointSize", C.c_float), ("maximumPointSize", C.c_float),
         ("pointSizeScale", C.c_float), ("applyTint", C.c_int32), ("tintAlphaIsCoverage", C.c_int32),
-        ("motionBlur", C.c_float), ("reserved", C.c_int32 * 5),
+        ("motionBlur", C.c_float), ("holeFillIterations", C.c_int32), ("motionBlurSamples", C.c_int32),
+        ("motionBlurMaxSpread", C.c_float), ("reserved", C.c_int32 * 2),
     ]
 
 

@@ -100,7 +100,8 @@ class CConfig(C.Structure):
         ("enableLODDither", C.c_int32), ("colorizeChunks", C.c_int32), ("colorizeOverdraw", C.c_int32),
         ("pointSizeMode", C.c_int32), ("minimumPointSize", C.c_float), ("maximumPointSize", C.c_float),
         ("pointSizeScale", C.c_float), ("applyTint", C.c_int32), ("tintAlphaIsCoverage", C.c_int32),
-        ("motionBlur", C.c_float), ("reserved", C.c_int32 * 5),
+        ("motionBlur", C.c_float), ("holeFillIterations", C.c_int32), ("motionBlurSamples", C.c_int32),
+        ("motionBlurMaxSpread", C.c_float), ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -324,6 +325,9 @@ class ComputeRasteriserConfiguration:
     applyTint: bool = False
     tintAlphaIsCoverage: bool = False
     motionBlur: float = 0.0
+    holeFillIterations: int = 0
+    motionBlurSamples: int = 8
+    motionBlurMaxSpread: float = 64.0
 
     def to_c(self) -> CConfig:
         c = CConfig()
@@ -344,6 +348,9 @@ class ComputeRasteriserConfiguration:
         c.applyTint = int(bool(self.applyTint))
         c.tintAlphaIsCoverage = int(bool(self.tintAlphaIsCoverage))
         c.motionBlur = float(self.motionBlur)
+        c.holeFillIterations = int(self.holeFillIterations)
+        c.motionBlurSamples = int(self.motionBlurSamples)
+        c.motionBlurMaxSpread = float(self.motionBlurMaxSpread)
         return c
 
 

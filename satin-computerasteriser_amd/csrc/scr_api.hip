@@ -272,6 +272,9 @@ void scr_config_default(scr_config* c) {
     c->minimumPointSize = 1.0f;
     c->maximumPointSize = 1.0f;
     c->pointSizeScale = 1.0f;
+    c->holeFillIterations = 0;
+    c->motionBlurSamples = 8;
+    c->motionBlurMaxSpread = 64.0f;
 }
 
 // ------------------------------------------------------------------ context
@@ -976,9 +979,12 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
             }
             colors = c->dColors;
         }
-        scr::PassScope ps(ctx, "nearest_resolve");
-        SCR_HIP(ctx, scr::launch_nearest_resolve(s, p, frame->dNearest, colors, frame->dRGBA, frame->dDepthF));
-        return SCR_OK;
+        {
+            scr::PassScope ps(ctx, "nearest_resolve");
+            SCR_HIP(ctx, scr::launch_nearest_resolve(s, p, frame->dNearest, colors, frame->dRGBA, frame->dDepthF));
+        }
+        // encodeHoleFill (ComputeRasteriser.swift:298-320), after either mode
+        return cfg->holeFillIterations > 0 ? scr_frame_hole_fill(frame, cfg->holeFillIterations) : SCR_OK;
     }
 
     // encodeHighQualityAverage (:322-456)
@@ -1006,9 +1012,11 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
         SCR_HIP(ctx, scr::launch_color(s, ctx->numCUs, c->dev(), p, frame->plane(0), frame->plane(1), frame->plane(2),
                                        frame->plane(3), frame->plane(4)));
     }
-    scr::PassScope ps(ctx, "resolve");
-    SCR_HIP(ctx, scr::launch_resolve(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA, frame->dDepthF));
-    return SCR_OK;
+    {
+        scr::PassScope ps(ctx, "resolve");
+        SCR_HIP(ctx, scr::launch_resolve(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA, frame->dDepthF));
+    }
+    return cfg->holeFillIterations > 0 ? scr_frame_hole_fill(frame, cfg->holeFillIterations) : SCR_OK;
 }
 
 // ------------------------------------------------------------------ differentiable HQS

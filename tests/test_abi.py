@@ -44,3 +44,5 @@ def test_config_defaults_match_reference():
     # ComputeRasteriserTypes.swift:97-119
     assert abs(c.depthTolerance - 0.01) < 1e-9 and c.enableFrustumCulling == 1 and c.enableCLOD == 1
     assert c.minimumPointSize == 1.0 and c.maximumPointSize == 1.0 and c.pointSizeScale == 1.0
+    assert c.holeFillIterations == 0 and c.motionBlurSamples == 8 and c.motionBlurMaxSpread == 64.0
+    assert ctypes.sizeof(scr.CConfig) == 96

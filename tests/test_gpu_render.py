@@ -308,3 +308,14 @@ def test_hole_fill_on_a_sparse_frame(ctx, iterations):
     assert bad.size == 0, (bad[:4], after[tuple(bad[0])], want[tuple(bad[0])])
     ras.frame.holeFill(0)
     assert np.array_equal(ras.read(scr.PLANE_RGBA8), after)
+    # configuration.holeFillIterations runs the same passes inside the frame's
+    # encode (ComputeRasteriser.swift:298-320), in both modes
+    ras.configuration = scr.ComputeRasteriserConfiguration(holeFillIterations=iterations)
+    ras.render(cam)
+    assert np.array_equal(ras.read(scr.PLANE_RGBA8), want)
+    ras.configuration = scr.ComputeRasteriserConfiguration(mode=scr.MODE_NEAREST_POINT)
+    ras.render(cam)
+    near = ras.read(scr.PLANE_RGBA8)
+    ras.configuration.holeFillIterations = iterations
+    ras.render(cam)
+    assert np.array_equal(ras.read(scr.PLANE_RGBA8), orc.hole_fill(near, iterations))

@@ -109,8 +109,8 @@ typedef struct scr_config {
     float maximumPointSize;
     float pointSizeScale;
     int32_t applyTint;
-    int32_t tintAlphaIsCoverage; /* translucent-defocus OIT: not supported, must be 0 */
-    float motionBlur;            /* motion blur: not supported, must be 0 */
+    int32_t tintAlphaIsCoverage; /* translucent defocus (with applyTint): weighted-blend passes */
+    float motionBlur;            /* > 0: splats swept along their screen velocity, weighted blend */
     int32_t holeFillIterations;  /* hole-fill passes after the resolve (<= 0: none) */
     int32_t motionBlurSamples;
     float motionBlurMaxSpread;

@@ -72,7 +72,7 @@ def lib() -> C.CDLL:
         u32 = C.c_uint32
         vp = C.c_void_p
         sig = {
-            "orc_log2f": (f, [f]), "orc_hash_unit": (f, [u32]), "orc_depth_to_uint": (u32, [f]),
+            "orc_log2f": (f, [f]), "orc_expf": (f, [f]), "orc_hash_unit": (f, [u32]), "orc_depth_to_uint": (u32, [f]),
             "orc_uint_to_depth": (f, [u32]), "orc_f2u": (u32, [f]), "orc_f2i": (C.c_int32, [f]),
             "orc_mat4_mul": (None, [vp, vp, vp]), "orc_mat4_vec": (None, [vp, vp, vp]),
             "orc_transform_point": (None, [vp, vp, vp]),
@@ -160,6 +160,11 @@ def decode_point(batch: np.ndarray, words, level: int, fused: bool = True) -> np
     lib().orc_decode_point(0, b.ctypes.data, w[0].ctypes.data, w[1].ctypes.data, w[2].ctypes.data, level,
                            1 if fused else 0, out.ctypes.data)
     return out
+
+
+def expf(x: float) -> float:
+    """Contract C8 e^x (the coverage resolve's exponential)."""
+    return lib().orc_expf(x)
 
 
 def lod_threshold(px: float, bias: int) -> float:
@@ -286,11 +291,13 @@ class Cloud:
             batches[i] = b
         return Cloud(batches, packed.files, low, med, high, colors, levels)
 
-    def composed(self, vp) -> "Cloud":
-        """Apply updateFiles(viewProjection, model) to a copy."""
+    def composed(self, vp, prev_vp=None) -> "Cloud":
+        """Apply updateFiles(viewProjection, model, prevViewProjection) to a copy."""
         files = np.zeros_like(self.files)
+        prev = None if prev_vp is None else _f32(prev_vp)
         lib().orc_update_files(self.files.ctypes.data, len(self.files), _f32(vp).ctypes.data,
-                               _f32(self.model).ctypes.data, None, None, files.ctypes.data)
+                               _f32(self.model).ctypes.data, None, None if prev is None else prev.ctypes.data,
+                               files.ctypes.data)
         c = Cloud(self.batches, files, self.xyzLow, self.xyzMed, self.xyzHigh, self.colors, self.levels, self.tints)
         c.model = self.model
         return c
@@ -344,10 +351,12 @@ def cull(cloud: Cloud, cfg, view, proj, width: int, height: int) -> np.ndarray:
     return out[:n]
 
 
-def render_hqs(clouds: Sequence[Cloud], cfg, view, proj, width: int, height: int, threads: int = 1) -> dict:
-    """HQS frame: returns dict of planes (H, W)."""
+def render_hqs(clouds: Sequence[Cloud], cfg, view, proj, width: int, height: int, threads: int = 1,
+               prev_view_proj=None) -> dict:
+    """HQS frame: returns dict of planes (H, W).  prev_view_proj is the
+    previous frame's projection . view (motion blur; default: this frame's)."""
     vp = mat_mul(proj, view)
-    composed = [c.composed(vp) for c in clouds]
+    composed = [c.composed(vp, prev_view_proj) for c in clouds]
     structs = (OrcCloud * max(1, len(composed)))(*[c.c_struct() for c in composed])
     n = width * height
     planes = {k: np.zeros(n, np.uint32) for k in ("depth", "red", "green", "blue", "count", "rgba8")}

@@ -43,6 +43,23 @@ static float orc_rcp(float x) { return 1.0f / x; } /* C2: correctly rounded */
  * x = 2^e * m, m in [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m-1)/(m+1),
  * atanh series to s^21; result = e + ln(m) * log2(e).  Only correctly rounded
  * double +,-,*,/ are used, so any IEEE machine gives the same bits. */
+/* Contract C8: e^x for the coverage resolve, in double with + - * / and
+ * exact operations only (rint, ldexp), rounded to binary32 once: k = rint(x /
+ * ln2) taken as x * (1/ln2); r = (x - k ln2_hi) - k ln2_lo with fdlibm's
+ * split of ln2; e^r as 1 + (r/1)(1 + (r/2)(1 + ... (1 + r/13))) evaluated
+ * from n = 13 down; times 2^k. */
+float orc_expf(float xf) {
+    if (xf != xf) return xf;
+    if (xf > 100.0f) return INFINITY;
+    if (xf < -110.0f) return 0.0f;
+    const double x = (double)xf;
+    const double k = rint(x * 1.4426950408889634);
+    const double r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+    double e = 1.0;
+    for (int n = 13; n >= 1; n--) e = 1.0 + (r / (double)n) * e;
+    return (float)ldexp(e, (int)k);
+}
+
 float orc_log2f(float x) {
     union { float f; uint32_t u; } v;
     v.f = x;
@@ -453,6 +470,11 @@ static void orc_depth_items(const orc_job* j) {
         const scr_raster_file* f = &c->files[b->fileIndex];
         for (uint32_t li = 0; li < vb->activePoints; li++) {
             const uint32_t pi = b->firstPoint + li;
+            /* translucent defocus: the discard sentinel claims no depth
+             * (DepthPass/Shaders.metal:60-67) */
+            if (j->cfg->applyTint != 0 && j->cfg->tintAlphaIsCoverage != 0 && c->tints &&
+                c->tints[(size_t)pi * 4 + 3] < 0.0f)
+                continue;
             orc_frag fr;
             if (!orc_front(c, b, f, vb, pi, j->cfg, j->view, j->proj, j->width, j->height, 0, 1, &fr))
                 continue;
@@ -465,6 +487,66 @@ static void orc_depth_items(const orc_job* j) {
                     if (fr.depth > *d) *d = fr.depth;
                 }
         }
+    }
+}
+
+/* colorPassUpdate, weighted-blend path (ColorPass/Shaders.metal:133-209):
+ * no depth test; the fragment adds colour * a * 4096 and a * 4096 to every
+ * footprint pixel of each motion-blur sample, the splat swept from its pixel
+ * back toward its previous frame's pixel (file.prevTransform).  Integer pixel
+ * differences wrap as the shader's int2 arithmetic does. */
+static void orc_blend_fragment(const orc_job* j, const scr_raster_file* f, const orc_frag* fr, uint32_t r,
+                               uint32_t g, uint32_t bl, float a) {
+    if (a <= 0.0f) return;
+    const int W = j->width, H = j->height;
+    int samples = 1;
+    float stepX = 0.0f, stepY = 0.0f;
+    if (j->cfg->motionBlur > 0.0f) {
+        float prev[4];
+        orc_transform_point(f->prevTransform, fr->point, prev);
+        if (prev[3] > 0.0f) {
+            const float prw = orc_rcp(prev[3]);
+            const int ppx = orc_f2i(fmaf(prev[0] * prw, 0.5f, 0.5f) * (float)W);
+            const int ppy = H - 1 - orc_f2i(fmaf(prev[1] * prw, 0.5f, 0.5f) * (float)H);
+            float vx = (float)(int32_t)((uint32_t)fr->px - (uint32_t)ppx) * j->cfg->motionBlur;
+            float vy = (float)(int32_t)((uint32_t)fr->py - (uint32_t)ppy) * j->cfg->motionBlur;
+            float len = sqrtf(vx * vx + vy * vy);
+            if (len > j->cfg->motionBlurMaxSpread) {
+                const float k = j->cfg->motionBlurMaxSpread * orc_rcp(len);
+                vx = vx * k;
+                vy = vy * k;
+                len = j->cfg->motionBlurMaxSpread;
+            }
+            if (len > 0.75f) {
+                int n = orc_f2i(ceilf(len));
+                if (n < 1) n = 1;
+                if (n > j->cfg->motionBlurSamples) n = j->cfg->motionBlurSamples;
+                samples = n;
+                const float d = orc_rcp((float)(n - 1 > 1 ? n - 1 : 1));
+                stepX = vx * d;
+                stepY = vy * d;
+            }
+        }
+    }
+    const float aN = a * orc_rcp((float)samples);
+    const uint32_t cr = orc_f2u((((float)r * ORC_INV255) * aN) * 4096.0f + 0.5f);
+    const uint32_t cg = orc_f2u((((float)g * ORC_INV255) * aN) * 4096.0f + 0.5f);
+    const uint32_t cb = orc_f2u((((float)bl * ORC_INV255) * aN) * 4096.0f + 0.5f);
+    const uint32_t ca = orc_f2u(aN * 4096.0f + 0.5f);
+    for (int s = 0; s < samples; s++) {
+        const int cx = (int32_t)((uint32_t)fr->px - (uint32_t)orc_f2i(roundf(stepX * (float)s)));
+        const int cy = (int32_t)((uint32_t)fr->py - (uint32_t)orc_f2i(roundf(stepY * (float)s)));
+        for (int oy = -fr->radius; oy <= fr->radius; oy++)
+            for (int ox = -fr->radius; ox <= fr->radius; ox++) {
+                if (!orc_inside_footprint(ox, oy, fr->radius)) continue;
+                const int tx = (int32_t)((uint32_t)cx + (uint32_t)ox), ty = (int32_t)((uint32_t)cy + (uint32_t)oy);
+                if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
+                const size_t p = (size_t)ty * W + tx;
+                j->r[p] += cr;
+                j->g[p] += cg;
+                j->b[p] += cb;
+                j->n[p] += ca;
+            }
     }
 }
 
@@ -487,12 +569,19 @@ static void orc_color_items(const orc_job* j) {
             if (j->cfg->colorizeChunks != 0) color = vb->batchIndex * 1234567u;
             else if (j->cfg->colorizeOverdraw != 0) color = 0x00010101u;
             uint32_t r = color & 0xffu, g = (color >> 8) & 0xffu, bl = (color >> 16) & 0xffu;
+            const int coverageMode = j->cfg->applyTint != 0 && j->cfg->tintAlphaIsCoverage != 0;
+            const int blend = coverageMode || j->cfg->motionBlur > 0.0f;
+            float cover = 1.0f;
             if (j->cfg->applyTint != 0) {
                 /* clouds without a tint buffer read a zeroed stand-in
                  * (ComputeRasteriser.swift:333-342) */
                 static const float zero4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
                 const float* t = c->tints ? c->tints + (size_t)pi * 4 : zero4;
                 if (t[3] < 0.0f) continue;
+                if (coverageMode) {
+                    cover = orc_saturate(1.0f - t[3]);
+                    goto blended;
+                }
                 const float w = orc_saturate(t[3]);
                 const float o[3] = {(float)r * ORC_INV255, (float)g * ORC_INV255, (float)bl * ORC_INV255};
                 float m[3];
@@ -500,6 +589,11 @@ static void orc_color_items(const orc_job* j) {
                 r = orc_f2u(orc_saturate(m[0]) * 255.0f + 0.5f);
                 g = orc_f2u(orc_saturate(m[1]) * 255.0f + 0.5f);
                 bl = orc_f2u(orc_saturate(m[2]) * 255.0f + 0.5f);
+            }
+        blended:
+            if (blend) {
+                orc_blend_fragment(j, f, &fr, r, g, bl, cover);
+                continue;
             }
             for (int oy = -fr.radius; oy <= fr.radius; oy++)
                 for (int ox = -fr.radius; ox <= fr.radius; ox++) {
@@ -622,6 +716,27 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
     free(items);
     if (!ok) return -1;
 
+    /* resolveUpdate, coverage path (Resolve/Shaders.metal:27-45) */
+    if ((cfg->applyTint != 0 && cfg->tintAlphaIsCoverage != 0) || cfg->motionBlur > 0.0f) {
+        for (size_t p = 0; p < npx; p++) {
+            const uint32_t a = out->count[p];
+            if (a == 0u) {
+                if (out->rgba8)
+                    out->rgba8[p] = orc_pack_rgba(cfg->backgroundColor[0], cfg->backgroundColor[1],
+                                                  cfg->backgroundColor[2], 0.0f);
+                if (out->depthF32) out->depthF32[p] = 0.0f;
+                continue;
+            }
+            const float inv = orc_rcp((float)a);
+            const float sumA = (float)a * (1.0f / 4096.0f);
+            const float alpha = orc_saturate(1.0f - orc_expf(-sumA));
+            if (out->rgba8)
+                out->rgba8[p] = orc_pack_rgba((float)out->red[p] * inv, (float)out->green[p] * inv,
+                                              (float)out->blue[p] * inv, alpha);
+            if (out->depthF32) out->depthF32[p] = orc_uint_to_depth(out->depth[p]);
+        }
+        return 0;
+    }
     /* resolveUpdate (count-average path) — Resolve/Shaders.metal:13-57 */
     for (size_t p = 0; p < npx; p++) {
         const uint32_t cnt = out->count[p];

@@ -41,6 +41,7 @@ extern "C" {
 
 /* ---------------- arithmetic helpers (exported for pinning tests) */
 float orc_log2f(float x);
+float orc_expf(float x);
 float orc_hash_unit(uint32_t x);
 uint32_t orc_depth_to_uint(float ndcZ);
 float orc_uint_to_depth(uint32_t d);

@@ -105,13 +105,14 @@ void compose_files(const scr_cloud* c, const float* model, const float* vp, cons
     }
 }
 
-int check_config(scr_ctx* ctx, const scr_config* cfg) {
+int check_config(scr_ctx* ctx, const scr_config* cfg, bool allowCoverage) {
     if (!cfg) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "config is NULL");
     if (cfg->mode != SCR_MODE_HIGH_QUALITY_AVERAGE && cfg->mode != SCR_MODE_NEAREST_POINT)
         return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown render mode");
-    if (cfg->tintAlphaIsCoverage != 0 && cfg->applyTint != 0)
-        return set_error(ctx, SCR_ERR_UNSUPPORTED, "translucent-defocus (tintAlphaIsCoverage) is not supported");
-    if (cfg->motionBlur > 0.0f) return set_error(ctx, SCR_ERR_UNSUPPORTED, "motion blur is not supported");
+    if (!allowCoverage && cfg->tintAlphaIsCoverage != 0 && cfg->applyTint != 0)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "translucent defocus (tintAlphaIsCoverage) needs scr_render");
+    if (!allowCoverage && cfg->motionBlur > 0.0f)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "motion blur needs scr_render");
     if (cfg->pointSizeMode != 0 && cfg->pointSizeMode != 1)
         return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown point size mode");
     return SCR_OK;
@@ -946,7 +947,7 @@ int scr_frame_read(scr_frame* f, int which, void* host, size_t bytes) {
 int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* const* clouds, int nClouds,
                scr_frame* frame) {
     if (!ctx || !cam || !frame || nClouds < 0 || (nClouds > 0 && !clouds)) return SCR_ERR_INVALID_ARGUMENT;
-    int rc = check_config(ctx, cfg);
+    int rc = check_config(ctx, cfg, true);
     if (rc != SCR_OK) return rc;
     for (int i = 0; i < nClouds; i++)
         if (!clouds[i] || clouds[i]->ctx != ctx) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "bad cloud handle");
@@ -954,6 +955,12 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
     float vp[16];
     scr::mat_mul(cam->projectionMatrix, cam->viewMatrix, vp);
     hipStream_t s = ctx->stream;
+    // the previous frame's view-projection of this viewport, for motion-blur
+    // velocity (ComputeRasteriser.swift:255-266: the current one the first time)
+    float prevVP[16];
+    std::memcpy(prevVP, frame->hasPrevVP ? frame->prevVP : vp, sizeof prevVP);
+    std::memcpy(frame->prevVP, vp, sizeof vp);
+    frame->hasPrevVP = true;
 
     if (cfg->mode == SCR_MODE_NEAREST_POINT) {
         // encodeNearestPoint: first cloud with resident batches only (:483-497)
@@ -966,7 +973,7 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
         }
         const uint32_t* colors = nullptr;
         if (c) {
-            rc = upload_files(c, c->model, vp, nullptr, nullptr);
+            rc = upload_files(c, c->model, vp, nullptr, prevVP);
             if (rc != SCR_OK) return rc;
             const CloudDev d = c->dev();
             {
@@ -996,25 +1003,45 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
     for (int i = 0; i < nClouds; i++)
         if (clouds[i]->residentBatchCount > 0) resident.push_back(clouds[i]);
     for (scr_cloud* c : resident) {
-        rc = upload_files(c, c->model, vp, nullptr, nullptr);
+        rc = upload_files(c, c->model, vp, nullptr, prevVP);
         if (rc != SCR_OK) return rc;
     }
     for (scr_cloud* c : resident) {
         scr::PassScope ps(ctx, "cull");
         SCR_HIP(ctx, scr::launch_cull(s, c->dev(), p));
     }
+    // translucent defocus and motion blur take the weighted-blend passes
+    // (ComputeRasteriser.swift:233-246)
+    scr::CoverageParams q;
+    q.coverageMode = (cfg->applyTint != 0 && cfg->tintAlphaIsCoverage != 0) ? 1 : 0;
+    q.motionBlur = cfg->motionBlur;
+    q.samples = cfg->motionBlurSamples;
+    q.maxSpread = cfg->motionBlurMaxSpread;
+    const bool blend = q.coverageMode != 0 || cfg->motionBlur > 0.0f;
     for (scr_cloud* c : resident) {
         scr::PassScope ps(ctx, "depth");
-        SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
+        if (q.coverageMode != 0)
+            SCR_HIP(ctx, scr::launch_depth_coverage(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
+        else
+            SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
     }
     for (scr_cloud* c : resident) {
         scr::PassScope ps(ctx, "color");
-        SCR_HIP(ctx, scr::launch_color(s, ctx->numCUs, c->dev(), p, frame->plane(0), frame->plane(1), frame->plane(2),
-                                       frame->plane(3), frame->plane(4)));
+        if (blend)
+            SCR_HIP(ctx, scr::launch_color_coverage(s, ctx->numCUs, c->dev(), p, q, frame->plane(1), frame->plane(2),
+                                                    frame->plane(3), frame->plane(4)));
+        else
+            SCR_HIP(ctx, scr::launch_color(s, ctx->numCUs, c->dev(), p, frame->plane(0), frame->plane(1),
+                                           frame->plane(2), frame->plane(3), frame->plane(4)));
     }
     {
         scr::PassScope ps(ctx, "resolve");
-        SCR_HIP(ctx, scr::launch_resolve(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA, frame->dDepthF));
+        if (blend)
+            SCR_HIP(ctx, scr::launch_resolve_coverage(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA,
+                                                      frame->dDepthF));
+        else
+            SCR_HIP(ctx, scr::launch_resolve(s, p, frame->dPlanes, frame->planeStride(), frame->dRGBA,
+                                             frame->dDepthF));
     }
     return cfg->holeFillIterations > 0 ? scr_frame_hole_fill(frame, cfg->holeFillIterations) : SCR_OK;
 }

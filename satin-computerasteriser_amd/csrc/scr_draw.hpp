@@ -257,6 +257,7 @@ struct ChunkFrags {
     int py[PPT];      // y-flipped row
     int r[PPT];       // footprint radius
     float z[PPT];     // ndc z
+    float wx[PPT], wy[PPT], wz[PPT];  // decoded position (only the motion-blur sweep reads it)
 };
 
 template <int PPT>
@@ -380,6 +381,9 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         f.py[j] = p.height - 1 - iy;
         f.z[j] = nz;
         f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, wx[j], wy[j], wz[j]));
+        f.wx[j] = wx[j];
+        f.wy[j] = wy[j];
+        f.wz[j] = wz[j];
     }
 }
 
@@ -1273,6 +1277,26 @@ __device__ __forceinline__ void resolve_pixel(const PassParams& p, const uint32_
     const float g = ((float)planes[2 * stride + i] * inv) * kInv255;
     const float b = ((float)planes[3 * stride + i] * inv) * kInv255;
     rgba[i] = pack_rgba(r, g, b, 1.0f);
+    depthf[i] = uint_to_depth(planes[i]);
+}
+
+// resolveUpdate, coverage path (Resolve/Shaders.metal:27-45): the planes hold
+// sum(colour * a * 4096) and sum(a * 4096); colour = their ratio, alpha =
+// saturate(1 - e^-sum(a)).
+__device__ __forceinline__ void resolve_coverage_pixel(const PassParams& p, const uint32_t* __restrict__ planes,
+                                                       size_t stride, size_t i, uint32_t* __restrict__ rgba,
+                                                       float* __restrict__ depthf) {
+    const uint32_t a = planes[4 * stride + i];
+    if (a == 0u) {
+        rgba[i] = pack_rgba(p.bg[0], p.bg[1], p.bg[2], 0.0f);
+        depthf[i] = 0.0f;
+        return;
+    }
+    const float inv = rcp((float)a);
+    const float sumA = (float)a * (1.0f / 4096.0f);
+    const float alpha = saturate(1.0f - scr_expf(-sumA));
+    rgba[i] = pack_rgba((float)planes[stride + i] * inv, (float)planes[2 * stride + i] * inv,
+                        (float)planes[3 * stride + i] * inv, alpha);
     depthf[i] = uint_to_depth(planes[i]);
 }
 

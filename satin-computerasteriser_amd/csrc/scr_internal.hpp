@@ -106,6 +106,8 @@ struct scr_frame {
     unsigned long long* dSums = nullptr;     // differentiable mode: 3 int64 planes of planeStride (on first use)
     uint32_t* dRGBA = nullptr;
     uint32_t* dRGBAScratch = nullptr;  // hole fill ping-pong plane (on first use)
+    float prevVP[16];                  // last render's view-projection (motion-blur velocity)
+    bool hasPrevVP = false;
     float* dDepthF = nullptr;
     size_t pixels() const { return (size_t)width * (size_t)height; }
     // planes start on 256-byte boundaries so vector stores stay aligned
@@ -133,7 +135,9 @@ void ctx_retain(scr_ctx* ctx);
 void ctx_release(scr_ctx* ctx);   // frees the context with its last reference
 
 // Shared by the single-view and multi-view render paths (scr_api.hip).
-int check_config(scr_ctx* ctx, const scr_config* cfg);
+// allowCoverage: translucent defocus and motion blur are accepted (the
+// single-view render implements them; multi-view and differentiable mode do not)
+int check_config(scr_ctx* ctx, const scr_config* cfg, bool allowCoverage = false);
 PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, int height);
 int flush_mirror(scr_cloud* c);
 // updateFiles composition (ComputeRasteriserPointCloud.swift:487-515) of every
@@ -146,6 +150,19 @@ void compose_files(const scr_cloud* c, const float* model, const float* vp, cons
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
 hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p);
+// Translucent-defocus / motion-blur settings of the coverage colour pass.
+struct CoverageParams {
+    int coverageMode;   // applyTint && tintAlphaIsCoverage
+    float motionBlur;   // > 0: sweep each splat along its screen velocity
+    int samples;        // motionBlurSamples
+    float maxSpread;    // motionBlurMaxSpread
+};
+hipError_t launch_depth_coverage(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, uint32_t* depth);
+hipError_t launch_color_coverage(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p,
+                                 const CoverageParams& q, uint32_t* red, uint32_t* green, uint32_t* blue,
+                                 uint32_t* count);
+hipError_t launch_resolve_coverage(hipStream_t s, const PassParams& p, const uint32_t* planes, size_t stride,
+                                   uint32_t* rgba, float* depthf);
 hipError_t launch_hole_fill(hipStream_t s, const uint32_t* in, uint32_t* out, int w, int h);
 hipError_t launch_check_fast_rcp(hipStream_t s, int numCUs, unsigned long long* mismatches, uint32_t* first);
 hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t spots, int add, int iters,

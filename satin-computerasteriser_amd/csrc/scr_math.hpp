@@ -39,6 +39,25 @@ __device__ inline float rcp_fast(float x) {
     return fmaf(e, r, r);
 }
 
+// e^x for binary32 x (the coverage resolve's 1 - e^-sum): evaluated in
+// double with + - * / only and rounded to binary32 once, so the device and
+// a CPU evaluation of the same steps agree bit for bit.  x = k ln2 + r with
+// k = rint(x / ln2) and r = (x - k ln2_hi) - k ln2_lo (the fdlibm split: k
+// ln2_hi is exact for |k| < 2^11), |r| <= 0.35; e^r by the Horner form
+// 1 + r/1 (1 + r/2 (1 + ... (1 + r/13))), whose truncation error (< 1e-17)
+// is below the double rounding; then the exact scaling by 2^k.
+__host__ __device__ inline float scr_expf(float xf) {
+    if (!(xf == xf)) return xf;
+    if (xf > 100.0f) return __builtin_inff();
+    if (xf < -110.0f) return 0.0f;
+    const double x = (double)xf;
+    const double k = rint(x * 1.4426950408889634);
+    const double r = (x - k * 6.93147180369123816490e-01) - k * 1.90821492927058770002e-10;
+    double e = 1.0;
+    for (int n = 13; n >= 1; n--) e = 1.0 + (r / (double)n) * e;
+    return (float)ldexp(e, (int)k);
+}
+
 __host__ __device__ inline uint32_t f2u(float x) {
     if (!(x > 0.0f)) return 0u;
     if (x >= kTwo32) return 0xffffffffu;

@@ -142,6 +142,174 @@ hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsig
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- coverage (OIT) passes
+// Translucent defocus (applyTint with tintAlphaIsCoverage) and motion blur
+// replace the depth-tested colour sum by a weighted blend; these passes are
+// one point per thread, one visible batch per workgroup iteration, with
+// global atomics (outside the measured opaque path).
+
+namespace {
+using namespace draw;
+
+// One point through the shared front end (decode, projection, tests,
+// footprint radius); false when it has no fragment.
+template <int kLevel>
+__device__ __forceinline__ bool coverage_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li,
+                                               ChunkFrags<1>& f) {
+    ChunkRaw<1> raw;
+    chunk_fetch<kLevel, 1>(c, bc, li, false, raw);
+    chunk_decode<false, false, kLevel, 1>(c, p, bc, li, raw, f);
+    return f.ok[0];
+}
+
+__device__ __forceinline__ bool coverage_front_any(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
+                                                   uint32_t li, ChunkFrags<1>& f) {
+    if (bc.level == 0) return coverage_front<0>(c, p, bc, li, f);
+    if (bc.level == 1) return coverage_front<1>(c, p, bc, li, f);
+    return coverage_front<2>(c, p, bc, li, f);
+}
+
+// depthPassUpdate in translucent-defocus mode (DepthPass/Shaders.metal:60-67):
+// a point whose tint alpha is the discard sentinel (< 0) claims no depth.
+__global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
+    const uint32_t nvis = *c.counter;
+    for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
+        const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
+        for (uint32_t li = threadIdx.x; li < bc.active; li += blockDim.x) {
+            const uint32_t pi = bc.first + li;
+            if (c.tints && c.tints[pi].w < 0.0f) continue;
+            ChunkFrags<1> f;
+            if (!coverage_front_any(c, p, bc, li, f)) continue;
+            const uint32_t d = depth_to_uint_hw(f.z[0]);
+            const int rad = f.r[0];
+            for (int oy = -rad; oy <= rad; oy++)
+                for (int ox = -rad; ox <= rad; ox++) {
+                    if (!inside_footprint(ox, oy, rad)) continue;
+                    const int tx = f.px[0] + ox, ty = f.py[0] + oy;
+                    if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                    gmax(depth + (size_t)ty * p.width + tx, d);
+                }
+        }
+    }
+}
+
+// colorPassUpdate, weighted-blend path (ColorPass/Shaders.metal:96-209): no
+// depth test; each fragment adds colour * a * 4096 and a * 4096 (a = 1 -
+// tint alpha in defocus mode, else 1), split over the motion-blur sweep from
+// the point's pixel back toward its previous frame's pixel.
+__global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p, CoverageParams q,
+                                                        uint32_t* __restrict__ red, uint32_t* __restrict__ green,
+                                                        uint32_t* __restrict__ blue, uint32_t* __restrict__ count) {
+    const uint32_t nvis = *c.counter;
+    for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
+        const scr_visible_batch vb = c.visible[v];
+        const BatchCtx bc = batch_ctx(c, p, vb, 0xffffffffu);
+        const scr_raster_file* file = c.files + c.batches[vb.batchIndex].fileIndex;
+        for (uint32_t li = threadIdx.x; li < bc.active; li += blockDim.x) {
+            const uint32_t pi = bc.first + li;
+            ChunkFrags<1> f;
+            if (!coverage_front_any(c, p, bc, li, f)) continue;
+            uint32_t color = c.colors[pi];
+            if (p.colorizeChunks != 0) color = vb.batchIndex * 1234567u;
+            else if (p.colorizeOverdraw != 0) color = 0x00010101u;
+            uint32_t r = color & 0xffu, g = (color >> 8) & 0xffu, b = (color >> 16) & 0xffu;
+            float a = 1.0f;
+            if (p.applyTint != 0) {
+                const float4 t = c.tints ? c.tints[pi] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (t.w < 0.0f) continue;
+                if (q.coverageMode != 0) {
+                    a = saturate(1.0f - t.w);
+                } else {
+                    const float w = saturate(t.w);
+                    const float ox = (float)r * kInv255, oy = (float)g * kInv255, oz = (float)b * kInv255;
+                    r = f2u(saturate(ox + (saturate(t.x) - ox) * w) * 255.0f + 0.5f);
+                    g = f2u(saturate(oy + (saturate(t.y) - oy) * w) * 255.0f + 0.5f);
+                    b = f2u(saturate(oz + (saturate(t.z) - oz) * w) * 255.0f + 0.5f);
+                }
+            }
+            if (a <= 0.0f) continue;
+            // motion blur (:146-167): pixel differences wrap as the shader's int2 does
+            int samples = 1;
+            float sx = 0.0f, sy = 0.0f;
+            if (q.motionBlur > 0.0f) {
+                const f4 prev = mul_point(file->prevTransform, f.wx[0], f.wy[0], f.wz[0]);
+                if (prev.w > 0.0f) {
+                    const float prw = rcp(prev.w);
+                    const int ppx = f2i_hw(fmaf(prev.x * prw, 0.5f, 0.5f) * (float)p.width);
+                    const int ppy = p.height - 1 - f2i_hw(fmaf(prev.y * prw, 0.5f, 0.5f) * (float)p.height);
+                    float vx = (float)(int)((uint32_t)f.px[0] - (uint32_t)ppx) * q.motionBlur;
+                    float vy = (float)(int)((uint32_t)f.py[0] - (uint32_t)ppy) * q.motionBlur;
+                    float len = sqrtf(vx * vx + vy * vy);
+                    if (len > q.maxSpread) {
+                        const float k = q.maxSpread * rcp(len);
+                        vx = vx * k;
+                        vy = vy * k;
+                        len = q.maxSpread;
+                    }
+                    if (len > 0.75f) {
+                        samples = min(max(f2i_hw(ceilf(len)), 1), q.samples);
+                        const float d = rcp((float)max(samples - 1, 1));
+                        sx = vx * d;
+                        sy = vy * d;
+                    }
+                }
+            }
+            const float aN = a * rcp((float)samples);
+            const uint32_t cr = f2u((((float)r * kInv255) * aN) * 4096.0f + 0.5f);
+            const uint32_t cg = f2u((((float)g * kInv255) * aN) * 4096.0f + 0.5f);
+            const uint32_t cb = f2u((((float)b * kInv255) * aN) * 4096.0f + 0.5f);
+            const uint32_t ca = f2u(aN * 4096.0f + 0.5f);
+            const int rad = f.r[0];
+            for (int k = 0; k < samples; k++) {
+                const int cx = (int)((uint32_t)f.px[0] - (uint32_t)f2i_hw(roundf(sx * (float)k)));
+                const int cy = (int)((uint32_t)f.py[0] - (uint32_t)f2i_hw(roundf(sy * (float)k)));
+                for (int oy = -rad; oy <= rad; oy++)
+                    for (int ox = -rad; ox <= rad; ox++) {
+                        if (!inside_footprint(ox, oy, rad)) continue;
+                        const int tx = (int)((uint32_t)cx + (uint32_t)ox), ty = (int)((uint32_t)cy + (uint32_t)oy);
+                        if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                        const size_t pix = (size_t)ty * p.width + tx;
+                        gadd(red + pix, cr);
+                        gadd(green + pix, cg);
+                        gadd(blue + pix, cb);
+                        gadd(count + pix, ca);
+                    }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_resolve_coverage(PassParams p, const uint32_t* __restrict__ planes,
+                                                          size_t stride, uint32_t* __restrict__ rgba,
+                                                          float* __restrict__ depthf) {
+    const size_t n = (size_t)p.width * (size_t)p.height;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    resolve_coverage_pixel(p, planes, stride, i, rgba, depthf);
+}
+
+}  // namespace
+
+hipError_t launch_depth_coverage(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, uint32_t* depth) {
+    hipLaunchKernelGGL(k_depth_coverage, dim3(draw_grid(numCUs, c.nSlots)), dim3(256), 0, s, c, p, depth);
+    return hipGetLastError();
+}
+
+hipError_t launch_color_coverage(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p,
+                                 const CoverageParams& q, uint32_t* red, uint32_t* green, uint32_t* blue,
+                                 uint32_t* count) {
+    hipLaunchKernelGGL(k_color_coverage, dim3(draw_grid(numCUs, c.nSlots)), dim3(256), 0, s, c, p, q, red, green,
+                       blue, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_coverage(hipStream_t s, const PassParams& p, const uint32_t* planes, size_t stride,
+                                   uint32_t* rgba, float* depthf) {
+    const size_t n = (size_t)p.width * (size_t)p.height;
+    hipLaunchKernelGGL(k_resolve_coverage, dim3(grid_for(n, 256)), dim3(256), 0, s, p, planes, stride, rgba, depthf);
+    return hipGetLastError();
+}
+
 // One hole-fill pass over a width x height RGBA8 plane, 16 x 16 pixels per
 // workgroup (HoleFillProcessor.swift:37-44).
 __global__ __launch_bounds__(256) void k_hole_fill(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,

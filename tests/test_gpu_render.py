@@ -5,7 +5,7 @@ import pytest
 
 from _pkg import orc, scr
 from fixtures import jittered_cloud, lattice, solid_cube
-from helpers import (MODEL, assert_planes_equal, config1_camera, gpu_cloud, multi_file_packed, oracle_cloud,
+from helpers import (MODEL, PLANES, assert_planes_equal, config1_camera, gpu_cloud, multi_file_packed, oracle_cloud,
                      render_gpu_hqs, render_gpu_nearest)
 
 pytestmark = pytest.mark.gpu
@@ -319,3 +319,61 @@ def test_hole_fill_on_a_sparse_frame(ctx, iterations):
     ras.configuration.holeFillIterations = iterations
     ras.render(cam)
     assert np.array_equal(ras.read(scr.PLANE_RGBA8), orc.hole_fill(near, iterations))
+
+
+def _tints(n, seed):
+    rng = np.random.default_rng(seed)
+    t = rng.uniform(-0.2, 1.0, (n, 4)).astype(np.float32)
+    t[rng.random(n) < 0.1, 3] = -1.0        # discard sentinel
+    t[rng.random(n) < 0.1, 3] = 1.5         # fully defocused: coverage 0
+    return t
+
+
+def test_translucent_defocus_matches_oracle(ctx, jitter):
+    # tintAlphaIsCoverage: the depth pass drops the discard sentinel, the
+    # colour pass blends colour * (1 - tint alpha) with no depth test, the
+    # resolve returns the weighted mean with alpha 1 - e^-sum(a)
+    cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    cfg = scr.ComputeRasteriserConfiguration(applyTint=True, tintAlphaIsCoverage=True, minimumPointSize=1,
+                                             maximumPointSize=3, pointSizeScale=20.0)
+    g, o = gpu_cloud(ctx, jitter), oracle_cloud(jitter)
+    t = _tints(len(o.colors), 21)
+    g.setTints(t)
+    o.tints = t
+    gpu = render_gpu_hqs(ctx, [g], cfg, cam, 320, 240)
+    ref = orc.render_hqs([o], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    alpha = ref["rgba8"] >> 24
+    assert ref["count"].sum() > 10000 and ((alpha > 0) & (alpha < 255)).sum() > 500
+    assert_planes_equal(gpu, ref)
+
+
+@pytest.mark.parametrize("coverage", [False, True])
+def test_motion_blur_matches_oracle(ctx, jitter, coverage):
+    # the frame remembers the previous render's view-projection: the second
+    # render sweeps each splat back toward where the first camera saw it
+    camA = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    camB = scr.Camera.perspective((10.2, 7.6, 13.4), (0.3, 0, 0), fov=55, aspect=4 / 3)
+    cfg = scr.ComputeRasteriserConfiguration(motionBlur=0.7, motionBlurSamples=6, motionBlurMaxSpread=9.0,
+                                             applyTint=coverage, tintAlphaIsCoverage=coverage,
+                                             minimumPointSize=1, maximumPointSize=2, pointSizeScale=20.0)
+    g, o = gpu_cloud(ctx, jitter), oracle_cloud(jitter)
+    if coverage:
+        t = _tints(len(o.colors), 22)
+        g.setTints(t)
+        o.tints = t
+    ras = scr.ComputeRasteriser(ctx, cfg)
+    ras.addPointCloud(g)
+    ras.resize(320, 240)
+    ras.render(camA)            # first render: no previous pose, no sweep
+    first = {k: ras.read(v) for k, v in PLANES.items()}
+    first["depthF32"] = first["depthF32"].view(np.float32)
+    ref = orc.render_hqs([o], cfg, camA.viewMatrix, camA.projectionMatrix, 320, 240)
+    assert_planes_equal(first, ref)
+    ras.render(camB)
+    gpu = {k: ras.read(v) for k, v in PLANES.items()}
+    gpu["depthF32"] = gpu["depthF32"].view(np.float32)
+    prev = orc.mat_mul(camA.projectionMatrix, camA.viewMatrix)
+    ref = orc.render_hqs([o], cfg, camB.viewMatrix, camB.projectionMatrix, 320, 240, prev_view_proj=prev)
+    still = orc.render_hqs([o], cfg, camB.viewMatrix, camB.projectionMatrix, 320, 240)
+    assert (ref["count"] != still["count"]).sum() > 1000     # the sweep moved energy
+    assert_planes_equal(gpu, ref)

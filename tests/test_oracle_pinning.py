@@ -321,3 +321,21 @@ def test_hole_fill_closed_forms_and_loops():
         img[rng.random(shape) < 0.6] &= 0x00FFFFFF          # mostly holes
         for it in (1, 2, 3):
             assert np.array_equal(orc.hole_fill(img, it), _hole_fill_loops(img, it)), (shape, it)
+
+
+def test_expf_is_exp_rounded_once():
+    # contract C8: e^x from double + - * / steps, rounded once to binary32;
+    # against exp in float64 rounded to binary32 (the correctly rounded value
+    # except for inputs within 1e-16 of a rounding boundary, none drawn here)
+    rng = np.random.default_rng(8)
+    xs = np.concatenate([rng.uniform(-104, 88, 20000), rng.uniform(-1, 1, 5000), rng.uniform(0, 16, 5000),
+                         -np.arange(0, 70, 0.125)]).astype(np.float32)
+    bad = 0
+    for x in xs:
+        got = np.float32(orc.expf(float(x)))
+        want = np.float32(np.exp(np.float64(x)))
+        bad += int(got.view(np.uint32) != want.view(np.uint32))
+    assert bad == 0
+    assert orc.expf(0.0) == 1.0 and orc.expf(float("-inf")) == 0.0 and orc.expf(float("inf")) == float("inf")
+    assert math.isnan(orc.expf(float("nan"))) and orc.expf(-120.0) == 0.0 and orc.expf(89.0) == float("inf")
+    assert orc.expf(1.0) == np.float32(math.e)

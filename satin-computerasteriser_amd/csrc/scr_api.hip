@@ -113,6 +113,8 @@ int check_config(scr_ctx* ctx, const scr_config* cfg, bool allowCoverage) {
         return set_error(ctx, SCR_ERR_UNSUPPORTED, "translucent defocus (tintAlphaIsCoverage) needs scr_render");
     if (!allowCoverage && cfg->motionBlur > 0.0f)
         return set_error(ctx, SCR_ERR_UNSUPPORTED, "motion blur needs scr_render");
+    if (!allowCoverage && cfg->holeFillIterations > 0)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "hole fill runs in scr_render or scr_frame_hole_fill");
     if (cfg->pointSizeMode != 0 && cfg->pointSizeMode != 1)
         return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown point size mode");
     return SCR_OK;

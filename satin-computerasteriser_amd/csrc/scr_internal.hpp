@@ -135,8 +135,9 @@ void ctx_retain(scr_ctx* ctx);
 void ctx_release(scr_ctx* ctx);   // frees the context with its last reference
 
 // Shared by the single-view and multi-view render paths (scr_api.hip).
-// allowCoverage: translucent defocus and motion blur are accepted (the
-// single-view render implements them; multi-view and differentiable mode do not)
+// allowCoverage: translucent defocus, motion blur and holeFillIterations are
+// accepted (the single-view render implements them; multi-view and
+// differentiable mode do not)
 int check_config(scr_ctx* ctx, const scr_config* cfg, bool allowCoverage = false);
 PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, int height);
 int flush_mirror(scr_cloud* c);

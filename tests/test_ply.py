@@ -186,3 +186,43 @@ def test_ascii_float_spellings():
     assert pos[1, 0] == np.inf and pos[1, 1] == -np.inf and np.isnan(pos[1, 2])
     with pytest.raises(scr.PLYPointCloudLoaderError):
         scr.PLYPointCloudLoader.parse(head + b"1\t2 3\n4 5 6\n")
+
+
+_SCALARS = ["char", "uchar", "short", "ushort", "int", "uint", "float", "double"]
+
+
+def _value(t, u):
+    # a value of type t from u in [0, 1)
+    if t in _MAX:
+        info = np.iinfo({"b": np.int8, "B": np.uint8, "h": np.int16, "H": np.uint16,
+                         "i": np.int32, "I": np.uint32}[_FMT[t]])
+        return int(info.min + int(u * (int(info.max) - int(info.min))))
+    return float(np.float32((u - 0.5) * 1000.0))
+
+
+try:
+    from hypothesis import given, settings, strategies as st
+except ImportError:  # pragma: no cover
+    given = None
+
+if given is not None:
+    @settings(max_examples=60, deadline=None)
+    @given(st.lists(st.sampled_from(_SCALARS), min_size=0, max_size=4),
+           st.lists(st.floats(0, 0.999), min_size=0, max_size=40), st.integers(0, 3), st.data())
+    def test_random_layouts_parse_alike_and_truncation_is_reported(extra, us, pos, data):
+        names = ["red", "green", "blue", "intensity"]
+        props = [("float", "x"), ("float", "y"), ("float", "z")]
+        for k, t in enumerate(extra):
+            props.insert(min(pos, len(props)), (t, names[k]))
+        n = len(us) // max(1, len(props))
+        rows = [tuple(_value(t, us[(i * len(props) + k) % max(1, len(us))]) for k, (t, _) in enumerate(props))
+                for i in range(n)]
+        pa, ca = scr.PLYPointCloudLoader.parse(_ply("ascii", props, rows))
+        binary = _ply("binary_little_endian", props, rows)
+        pb, cb = scr.PLYPointCloudLoader.parse(binary)
+        assert pa.shape == (n, 4) and np.array_equal(pa, pb) and np.array_equal(ca, cb)
+        if n:
+            cut = data.draw(st.integers(1, len(binary) - binary.index(b"end_header\n") - 11))
+            with pytest.raises(scr.PLYPointCloudLoaderError) as e:
+                scr.PLYPointCloudLoader.parse(binary[:-cut])
+            assert e.value.code == -16

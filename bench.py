@@ -77,6 +77,96 @@ def overview_camera(scr, width, height):
 FLY_SCENE = dict(seed=0x5C0003, halfExtent=4000.0, numBoxes=32000)
 
 
+def blend_bench(scr, ctx, ras, cloud, cam, processed, steps, width, height):
+    """Weighted-blend modes on the headline frame: translucent defocus (tints
+    all zero: every point fully in focus, so every fragment blends) and motion
+    blur between two poses rendered alternately.  The blend colour pass issues
+    4 global atomics per fragment; with 1-px points and no sweep a fragment is
+    a processed point."""
+    cloud.deviceView(withTints=True)
+    out = {}
+    ras.configuration = scr.ComputeRasteriserConfiguration(enableCLOD=False, applyTint=True,
+                                                           tintAlphaIsCoverage=True)
+    out["defocus_ms_per_frame"] = round(timed_frames(ctx, ras, cam, steps, 1), 4)
+    color_ms = pass_profile(ctx, ras, cam, steps)["color"]
+    peak = ctx.measure_atomic_rate(add=True) / 1e9
+    achieved = 4.0 * processed / (color_ms * 1e-3) / 1e9
+    out["defocus_color_ms"] = round(color_ms, 4)
+    out["defocus_atomic_roofline"] = {"bound": "atomics", "achieved": round(achieved, 2), "peak": round(peak, 2),
+                                      "unit": "Gatomic/s", "frac": round(achieved / peak, 4),
+                                      "atomics": 4 * processed}
+    cloud.setTints(None)
+    # the overview pose with the eye 40 m to the side and 25 m up
+    cam2 = scr.Camera(scr.look_at((40.0, -7200.0, 5225.0), (0.0, 300.0, 0.0), (0.0, 0.0, 1.0)),
+                      scr.perspective_reverse_z(55.0, width / height, 10.0, 30000.0))
+    ras.configuration = scr.ComputeRasteriserConfiguration(enableCLOD=False, motionBlur=0.5, motionBlurSamples=4,
+                                                           motionBlurMaxSpread=8.0)
+    for k in range(2):
+        ras.render(cam if k % 2 == 0 else cam2)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ras.render(cam if k % 2 == 0 else cam2)
+    ctx.sync()
+    out["motion_blur_ms_per_frame"] = round((time.perf_counter() - t0) * 1e3 / steps, 4)
+    return out
+
+
+def hole_fill_bench(scr, ctx, ras, cam, width, height, passes=20):
+    """Hole-fill passes over the 1080p output: 4 B read + 4 B written per
+    pixel and pass (the neighbour reads hit the cache), against HBM peak."""
+    ras.render(cam)
+    ras.frame.holeFill(2)
+    ctx.sync()
+    ctx.profile(True)
+    for _ in range(passes):
+        ras.frame.holeFill(2)
+    prof = ctx.collect_profile()
+    ctx.profile(False)
+    tot, n = prof["hole_fill"]
+    ms = tot / n / 2.0
+    gbs = 8.0 * width * height / (ms * 1e-3) / 1e9
+    return {"ms_per_pass": round(ms, 5), "achieved_GBs": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def ply_bench(scr, ctx, n_binary=10_000_000, n_ascii=1_000_000):
+    """PLY input: host parse rate of a binary little-endian file (x y z float,
+    red green blue uchar) and of an ASCII one, and the binary file's parse +
+    GPU pack."""
+    rng = np.random.default_rng(0x91E)
+    rec = np.zeros(n_binary, [("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("r", "u1"), ("g", "u1"), ("b", "u1")])
+    for k in "xyz":
+        rec[k] = rng.uniform(-100, 100, n_binary).astype(np.float32)
+    for k in "rgb":
+        rec[k] = rng.integers(0, 256, n_binary, dtype=np.uint8)
+    head = ("ply\nformat binary_little_endian 1.0\nelement vertex %d\nproperty float x\nproperty float y\n"
+            "property float z\nproperty uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n")
+    binary = (head % n_binary).encode() + rec.tobytes()
+    t0 = time.perf_counter()
+    pos, col = scr.PLYPointCloudLoader.parse(binary)
+    t_bin = time.perf_counter() - t0
+    sub = rec[:n_ascii]
+    body = "\n".join(f"{a:.6g} {b:.6g} {c:.6g} {r} {g} {bl}" for a, b, c, r, g, bl in
+                     zip(sub["x"], sub["y"], sub["z"], sub["r"], sub["g"], sub["b"]))
+    ascii_ = ((head.replace("binary_little_endian", "ascii") % n_ascii) + body).encode()
+    t0 = time.perf_counter()
+    scr.PLYPointCloudLoader.parse(ascii_)
+    t_asc = time.perf_counter() - t0
+    cloud = scr.ComputeRasteriserPointCloud(ctx, 1, scr.DEFAULT_POINTS_PER_BATCH)
+    ctx.sync()
+    t0 = time.perf_counter()
+    scr.PLYPointCloudLoader.packInto(scr.GPUPacker(ctx, keyBits=63), cloud, binary)
+    ctx.sync()
+    t_pack = time.perf_counter() - t0
+    cloud.close()
+    del pos, col
+    return {"binary_points": n_binary, "binary_parse_ms": round(t_bin * 1e3, 1),
+            "binary_MB_per_s": round(len(binary) / t_bin / 1e6, 1),
+            "ascii_points": n_ascii, "ascii_parse_ms": round(t_asc * 1e3, 1),
+            "ascii_MB_per_s": round(len(ascii_) / t_asc / 1e6, 1),
+            "binary_parse_and_gpu_pack_ms": round(t_pack * 1e3, 1), "host_threads": 1}
+
+
 def flythrough_bench(scr, ctx, cloud, frames=120, width=2560, height=1440):
     """The 120-frame CLOD fly-through (Catmull-Rom spline through 8 seeded
     control points 50-400 m above the terrain) at 2560x1440, CLOD and dither on."""
@@ -523,6 +613,12 @@ def main() -> int:
         ras.configuration = scr.ComputeRasteriserConfiguration(mode=scr.MODE_NEAREST_POINT, enableCLOD=False)
         ms_n = timed_frames(ctx, ras, cam, args.steps, 1)
         extra["nearest"] = {"ms_per_frame": round(ms_n, 4), "Gpts_per_s": round(processed / (ms_n * 1e-3) / 1e9, 3)}
+        log("weighted-blend modes (defocus, motion blur), hole fill, PLY input")
+        extra["blend_modes"] = blend_bench(scr, ctx, ras, cloud, cam, processed, max(3, args.steps // 10), W, H)
+        ras.configuration = cfg
+        extra["hole_fill_1080p"] = hole_fill_bench(scr, ctx, ras, cam, W, H)
+        extra["ply"] = ply_bench(scr, ctx)
+        log(f"blend {extra['blend_modes']} hole fill {extra['hole_fill_1080p']} ply {extra['ply']}")
         ras.configuration = cfg
         log("CLOD fly-through, 120 frames at 2560x1440")
         extra["flythrough_120_1440p"] = flythrough_bench(scr, ctx, cloud)

@@ -523,7 +523,8 @@ __device__ __forceinline__ ChunkBox chunk_box(const ChunkFrags<PPT>& f, const Pa
 // camera.  The draw bodies test each fragment against it, so a fragment
 // outside still reaches the planes: the rectangle only decides where the
 // group's fragments meet.  Called by every lane of the wave.
-__device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, uint32_t points, int cap) {
+__device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, uint32_t points, int cap,
+                                                int extra = 0) {
     const int corner = (int)(__lane_id() & 7u);
     // sx * steps is the batch size (power-of-two scaling, exact)
     const float steps = bc.level == 0 ? 1073741824.0f : (bc.level == 1 ? 1048576.0f : 1024.0f);
@@ -552,7 +553,8 @@ __device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const Batch
         z1 = fmaxf(z1, __shfl_xor(z1, o));
         ok &= __shfl_xor(ok, o);
     }
-    const int r = (bc.rconst >= 0 ? bc.rconst : p.maxRadius) + 1;
+    // `extra` widens it further (the motion-blur sweep's reach)
+    const int r = (bc.rconst >= 0 ? bc.rconst : p.maxRadius) + 1 + extra;
     const int ix0 = __builtin_amdgcn_readfirstlane((int)floorf(x0)) - r;
     const int ix1 = __builtin_amdgcn_readfirstlane((int)floorf(x1)) + r;
     const int iy0 = __builtin_amdgcn_readfirstlane((int)floorf(y0)) - r;

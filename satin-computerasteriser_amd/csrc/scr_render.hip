@@ -171,10 +171,18 @@ __device__ __forceinline__ bool coverage_front_any(const CloudDev& c, const Pass
 
 // depthPassUpdate in translucent-defocus mode (DepthPass/Shaders.metal:60-67):
 // a point whose tint alpha is the discard sentinel (< 0) claims no depth.
+// Maxima meet in an LDS tile over the batch rectangle, flushed once.
 __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
+    __shared__ uint32_t tile[kDepthTile];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
         const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
+        const ChunkBox seg = segment_box(p, bc, bc.active, kDepthTile);
+        const int area = seg.w * seg.h;
+        if (seg.lds) {
+            for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) lds_store(tile + i, 0u);
+            __syncthreads();
+        }
         for (uint32_t li = threadIdx.x; li < bc.active; li += blockDim.x) {
             const uint32_t pi = bc.first + li;
             if (c.tints && c.tints[pi].w < 0.0f) continue;
@@ -187,8 +195,17 @@ __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p
                     if (!inside_footprint(ox, oy, rad)) continue;
                     const int tx = f.px[0] + ox, ty = f.py[0] + oy;
                     if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                    gmax(depth + (size_t)ty * p.width + tx, d);
+                    if (seg.lds && box_holds(seg, tx, ty)) lds_max(tile + seg.local(tx, ty), d);
+                    else gmax(depth + (size_t)ty * p.width + tx, d);
                 }
+        }
+        if (seg.lds) {
+            __syncthreads();
+            for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) {
+                const uint32_t d = lds_load(tile + i);
+                if (d != 0u) gmax(depth + seg.pixel(i, p.width), d);
+            }
+            __syncthreads();
         }
     }
 }
@@ -197,14 +214,34 @@ __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p
 // depth test; each fragment adds colour * a * 4096 and a * 4096 (a = 1 -
 // tint alpha in defocus mode, else 1), split over the motion-blur sweep from
 // the point's pixel back toward its previous frame's pixel.
+// The sums of a batch meet in an LDS tile over the batch rectangle (widened
+// by the sweep's reach) and are flushed once, as in the opaque colour pass:
+// per pixel and batch they stay below 2^32 (65,535 points, each adding at
+// most 4096 + 0.5 per sample to a pixel over a sweep that splits it), and
+// integer sums do not depend on the order they are made in.
 __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p, CoverageParams q,
                                                         uint32_t* __restrict__ red, uint32_t* __restrict__ green,
                                                         uint32_t* __restrict__ blue, uint32_t* __restrict__ count) {
+    __shared__ unsigned long long tileRG[kColorTile];
+    __shared__ unsigned long long tileBA[kColorTile];
+    // a sweep reaches at most maxSpread + 1 pixels from the point's pixel
+    const int reach = q.motionBlur > 0.0f ? ((q.maxSpread >= 0.0f && q.maxSpread < 4096.0f)
+                                                 ? (int)ceilf(q.maxSpread) + 1 : (1 << 20))
+                                          : 0;
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
         const scr_visible_batch vb = c.visible[v];
         const BatchCtx bc = batch_ctx(c, p, vb, 0xffffffffu);
         const scr_raster_file* file = c.files + c.batches[vb.batchIndex].fileIndex;
+        const ChunkBox seg = segment_box(p, bc, bc.active, kColorTile, reach);
+        const int area = seg.w * seg.h;
+        if (seg.lds) {
+            for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) {
+                lds_store(tileRG + i, 0ull);
+                lds_store(tileBA + i, 0ull);
+            }
+            __syncthreads();
+        }
         for (uint32_t li = threadIdx.x; li < bc.active; li += blockDim.x) {
             const uint32_t pi = bc.first + li;
             ChunkFrags<1> f;
@@ -268,6 +305,12 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
                         if (!inside_footprint(ox, oy, rad)) continue;
                         const int tx = (int)((uint32_t)cx + (uint32_t)ox), ty = (int)((uint32_t)cy + (uint32_t)oy);
                         if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
+                        if (seg.lds && box_holds(seg, tx, ty)) {
+                            const int k = seg.local(tx, ty);
+                            lds_add(tileRG + k, (unsigned long long)cr | ((unsigned long long)cg << 32));
+                            lds_add(tileBA + k, (unsigned long long)cb | ((unsigned long long)ca << 32));
+                            continue;
+                        }
                         const size_t pix = (size_t)ty * p.width + tx;
                         gadd(red + pix, cr);
                         gadd(green + pix, cg);
@@ -275,6 +318,21 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
                         gadd(count + pix, ca);
                     }
             }
+        }
+        if (seg.lds) {
+            __syncthreads();
+            for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) {
+                const unsigned long long ba = lds_load(tileBA + i);
+                // cr, cg, cb <= ca: a pixel with no coverage has no colour either
+                if ((uint32_t)(ba >> 32) == 0u) continue;
+                const unsigned long long rg = lds_load(tileRG + i);
+                const size_t pix = seg.pixel(i, p.width);
+                gadd(red + pix, (uint32_t)rg);
+                gadd(green + pix, (uint32_t)(rg >> 32));
+                gadd(blue + pix, (uint32_t)ba);
+                gadd(count + pix, (uint32_t)(ba >> 32));
+            }
+            __syncthreads();
         }
     }
 }

@@ -153,20 +153,22 @@ using namespace draw;
 
 // One point through the shared front end (decode, projection, tests,
 // footprint radius); false when it has no fragment.
+// kCovPPT points per thread: their loads are issued together
+constexpr int kCovPPT = 4;
+
 template <int kLevel>
-__device__ __forceinline__ bool coverage_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li,
-                                               ChunkFrags<1>& f) {
-    ChunkRaw<1> raw;
-    chunk_fetch<kLevel, 1>(c, bc, li, false, raw);
-    chunk_decode<false, false, kLevel, 1>(c, p, bc, li, raw, f);
-    return f.ok[0];
+__device__ __forceinline__ void coverage_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
+                                               uint32_t li0, bool vec, ChunkFrags<kCovPPT>& f) {
+    ChunkRaw<kCovPPT> raw;
+    chunk_fetch<kLevel, kCovPPT>(c, bc, li0, vec, raw);
+    chunk_decode<false, false, kLevel, kCovPPT>(c, p, bc, li0, raw, f);
 }
 
-__device__ __forceinline__ bool coverage_front_any(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
-                                                   uint32_t li, ChunkFrags<1>& f) {
-    if (bc.level == 0) return coverage_front<0>(c, p, bc, li, f);
-    if (bc.level == 1) return coverage_front<1>(c, p, bc, li, f);
-    return coverage_front<2>(c, p, bc, li, f);
+__device__ __forceinline__ void coverage_front_any(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
+                                                   uint32_t li0, bool vec, ChunkFrags<kCovPPT>& f) {
+    if (bc.level == 0) coverage_front<0>(c, p, bc, li0, vec, f);
+    else if (bc.level == 1) coverage_front<1>(c, p, bc, li0, vec, f);
+    else coverage_front<2>(c, p, bc, li0, vec, f);
 }
 
 // depthPassUpdate in translucent-defocus mode (DepthPass/Shaders.metal:60-67):
@@ -183,21 +185,26 @@ __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p
             for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) lds_store(tile + i, 0u);
             __syncthreads();
         }
-        for (uint32_t li = threadIdx.x; li < bc.active; li += blockDim.x) {
-            const uint32_t pi = bc.first + li;
-            if (c.tints && c.tints[pi].w < 0.0f) continue;
-            ChunkFrags<1> f;
-            if (!coverage_front_any(c, p, bc, li, f)) continue;
-            const uint32_t d = depth_to_uint_hw(f.z[0]);
-            const int rad = f.r[0];
+        const bool vec = vec_ok<kCovPPT>(bc);
+        for (uint32_t cb = 0; cb < bc.active; cb += blockDim.x * kCovPPT) {
+            const uint32_t li0 = cb + threadIdx.x * kCovPPT;
+            ChunkFrags<kCovPPT> f;
+            coverage_front_any(c, p, bc, li0, vec, f);
+#pragma unroll
+            for (int j = 0; j < kCovPPT; j++) {
+            if (!f.ok[j]) continue;
+            if (c.tints && c.tints[bc.first + li0 + (uint32_t)j].w < 0.0f) continue;
+            const uint32_t d = depth_to_uint_hw(f.z[j]);
+            const int rad = f.r[j];
             for (int oy = -rad; oy <= rad; oy++)
                 for (int ox = -rad; ox <= rad; ox++) {
                     if (!inside_footprint(ox, oy, rad)) continue;
-                    const int tx = f.px[0] + ox, ty = f.py[0] + oy;
+                    const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                     if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
                     if (seg.lds && box_holds(seg, tx, ty)) lds_max(tile + seg.local(tx, ty), d);
                     else gmax(depth + (size_t)ty * p.width + tx, d);
                 }
+            }
         }
         if (seg.lds) {
             __syncthreads();
@@ -242,10 +249,19 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
             }
             __syncthreads();
         }
-        for (uint32_t li = threadIdx.x; li < bc.active; li += blockDim.x) {
-            const uint32_t pi = bc.first + li;
-            ChunkFrags<1> f;
-            if (!coverage_front_any(c, p, bc, li, f)) continue;
+        const bool vec = vec_ok<kCovPPT>(bc);
+        for (uint32_t cb = 0; cb < bc.active; cb += blockDim.x * kCovPPT) {
+            const uint32_t li0 = cb + threadIdx.x * kCovPPT;
+            ChunkFrags<kCovPPT> f;
+            coverage_front_any(c, p, bc, li0, vec, f);
+            // runs of the thread's consecutive single-pixel fragments on one
+            // tile pixel (Morton order) are summed in registers and added once
+            int runK = -1;
+            unsigned long long runRG = 0ull, runBA = 0ull;
+#pragma unroll
+            for (int j = 0; j < kCovPPT; j++) {
+            if (!f.ok[j]) continue;
+            const uint32_t pi = bc.first + li0 + (uint32_t)j;
             uint32_t color = c.colors[pi];
             if (p.colorizeChunks != 0) color = vb.batchIndex * 1234567u;
             else if (p.colorizeOverdraw != 0) color = 0x00010101u;
@@ -269,13 +285,13 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
             int samples = 1;
             float sx = 0.0f, sy = 0.0f;
             if (q.motionBlur > 0.0f) {
-                const f4 prev = mul_point(file->prevTransform, f.wx[0], f.wy[0], f.wz[0]);
+                const f4 prev = mul_point(file->prevTransform, f.wx[j], f.wy[j], f.wz[j]);
                 if (prev.w > 0.0f) {
                     const float prw = rcp(prev.w);
                     const int ppx = f2i_hw(fmaf(prev.x * prw, 0.5f, 0.5f) * (float)p.width);
                     const int ppy = p.height - 1 - f2i_hw(fmaf(prev.y * prw, 0.5f, 0.5f) * (float)p.height);
-                    float vx = (float)(int)((uint32_t)f.px[0] - (uint32_t)ppx) * q.motionBlur;
-                    float vy = (float)(int)((uint32_t)f.py[0] - (uint32_t)ppy) * q.motionBlur;
+                    float vx = (float)(int)((uint32_t)f.px[j] - (uint32_t)ppx) * q.motionBlur;
+                    float vy = (float)(int)((uint32_t)f.py[j] - (uint32_t)ppy) * q.motionBlur;
                     float len = sqrtf(vx * vx + vy * vy);
                     if (len > q.maxSpread) {
                         const float k = q.maxSpread * rcp(len);
@@ -296,10 +312,28 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
             const uint32_t cg = f2u((((float)g * kInv255) * aN) * 4096.0f + 0.5f);
             const uint32_t cb = f2u((((float)b * kInv255) * aN) * 4096.0f + 0.5f);
             const uint32_t ca = f2u(aN * 4096.0f + 0.5f);
-            const int rad = f.r[0];
+            const int rad = f.r[j];
+            if (rad == 0 && samples == 1 && seg.lds && box_holds(seg, f.px[j], f.py[j])) {
+                const int k = seg.local(f.px[j], f.py[j]);
+                const unsigned long long rg = (unsigned long long)cr | ((unsigned long long)cg << 32);
+                const unsigned long long ba = (unsigned long long)cb | ((unsigned long long)ca << 32);
+                if (k == runK) {
+                    runRG += rg;
+                    runBA += ba;
+                    continue;
+                }
+                if (runK >= 0) {
+                    lds_add(tileRG + runK, runRG);
+                    lds_add(tileBA + runK, runBA);
+                }
+                runK = k;
+                runRG = rg;
+                runBA = ba;
+                continue;
+            }
             for (int k = 0; k < samples; k++) {
-                const int cx = (int)((uint32_t)f.px[0] - (uint32_t)f2i_hw(roundf(sx * (float)k)));
-                const int cy = (int)((uint32_t)f.py[0] - (uint32_t)f2i_hw(roundf(sy * (float)k)));
+                const int cx = (int)((uint32_t)f.px[j] - (uint32_t)f2i_hw(roundf(sx * (float)k)));
+                const int cy = (int)((uint32_t)f.py[j] - (uint32_t)f2i_hw(roundf(sy * (float)k)));
                 for (int oy = -rad; oy <= rad; oy++)
                     for (int ox = -rad; ox <= rad; ox++) {
                         if (!inside_footprint(ox, oy, rad)) continue;
@@ -317,6 +351,11 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
                         gadd(blue + pix, cb);
                         gadd(count + pix, ca);
                     }
+            }
+            }
+            if (runK >= 0) {
+                lds_add(tileRG + runK, runRG);
+                lds_add(tileBA + runK, runBA);
             }
         }
         if (seg.lds) {

@@ -226,11 +226,40 @@ __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p
 // per pixel and batch they stay below 2^32 (65,535 points, each adding at
 // most 4096 + 0.5 per sample to a pixel over a sweep that splits it), and
 // integer sums do not depend on the order they are made in.
+// Adds a blend tile's sums to the planes (a pixel with no coverage has no
+// colour either: each fragment's colour sums are at most its coverage sum).
+__device__ __forceinline__ void coverage_tile_flush(const ChunkBox& box, const PassParams& p,
+                                                    const unsigned long long* tileRG,
+                                                    const unsigned long long* tileBA, uint32_t* red,
+                                                    uint32_t* green, uint32_t* blue, uint32_t* count) {
+    const int area = box.w * box.h;
+    for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) {
+        const unsigned long long ba = lds_load(tileBA + i);
+        if ((uint32_t)(ba >> 32) == 0u) continue;
+        const unsigned long long rg = lds_load(tileRG + i);
+        const size_t pix = box.pixel(i, p.width);
+        gadd(red + pix, (uint32_t)rg);
+        gadd(green + pix, (uint32_t)(rg >> 32));
+        gadd(blue + pix, (uint32_t)ba);
+        gadd(count + pix, (uint32_t)(ba >> 32));
+    }
+}
+
+__device__ __forceinline__ void coverage_tile_clear(const ChunkBox& box, unsigned long long* tileRG,
+                                                    unsigned long long* tileBA) {
+    const int area = box.w * box.h;
+    for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) {
+        lds_store(tileRG + i, 0ull);
+        lds_store(tileBA + i, 0ull);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p, CoverageParams q,
                                                         uint32_t* __restrict__ red, uint32_t* __restrict__ green,
                                                         uint32_t* __restrict__ blue, uint32_t* __restrict__ count) {
     __shared__ unsigned long long tileRG[kColorTile];
     __shared__ unsigned long long tileBA[kColorTile];
+    __shared__ int redbuf[5 * kMaxWaves];
     // a sweep reaches at most maxSpread + 1 pixels from the point's pixel
     const int reach = q.motionBlur > 0.0f ? ((q.maxSpread >= 0.0f && q.maxSpread < 4096.0f)
                                                  ? (int)ceilf(q.maxSpread) + 1 : (1 << 20))
@@ -241,12 +270,8 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
         const BatchCtx bc = batch_ctx(c, p, vb, 0xffffffffu);
         const scr_raster_file* file = c.files + c.batches[vb.batchIndex].fileIndex;
         const ChunkBox seg = segment_box(p, bc, bc.active, kColorTile, reach);
-        const int area = seg.w * seg.h;
         if (seg.lds) {
-            for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) {
-                lds_store(tileRG + i, 0ull);
-                lds_store(tileBA + i, 0ull);
-            }
+            coverage_tile_clear(seg, tileRG, tileBA);
             __syncthreads();
         }
         const bool vec = vec_ok<kCovPPT>(bc);
@@ -254,6 +279,16 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
             const uint32_t li0 = cb + threadIdx.x * kCovPPT;
             ChunkFrags<kCovPPT> f;
             coverage_front_any(c, p, bc, li0, vec, f);
+            // a batch too wide for one tile: the chunk's own rectangle, when
+            // it fits (as the opaque colour pass does)
+            ChunkBox box = seg;
+            if (!seg.lds) {
+                box = chunk_box<false, 256, kCovPPT>(f, p, kColorTile, redbuf);
+                if (box.lds) {
+                    coverage_tile_clear(box, tileRG, tileBA);
+                    __syncthreads();
+                }
+            }
             // runs of the thread's consecutive single-pixel fragments on one
             // tile pixel (Morton order) are summed in registers and added once
             int runK = -1;
@@ -313,8 +348,8 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
             const uint32_t cb = f2u((((float)b * kInv255) * aN) * 4096.0f + 0.5f);
             const uint32_t ca = f2u(aN * 4096.0f + 0.5f);
             const int rad = f.r[j];
-            if (rad == 0 && samples == 1 && seg.lds && box_holds(seg, f.px[j], f.py[j])) {
-                const int k = seg.local(f.px[j], f.py[j]);
+            if (rad == 0 && samples == 1 && box.lds && box_holds(box, f.px[j], f.py[j])) {
+                const int k = box.local(f.px[j], f.py[j]);
                 const unsigned long long rg = (unsigned long long)cr | ((unsigned long long)cg << 32);
                 const unsigned long long ba = (unsigned long long)cb | ((unsigned long long)ca << 32);
                 if (k == runK) {
@@ -339,8 +374,8 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
                         if (!inside_footprint(ox, oy, rad)) continue;
                         const int tx = (int)((uint32_t)cx + (uint32_t)ox), ty = (int)((uint32_t)cy + (uint32_t)oy);
                         if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                        if (seg.lds && box_holds(seg, tx, ty)) {
-                            const int k = seg.local(tx, ty);
+                        if (box.lds && box_holds(box, tx, ty)) {
+                            const int k = box.local(tx, ty);
                             lds_add(tileRG + k, (unsigned long long)cr | ((unsigned long long)cg << 32));
                             lds_add(tileBA + k, (unsigned long long)cb | ((unsigned long long)ca << 32));
                             continue;
@@ -357,20 +392,15 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
                 lds_add(tileRG + runK, runRG);
                 lds_add(tileBA + runK, runBA);
             }
+            if (!seg.lds && box.lds) {
+                __syncthreads();
+                coverage_tile_flush(box, p, tileRG, tileBA, red, green, blue, count);
+                __syncthreads();
+            }
         }
         if (seg.lds) {
             __syncthreads();
-            for (int i = (int)threadIdx.x; i < area; i += (int)blockDim.x) {
-                const unsigned long long ba = lds_load(tileBA + i);
-                // cr, cg, cb <= ca: a pixel with no coverage has no colour either
-                if ((uint32_t)(ba >> 32) == 0u) continue;
-                const unsigned long long rg = lds_load(tileRG + i);
-                const size_t pix = seg.pixel(i, p.width);
-                gadd(red + pix, (uint32_t)rg);
-                gadd(green + pix, (uint32_t)(rg >> 32));
-                gadd(blue + pix, (uint32_t)ba);
-                gadd(count + pix, (uint32_t)(ba >> 32));
-            }
+            coverage_tile_flush(seg, p, tileRG, tileBA, red, green, blue, count);
             __syncthreads();
         }
     }

@@ -176,6 +176,7 @@ __device__ __forceinline__ void coverage_front_any(const CloudDev& c, const Pass
 // Maxima meet in an LDS tile over the batch rectangle, flushed once.
 __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
     __shared__ uint32_t tile[kDepthTile];
+    __shared__ int redbuf[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
         const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
@@ -190,6 +191,15 @@ __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p
             const uint32_t li0 = cb + threadIdx.x * kCovPPT;
             ChunkFrags<kCovPPT> f;
             coverage_front_any(c, p, bc, li0, vec, f);
+            // a batch too wide for one tile: the chunk's own rectangle
+            ChunkBox box = seg;
+            if (!seg.lds) {
+                box = chunk_box<false, 256, kCovPPT>(f, p, kDepthTile, redbuf);
+                if (box.lds) {
+                    for (int i = (int)threadIdx.x; i < box.w * box.h; i += (int)blockDim.x) lds_store(tile + i, 0u);
+                    __syncthreads();
+                }
+            }
 #pragma unroll
             for (int j = 0; j < kCovPPT; j++) {
             if (!f.ok[j]) continue;
@@ -201,9 +211,17 @@ __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p
                     if (!inside_footprint(ox, oy, rad)) continue;
                     const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                     if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                    if (seg.lds && box_holds(seg, tx, ty)) lds_max(tile + seg.local(tx, ty), d);
+                    if (box.lds && box_holds(box, tx, ty)) lds_max(tile + box.local(tx, ty), d);
                     else gmax(depth + (size_t)ty * p.width + tx, d);
                 }
+            }
+            if (!seg.lds && box.lds) {
+                __syncthreads();
+                for (int i = (int)threadIdx.x; i < box.w * box.h; i += (int)blockDim.x) {
+                    const uint32_t d = lds_load(tile + i);
+                    if (d != 0u) gmax(depth + box.pixel(i, p.width), d);
+                }
+                __syncthreads();
             }
         }
         if (seg.lds) {

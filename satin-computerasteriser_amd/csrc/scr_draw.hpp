@@ -1307,16 +1307,18 @@ __device__ __forceinline__ void resolve_coverage_pixel(const PassParams& p, cons
 // takes the mean of its valid 8-neighbours' colours (each byte read as
 // byte * (1/255), summed in the shader's row-major order, divided as
 // sum * rcp(count)) with alpha 1, or stays as it is when none is valid.
-__device__ __forceinline__ uint32_t hole_fill_pixel(const uint32_t* __restrict__ in, int w, int h, int x, int y) {
-    const uint32_t c = in[(size_t)y * w + x];
+// c points at the texel in a row-major block of texels `pitch` wide whose
+// ring of eight neighbours is present (texels outside the image held as 0).
+__device__ __forceinline__ uint32_t hole_fill_texels(const uint32_t* c0, int pitch) {
+    const uint32_t c = c0[0];
     if ((c >> 24) >= 128u) return c;
     float sr = 0.0f, sg = 0.0f, sb = 0.0f, n = 0.0f;
+#pragma unroll
     for (int oy = -1; oy <= 1; oy++)
+#pragma unroll
         for (int ox = -1; ox <= 1; ox++) {
             if (ox == 0 && oy == 0) continue;
-            const int sx = x + ox, sy = y + oy;
-            if (sx < 0 || sx >= w || sy < 0 || sy >= h) continue;
-            const uint32_t q = in[(size_t)sy * w + sx];
+            const uint32_t q = c0[oy * pitch + ox];
             if ((q >> 24) < 128u) continue;
             sr += (float)(q & 0xffu) * kInv255;
             sg += (float)((q >> 8) & 0xffu) * kInv255;

@@ -457,12 +457,22 @@ hipError_t launch_resolve_coverage(hipStream_t s, const PassParams& p, const uin
 
 // One hole-fill pass over a width x height RGBA8 plane, 16 x 16 pixels per
 // workgroup (HoleFillProcessor.swift:37-44).
+// The block's 18 x 18 texels (its 16 x 16 pixels and a one-pixel ring) are
+// staged in LDS; a texel outside the image is staged as alpha 0, which no
+// neighbour sum counts, the same as skipping it.
 __global__ __launch_bounds__(256) void k_hole_fill(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                    int w, int h) {
-    const int x = (int)(blockIdx.x * 16u + (threadIdx.x & 15u));
-    const int y = (int)(blockIdx.y * 16u + (threadIdx.x >> 4));
+    __shared__ uint32_t t[18 * 18];
+    const int bx = (int)(blockIdx.x * 16u) - 1, by = (int)(blockIdx.y * 16u) - 1;
+    for (int i = (int)threadIdx.x; i < 18 * 18; i += 256) {
+        const int sx = bx + i % 18, sy = by + i / 18;
+        t[i] = (sx >= 0 && sx < w && sy >= 0 && sy < h) ? in[(size_t)sy * w + sx] : 0u;
+    }
+    __syncthreads();
+    const int lx = (int)(threadIdx.x & 15u), ly = (int)(threadIdx.x >> 4);
+    const int x = bx + 1 + lx, y = by + 1 + ly;
     if (x >= w || y >= h) return;
-    out[(size_t)y * w + x] = draw::hole_fill_pixel(in, w, h, x, y);
+    out[(size_t)y * w + x] = draw::hole_fill_texels(t + (ly + 1) * 18 + (lx + 1), 18);
 }
 
 hipError_t launch_hole_fill(hipStream_t s, const uint32_t* in, uint32_t* out, int w, int h) {

@@ -28,7 +28,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/scr.h"
@@ -211,12 +213,12 @@ bool parse_int(const std::string& s, long long* v) {
 
 // Swift Float(String): the C-locale strtof reading of the whole field,
 // correctly rounded; leading whitespace is not accepted.
-bool parse_float(const std::string& s, float* v) {
+bool parse_float(const char* s, size_t n, float* v) {
     static locale_t c_locale = newlocale(LC_ALL_MASK, "C", (locale_t)0);
-    if (s.empty() || s[0] == ' ' || (s[0] >= '\t' && s[0] <= '\r')) return false;
+    if (n == 0 || s[0] == ' ' || (s[0] >= '\t' && s[0] <= '\r')) return false;
     char* end = nullptr;
-    const float r = strtof_l(s.c_str(), &end, c_locale);
-    if (end != s.c_str() + s.size()) return false;
+    const float r = strtof_l(s, &end, c_locale);
+    if (end != s + n) return false;
     *v = r;
     return true;
 }
@@ -356,6 +358,68 @@ bool parse_header(const uint8_t* d, size_t n, Header* h, Error* err) {
 
 const Error kTruncated{SCR_ERR_PLY_TRUNCATED_BODY, "The PLY file ended before all vertex data could be read."};
 
+struct Span {
+    size_t off, len;
+};
+
+// The first `limit` non-empty lines of b[0, n) (same separators as lines_of).
+std::vector<Span> line_spans(const uint8_t* b, size_t n, size_t limit) {
+    std::vector<Span> out;
+    out.reserve(std::min(limit, n / 2 + 1));
+    const uint8_t* end = b + n;
+    size_t start = 0, i = 0;
+    while (i < n && out.size() < limit) {
+        const size_t k = newline_at(b + i, end);
+        if (k == 0) {
+            i++;
+            continue;
+        }
+        if (i > start) out.push_back({start, i - start});
+        i += k;
+        start = i;
+    }
+    if (out.size() < limit && n > start) out.push_back({start, n - start});
+    return out;
+}
+
+// One ASCII vertex line: 0 parsed; 1 fewer values than properties (checked
+// before any value is read, as the reference counts them first); 2 the value
+// of property *bad does not parse.
+int parse_vertex_line(const char* s, size_t len, const Header& h, float* p, float* c, size_t* bad) {
+    const size_t nf = h.fields.size();
+    Span tok[64];
+    std::vector<Span> more;
+    Span* t = nf <= 64 ? tok : (more.resize(nf), more.data());
+    size_t have = 0, i = 0;
+    while (i < len && have < nf) {
+        while (i < len && s[i] == ' ') i++;
+        size_t j = i;
+        while (j < len && s[j] != ' ') j++;
+        if (j > i) t[have++] = {i, j - i};
+        i = j;
+    }
+    if (have < nf) return 1;
+    char buf[72];
+    for (size_t k = 0; k < nf; k++) {
+        float v;
+        bool ok;
+        if (t[k].len < sizeof(buf)) {
+            memcpy(buf, s + t[k].off, t[k].len);
+            buf[t[k].len] = '\0';
+            ok = parse_float(buf, t[k].len, &v);
+        } else {
+            const std::string w(s + t[k].off, t[k].len);
+            ok = parse_float(w.c_str(), w.size(), &v);
+        }
+        if (!ok) {
+            *bad = k;
+            return 2;
+        }
+        assign(h.fields[k], v, p, c);
+    }
+    return 0;
+}
+
 // Parses the body; writes the vertices when pos and col are given.
 bool parse_body(const uint8_t* d, size_t n, const Header& h, float* pos, float* col, Error* err) {
     const size_t count = (size_t)h.vertices;
@@ -364,32 +428,50 @@ bool parse_body(const uint8_t* d, size_t n, const Header& h, float* pos, float* 
             *err = kTruncated;
             return false;
         }
-        const std::vector<std::string> lines = lines_of(d + h.body, n - h.body);
+        const char* text = (const char*)(d + h.body);
+        const std::vector<Span> lines = line_spans(d + h.body, n - h.body, count);
         if (lines.size() < count) {
             *err = kTruncated;
             return false;
         }
-        for (size_t i = 0; i < count; i++) {
-            const std::vector<std::string> vals = fields_of(lines[i]);
-            if (vals.size() < h.fields.size()) {
-                *err = kTruncated;
-                return false;
-            }
-            float p[3] = {0.0f, 0.0f, 0.0f}, c[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-            for (size_t k = 0; k < h.fields.size(); k++) {
-                float v;
-                if (!parse_float(vals[k], &v)) {
-                    *err = {SCR_ERR_PLY_INVALID_ASCII_VALUE, "Could not parse property '" + h.fields[k].name +
-                                                                 "' on vertex line " + std::to_string(i + 1) + "."};
-                    return false;
+        // lines parse independently: ranges on up to 16 threads; the error
+        // reported is the one of the first failing line, as a sequential parse
+        // would report it
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), std::max<size_t>(1, count / 32768));
+        std::vector<size_t> badLine(nt, (size_t)-1), badField(nt, 0);
+        std::vector<int> badKind(nt, 0);
+        auto work = [&](size_t t) {
+            const size_t a = count * t / nt, b = count * (t + 1) / nt;
+            for (size_t i = a; i < b; i++) {
+                float p[3] = {0.0f, 0.0f, 0.0f}, c[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+                size_t bf = 0;
+                const int r = parse_vertex_line(text + lines[i].off, lines[i].len, h, p, c, &bf);
+                if (r != 0) {
+                    badLine[t] = i, badKind[t] = r, badField[t] = bf;
+                    return;
                 }
-                assign(h.fields[k], v, p, c);
+                if (pos) {
+                    float* q = pos + 4 * i;
+                    q[0] = p[0], q[1] = p[1], q[2] = p[2], q[3] = 0.0f;
+                    memcpy(col + 4 * i, c, sizeof(c));
+                }
             }
-            if (pos) {
-                float* q = pos + 4 * i;
-                q[0] = p[0], q[1] = p[1], q[2] = p[2], q[3] = 0.0f;
-                memcpy(col + 4 * i, c, sizeof(c));
-            }
+        };
+        if (nt == 1) {
+            work(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (size_t t = 0; t < nt; t++) pool.emplace_back(work, t);
+            for (auto& th : pool) th.join();
+        }
+        for (size_t t = 0; t < nt; t++) {
+            if (badLine[t] == (size_t)-1) continue;
+            if (badKind[t] == 1) *err = kTruncated;
+            else
+                *err = {SCR_ERR_PLY_INVALID_ASCII_VALUE, "Could not parse property '" + h.fields[badField[t]].name +
+                                                             "' on vertex line " + std::to_string(badLine[t] + 1) + "."};
+            return false;
         }
         return true;
     }

@@ -165,7 +165,8 @@ def ply_bench(scr, ctx, n_binary=10_000_000, n_ascii=1_000_000):
             "binary_MB_per_s": round(len(binary) / t_bin / 1e6, 1),
             "ascii_points": n_ascii, "ascii_parse_ms": round(t_asc * 1e3, 1),
             "ascii_MB_per_s": round(len(ascii_) / t_asc / 1e6, 1),
-            "binary_parse_and_gpu_pack_ms": round(t_pack * 1e3, 1), "binary_threads": 1,
+            "binary_parse_and_gpu_pack_ms": round(t_pack * 1e3, 1),
+            "binary_threads": min(16, os.cpu_count() or 1, max(1, n_binary // 262144)),
             "ascii_threads": min(16, os.cpu_count() or 1, max(1, n_ascii // 32768))}
 
 

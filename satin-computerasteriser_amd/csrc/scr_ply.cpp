@@ -480,13 +480,25 @@ bool parse_body(const uint8_t* d, size_t n, const Header& h, float* pos, float* 
         return false;
     }
     if (!pos) return true;
-    for (size_t i = 0; i < count; i++) {
-        const uint8_t* rec = d + h.body + i * h.stride;
-        float p[3] = {0.0f, 0.0f, 0.0f}, c[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-        for (const Field& f : h.fields) assign(f, read_scalar(f.type, rec + f.offset), p, c);
-        float* q = pos + 4 * i;
-        q[0] = p[0], q[1] = p[1], q[2] = p[2], q[3] = 0.0f;
-        memcpy(col + 4 * i, c, sizeof(c));
+    // records convert independently: ranges on up to 16 threads
+    auto work = [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++) {
+            const uint8_t* rec = d + h.body + i * h.stride;
+            float p[3] = {0.0f, 0.0f, 0.0f}, c[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+            for (const Field& f : h.fields) assign(f, read_scalar(f.type, rec + f.offset), p, c);
+            float* q = pos + 4 * i;
+            q[0] = p[0], q[1] = p[1], q[2] = p[2], q[3] = 0.0f;
+            memcpy(col + 4 * i, c, sizeof(c));
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>(std::min<size_t>(16, hw), std::max<size_t>(1, count / 262144));
+    if (nt == 1) {
+        work(0, count);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nt; t++) pool.emplace_back(work, count * t / nt, count * (t + 1) / nt);
+        for (auto& th : pool) th.join();
     }
     return true;
 }

@@ -539,8 +539,8 @@ def main() -> int:
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): this build has no multi-GPU "
-              "path; see the sharding notes in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the library has no "
+              "multi-GPU path", file=sys.stderr)
         return 2
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs as a single process on one GPU", file=sys.stderr)

@@ -2,7 +2,7 @@
  * scr_oracle.h — CPU oracle (test infrastructure only).
  *
  * A scalar C restatement of the reference's compute-rasteriser passes and CPU
- * packer, used as the checker for the HIP path.  It is linked only by the test
+ * packer, against which the HIP path is tested.  It is linked only by the test
  * suite, the smoke check and the CPU-baseline leg of the benchmark; the
  * product library never calls it.
  *

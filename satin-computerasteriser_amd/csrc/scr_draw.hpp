@@ -20,9 +20,13 @@ namespace draw {
 
 // Draw workgroups hold NT threads (a multiple of 64) that own PPT points each.
 constexpr int kMaxWaves = 8;        // NT <= 512
-constexpr int kDepthTile = 4096;    // u32 per pixel: 16 KB
+constexpr int kDepthTile = 8192;    // u32 per pixel: 32 KB
 constexpr int kColorTile = 2016;    // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 39.4 KB, four workgroups per CU
-constexpr int kNearestTile = 2048;  // u64 per pixel: 16 KB
+constexpr int kNearestTile = 4096;  // u64 per pixel: 32 KB
+// The multi-view draw kernels keep half-size tiles: their 128-thread shapes
+// stay at 4 waves per SIMD within a CU's LDS (measured faster there).
+constexpr int kMVDepthTile = 4096;
+constexpr int kMVNearestTile = 2048;
 constexpr int kSpare = 64;          // words past a tile: one per lane, the target of no-op updates
 
 // LDS tiles are read and updated through pointers typed as LDS: code that
@@ -401,7 +405,7 @@ __device__ __forceinline__ void chunk_front(const CloudDev& c, const PassParams&
 // the screen) and the number of fragment centres.
 struct ChunkBox {
     int x0, y0, w, h, frags;
-    // k / w == umulhi(k, magic) for 2 <= w <= 4096 and k < 4096 (checked
+    // k / w == umulhi(k, magic) for 2 <= w <= 8192 and k < 8192 (checked
     // exhaustively); w == 1 has no 32-bit magic and divides by itself
     uint32_t magic;
     bool lds;
@@ -680,12 +684,12 @@ __device__ __forceinline__ void depth_flush(const ChunkBox& box, const PassParam
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, int kLevel>
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel, int kTile>
 __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red,
         const ChunkBox& seg) {
     using G = Group<kWave, NT>;
-    constexpr int kCap = kDepthTile / G::kCount;
+    constexpr int kCap = kTile / G::kCount;
     uint32_t* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
@@ -693,7 +697,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
         const int area = seg.w * seg.h;
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
         G::sync();
-        uint32_t* spare = ldsAll + kDepthTile + __lane_id();
+        uint32_t* spare = ldsAll + kTile + __lane_id();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             ChunkFrags<PPT> f;
             chunk_front<false, kR0, kLevel, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
@@ -724,16 +728,16 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile>
 __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red) {
     using G = Group<kWave, NT>;
-    constexpr int kCap = kDepthTile / G::kCount;
+    constexpr int kCap = kTile / G::kCount;
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     // one straight-line copy of the body per position precision
-    if (bc.level == 0) depth_chunks_level<kR0, kWave, NT, PPT, 0>(c, p, bc, lo, depth, ldsAll, red, seg);
-    else if (bc.level == 1) depth_chunks_level<kR0, kWave, NT, PPT, 1>(c, p, bc, lo, depth, ldsAll, red, seg);
-    else depth_chunks_level<kR0, kWave, NT, PPT, 2>(c, p, bc, lo, depth, ldsAll, red, seg);
+    if (bc.level == 0) depth_chunks_level<kR0, kWave, NT, PPT, 0, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
+    else if (bc.level == 1) depth_chunks_level<kR0, kWave, NT, PPT, 1, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
+    else depth_chunks_level<kR0, kWave, NT, PPT, 2, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
 }
 
 // ---------------------------------------------------------------- colour pass body
@@ -1170,12 +1174,12 @@ __device__ __forceinline__ void nearest_flush(const ChunkBox& box, const PassPar
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, int kLevel>
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel, int kTile>
 __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                       unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red,
         const ChunkBox& seg) {
     using G = Group<kWave, NT>;
-    constexpr int kCap = kNearestTile / G::kCount;
+    constexpr int kCap = kTile / G::kCount;
     unsigned long long* lds = ldsAll + G::index() * kCap;
     const bool vec = vec_ok<PPT>(bc);
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
@@ -1183,7 +1187,7 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
         const int area = seg.w * seg.h;
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
         G::sync();
-        unsigned long long* spare = ldsAll + kNearestTile + __lane_id();
+        unsigned long long* spare = ldsAll + kTile + __lane_id();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
             ChunkFrags<PPT> f;
@@ -1216,27 +1220,27 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT, int kTile = kNearestTile>
 __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                       unsigned long long* __restrict__ packed, unsigned long long* ldsAll, int* red) {
     using G = Group<kWave, NT>;
-    constexpr int kCap = kNearestTile / G::kCount;
+    constexpr int kCap = kTile / G::kCount;
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     // one straight-line copy of the body per position precision
-    if (bc.level == 0) nearest_chunks_level<kR0, kWave, NT, PPT, 0>(c, p, bc, lo, packed, ldsAll, red, seg);
-    else if (bc.level == 1) nearest_chunks_level<kR0, kWave, NT, PPT, 1>(c, p, bc, lo, packed, ldsAll, red, seg);
-    else nearest_chunks_level<kR0, kWave, NT, PPT, 2>(c, p, bc, lo, packed, ldsAll, red, seg);
+    if (bc.level == 0) nearest_chunks_level<kR0, kWave, NT, PPT, 0, kTile>(c, p, bc, lo, packed, ldsAll, red, seg);
+    else if (bc.level == 1) nearest_chunks_level<kR0, kWave, NT, PPT, 1, kTile>(c, p, bc, lo, packed, ldsAll, red, seg);
+    else nearest_chunks_level<kR0, kWave, NT, PPT, 2, kTile>(c, p, bc, lo, packed, ldsAll, red, seg);
 }
 
 // ---------------------------------------------------------------- per-batch entry points
 // A batch whose points all have radius 0 runs the single-pixel code.
 
-template <bool kR0, bool kWave, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile>
 __device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                    uint32_t hi, uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
-    if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT>(c, p, bc, lo, depth, lds, red);
-    else depth_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, depth, lds, red);
+    if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT, kTile>(c, p, bc, lo, depth, lds, red);
+    else depth_chunks<kR0, kWave, NT, PPT, kTile>(c, p, bc, lo, depth, lds, red);
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
@@ -1248,13 +1252,13 @@ __device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams&
     else color_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT, int kTile = kNearestTile>
 __device__ __forceinline__ void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
                                      uint32_t hi, unsigned long long* __restrict__ packed, unsigned long long* lds,
                                      int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, hi);
-    if (!kR0 && bc.rconst == 0) nearest_chunks<true, kWave, NT, PPT>(c, p, bc, lo, packed, lds, red);
-    else nearest_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, packed, lds, red);
+    if (!kR0 && bc.rconst == 0) nearest_chunks<true, kWave, NT, PPT, kTile>(c, p, bc, lo, packed, lds, red);
+    else nearest_chunks<kR0, kWave, NT, PPT, kTile>(c, p, bc, lo, packed, lds, red);
 }
 
 // ---------------------------------------------------------------- resolves

@@ -68,11 +68,11 @@ __device__ inline void drain(const MVView* __restrict__ views, const MVItem* __r
 template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* __restrict__ views, const MVItem* __restrict__ items,
                                                  uint32_t nUnits, int splits, uint32_t* heads) {
-    __shared__ uint32_t lds[kDepthTile + kSpare];
+    __shared__ uint32_t lds[kMVDepthTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
                    [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       depth_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, lds, red);
+                       depth_batch<kR0, kWave, NT, PPT, kMVDepthTile>(V.c, V.p, vb, lo, hi, V.planes, lds, red);
                    });
 }
 
@@ -96,11 +96,11 @@ template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView* __restrict__ views,
                                                    const MVItem* __restrict__ items, uint32_t nUnits, int splits,
                                                    uint32_t* heads) {
-    __shared__ unsigned long long lds[kNearestTile + kSpare];
+    __shared__ unsigned long long lds[kMVNearestTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
                    [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       nearest_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.nearest, lds, red);
+                       nearest_batch<kR0, kWave, NT, PPT, kMVNearestTile>(V.c, V.p, vb, lo, hi, V.nearest, lds, red);
                    });
 }
 

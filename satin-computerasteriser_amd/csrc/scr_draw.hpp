@@ -20,10 +20,10 @@ namespace draw {
 
 // Draw workgroups hold NT threads (a multiple of 64) that own PPT points each.
 constexpr int kMaxWaves = 8;        // NT <= 512
-constexpr int kDepthTile = 8192;    // u32 per pixel: 32 KB
+constexpr int kDepthTile = 9984;    // u32 per pixel: 39 KB; with the spare and reduction words four workgroups per CU
 constexpr int kColorTile = 2016;    // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 39.4 KB, four workgroups per CU
-constexpr int kNearestTile = 4096;  // u64 per pixel: 32 KB
-// The multi-view draw kernels keep half-size tiles: their 128-thread shapes
+constexpr int kNearestTile = 4992;  // u64 per pixel: 39 KB, likewise four per CU
+// The multi-view draw kernels keep smaller tiles: their 128-thread shapes
 // stay at 4 waves per SIMD within a CU's LDS (measured faster there).
 constexpr int kMVDepthTile = 4096;
 constexpr int kMVNearestTile = 2048;
@@ -405,7 +405,7 @@ __device__ __forceinline__ void chunk_front(const CloudDev& c, const PassParams&
 // the screen) and the number of fragment centres.
 struct ChunkBox {
     int x0, y0, w, h, frags;
-    // k / w == umulhi(k, magic) for 2 <= w <= 8192 and k < 8192 (checked
+    // k / w == umulhi(k, magic) for 2 <= w <= 16384 and k < 16384 (checked
     // exhaustively); w == 1 has no 32-bit magic and divides by itself
     uint32_t magic;
     bool lds;

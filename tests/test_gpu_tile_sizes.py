@@ -1,5 +1,5 @@
 """Batches whose screen rectangle sits on either side of each LDS tile
-capacity: single-view depth 8,192 pixels and nearest 4,096, multi-view depth
+capacity: single-view depth 9,984 pixels and nearest 4,992, multi-view depth
 4,096 and nearest 2,048, translucent-defocus depth 4,096, colour 2,016.  A
 rectangle within a kernel's capacity takes the one-tile-per-batch path, a
 larger one the per-chunk tiles, so one patch size can take different paths in
@@ -14,7 +14,8 @@ from helpers import PLANES, assert_planes_equal, gpu_cloud, oracle_cloud, render
 pytestmark = pytest.mark.gpu
 
 W, H = 320, 240
-# screen side of the patch in pixels: areas 1,600 / 3,136 / 6,400 / 10,000 / 14,400
+# screen side of the patch in pixels; the batch rectangles measure about 42, 58, 82,
+# 102 and 122 pixels across (areas 1,764 / 3,364 / 6,724 / 10,404 / 14,884)
 SIDES = (40, 56, 80, 100, 120)
 NEAREST_PLANES = {"depth": scr.PLANE_NEAREST_DEPTH, "index": scr.PLANE_NEAREST_INDEX, "rgba8": scr.PLANE_RGBA8}
 

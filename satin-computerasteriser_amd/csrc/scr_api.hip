@@ -1136,7 +1136,7 @@ int scr_diff_backward(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
 
 int scr_pick(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* cloud, scr_frame* frame,
              float ndcX, float ndcY, int searchRadius, uint32_t* outIndex, int* found) {
-    if (!ctx || !cam || !cloud || !frame || !outIndex || !found) return SCR_ERR_INVALID_ARGUMENT;
+    if (!ctx || !cfg || !cam || !cloud || !frame || !outIndex || !found) return SCR_ERR_INVALID_ARGUMENT;
     *found = 0;
     const int W = frame->width, H = frame->height;
     if (cloud->residentBatchCount <= 0) return SCR_OK;

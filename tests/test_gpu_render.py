@@ -272,6 +272,20 @@ def test_pick_empty_space_returns_none(ctx):
     assert ras.pickPointIndex((0.95, 0.95), cloud, cam) is None
 
 
+def test_pick_rejects_a_null_configuration(ctx):
+    import ctypes as C
+    packed = orc.pack([(0.0, 0.0, 0.0)], [(1, 1, 1, 1)])
+    cloud = gpu_cloud(ctx, packed)
+    ras = scr.ComputeRasteriser(ctx, scr.ComputeRasteriserConfiguration())
+    ras.addPointCloud(cloud)
+    ras.resize(16, 16)
+    cam = scr.Camera.perspective((0, 0, 5), (0, 0, 0), fov=45, aspect=1.0).to_c()
+    out, found = C.c_uint32(), C.c_int()
+    rc = ctx.lib.scr_pick(ctx.handle, None, C.byref(cam), cloud.handle, ras.frame.handle, 0.0, 0.0, 4,
+                          C.byref(out), C.byref(found))
+    assert rc == -1  # SCR_ERR_INVALID_ARGUMENT
+
+
 def test_empty_and_all_culled_frames(ctx, cube64):
     cfg = scr.ComputeRasteriserConfiguration(backgroundColor=(0.2, 0.4, 0.6, 1.0))
     bg = orc.render_hqs([], cfg, np.eye(4, dtype=np.float32), np.eye(4, dtype=np.float32), 16, 8)

@@ -236,6 +236,7 @@ hipError_t alloc_pool(scr_cloud* c) {
     if (e == hipSuccess) e = dalloc(c->dLevels, pts);
     if (e == hipSuccess) e = dalloc(c->dBatches, (size_t)n);
     if (e == hipSuccess) e = dalloc(c->dVisible, (size_t)n);
+    if (e == hipSuccess) e = dalloc(c->dBounds, (size_t)n);
     if (e == hipSuccess) e = hipMemsetAsync(c->dLevels, 0, pts, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess)
@@ -251,6 +252,7 @@ void free_pool(scr_cloud* c) {
     dfree(c->dLevels);
     dfree(c->dBatches);
     dfree(c->dVisible);
+    dfree(c->dBounds);
     if (c->tintsOwned) dfree(c->dTints);
     c->dTints = nullptr;
     c->tintsOwned = false;

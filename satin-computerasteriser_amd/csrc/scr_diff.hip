@@ -30,7 +30,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_color(CloudDev c, PassParams 
     __shared__ DiffLds L;
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
-        const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
+        const BatchCtx bc = batch_ctx(c, p, c.visible[v], c.bounds + v, 0xffffffffu);
         if (!kR0 && bc.rconst == 0) diff_color_chunks<true>(c, p, bc, 0u, params, depth, sums, stride, count, L);
         else diff_color_chunks<kR0>(c, p, bc, 0u, params, depth, sums, stride, count, L);
     }
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_backward(CloudDev c, PassPara
                                                             double* __restrict__ grad) {
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
-        const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
+        const BatchCtx bc = batch_ctx(c, p, c.visible[v], c.bounds + v, 0xffffffffu);
         if (!kR0 && bc.rconst == 0) diff_backward_chunks<true>(c, p, bc, 0u, depth, count, dLdImage, grad);
         else diff_backward_chunks<kR0>(c, p, bc, 0u, depth, count, dLdImage, grad);
     }

@@ -140,8 +140,10 @@ __device__ __forceinline__ float pixel_size_on_screen(const scr_raster_batch& b,
 }
 
 // The decision of cullUpdate (Cull/Shaders.metal:19-83) for slot `gid`;
-// true when the batch is drawn, with its VisibleBatch record in `vb`.
-__device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p, int gid, scr_visible_batch& vb) {
+// true when the batch is drawn, with its VisibleBatch record in `vb` and what
+// its box proves about its points (batch_bounds) in `bb`.
+__device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p, int gid, scr_visible_batch& vb,
+                                          BatchBounds& bb) {
     const scr_raster_batch b = c.batches[gid];
     if (b.state == 0) return false;  // residency gate, Cull/Shaders.metal:34-36
     if ((int)b.fileIndex >= c.nFiles) return false;
@@ -169,6 +171,7 @@ __device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p
     vb.level = level;
     vb.lodThreshold = thr;
     vb.activePoints = active;
+    bb = batch_bounds(f->transform, b, p.width, p.height, batch_const_radius(p, f->world, b));
     return true;
 }
 
@@ -182,47 +185,22 @@ struct BatchCtx {
     bool keepAll;       // no point of the batch can fail the LOD/dither test
     int rconst;         // footprint radius shared by the batch's points, or -1
     bool fastRcp;       // every point's clip w lies in [2^-125, 2^125] (rcp_fast is exact)
+    bool proven;        // every fragment centre lies in [bx0, bx1] x [by0, by1] (column, unflipped row), z <= zmax
+    bool inside;        // no point fails the clip-w, NDC or viewport tests
+    int bx0, bx1, by0, by1;
+    float zmax;
     uint32_t first, active, batchIndex;   // active: end of the drawn range
     float T[16];        // file.transform (held in registers: the atomics below
                         // would otherwise force reloads through memory)
     const float* W;     // file.world (only read for per-point sizes)
 };
 
-// Whether the float clip w of every point of the batch lies in
-// [2^-125, 2^125], where rcp_fast equals 1.0f / w.  The decoded positions lie
-// in the box [min, min + size] (fma(q, s, min) is monotone in q < 2^bits), w
-// is affine in the position, so its exact value over the box lies between
-// its corner values; mul_point rounds four times, each time by at most 2^-24
-// of a partial sum bounded by A = |m3 x| + |m7 y| + |m11 z| + |m15| (largest
-// at a corner), so the float w is within 2^-22 A of the exact one; the test
-// allows 2^-20 A.  Corner values in double (products of floats are exact),
-// one corner per lane of each 8-lane group.  Called by every lane of the wave.
-__device__ __forceinline__ bool clip_w_in_fast_range(const BatchCtx& bc) {
-    const int corner = (int)(__lane_id() & 7u);
-    const float steps = bc.level == 0 ? 1073741824.0f : (bc.level == 1 ? 1048576.0f : 1024.0f);
-    const double x = (corner & 1) ? (double)(bc.mnx + bc.sx * steps) : (double)bc.mnx;
-    const double y = (corner & 2) ? (double)(bc.mny + bc.sy * steps) : (double)bc.mny;
-    const double z = (corner & 4) ? (double)(bc.mnz + bc.sz * steps) : (double)bc.mnz;
-    const double tx = (double)bc.T[3] * x, ty = (double)bc.T[7] * y, tz = (double)bc.T[11] * z;
-    double w = ((tx + ty) + tz) + (double)bc.T[15];
-    double wmax = w;
-    double a = ((fabs(tx) + fabs(ty)) + fabs(tz)) + fabs((double)bc.T[15]);
-#pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-        w = fmin(w, __shfl_xor(w, o));
-        wmax = fmax(wmax, __shfl_xor(wmax, o));
-        a = fmax(a, __shfl_xor(a, o));
-    }
-    const double e = a * 0x1p-20;
-    // NaN or infinite corners fail both comparisons
-    const int ok = (w - e >= 0x1p-125 && wmax + e <= 0x1p125) ? 1 : 0;
-    return __builtin_amdgcn_readfirstlane(ok) != 0;
-}
-
 // `hi` bounds the local point indices drawn (at most the active prefix).
+// `bbp` points at the batch's bounds from the cull pass.
 __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
-                                     uint32_t hi) {
+                                              const BatchBounds* bbp, uint32_t hi) {
     const scr_raster_batch b = gload_rec(c.batches + vb.batchIndex);
+    const BatchBounds bb = gload_rec(bbp);
     BatchCtx bc;
     bc.level = vb.level;
     bc.lodThreshold = vb.lodThreshold;
@@ -242,8 +220,15 @@ __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParam
 #pragma unroll
     for (int i = 0; i < 16; i++) bc.T[i] = gload(f->transform + i);
     bc.W = f->world;
-    bc.rconst = batch_const_radius(p, f->world, b);
-    bc.fastRcp = clip_w_in_fast_range(bc);
+    bc.rconst = bb.rconst;
+    bc.fastRcp = (bb.flags & kBoundsFastRcp) != 0u;
+    bc.proven = (bb.flags & kBoundsProven) != 0u;
+    bc.inside = (bb.flags & kBoundsInside) != 0u;
+    bc.bx0 = bb.x0;
+    bc.bx1 = bb.x1;
+    bc.by0 = bb.y0;
+    bc.by1 = bb.y1;
+    bc.zmax = bb.zmax;
     return bc;
 }
 
@@ -313,10 +298,42 @@ __device__ __forceinline__ void chunk_fetch(const CloudDev& c, const BatchCtx& b
     }
 }
 
+// The clip-w, NDC and viewport tests (skipped when kInside: the batch box
+// proves every point passes them), the y flip and the footprint radius.
+template <bool kNearestZ, bool kR0, bool kInside, int PPT>
+__device__ __forceinline__ void chunk_tests(const PassParams& p, const BatchCtx& bc, const f4* clip, const float* rw,
+                                            const bool* keep, const float* wx, const float* wy, const float* wz,
+                                            ChunkFrags<PPT>& f) {
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        bool ok = keep[j];
+        const float nx = clip[j].x * rw[j], ny = clip[j].y * rw[j], nz = clip[j].z * rw[j];
+        const int ix = f2i_hw(fmaf(nx, 0.5f, 0.5f) * (float)p.width);
+        const int iy = f2i_hw(fmaf(ny, 0.5f, 0.5f) * (float)p.height);
+        if (!kInside) {
+            ok = ok && !(clip[j].w <= 0.0f);
+            ok = ok && !(nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f);
+            if (kNearestZ) ok = ok && !(nz <= 0.0f || nz > 1.0f);
+            else ok = ok && !(nz < 0.0f || nz > 1.0f);
+            // ix, iy >= 0 whenever the tests above pass: fma(n, 0.5, 0.5) of
+            // n in [-1, 1] lies in [0, 1], and a NaN converts to 0
+            ok = ok && !(ix >= p.width || iy >= p.height);
+        }
+        f.ok[j] = ok;
+        f.px[j] = ix;
+        f.py[j] = p.height - 1 - iy;
+        f.z[j] = nz;
+        f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, wx[j], wy[j], wz[j]));
+        f.wx[j] = wx[j];
+        f.wy[j] = wy[j];
+        f.wz[j] = wz[j];
+    }
+}
+
 // Runs the front end of every point pass on the fetched points: dither/LOD
 // skip, decodePoint (Common.metal:73-117), projection, clip and viewport
 // tests, y flip, footprint radius.
-template <bool kNearestZ, bool kR0, int kLevel, int PPT>
+template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kUseInside = true>
 __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                     const ChunkRaw<PPT>& r, ChunkFrags<PPT>& f) {
     const uint32_t base = bc.first + li0;
@@ -357,7 +374,6 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         wy[j] = fmaf((float)qy, bc.sy, bc.mny);
         wz[j] = fmaf((float)qz, bc.sz, bc.mnz);
         clip[j] = mul_point(bc.T, wx[j], wy[j], wz[j]);
-        keep[j] = keep[j] && !(clip[j].w <= 0.0f);
     }
     // 1 / w, one wave-uniform branch per chunk
     float rw[PPT];
@@ -368,27 +384,10 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
 #pragma unroll
         for (int j = 0; j < PPT; j++) rw[j] = rcp(clip[j].w);
     }
-#pragma unroll
-    for (int j = 0; j < PPT; j++) {
-        bool ok = keep[j];
-        const float nx = clip[j].x * rw[j], ny = clip[j].y * rw[j], nz = clip[j].z * rw[j];
-        ok = ok && !(nx < -1.0f || ny < -1.0f || nx > 1.0f || ny > 1.0f);
-        if (kNearestZ) ok = ok && !(nz <= 0.0f || nz > 1.0f);
-        else ok = ok && !(nz < 0.0f || nz > 1.0f);
-        const int ix = f2i_hw(fmaf(nx, 0.5f, 0.5f) * (float)p.width);
-        const int iy = f2i_hw(fmaf(ny, 0.5f, 0.5f) * (float)p.height);
-        // ix, iy >= 0 whenever the tests above pass: fma(n, 0.5, 0.5) of
-        // n in [-1, 1] lies in [0, 1], and a NaN converts to 0
-        ok = ok && !(ix >= p.width || iy >= p.height);
-        f.ok[j] = ok;
-        f.px[j] = ix;
-        f.py[j] = p.height - 1 - iy;
-        f.z[j] = nz;
-        f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, wx[j], wy[j], wz[j]));
-        f.wx[j] = wx[j];
-        f.wy[j] = wy[j];
-        f.wz[j] = wz[j];
-    }
+    // a batch whose box proves that every point passes the clip-w, NDC and
+    // viewport tests (kBoundsInside) skips them: one uniform branch per chunk
+    if (kUseInside && kR0 && bc.inside) chunk_tests<kNearestZ, kR0, true, PPT>(p, bc, clip, rw, keep, wx, wy, wz, f);
+    else chunk_tests<kNearestZ, kR0, false, PPT>(p, bc, clip, rw, keep, wx, wy, wz, f);
 }
 
 template <bool kNearestZ, bool kR0, int kLevel, int PPT>
@@ -409,6 +408,7 @@ struct ChunkBox {
     // exhaustively); w == 1 has no 32-bit magic and divides by itself
     uint32_t magic;
     bool lds;
+    bool none;   // (segment_box) the batch provably has no fragment
     float zmax;  // (segment_box) above every point's ndc z; +inf when unknown
     // every on-screen footprint pixel of the chunk lies inside the box (it is
     // the clamped union of the footprints), so no bounds test is needed
@@ -504,6 +504,7 @@ __device__ __forceinline__ ChunkBox chunk_box(const ChunkFrags<PPT>& f, const Pa
     }
     ChunkBox b;
     b.zmax = __builtin_inff();
+    b.none = false;
     b.frags = n;
     b.x0 = max(x0, 0);
     b.y0 = max(y0, 0);
@@ -518,68 +519,37 @@ __device__ __forceinline__ ChunkBox chunk_box(const ChunkFrags<PPT>& f, const Pa
     return b;
 }
 
-// Screen rectangle of a whole batch: the batch box [min, min + size] (every
-// decoded position lies in it) through file.transform, one corner per lane of
-// each 8-lane group, widened by a pixel for the float error of the per-point
-// arithmetic and by the largest footprint radius, clamped to the screen.
-// `lds` is set when the rectangle fits `cap` pixels, is small next to the
-// `points` the group draws from it, and every corner lies in front of the
-// camera.  The draw bodies test each fragment against it, so a fragment
-// outside still reaches the planes: the rectangle only decides where the
-// group's fragments meet.  Called by every lane of the wave.
+// Screen rectangle of a whole batch: the fragment-centre rectangle the cull
+// pass proved from the batch box (batch_bounds), widened by the largest
+// footprint radius (and by `extra`, the motion-blur sweep's reach), clamped to
+// the screen.  Without that proof the batch has no rectangle (lds false,
+// zmax +inf).  `none` is set when the proven rectangle misses the screen: the
+// batch has no fragment at all.  `lds` is set when the rectangle fits `cap`
+// pixels and is small next to the `points` the group draws from it; then
+// every fragment of the batch (footprint pixels on the screen included)
+// lies inside it.
 __device__ __forceinline__ ChunkBox segment_box(const PassParams& p, const BatchCtx& bc, uint32_t points, int cap,
                                                 int extra = 0) {
-    const int corner = (int)(__lane_id() & 7u);
-    // sx * steps is the batch size (power-of-two scaling, exact)
-    const float steps = bc.level == 0 ? 1073741824.0f : (bc.level == 1 ? 1048576.0f : 1024.0f);
-    const float x = (corner & 1) ? bc.mnx + bc.sx * steps : bc.mnx;
-    const float y = (corner & 2) ? bc.mny + bc.sy * steps : bc.mny;
-    const float z = (corner & 4) ? bc.mnz + bc.sz * steps : bc.mnz;
-    const f4 clip = mul_point(bc.T, x, y, z);
-    const float rw = 1.0f / clip.w;
-    // screen x and the unflipped row of the corner, kept finite and far from
-    // the int range (a corner off the screen only widens the clamp below)
-    const float lim = 65536.0f;
-    float sx = fmaf(clip.x * rw, 0.5f, 0.5f) * (float)p.width;
-    float sy = fmaf(clip.y * rw, 0.5f, 0.5f) * (float)p.height;
-    int ok = (clip.w > 0.0f && sx == sx && sy == sy) ? 1 : 0;
-    sx = fminf(fmaxf(sx, -lim), lim);
-    sy = fminf(fmaxf(sy, -lim), lim);
-    // ndc z = clip.z / clip.w is linear-fractional with w > 0 over the box,
-    // so its largest value is at a corner
-    float x0 = sx, x1 = sx, y0 = sy, y1 = sy, z1 = clip.z * rw;
-#pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-        x0 = fminf(x0, __shfl_xor(x0, o));
-        x1 = fmaxf(x1, __shfl_xor(x1, o));
-        y0 = fminf(y0, __shfl_xor(y0, o));
-        y1 = fmaxf(y1, __shfl_xor(y1, o));
-        z1 = fmaxf(z1, __shfl_xor(z1, o));
-        ok &= __shfl_xor(ok, o);
-    }
-    // `extra` widens it further (the motion-blur sweep's reach)
-    const int r = (bc.rconst >= 0 ? bc.rconst : p.maxRadius) + 1 + extra;
-    const int ix0 = __builtin_amdgcn_readfirstlane((int)floorf(x0)) - r;
-    const int ix1 = __builtin_amdgcn_readfirstlane((int)floorf(x1)) + r;
-    const int iy0 = __builtin_amdgcn_readfirstlane((int)floorf(y0)) - r;
-    const int iy1 = __builtin_amdgcn_readfirstlane((int)floorf(y1)) + r;
     ChunkBox b;
-    // the per-point z differs from the corner bound by a few ulps of values
-    // near [0, 1]; 1e-4 covers it many times over
-    const int zb = __builtin_amdgcn_readfirstlane(__float_as_int(z1 + 1e-4f));
-    b.zmax = (__builtin_amdgcn_readfirstlane(ok) != 0 && __int_as_float(zb) == __int_as_float(zb))
-                 ? __int_as_float(zb)
-                 : __builtin_inff();
     b.frags = 0;
-    b.x0 = max(ix0, 0);
-    const int bx1 = min(ix1, p.width - 1);
+    b.none = false;
+    b.zmax = __builtin_inff();
+    b.x0 = b.y0 = b.w = b.h = 0;
+    b.lds = false;
+    b.magic = 0u;
+    if (!bc.proven) return b;
+    const int r = (bc.rconst >= 0 ? bc.rconst : p.maxRadius) + extra;
+    b.zmax = bc.zmax;
+    b.x0 = max(bc.bx0 - r, 0);
+    const int bx1 = min(bc.bx1 + r, p.width - 1);
     // rows are flipped: py = height - 1 - iy
-    b.y0 = max(p.height - 1 - iy1, 0);
-    const int by1 = min(p.height - 1 - iy0, p.height - 1);
+    b.y0 = max(p.height - 1 - (bc.by1 + r), 0);
+    const int by1 = min(p.height - 1 - (bc.by0 - r), p.height - 1);
     b.w = bx1 >= b.x0 ? bx1 - b.x0 + 1 : 0;
     b.h = by1 >= b.y0 ? by1 - b.y0 + 1 : 0;
     const long long area = (long long)b.w * b.h;
-    b.lds = __builtin_amdgcn_readfirstlane(ok) != 0 && area <= cap && area <= 8LL * points;
+    b.none = area == 0;
+    b.lds = area > 0 && area <= cap && area <= 8LL * points;
     b.magic = (b.lds && b.w > 1) ? 0xffffffffu / (uint32_t)b.w + 1u : 0u;
     return b;
 }
@@ -599,10 +569,10 @@ __device__ __forceinline__ bool box_holds(const ChunkBox& b, int x, int y) {
 // (segment_box), every chunk of the group adds to one tile that is flushed
 // once; otherwise each chunk reduces its own rectangle and flushes it.
 
-// A chunk's depth maxima.  With box.lds, fragments inside the box go to the
-// LDS tile (kCheck: each is tested, the box being a bound of the batch; else
-// the box is the chunk's own and holds them all) and the rest to the plane.
-template <bool kR0, bool kCheck, int PPT>
+// A chunk's depth maxima: with box.lds into the LDS tile (the box, the
+// chunk's own rectangle or the batch's proven one, holds every fragment),
+// else straight to the plane.
+template <bool kR0, int PPT>
 __device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const ChunkBox& box, const PassParams& p, uint32_t* lds,
                                    uint32_t* __restrict__ depth) {
     // single-pixel points: a thread's consecutive points often share a
@@ -615,10 +585,6 @@ __device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const Chun
         if (!f.ok[j]) continue;
         const uint32_t d = depth_to_uint_hw(f.z[j]);
         if (kR0) {
-            if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
-                gmax(depth + (size_t)f.py[j] * p.width + f.px[j], d);
-                continue;
-            }
             const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
             if (k == lastK && d <= lastD) continue;
             lastD = (k == lastK) ? max(lastD, d) : d;
@@ -633,41 +599,32 @@ __device__ __forceinline__ void depth_frags(const ChunkFrags<PPT>& f, const Chun
                 if (!inside_footprint(ox, oy, r)) continue;
                 const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                 if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                if (box.lds && (!kCheck || box_holds(box, tx, ty))) lds_max(lds + box.local(tx, ty), d);
+                if (box.lds) lds_max(lds + box.local(tx, ty), d);
                 else gmax(depth + (size_t)ty * p.width + tx, d);
             }
     }
 }
 
-// depth_frags for single-pixel points meeting in a batch tile, with no
-// per-fragment branches: a fragment that is invalid, outside the box or
-// already exceeded by this thread aims its max at the lane's spare word past
-// the tile, which nothing reads; fragments outside the box (none in practice, the
-// box bounding the batch) take one wave-uniform branch to the plane.
+// depth_frags for single-pixel points meeting in a batch tile (the proven
+// rectangle holds every fragment), with no per-fragment branches: a fragment
+// that is invalid or already exceeded by this thread aims its max at the
+// lane's spare word past the tile, which nothing reads.
 template <int PPT>
 __device__ __forceinline__ void depth_frags_tile(const ChunkFrags<PPT>& f, const ChunkBox& box,
                                                  const PassParams& p, uint32_t* lds, uint32_t* spare,
                                                  uint32_t* __restrict__ depth) {
     int lastK = -1;
     uint32_t lastD = 0u;
-    bool outside = false;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const uint32_t d = depth_to_uint_hw(f.z[j]);
-        const bool in = f.ok[j] & box_holds(box, f.px[j], f.py[j]);
-        outside = outside | (f.ok[j] & !in);
+        const bool in = f.ok[j];
         const int k = box.local(f.px[j], f.py[j]);
         const bool fresh = in & !((k == lastK) & (d <= lastD));
         lastD = fresh ? d : lastD;
         lastK = fresh ? k : lastK;
         // the spare words are never read, so any value may land there
         lds_max(fresh ? lds + k : spare, d);
-    }
-    if (__ballot(outside) != 0ull) {
-#pragma unroll
-        for (int j = 0; j < PPT; j++)
-            if (f.ok[j] && !box_holds(box, f.px[j], f.py[j]))
-                gmax(depth + (size_t)f.py[j] * p.width + f.px[j], depth_to_uint_hw(f.z[j]));
     }
 }
 
@@ -702,7 +659,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
             ChunkFrags<PPT> f;
             chunk_front<false, kR0, kLevel, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
             if (kR0) depth_frags_tile<PPT>(f, seg, p, lds, spare, depth);
-            else depth_frags<kR0, true, PPT>(f, seg, p, lds, depth);
+            else depth_frags<kR0, PPT>(f, seg, p, lds, depth);
         }
         G::sync();
         depth_flush<G>(seg, p, lds, depth);
@@ -719,7 +676,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
             for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
             G::sync();
         }
-        depth_frags<kR0, false, PPT>(f, box, p, lds, depth);
+        depth_frags<kR0, PPT>(f, box, p, lds, depth);
         if (box.lds) {
             G::sync();
             depth_flush<G>(box, p, lds, depth);
@@ -734,6 +691,7 @@ __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams
     using G = Group<kWave, NT>;
     constexpr int kCap = kTile / G::kCount;
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    if (seg.none) return;  // no point of the batch reaches the screen
     // one straight-line copy of the body per position precision
     if (bc.level == 0) depth_chunks_level<kR0, kWave, NT, PPT, 0, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
     else if (bc.level == 1) depth_chunks_level<kR0, kWave, NT, PPT, 1, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
@@ -741,6 +699,10 @@ __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams
 }
 
 // ---------------------------------------------------------------- colour pass body
+// Whether the colour pass skips the per-point clip tests of batches proven
+// inside the view (the depth and nearest passes do).
+constexpr bool kColorInside = false;
+
 // colorPassUpdate (opaque path) — ColorPass/Shaders.metal:30-219, one visible batch.
 struct ColorLds {
     uint32_t* depth;   // per tile pixel: color_threshold of its depth, as float bits
@@ -843,7 +805,7 @@ __device__ __forceinline__ void chunk_colors(const CloudDev& c, const PassParams
 // A chunk's colour sums (same routes as depth_frags).  Single-pixel points:
 // runs of a thread's consecutive fragments on one tile pixel (Morton order)
 // are summed in registers and added once.
-template <bool kR0, bool kCheck, int PPT>
+template <bool kR0, int PPT>
 __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint32_t* rgb, const ChunkBox& box,
                                    const PassParams& p, const uint32_t* __restrict__ depth, uint32_t* red,
                                    uint32_t* green, uint32_t* blue, uint32_t* count, ColorLds L) {
@@ -856,16 +818,6 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
         const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
         const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
         if (kR0) {
-            if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
-                const size_t pix = (size_t)f.py[j] * p.width + f.px[j];
-                const uint32_t d = gload(depth + pix);
-                if (d == 0u || !(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-                gadd(red + pix, r8);
-                gadd(green + pix, g8);
-                gadd(blue + pix, b8);
-                gadd(count + pix, 1u);
-                continue;
-            }
             const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
             const float thr = box.lds ? __uint_as_float(lds_load(L.depth + k)) : color_threshold(gload(depth + k), p);
             if (!(f.z[j] >= thr)) continue;
@@ -887,7 +839,7 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
                 const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                 if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
                 const size_t pix = (size_t)ty * p.width + tx;
-                const bool inTile = box.lds && (!kCheck || box_holds(box, tx, ty));
+                const bool inTile = box.lds;
                 const int k = inTile ? box.local(tx, ty) : 0;
                 const float thr =
                     inTile ? __uint_as_float(lds_load(L.depth + k)) : color_threshold(gload(depth + pix), p);
@@ -906,9 +858,9 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
     if (kR0 && runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
 }
 
-// color_frags for single-pixel points meeting in a batch tile: one branch
-// per fragment (accepted or not) instead of one per test; fragments outside
-// the box (none in practice) take one wave-uniform branch to the planes.
+// color_frags for single-pixel points meeting in a batch tile (the proven
+// rectangle holds every fragment): one branch per fragment (accepted or not)
+// instead of one per test.
 template <int PPT>
 __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const uint32_t* rgb, const ChunkBox& box,
                                                  const PassParams& p, const uint32_t* __restrict__ depth,
@@ -916,12 +868,9 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
                                                  ColorLds L) {
     int runK = -1;
     unsigned long long runRG = 0ull, runBN = 0ull;
-    bool outside = false;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        const bool valid = f.ok[j];
-        const bool in = valid & box_holds(box, f.px[j], f.py[j]);
-        outside = outside | (valid & !in);
+        const bool in = f.ok[j];
         const int k = in ? box.local(f.px[j], f.py[j]) : 0;
         const bool pass = in & (f.z[j] >= __uint_as_float(lds_load(L.depth + k)));
         if (!pass) continue;
@@ -944,19 +893,6 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
     if (runK >= 0) {
         lds_add(L.rg + runK, runRG);
         lds_add(L.bn + runK, runBN);
-    }
-    if (__ballot(outside) != 0ull) {
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            if (!f.ok[j] || box_holds(box, f.px[j], f.py[j])) continue;
-            const size_t pix = (size_t)f.py[j] * p.width + f.px[j];
-            const uint32_t d = gload(depth + pix);
-            if (d == 0u || !(f.z[j] >= uint_to_depth(d) * p.depthKeep)) continue;
-            gadd(red + pix, rgb[j] & 0xffu);
-            gadd(green + pix, (rgb[j] >> 8) & 0xffu);
-            gadd(blue + pix, (rgb[j] >> 16) & 0xffu);
-            gadd(count + pix, 1u);
-        }
     }
 }
 
@@ -1037,10 +973,10 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
             uint32_t rgb[PPT];
             color_fetch<PPT>(c, bc, li0, vec, rgb);
             ChunkFrags<PPT> f;
-            chunk_decode<false, kR0, kLevel, PPT>(c, p, bc, li0, raw, f);
+            chunk_decode<false, kR0, kLevel, PPT, kColorInside>(c, p, bc, li0, raw, f);
             chunk_colors<PPT>(c, p, bc, li0, f, rgb);
             if (kR0) color_frags_tile<PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
-            else color_frags<kR0, true, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
+            else color_frags<kR0, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
         }
         G::sync();
         color_tile_flush<G>(seg, p, L, red, green, blue, count);
@@ -1054,7 +990,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
         uint32_t rgb[PPT];
         color_fetch<PPT>(c, bc, li0, vec, rgb);
         ChunkFrags<PPT> f;
-        chunk_decode<false, kR0, kLevel, PPT>(c, p, bc, li0, raw, f);
+        chunk_decode<false, kR0, kLevel, PPT, kColorInside>(c, p, bc, li0, raw, f);
         chunk_colors<PPT>(c, p, bc, li0, f, rgb);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
         if (box.frags == 0) continue;
@@ -1062,7 +998,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
             color_stage<G>(box, p, depth, L);
             G::sync();
         }
-        color_frags<kR0, false, PPT>(f, rgb, box, p, depth, red, green, blue, count, L);
+        color_frags<kR0, PPT>(f, rgb, box, p, depth, red, green, blue, count, L);
         if (box.lds) {
             G::sync();
             color_tile_flush<G>(box, p, L, red, green, blue, count);
@@ -1078,6 +1014,7 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
     using G = Group<kWave, NT>;
     constexpr int kCap = kColorTile / G::kCount;
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    if (seg.none) return;  // no point of the batch reaches the screen
     // one straight-line copy of the body per position precision
     if (bc.level == 0) color_chunks_level<kR0, kWave, NT, PPT, 0>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
     else if (bc.level == 1) color_chunks_level<kR0, kWave, NT, PPT, 1>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
@@ -1087,7 +1024,7 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
 // ---------------------------------------------------------------- nearest pass body
 // nearestDepthUpdate + nearestIndexUpdate (NearestDepth/Shaders.metal:14-91,
 // NearestIndex/Shaders.metal:14-94) as one max of depth << 32 | ~index.
-template <bool kR0, bool kCheck, int PPT>
+template <bool kR0, int PPT>
 __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t base, const ChunkBox& box,
                                      const PassParams& p, unsigned long long* lds,
                                      unsigned long long* __restrict__ packed) {
@@ -1102,10 +1039,6 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
         const unsigned long long key =
             ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~pi);
         if (kR0) {
-            if (kCheck && !box_holds(box, f.px[j], f.py[j])) {
-                gmax(packed + (size_t)f.py[j] * p.width + f.px[j], key);
-                continue;
-            }
             // as in the depth pass: skip a max this thread already exceeded
             const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
             if (k == lastK && key <= lastKey) continue;
@@ -1121,16 +1054,16 @@ __device__ __forceinline__ void nearest_frags(const ChunkFrags<PPT>& f, uint32_t
                 if (!inside_footprint(ox, oy, r)) continue;
                 const int tx = f.px[j] + ox, ty = f.py[j] + oy;
                 if (tx < 0 || tx >= p.width || ty < 0 || ty >= p.height) continue;
-                if (box.lds && (!kCheck || box_holds(box, tx, ty))) lds_max(lds + box.local(tx, ty), key);
+                if (box.lds) lds_max(lds + box.local(tx, ty), key);
                 else gmax(packed + (size_t)ty * p.width + tx, key);
             }
     }
 }
 
-// nearest_frags for single-pixel points meeting in a batch tile, with no
-// per-fragment branch (as depth_frags_tile): a fragment that is invalid,
-// outside the box or already exceeded by this thread aims its max at the
-// lane's spare word, which nothing reads.
+// nearest_frags for single-pixel points meeting in a batch tile (the proven
+// rectangle holds every fragment), with no per-fragment branch (as
+// depth_frags_tile): a fragment that is invalid or already exceeded by this
+// thread aims its max at the lane's spare word, which nothing reads.
 template <int PPT>
 __device__ __forceinline__ void nearest_frags_tile(const ChunkFrags<PPT>& f, uint32_t base, const ChunkBox& box,
                                                    const PassParams& p, unsigned long long* lds,
@@ -1138,28 +1071,19 @@ __device__ __forceinline__ void nearest_frags_tile(const ChunkFrags<PPT>& f, uin
                                                    unsigned long long* __restrict__ packed) {
     int lastK = -1;
     unsigned long long lastKey = 0ull;
-    bool outside = false;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         // max of depth << 32 | ~index == the two-pass rule: deepest
         // reverse-Z depth, then the lowest point index among equal depths
         const unsigned long long key =
             ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~(base + (uint32_t)j));
-        const bool in = f.ok[j] & box_holds(box, f.px[j], f.py[j]);
-        outside = outside | (f.ok[j] & !in);
+        const bool in = f.ok[j];
         const int k = box.local(f.px[j], f.py[j]);
         const bool fresh = in & !((k == lastK) & (key <= lastKey));
         lastKey = fresh ? key : lastKey;
         lastK = fresh ? k : lastK;
         // the spare words are never read, so any value may land there
         lds_max(fresh ? lds + k : spare, key);
-    }
-    if (__ballot(outside) != 0ull) {
-#pragma unroll
-        for (int j = 0; j < PPT; j++)
-            if (f.ok[j] && !box_holds(box, f.px[j], f.py[j]))
-                gmax(packed + (size_t)f.py[j] * p.width + f.px[j],
-                     ((unsigned long long)depth_to_uint_hw(f.z[j]) << 32) | (unsigned long long)(~(base + (uint32_t)j)));
     }
 }
 
@@ -1193,7 +1117,7 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
             ChunkFrags<PPT> f;
             chunk_front<true, kR0, kLevel, PPT>(c, p, bc, li0, vec, f);
             if (kR0) nearest_frags_tile<PPT>(f, bc.first + li0, seg, p, lds, spare, packed);
-            else nearest_frags<kR0, true, PPT>(f, bc.first + li0, seg, p, lds, packed);
+            else nearest_frags<kR0, PPT>(f, bc.first + li0, seg, p, lds, packed);
         }
         G::sync();
         nearest_flush<G>(seg, p, lds, packed);
@@ -1211,7 +1135,7 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
             for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
             G::sync();
         }
-        nearest_frags<kR0, false, PPT>(f, bc.first + li0, box, p, lds, packed);
+        nearest_frags<kR0, PPT>(f, bc.first + li0, box, p, lds, packed);
         if (box.lds) {
             G::sync();
             nearest_flush<G>(box, p, lds, packed);
@@ -1226,6 +1150,7 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
     using G = Group<kWave, NT>;
     constexpr int kCap = kTile / G::kCount;
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
+    if (seg.none) return;  // no point of the batch reaches the screen
     // one straight-line copy of the body per position precision
     if (bc.level == 0) nearest_chunks_level<kR0, kWave, NT, PPT, 0, kTile>(c, p, bc, lo, packed, ldsAll, red, seg);
     else if (bc.level == 1) nearest_chunks_level<kR0, kWave, NT, PPT, 1, kTile>(c, p, bc, lo, packed, ldsAll, red, seg);
@@ -1236,27 +1161,30 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
 // A batch whose points all have radius 0 runs the single-pixel code.
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile>
-__device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
-                                   uint32_t hi, uint32_t* __restrict__ depth, uint32_t* lds, int* red) {
-    const BatchCtx bc = batch_ctx(c, p, vb, hi);
+__device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
+                                            const BatchBounds* bb, uint32_t lo, uint32_t hi, uint32_t* __restrict__ depth,
+                                            uint32_t* lds, int* red) {
+    const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
     if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT, kTile>(c, p, bc, lo, depth, lds, red);
     else depth_chunks<kR0, kWave, NT, PPT, kTile>(c, p, bc, lo, depth, lds, red);
 }
 
 template <bool kR0, bool kWave, int NT, int PPT>
-__device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
-                                   uint32_t hi, const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
-                                   uint32_t* blue, uint32_t* count, ColorLds L) {
-    const BatchCtx bc = batch_ctx(c, p, vb, hi);
+__device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
+                                            const BatchBounds* bb, uint32_t lo, uint32_t hi,
+                                            const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
+                                            uint32_t* blue, uint32_t* count, ColorLds L) {
+    const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
     if (!kR0 && bc.rconst == 0) color_chunks<true, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
     else color_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
 }
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kNearestTile>
-__device__ __forceinline__ void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb, uint32_t lo,
-                                     uint32_t hi, unsigned long long* __restrict__ packed, unsigned long long* lds,
-                                     int* red) {
-    const BatchCtx bc = batch_ctx(c, p, vb, hi);
+__device__ __forceinline__ void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
+                                              const BatchBounds* bb, uint32_t lo, uint32_t hi,
+                                              unsigned long long* __restrict__ packed, unsigned long long* lds,
+                                              int* red) {
+    const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
     if (!kR0 && bc.rconst == 0) nearest_chunks<true, kWave, NT, PPT, kTile>(c, p, bc, lo, packed, lds, red);
     else nearest_chunks<kR0, kWave, NT, PPT, kTile>(c, p, bc, lo, packed, lds, red);
 }

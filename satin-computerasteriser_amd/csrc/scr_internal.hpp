@@ -43,6 +43,7 @@ struct CloudDev {
     const uint8_t* levels;
     const float4* tints;           // nullptr: zero stand-in
     scr_visible_batch* visible;    // cull output
+    scr::BatchBounds* bounds;      // cull output, one per visible entry (what its box proves)
     uint32_t* counter;             // visible count
     int nSlots;
     int nFiles;
@@ -77,6 +78,7 @@ struct scr_cloud {
     float4* dTints = nullptr;
     bool tintsOwned = false;
     scr_visible_batch* dVisible = nullptr;
+    scr::BatchBounds* dBounds = nullptr;
     uint32_t* dCounter = nullptr;
 
     size_t poolPoints() const { return (size_t)maxResidentBatches * (size_t)pointsPerBatch; }
@@ -91,6 +93,7 @@ struct scr_cloud {
         c.levels = dLevels;
         c.tints = dTints;
         c.visible = dVisible;
+        c.bounds = dBounds;
         c.counter = dCounter;
         c.nSlots = maxResidentBatches;
         c.nFiles = (int)files.size();

@@ -316,6 +316,153 @@ __host__ __device__ inline int batch_const_radius(const PassParams& p, const flo
 // Host-side radius for lo == hi (size clamps to lo for every point).
 __host__ inline int const_radius_for(float lo) { return radius_of_size(lo, lo, lo); }
 
+// ---------------------------------------------------------------- batch bounds
+//
+// What the cull pass proves about every point of a visible batch from the
+// eight corners of its box, for the draw passes to rely on.  The decoded
+// positions fma(q, size * 2^-bits, min) lie in the float box [min, c1] with
+// c1 = min + size (fma is monotone in q < 2^bits and rounding is monotone).
+// The point transform is affine, so the exact clip coordinates of a point
+// lie between the corner values, and since w > 0 on the whole box (all
+// corners), so do x / w, y / w and z / w.  The float clip coordinates the
+// passes compute (mul_point: four roundings of partial sums bounded by
+// A = |m0 x| + |m4 y| + |m8 z| + |m12|, largest at a corner) are within
+// e = 2^-20 A of the exact ones (the rounding error is below 2^-22 A), so the
+// float ratio n = x_f * rcp(w_f) satisfies
+//     |n - x / w| <= (e_x + N e_w) / (w_min - e_w) = D,  plus |n| 2^-22.9
+// for the two roundings of rcp and the product (N = the largest |x / w| of
+// the corners).  The viewport (fma(n, 0.5, 0.5) * W) adds at most 2^-23 W.
+// Corner values are evaluated in double: each product of two floats is exact
+// there and the sums' rounding (2^-52 A) is far inside the allowance.
+struct BatchBounds {
+    int32_t x0, x1, y0, y1;  // kProven: every fragment centre's column ix and unflipped row iy lie in [x0, x1] x [y0, y1]
+    float zmax;              // kProven: every fragment's ndc z is <= zmax
+    uint32_t flags;          // kBoundsProven | kBoundsInside | kBoundsFastRcp
+    int32_t rconst;          // footprint radius every point of the batch computes, or -1
+    uint32_t pad;
+};
+constexpr uint32_t kBoundsProven = 1u;   // the rectangle and zmax hold
+constexpr uint32_t kBoundsInside = 2u;   // no point fails the w, NDC or viewport tests (nearest's ndc z > 0 included)
+constexpr uint32_t kBoundsFastRcp = 4u;  // every point's clip w lies in [2^-125, 2^125] (rcp_fast is exact)
+
+// One corner through the first or second half of m in double: clip value
+// and magnitude A per row.
+struct CornerTerms {
+    double c[4], a[4];
+};
+__host__ __device__ inline CornerTerms corner_terms(const float* m, float x, float y, float z) {
+    CornerTerms t;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double tx = (double)m[i] * (double)x, ty = (double)m[4 + i] * (double)y, tz = (double)m[8 + i] * (double)z;
+        t.c[i] = ((tx + ty) + tz) + (double)m[12 + i];
+        t.a[i] = ((fabs(tx) + fabs(ty)) + fabs(tz)) + fabs((double)m[12 + i]);
+    }
+    return t;
+}
+
+// Running extremes over the corners.
+struct CornerExtremes {
+    double wmin, wmax, nmin[3], nmax[3], amax[4];
+    int bad;  // a corner behind the camera or not finite
+};
+__host__ __device__ inline CornerExtremes corner_extremes(const CornerTerms& t) {
+    CornerExtremes e;
+    const double w = t.c[3];
+    e.bad = (w > 0.0 && w < 1e300) ? 0 : 1;
+    e.wmin = e.wmax = w;
+#pragma unroll
+    for (int i = 0; i < 3; i++) e.nmin[i] = e.nmax[i] = e.bad ? 0.0 : t.c[i] / w;
+#pragma unroll
+    for (int i = 0; i < 4; i++) e.amax[i] = t.a[i];
+    for (int i = 0; i < 4; i++) e.bad |= (t.c[i] == t.c[i] && fabs(t.c[i]) < 1e300 && t.a[i] < 1e300) ? 0 : 1;
+    return e;
+}
+__host__ __device__ inline CornerExtremes combine(const CornerExtremes& a, const CornerExtremes& b) {
+    CornerExtremes e;
+    e.bad = a.bad | b.bad;
+    e.wmin = fmin(a.wmin, b.wmin);
+    e.wmax = fmax(a.wmax, b.wmax);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        e.nmin[i] = fmin(a.nmin[i], b.nmin[i]);
+        e.nmax[i] = fmax(a.nmax[i], b.nmax[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) e.amax[i] = fmax(a.amax[i], b.amax[i]);
+    return e;
+}
+
+// The float box of the batch's decoded positions: min and min + size (the
+// corner the decode reaches with q = 2^bits; size * 2^-bits * 2^bits = size).
+__host__ __device__ inline void batch_box(const scr_raster_batch& b, float lo[3], float hi[3]) {
+    lo[0] = b.minX;
+    lo[1] = b.minY;
+    lo[2] = b.minZ;
+    hi[0] = b.minX + fmaxf(b.maxX - b.minX, 0.000001f);
+    hi[1] = b.minY + fmaxf(b.maxY - b.minY, 0.000001f);
+    hi[2] = b.minZ + fmaxf(b.maxZ - b.minZ, 0.000001f);
+}
+
+// Bounds from the corners' extremes, for a width x height viewport.
+__host__ __device__ inline BatchBounds finish_bounds(const CornerExtremes& e, int width, int height, int rconst) {
+    BatchBounds bb;
+    bb.x0 = bb.y0 = 0;
+    bb.x1 = bb.y1 = -1;
+    bb.zmax = __builtin_inff();
+    bb.flags = 0u;
+    bb.rconst = rconst;
+    bb.pad = 0u;
+    if (e.bad) return bb;
+    const double ew = e.amax[3] * 0x1p-20;
+    if (e.wmin - ew >= 0x1p-125 && e.wmax + ew <= 0x1p125) bb.flags |= kBoundsFastRcp;
+    const double den = e.wmin - ew;
+    if (!(den > 0.0) || width > (1 << 20) || height > (1 << 20)) return bb;
+    double d[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const double n = fmax(fabs(e.nmin[i]), fabs(e.nmax[i]));
+        const double di = (e.amax[i] * 0x1p-20 + n * ew) / den;
+        d[i] = di + (n + di) * 0x1p-22;
+        if (!(d[i] < 1e30)) return bb;
+    }
+    // fragment columns: trunc(fl(fl(n / 2 + 1 / 2) * W)) of n in [-1, 1]
+    const double W = (double)width, H = (double)height;
+    const double sx = W * 0x1p-21 + 0x1p-8, sy = H * 0x1p-21 + 0x1p-8;
+    const double lim = 0x1p30;
+    const double lx = fmin(fmax((0.5 * (e.nmin[0] - d[0]) + 0.5) * W - sx, -lim), lim);
+    const double hx = fmin(fmax((0.5 * (e.nmax[0] + d[0]) + 0.5) * W + sx, -lim), lim);
+    const double ly = fmin(fmax((0.5 * (e.nmin[1] - d[1]) + 0.5) * H - sy, -lim), lim);
+    const double hy = fmin(fmax((0.5 * (e.nmax[1] + d[1]) + 0.5) * H + sy, -lim), lim);
+    bb.x0 = (int32_t)floor(lx);
+    bb.x1 = (int32_t)floor(hx);
+    bb.y0 = (int32_t)floor(ly);
+    bb.y1 = (int32_t)floor(hy);
+    const double z = e.nmax[2] + d[2];
+    bb.zmax = (float)(z + fabs(z) * 0x1p-20 + 0x1p-60);  // rounded to a float above z
+    bb.flags |= kBoundsProven;
+    // every point passes: ndc x, y in [-1, 1 - 2^-20] (so the viewport
+    // column stays below W), ndc z in (0, 1]
+    const double top = 1.0 - 0x1p-20;
+    if (e.nmin[0] - d[0] >= -1.0 && e.nmax[0] + d[0] <= top && e.nmin[1] - d[1] >= -1.0 && e.nmax[1] + d[1] <= top &&
+        e.nmin[2] - d[2] > 0.0 && z <= 1.0)
+        bb.flags |= kBoundsInside;
+    return bb;
+}
+
+// Bounds of a batch under transform m (file.transform), all corners in one
+// thread (the cull pass).
+__host__ __device__ inline BatchBounds batch_bounds(const float* m, const scr_raster_batch& b, int width, int height,
+                                                    int rconst) {
+    float lo[3], hi[3];
+    batch_box(b, lo, hi);
+    CornerExtremes e = corner_extremes(corner_terms(m, lo[0], lo[1], lo[2]));
+    for (int k = 1; k < 8; k++)
+        e = combine(e, corner_extremes(corner_terms(m, (k & 1) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1],
+                                                    (k & 4) ? hi[2] : lo[2])));
+    return finish_bounds(e, width, height, rconst);
+}
+
 // insidePointFootprint — Common.metal:63-71
 __host__ __device__ inline bool inside_footprint(int ox, int oy, int r) {
     if (r <= 0) return ox == 0 && oy == 0;

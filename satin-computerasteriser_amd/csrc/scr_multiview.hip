@@ -72,7 +72,7 @@ __global__ __launch_bounds__(256) void k_mv_cull(const MVView* __restrict__ view
     MVItem it;
     it.view = blockIdx.y;
     it.pad[0] = it.pad[1] = it.pad[2] = 0u;
-    if (!cull_slot(V.c, V.p, slot, it.vb)) {
+    if (!cull_slot(V.c, V.p, slot, it.vb, it.bounds)) {
         it.vb.batchIndex = (uint32_t)slot;
         it.vb.level = 0;
         it.vb.lodThreshold = 0.0f;
@@ -105,8 +105,9 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_color(const MVView* __
                                                                 const float4* __restrict__ params, uint32_t* heads) {
     __shared__ DiffLds L;
     drain<kDiffThreads, kDiffPPT>(views, items, nUnits, 1, heads + 3,
-                                  [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                                      const BatchCtx bc = batch_ctx(V.c, V.p, vb, hi);
+                                  [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo,
+                                      uint32_t hi) {
+                                      const BatchCtx bc = batch_ctx(V.c, V.p, vb, bb, hi);
                                       const size_t s = (size_t)V.stride;
                                       if (!kR0 && bc.rconst == 0)
                                           diff_color_chunks<true>(V.c, V.p, bc, lo, params, V.planes, V.sums, s,
@@ -148,10 +149,11 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_backward(const MVView*
         for (int j = 0; j < kDiffPPT; j++) touched[j] = false;
         uint32_t first = 0u;
         for (int k = 0; k < nViews; k++) {
-            const MVItem it = items[(size_t)slot * (size_t)nViews + (size_t)k];
-            if (it.vb.activePoints <= lo) continue;
-            const MVView& V = views[__builtin_amdgcn_readfirstlane(it.view)];
-            const BatchCtx bc = batch_ctx(V.c, V.p, it.vb, lo + kChunk);
+            const MVItem* ip = items + ((size_t)slot * (size_t)nViews + (size_t)k);
+            const scr_visible_batch vb = ip->vb;
+            if (vb.activePoints <= lo) continue;
+            const MVView& V = views[__builtin_amdgcn_readfirstlane(ip->view)];
+            const BatchCtx bc = batch_ctx(V.c, V.p, vb, &ip->bounds, lo + kChunk);
             first = bc.first;
             const size_t s = (size_t)V.stride;
             const bool vec = vec_ok<kDiffPPT>(bc);
@@ -434,6 +436,7 @@ int stage_params(scr_multiview* mv, const scr_config* cfg, const scr_camera* cam
         V.c = c->dev();
         V.c.files = dFiles + fileOff;
         V.c.visible = nullptr;
+        V.c.bounds = nullptr;
         V.c.counter = nullptr;
         fileOff += c->files.size();
         V.planes = f->dPlanes;

@@ -32,8 +32,9 @@ struct MVItem {
     uint32_t view;
     uint32_t pad[3];
     scr_visible_batch vb;           // activePoints == 0: culled
+    BatchBounds bounds;             // what the batch box proves under this view (cull)
 };
-static_assert(sizeof(MVItem) == 32, "MVItem is two 16-byte words");
+static_assert(sizeof(MVItem) == 64, "MVItem is four 16-byte words");
 
 constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
 
@@ -61,7 +62,7 @@ __device__ inline void drain(const MVView* __restrict__ views, const MVItem* __r
         const uint32_t lo = part * per * kChunk;
         if (lo >= active) continue;
         const uint32_t hi = min(active, lo + per * kChunk);
-        body(views[__builtin_amdgcn_readfirstlane(it.view)], it.vb, lo, hi);
+        body(views[__builtin_amdgcn_readfirstlane(it.view)], it.vb, &items[item].bounds, lo, hi);
     }
 }
 
@@ -71,8 +72,8 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* 
     __shared__ uint32_t lds[kMVDepthTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
-                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       depth_batch<kR0, kWave, NT, PPT, kMVDepthTile>(V.c, V.p, vb, lo, hi, V.planes, lds, red);
+                   [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo, uint32_t hi) {
+                       depth_batch<kR0, kWave, NT, PPT, kMVDepthTile>(V.c, V.p, vb, bb, lo, hi, V.planes, lds, red);
                    });
 }
 
@@ -84,10 +85,10 @@ __global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(c
     __shared__ unsigned long long ldsBC[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
-                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
+                   [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo, uint32_t hi) {
                        const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
                        const size_t s = (size_t)V.stride;
-                       color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
+                       color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, bb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
                                                  V.planes + 3 * s, V.planes + 4 * s, L);
                    });
 }
@@ -99,8 +100,8 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView
     __shared__ unsigned long long lds[kMVNearestTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
-                   [&](const MVView& V, const scr_visible_batch& vb, uint32_t lo, uint32_t hi) {
-                       nearest_batch<kR0, kWave, NT, PPT, kMVNearestTile>(V.c, V.p, vb, lo, hi, V.nearest, lds, red);
+                   [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo, uint32_t hi) {
+                       nearest_batch<kR0, kWave, NT, PPT, kMVNearestTile>(V.c, V.p, vb, bb, lo, hi, V.nearest, lds, red);
                    });
 }
 

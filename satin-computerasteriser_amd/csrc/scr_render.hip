@@ -27,9 +27,11 @@ __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= c.nSlots) return;
     scr_visible_batch vb;
-    if (!cull_slot(c, p, gid, vb)) return;
+    BatchBounds bb;
+    if (!cull_slot(c, p, gid, vb, bb)) return;
     const uint32_t slot = atomicAdd(c.counter, 1u);
     c.visible[slot] = vb;
+    c.bounds[slot] = bb;
 }
 
 // The draw kernels are persistent grids: workgroup b takes visible batches
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_depth(CloudDev c
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        depth_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, depth, lds, red);
+        depth_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], c.bounds + v, 0u, 0xffffffffu, depth, lds, red);
 }
 
 template <bool kR0>
@@ -55,7 +57,7 @@ __global__ __launch_bounds__(kThreads, kR0 ? 4 : 3) void k_color(CloudDev c, Pas
     const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        color_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, depth, red, green, blue, count, L);
+        color_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], c.bounds + v, 0u, 0xffffffffu, depth, red, green, blue, count, L);
 }
 
 __global__ __launch_bounds__(256) void k_resolve(PassParams p, const uint32_t* __restrict__ planes, size_t stride,
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_nearest(CloudDev
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        nearest_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], 0u, 0xffffffffu, packed, lds, red);
+        nearest_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], c.bounds + v, 0u, 0xffffffffu, packed, lds, red);
 }
 
 __global__ __launch_bounds__(256) void k_nearest_resolve(PassParams p, const unsigned long long* __restrict__ packed,
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(256) void k_depth_coverage(CloudDev c, PassParams p
     __shared__ int redbuf[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
-        const BatchCtx bc = batch_ctx(c, p, c.visible[v], 0xffffffffu);
+        const BatchCtx bc = batch_ctx(c, p, c.visible[v], c.bounds + v, 0xffffffffu);
         const ChunkBox seg = segment_box(p, bc, bc.active, kCoverageDepthTile);
         const int area = seg.w * seg.h;
         if (seg.lds) {
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x) {
         const scr_visible_batch vb = c.visible[v];
-        const BatchCtx bc = batch_ctx(c, p, vb, 0xffffffffu);
+        const BatchCtx bc = batch_ctx(c, p, vb, c.bounds + v, 0xffffffffu);
         const scr_raster_file* file = c.files + c.batches[vb.batchIndex].fileIndex;
         const ChunkBox seg = segment_box(p, bc, bc.active, kColorTile, reach);
         if (seg.lds) {

@@ -272,6 +272,38 @@ def test_pick_empty_space_returns_none(ctx):
     assert ras.pickPointIndex((0.95, 0.95), cloud, cam) is None
 
 
+@pytest.mark.parametrize("offset", [5e5, 2e4])
+def test_far_offset_close_camera_matches_oracle(ctx, offset):
+    """A batch far from the origin seen from about a unit away: the
+    transform's partial sums dwarf the clip w, so each point's float
+    projection strays from the exact projection of the batch box by pixels.
+    An occluder in front covers the middle of the far batch; the colour pass
+    may skip a batch only where the cull pass's bounds prove every fragment
+    hidden, so the frame must equal the oracle's."""
+    o = np.float32(offset)
+    g = np.linspace(0.0, 1.0, 24, dtype=np.float32)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    far = np.stack([X.ravel() + o, Y.ravel() + o, Z.ravel()], 1).astype(np.float32)
+    h = np.linspace(0.15, 0.85, 70, dtype=np.float32)
+    A, B = np.meshgrid(h, h, indexing="ij")
+    occ = np.stack([A.ravel() + o, np.full(A.size, o - np.float32(0.3), np.float32), B.ravel()], 1)
+    col_far = np.tile(np.array([[0.0, 0.2, 1.0, 1.0]], np.float32), (len(far), 1))
+    col_occ = np.tile(np.array([[1.0, 0.1, 0.0, 1.0]], np.float32), (len(occ), 1))
+    packs = [orc.pack(far, col_far, pointsPerBatch=4096), orc.pack(occ, col_occ, pointsPerBatch=4096)]
+    cam = scr.Camera.perspective((o + 0.5, o - 1.4, 0.55), (o + 0.5, o + 0.5, 0.5), (0.0, 0.0, 1.0), fov=70,
+                                 aspect=16 / 9, near=0.05, far=50.0)
+    for sizes in (dict(), dict(minimumPointSize=1, maximumPointSize=4, pointSizeScale=3.0)):
+        cfg = scr.ComputeRasteriserConfiguration(enableCLOD=False, **sizes)
+        gpu = render_gpu_hqs(ctx, [gpu_cloud(ctx, p) for p in packs], cfg, cam, 640, 360)
+        ref = orc.render_hqs([oracle_cloud(p) for p in packs], cfg, cam.viewMatrix, cam.projectionMatrix, 640, 360)
+        assert ref["count"].sum() > 1000 and (ref["rgba8"] & 0xFF == 255).sum() > 100  # the occluder is seen
+        assert_planes_equal(gpu, ref)
+        cfg.mode = scr.MODE_NEAREST_POINT
+        gpu = render_gpu_nearest(ctx, [gpu_cloud(ctx, packs[0])], cfg, cam, 640, 360)
+        ref = orc.render_nearest(oracle_cloud(packs[0]), cfg, cam.viewMatrix, cam.projectionMatrix, 640, 360)
+        assert_planes_equal(gpu, ref)
+
+
 def test_pick_rejects_a_null_configuration(ctx):
     import ctypes as C
     packed = orc.pack([(0.0, 0.0, 0.0)], [(1, 1, 1, 1)])

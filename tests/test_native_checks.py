@@ -22,3 +22,19 @@ def test_batch_radius_bound_never_misnames_a_point(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "hoisted" in out.stdout
+
+
+@pytest.mark.skipif(not Path(HIPCC).exists(), reason="hipcc not installed")
+def test_batch_bounds_hold_for_every_point(tmp_path):
+    """batch_bounds() (the cull pass's proof of a batch's fragment rectangle,
+    depth bound, clip/viewport pass and reciprocal range) against the draw
+    passes' exact per-point arithmetic, on ordinary and far-offset/close-camera
+    batches; without its rounding allowance the same cases must fail."""
+    exe = tmp_path / "batch_bounds_check"
+    src = REPO / "tests" / "native" / "batch_bounds_check.cpp"
+    subprocess.run([HIPCC, "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off",
+                    "-fno-fast-math", "-I", str(REPO / "include"), "-I", str(REPO / "satin-computerasteriser_amd" / "csrc"),
+                    str(src), "-o", str(exe)], check=True, capture_output=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "bounds hold" in out.stdout

@@ -62,12 +62,28 @@ def build() -> Path:
     return LIB_PATH
 
 
+def use_library(path: Optional[Path] = None) -> Optional[Path]:
+    """Serve every later call from the oracle library at ``path`` (a build of
+    scr_oracle.c with other compile flags, e.g. a planted mistake for the
+    mutation test), or from the default build with None; returns the path in
+    use before."""
+    global _lib, _path
+    prev = _path
+    _lib = None
+    _path = None if path is None else Path(path)
+    return prev
+
+
+_path: Optional[Path] = None
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
+        path = _path or LIB_PATH
+        if path == LIB_PATH and not LIB_PATH.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
+        L = C.CDLL(str(path))
         f = C.c_float
         u32 = C.c_uint32
         vp = C.c_void_p
@@ -80,6 +96,9 @@ def lib() -> C.CDLL:
             "orc_morton30": (u32, [u32, u32, u32]), "orc_morton63": (C.c_uint64, [u32, u32, u32]),
             "orc_inside_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int]),
             "orc_precision_level": (C.c_int, [f]), "orc_lod_threshold": (f, [f, C.c_int]),
+            "orc_intersects_frustum": (C.c_int, [vp, vp, vp]),
+            "orc_pixel_size_on_screen": (f, [vp, vp, vp, vp, C.c_int, C.c_int]),
+            "orc_footprint_radius": (C.c_int, [vp, vp, vp, vp, C.c_int, C.c_int, vp]),
             "orc_update_files": (None, [vp, C.c_int, vp, vp, vp, vp, vp]),
             "orc_cull": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp]),
             "orc_render_hqs": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.POINTER(OrcPlanes), C.c_int]),
@@ -171,6 +190,43 @@ def lod_threshold(px: float, bias: int) -> float:
     return lib().orc_lod_threshold(px, bias)
 
 
+def intersects_frustum(m, mn, mx) -> bool:
+    """intersectsFrustum (Common.metal:119-153) of the box [mn, mx] under m."""
+    m, mn, mx = _f32(m), _f32(mn), _f32(mx)   # held until the call returns
+    return bool(lib().orc_intersects_frustum(m.ctypes.data, mn.ctypes.data, mx.ctypes.data))
+
+
+def _file(world=None) -> np.ndarray:
+    f = identity_files(1)
+    if world is not None:
+        f["world"] = _f32(world).reshape(4, 4)
+    return f
+
+
+def pixel_size_on_screen(mn, mx, view, proj, width: int, height: int, world=None) -> float:
+    """pixelSizeOnScreen (Common.metal:155-172) of the batch box [mn, mx]."""
+    b = np.zeros(1, RASTER_BATCH_DTYPE)
+    b["minX"], b["minY"], b["minZ"] = _f32(mn)
+    b["maxX"], b["maxY"], b["maxZ"] = _f32(mx)
+    f = _file(world)
+    view, proj = _f32(view), _f32(proj)
+    return lib().orc_pixel_size_on_screen(b.ctypes.data, f.ctypes.data, view.ctypes.data, proj.ctypes.data, width,
+                                          height)
+
+
+def precision_level(px: float) -> int:
+    return lib().orc_precision_level(px)
+
+
+def footprint_radius(point, cfg, view, proj, width: int, height: int, world=None) -> int:
+    """pointFootprintRadius (Common.metal:18-61) of a decoded point."""
+    f = _file(world)
+    c = config_from(cfg)
+    point, view, proj = _f32(point), _f32(view), _f32(proj)
+    return lib().orc_footprint_radius(point.ctypes.data, f.ctypes.data, view.ctypes.data, proj.ctypes.data, width,
+                                      height, C.addressof(c))
+
+
 # ---------------------------------------------------------------- fixtures
 def cube_grid(n: int, extent: float = 1.0):
     """cubeGrid lattice (positions (N,3) float32, colors (N,4) float32)."""
@@ -244,7 +300,8 @@ def morton_keys(positions, keyBits: int = 30) -> np.ndarray:
 def lod_levels(sorted_positions, bmin, bmax, lodLevels: int = 4, coarse: int = 64) -> np.ndarray:
     sp = _f32(sorted_positions).reshape(-1, 3)
     out = np.zeros(sp.shape[0], np.uint8)
-    lib().orc_lod_levels(sp.ctypes.data, sp.shape[0], _f32(bmin).ctypes.data, _f32(bmax).ctypes.data, lodLevels,
+    bmin, bmax = _f32(bmin), _f32(bmax)
+    lib().orc_lod_levels(sp.ctypes.data, sp.shape[0], bmin.ctypes.data, bmax.ctypes.data, lodLevels,
                          coarse, out.ctypes.data)
     return out
 
@@ -295,8 +352,8 @@ class Cloud:
         """Apply updateFiles(viewProjection, model, prevViewProjection) to a copy."""
         files = np.zeros_like(self.files)
         prev = None if prev_vp is None else _f32(prev_vp)
-        lib().orc_update_files(self.files.ctypes.data, len(self.files), _f32(vp).ctypes.data,
-                               _f32(self.model).ctypes.data, None, None if prev is None else prev.ctypes.data,
+        vp, model = _f32(vp), _f32(self.model)
+        lib().orc_update_files(self.files.ctypes.data, len(self.files), vp.ctypes.data, model.ctypes.data, None, None if prev is None else prev.ctypes.data,
                                files.ctypes.data)
         c = Cloud(self.batches, files, self.xyzLow, self.xyzMed, self.xyzHigh, self.colors, self.levels, self.tints)
         c.model = self.model
@@ -341,12 +398,13 @@ def config_from(cfg) -> Config:
 
 
 def cull(cloud: Cloud, cfg, view, proj, width: int, height: int) -> np.ndarray:
+    view, proj = _f32(view), _f32(proj)   # held until the calls return
     vp = mat_mul(proj, view)
     cc = cloud.composed(vp)
     s = cc.c_struct()
     out = np.zeros(max(1, len(cc.batches)), VISIBLE_BATCH_DTYPE)
     c = config_from(cfg)
-    n = lib().orc_cull(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width, height,
+    n = lib().orc_cull(C.byref(s), C.addressof(c), view.ctypes.data, proj.ctypes.data, width, height,
                        out.ctypes.data)
     return out[:n]
 
@@ -355,6 +413,7 @@ def render_hqs(clouds: Sequence[Cloud], cfg, view, proj, width: int, height: int
                prev_view_proj=None) -> dict:
     """HQS frame: returns dict of planes (H, W).  prev_view_proj is the
     previous frame's projection . view (motion blur; default: this frame's)."""
+    view, proj = _f32(view), _f32(proj)   # held until the calls return
     vp = mat_mul(proj, view)
     composed = [c.composed(vp, prev_view_proj) for c in clouds]
     structs = (OrcCloud * max(1, len(composed)))(*[c.c_struct() for c in composed])
@@ -363,14 +422,15 @@ def render_hqs(clouds: Sequence[Cloud], cfg, view, proj, width: int, height: int
     planes["depthF32"] = np.zeros(n, np.float32)
     pl = OrcPlanes(*[planes[k].ctypes.data for k in ("depth", "red", "green", "blue", "count", "rgba8", "depthF32")])
     c = config_from(cfg)
-    rc = lib().orc_render_hqs(C.addressof(structs), len(composed), C.addressof(c), _f32(view).ctypes.data,
-                              _f32(proj).ctypes.data, width, height, C.byref(pl), threads)
+    rc = lib().orc_render_hqs(C.addressof(structs), len(composed), C.addressof(c), view.ctypes.data,
+                              proj.ctypes.data, width, height, C.byref(pl), threads)
     if rc != 0:
         raise RuntimeError("orc_render_hqs failed")
     return {k: v.reshape(height, width) for k, v in planes.items()}
 
 
 def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int) -> dict:
+    view, proj = _f32(view), _f32(proj)   # held until the calls return
     vp = mat_mul(proj, view)
     cc = cloud.composed(vp)
     s = cc.c_struct()
@@ -380,7 +440,7 @@ def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int) -> di
     rgba = np.zeros(n, np.uint32)
     df = np.zeros(n, np.float32)
     c = config_from(cfg)
-    lib().orc_render_nearest(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width,
+    lib().orc_render_nearest(C.byref(s), C.addressof(c), view.ctypes.data, proj.ctypes.data, width,
                              height, d.ctypes.data, idx.ctypes.data, rgba.ctypes.data, df.ctypes.data)
     return {"depth": d.reshape(height, width), "index": idx.reshape(height, width),
             "rgba8": rgba.reshape(height, width), "depthF32": df.reshape(height, width)}
@@ -389,6 +449,7 @@ def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int) -> di
 def render_diff(cloud: Cloud, cfg, view, proj, width: int, height: int, params, threads: int = 1) -> dict:
     """Differentiable HQS forward: depth, count (H, W); sums int64 and image
     float32 (H, W, 3).  params: float32 (poolPoints, 4), rgb used."""
+    view, proj = _f32(view), _f32(proj)   # held until the calls return
     vp = mat_mul(proj, view)
     cc = cloud.composed(vp)
     s = cc.c_struct()
@@ -399,7 +460,7 @@ def render_diff(cloud: Cloud, cfg, view, proj, width: int, height: int, params, 
     sums = np.zeros(n * 3, np.int64)
     img = np.zeros(n * 3, np.float32)
     c = config_from(cfg)
-    rc = lib().orc_render_diff(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width,
+    rc = lib().orc_render_diff(C.byref(s), C.addressof(c), view.ctypes.data, proj.ctypes.data, width,
                                height, prm.ctypes.data, d.ctypes.data, cnt.ctypes.data, sums.ctypes.data,
                                img.ctypes.data, threads)
     if rc != 0:
@@ -411,6 +472,7 @@ def render_diff(cloud: Cloud, cfg, view, proj, width: int, height: int, params, 
 def diff_backward(cloud: Cloud, cfg, view, proj, width: int, height: int, depth, count, dLdImage, grad,
                   mask=None) -> None:
     """FP64 reference gradient: grad (poolPoints, 4) float64 += per-point sums."""
+    view, proj = _f32(view), _f32(proj)   # held until the calls return
     vp = mat_mul(proj, view)
     cc = cloud.composed(vp)
     s = cc.c_struct()
@@ -420,7 +482,7 @@ def diff_backward(cloud: Cloud, cfg, view, proj, width: int, height: int, depth,
     g = np.ascontiguousarray(dLdImage, np.float32)
     m = None if mask is None else np.ascontiguousarray(mask, np.uint8)
     c = config_from(cfg)
-    rc = lib().orc_diff_backward(C.byref(s), C.addressof(c), _f32(view).ctypes.data, _f32(proj).ctypes.data, width,
+    rc = lib().orc_diff_backward(C.byref(s), C.addressof(c), view.ctypes.data, proj.ctypes.data, width,
                                  height, d.ctypes.data, cnt.ctypes.data, g.ctypes.data, grad.ctypes.data, _p(m))
     if rc != 0:
         raise RuntimeError("orc_diff_backward failed")
@@ -428,15 +490,15 @@ def diff_backward(cloud: Cloud, cfg, view, proj, width: int, height: int, depth,
 
 def nearest_cpu_reference(cloud: Cloud, width: int, height: int, view=None, proj=None,
                           enableFrustumCulling: bool = False) -> dict:
-    view = np.eye(4, dtype=np.float32) if view is None else view
-    proj = np.eye(4, dtype=np.float32) if proj is None else proj
+    view = np.eye(4, dtype=np.float32) if view is None else _f32(view)
+    proj = np.eye(4, dtype=np.float32) if proj is None else _f32(proj)
     vp = mat_mul(proj, view)
     cc = cloud.composed(vp)
     s = cc.c_struct()
     n = max(width, 0) * max(height, 0)
     d = np.zeros(max(n, 1), np.uint32)
     idx = np.zeros(max(n, 1), np.uint32)
-    lib().orc_nearest_cpu_reference(C.byref(s), width, height, _f32(view).ctypes.data, _f32(proj).ctypes.data,
+    lib().orc_nearest_cpu_reference(C.byref(s), width, height, view.ctypes.data, proj.ctypes.data,
                                     int(enableFrustumCulling), d.ctypes.data, idx.ctypes.data)
     return {"depths": d[:n], "indices": idx[:n]}
 

@@ -16,6 +16,14 @@
 #define ORC_INV_TWO32 2.3283064365386963e-10f /* 2^-32, exact */
 #define ORC_INV255 (1.0f / 255.0f)
 
+/* Planted mistakes, compiled in only by the mutation test
+ * (tests/test_oracle_mutants.py builds this file with -DORC_MUTANT=k and
+ * requires a pin of tests/test_oracle_pinning.py to fail for every k).  The
+ * default, 0, is the oracle itself. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* ------------------------------------------------------------------ C4 */
 uint32_t orc_f2u(float x) {
     if (!(x > 0.0f)) return 0u;                 /* NaN, <= 0 */
@@ -194,7 +202,11 @@ int orc_footprint_radius(const float p[3], const scr_raster_file* file, const fl
         if (ortho) {
             ps = (2.0f * cfg->pointSizeScale) * focal;
         } else {
+#if ORC_MUTANT == 10
+            const float viewZ = fmaxf(orc_len3(vp[0], vp[1], vp[2]), 0.000001f);
+#else
             const float viewZ = fmaxf(-vp[2], 0.000001f);
+#endif
             ps = ((2.0f * cfg->pointSizeScale) * focal) * orc_rcp(viewZ);
         }
     } else {
@@ -207,7 +219,11 @@ int orc_footprint_radius(const float p[3], const scr_raster_file* file, const fl
     }
     if (!isfinite(ps)) ps = lo;
     ps = fminf(fmaxf(ps, lo), hi);
+#if ORC_MUTANT == 2
+    int r = orc_f2i(floorf(fmaxf(ps - 1.0f, 0.0f) * 0.5f));
+#else
     int r = orc_f2i(ceilf(fmaxf(ps - 1.0f, 0.0f) * 0.5f));
+#endif
     if (r < 0) r = 0;
     if (r > 16) r = 16;
     return r;
@@ -269,12 +285,17 @@ static void orc_make_plane(float x, float y, float z, float w, float n[3], float
 int orc_intersects_frustum(const float* m, const float mn[3], const float mx[3]) {
 #define M(c, r) m[(c) * 4 + (r)]
     const float pl[6][4] = {
-        {M(0, 3) - M(0, 0), M(1, 3) - M(1, 0), M(2, 3) - M(2, 0), M(3, 3) - M(3, 0)},
+        {M(0, 3) - M(0, ORC_MUTANT == 1), M(1, 3) - M(1, ORC_MUTANT == 1), M(2, 3) - M(2, ORC_MUTANT == 1),
+         M(3, 3) - M(3, ORC_MUTANT == 1)},
         {M(0, 3) + M(0, 0), M(1, 3) + M(1, 0), M(2, 3) + M(2, 0), M(3, 3) + M(3, 0)},
         {M(0, 3) + M(0, 1), M(1, 3) + M(1, 1), M(2, 3) + M(2, 1), M(3, 3) + M(3, 1)},
         {M(0, 3) - M(0, 1), M(1, 3) - M(1, 1), M(2, 3) - M(2, 1), M(3, 3) - M(3, 1)},
         {M(0, 3) - M(0, 2), M(1, 3) - M(1, 2), M(2, 3) - M(2, 2), M(3, 3) - M(3, 2)},
+#if ORC_MUTANT == 11
+        {M(0, 3) + M(0, 2), M(1, 3) + M(1, 2), M(2, 3) + M(2, 2), M(3, 3) + M(3, 2)},
+#else
         {M(0, 2), M(1, 2), M(2, 2), M(3, 2)},
+#endif
     };
 #undef M
     for (int i = 0; i < 6; i++) {
@@ -313,7 +334,7 @@ float orc_pixel_size_on_screen(const scr_raster_batch* b, const scr_raster_file*
     pc[1] = pc[1] * rc;
     pe[0] = pe[0] * re;
     pe[1] = pe[1] * re;
-    const float sz[2] = {(float)width, (float)height};
+    const float sz[2] = {(float)(ORC_MUTANT == 4 ? height : width), (float)height};
     float sc[2], se[2];
     for (int i = 0; i < 2; i++) {
         sc[i] = (sz[i] * (pc[i] + 1.0f)) * 0.5f;
@@ -325,8 +346,8 @@ float orc_pixel_size_on_screen(const scr_raster_batch* b, const scr_raster_file*
 
 /* precisionLevel — Common.metal:174-181 / Cull/Shaders.metal:47-53 */
 int orc_precision_level(float px) {
-    if (px < 100.0f) return 4;
-    if (px < 200.0f) return 3;
+    if (px < (ORC_MUTANT == 8 ? 200.0f : 100.0f)) return 4;
+    if (px < (ORC_MUTANT == 8 ? 100.0f : 200.0f)) return 3;
     if (px < 500.0f) return 2;
     if (px < 10000.0f) return 1;
     return 0;
@@ -334,7 +355,7 @@ int orc_precision_level(float px) {
 
 /* lodThresholdFromPixelSize — Common.metal:185-187 */
 float orc_lod_threshold(float px, int lodBias) {
-    return (orc_log2f(fmaxf(px, 1.0f) * (1.0f / 25.0f)) + 3.0f) + (float)lodBias;
+    return (orc_log2f(fmaxf(px, 1.0f) * (1.0f / 25.0f)) + (ORC_MUTANT == 5 ? 0.0f : 3.0f)) + (float)lodBias;
 }
 
 /* ------------------------------------------------------------------ host composition */
@@ -528,13 +549,14 @@ static void orc_blend_fragment(const orc_job* j, const scr_raster_file* f, const
             }
         }
     }
-    const float aN = a * orc_rcp((float)samples);
+    const float aN = ORC_MUTANT == 6 ? a : a * orc_rcp((float)samples);
     const uint32_t cr = orc_f2u((((float)r * ORC_INV255) * aN) * 4096.0f + 0.5f);
     const uint32_t cg = orc_f2u((((float)g * ORC_INV255) * aN) * 4096.0f + 0.5f);
     const uint32_t cb = orc_f2u((((float)bl * ORC_INV255) * aN) * 4096.0f + 0.5f);
     const uint32_t ca = orc_f2u(aN * 4096.0f + 0.5f);
     for (int s = 0; s < samples; s++) {
-        const int cx = (int32_t)((uint32_t)fr->px - (uint32_t)orc_f2i(roundf(stepX * (float)s)));
+        const int cx = ORC_MUTANT == 9 ? (int32_t)((uint32_t)fr->px + (uint32_t)orc_f2i(roundf(stepX * (float)s)))
+                                        : (int32_t)((uint32_t)fr->px - (uint32_t)orc_f2i(roundf(stepX * (float)s)));
         const int cy = (int32_t)((uint32_t)fr->py - (uint32_t)orc_f2i(roundf(stepY * (float)s)));
         for (int oy = -fr->radius; oy <= fr->radius; oy++)
             for (int ox = -fr->radius; ox <= fr->radius; ox++) {
@@ -728,7 +750,7 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
                 continue;
             }
             const float inv = orc_rcp((float)a);
-            const float sumA = (float)a * (1.0f / 4096.0f);
+            const float sumA = (float)a * (ORC_MUTANT == 7 ? 1.0f : 1.0f / 4096.0f);
             const float alpha = orc_saturate(1.0f - orc_expf(-sumA));
             if (out->rgba8)
                 out->rgba8[p] = orc_pack_rgba((float)out->red[p] * inv, (float)out->green[p] * inv,
@@ -787,7 +809,11 @@ int orc_render_nearest(const orc_cloud* c, const scr_config* cfg, const float* v
                         if (pass == 0) {
                             if (fr.depth > depths[p]) depths[p] = fr.depth;
                         } else if (fr.depth == depths[p]) {
+#if ORC_MUTANT == 3
+                            if (indices[p] == 0xffffffffu || pi > indices[p]) indices[p] = pi;
+#else
                             if (pi < indices[p]) indices[p] = pi;
+#endif
                         }
                     }
             }

@@ -338,3 +338,296 @@ def test_expf_is_exp_rounded_once():
     assert orc.expf(0.0) == 1.0 and orc.expf(float("-inf")) == 0.0 and orc.expf(float("inf")) == float("inf")
     assert math.isnan(orc.expf(float("nan"))) and orc.expf(-120.0) == 0.0 and orc.expf(89.0) == float("inf")
     assert orc.expf(1.0) == np.float32(math.e)
+
+
+# ---------------------------------------------------------------- cull geometry
+# The pins below are module-level functions so that the mutation test
+# (test_oracle_mutants.py) can run them against oracles with planted mistakes.
+
+def _look_at(eye, target, up=(0.0, 1.0, 0.0)):
+    return scr.look_at(eye, target, up)
+
+
+# View-space boxes for a camera at the origin looking down -Z.  Perspective:
+# fov 90 deg, aspect 2, near 1, far 100, so the frustum is |x| <= -2z,
+# |y| <= -z, -100 <= z <= -1 (an x/y mix-up fails the 12..15 cases).
+PERSPECTIVE_BOXES = [
+    ((-0.5, -0.5, -10.5), (0.5, 0.5, -9.5), True),
+    ((-30.0, -0.5, -10.5), (-25.0, 0.5, -9.5), False),   # left of the frustum
+    ((-22.0, -0.5, -10.5), (-18.0, 0.5, -9.5), True),    # across the left plane
+    ((25.0, -0.5, -10.5), (30.0, 0.5, -9.5), False),     # right
+    ((18.0, -0.5, -10.5), (22.0, 0.5, -9.5), True),
+    ((-0.5, -15.0, -10.5), (0.5, -12.0, -9.5), False),   # below
+    ((-0.5, -11.0, -10.5), (0.5, -9.0, -9.5), True),
+    ((-0.5, 12.0, -10.5), (0.5, 15.0, -9.5), False),     # above
+    ((-0.5, 9.0, -10.5), (0.5, 11.0, -9.5), True),
+    ((12.0, -0.5, -10.5), (15.0, 0.5, -9.5), True),      # inside only because the frustum is twice as wide
+    ((-0.1, -0.1, -0.8), (0.1, 0.1, -0.2), False),       # before the near plane
+    ((-0.1, -0.1, -1.5), (0.1, 0.1, -0.5), True),
+    ((-1.0, -1.0, -150.0), (1.0, 1.0, -120.0), False),   # past the far plane
+    ((-1.0, -1.0, -110.0), (1.0, 1.0, -90.0), True),
+    ((-1.0, -1.0, 5.0), (1.0, 1.0, 10.0), False),        # behind the camera
+]
+# Orthographic, 10 high, aspect 1, near 1, far 50: |x|, |y| <= 5, -50 <= z <= -1.
+ORTHO_BOXES = [
+    ((-1.0, -1.0, -20.0), (1.0, 1.0, -10.0), True),
+    ((-9.0, -1.0, -20.0), (-6.0, 1.0, -10.0), False),
+    ((-6.0, -1.0, -20.0), (-4.0, 1.0, -10.0), True),
+    ((6.0, -1.0, -20.0), (9.0, 1.0, -10.0), False),
+    ((-1.0, 6.0, -20.0), (1.0, 9.0, -10.0), False),
+    ((-1.0, -9.0, -20.0), (1.0, -6.0, -10.0), False),
+    ((-1.0, -1.0, -1.5), (1.0, 1.0, -1.2), True),        # just past the near plane
+    ((-1.0, -1.0, -0.9), (1.0, 1.0, -0.1), False),
+    ((-1.0, -1.0, -80.0), (1.0, 1.0, -60.0), False),     # past the far plane
+    ((-1.0, -1.0, -55.0), (1.0, 1.0, -45.0), True),
+]
+
+
+def pin_frustum():
+    """intersectsFrustum (Common.metal:119-153): boxes inside, outside and
+    across each of the six planes, perspective and orthographic, also with
+    the camera moved (the planes' translation column)."""
+    P = scr.perspective_reverse_z(90.0, 2.0, 1.0, 100.0)
+    O = scr.orthographic_reverse_z(10.0, 1.0, 1.0, 50.0)
+    for proj, boxes in ((P, PERSPECTIVE_BOXES), (O, ORTHO_BOXES)):
+        for shift in (0.0, 100.0):
+            V = _look_at((shift, 0.0, 0.0), (shift, 0.0, -1.0))
+            m = orc.mat_mul(proj, V)
+            for mn, mx, want in boxes:
+                lo = (mn[0] + shift, mn[1], mn[2])
+                hi = (mx[0] + shift, mx[1], mx[2])
+                assert orc.intersects_frustum(m, lo, hi) == want, (proj[3, 3], shift, mn, mx)
+
+
+def test_frustum_planes():
+    pin_frustum()
+
+
+def pin_pixel_size_and_precision():
+    """pixelSizeOnScreen (Common.metal:155-172): the box centre and the
+    centre moved by the box diagonal R along view x share clip w, so the
+    distance is W/2 * P00 * R / w (perspective w = view depth, orthographic
+    w = 1), for any centre; precisionLevel (:174-181) cuts at 100, 200, 500
+    and 10,000 px."""
+    rng = np.random.default_rng(11)
+    for fov, W, H in ((90.0, 1000, 1000), (50.0, 1920, 1080), (20.0, 640, 360)):
+        P = scr.perspective_reverse_z(fov, W / H, 0.1, 1e5)
+        for _ in range(40):
+            d = float(rng.uniform(2.0, 400.0))
+            c = np.array([rng.uniform(-0.3, 0.3) * d, rng.uniform(-0.3, 0.3) * d, -d])
+            half = rng.uniform(0.05, 3.0, 3)
+            mn, mx = (c - half).astype(np.float32), (c + half).astype(np.float32)
+            R = float(np.linalg.norm(mx.astype(np.float64) - mn.astype(np.float64)))
+            cz = (float(mn[2]) + float(mx[2])) / 2.0
+            want = W / 2.0 * float(P[0, 0]) * R / -cz
+            got = orc.pixel_size_on_screen(mn, mx, np.eye(4, dtype=np.float32), P, W, H)
+            assert abs(got - want) <= 1e-4 * want + 1e-3, (fov, W, H, got, want)
+    O = scr.orthographic_reverse_z(8.0, 2.0, 0.5, 100.0)
+    mn, mx = np.float32([-1.0, -2.0, -10.0]), np.float32([2.0, 2.0, -8.0])
+    got = orc.pixel_size_on_screen(mn, mx, np.eye(4, dtype=np.float32), O, 800, 400)
+    assert abs(got - 400.0 * float(O[0, 0]) * float(np.linalg.norm(mx - mn))) < 1e-3
+    # precision level around each threshold
+    for px, level in ((0.0, 4), (99.99, 4), (100.0, 3), (199.99, 3), (200.0, 2), (499.9, 2), (500.0, 1),
+                      (9999.0, 1), (10000.0, 0), (1e9, 0)):
+        assert orc.precision_level(px) == level, (px, level)
+    # the same through pixelSizeOnScreen: an on-axis box of diagonal R at depth d
+    P = scr.perspective_reverse_z(90.0, 1.0, 0.1, 1e5)      # P00 = 1: px = 500 R / d at W = 1000
+    for want_px, level in ((99.0, 4), (101.0, 3), (199.0, 3), (201.0, 2), (499.0, 2), (501.0, 1),
+                           (9990.0, 1), (10010.0, 0)):
+        R = want_px * 4.0 / 500.0
+        mn, mx = np.float32([-R / 2, 0.0, -4.0]), np.float32([R / 2, 0.0, -4.0])
+        px = orc.pixel_size_on_screen(mn, mx, np.eye(4, dtype=np.float32), P, 1000, 1000)
+        assert abs(px - want_px) < 0.05 and orc.precision_level(px) == level, (want_px, px)
+
+
+def test_pixel_size_and_precision_level():
+    pin_pixel_size_and_precision()
+
+
+def pin_lod_threshold():
+    """lodThresholdFromPixelSize (Common.metal:185-187): log2(max(px, 1) / 25)
+    + 3 + bias, so 25 * 2^k px gives exactly k + 3 + bias; and the cull's use
+    of it (Cull/Shaders.metal:55-74): CLOD off gives 99 and every point,
+    CLOD on gives max(0, threshold) and the prefix cum[clamp(floor(t + 0.5),
+    0, 7)] of the batch's level counts."""
+    for k in range(-4, 12):
+        for bias in (-2, 0, 3):
+            assert orc.lod_threshold(25.0 * 2.0 ** k, bias) == float(k + 3 + bias), (k, bias)
+    assert orc.lod_threshold(0.25, 0) == orc.lod_threshold(1.0, 0)       # max(px, 1)
+    assert abs(orc.lod_threshold(1.0, 0) - (math.log2(1 / 25) + 3)) < 1e-6
+    # one batch of 100 points with level counts cum = 1, 5, 5, 40, 60, 75, 90, 100
+    cum = [1, 5, 5, 40, 60, 75, 90, 100]
+    levels = np.repeat(np.arange(8, dtype=np.uint8), np.diff([0] + cum))
+    pos = np.zeros((100, 3), np.float32)
+    pos[:, 0] = np.linspace(-0.1, 0.1, 100, dtype=np.float32)
+    packed = orc.pack(pos, np.ones((100, 4), np.float32), pointsPerBatch=100, lodLevels=1)
+    cloud = orc.Cloud.from_packed(packed)
+    cloud.levels[:100] = levels
+    b = cloud.batches
+    b["lodCum01"], b["lodCum23"] = cum[0] | (cum[1] << 16), cum[2] | (cum[3] << 16)
+    b["lodCum45"], b["lodCum67"] = cum[4] | (cum[5] << 16), cum[6] | (cum[7] << 16)
+    P = scr.perspective_reverse_z(90.0, 1.0, 0.1, 1e4)
+    box_r = float(np.linalg.norm([b["maxX"][0] - b["minX"][0], b["maxY"][0] - b["minY"][0],
+                                  b["maxZ"][0] - b["minZ"][0]]))
+    for k, bias in ((2, 0), (0, 0), (1, 1), (4, 2), (-3, -1)):
+        # a camera at distance d along +z sees the batch at 25 * 2^k px (W = 1000, P00 = 1)
+        d = 500.0 * box_r / (25.0 * 2.0 ** k)
+        V = _look_at((0.0, 0.0, d), (0.0, 0.0, 0.0))
+        for clod in (True, False):
+            cfg = scr.ComputeRasteriserConfiguration(enableCLOD=clod, lodBias=bias, enableFrustumCulling=False)
+            vb = orc.cull(cloud, cfg, V, P, 1000, 1000)
+            assert len(vb) == 1
+            if not clod:
+                assert vb["lodThreshold"][0] == 99.0 and vb["activePoints"][0] == 100
+                continue
+            t = max(0.0, k + 3 + bias)
+            assert abs(vb["lodThreshold"][0] - t) < 1e-3, (k, bias, vb["lodThreshold"][0])
+            lmax = min(max(math.floor(t + 0.5), 0), 7)
+            assert vb["activePoints"][0] == cum[lmax], (k, bias, lmax)
+
+
+def test_lod_threshold_and_survivor_prefix():
+    pin_lod_threshold()
+
+
+def pin_footprint_radius():
+    """pointFootprintRadius (Common.metal:18-61) against hand-computed radii:
+    size = scale / |view position| (screen space) or 2 * scale * focal /
+    view depth (world space, focal = H / 2 * P11), or scale / 2 * scale *
+    focal under an orthographic projection; non-finite sizes take the
+    minimum; clamped to [lo, hi]; radius = ceil(max(size - 1, 0) / 2) capped
+    at 16."""
+    I = np.eye(4, dtype=np.float32)
+    P = scr.perspective_reverse_z(90.0, 1.0, 0.1, 1000.0)            # P11 = 1
+    O = scr.orthographic_reverse_z(4.0, 1.0, 0.1, 1000.0)            # P11 = 0.5
+    S, Wd = scr.POINT_SIZE_SCREEN_SPACE, scr.POINT_SIZE_WORLD_SPACE
+    cases = [
+        # (mode, min, max, scale, point, proj, H, radius)
+        (S, 1, 9, 10.0, (0, 0, -2), P, 100, 2),        # size 5
+        (S, 1, 9, 10.0, (0, 0, -4), P, 100, 1),        # size 2.5: ceil(0.75)
+        (S, 1, 9, 10.0, (0, 0, -10), P, 100, 0),       # size 1
+        (S, 1, 9, 10.0, (0, 0, -1), P, 100, 4),        # size 10 -> clamped to 9
+        (S, 1, 9, 16.0, (4, 0, -3), P, 100, 2),        # |v| = 5: size 3.2 (view depth 3 would give 5.3)
+        (S, 3, 9, float("nan"), (0, 0, -2), P, 100, 1),  # non-finite: the minimum, 3
+        (S, 1, 100, 1e6, (0, 0, -2), P, 100, 16),      # size 100 -> 50, capped at 16
+        (Wd, 1, 16, 0.14, (4, 0, -3), P, 100, 2),      # 2 * 0.14 * 50 / 3 = 4.67 (|v| = 5 would give 2.8)
+        (Wd, 1, 16, 0.07, (0, 0, -2), P, 100, 2),      # 2 * 0.07 * 50 / 2 = 3.5: ceil(1.25)
+        (S, 1, 16, 7.0, (3, 1, -9), O, 100, 3),        # orthographic screen space: size = scale
+        (Wd, 1, 16, 0.09, (3, 1, -9), O, 100, 2),      # orthographic world space: 2 * 0.09 * 25 = 4.5
+    ]
+    for mode, lo, hi, scale, pt, proj, H, want in cases:
+        cfg = scr.ComputeRasteriserConfiguration(pointSizeMode=mode, minimumPointSize=lo, maximumPointSize=hi,
+                                                 pointSizeScale=scale)
+        got = orc.footprint_radius(np.float32(pt), cfg, I, proj, 2 * H, H)
+        assert got == want, (mode, lo, hi, scale, pt, got, want)
+    # the file's world matrix moves the point first: (0, 0, 0) placed at z = -2
+    cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1, maximumPointSize=9, pointSizeScale=10.0)
+    world = np.eye(4, dtype=np.float32)
+    world[3, 2] = -2.0
+    assert orc.footprint_radius(np.float32([0, 0, 0]), cfg, I, P, 200, 100, world=world) == 2
+
+
+def test_footprint_radius_closed_forms():
+    pin_footprint_radius()
+
+
+# ---------------------------------------------------------------- nearest
+def _lattice_cloud(n=24, dup=40):
+    """A single-batch lattice whose decoded positions are exact (box size 2
+    and 1, 10-bit decode) plus `dup` exact duplicates of lattice points."""
+    g = np.linspace(-1.0, 1.0, n, dtype=np.float32)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    z = (np.arange(n * n) % 7 + 1).astype(np.float32) / 8.0
+    pos = np.stack([X.ravel(), Y.ravel(), z], 1)
+    pos = np.concatenate([pos, [[-1.0, -1.0, 0.0], [1.0, 1.0, 1.0]], pos[::(n * n) // dup][:dup]]).astype(np.float32)
+    col = np.random.default_rng(3).random((len(pos), 4)).astype(np.float32)
+    packed = orc.pack(pos, col, pointsPerBatch=len(pos), lodLevels=1)
+    return orc.Cloud.from_packed(packed)
+
+
+def pin_nearest_matches_cpu_reference():
+    """The render passes' nearest mode (NearestDepth/NearestIndex) equals the
+    restated NearestPointCPUReference (NearestPointCPUReference.swift:16-81)
+    where their arithmetic coincides (exact lattice decode, identity camera):
+    same depths, same winning indices, duplicates resolved to the lowest
+    index; plus a hand-built tie at one pixel."""
+    cloud = _lattice_cloud()
+    I = np.eye(4, dtype=np.float32)
+    cfg = scr.ComputeRasteriserConfiguration(mode=scr.MODE_NEAREST_POINT, enableCLOD=False,
+                                             enableFrustumCulling=False)
+    for W, H in ((64, 64), (48, 30)):
+        out = orc.render_nearest(cloud, cfg, I, I, W, H)
+        ref = orc.nearest_cpu_reference(cloud, W, H, I, I)
+        assert (out["depth"] > 0).sum() > 100
+        assert np.array_equal(out["depth"].ravel(), ref["depths"])
+        assert np.array_equal(out["index"].ravel(), ref["indices"])
+    # three points on one pixel: two at the same (front) depth, one behind
+    pos = np.float32([[0.3, 0.3, 0.5], [0.3, 0.3, 0.75], [0.3, 0.3, 0.75], [-0.5, -0.5, 0.25]])
+    col = np.float32([[1, 0, 0, 1], [0, 1, 0, 1], [0, 0, 1, 1], [1, 1, 1, 1]])
+    packed = orc.pack(pos, col, pointsPerBatch=4, lodLevels=1)
+    c = orc.Cloud.from_packed(packed)
+    out = orc.render_nearest(c, cfg, I, I, 8, 8)
+    front = [i for i in range(4) if packed.sourceIndices[i] in (1, 2)]
+    iy = 8 - 1 - int(np.float32(0.65) * 8)
+    ix = int(np.float32(0.65) * 8)
+    assert out["index"][iy, ix] == min(front), (out["index"][iy, ix], front)
+    # z = 0.75 is the box's maximum: it decodes to min + 1023 steps of 10 bits
+    assert out["depth"][iy, ix] == orc.depth_to_uint(float(np.float32(0.25 + 1023 * (0.5 / 1024))))
+
+
+def test_nearest_equals_cpu_reference_and_lowest_index_wins():
+    pin_nearest_matches_cpu_reference()
+
+
+# ---------------------------------------------------------------- weighted blend
+def _one_point(x=0.0, y=0.0, z=0.5, rgb=(1.0, 1.0, 1.0)):
+    packed = orc.pack([(x, y, z)], [(*rgb, 1.0)], pointsPerBatch=1, lodLevels=1)
+    return orc.Cloud.from_packed(packed)
+
+
+def pin_blend_sums_and_coverage():
+    """Weighted-blend accumulation (ColorPass/Shaders.metal:146-198) and its
+    resolve (Resolve/Shaders.metal:27-45) for a single point with an
+    identity camera: coverage a = 1 - tint alpha, sums uint(c / 255 * a / N *
+    4096 + 0.5) on each of the N sweep taps; colour = sum / coverage sum,
+    alpha = 1 - e^(-coverage sum / 4096).  Values chosen so the exact
+    rationals sit far from rounding boundaries."""
+    I = np.eye(4, dtype=np.float32)
+    W = H = 64
+    cfg = scr.ComputeRasteriserConfiguration(applyTint=True, tintAlphaIsCoverage=True, enableCLOD=False,
+                                             enableFrustumCulling=False)
+    for tint_a, rgb, want_c, want_a in ((0.5, (1.0, 0.2, 0.0), (2048, 410, 0), 2048),
+                                        (0.75, (1.0, 1.0, 1.0), (1024, 1024, 1024), 1024),
+                                        (0.0, (0.6, 0.4, 1.0), (2458, 1638, 4096), 4096)):
+        c = _one_point(rgb=rgb)
+        c.tints = np.float32([[0.0, 0.0, 0.0, tint_a]])
+        out = orc.render_hqs([c], cfg, I, I, W, H)
+        iy, ix = 31, 32
+        assert (out["red"][iy, ix], out["green"][iy, ix], out["blue"][iy, ix]) == want_c
+        assert out["count"][iy, ix] == want_a and out["count"].sum() == want_a
+        rgba = int(out["rgba8"][iy, ix])
+        alpha = 1.0 - math.exp(-want_a / 4096.0)
+        assert rgba >> 24 == int(np.rint(alpha * 255.0)), (rgba >> 24, alpha)
+        for ch in range(3):
+            assert (rgba >> (8 * ch)) & 0xFF == int(np.rint(want_c[ch] / want_a * 255.0))
+    # motion blur: the previous view-projection shifts the point 8 px left and
+    # 4 rows down, so the sweep (scaled by 0.5) is (4, 2): 5 taps, step (1, 0.5)
+    c = _one_point()
+    prev = np.eye(4, dtype=np.float32)
+    prev[3, 0], prev[3, 1] = -0.25, 0.125
+    cfg = scr.ComputeRasteriserConfiguration(motionBlur=0.5, motionBlurSamples=8, motionBlurMaxSpread=64.0,
+                                             enableCLOD=False, enableFrustumCulling=False)
+    out = orc.render_hqs([c], cfg, I, I, W, H, prev_view_proj=prev)
+    taps = [(31, 32), (30, 31), (30, 30), (29, 29), (29, 28)]     # (row, column): centre - round(s * step)
+    got = {(int(r), int(q)) for r, q in zip(*np.nonzero(out["count"]))}
+    assert got == set(taps), sorted(got)
+    for r, q in taps:
+        assert out["count"][r, q] == 819 and out["red"][r, q] == 819    # uint(4096 / 5 + 0.5)
+    # a sweep longer than the sample cap: 8 taps
+    cfg.motionBlurSamples = 3
+    out = orc.render_hqs([c], cfg, I, I, W, H, prev_view_proj=prev)
+    assert (out["count"] > 0).sum() == 3 and out["count"][31, 32] == 1365      # uint(4096 / 3 + 0.5)
+
+
+def test_blend_sums_sweep_and_coverage_alpha():
+    pin_blend_sums_and_coverage()

@@ -70,8 +70,7 @@ def quiet_host():
 
 
 def overview_camera(scr, width, height):
-    return scr.Camera(scr.look_at((0.0, -7200.0, 5200.0), (0.0, 300.0, 0.0), (0.0, 0.0, 1.0)),
-                      scr.perspective_reverse_z(55.0, width / height, 10.0, 30000.0))
+    return scr.overview_camera(width, height)
 
 
 FLY_SCENE = dict(seed=0x5C0003, halfExtent=4000.0, numBoxes=32000)
@@ -275,14 +274,7 @@ def atomic_roofline(ctx, kernel_tag: str, ms: float, add: bool):
 
 
 def cloud_bounds(scr, cloud):
-    b = cloud.readPool(scr.POOL_BATCHES)
-    b = b[b["state"] != 0]
-    lo = np.array([b["minX"].min(), b["minY"].min(), b["minZ"].min()], np.float64)
-    hi = np.array([b["maxX"].max(), b["maxY"].max(), b["maxZ"].max()], np.float64)
-    return (lo + hi) * 0.5, float(np.linalg.norm(hi - lo))
-
-
-MV_SLOTS = (1_000_000, 5_000_000, 20_000_000, 50_000_000)
+    return scr.cloud_bounds(cloud)
 
 
 def multiview_bench(scr, ctx, steps, warmup, n_views=64):
@@ -293,27 +285,7 @@ def multiview_bench(scr, ctx, steps, warmup, n_views=64):
     Batched (one launch per phase for all views, graph replay, autotuned
     shapes) against one scr_render per view on the same GPU."""
     import ctypes as C
-    clouds, centres, diags = [], [], []
-    packer = scr.GPUPacker(ctx, keyBits=63)
-    for i, n in enumerate(MV_SLOTS):
-        pos, col = scr.gen_scene(ctx, n, on_device=True, seed=0x5C0003 + i, halfExtent=1000.0, numBoxes=2000)
-        c = scr.ComputeRasteriserPointCloud(ctx, 1, scr.DEFAULT_POINTS_PER_BATCH)
-        packer.packDevice(c, pos, col, n)
-        ctx.sync()
-        pos.free()
-        col.free()
-        ctr, d = cloud_bounds(scr, c)
-        clouds.append(c)
-        centres.append(ctr)
-        diags.append(d)
-    cams, sizes, which = [], [], []
-    for v in range(n_views):
-        k = v % len(clouds)
-        cc, ss = scr.sphere_views(centres[k], 1.5 * diags[k], 1, 0x5C0004 * 1000 + v)
-        cams += cc
-        sizes += ss
-        which.append(k)
-    cfg = scr.ComputeRasteriserConfiguration(minimumPointSize=1.0, maximumPointSize=3.0, pointSizeScale=1500.0)
+    clouds, cams, sizes, which, cfg = scr.multiview_workload(ctx, n_views)
     pixels = sum(w * h for w, h in sizes)
 
     # one launch sequence per view: scr_render with prepared arguments

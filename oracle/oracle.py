@@ -102,7 +102,8 @@ def lib() -> C.CDLL:
             "orc_update_files": (None, [vp, C.c_int, vp, vp, vp, vp, vp]),
             "orc_cull": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp]),
             "orc_render_hqs": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.POINTER(OrcPlanes), C.c_int]),
-            "orc_render_nearest": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]),
+            "orc_render_nearest": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp,
+                                             C.c_int]),
             "orc_nearest_cpu_reference": (None, [C.POINTER(OrcCloud), C.c_int, C.c_int, vp, vp, C.c_int, vp, vp]),
             "orc_pick_search": (C.c_int, [vp, C.c_int, C.c_int, f, f, C.c_int, C.POINTER(u32)]),
             "orc_pack": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -429,7 +430,7 @@ def render_hqs(clouds: Sequence[Cloud], cfg, view, proj, width: int, height: int
     return {k: v.reshape(height, width) for k, v in planes.items()}
 
 
-def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int) -> dict:
+def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int, threads: int = 1) -> dict:
     view, proj = _f32(view), _f32(proj)   # held until the calls return
     vp = mat_mul(proj, view)
     cc = cloud.composed(vp)
@@ -441,7 +442,7 @@ def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int) -> di
     df = np.zeros(n, np.float32)
     c = config_from(cfg)
     lib().orc_render_nearest(C.byref(s), C.addressof(c), view.ctypes.data, proj.ctypes.data, width,
-                             height, d.ctypes.data, idx.ctypes.data, rgba.ctypes.data, df.ctypes.data)
+                             height, d.ctypes.data, idx.ctypes.data, rgba.ctypes.data, df.ctypes.data, threads)
     return {"depth": d.reshape(height, width), "index": idx.reshape(height, width),
             "rgba8": rgba.reshape(height, width), "depthF32": df.reshape(height, width)}
 

@@ -780,45 +780,120 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
 }
 
 /* ------------------------------------------------------------------ nearest */
+typedef struct orc_nearest_job {
+    const orc_cloud* c;
+    const scr_visible_batch* vis;
+    int first, last, pass;
+    const scr_config* cfg;
+    const float* view;
+    const float* proj;
+    int width, height;
+    const uint32_t* depthsIn; /* pass 1: the merged depths */
+    uint32_t* out;            /* pass 0: depth maxima; pass 1: index minima */
+} orc_nearest_job;
+
+/* pass 0: nearestDepthUpdate (NearestDepth/Shaders.metal:14-91)
+ * pass 1: nearestIndexUpdate (NearestIndex/Shaders.metal:14-94) */
+static void* orc_nearest_main(void* arg) {
+    const orc_nearest_job* j = (const orc_nearest_job*)arg;
+    const orc_cloud* c = j->c;
+    for (int v = j->first; v < j->last; v++) {
+        const scr_visible_batch* vb = &j->vis[v];
+        const scr_raster_batch* b = &c->batches[vb->batchIndex];
+        const scr_raster_file* f = &c->files[b->fileIndex];
+        for (uint32_t li = 0; li < vb->activePoints; li++) {
+            const uint32_t pi = b->firstPoint + li;
+            orc_frag fr;
+            if (!orc_front(c, b, f, vb, pi, j->cfg, j->view, j->proj, j->width, j->height, 1, 1, &fr)) continue;
+            for (int oy = -fr.radius; oy <= fr.radius; oy++)
+                for (int ox = -fr.radius; ox <= fr.radius; ox++) {
+                    if (!orc_inside_footprint(ox, oy, fr.radius)) continue;
+                    const int tx = fr.px + ox, ty = fr.py + oy;
+                    if (tx < 0 || tx >= j->width || ty < 0 || ty >= j->height) continue;
+                    const size_t p = (size_t)ty * j->width + tx;
+                    if (j->pass == 0) {
+                        if (fr.depth > j->out[p]) j->out[p] = fr.depth;
+                    } else if (fr.depth == j->depthsIn[p]) {
+#if ORC_MUTANT == 3
+                        if (j->out[p] == 0xffffffffu || pi > j->out[p]) j->out[p] = pi;
+#else
+                        if (pi < j->out[p]) j->out[p] = pi;
+#endif
+                    }
+                }
+        }
+    }
+    return NULL;
+}
+
 int orc_render_nearest(const orc_cloud* c, const scr_config* cfg, const float* view,
                        const float* proj, int width, int height, uint32_t* depths,
-                       uint32_t* indices, uint32_t* rgba8, float* depthF32) {
+                       uint32_t* indices, uint32_t* rgba8, float* depthF32, int nThreads) {
     const size_t npx = (size_t)width * (size_t)height;
     /* clearWinnerUpdate — ClearWinner/Shaders.metal:7-19 */
     for (size_t p = 0; p < npx; p++) { depths[p] = 0u; indices[p] = 0xffffffffu; }
     scr_visible_batch* vis = (scr_visible_batch*)malloc(sizeof(scr_visible_batch) * (size_t)(c->nSlots > 0 ? c->nSlots : 1));
     if (!vis) return -1;
     const int nv = orc_cull(c, cfg, view, proj, width, height, vis);
-    for (int pass = 0; pass < 2; pass++) {
-        /* pass 0: nearestDepthUpdate (NearestDepth/Shaders.metal:14-91)
-         * pass 1: nearestIndexUpdate (NearestIndex/Shaders.metal:14-94) */
-        for (int v = 0; v < nv; v++) {
-            const scr_visible_batch* vb = &vis[v];
-            const scr_raster_batch* b = &c->batches[vb->batchIndex];
-            const scr_raster_file* f = &c->files[b->fileIndex];
-            for (uint32_t li = 0; li < vb->activePoints; li++) {
-                const uint32_t pi = b->firstPoint + li;
-                orc_frag fr;
-                if (!orc_front(c, b, f, vb, pi, cfg, view, proj, width, height, 1, 1, &fr)) continue;
-                for (int oy = -fr.radius; oy <= fr.radius; oy++)
-                    for (int ox = -fr.radius; ox <= fr.radius; ox++) {
-                        if (!orc_inside_footprint(ox, oy, fr.radius)) continue;
-                        const int tx = fr.px + ox, ty = fr.py + oy;
-                        if (tx < 0 || tx >= width || ty < 0 || ty >= height) continue;
-                        const size_t p = (size_t)ty * width + tx;
-                        if (pass == 0) {
-                            if (fr.depth > depths[p]) depths[p] = fr.depth;
-                        } else if (fr.depth == depths[p]) {
-#if ORC_MUTANT == 3
-                            if (indices[p] == 0xffffffffu || pi > indices[p]) indices[p] = pi;
-#else
-                            if (pi < indices[p]) indices[p] = pi;
-#endif
-                        }
-                    }
+    if (nThreads < 1) nThreads = 1;
+    if (nThreads > nv) nThreads = nv > 0 ? nv : 1;
+    /* threads over visible-batch ranges, each with a private plane; the
+     * planes merge by max (depths) and min (indices), so the result is the
+     * single-thread one */
+    orc_nearest_job* jobs = (orc_nearest_job*)calloc((size_t)nThreads, sizeof(orc_nearest_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nThreads, sizeof(pthread_t));
+    int ok = jobs && th;
+    for (int pass = 0; ok && pass < 2; pass++) {
+        for (int t = 0; t < nThreads; t++) {
+            orc_nearest_job* j = &jobs[t];
+            j->c = c;
+            j->vis = vis;
+            j->first = (int)((long long)nv * t / nThreads);
+            j->last = (int)((long long)nv * (t + 1) / nThreads);
+            j->pass = pass;
+            j->cfg = cfg;
+            j->view = view;
+            j->proj = proj;
+            j->width = width;
+            j->height = height;
+            j->depthsIn = depths;
+            if (nThreads == 1) {
+                j->out = pass == 0 ? depths : indices;
+            } else {
+                j->out = (uint32_t*)malloc(npx * 4 + 4);
+                if (!j->out) { ok = 0; break; }
+                for (size_t p = 0; p < npx; p++) j->out[p] = pass == 0 ? 0u : 0xffffffffu;
             }
         }
+        if (!ok) break;
+        if (nThreads == 1) {
+            orc_nearest_main(&jobs[0]);
+            continue;
+        }
+        for (int t = 0; t < nThreads; t++) pthread_create(&th[t], NULL, orc_nearest_main, &jobs[t]);
+        for (int t = 0; t < nThreads; t++) pthread_join(th[t], NULL);
+        for (int t = 0; t < nThreads; t++) {
+            uint32_t* o = jobs[t].out;
+            if (pass == 0) {
+                for (size_t p = 0; p < npx; p++) if (o[p] > depths[p]) depths[p] = o[p];
+            } else {
+                for (size_t p = 0; p < npx; p++) {
+#if ORC_MUTANT == 3
+                    if (o[p] != 0xffffffffu && (indices[p] == 0xffffffffu || o[p] > indices[p])) indices[p] = o[p];
+#else
+                    if (o[p] < indices[p]) indices[p] = o[p];
+#endif
+                }
+            }
+            free(o);
+            jobs[t].out = NULL;
+        }
     }
+    if (jobs && nThreads > 1)
+        for (int t = 0; t < nThreads; t++) free(jobs[t].out);
+    free(jobs);
+    free(th);
+    if (!ok) { free(vis); return -1; }
     free(vis);
     /* nearestResolveUpdate — NearestResolve/Shaders.metal:8-38 */
     for (size_t p = 0; p < npx; p++) {

@@ -102,10 +102,12 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
                    const float* proj, int width, int height, orc_hqs_planes* out, int nThreads);
 
 /* Nearest mode: clearWinner, cull, nearestDepth, nearestIndex, nearestResolve
- * (ComputeRasteriser.swift:483-553) for one cloud. */
+ * (ComputeRasteriser.swift:483-553) for one cloud, on nThreads threads over
+ * visible-batch ranges (private planes merged by max, then min: the result
+ * does not depend on nThreads). */
 int orc_render_nearest(const orc_cloud* cloud, const scr_config* cfg, const float* view,
                        const float* proj, int width, int height, uint32_t* depths,
-                       uint32_t* indices, uint32_t* rgba8OrNull, float* depthF32OrNull);
+                       uint32_t* indices, uint32_t* rgba8OrNull, float* depthF32OrNull, int nThreads);
 
 /* NearestPointCPUReference.render — NearestPointCPUReference.swift:16-81 */
 void orc_nearest_cpu_reference(const orc_cloud* cloud, int width, int height, const float* view,

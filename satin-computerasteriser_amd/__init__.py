@@ -1161,5 +1161,63 @@ def flythrough_cameras(frames: int = 120, pathSeed: int = 0x5C0003, width: int =
     return cams
 
 
+# ---------------------------------------------------------------- benchmark workloads
+# The scenes and cameras the benchmark measures and the full-size parity
+# tests render (synthetic inputs; no reference counterpart).
+
+HEADLINE_SCENE = dict(seed=0x5C0003, halfExtent=4000.0, numBoxes=32000)  # 100M points, 8 km square
+MULTIVIEW_SLOTS = (1_000_000, 5_000_000, 20_000_000, 50_000_000)
+
+
+def overview_camera(width: int, height: int) -> Camera:
+    """The headline frame's pose over the 8 km scene: every batch in view."""
+    return Camera(look_at((0.0, -7200.0, 5200.0), (0.0, 300.0, 0.0), (0.0, 0.0, 1.0)),
+                  perspective_reverse_z(55.0, width / height, 10.0, 30000.0))
+
+
+def generate_packed_cloud(ctx: Context, count: int, keyBits: int = 63, **scene) -> "ComputeRasteriserPointCloud":
+    """A generated scene, generated and packed on the GPU (inputs never leave HBM)."""
+    pos, col = gen_scene(ctx, count, on_device=True, **scene)
+    cloud = ComputeRasteriserPointCloud(ctx, 1, DEFAULT_POINTS_PER_BATCH)
+    GPUPacker(ctx, keyBits=keyBits).packDevice(cloud, pos, col, count)
+    ctx.sync()
+    pos.free()
+    col.free()
+    return cloud
+
+
+def cloud_bounds(cloud: "ComputeRasteriserPointCloud"):
+    """Centre and diagonal of the resident batches' boxes."""
+    b = cloud.readPool(POOL_BATCHES)
+    b = b[b["state"] != 0]
+    lo = np.array([b["minX"].min(), b["minY"].min(), b["minZ"].min()], np.float64)
+    hi = np.array([b["maxX"].max(), b["maxY"].max(), b["maxZ"].max()], np.float64)
+    return (lo + hi) * 0.5, float(np.linalg.norm(hi - lo))
+
+
+def multiview_workload(ctx: Context, n_views: int = 64, slots=MULTIVIEW_SLOTS):
+    """Batched multi-view workload: clouds of 1M/5M/20M/50M points of the 2 km
+    scene (seeds 0x5C0003..6); view v renders cloud v mod 4 from an eye
+    uniform on the sphere of 1.5x the cloud's diagonal at a resolution drawn
+    from {256x256, 512x384, 1024x768, 1920x1080}, screen-space points 1-3 px.
+    Returns (clouds, cameras, sizes, viewClouds, configuration)."""
+    clouds, centres, diags = [], [], []
+    for i, n in enumerate(slots):
+        c = generate_packed_cloud(ctx, n, seed=0x5C0003 + i, halfExtent=1000.0, numBoxes=2000)
+        ctr, d = cloud_bounds(c)
+        clouds.append(c)
+        centres.append(ctr)
+        diags.append(d)
+    cams, sizes, which = [], [], []
+    for v in range(n_views):
+        k = v % len(clouds)
+        cc, ss = sphere_views(centres[k], 1.5 * diags[k], 1, 0x5C0004 * 1000 + v)
+        cams += cc
+        sizes += ss
+        which.append(k)
+    cfg = ComputeRasteriserConfiguration(minimumPointSize=1.0, maximumPointSize=3.0, pointSizeScale=1500.0)
+    return clouds, cams, sizes, which, cfg
+
+
 def version() -> str:
     return load_library().scr_version().decode()

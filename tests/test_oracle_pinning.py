@@ -631,3 +631,17 @@ def pin_blend_sums_and_coverage():
 
 def test_blend_sums_sweep_and_coverage_alpha():
     pin_blend_sums_and_coverage()
+
+
+def test_nearest_independent_of_threads():
+    pos, col = jittered_cloud(20000, 0x4E4)
+    packed = orc.pack(pos, col, pointsPerBatch=500)
+    cloud = orc.Cloud.from_packed(packed)
+    cam = scr.Camera.perspective((10.0, 4.0, 12.0), (0, 0, 0), fov=45, aspect=1.5)
+    cfg = scr.ComputeRasteriserConfiguration(mode=1, minimumPointSize=1, maximumPointSize=4, pointSizeScale=30.0)
+    ref = orc.render_nearest(cloud, cfg, cam.viewMatrix, cam.projectionMatrix, 120, 80)
+    assert (ref["depth"] > 0).sum() > 1000
+    for t in (2, 5, 16):
+        out = orc.render_nearest(cloud, cfg, cam.viewMatrix, cam.projectionMatrix, 120, 80, threads=t)
+        for k in ref:
+            assert np.array_equal(ref[k], out[k]), (t, k)

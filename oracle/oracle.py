@@ -108,6 +108,9 @@ def lib() -> C.CDLL:
             "orc_pick_search": (C.c_int, [vp, C.c_int, C.c_int, f, f, C.c_int, C.POINTER(u32)]),
             "orc_pack": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
                                    C.POINTER(OrcPackOut)]),
+            "orc_pack_gpu": (C.c_int, [vp, C.c_int, vp, C.c_size_t, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(OrcPackOut)]),
+            "orc_gpu_lod_enabled": (C.c_int, [C.c_int, C.c_int]),
             "orc_morton_keys": (None, [vp, C.c_int, C.c_size_t, C.c_int, vp]),
             "orc_lod_levels": (None, [vp, C.c_size_t, vp, vp, C.c_int, C.c_int, vp]),
             "orc_cube_grid": (None, [C.c_int, f, vp, vp]),
@@ -270,9 +273,15 @@ def identity_files(n: int = 1) -> np.ndarray:
     return f
 
 
+def gpu_lod_enabled(lodLevels: int, coarseVoxelDivisions: int) -> bool:
+    """GPUPacker.init's LOD gate (GPUPacker.swift:35, 86-99)."""
+    return bool(lib().orc_gpu_lod_enabled(lodLevels, coarseVoxelDivisions))
+
+
 def pack(positions, colors, pointsPerBatch: int = 10240, lodLevels: int = 4, coarseVoxelDivisions: int = 64,
-         keyBits: int = 30) -> Packed:
-    """PackedPointCloudFixtures.pack (stable Morton order; keyBits 30 or 63)."""
+         keyBits: int = 30, gpuRules: bool = False) -> Packed:
+    """PackedPointCloudFixtures.pack (stable Morton order; keyBits 30 or 63);
+    gpuRules: under GPUPacker's LOD gate, what GPUPacker.pack produces."""
     pos = _f32(positions).reshape(-1, 3)
     col = _f32(colors).reshape(-1, 4)
     n = pos.shape[0]
@@ -282,8 +291,9 @@ def pack(positions, colors, pointsPerBatch: int = 10240, lodLevels: int = 4, coa
     o.batches, o.xyzLow, o.xyzMed, o.xyzHigh = _p(out.batches), _p(out.xyzLow), _p(out.xyzMed), _p(out.xyzHigh)
     o.colors, o.levels, o.sourceIndices, o.orderedPositions = (_p(out.colors), _p(out.levels),
                                                                _p(out.sourceIndices), _p(out.orderedPositions))
-    rc = lib().orc_pack(pos.ctypes.data, 3, col.ctypes.data, n, pointsPerBatch, lodLevels, coarseVoxelDivisions,
-                        keyBits, C.byref(o))
+    fn = lib().orc_pack_gpu if gpuRules else lib().orc_pack
+    rc = fn(pos.ctypes.data, 3, col.ctypes.data, n, pointsPerBatch, lodLevels, coarseVoxelDivisions, keyBits,
+            C.byref(o))
     if rc < 0:
         raise ValueError(f"orc_pack failed ({rc})")
     out.boundsMin[:] = list(o.boundsMin)

@@ -1077,9 +1077,36 @@ void orc_lod_levels(const float* sp, size_t count, const float bmin[3], const fl
     free(table);
 }
 
+/* GPUPacker.init's LOD gate — GPUPacker.swift:35, 86-99: the dense grid has
+ * at most (coarse * 2^(L - 2) + 2)^3 cells; above 268,435,456 the GPU packer
+ * skips LOD and leaves every level at the finest (L - 1). */
+int orc_gpu_lod_enabled(int lodLevels, int coarse) {
+    if (lodLevels < 1) lodLevels = 1;
+    if (lodLevels > 8) lodLevels = 8;
+    if (coarse < 1) coarse = 1;
+    if (lodLevels <= 1) return 0;
+    const long long maxDim = (long long)coarse * (1ll << (lodLevels - 2)) + 2;
+    if (maxDim > 1000000) return 0;  /* the cube would exceed the cap many times over */
+    return maxDim * maxDim * maxDim <= 268435456ll;
+}
+
+static int orc_pack_impl(const float* pos, int stride, const float* col, size_t count, int ppb, int lodLevels,
+                         int coarse, int keyBits, int gpuRules, orc_pack_out* out);
+
 /* pack — PackedPointCloudFixtures.swift:30-167 */
 int orc_pack(const float* pos, int stride, const float* col, size_t count, int ppb, int lodLevels,
              int coarse, int keyBits, orc_pack_out* out) {
+    return orc_pack_impl(pos, stride, col, count, ppb, lodLevels, coarse, keyBits, 0, out);
+}
+
+/* The same pack under GPUPacker's LOD gate (orc_gpu_lod_enabled). */
+int orc_pack_gpu(const float* pos, int stride, const float* col, size_t count, int ppb, int lodLevels,
+                 int coarse, int keyBits, orc_pack_out* out) {
+    return orc_pack_impl(pos, stride, col, count, ppb, lodLevels, coarse, keyBits, 1, out);
+}
+
+static int orc_pack_impl(const float* pos, int stride, const float* col, size_t count, int ppb, int lodLevels,
+                         int coarse, int keyBits, int gpuRules, orc_pack_out* out) {
     if (ppb < 1 || ppb > 65535 || (keyBits != 30 && keyBits != 63) || (stride != 3 && stride != 4)) return -1;
     if (count == 0) {
         for (int a = 0; a < 3; a++) { out->boundsMin[a] = 0.0f; out->boundsMax[a] = 0.0f; }
@@ -1114,7 +1141,8 @@ int orc_pack(const float* pos, int stride, const float* col, size_t count, int p
     free(keys);
     free(ki);
 
-    orc_lod_levels(sp, count, out->boundsMin, out->boundsMax, lodLevels, coarse, lv);
+    if (gpuRules && !orc_gpu_lod_enabled(lodLevels, coarse)) memset(lv, lodLevels - 1, count);
+    else orc_lod_levels(sp, count, out->boundsMin, out->boundsMax, lodLevels, coarse, lv);
 
     /* bucketSortBatchSlicesByLevel — :175-214 */
     for (size_t first = 0; first < count; first += (size_t)ppb) {

@@ -136,6 +136,14 @@ typedef struct orc_pack_out {
 int orc_pack(const float* positions, int posStride, const float* colors, size_t count,
              int pointsPerBatch, int lodLevels, int coarseVoxelDivisions, int keyBits,
              orc_pack_out* out);
+/* GPUPacker's LOD gate (GPUPacker.swift:35, 86-99): 1 when the dense grid's
+ * cell bound (coarse * 2^(L - 2) + 2)^3 is within 268,435,456, else 0 (the GPU
+ * packer then leaves every level at L - 1). */
+int orc_gpu_lod_enabled(int lodLevels, int coarseVoxelDivisions);
+/* orc_pack under that gate: what GPUPacker.pack produces. */
+int orc_pack_gpu(const float* positions, int posStride, const float* colors, size_t count,
+                 int pointsPerBatch, int lodLevels, int coarseVoxelDivisions, int keyBits,
+                 orc_pack_out* out);
 /* Morton key per point (the sort key the packer uses), keyBits 30 or 63. */
 void orc_morton_keys(const float* positions, int posStride, size_t count, int keyBits,
                      uint64_t* keys);

@@ -33,6 +33,7 @@ void ctx_release(scr_ctx* ctx) {
     }
     for (auto e : ctx->eventPool) (void)hipEventDestroy(e);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->packBounds) (void)hipFree(ctx->packBounds);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -300,6 +301,12 @@ int scr_ctx_create(int device, scr_ctx** out) {
         return SCR_ERR_HIP;
     }
     ctx->numCUs = prop.multiProcessorCount;
+    // the pack's bounds words
+    if (hipMalloc((void**)&ctx->packBounds, 256) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return SCR_ERR_HIP;
+    }
     *out = ctx;
     return SCR_OK;
 }

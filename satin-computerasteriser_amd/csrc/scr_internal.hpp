@@ -30,6 +30,7 @@ struct scr_ctx {
     // scratch owned by the context (grown on demand, freed at destroy)
     void* scratch = nullptr;
     size_t scratchBytes = 0;
+    uint32_t* packBounds = nullptr;  // the pack's six bounds words (256 B, allocated with the context)
 };
 
 // Device view of one cloud, passed by value to the point kernels.
@@ -201,7 +202,8 @@ struct PackDest {
     uint32_t ppb;
 };
 
-// GPU pack of n points (device float4 positions / colours) into dst (scr_pack.hip).
+// GPU pack of n points (device float4 positions / colours) into dst
+// (scr_pack.hip), in the context's scratch, grown on demand.
 int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, const scr_pack_options& opt,
                 const PackDest& dst, uint32_t* hostSourceIndices);
 

@@ -526,6 +526,8 @@ struct Arena {
 }  // namespace
 
 
+// Scratch bytes of a pack of n points into nb batches with a dense LOD grid
+// of gridCells cells (the six bounds words are not included).
 static size_t pack_scratch_bytes(size_t n, bool wide, size_t nb, size_t gridCells) {
     const size_t numTiles = (n + kTile - 1) / kTile;
     const size_t scanN = 256 * numTiles;
@@ -535,12 +537,13 @@ static size_t pack_scratch_bytes(size_t n, bool wide, size_t nb, size_t gridCell
     b += n * 16;                                // sorted positions
     b += n;                                     // Morton-order levels
     b += 2 * scanN * 4 + (scanN / kScanSeg + 2) * 4;  // tile hist, offsets, scan sums
-    b += 8 * 256 * 4 + 64;                      // digit histograms, bounds
+    b += 8 * 256 * 4 + 64;                      // digit histograms
     b += nb * 16;                               // cum counts
     b += gridCells * 4;
     b += 32 * 256;                              // alignment slack
     return b;
 }
+
 
 template <typename K>
 static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*& vout, size_t n, uint32_t* ghist,
@@ -554,8 +557,8 @@ static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*
     if (e != hipSuccess) return hip_error(ctx, e, "radix hist clear");
     hipLaunchKernelGGL(k_all_digit_hist<K>, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
                        dim3(kThreads), 0, s, kin, n, ghist);
-    std::vector<uint32_t> h((size_t)P * 256);
-    e = hipMemcpyAsync(h.data(), ghist, h.size() * 4, hipMemcpyDeviceToHost, s);
+    uint32_t h[8 * 256];
+    e = hipMemcpyAsync(h, ghist, (size_t)P * 256 * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, "radix hist");
     int run = 0;
@@ -581,7 +584,18 @@ static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*
     return SCR_OK;
 }
 
-// Run the nine stages for n points into `dst`.  pos/col are device float4 arrays.
+// GPUPacker.init's LOD gate (GPUPacker.swift:35, 86-99): the dense grid has at
+// most (coarse * 2^(L - 2) + 2)^3 cells (cells per axis floor(extent /
+// voxel) + 1 at the finest claimed level); above 268,435,456 the packer
+// skips LOD and leaves every level at the finest.
+bool gpu_lod_enabled(int lodLevels, int coarse) {
+    if (lodLevels <= 1) return false;
+    const long long maxDim = (long long)coarse * (1ll << (lodLevels - 2)) + 2;
+    return maxDim <= 1000000 && maxDim * maxDim * maxDim <= 268435456ll;
+}
+
+// Run the nine stages for n points into `dst`.  pos/col are device float4
+// arrays; the stages work in the context's scratch, grown on demand.
 int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, const scr_pack_options& opt,
                 const PackDest& dst, uint32_t* hostSourceIndices) {
     hipStream_t s = ctx->stream;
@@ -591,26 +605,23 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     const int lodLevels = std::min(std::max(opt.lodLevels, 1), 8);
     const int coarse = std::max(opt.coarseVoxelDivisions, 1);
     const uint8_t maxLevel = (uint8_t)(lodLevels - 1);
+    const bool lodOn = gpu_lod_enabled(lodLevels, coarse);
+    hipError_t e = hipSuccess;
 
-    // bounds first (the LOD grid size depends on them)
-    uint32_t* dBounds = nullptr;
-    hipError_t e = hipMalloc((void**)&dBounds, 6 * sizeof(uint32_t));
-    if (e != hipSuccess) return hip_error(ctx, e, "pack bounds alloc");
+    // bounds first (the LOD grid size depends on them), in the context's
+    // bounds words (allocated with the context)
+    uint32_t* dBounds = ctx->packBounds;
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, dBounds);
     hipLaunchKernelGGL(k_bounds, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
                        dim3(kThreads), 0, s, pos, n, dBounds);
     uint32_t hb[6];
     e = hipMemcpyAsync(hb, dBounds, sizeof hb, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-        (void)hipFree(dBounds);
-        return hip_error(ctx, e, "pack bounds");
-    }
-    // dense LOD grid dims at the finest claimed level (cells per axis grow by 2 per level)
+    if (e != hipSuccess) return hip_error(ctx, e, "pack bounds");
+    // dense LOD grid dims at the finest claimed level (cells per axis grow by
+    // 2 per level), within the gate's bound
     uint64_t gridCells = 0;
-    LodGrid lg{};
-    uint32_t dims[3] = {1, 1, 1};
-    if (lodLevels > 1) {
+    if (lodOn) {
         float mn[3], ext[3];
         for (int a = 0; a < 3; a++) {
             mn[a] = float_unflip(hb[a]);
@@ -619,12 +630,8 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
         const float longest = std::fmax(ext[0], std::fmax(ext[1], ext[2]));
         const float voxel = (longest / (float)coarse) * std::pow(0.5f, (float)(lodLevels - 2));
         const float inv = 1.0f / std::fmax(voxel, 0.000001f);
-        for (int a = 0; a < 3; a++) dims[a] = (uint32_t)f2u(std::floor(ext[a] * inv)) + 1u;
-        gridCells = (uint64_t)dims[0] * dims[1] * dims[2];
-        if (gridCells > (1ull << 30)) {
-            (void)hipFree(dBounds);
-            return set_error(ctx, SCR_ERR_UNSUPPORTED, "LOD grid exceeds 2^30 cells; lower lodLevels or coarseVoxelDivisions");
-        }
+        gridCells = 1;
+        for (int a = 0; a < 3; a++) gridCells *= (uint64_t)f2u(std::floor(ext[a] * inv)) + 1u;
     }
     const size_t need = pack_scratch_bytes(n, wide, nb, (size_t)gridCells);
     if (ctx->scratchBytes < need) {
@@ -632,10 +639,7 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
         ctx->scratch = nullptr;
         ctx->scratchBytes = 0;
         e = hipMalloc(&ctx->scratch, need);
-        if (e != hipSuccess) {
-            (void)hipFree(dBounds);
-            return hip_error(ctx, e, "pack scratch alloc");
-        }
+        if (e != hipSuccess) return hip_error(ctx, e, "pack scratch alloc");
         ctx->scratchBytes = need;
     }
     Arena ar{(char*)ctx->scratch};
@@ -672,17 +676,14 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
         rc = radix_sort<uint32_t>(ctx, kin, vin, kout, vout, n, ghist, thist, offs, sums, nullptr);
         sortedIdx = vin;
     }
-    if (rc != SCR_OK) {
-        (void)hipFree(dBounds);
-        return rc;
-    }
+    if (rc != SCR_OK) return rc;
     uint32_t* permOut = sortedIdx == idxA ? idxB : idxA;
 
     // 4 gather Morton-order positions
     hipLaunchKernelGGL(k_gather_pos, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, sortedIdx, n, sp);
     // 5 LOD levels
     hipLaunchKernelGGL(k_lod_init, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, lvM, n, maxLevel);
-    for (int level = 0; level < lodLevels - 1; level++) {
+    for (int level = 0; lodOn && level < lodLevels - 1; level++) {
         LodGrid g;
         g.voxelScale = std::pow(0.5f, (float)level);
         g.coarse = (float)coarse;
@@ -721,7 +722,6 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     if (e == hipSuccess && hostSourceIndices)
         e = hipMemcpyAsync(hostSourceIndices, permOut, n * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(dBounds);
     if (e != hipSuccess) return hip_error(ctx, e, "pack");
     return SCR_OK;
 }

@@ -235,3 +235,21 @@ def test_gen_scene_is_deterministic_and_host_equals_device(ctx):
     assert np.allclose(v - np.floor(v), 0.5, atol=1e-3)
     # 70 % terrain / 30 % box faces, inside the square
     assert np.all(np.abs(a_pos[:70_000, :2]) <= 1000.0)
+
+
+@pytest.mark.parametrize("levels,coarse", [(8, 64), (6, 64), (6, 4)])
+def test_gpu_pack_lod_gate(ctx, levels, coarse):
+    """Above GPUPacker's grid cap (GPUPacker.swift:35, 86-99) the GPU pack
+    leaves every level at the finest instead of failing; below it, LOD runs."""
+    pos, col = jittered_cloud(6000, 0x6A7E)
+    ref = orc.pack(pos, col, pointsPerBatch=1000, lodLevels=levels, coarseVoxelDivisions=coarse, gpuRules=True)
+    cloud = scr.ComputeRasteriserPointCloud(ctx, 1, 1000)
+    src = scr.GPUPacker(ctx, lodLevels=levels, coarseVoxelDivisions=coarse, keyBits=30).pack(cloud, pos, col,
+                                                                                         sourceIndices=True)
+    assert np.array_equal(src, ref.sourceIndices)
+    _assert_pack_equal(cloud, ref)
+    lv = cloud.readPool(scr.POOL_LEVELS)[:6000]
+    if orc.gpu_lod_enabled(levels, coarse):
+        assert len(set(lv.tolist())) > 2
+    else:
+        assert np.all(lv == levels - 1)

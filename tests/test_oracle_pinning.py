@@ -645,3 +645,25 @@ def test_nearest_independent_of_threads():
         out = orc.render_nearest(cloud, cfg, cam.viewMatrix, cam.projectionMatrix, 120, 80, threads=t)
         for k in ref:
             assert np.array_equal(ref[k], out[k]), (t, k)
+
+
+def test_gpu_lod_gate_closed_forms():
+    """GPUPacker.init (GPUPacker.swift:35, 86-99): LOD runs when
+    (coarse * 2^(L - 2) + 2)^3 <= 268,435,456, else every level stays L - 1."""
+    for levels, coarse, on in ((4, 64, True), (5, 64, True), (6, 64, False), (8, 64, False), (1, 64, False),
+                               (6, 40, True), (6, 41, False), (2, 643, True), (2, 644, False), (3, 321, True),
+                               (3, 322, False)):
+        assert orc.gpu_lod_enabled(levels, coarse) == on, (levels, coarse)
+        dim = coarse * 2 ** (levels - 2) + 2
+        assert on == (levels > 1 and dim ** 3 <= 268_435_456)
+    pos, col = jittered_cloud(2000, 0x10D)
+    off = orc.pack(pos, col, pointsPerBatch=300, lodLevels=8, coarseVoxelDivisions=64, gpuRules=True)
+    assert np.all(off.levels == 7)
+    cum = off.batches["lodCum67"] >> 16
+    assert np.array_equal(cum, off.batches["numPoints"])
+    assert np.all(off.batches["lodCum01"] == 0) and np.all(off.batches["lodCum45"] == 0)
+    # below the cap the gate changes nothing
+    a = orc.pack(pos, col, pointsPerBatch=300, lodLevels=6, coarseVoxelDivisions=4, gpuRules=True)
+    b = orc.pack(pos, col, pointsPerBatch=300, lodLevels=6, coarseVoxelDivisions=4)
+    assert np.array_equal(a.levels, b.levels) and a.batches.tobytes() == b.batches.tobytes()
+    assert len(set(a.levels.tolist())) > 2

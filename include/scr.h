@@ -114,7 +114,8 @@ typedef struct scr_config {
     int32_t holeFillIterations;  /* hole-fill passes after the resolve (<= 0: none) */
     int32_t motionBlurSamples;
     float motionBlurMaxSpread;
-    int32_t reserved[2];
+    int32_t applyDisplacement;   /* add each point's displacement before projection (depth, colour, blend passes) */
+    int32_t reserved[1];
 } scr_config;
 
 /* Reference defaults, ComputeRasteriserTypes.swift:97-119. */
@@ -221,6 +222,16 @@ int scr_cloud_update_files(scr_cloud* cloud, const float viewProjection[16], con
  * nPoints == maxResidentBatches * pointsPerBatch.  onDevice != 0: rgba is a
  * device pointer.  rgba == NULL drops the tint buffer. */
 int scr_cloud_set_tints(scr_cloud* cloud, const float* rgba, size_t nPoints, int onDevice);
+/* displacementBuffer (ComputeRasteriserPointCloud.swift:91-94): one float3
+ * per pool point in the 16-byte stride of a Metal float3 array (x, y, z,
+ * unused), nPoints == maxResidentBatches * pointsPerBatch, copied in.  A
+ * render with applyDisplacement draws point + displacement in the depth,
+ * colour and weighted-blend passes (DepthPass/Shaders.metal:69-74,
+ * ColorPass/Shaders.metal:72-78; nearest mode does not displace); a NaN
+ * component removes the point from all of them.  Clouds without a buffer read
+ * zeros (ComputeRasteriser.swift:333-342).  xyz4 == NULL drops the buffer
+ * (and the previous-frame copies kept for motion blur). */
+int scr_cloud_set_displacements(scr_cloud* cloud, const float* xyz4, size_t nPoints, int onDevice);
 
 /* Device view of a cloud's pool for user kernels — what the TintPass /
  * DisplacementPass preamble binds for user Metal kernels (TintPass.swift:215-338,
@@ -239,13 +250,19 @@ typedef struct scr_cloud_view {
     uint32_t pointsPerBatch;
     uint32_t slots;                  /* maxResidentBatches */
     void* stream;
+    float* displacements;            /* float4 per pool point (xyz used): DisplacementPass output */
 } scr_cloud_view;
 /* Fills *out after committing pending batch updates (commitBatchUpdates).
- * withTints != 0 also makes sure the cloud has a tint buffer (a new one holds
- * zeros; one installed with scr_cloud_set_tints is returned as is), for a
- * user kernel to write and a render with applyTint to read (TintPass.swift:
- * 130-184).  The pointers stay valid until the pool is resized or destroyed. */
-int scr_cloud_device_view(scr_cloud* cloud, int withTints, scr_cloud_view* out);
+ * flags: SCR_VIEW_TINTS makes sure the cloud has a tint buffer (a new one
+ * holds zeros; one installed with scr_cloud_set_tints is returned as is) for
+ * a user kernel to write and a render with applyTint to read (TintPass.swift:
+ * 130-184); SCR_VIEW_DISPLACEMENTS does the same for the displacement buffer,
+ * which a render with applyDisplacement reads (DisplacementPass.swift:120-172
+ * allocates it and turns applyDisplacement on).  The pointers stay valid until
+ * the pool is resized or destroyed. */
+#define SCR_VIEW_TINTS 1
+#define SCR_VIEW_DISPLACEMENTS 2
+int scr_cloud_device_view(scr_cloud* cloud, int flags, scr_cloud_view* out);
 
 #define SCR_POOL_XYZ_LOW 0
 #define SCR_POOL_XYZ_MED 1
@@ -358,6 +375,16 @@ int scr_frame_hole_fill(scr_frame* frame, int iterations);
  * and calls scr_cloud_update_files for every cloud with its model matrix. */
 int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam,
                scr_cloud* const* clouds, int nClouds, scr_frame* frame);
+/* The same render for the camera `cameraKey` names.  Motion blur sweeps from
+ * the previous frame's pose of that camera: the frame keeps the last
+ * view-projection per key (ComputeRasteriser.swift:255-266 keys it by the
+ * camera object so stereo eyes do not collide; the current one is used the
+ * first time a key is seen), and each displaced cloud keeps per key a copy of
+ * the displacements its last motion-blurred render used, the previous
+ * displacements of the next one (ComputeRasteriser.swift:434-480).
+ * scr_render is scr_render_camera with cameraKey 0. */
+int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, uint64_t cameraKey,
+                      scr_cloud* const* clouds, int nClouds, scr_frame* frame);
 
 /* pickPointIndex(atNDC:in:camera:searchRadius:) — ComputeRasteriser.swift:609-670.
  * Runs the nearest-index pass for one cloud into `frame`, reads back a band of

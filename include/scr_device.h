@@ -19,6 +19,14 @@
  * Tints are read by a render whose configuration sets applyTint: the colour
  * is mixed toward saturate(tint.rgb) by saturate(tint.a); a negative alpha
  * drops the point (ColorPass/Shaders.metal:95-133).
+ *
+ * A displacement kernel (DisplacementPass.swift:120-172) is written the same
+ * way against the view of scr_cloud_device_view(cloud, SCR_VIEW_DISPLACEMENTS,
+ * &v): it resolves its point with scr_resolve_displacement_thread and writes
+ * reinterpret_cast<float4*>(v.displacements)[i] = make_float4(dx, dy, dz, 0).
+ * A render with applyDisplacement draws position + displacement in the
+ * depth, colour and weighted-blend passes; a NaN component removes the point
+ * from all of them (DepthPass/Shaders.metal:69-74, ColorPass/Shaders.metal:72-78).
  */
 #ifndef SCR_DEVICE_H
 #define SCR_DEVICE_H
@@ -39,6 +47,13 @@ __device__ inline bool scr_resolve_tint_thread(uint32_t id, const scr_cloud_view
     if (batch->state == 0 || *localOffset >= batch->numPoints) return false;
     *pointIndex = batch->firstPoint + *localOffset;
     return true;
+}
+
+/* scr_resolveDisplacementThread (DisplacementPass.swift:305-318): the same
+ * thread-to-point mapping as the tint preamble's. */
+__device__ inline bool scr_resolve_displacement_thread(uint32_t id, const scr_cloud_view* v, scr_raster_batch* batch,
+                                                       uint32_t* pointIndex, uint32_t* localOffset) {
+    return scr_resolve_tint_thread(id, v, batch, pointIndex, localOffset);
 }
 
 /* scr_decodePointAt: the point's position at the precision its level byte

@@ -36,7 +36,7 @@ class OrcCloud(C.Structure):
     _fields_ = [
         ("batches", C.c_void_p), ("nSlots", C.c_int), ("files", C.c_void_p), ("nFiles", C.c_int),
         ("xyzLow", C.c_void_p), ("xyzMed", C.c_void_p), ("xyzHigh", C.c_void_p), ("colors", C.c_void_p),
-        ("levels", C.c_void_p), ("tints", C.c_void_p),
+        ("levels", C.c_void_p), ("tints", C.c_void_p), ("disp", C.c_void_p), ("prevDisp", C.c_void_p),
     ]
 
 
@@ -321,7 +321,8 @@ def lod_levels(sorted_positions, bmin, bmax, lodLevels: int = 4, coarse: int = 6
 class Cloud:
     """Pool-layout arrays of one cloud, as the rasteriser reads them."""
 
-    def __init__(self, batches, files, xyzLow, xyzMed, xyzHigh, colors, levels, tints=None):
+    def __init__(self, batches, files, xyzLow, xyzMed, xyzHigh, colors, levels, tints=None, disp=None,
+                 prevDisp=None):
         self.batches = np.ascontiguousarray(batches, RASTER_BATCH_DTYPE)
         self.files = np.ascontiguousarray(files, RASTER_FILE_DTYPE)
         self.xyzLow = np.ascontiguousarray(xyzLow, np.uint32)
@@ -330,6 +331,10 @@ class Cloud:
         self.colors = np.ascontiguousarray(colors, np.uint32)
         self.levels = np.ascontiguousarray(levels, np.uint8)
         self.tints = None if tints is None else np.ascontiguousarray(tints, np.float32).reshape(-1, 4)
+        # displacements, float4 per pool point (xyz used); prevDisp: the
+        # previous frame's (None: the current ones)
+        self.disp = None if disp is None else np.ascontiguousarray(disp, np.float32).reshape(-1, 4)
+        self.prevDisp = None if prevDisp is None else np.ascontiguousarray(prevDisp, np.float32).reshape(-1, 4)
         self.model = np.eye(4, dtype=np.float32)
 
     @staticmethod
@@ -366,7 +371,8 @@ class Cloud:
         vp, model = _f32(vp), _f32(self.model)
         lib().orc_update_files(self.files.ctypes.data, len(self.files), vp.ctypes.data, model.ctypes.data, None, None if prev is None else prev.ctypes.data,
                                files.ctypes.data)
-        c = Cloud(self.batches, files, self.xyzLow, self.xyzMed, self.xyzHigh, self.colors, self.levels, self.tints)
+        c = Cloud(self.batches, files, self.xyzLow, self.xyzMed, self.xyzHigh, self.colors, self.levels, self.tints,
+                  self.disp, self.prevDisp)
         c.model = self.model
         return c
 
@@ -377,6 +383,8 @@ class Cloud:
         s.xyzLow, s.xyzMed, s.xyzHigh = self.xyzLow.ctypes.data, self.xyzMed.ctypes.data, self.xyzHigh.ctypes.data
         s.colors, s.levels = self.colors.ctypes.data, self.levels.ctypes.data
         s.tints = None if self.tints is None else self.tints.ctypes.data
+        s.disp = None if self.disp is None else self.disp.ctypes.data
+        s.prevDisp = None if self.prevDisp is None else self.prevDisp.ctypes.data
         return s
 
 
@@ -389,7 +397,7 @@ class Config(C.Structure):
         ("pointSizeMode", C.c_int32), ("minimumPointSize", C.c_float), ("maximumPointSize", C.c_float),
         ("pointSizeScale", C.c_float), ("applyTint", C.c_int32), ("tintAlphaIsCoverage", C.c_int32),
         ("motionBlur", C.c_float), ("holeFillIterations", C.c_int32), ("motionBlurSamples", C.c_int32),
-        ("motionBlurMaxSpread", C.c_float), ("reserved", C.c_int32 * 2),
+        ("motionBlurMaxSpread", C.c_float), ("applyDisplacement", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 

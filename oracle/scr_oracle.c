@@ -425,7 +425,8 @@ int orc_cull(const orc_cloud* c, const scr_config* cfg, const float* view, const
  * nearestIndexUpdate (NearestDepth :37-72, NearestIndex :37-72).
  * Returns 1 when the point survives; fills ndcZ, pixel (x, y-flipped), radius. */
 typedef struct orc_frag {
-    float point[3];
+    float base[3];   /* decoded position */
+    float point[3];  /* decoded position + displacement */
     float ndcZ;
     int px, py;
     uint32_t depth;
@@ -439,7 +440,17 @@ static int orc_front(const orc_cloud* c, const scr_raster_batch* b, const scr_ra
     const float pointLevel = (float)((uint32_t)c->levels[pointIndex] & 7u);
     const float dither = (cfg->enableLODDither != 0) ? orc_hash_unit(pointIndex) : 0.5f;
     if (dither >= (vb->lodThreshold - pointLevel) + 0.5f) return 0;
-    orc_decode_point(pointIndex, b, c->xyzLow, c->xyzMed, c->xyzHigh, vb->level, 1, fr->point);
+    orc_decode_point(pointIndex, b, c->xyzLow, c->xyzMed, c->xyzHigh, vb->level, 1, fr->base);
+    for (int a = 0; a < 3; a++) fr->point[a] = fr->base[a];
+    /* displacement (DepthPass/Shaders.metal:69-74, ColorPass/Shaders.metal:
+     * 72-78; the nearest passes do not displace): point += displacement, a
+     * cloud without a buffer reads zeros, a NaN component drops the point */
+    if (cfg->applyDisplacement != 0 && !nearestZ) {
+        static const float zero4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float* d = c->disp ? c->disp + (size_t)pointIndex * 4 : zero4;
+        for (int a = 0; a < 3; a++) fr->point[a] = fr->point[a] + d[a];
+        if (ORC_MUTANT != 12 && (isnan(fr->point[0]) || isnan(fr->point[1]) || isnan(fr->point[2]))) return 0;
+    }
     float clip[4];
     orc_transform_point(f->transform, fr->point, clip);
     if (clip[3] <= 0.0f) return 0;
@@ -516,15 +527,26 @@ static void orc_depth_items(const orc_job* j) {
  * footprint pixel of each motion-blur sample, the splat swept from its pixel
  * back toward its previous frame's pixel (file.prevTransform).  Integer pixel
  * differences wrap as the shader's int2 arithmetic does. */
-static void orc_blend_fragment(const orc_job* j, const scr_raster_file* f, const orc_frag* fr, uint32_t r,
-                               uint32_t g, uint32_t bl, float a) {
+static void orc_blend_fragment(const orc_job* j, const orc_cloud* c, uint32_t pi, const scr_raster_file* f,
+                               const orc_frag* fr, uint32_t r, uint32_t g, uint32_t bl, float a) {
     if (a <= 0.0f) return;
     const int W = j->width, H = j->height;
     int samples = 1;
     float stepX = 0.0f, stepY = 0.0f;
     if (j->cfg->motionBlur > 0.0f) {
+        /* the sweep starts from base + the previous displacement
+         * (ColorPass/Shaders.metal:150-153; the current displacement when
+         * there is no previous one, zeros without a buffer) */
+        float prevPoint[3] = {fr->base[0], fr->base[1], fr->base[2]};
+        if (ORC_MUTANT == 13)
+            for (int a = 0; a < 3; a++) prevPoint[a] = fr->point[a];
+        else if (j->cfg->applyDisplacement != 0) {
+            static const float zero4[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            const float* d = c->prevDisp ? c->prevDisp + (size_t)pi * 4 : (c->disp ? c->disp + (size_t)pi * 4 : zero4);
+            for (int a = 0; a < 3; a++) prevPoint[a] = prevPoint[a] + d[a];
+        }
         float prev[4];
-        orc_transform_point(f->prevTransform, fr->point, prev);
+        orc_transform_point(f->prevTransform, prevPoint, prev);
         if (prev[3] > 0.0f) {
             const float prw = orc_rcp(prev[3]);
             const int ppx = orc_f2i(fmaf(prev[0] * prw, 0.5f, 0.5f) * (float)W);
@@ -614,7 +636,7 @@ static void orc_color_items(const orc_job* j) {
             }
         blended:
             if (blend) {
-                orc_blend_fragment(j, f, &fr, r, g, bl, cover);
+                orc_blend_fragment(j, c, pi, f, &fr, r, g, bl, cover);
                 continue;
             }
             for (int oy = -fr.radius; oy <= fr.radius; oy++)

@@ -75,6 +75,8 @@ typedef struct orc_cloud {
     const uint32_t* colors;
     const uint8_t* levels;
     const float* tints;              /* float4 per pool point, or NULL */
+    const float* disp;               /* displacement, float4 per pool point (xyz), or NULL: zeros */
+    const float* prevDisp;           /* previous frame's displacement, or NULL: `disp` */
 } orc_cloud;
 
 /* updateFiles — ComputeRasteriserPointCloud.swift:487-515 */

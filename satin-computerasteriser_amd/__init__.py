@@ -101,7 +101,7 @@ class CConfig(C.Structure):
         ("pointSizeMode", C.c_int32), ("minimumPointSize", C.c_float), ("maximumPointSize", C.c_float),
         ("pointSizeScale", C.c_float), ("applyTint", C.c_int32), ("tintAlphaIsCoverage", C.c_int32),
         ("motionBlur", C.c_float), ("holeFillIterations", C.c_int32), ("motionBlurSamples", C.c_int32),
-        ("motionBlurMaxSpread", C.c_float), ("reserved", C.c_int32 * 2),
+        ("motionBlurMaxSpread", C.c_float), ("applyDisplacement", C.c_int32), ("reserved", C.c_int32 * 1),
     ]
 
 
@@ -120,7 +120,8 @@ class CCloudView(C.Structure):
     """scr_cloud_view: device pointers of a cloud's pool for user kernels."""
     _fields_ = [("batches", C.c_void_p), ("xyzLow", C.c_void_p), ("xyzMed", C.c_void_p), ("xyzHigh", C.c_void_p),
                 ("colors", C.c_void_p), ("levels", C.c_void_p), ("tints", C.c_void_p),
-                ("pointsPerBatch", C.c_uint32), ("slots", C.c_uint32), ("stream", C.c_void_p)]
+                ("pointsPerBatch", C.c_uint32), ("slots", C.c_uint32), ("stream", C.c_void_p),
+                ("displacements", C.c_void_p)]
 
 
 class CPackOptions(C.Structure):
@@ -179,6 +180,7 @@ _SIGNATURES = {
     "scr_cloud_set_model": (_I, [_VP, _VP]),
     "scr_cloud_update_files": (_I, [_VP, _VP, _VP, _VP, _VP]),
     "scr_cloud_set_tints": (_I, [_VP, _VP, _SZ, _I]),
+    "scr_cloud_set_displacements": (_I, [_VP, _VP, _SZ, _I]),
     "scr_cloud_device_view": (_I, [_VP, _I, _VP]),
     "scr_cloud_read_pool": (_I, [_VP, _I, _VP, _SZ]),
     "scr_cloud_read_visible": (_I, [_VP, _VP, _I, C.POINTER(_I)]),
@@ -193,6 +195,7 @@ _SIGNATURES = {
     "scr_frame_hole_fill": (_I, [_VP, _I]),
     "scr_frame_read": (_I, [_VP, _I, _VP, _SZ]),
     "scr_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _I, _VP]),
+    "scr_render_camera": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), C.c_uint64, _VP, _I, _VP]),
     "scr_pick": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, C.c_float, C.c_float, _I,
                       C.POINTER(C.c_uint32), C.POINTER(_I)]),
     "scr_diff_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _VP, _VP, _VP]),
@@ -328,6 +331,7 @@ class ComputeRasteriserConfiguration:
     holeFillIterations: int = 0
     motionBlurSamples: int = 8
     motionBlurMaxSpread: float = 64.0
+    applyDisplacement: bool = False
 
     def to_c(self) -> CConfig:
         c = CConfig()
@@ -351,6 +355,7 @@ class ComputeRasteriserConfiguration:
         c.holeFillIterations = int(self.holeFillIterations)
         c.motionBlurSamples = int(self.motionBlurSamples)
         c.motionBlurMaxSpread = float(self.motionBlurMaxSpread)
+        c.applyDisplacement = int(bool(self.applyDisplacement))
         return c
 
 
@@ -623,12 +628,25 @@ class ComputeRasteriserPointCloud:
     def setTintsDevice(self, ptr: int, nPoints: int) -> None:
         self.ctx.check(self.lib.scr_cloud_set_tints(self.handle, ptr, nPoints, 1))
 
-    def deviceView(self, withTints: bool = False) -> CCloudView:
+    def setDisplacements(self, xyz: Optional[np.ndarray]) -> None:
+        """displacementBuffer (ComputeRasteriserPointCloud.swift:91-94): one
+        displacement per pool point, (n, 3) or (n, 4) (the 16-byte stride of a
+        Metal float3 array), or None to drop it."""
+        if xyz is None:
+            self.ctx.check(self.lib.scr_cloud_set_displacements(self.handle, None, 0, 0))
+            return
+        d = np.zeros((np.asarray(xyz).reshape(len(xyz), -1).shape[0], 4), np.float32)
+        d[:, :3] = np.asarray(xyz, np.float32).reshape(len(xyz), -1)[:, :3]
+        self.ctx.check(self.lib.scr_cloud_set_displacements(self.handle, d.ctypes.data, d.shape[0], 0))
+
+    def deviceView(self, withTints: bool = False, withDisplacements: bool = False) -> CCloudView:
         """The pool's device pointers and the context stream for a user HIP
-        kernel (scr_device.h) — what TintPass binds for its user kernel
-        (TintPass.swift:215-338); withTints makes sure a tint buffer exists."""
+        kernel (scr_device.h) — what TintPass / DisplacementPass bind for their
+        user kernels (TintPass.swift:215-338, DisplacementPass.swift:120-172);
+        withTints / withDisplacements make sure those buffers exist."""
         v = CCloudView()
-        self.ctx.check(self.lib.scr_cloud_device_view(self.handle, int(bool(withTints)), C.byref(v)))
+        flags = (1 if withTints else 0) | (2 if withDisplacements else 0)
+        self.ctx.check(self.lib.scr_cloud_device_view(self.handle, flags, C.byref(v)))
         return v
 
     def readPool(self, which: int) -> np.ndarray:
@@ -716,8 +734,11 @@ class ComputeRasteriser:
         if self.frame is None or (self.frame.width, self.frame.height) != (int(width), int(height)):
             self.frame = Frame(self.ctx, width, height)
 
-    def render(self, camera: Camera) -> None:
-        """update(renderContext:camera:viewport:index:) + encode(_:) for one frame."""
+    def render(self, camera: Camera, cameraKey: int = 0) -> None:
+        """update(renderContext:camera:viewport:index:) + encode(_:) for one
+        frame.  Motion blur sweeps from the previous frame of the camera
+        `cameraKey` names (the reference keys it by the camera object, so the
+        eyes of a stereo pair pass different keys)."""
         if self.frame is None:
             raise RuntimeError("resize() first")
         for cl in self.pointClouds:
@@ -726,8 +747,8 @@ class ComputeRasteriser:
         handles = (C.c_void_p * max(1, len(self.pointClouds)))(*[c.handle.value for c in self.pointClouds])
         cfg = self.configuration.to_c()
         cam = camera.to_c()
-        self.ctx.check(self.ctx.lib.scr_render(self.ctx.handle, C.byref(cfg), C.byref(cam), handles,
-                                               len(self.pointClouds), self.frame.handle))
+        self.ctx.check(self.ctx.lib.scr_render_camera(self.ctx.handle, C.byref(cfg), C.byref(cam), int(cameraKey),
+                                                      handles, len(self.pointClouds), self.frame.handle))
 
     def read(self, which: int) -> np.ndarray:
         return self.frame.read(which).reshape(self.frame.height, self.frame.width)

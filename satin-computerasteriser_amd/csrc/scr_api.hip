@@ -116,6 +116,8 @@ int check_config(scr_ctx* ctx, const scr_config* cfg, bool allowCoverage) {
         return set_error(ctx, SCR_ERR_UNSUPPORTED, "motion blur needs scr_render");
     if (!allowCoverage && cfg->holeFillIterations > 0)
         return set_error(ctx, SCR_ERR_UNSUPPORTED, "hole fill runs in scr_render or scr_frame_hole_fill");
+    if (!allowCoverage && cfg->applyDisplacement != 0)
+        return set_error(ctx, SCR_ERR_UNSUPPORTED, "displacement needs scr_render");
     if (cfg->pointSizeMode != 0 && cfg->pointSizeMode != 1)
         return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown point size mode");
     return SCR_OK;
@@ -140,6 +142,8 @@ PassParams make_params(const scr_config* cfg, const scr_camera* cam, int width, 
     p.colorizeChunks = cfg->colorizeChunks;
     p.colorizeOverdraw = cfg->colorizeOverdraw;
     p.applyTint = cfg->applyTint;
+    // the nearest passes do not displace (ComputeRasteriser.swift:727-731)
+    p.applyDisplacement = (cfg->applyDisplacement != 0 && cfg->mode == SCR_MODE_HIGH_QUALITY_AVERAGE) ? 1 : 0;
     const float mn = cfg->minimumPointSize, mx = cfg->maximumPointSize;
     p.lo = std::fmax(std::fmin(mn, mx), 1.0f);
     p.hi = std::fmax(std::fmax(mn, mx), p.lo);
@@ -180,6 +184,13 @@ hipError_t dalloc(T*& p, size_t count) {
 }
 
 void free_pool(scr_cloud* c);
+
+// The displacement buffer and its per-camera previous-frame copies.
+void drop_displacements(scr_cloud* c) {
+    dfree(c->dDisp);
+    for (auto& kv : c->prevDisp) dfree(kv.second);
+    c->prevDisp.clear();
+}
 
 void release_cloud(scr_cloud* c) {
     if (!c) return;
@@ -254,6 +265,7 @@ void free_pool(scr_cloud* c) {
     dfree(c->dBatches);
     dfree(c->dVisible);
     dfree(c->dBounds);
+    drop_displacements(c);
     if (c->tintsOwned) dfree(c->dTints);
     c->dTints = nullptr;
     c->tintsOwned = false;
@@ -653,12 +665,34 @@ int scr_cloud_set_tints(scr_cloud* c, const float* rgba, size_t nPoints, int onD
     return SCR_OK;
 }
 
-int scr_cloud_device_view(scr_cloud* c, int withTints, scr_cloud_view* out) {
+int scr_cloud_set_displacements(scr_cloud* c, const float* xyz4, size_t nPoints, int onDevice) {
+    if (!c) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = c->ctx;
+    SCR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!xyz4) {
+        drop_displacements(c);
+        return SCR_OK;
+    }
+    if (nPoints != c->poolPoints())
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT,
+                         "displacement buffer must hold maxResidentBatches * pointsPerBatch");
+    if (!c->dDisp) SCR_HIP(ctx, dalloc(c->dDisp, nPoints));
+    SCR_HIP(ctx, hipMemcpy(c->dDisp, xyz4, nPoints * sizeof(float4),
+                           onDevice ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
+    return SCR_OK;
+}
+
+int scr_cloud_device_view(scr_cloud* c, int flags, scr_cloud_view* out) {
     if (!c || !out) return SCR_ERR_INVALID_ARGUMENT;
     scr_ctx* ctx = c->ctx;
     const int rc = flush_mirror(c);
     if (rc != SCR_OK) return rc;
-    if (withTints && !c->dTints) {
+    if ((flags & SCR_VIEW_DISPLACEMENTS) && !c->dDisp) {
+        SCR_HIP(ctx, dalloc(c->dDisp, c->poolPoints()));
+        SCR_HIP(ctx, scr::launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(c->dDisp), c->poolPoints() * 4,
+                                          0u));
+    }
+    if ((flags & SCR_VIEW_TINTS) && !c->dTints) {
         SCR_HIP(ctx, dalloc(c->dTints, c->poolPoints()));
         c->tintsOwned = true;
         SCR_HIP(ctx, scr::launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(c->dTints), c->poolPoints() * 4,
@@ -671,6 +705,7 @@ int scr_cloud_device_view(scr_cloud* c, int withTints, scr_cloud_view* out) {
     out->colors = c->dColors;
     out->levels = c->dLevels;
     out->tints = reinterpret_cast<float*>(c->dTints);
+    out->displacements = reinterpret_cast<float*>(c->dDisp);
     out->pointsPerBatch = (uint32_t)c->pointsPerBatch;
     out->slots = (uint32_t)c->maxResidentBatches;
     out->stream = ctx->stream;
@@ -957,6 +992,11 @@ int scr_frame_read(scr_frame* f, int which, void* host, size_t bytes) {
 
 int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* const* clouds, int nClouds,
                scr_frame* frame) {
+    return scr_render_camera(ctx, cfg, cam, 0, clouds, nClouds, frame);
+}
+
+int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, uint64_t cameraKey,
+                      scr_cloud* const* clouds, int nClouds, scr_frame* frame) {
     if (!ctx || !cam || !frame || nClouds < 0 || (nClouds > 0 && !clouds)) return SCR_ERR_INVALID_ARGUMENT;
     int rc = check_config(ctx, cfg, true);
     if (rc != SCR_OK) return rc;
@@ -966,12 +1006,14 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
     float vp[16];
     scr::mat_mul(cam->projectionMatrix, cam->viewMatrix, vp);
     hipStream_t s = ctx->stream;
-    // the previous frame's view-projection of this viewport, for motion-blur
-    // velocity (ComputeRasteriser.swift:255-266: the current one the first time)
+    // the previous view-projection of this camera, for motion-blur velocity
+    // (ComputeRasteriser.swift:255-266: the current one the first time)
     float prevVP[16];
-    std::memcpy(prevVP, frame->hasPrevVP ? frame->prevVP : vp, sizeof prevVP);
-    std::memcpy(frame->prevVP, vp, sizeof vp);
-    frame->hasPrevVP = true;
+    auto seen = frame->prevVP.find(cameraKey);
+    std::memcpy(prevVP, seen != frame->prevVP.end() ? seen->second.data() : vp, sizeof prevVP);
+    std::array<float, 16> cur;
+    std::memcpy(cur.data(), vp, sizeof vp);
+    frame->prevVP[cameraKey] = cur;
 
     if (cfg->mode == SCR_MODE_NEAREST_POINT) {
         // encodeNearestPoint: first cloud with resident batches only (:483-497)
@@ -1017,9 +1059,19 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
         rc = upload_files(c, c->model, vp, nullptr, prevVP);
         if (rc != SCR_OK) return rc;
     }
+    // each cloud's device view; the previous displacements of this camera
+    // (the current ones until a motion-blurred render of it has kept a copy,
+    // ColorPassProcessor: prevDisplacementBuffer ?? displacementBuffer)
+    std::vector<CloudDev> devs;
     for (scr_cloud* c : resident) {
+        CloudDev d = c->dev();
+        auto it = c->prevDisp.find(cameraKey);
+        if (it != c->prevDisp.end()) d.prevDisp = it->second;
+        devs.push_back(d);
+    }
+    for (const CloudDev& d : devs) {
         scr::PassScope ps(ctx, "cull");
-        SCR_HIP(ctx, scr::launch_cull(s, c->dev(), p));
+        SCR_HIP(ctx, scr::launch_cull(s, d, p));
     }
     // translucent defocus and motion blur take the weighted-blend passes
     // (ComputeRasteriser.swift:233-246)
@@ -1029,21 +1081,36 @@ int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_c
     q.samples = cfg->motionBlurSamples;
     q.maxSpread = cfg->motionBlurMaxSpread;
     const bool blend = q.coverageMode != 0 || cfg->motionBlur > 0.0f;
-    for (scr_cloud* c : resident) {
+    for (const CloudDev& d : devs) {
         scr::PassScope ps(ctx, "depth");
         if (q.coverageMode != 0)
-            SCR_HIP(ctx, scr::launch_depth_coverage(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
+            SCR_HIP(ctx, scr::launch_depth_coverage(s, ctx->numCUs, d, p, frame->plane(0)));
         else
-            SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, c->dev(), p, frame->plane(0)));
+            SCR_HIP(ctx, scr::launch_depth(s, ctx->numCUs, d, p, frame->plane(0)));
     }
-    for (scr_cloud* c : resident) {
+    for (const CloudDev& d : devs) {
         scr::PassScope ps(ctx, "color");
         if (blend)
-            SCR_HIP(ctx, scr::launch_color_coverage(s, ctx->numCUs, c->dev(), p, q, frame->plane(1), frame->plane(2),
+            SCR_HIP(ctx, scr::launch_color_coverage(s, ctx->numCUs, d, p, q, frame->plane(1), frame->plane(2),
                                                     frame->plane(3), frame->plane(4)));
         else
-            SCR_HIP(ctx, scr::launch_color(s, ctx->numCUs, c->dev(), p, frame->plane(0), frame->plane(1),
+            SCR_HIP(ctx, scr::launch_color(s, ctx->numCUs, d, p, frame->plane(0), frame->plane(1),
                                            frame->plane(2), frame->plane(3), frame->plane(4)));
+    }
+    // with motion blur, each displaced cloud keeps this frame's displacements
+    // as this camera's previous ones (ComputeRasteriser.swift:451-480)
+    if (cfg->motionBlur > 0.0f) {
+        for (scr_cloud* c : resident) {
+            if (!c->dDisp) continue;
+            auto it = c->prevDisp.find(cameraKey);
+            if (it == c->prevDisp.end()) {
+                float4* fresh = nullptr;
+                SCR_HIP(ctx, dalloc(fresh, c->poolPoints()));
+                it = c->prevDisp.emplace(cameraKey, fresh).first;
+            }
+            SCR_HIP(ctx, hipMemcpyAsync(it->second, c->dDisp, c->poolPoints() * sizeof(float4),
+                                        hipMemcpyDeviceToDevice, s));
+        }
     }
     {
         scr::PassScope ps(ctx, "resolve");
@@ -1149,8 +1216,14 @@ int scr_pick(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_clo
     *found = 0;
     const int W = frame->width, H = frame->height;
     if (cloud->residentBatchCount <= 0) return SCR_OK;
+    // the nearest index pass alone (ComputeRasteriser.swift:629-651): the
+    // blend, hole-fill and displacement settings of the frame do not apply
     scr_config c2 = *cfg;
     c2.mode = SCR_MODE_NEAREST_POINT;
+    c2.motionBlur = 0.0f;
+    c2.tintAlphaIsCoverage = 0;
+    c2.holeFillIterations = 0;
+    c2.applyDisplacement = 0;
     int rc = check_config(ctx, &c2);
     if (rc != SCR_OK) return rc;
     // NDC (y up) → buffer pixel (row 0 at the top), ComputeRasteriser.swift:617-621

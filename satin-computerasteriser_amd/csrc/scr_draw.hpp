@@ -171,7 +171,14 @@ __device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p
     vb.level = level;
     vb.lodThreshold = thr;
     vb.activePoints = active;
-    bb = batch_bounds(f->transform, b, p.width, p.height, batch_const_radius(p, f->world, b));
+    if (p.applyDisplacement != 0) {
+        // displaced points leave the box: nothing about them follows from it
+        bb = BatchBounds{};
+        bb.rconst = p.constRadius;
+        bb.zmax = __builtin_inff();
+    } else {
+        bb = batch_bounds(f->transform, b, p.width, p.height, batch_const_radius(p, f->world, b));
+    }
     return true;
 }
 
@@ -299,11 +306,13 @@ __device__ __forceinline__ void chunk_fetch(const CloudDev& c, const BatchCtx& b
 }
 
 // The clip-w, NDC and viewport tests (skipped when kInside: the batch box
-// proves every point passes them), the y flip and the footprint radius.
+// proves every point passes them), the y flip and the footprint radius (of
+// the drawn position x, y, z; bx, by, bz: the decoded one, kept for the
+// motion-blur sweep).
 template <bool kNearestZ, bool kR0, bool kInside, int PPT>
 __device__ __forceinline__ void chunk_tests(const PassParams& p, const BatchCtx& bc, const f4* clip, const float* rw,
                                             const bool* keep, const float* wx, const float* wy, const float* wz,
-                                            ChunkFrags<PPT>& f) {
+                                            const float* bx, const float* by, const float* bz, ChunkFrags<PPT>& f) {
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         bool ok = keep[j];
@@ -324,16 +333,18 @@ __device__ __forceinline__ void chunk_tests(const PassParams& p, const BatchCtx&
         f.py[j] = p.height - 1 - iy;
         f.z[j] = nz;
         f.r[j] = (kR0 || !ok) ? 0 : (bc.rconst >= 0 ? bc.rconst : footprint_radius(p, bc.W, wx[j], wy[j], wz[j]));
-        f.wx[j] = wx[j];
-        f.wy[j] = wy[j];
-        f.wz[j] = wz[j];
+        f.wx[j] = bx[j];
+        f.wy[j] = by[j];
+        f.wz[j] = bz[j];
     }
 }
 
 // Runs the front end of every point pass on the fetched points: dither/LOD
-// skip, decodePoint (Common.metal:73-117), projection, clip and viewport
-// tests, y flip, footprint radius.
-template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kUseInside = true>
+// skip, decodePoint (Common.metal:73-117), displacement (kDisp: position +
+// displacement, a NaN component drops the point; DepthPass/Shaders.metal:
+// 69-74, ColorPass/Shaders.metal:72-78), projection, clip and viewport tests,
+// y flip, footprint radius.
+template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kUseInside = true, bool kDisp = false>
 __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                     const ChunkRaw<PPT>& r, ChunkFrags<PPT>& f) {
     const uint32_t base = bc.first + li0;
@@ -342,7 +353,8 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
     // (each written as the negation of its reject condition, so NaNs behave
     // as in the Metal source) and the fragment is kept or dropped once.
     f4 clip[PPT];
-    float wx[PPT], wy[PPT], wz[PPT];
+    float wx[PPT], wy[PPT], wz[PPT];      // decoded
+    float dx[PPT], dy[PPT], dz[PPT];      // drawn (decoded + displacement)
     bool keep[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
@@ -373,7 +385,20 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         wx[j] = fmaf((float)qx, bc.sx, bc.mnx);
         wy[j] = fmaf((float)qy, bc.sy, bc.mny);
         wz[j] = fmaf((float)qz, bc.sz, bc.mnz);
-        clip[j] = mul_point(bc.T, wx[j], wy[j], wz[j]);
+        if (kDisp) {
+            // a cloud without a displacement buffer reads zeros; points past
+            // the drawn range read nothing
+            const float4 d = (c.disp && (uint32_t)j < n) ? gload_rec(c.disp + pi) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            dx[j] = wx[j] + d.x;
+            dy[j] = wy[j] + d.y;
+            dz[j] = wz[j] + d.z;
+            keep[j] = keep[j] && !(__builtin_isnan(dx[j]) || __builtin_isnan(dy[j]) || __builtin_isnan(dz[j]));
+        } else {
+            dx[j] = wx[j];
+            dy[j] = wy[j];
+            dz[j] = wz[j];
+        }
+        clip[j] = mul_point(bc.T, dx[j], dy[j], dz[j]);
     }
     // 1 / w, one wave-uniform branch per chunk
     float rw[PPT];
@@ -386,16 +411,18 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
     }
     // a batch whose box proves that every point passes the clip-w, NDC and
     // viewport tests (kBoundsInside) skips them: one uniform branch per chunk
-    if (kUseInside && kR0 && bc.inside) chunk_tests<kNearestZ, kR0, true, PPT>(p, bc, clip, rw, keep, wx, wy, wz, f);
-    else chunk_tests<kNearestZ, kR0, false, PPT>(p, bc, clip, rw, keep, wx, wy, wz, f);
+    if (kUseInside && !kDisp && kR0 && bc.inside)
+        chunk_tests<kNearestZ, kR0, true, PPT>(p, bc, clip, rw, keep, dx, dy, dz, wx, wy, wz, f);
+    else
+        chunk_tests<kNearestZ, kR0, false, PPT>(p, bc, clip, rw, keep, dx, dy, dz, wx, wy, wz, f);
 }
 
-template <bool kNearestZ, bool kR0, int kLevel, int PPT>
+template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kDisp = false>
 __device__ __forceinline__ void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                    bool vec, ChunkFrags<PPT>& f) {
     ChunkRaw<PPT> r;
     chunk_fetch<kLevel, PPT>(c, bc, li0, vec, r);
-    chunk_decode<kNearestZ, kR0, kLevel, PPT>(c, p, bc, li0, r, f);
+    chunk_decode<kNearestZ, kR0, kLevel, PPT, true, kDisp>(c, p, bc, li0, r, f);
 }
 
 // ---------------------------------------------------------------- chunk rectangle
@@ -641,7 +668,7 @@ __device__ __forceinline__ void depth_flush(const ChunkBox& box, const PassParam
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, int kLevel, int kTile>
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel, int kTile, bool kDisp = false>
 __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red,
         const ChunkBox& seg) {
@@ -657,7 +684,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
         uint32_t* spare = ldsAll + kTile + __lane_id();
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             ChunkFrags<PPT> f;
-            chunk_front<false, kR0, kLevel, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
+            chunk_front<false, kR0, kLevel, PPT, kDisp>(c, p, bc, cb + G::rank() * PPT, vec, f);
             if (kR0) depth_frags_tile<PPT>(f, seg, p, lds, spare, depth);
             else depth_frags<kR0, PPT>(f, seg, p, lds, depth);
         }
@@ -668,7 +695,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
     }
     for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
         ChunkFrags<PPT> f;
-        chunk_front<false, kR0, kLevel, PPT>(c, p, bc, cb + G::rank() * PPT, vec, f);
+        chunk_front<false, kR0, kLevel, PPT, kDisp>(c, p, bc, cb + G::rank() * PPT, vec, f);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
         if (box.frags == 0) continue;
         if (box.lds) {
@@ -685,7 +712,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile>
+template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile, bool kDisp = false>
 __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     uint32_t* __restrict__ depth, uint32_t* ldsAll, int* red) {
     using G = Group<kWave, NT>;
@@ -693,9 +720,9 @@ __device__ __forceinline__ void depth_chunks(const CloudDev& c, const PassParams
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.none) return;  // no point of the batch reaches the screen
     // one straight-line copy of the body per position precision
-    if (bc.level == 0) depth_chunks_level<kR0, kWave, NT, PPT, 0, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
-    else if (bc.level == 1) depth_chunks_level<kR0, kWave, NT, PPT, 1, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
-    else depth_chunks_level<kR0, kWave, NT, PPT, 2, kTile>(c, p, bc, lo, depth, ldsAll, red, seg);
+    if (bc.level == 0) depth_chunks_level<kR0, kWave, NT, PPT, 0, kTile, kDisp>(c, p, bc, lo, depth, ldsAll, red, seg);
+    else if (bc.level == 1) depth_chunks_level<kR0, kWave, NT, PPT, 1, kTile, kDisp>(c, p, bc, lo, depth, ldsAll, red, seg);
+    else depth_chunks_level<kR0, kWave, NT, PPT, 2, kTile, kDisp>(c, p, bc, lo, depth, ldsAll, red, seg);
 }
 
 // ---------------------------------------------------------------- colour pass body
@@ -949,7 +976,7 @@ __device__ __forceinline__ void color_tile_flush(const ChunkBox& box, const Pass
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, int kLevel>
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel, bool kDisp = false>
 __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                     uint32_t* blue, uint32_t* count, ColorLds LAll,
@@ -973,7 +1000,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
             uint32_t rgb[PPT];
             color_fetch<PPT>(c, bc, li0, vec, rgb);
             ChunkFrags<PPT> f;
-            chunk_decode<false, kR0, kLevel, PPT, kColorInside>(c, p, bc, li0, raw, f);
+            chunk_decode<false, kR0, kLevel, PPT, kColorInside, kDisp>(c, p, bc, li0, raw, f);
             chunk_colors<PPT>(c, p, bc, li0, f, rgb);
             if (kR0) color_frags_tile<PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
             else color_frags<kR0, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
@@ -990,7 +1017,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
         uint32_t rgb[PPT];
         color_fetch<PPT>(c, bc, li0, vec, rgb);
         ChunkFrags<PPT> f;
-        chunk_decode<false, kR0, kLevel, PPT, kColorInside>(c, p, bc, li0, raw, f);
+        chunk_decode<false, kR0, kLevel, PPT, kColorInside, kDisp>(c, p, bc, li0, raw, f);
         chunk_colors<PPT>(c, p, bc, li0, f, rgb);
         const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
         if (box.frags == 0) continue;
@@ -1007,7 +1034,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT, bool kDisp = false>
 __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                     uint32_t* blue, uint32_t* count, ColorLds LAll) {
@@ -1016,9 +1043,9 @@ __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.none) return;  // no point of the batch reaches the screen
     // one straight-line copy of the body per position precision
-    if (bc.level == 0) color_chunks_level<kR0, kWave, NT, PPT, 0>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
-    else if (bc.level == 1) color_chunks_level<kR0, kWave, NT, PPT, 1>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
-    else color_chunks_level<kR0, kWave, NT, PPT, 2>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    if (bc.level == 0) color_chunks_level<kR0, kWave, NT, PPT, 0, kDisp>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    else if (bc.level == 1) color_chunks_level<kR0, kWave, NT, PPT, 1, kDisp>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    else color_chunks_level<kR0, kWave, NT, PPT, 2, kDisp>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
 }
 
 // ---------------------------------------------------------------- nearest pass body
@@ -1160,23 +1187,23 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
 // ---------------------------------------------------------------- per-batch entry points
 // A batch whose points all have radius 0 runs the single-pixel code.
 
-template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile>
+template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile, bool kDisp = false>
 __device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
                                             const BatchBounds* bb, uint32_t lo, uint32_t hi, uint32_t* __restrict__ depth,
                                             uint32_t* lds, int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
-    if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT, kTile>(c, p, bc, lo, depth, lds, red);
-    else depth_chunks<kR0, kWave, NT, PPT, kTile>(c, p, bc, lo, depth, lds, red);
+    if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT, kTile, kDisp>(c, p, bc, lo, depth, lds, red);
+    else depth_chunks<kR0, kWave, NT, PPT, kTile, kDisp>(c, p, bc, lo, depth, lds, red);
 }
 
-template <bool kR0, bool kWave, int NT, int PPT>
+template <bool kR0, bool kWave, int NT, int PPT, bool kDisp = false>
 __device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
                                             const BatchBounds* bb, uint32_t lo, uint32_t hi,
                                             const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                             uint32_t* blue, uint32_t* count, ColorLds L) {
     const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
-    if (!kR0 && bc.rconst == 0) color_chunks<true, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
-    else color_chunks<kR0, kWave, NT, PPT>(c, p, bc, lo, depth, red, green, blue, count, L);
+    if (!kR0 && bc.rconst == 0) color_chunks<true, kWave, NT, PPT, kDisp>(c, p, bc, lo, depth, red, green, blue, count, L);
+    else color_chunks<kR0, kWave, NT, PPT, kDisp>(c, p, bc, lo, depth, red, green, blue, count, L);
 }
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kNearestTile>

@@ -4,7 +4,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -43,6 +45,8 @@ struct CloudDev {
     const uint32_t* colors;
     const uint8_t* levels;
     const float4* tints;           // nullptr: zero stand-in
+    const float4* disp;            // displacement (xyz), nullptr: zero stand-in
+    const float4* prevDisp;        // previous frame's displacement (motion blur), nullptr: zero stand-in
     scr_visible_batch* visible;    // cull output
     scr::BatchBounds* bounds;      // cull output, one per visible entry (what its box proves)
     uint32_t* counter;             // visible count
@@ -78,6 +82,10 @@ struct scr_cloud {
     int filesNext = 0;
     float4* dTints = nullptr;
     bool tintsOwned = false;
+    float4* dDisp = nullptr;       // displacement buffer (owned), float4 per pool point
+    // per camera key: the displacements the last motion-blurred render of that
+    // camera drew (ComputeRasteriserPointCloud.swift:96-103), owned
+    std::map<uint64_t, float4*> prevDisp;
     scr_visible_batch* dVisible = nullptr;
     scr::BatchBounds* dBounds = nullptr;
     uint32_t* dCounter = nullptr;
@@ -93,6 +101,8 @@ struct scr_cloud {
         c.colors = dColors;
         c.levels = dLevels;
         c.tints = dTints;
+        c.disp = dDisp;
+        c.prevDisp = dDisp;
         c.visible = dVisible;
         c.bounds = dBounds;
         c.counter = dCounter;
@@ -110,8 +120,8 @@ struct scr_frame {
     unsigned long long* dSums = nullptr;     // differentiable mode: 3 int64 planes of planeStride (on first use)
     uint32_t* dRGBA = nullptr;
     uint32_t* dRGBAScratch = nullptr;  // hole fill ping-pong plane (on first use)
-    float prevVP[16];                  // last render's view-projection (motion-blur velocity)
-    bool hasPrevVP = false;
+    // last view-projection rendered per camera key (motion-blur velocity)
+    std::map<uint64_t, std::array<float, 16>> prevVP;
     float* dDepthF = nullptr;
     size_t pixels() const { return (size_t)width * (size_t)height; }
     // planes start on 256-byte boundaries so vector stores stay aligned

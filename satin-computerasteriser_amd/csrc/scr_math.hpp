@@ -211,6 +211,7 @@ struct PassParams {
     int colorizeChunks;
     int colorizeOverdraw;
     int applyTint;
+    int applyDisplacement;  // points are drawn at position + displacement (depth, colour, blend passes)
     int constRadius;        // >= 0: footprint radius is this constant (lo == hi); -1: per point
     int maxRadius;          // upper bound of the footprint radius (radius of hi)
     int ortho;              // proj[3][3] == 1

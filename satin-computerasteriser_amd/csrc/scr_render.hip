@@ -37,16 +37,19 @@ __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
 // The draw kernels are persistent grids: workgroup b takes visible batches
 // b, b + grid, ... up to the count the cull kernel left on the device.
 
-template <bool kR0>
+// kDisp: points drawn at position + displacement (a kernel of its own, so
+// the undisplaced kernels keep their code).
+template <bool kR0, bool kDisp>
 __global__ __launch_bounds__(kThreads, draw_waves(kPPT)) void k_depth(CloudDev c, PassParams p, uint32_t* __restrict__ depth) {
     __shared__ uint32_t lds[kDepthTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        depth_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], c.bounds + v, 0u, 0xffffffffu, depth, lds, red);
+        depth_batch<kR0, kWaveTiles, kThreads, kPPT, kDepthTile, kDisp>(c, p, c.visible[v], c.bounds + v, 0u,
+                                                                         0xffffffffu, depth, lds, red);
 }
 
-template <bool kR0>
+template <bool kR0, bool kDisp>
 __global__ __launch_bounds__(kThreads, kR0 ? 4 : 3) void k_color(CloudDev c, PassParams p, const uint32_t* __restrict__ depth,
                                                     uint32_t* red, uint32_t* green, uint32_t* blue,
                                                     uint32_t* count) {
@@ -57,7 +60,8 @@ __global__ __launch_bounds__(kThreads, kR0 ? 4 : 3) void k_color(CloudDev c, Pas
     const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
-        color_batch<kR0, kWaveTiles, kThreads, kPPT>(c, p, c.visible[v], c.bounds + v, 0u, 0xffffffffu, depth, red, green, blue, count, L);
+        color_batch<kR0, kWaveTiles, kThreads, kPPT, kDisp>(c, p, c.visible[v], c.bounds + v, 0u, 0xffffffffu, depth,
+                                                           red, green, blue, count, L);
 }
 
 __global__ __launch_bounds__(256) void k_resolve(PassParams p, const uint32_t* __restrict__ planes, size_t stride,
@@ -158,19 +162,26 @@ using namespace draw;
 // kCovPPT points per thread: their loads are issued together
 constexpr int kCovPPT = 4;
 
-template <int kLevel>
+template <int kLevel, bool kDisp>
 __device__ __forceinline__ void coverage_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
                                                uint32_t li0, bool vec, ChunkFrags<kCovPPT>& f) {
     ChunkRaw<kCovPPT> raw;
     chunk_fetch<kLevel, kCovPPT>(c, bc, li0, vec, raw);
-    chunk_decode<false, false, kLevel, kCovPPT>(c, p, bc, li0, raw, f);
+    chunk_decode<false, false, kLevel, kCovPPT, true, kDisp>(c, p, bc, li0, raw, f);
+}
+
+template <bool kDisp>
+__device__ __forceinline__ void coverage_front_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
+                                                     uint32_t li0, bool vec, ChunkFrags<kCovPPT>& f) {
+    if (bc.level == 0) coverage_front<0, kDisp>(c, p, bc, li0, vec, f);
+    else if (bc.level == 1) coverage_front<1, kDisp>(c, p, bc, li0, vec, f);
+    else coverage_front<2, kDisp>(c, p, bc, li0, vec, f);
 }
 
 __device__ __forceinline__ void coverage_front_any(const CloudDev& c, const PassParams& p, const BatchCtx& bc,
                                                    uint32_t li0, bool vec, ChunkFrags<kCovPPT>& f) {
-    if (bc.level == 0) coverage_front<0>(c, p, bc, li0, vec, f);
-    else if (bc.level == 1) coverage_front<1>(c, p, bc, li0, vec, f);
-    else coverage_front<2>(c, p, bc, li0, vec, f);
+    if (p.applyDisplacement != 0) coverage_front_level<true>(c, p, bc, li0, vec, f);
+    else coverage_front_level<false>(c, p, bc, li0, vec, f);
 }
 
 // depthPassUpdate in translucent-defocus mode (DepthPass/Shaders.metal:60-67):
@@ -344,7 +355,20 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
             int samples = 1;
             float sx = 0.0f, sy = 0.0f;
             if (q.motionBlur > 0.0f) {
-                const f4 prev = mul_point(file->prevTransform, f.wx[j], f.wy[j], f.wz[j]);
+                // from the decoded position + the previous displacement
+                // (ColorPass/Shaders.metal:150-153)
+                float bx = f.wx[j], by = f.wy[j], bz = f.wz[j];
+                if (p.applyDisplacement != 0 && c.prevDisp) {
+                    const float4 d = c.prevDisp[pi];
+                    bx = bx + d.x;
+                    by = by + d.y;
+                    bz = bz + d.z;
+                } else if (p.applyDisplacement != 0) {
+                    bx = bx + 0.0f;
+                    by = by + 0.0f;
+                    bz = bz + 0.0f;
+                }
+                const f4 prev = mul_point(file->prevTransform, bx, by, bz);
                 if (prev.w > 0.0f) {
                     const float prw = rcp(prev.w);
                     const int ppx = f2i_hw(fmaf(prev.x * prw, 0.5f, 0.5f) * (float)p.width);
@@ -529,21 +553,27 @@ hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p) {
 }
 
 hipError_t launch_depth(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, uint32_t* depth) {
-    if (p.constRadius == 0)
-        hipLaunchKernelGGL(k_depth<true>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kThreads), 0, s, c, p, depth);
-    else
-        hipLaunchKernelGGL(k_depth<false>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kThreads), 0, s, c, p, depth);
+    const dim3 grid(draw_grid(numCUs, c.nSlots));
+    const bool r0 = p.constRadius == 0, disp = p.applyDisplacement != 0;
+    if (disp && r0) hipLaunchKernelGGL((k_depth<true, true>), grid, dim3(kThreads), 0, s, c, p, depth);
+    else if (disp) hipLaunchKernelGGL((k_depth<false, true>), grid, dim3(kThreads), 0, s, c, p, depth);
+    else if (r0) hipLaunchKernelGGL((k_depth<true, false>), grid, dim3(kThreads), 0, s, c, p, depth);
+    else hipLaunchKernelGGL((k_depth<false, false>), grid, dim3(kThreads), 0, s, c, p, depth);
     return hipGetLastError();
 }
 
 hipError_t launch_color(hipStream_t s, int numCUs, const CloudDev& c, const PassParams& p, const uint32_t* depth,
                         uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count) {
-    if (p.constRadius == 0)
-        hipLaunchKernelGGL(k_color<true>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kThreads), 0, s, c, p, depth,
-                           red, green, blue, count);
+    const dim3 grid(draw_grid(numCUs, c.nSlots));
+    const bool r0 = p.constRadius == 0, disp = p.applyDisplacement != 0;
+    if (disp && r0)
+        hipLaunchKernelGGL((k_color<true, true>), grid, dim3(kThreads), 0, s, c, p, depth, red, green, blue, count);
+    else if (disp)
+        hipLaunchKernelGGL((k_color<false, true>), grid, dim3(kThreads), 0, s, c, p, depth, red, green, blue, count);
+    else if (r0)
+        hipLaunchKernelGGL((k_color<true, false>), grid, dim3(kThreads), 0, s, c, p, depth, red, green, blue, count);
     else
-        hipLaunchKernelGGL(k_color<false>, dim3(draw_grid(numCUs, c.nSlots)), dim3(kThreads), 0, s, c, p, depth,
-                           red, green, blue, count);
+        hipLaunchKernelGGL((k_color<false, false>), grid, dim3(kThreads), 0, s, c, p, depth, red, green, blue, count);
     return hipGetLastError();
 }
 

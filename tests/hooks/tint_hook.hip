@@ -1,5 +1,5 @@
 // A user tint kernel written against scr_device.h, built by build_hooks.py
-// into libtint_hook.so for the hook tests: points of one height band are
+// into libuser_hooks.so for the hook tests: points of one height band are
 // dropped, higher points take their inverted colour, lower ones a tint from
 // their x and slot offset.  The tests restate it on the CPU.
 #include "scr_device.h"

@@ -232,7 +232,7 @@ def test_torch_adam_fit_reduces_loss(ctx, diffscene):
 def test_diff_rejects_nearest_and_tint_modes(ctx, diffscene):
     cloud, ocloud, gt, cams = diffscene
     for kw in (dict(mode=scr.MODE_NEAREST_POINT), dict(applyTint=True), dict(holeFillIterations=2),
-               dict(motionBlur=0.5)):
+               dict(motionBlur=0.5), dict(applyDisplacement=True)):
         r = scr.DifferentiableHQS(ctx, cloud, [(W, H)], scr.ComputeRasteriserConfiguration(**kw))
         p = scr.DeviceBuffer(ctx, r.poolPoints * 16)
         img = scr.DeviceBuffer(ctx, W * H * 12)

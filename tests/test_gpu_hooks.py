@@ -15,7 +15,7 @@ from helpers import assert_planes_equal, gpu_cloud, oracle_cloud, render_gpu_hqs
 
 pytestmark = pytest.mark.gpu
 
-HOOK = Path(__file__).resolve().parent / "hooks" / "libtint_hook.so"
+HOOK = Path(__file__).resolve().parent / "hooks" / "libuser_hooks.so"
 
 
 def _hook():
@@ -24,6 +24,8 @@ def _hook():
     lib = C.CDLL(str(HOOK))
     lib.tint_hook_run.restype = C.c_int
     lib.tint_hook_run.argtypes = [C.c_void_p, C.c_float, C.c_float]
+    lib.displace_hook_run.restype = C.c_int
+    lib.displace_hook_run.argtypes = [C.c_void_p]
     return lib
 
 
@@ -88,3 +90,50 @@ def test_device_view_without_tints_and_after_set_tints(ctx):
     back = np.zeros_like(t)
     ctx.check(ctx.lib.scr_memcpy_d2h(ctx.handle, back.ctypes.data, v2.tints, back.nbytes))
     assert np.array_equal(back, t)
+
+
+def _cpu_displacements(o: orc.Cloud) -> np.ndarray:
+    """displace_wave (tests/hooks/displace_hook.hip) on the CPU, zero where no thread writes."""
+    f32 = np.float32
+    out = np.zeros((o.xyzLow.shape[0], 4), np.float32)
+    for s, b in enumerate(o.batches):
+        if b["state"] == 0:
+            continue
+        for local in range(int(b["numPoints"])):
+            i = int(b["firstPoint"]) + local
+            level = int(o.levels[i]) & 7
+            p = orc.decode_point(b, (o.xyzLow[i], o.xyzMed[i], o.xyzHigh[i]), min(level, 2), fused=True)
+            if local % 17 == 0:
+                out[i] = (0.0, np.nan, 0.0, 0.0)
+            else:
+                out[i] = (0.0, f32(local) * f32(1.0 / 4096.0), p[0] * f32(0.25), 0.0)
+    return out
+
+
+def test_user_displacement_kernel_matches_cpu_and_renders_like_the_oracle(ctx):
+    """DisplacementPass (DisplacementPass.swift:120-172): a HIP displacement
+    kernel built against scr_device.h writes the cloud's displacement buffer;
+    a render with applyDisplacement draws the displaced points, NaN ones
+    removed, as the oracle does."""
+    pos, col = jittered_cloud(6000, 0xD15)
+    packed = orc.pack(pos, col, pointsPerBatch=500, lodLevels=4)
+    g, o = gpu_cloud(ctx, packed), oracle_cloud(packed)
+    view = g.deviceView(withDisplacements=True)
+    assert view.displacements and not view.tints
+    assert _hook().displace_hook_run(C.addressof(view)) == 0
+    got = np.zeros((o.xyzLow.shape[0], 4), np.float32)
+    ctx.check(ctx.lib.scr_memcpy_d2h(ctx.handle, got.ctypes.data, view.displacements, got.nbytes))
+    want = _cpu_displacements(o)
+    assert np.isnan(want[:, 1]).sum() > 100
+    bad = np.argwhere(got.view(np.uint32) != want.view(np.uint32))
+    assert bad.size == 0, (bad[:4], got[bad[:4, 0]], want[bad[:4, 0]])
+    cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    o.disp = want
+    for kw in (dict(), dict(minimumPointSize=1, maximumPointSize=3, pointSizeScale=20.0)):
+        cfg = scr.ComputeRasteriserConfiguration(applyDisplacement=True, **kw)
+        gpu = render_gpu_hqs(ctx, [g], cfg, cam, 320, 240)
+        ref = orc.render_hqs([o], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+        still = orc.render_hqs([o], scr.ComputeRasteriserConfiguration(**kw), cam.viewMatrix, cam.projectionMatrix,
+                               320, 240)
+        assert ref["count"].sum() > 1000 and (ref["count"] != still["count"]).sum() > 500
+        assert_planes_equal(gpu, ref)

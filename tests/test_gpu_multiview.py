@@ -235,7 +235,8 @@ def test_multiview_argument_errors(ctx, scene):
         mv.tuning = {"color": {"splits": 3}}
     mv.close()
     # the weighted-blend passes and the hole fill belong to scr_render
-    for kw in (dict(holeFillIterations=1), dict(motionBlur=0.5), dict(applyTint=True, tintAlphaIsCoverage=True)):
+    for kw in (dict(holeFillIterations=1), dict(motionBlur=0.5), dict(applyTint=True, tintAlphaIsCoverage=True),
+               dict(applyDisplacement=True)):
         mv = scr.MultiViewRasteriser(ctx, clouds[:1], [(64, 64)], [0], scr.ComputeRasteriserConfiguration(**kw))
         cam = scr.Camera.perspective((5.0, 4.0, 6.0), (0, 0, 0), fov=50, aspect=1.0)
         with pytest.raises(scr.ScrError) as e:

@@ -423,3 +423,120 @@ def test_motion_blur_matches_oracle(ctx, jitter, coverage):
     still = orc.render_hqs([o], cfg, camB.viewMatrix, camB.projectionMatrix, 320, 240)
     assert (ref["count"] != still["count"]).sum() > 1000     # the sweep moved energy
     assert_planes_equal(gpu, ref)
+
+
+def _displacements(n, seed, nan_every=23):
+    rng = np.random.default_rng(seed)
+    d = np.zeros((n, 4), np.float32)
+    d[:, :3] = rng.normal(scale=0.15, size=(n, 3)).astype(np.float32)
+    d[::nan_every, int(seed) % 3] = np.nan                  # removed points
+    d[5::41, :3] *= np.float32(200.0)                        # thrown off the screen
+    return d
+
+
+@pytest.mark.parametrize("sizes", [dict(), dict(minimumPointSize=1, maximumPointSize=3, pointSizeScale=25.0)])
+def test_displacement_matches_oracle(ctx, jitter, sizes):
+    """applyDisplacement (DepthPass/Shaders.metal:69-74, ColorPass/Shaders.metal:
+    72-78): HQS planes of displaced points, NaN ones removed, bit for bit;
+    nearest mode ignores the displacements; a cloud without a buffer reads
+    zeros."""
+    cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    g, o = gpu_cloud(ctx, jitter), oracle_cloud(jitter)
+    d = _displacements(len(o.colors), 31)
+    g.setDisplacements(d)
+    o.disp = d
+    cfg = scr.ComputeRasteriserConfiguration(applyDisplacement=True, **sizes)
+    gpu = render_gpu_hqs(ctx, [g], cfg, cam, 320, 240)
+    ref = orc.render_hqs([o], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    still = orc.render_hqs([oracle_cloud(jitter)], scr.ComputeRasteriserConfiguration(**sizes), cam.viewMatrix,
+                           cam.projectionMatrix, 320, 240)
+    assert ref["count"].sum() > 1000 and (ref["count"] != still["count"]).sum() > 1000
+    assert_planes_equal(gpu, ref)
+    cfg.mode = scr.MODE_NEAREST_POINT
+    gpu = render_gpu_nearest(ctx, [g], cfg, cam, 320, 240)
+    ref = orc.render_nearest(oracle_cloud(jitter), cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert_planes_equal(gpu, ref)
+    # two clouds, one without a buffer (zeros), in one frame
+    bare = gpu_cloud(ctx, jitter, MODEL)
+    ob = oracle_cloud(jitter, MODEL)
+    cfg = scr.ComputeRasteriserConfiguration(applyDisplacement=True, **sizes)
+    gpu = render_gpu_hqs(ctx, [g, bare], cfg, cam, 320, 240)
+    ref = orc.render_hqs([o, ob], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240)
+    assert_planes_equal(gpu, ref)
+
+
+def test_displaced_motion_blur_and_defocus_per_camera(ctx, jitter):
+    """Motion blur sweeps from the pose and displacements the same camera key
+    last rendered (ComputeRasteriser.swift:255-266, 434-480): the previous
+    view-projection per key in the frame, the previous displacements per key
+    in the cloud (a copy made by each motion-blurred render; the current ones
+    before the first); translucent defocus on the same displaced points."""
+    camA = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    camB = scr.Camera.perspective((10.4, 7.5, 13.3), (0.2, 0, 0), fov=55, aspect=4 / 3)
+    cfg = scr.ComputeRasteriserConfiguration(applyDisplacement=True, motionBlur=0.6, motionBlurSamples=6,
+                                             motionBlurMaxSpread=10.0, minimumPointSize=1, maximumPointSize=2,
+                                             pointSizeScale=20.0)
+    g = gpu_cloud(ctx, jitter)
+    n = g.capacity[0] * g.capacity[1]
+    D0, D1 = _displacements(n, 7, 29), _displacements(n, 8, 31)
+    ras = scr.ComputeRasteriser(ctx, cfg)
+    ras.addPointCloud(g)
+    ras.resize(320, 240)
+
+    def check(cam, key, disp, prev_vp, prev_disp, what):
+        ras.render(cam, cameraKey=key)
+        gpu = {k: ras.read(v) for k, v in PLANES.items()}
+        gpu["depthF32"] = gpu["depthF32"].view(np.float32)
+        o = oracle_cloud(jitter)
+        o.disp, o.prevDisp = disp, prev_disp
+        ref = orc.render_hqs([o], cfg, cam.viewMatrix, cam.projectionMatrix, 320, 240, prev_view_proj=prev_vp)
+        assert ref["count"].sum() > 1000, what
+        assert_planes_equal(gpu, ref)
+        return ref
+
+    vpA = orc.mat_mul(camA.projectionMatrix, camA.viewMatrix)
+    vpB = orc.mat_mul(camB.projectionMatrix, camB.viewMatrix)
+    g.setDisplacements(D0)
+    check(camA, 5, D0, None, None, "first render of key 5: no velocity")
+    g.setDisplacements(D1)
+    moved = check(camB, 5, D1, vpA, D0, "key 5 again: camera and displacement velocity")
+    still = orc.render_hqs([oracle_cloud(jitter)], cfg, camB.viewMatrix, camB.projectionMatrix, 320, 240)
+    assert (moved["count"] != still["count"]).sum() > 1000
+    check(camB, 6, D1, None, None, "first render of key 6")
+    check(camA, 5, D1, vpB, D1, "key 5's history is its own")
+    # translucent defocus of the displaced points (no motion blur, no history)
+    cfg = scr.ComputeRasteriserConfiguration(applyDisplacement=True, applyTint=True, tintAlphaIsCoverage=True,
+                                             minimumPointSize=1, maximumPointSize=2, pointSizeScale=20.0)
+    t = _tints(n, 5)
+    g.setTints(t)
+    ras.configuration = cfg
+    ras.render(camA, cameraKey=9)
+    gpu = {k: ras.read(v) for k, v in PLANES.items()}
+    gpu["depthF32"] = gpu["depthF32"].view(np.float32)
+    o = oracle_cloud(jitter)
+    o.disp, o.tints = D1, t
+    ref = orc.render_hqs([o], cfg, camA.viewMatrix, camA.projectionMatrix, 320, 240)
+    assert_planes_equal(gpu, ref)
+
+
+def test_motion_blur_history_is_per_camera(ctx, jitter):
+    """Two cameras alternating into one frame (the two eyes of a stereo pair,
+    ComputeRasteriser.swift:255-266): each sweeps from its own previous pose."""
+    eyes = [scr.Camera.perspective((11.0 + dx, 7.0, 13.0), (dx, 0, 0), fov=55, aspect=4 / 3) for dx in (-0.3, 0.3)]
+    moved = [scr.Camera.perspective((10.5 + dx, 7.4, 13.2), (0.2 + dx, 0, 0), fov=55, aspect=4 / 3)
+             for dx in (-0.3, 0.3)]
+    cfg = scr.ComputeRasteriserConfiguration(motionBlur=0.6, motionBlurSamples=6, motionBlurMaxSpread=10.0,
+                                             minimumPointSize=1, maximumPointSize=2, pointSizeScale=20.0)
+    g, o = gpu_cloud(ctx, jitter), oracle_cloud(jitter)
+    ras = scr.ComputeRasteriser(ctx, cfg)
+    ras.addPointCloud(g)
+    ras.resize(320, 240)
+    for k in (0, 1):
+        ras.render(eyes[k], cameraKey=100 + k)
+    for k in (0, 1):
+        ras.render(moved[k], cameraKey=100 + k)
+        gpu = {name: ras.read(v) for name, v in PLANES.items()}
+        gpu["depthF32"] = gpu["depthF32"].view(np.float32)
+        prev = orc.mat_mul(eyes[k].projectionMatrix, eyes[k].viewMatrix)
+        ref = orc.render_hqs([o], cfg, moved[k].viewMatrix, moved[k].projectionMatrix, 320, 240, prev_view_proj=prev)
+        assert_planes_equal(gpu, ref)

@@ -26,6 +26,9 @@ MUTANTS = {
     9: ("the motion-blur sweep in the wrong direction", pins.pin_blend_sums_and_coverage),
     10: ("world-space point size from the view distance instead of the depth", pins.pin_footprint_radius),
     11: ("the near plane in the OpenGL w + z form", pins.pin_frustum),
+    12: ("no NaN cull of displaced points", pins.pin_displacement),
+    13: ("the motion-blur sweep from the displaced point instead of position + previous displacement",
+         pins.pin_displacement),
 }
 
 

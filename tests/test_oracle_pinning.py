@@ -667,3 +667,51 @@ def test_gpu_lod_gate_closed_forms():
     b = orc.pack(pos, col, pointsPerBatch=300, lodLevels=6, coarseVoxelDivisions=4)
     assert np.array_equal(a.levels, b.levels) and a.batches.tobytes() == b.batches.tobytes()
     assert len(set(a.levels.tolist())) > 2
+
+
+def pin_displacement():
+    """Displacement (DepthPass/Shaders.metal:69-74, ColorPass/Shaders.metal:
+    72-78, 150-153) for one point under an identity camera, 64 x 64: drawn at
+    position + displacement; a NaN component removes it (also from the
+    weighted blend, which has no depth test); a cloud without a buffer draws
+    undisplaced; nearest mode ignores it; the motion-blur sweep runs from the
+    current pixel back to that of position + previous displacement."""
+    I = np.eye(4, dtype=np.float32)
+    W = H = 64
+    base = dict(enableCLOD=False, enableFrustumCulling=False)
+    c = _one_point()
+    c.disp = np.float32([[0.5, 0.0, 0.0, 0.0]])
+    out = orc.render_hqs([c], scr.ComputeRasteriserConfiguration(applyDisplacement=True, **base), I, I, W, H)
+    assert out["count"][31, 48] == 1 and out["count"].sum() == 1               # x: int(0.75 * 64)
+    out = orc.render_hqs([c], scr.ComputeRasteriserConfiguration(**base), I, I, W, H)
+    assert out["count"][31, 32] == 1 and out["count"].sum() == 1                # applyDisplacement off
+    nearest = orc.render_nearest(c, scr.ComputeRasteriserConfiguration(mode=1, applyDisplacement=True, **base),
+                                 I, I, W, H)
+    assert nearest["index"][31, 32] == 0 and (nearest["depth"] > 0).sum() == 1
+    bare = _one_point()
+    out = orc.render_hqs([bare], scr.ComputeRasteriserConfiguration(applyDisplacement=True, **base), I, I, W, H)
+    assert out["count"][31, 32] == 1 and out["count"].sum() == 1                # no buffer: zeros
+    for nan_at in range(3):
+        c.disp = np.zeros((1, 4), np.float32)
+        c.disp[0, nan_at] = np.nan
+        for extra in (dict(), dict(applyTint=True, tintAlphaIsCoverage=True), dict(motionBlur=0.5)):
+            cfg = scr.ComputeRasteriserConfiguration(applyDisplacement=True, **base, **extra)
+            out = orc.render_hqs([c], cfg, I, I, W, H)
+            assert out["count"].sum() == 0 and out["depth"].sum() == 0, (nan_at, extra)
+    # sweep: current displacement (0.25, 0, 0) puts the point at column 40,
+    # previous (0, 0, 0) at 32; blur 0.5 gives 4 px: taps 40, 39, 37, 36
+    c.disp = np.float32([[0.25, 0.0, 0.0, 0.0]])
+    c.prevDisp = np.zeros((1, 4), np.float32)
+    cfg = scr.ComputeRasteriserConfiguration(applyDisplacement=True, motionBlur=0.5, motionBlurSamples=8, **base)
+    out = orc.render_hqs([c], cfg, I, I, W, H)
+    got = {(int(r), int(q)) for r, q in zip(*np.nonzero(out["count"]))}
+    assert got == {(31, 40), (31, 39), (31, 37), (31, 36)}, sorted(got)
+    assert all(out["count"][31, q] == 1024 for q in (40, 39, 37, 36))
+    # without a previous copy the sweep starts from the current displacement: no velocity
+    c.prevDisp = None
+    out = orc.render_hqs([c], cfg, I, I, W, H)
+    assert out["count"][31, 40] == 4096 and out["count"].sum() == 4096
+
+
+def test_displacement_closed_forms():
+    pin_displacement()

@@ -368,6 +368,43 @@ int scr_frame_read(scr_frame* frame, int which, void* host, size_t bytes);
  * an empty one takes the mean colour of its valid neighbours with alpha 1. */
 int scr_frame_hole_fill(scr_frame* frame, int iterations);
 
+/* ---- composite: ComputeRasteriser.draw (ComputeRasteriser.swift:672-683)
+ * and its fragment shaders (Pipelines/Post.metal:23-50).  Composites the
+ * frame's final RGBA8 plane (after any hole fill) onto a caller's colour
+ * target of the same size, enqueued on the context stream after the frame's
+ * last render:
+ *   SCR_COMPOSITE_OVER  (computeRasteriserPostFragment): every pixel, the cloud
+ *                       colour blended over the target, no depth;
+ *   SCR_COMPOSITE_DEPTH (computeRasteriserPostDepthFragment, writesSceneDepth):
+ *                       pixels where the cloud's reverse-Z depth d is <= 0 are
+ *                       left alone; elsewhere the fragment carries depth d, is
+ *                       kept where depthCompare(d, target depth) holds, writes
+ *                       d to the depth target when writeDepth, and blends.
+ * Blending and the depth compare are render-pipeline state the reference takes
+ * from its post material; here they are options: SCR_BLEND_REPLACE, or
+ * SCR_BLEND_OVER (non-premultiplied source-over: c = s.rgb * s.a + d.rgb * (1 -
+ * s.a), a = s.a + d.a * (1 - s.a), in binary32, bytes rounded to nearest-even).
+ * targetRGBA8: device, r | g<<8 | b<<16 | a<<24 per pixel; targetDepth: device
+ * floats (SCR_COMPOSITE_DEPTH only); pitch in pixels (>= width), row 0 = top. */
+#define SCR_COMPOSITE_OVER 0
+#define SCR_COMPOSITE_DEPTH 1
+#define SCR_BLEND_REPLACE 0
+#define SCR_BLEND_OVER 1
+#define SCR_COMPARE_ALWAYS 0
+#define SCR_COMPARE_GREATER 1
+#define SCR_COMPARE_GREATER_EQUAL 2
+#define SCR_COMPARE_LESS 3
+#define SCR_COMPARE_LESS_EQUAL 4
+typedef struct scr_composite_options {
+    int32_t mode;          /* SCR_COMPOSITE_* */
+    int32_t blend;         /* SCR_BLEND_* */
+    int32_t depthCompare;  /* SCR_COMPARE_*: fragment depth against target depth (reverse-Z: GREATER is nearer) */
+    int32_t writeDepth;    /* SCR_COMPOSITE_DEPTH: store the cloud depth where the fragment is kept */
+} scr_composite_options;
+void scr_composite_options_default(scr_composite_options* opt); /* DEPTH, OVER, GREATER, no depth write */
+int scr_frame_composite(scr_frame* frame, const scr_composite_options* opt, uint32_t* targetRGBA8, float* targetDepth,
+                        int targetPitch);
+
 /* ---- rendering: ComputeRasteriser.update (:201-268) + encode (:277-296).
  * HQS: clear → cull(all clouds) → depth(all) → colour(all) → resolve
  * (encodeHighQualityAverage :322-456).  Nearest: first cloud with resident

@@ -530,6 +530,67 @@ def pick_search(indices, width: int, height: int, ndcX: float, ndcY: float, radi
 
 
 # ---------------------------------------------------------------- post
+# Planted mistakes in the composite restatement, set only by the mutation
+# test (tests/test_oracle_mutants.py), which requires a pin to fail for each;
+# 0 is the oracle itself.
+COMPOSITE_MUTANT = 0
+
+COMPOSITE_OVER, COMPOSITE_DEPTH = 0, 1
+BLEND_REPLACE, BLEND_OVER = 0, 1
+COMPARE_ALWAYS, COMPARE_GREATER, COMPARE_GREATER_EQUAL, COMPARE_LESS, COMPARE_LESS_EQUAL = range(5)
+
+
+def composite(rgba8, depth, target, targetDepth=None, mode: int = COMPOSITE_DEPTH, blend: int = BLEND_OVER,
+              depthCompare: int = COMPARE_GREATER, writeDepth: bool = False):
+    """ComputeRasteriser.draw's composite of the cloud's colour and reverse-Z
+    depth onto a target (ComputeRasteriser.swift:672-683 draws a full-screen
+    quad with Post.metal:23-50).  The fragment function returns the colour
+    texel and, in the depth mode, discard_fragment() where depth <= 0
+    (Post.metal:44-46) and the depth as the fragment depth; the pipeline's
+    depth test (depthCompare against targetDepth, optional depth write) and
+    colour blend are the render pass's: SOURCE-OVER with non-premultiplied
+    source alpha, c = s.rgb * s.a + d.rgb * (1 - s.a), a = s.a + d.a * (1 - s.a),
+    or REPLACE.  UNORM8 reads are byte / 255 (correctly rounded binary32),
+    writes rint(saturate(x) * 255).  Returns (target, targetDepth) as new
+    arrays; (H, W) uint32 and float32."""
+    f32 = np.float32
+    c = np.ascontiguousarray(rgba8, np.uint32)
+    d = np.ascontiguousarray(depth, f32)
+    t = np.array(target, np.uint32, copy=True)
+    td = None if targetDepth is None else np.array(targetDepth, f32, copy=True)
+    keep = np.ones(c.shape, bool)
+    m = COMPOSITE_MUTANT
+    if mode == COMPOSITE_DEPTH:
+        keep = (d >= 0) if m == 16 else (d > 0)
+        if depthCompare == COMPARE_GREATER:
+            keep &= (d >= td) if m == 17 else (d > td)
+        elif depthCompare == COMPARE_GREATER_EQUAL:
+            keep &= d >= td
+        elif depthCompare == COMPARE_LESS:
+            keep &= d < td
+        elif depthCompare == COMPARE_LESS_EQUAL:
+            keep &= d <= td
+        if writeDepth:
+            td = np.where(keep | (m == 18), d, td).astype(f32)
+    if blend == BLEND_REPLACE:
+        out = c
+    else:
+        def ch(x, k):
+            return ((x >> np.uint32(8 * k)) & np.uint32(0xFF)).astype(f32) / f32(255)
+
+        def u8(x):
+            return np.rint(np.clip(x.astype(f32), f32(0), f32(1)) * f32(255)).astype(np.uint32)
+
+        sa = ch(c, 3)
+        k = sa if m == 14 else (f32(1) - sa).astype(f32)
+        out = u8(sa if m == 15 else (sa + (ch(t, 3) * k).astype(f32)).astype(f32)) << np.uint32(24)
+        for j in range(3):
+            v = ((ch(c, j) * sa).astype(f32) + (ch(t, j) * k).astype(f32)).astype(f32)
+            out |= u8(v) << np.uint32(8 * j)
+    t = np.where(keep, out, t).astype(np.uint32)
+    return t, td
+
+
 def hole_fill(rgba8: np.ndarray, iterations: int = 1) -> np.ndarray:
     """holeFillUpdate (HoleFill/Shaders.metal:10-48), ``iterations`` passes over
     an (H, W) uint32 RGBA8 image: a texel with alpha byte >= 128 is kept; an

@@ -105,6 +105,15 @@ class CConfig(C.Structure):
     ]
 
 
+class CCompositeOptions(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("blend", C.c_int32), ("depthCompare", C.c_int32), ("writeDepth", C.c_int32)]
+
+
+COMPOSITE_OVER, COMPOSITE_DEPTH = 0, 1
+BLEND_REPLACE, BLEND_OVER = 0, 1
+COMPARE_ALWAYS, COMPARE_GREATER, COMPARE_GREATER_EQUAL, COMPARE_LESS, COMPARE_LESS_EQUAL = range(5)
+
+
 class CCamera(C.Structure):
     _fields_ = [("viewMatrix", C.c_float * 16), ("projectionMatrix", C.c_float * 16)]
 
@@ -193,6 +202,8 @@ _SIGNATURES = {
     "scr_frame_create": (_I, [_VP, _I, _I, C.POINTER(_VP)]),
     "scr_frame_destroy": (_I, [_VP]),
     "scr_frame_hole_fill": (_I, [_VP, _I]),
+    "scr_composite_options_default": (None, [C.POINTER(CCompositeOptions)]),
+    "scr_frame_composite": (_I, [_VP, C.POINTER(CCompositeOptions), _VP, _VP, _I]),
     "scr_frame_read": (_I, [_VP, _I, _VP, _SZ]),
     "scr_render": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), _VP, _I, _VP]),
     "scr_render_camera": (_I, [_VP, C.POINTER(CConfig), C.POINTER(CCamera), C.c_uint64, _VP, _I, _VP]),
@@ -694,6 +705,18 @@ class Frame:
     def holeFill(self, iterations: int = 3) -> None:
         """HoleFillProcessor.update(_:iterations:) over the RGBA8 output, in place."""
         self.ctx.check(self.ctx.lib.scr_frame_hole_fill(self.handle, int(iterations)))
+
+    def composite(self, targetRGBA8: int, targetDepth: Optional[int] = None, pitch: Optional[int] = None,
+                  mode: int = COMPOSITE_DEPTH, blend: int = BLEND_OVER, depthCompare: int = COMPARE_GREATER,
+                  writeDepth: bool = False) -> None:
+        """ComputeRasteriser.draw (ComputeRasteriser.swift:672-683, Post.metal:23-50):
+        the frame's colour and depth onto a caller's device target (uint32
+        RGBA8 texels and float32 depths, ``pitch`` texels a row), enqueued on
+        the context stream; order the target's producers and consumers
+        against it with sync()."""
+        o = CCompositeOptions(int(mode), int(blend), int(depthCompare), int(bool(writeDepth)))
+        self.ctx.check(self.ctx.lib.scr_frame_composite(self.handle, C.byref(o), targetRGBA8, targetDepth,
+                                                        self.width if pitch is None else int(pitch)))
 
     def read(self, which: int) -> np.ndarray:
         dt = np.float32 if which == PLANE_DEPTH_F32 else (

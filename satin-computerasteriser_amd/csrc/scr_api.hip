@@ -988,6 +988,37 @@ int scr_frame_read(scr_frame* f, int which, void* host, size_t bytes) {
     return SCR_OK;
 }
 
+// ------------------------------------------------------------------ composite
+
+void scr_composite_options_default(scr_composite_options* o) {
+    if (!o) return;
+    o->mode = SCR_COMPOSITE_DEPTH;
+    o->blend = SCR_BLEND_OVER;
+    o->depthCompare = SCR_COMPARE_GREATER;
+    o->writeDepth = 0;
+}
+
+int scr_frame_composite(scr_frame* frame, const scr_composite_options* opt, uint32_t* targetRGBA8, float* targetDepth,
+                        int targetPitch) {
+    if (!frame) return SCR_ERR_INVALID_ARGUMENT;
+    scr_ctx* ctx = frame->ctx;
+    if (!opt || !targetRGBA8) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "NULL options or colour target");
+    if (opt->mode != SCR_COMPOSITE_OVER && opt->mode != SCR_COMPOSITE_DEPTH)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown composite mode");
+    if (opt->blend != SCR_BLEND_REPLACE && opt->blend != SCR_BLEND_OVER)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown blend");
+    if (opt->depthCompare < SCR_COMPARE_ALWAYS || opt->depthCompare > SCR_COMPARE_LESS_EQUAL)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "unknown depth compare");
+    if (opt->mode == SCR_COMPOSITE_DEPTH && !targetDepth)
+        return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "the depth composite needs a depth target");
+    if (targetPitch < frame->width) return set_error(ctx, SCR_ERR_INVALID_ARGUMENT, "target pitch below the width");
+    scr::PassScope ps(ctx, "composite");
+    SCR_HIP(ctx, scr::launch_composite(ctx->stream, frame->dRGBA, frame->dDepthF, frame->width, frame->height, *opt,
+                                       targetRGBA8, opt->mode == SCR_COMPOSITE_DEPTH ? targetDepth : nullptr,
+                                       targetPitch));
+    return SCR_OK;
+}
+
 // ------------------------------------------------------------------ render
 
 int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* const* clouds, int nClouds,

@@ -1289,6 +1289,47 @@ __device__ __forceinline__ uint32_t hole_fill_texels(const uint32_t* c0, int pit
     return pack_rgba(sr * r, sg * r, sb * r, 1.0f);
 }
 
+// The composite of ComputeRasteriser.draw (Post.metal:23-50) for one pixel:
+// cloud texel c (RGBA8) with reverse-Z depth d onto target texel *t and depth
+// *td.  The depth mode discards where d <= 0 (no cloud), keeps the fragment
+// where the depth compare holds, optionally stores d; blending as the options
+// say, with UNORM8 texels read as byte / 255 and written as rint(saturate(x) * 255).
+__device__ __forceinline__ float unorm8_in(uint32_t texel, int ch) {
+    return (float)((texel >> (8 * ch)) & 0xffu) / 255.0f;
+}
+
+__device__ __forceinline__ void composite_pixel(uint32_t c, float d, uint32_t* t, float* td,
+                                                const scr_composite_options& o) {
+    if (o.mode == SCR_COMPOSITE_DEPTH) {
+        if (d <= 0.0f) return;  // discard_fragment (Post.metal:44-46)
+        const float cur = *td;
+        bool pass = true;
+        switch (o.depthCompare) {
+            case SCR_COMPARE_GREATER: pass = d > cur; break;
+            case SCR_COMPARE_GREATER_EQUAL: pass = d >= cur; break;
+            case SCR_COMPARE_LESS: pass = d < cur; break;
+            case SCR_COMPARE_LESS_EQUAL: pass = d <= cur; break;
+            default: break;
+        }
+        if (!pass) return;
+        if (o.writeDepth) *td = d;
+    }
+    if (o.blend == SCR_BLEND_REPLACE) {
+        *t = c;
+        return;
+    }
+    const uint32_t dst = *t;
+    const float sa = unorm8_in(c, 3), k = 1.0f - sa;
+    uint32_t out = 0u;
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) {
+        const float v = unorm8_in(c, ch) * sa + unorm8_in(dst, ch) * k;
+        out |= unorm8(v) << (8 * ch);
+    }
+    out |= unorm8(sa + unorm8_in(dst, 3) * k) << 24;
+    *t = out;
+}
+
 // nearestResolveUpdate — NearestResolve/Shaders.metal:8-38
 __device__ __forceinline__ void nearest_resolve_pixel(const PassParams& p, const unsigned long long* __restrict__ packed,
                                              const uint32_t* __restrict__ colors, size_t i, uint32_t* __restrict__ rgba,

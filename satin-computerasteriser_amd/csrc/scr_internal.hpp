@@ -179,6 +179,8 @@ hipError_t launch_color_coverage(hipStream_t s, int numCUs, const CloudDev& c, c
 hipError_t launch_resolve_coverage(hipStream_t s, const PassParams& p, const uint32_t* planes, size_t stride,
                                    uint32_t* rgba, float* depthf);
 hipError_t launch_hole_fill(hipStream_t s, const uint32_t* in, uint32_t* out, int w, int h);
+hipError_t launch_composite(hipStream_t s, const uint32_t* rgba, const float* depthf, int w, int h,
+                            const scr_composite_options& o, uint32_t* target, float* targetDepth, int pitch);
 hipError_t launch_check_fast_rcp(hipStream_t s, int numCUs, unsigned long long* mismatches, uint32_t* first);
 hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t spots, int add, int iters,
                               uint64_t* atomics);

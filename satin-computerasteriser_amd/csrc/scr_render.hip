@@ -505,6 +505,26 @@ __global__ __launch_bounds__(256) void k_hole_fill(const uint32_t* __restrict__ 
     out[(size_t)y * w + x] = draw::hole_fill_texels(t + (ly + 1) * 18 + (lx + 1), 18);
 }
 
+// ComputeRasteriser.draw's composite (Post.metal:23-50), one thread per pixel.
+__global__ __launch_bounds__(256) void k_composite(const uint32_t* __restrict__ rgba, const float* __restrict__ depthf,
+                                                   int w, int h, scr_composite_options o, uint32_t* __restrict__ target,
+                                                   float* __restrict__ targetDepth, int pitch) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)w * (size_t)h) return;
+    const size_t x = i % (size_t)w, y = i / (size_t)w;
+    const size_t ti = y * (size_t)pitch + x;
+    draw::composite_pixel(rgba[i], depthf[i], target + ti, targetDepth ? targetDepth + ti : nullptr, o);
+}
+
+hipError_t launch_composite(hipStream_t s, const uint32_t* rgba, const float* depthf, int w, int h,
+                            const scr_composite_options& o, uint32_t* target, float* targetDepth, int pitch) {
+    const size_t n = (size_t)w * (size_t)h;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_composite, dim3(grid_for(n, 256)), dim3(256), 0, s, rgba, depthf, w, h, o, target,
+                       targetDepth, pitch);
+    return hipGetLastError();
+}
+
 hipError_t launch_hole_fill(hipStream_t s, const uint32_t* in, uint32_t* out, int w, int h) {
     const dim3 grid((unsigned)(w + 15) / 16u, (unsigned)(h + 15) / 16u);
     hipLaunchKernelGGL(k_hole_fill, grid, dim3(256), 0, s, in, out, w, h);

@@ -540,3 +540,74 @@ def test_motion_blur_history_is_per_camera(ctx, jitter):
         prev = orc.mat_mul(eyes[k].projectionMatrix, eyes[k].viewMatrix)
         ref = orc.render_hqs([o], cfg, moved[k].viewMatrix, moved[k].projectionMatrix, 320, 240, prev_view_proj=prev)
         assert_planes_equal(gpu, ref)
+
+
+COMPOSITE_CASES = [
+    dict(mode=scr.COMPOSITE_DEPTH, blend=scr.BLEND_OVER, depthCompare=scr.COMPARE_GREATER, writeDepth=False),
+    dict(mode=scr.COMPOSITE_DEPTH, blend=scr.BLEND_OVER, depthCompare=scr.COMPARE_GREATER_EQUAL, writeDepth=True),
+    dict(mode=scr.COMPOSITE_DEPTH, blend=scr.BLEND_REPLACE, depthCompare=scr.COMPARE_LESS, writeDepth=True),
+    dict(mode=scr.COMPOSITE_DEPTH, blend=scr.BLEND_REPLACE, depthCompare=scr.COMPARE_LESS_EQUAL, writeDepth=False),
+    dict(mode=scr.COMPOSITE_DEPTH, blend=scr.BLEND_OVER, depthCompare=scr.COMPARE_ALWAYS, writeDepth=True),
+    dict(mode=scr.COMPOSITE_OVER, blend=scr.BLEND_OVER, depthCompare=scr.COMPARE_GREATER, writeDepth=False),
+    dict(mode=scr.COMPOSITE_OVER, blend=scr.BLEND_REPLACE, depthCompare=scr.COMPARE_GREATER, writeDepth=False),
+]
+
+
+@pytest.mark.parametrize("opts", COMPOSITE_CASES)
+@pytest.mark.parametrize("pitch_pad", [0, 37])
+def test_composite_onto_a_target_matches_oracle(ctx, jitter, opts, pitch_pad):
+    # ComputeRasteriser.draw: the frame's colour and depth onto a caller's
+    # colour + depth target with random content (pitch wider than the frame
+    # in one case: the padding must stay untouched)
+    w, h = 320, 240
+    cam = scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3)
+    cfg = scr.ComputeRasteriserConfiguration(applyTint=True, tintAlphaIsCoverage=True, minimumPointSize=1,
+                                             maximumPointSize=3, pointSizeScale=20.0)
+    g = gpu_cloud(ctx, jitter)
+    g.setTints(_tints(len(oracle_cloud(jitter).colors), 33))
+    ras = scr.ComputeRasteriser(ctx, cfg)
+    ras.addPointCloud(g)
+    ras.resize(w, h)
+    ras.render(cam)
+    rgba, depth = ras.read(scr.PLANE_RGBA8), ras.read(scr.PLANE_DEPTH_F32).view(np.float32)
+    alpha = rgba >> 24
+    assert (depth > 0).mean() > 0.05 and ((alpha > 0) & (alpha < 255)).sum() > 500
+    pitch = w + pitch_pad
+    rng = np.random.default_rng(7 + pitch_pad)
+    target = rng.integers(0, 2**32, (h, pitch), dtype=np.uint64).astype(np.uint32)
+    # target depths straddle the cloud's, with exact ties on some texels
+    tdepth = rng.uniform(0.0, float(depth.max()) * 1.2, (h, pitch)).astype(np.float32)
+    tdepth[:, :w][::5] = depth[::5]
+    tb = scr.DeviceBuffer.from_array(ctx, target)
+    db = scr.DeviceBuffer.from_array(ctx, tdepth)
+    ras.frame.composite(tb.ptr, db.ptr, pitch, **opts)
+    got = tb.to_array(np.uint32, h * pitch).reshape(h, pitch)
+    gotd = db.to_array(np.float32, h * pitch).reshape(h, pitch)
+    want, wantd = orc.composite(rgba, depth, target[:, :w], tdepth[:, :w], **opts)
+    assert (want != target[:, :w]).sum() > 1000
+    bad = np.argwhere(got[:, :w] != want)
+    assert bad.size == 0, (opts, bad[:4], got[tuple(bad[0])], want[tuple(bad[0])])
+    assert np.array_equal(got[:, w:], target[:, w:])
+    if opts["mode"] == scr.COMPOSITE_DEPTH:
+        assert np.array_equal(gotd[:, :w].view(np.uint32), wantd.view(np.uint32))
+    assert np.array_equal(gotd[:, w:].view(np.uint32), tdepth[:, w:].view(np.uint32))
+
+
+def test_composite_rejects_bad_arguments(ctx, jitter):
+    ras = scr.ComputeRasteriser(ctx, scr.ComputeRasteriserConfiguration())
+    ras.addPointCloud(gpu_cloud(ctx, jitter))
+    ras.resize(64, 48)
+    ras.render(scr.Camera.perspective((11.0, 7.0, 13.0), (0, 0, 0), fov=55, aspect=4 / 3))
+    tb = scr.DeviceBuffer(ctx, 64 * 48 * 4)
+    db = scr.DeviceBuffer(ctx, 64 * 48 * 4)
+    with pytest.raises(scr.ScrError, match="depth target"):
+        ras.frame.composite(tb.ptr, None)
+    with pytest.raises(scr.ScrError, match="pitch"):
+        ras.frame.composite(tb.ptr, db.ptr, pitch=63)
+    with pytest.raises(scr.ScrError, match="compare"):
+        ras.frame.composite(tb.ptr, db.ptr, depthCompare=9)
+    with pytest.raises(scr.ScrError, match="mode"):
+        ras.frame.composite(tb.ptr, db.ptr, mode=5)
+    # the colour-only mode needs no depth target
+    ras.frame.composite(tb.ptr, None, mode=scr.COMPOSITE_OVER)
+    ctx.sync()

@@ -31,6 +31,15 @@ MUTANTS = {
          pins.pin_displacement),
 }
 
+# planted in the composite's numpy restatement (oracle.COMPOSITE_MUTANT = k)
+PY_MUTANTS = {
+    14: ("source-over's destination weight a instead of 1 - a", pins.pin_composite),
+    15: ("source-over alpha without the destination term", pins.pin_composite),
+    16: ("depth 0 texels drawn in the depth mode", pins.pin_composite),
+    17: ("the greater compare passing ties", pins.pin_composite),
+    18: ("the depth written where the compare fails", pins.pin_composite),
+}
+
 
 @pytest.fixture(scope="module")
 def mutant_libs(tmp_path_factory):
@@ -57,3 +66,15 @@ def test_each_planted_mistake_fails_its_pin(mutant_libs, k):
     finally:
         orc.use_library(prev)
     pin()  # and the real oracle passes it
+
+
+@pytest.mark.parametrize("k", sorted(PY_MUTANTS))
+def test_each_planted_composite_mistake_fails_its_pin(k):
+    what, pin = PY_MUTANTS[k]
+    orc.COMPOSITE_MUTANT = k
+    try:
+        with pytest.raises(AssertionError):
+            pin()
+    finally:
+        orc.COMPOSITE_MUTANT = 0
+    pin()

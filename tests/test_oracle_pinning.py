@@ -715,3 +715,65 @@ def pin_displacement():
 
 def test_displacement_closed_forms():
     pin_displacement()
+
+
+def pin_composite():
+    """ComputeRasteriser.draw's composite (Post.metal:23-50, ComputeRasteriser.
+    swift:672-683) against exact arithmetic and closed forms."""
+    # source-over on every source byte and alpha against a spread of target
+    # bytes: the byte written is the exact rational (s*a + t*(255 - a)) / 255
+    # rounded to nearest, wherever that value is not within 1e-3 of a tie;
+    # alpha is a + ta*(255 - a) / 255 the same way
+    s_, a_, t_ = np.meshgrid(np.arange(256), np.arange(256), np.arange(0, 256, 5), indexing="ij")
+    s_, a_, t_ = (x.ravel().astype(np.int64) for x in (s_, a_, t_))
+    src = (s_ | (s_ << 8) | (s_ << 16) | (a_ << 24)).astype(np.uint32)
+    dst = (t_ | ((255 - t_) << 8) | (t_ << 16) | (t_ << 24)).astype(np.uint32)
+    one = np.ones(src.shape, np.float32)
+    out, _ = orc.composite(src, one, dst, one * 0, mode=orc.COMPOSITE_OVER)
+    for shift, dv in ((0, t_), (8, 255 - t_), (16, t_)):
+        num = s_ * a_ + dv * (255 - a_)                 # value * 255
+        exact = num / 255.0
+        clear = np.abs(exact - np.floor(exact) - 0.5) > 1e-3
+        want = np.floor(exact + 0.5).astype(np.int64)
+        got = (out >> shift) & 0xFF
+        assert np.array_equal(got[clear], want[clear]), shift
+    exact = a_ + t_ * (255 - a_) / 255.0
+    clear = np.abs(exact - np.floor(exact) - 0.5) > 1e-3
+    assert np.array_equal(((out >> 24) & 0xFF)[clear], np.floor(exact + 0.5).astype(np.int64)[clear])
+    # opaque sources replace, transparent ones leave the target
+    assert np.array_equal(out[a_ == 255], src[a_ == 255])
+    assert np.array_equal(out[a_ == 0], dst[a_ == 0])
+    # depth mode: no cloud (depth <= 0, -0 included) never draws; the compare
+    # decides ties; the depth write follows the compare
+    d = np.float32([0.0, -0.0, -1.0, 0.5, 0.5, 0.5, 0.25])
+    td = np.float32([0.0, 0.0, 0.0, 0.25, 0.5, 0.75, 0.25])
+    c = np.full(7, 0xFF102030, np.uint32)
+    t = np.full(7, 0x80405060, np.uint32)
+    drawn = {orc.COMPARE_ALWAYS: [0, 0, 0, 1, 1, 1, 1], orc.COMPARE_GREATER: [0, 0, 0, 1, 0, 0, 0],
+             orc.COMPARE_GREATER_EQUAL: [0, 0, 0, 1, 1, 0, 1], orc.COMPARE_LESS: [0, 0, 0, 0, 0, 1, 0],
+             orc.COMPARE_LESS_EQUAL: [0, 0, 0, 0, 1, 1, 1]}
+    for cmp, want in drawn.items():
+        for write in (False, True):
+            o, od = orc.composite(c, d, t, td, depthCompare=cmp, writeDepth=write, blend=orc.BLEND_REPLACE)
+            w = np.array(want, bool)
+            assert np.array_equal(o == c, w), (cmp, write)
+            assert np.array_equal(od, np.where(w, d, td) if write else td), (cmp, write)
+    # the colour-only mode draws every texel, whatever the depths
+    o, _ = orc.composite(c, d, t, None, mode=orc.COMPOSITE_OVER, blend=orc.BLEND_REPLACE)
+    assert np.array_equal(o, c)
+    # a depth-tested replace with depth writes is a z-buffer: two layers with
+    # distinct depths give the same image in either order (greater = nearer)
+    rng = np.random.default_rng(5)
+    ca, cb, tt = (rng.integers(0, 2**32, 4096, dtype=np.uint64).astype(np.uint32) for _ in range(3))
+    da, db = (rng.choice(np.float32([0.0, 0.1, 0.2, 0.3, 0.4]), 4096) for _ in range(2))
+    db = np.where(db == da, np.float32(0.45), db).astype(np.float32)
+    z0 = np.zeros(4096, np.float32)
+    kw = dict(blend=orc.BLEND_REPLACE, writeDepth=True)
+    ab = orc.composite(cb, db, *orc.composite(ca, da, tt, z0, **kw), **kw)
+    ba = orc.composite(ca, da, *orc.composite(cb, db, tt, z0, **kw), **kw)
+    assert np.array_equal(ab[0], ba[0]) and np.array_equal(ab[1], ba[1])
+    assert np.array_equal(ab[1], np.maximum(da, db))
+
+
+def test_composite_closed_forms():
+    pin_composite()

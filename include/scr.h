@@ -139,7 +139,8 @@ int scr_ctx_create(int device, scr_ctx** out);
 int scr_ctx_destroy(scr_ctx* ctx);
 const char* scr_last_error(const scr_ctx* ctx);
 int scr_sync(scr_ctx* ctx);
-/* Version string of the library build (never NULL). */
+/* Version string of the library build (never NULL): version, target and the
+ * first 12 hex digits of the SHA-256 of the sources it was built from. */
 const char* scr_version(void);
 
 /* ---- per-pass timing (HIP events on the context stream; the reference's

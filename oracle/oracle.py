@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
             "orc_update_files": (None, [vp, C.c_int, vp, vp, vp, vp, vp]),
             "orc_cull": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp]),
             "orc_render_hqs": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.POINTER(OrcPlanes), C.c_int]),
+            "orc_last_hqs_pass_seconds": (None, [vp]),
             "orc_render_nearest": (C.c_int, [C.POINTER(OrcCloud), vp, vp, vp, C.c_int, C.c_int, vp, vp, vp, vp,
                                              C.c_int]),
             "orc_nearest_cpu_reference": (None, [C.POINTER(OrcCloud), C.c_int, C.c_int, vp, vp, C.c_int, vp, vp]),
@@ -446,6 +447,13 @@ def render_hqs(clouds: Sequence[Cloud], cfg, view, proj, width: int, height: int
     if rc != 0:
         raise RuntimeError("orc_render_hqs failed")
     return {k: v.reshape(height, width) for k, v in planes.items()}
+
+
+def last_hqs_pass_seconds() -> dict:
+    """Wall-clock seconds of the phases of the process's last render_hqs."""
+    out = np.zeros(5, np.float64)
+    lib().orc_last_hqs_pass_seconds(out.ctypes.data)
+    return dict(zip(("clear", "cull", "depth", "color", "resolve"), out.tolist()))
 
 
 def render_nearest(cloud: Cloud, cfg, view, proj, width: int, height: int, threads: int = 1) -> dict:

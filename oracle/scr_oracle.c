@@ -11,6 +11,7 @@
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define ORC_TWO32 4294967296.0f
 #define ORC_INV_TWO32 2.3283064365386963e-10f /* 2^-32, exact */
@@ -672,15 +673,31 @@ static uint32_t orc_pack_rgba(float r, float g, float b, float a) {
            ((uint32_t)orc_unorm8(a) << 24);
 }
 
+/* Wall-clock seconds of the phases of the process's last orc_render_hqs:
+ * clear, cull, depth, colour, resolve (for the CPU baseline's per-pass split). */
+static double orc_hqs_seconds[5];
+
+static double orc_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+void orc_last_hqs_pass_seconds(double out[5]) { memcpy(out, orc_hqs_seconds, sizeof orc_hqs_seconds); }
+
 int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, const float* view,
                    const float* proj, int width, int height, orc_hqs_planes* out, int nThreads) {
     const size_t npx = (size_t)width * (size_t)height;
+    double tick = orc_now();
+    memset(orc_hqs_seconds, 0, sizeof orc_hqs_seconds);
+#define ORC_LAP(k) do { const double t_ = orc_now(); orc_hqs_seconds[k] += t_ - tick; tick = t_; } while (0)
     /* clearUpdate — Clear/Shaders.metal:7-23 */
     memset(out->depth, 0, npx * 4);
     memset(out->red, 0, npx * 4);
     memset(out->green, 0, npx * 4);
     memset(out->blue, 0, npx * 4);
     memset(out->count, 0, npx * 4);
+    ORC_LAP(0);
 
     int total = 0;
     for (int c = 0; c < nClouds; c++) total += clouds[c].nSlots;
@@ -695,6 +712,7 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
     free(tmp);
     if (nThreads < 1) nThreads = 1;
     if (nThreads > nItems) nThreads = nItems > 0 ? nItems : 1;
+    ORC_LAP(1);
 
     /* depth phase (all clouds), then colour phase (all clouds):
      * ComputeRasteriser.swift:344-353 */
@@ -733,6 +751,7 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
         if (!ok) break;
         if (nThreads == 1) {
             orc_job_main(&jobs[0]);
+            ORC_LAP(2 + phase);
             continue;
         }
         for (int t = 0; t < nThreads; t++) pthread_create(&th[t], NULL, orc_job_main, &jobs[t]);
@@ -752,6 +771,7 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
             free(priv[t]);
             priv[t] = NULL;
         }
+        ORC_LAP(2 + phase);
     }
     if (priv) for (int t = 0; t < nThreads; t++) free(priv[t]);
     free(priv);
@@ -779,6 +799,7 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
                                               (float)out->blue[p] * inv, alpha);
             if (out->depthF32) out->depthF32[p] = orc_uint_to_depth(out->depth[p]);
         }
+        ORC_LAP(4);
         return 0;
     }
     /* resolveUpdate (count-average path) — Resolve/Shaders.metal:13-57 */
@@ -798,6 +819,8 @@ int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, 
         if (out->rgba8) out->rgba8[p] = orc_pack_rgba(r, g, b, 1.0f);
         if (out->depthF32) out->depthF32[p] = orc_uint_to_depth(out->depth[p]);
     }
+    ORC_LAP(4);
+#undef ORC_LAP
     return 0;
 }
 

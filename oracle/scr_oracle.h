@@ -100,6 +100,9 @@ typedef struct orc_hqs_planes {
     uint32_t* rgba8;
     float* depthF32;
 } orc_hqs_planes;
+/* Wall-clock seconds of the phases of the process's last orc_render_hqs
+ * (clear, cull, depth, colour, resolve). */
+void orc_last_hqs_pass_seconds(double out[5]);
 int orc_render_hqs(const orc_cloud* clouds, int nClouds, const scr_config* cfg, const float* view,
                    const float* proj, int width, int height, orc_hqs_planes* out, int nThreads);
 

@@ -35,6 +35,19 @@ COMMON_FLAGS = [
 ]
 
 
+def source_hash() -> str:
+    """First 12 hex digits of the SHA-256 of the library's sources (csrc and
+    include, in name order): scr_version() reports it, so measurements kept
+    beside the code can name the build they were taken on."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted(list(CSRC.glob("*")) + list((REPO / "include").glob("*.h"))):
+        if f.is_file() and f.suffix in (".hip", ".hpp", ".cpp", ".h"):
+            h.update(f.name.encode())
+            h.update(f.read_bytes())
+    return h.hexdigest()[:12]
+
+
 def _hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():
@@ -48,10 +61,10 @@ MV_DRAW = "scr_mv_draw.hip"
 MV_UNITS = [(ph, sh) for ph in range(3) for sh in range(6)]
 
 
-def _compile(unit, hipcc: str) -> Path:
+def _compile(unit, hipcc: str, src_hash: str) -> Path:
     src, defines, stem = unit
     obj = BUILD / (stem + ".o")
-    cmd = [hipcc, *COMMON_FLAGS, *defines, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc, *COMMON_FLAGS, f'-DSCR_SOURCE_HASH="{src_hash}"', *defines, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {stem}:\n{res.stderr}")
@@ -76,7 +89,8 @@ def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     units = _units()
     with cf.ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, 16, len(units))) as ex:
-        objs = list(ex.map(lambda u: _compile(u, hipcc), units))
+        src_hash = source_hash()
+        objs = list(ex.map(lambda u: _compile(u, hipcc, src_hash), units))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -275,7 +275,10 @@ void free_pool(scr_cloud* c) {
 
 extern "C" {
 
-const char* scr_version(void) { return "scr 0.1 (gfx950)"; }
+#ifndef SCR_SOURCE_HASH
+#define SCR_SOURCE_HASH "unknown"
+#endif
+const char* scr_version(void) { return "scr 0.1 (gfx950, source " SCR_SOURCE_HASH ")"; }
 
 void scr_config_default(scr_config* c) {
     if (!c) return;

@@ -219,42 +219,49 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t* __restr
         }
 }
 
-// Stable scatter of one tile: chunks of 256 keys in index order; inside a
-// chunk each wave ranks equal digits with 8 ballots; per-wave digit counts in
-// LDS give the cross-wave offsets; a running per-digit cursor carries across
-// chunks.
+// Stable scatter of one tile of kTile keys in two steps.  Rank: wave w owns
+// keys [w * kTile/4, (w + 1) * kTile/4) of the tile and walks them 64 at a
+// time in index order; lanes with equal digits find each other with 8
+// ballots, and a per-wave running count per digit in LDS gives each key its
+// rank among the wave's earlier keys of that digit.  Stage: per-digit wave
+// prefixes and the tile's digit starts (a block scan) place every key at its
+// tile-sorted position in LDS.  Write: thread t writes the staged keys t,
+// t + 256, ... to the digit's global offset plus their distance from the
+// digit's tile start, so each digit's run leaves as consecutive addresses.
 template <typename K>
 __global__ __launch_bounds__(kThreads) void k_scatter(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
                                                       int shift, uint32_t numTiles,
                                                       const uint32_t* __restrict__ offsets) {
-    __shared__ uint32_t running[256];
-    __shared__ uint32_t wcount[kThreads / 64][256];
+    constexpr int kWaves = kThreads / 64;
+    constexpr int kPerWave = kTile / kWaves;
+    __shared__ K sk[kTile];
+    __shared__ uint32_t sv[kTile];
+    __shared__ uint32_t wcnt[kWaves][256];
+    __shared__ uint32_t lstart[256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t scanTmp[kWaves];
     const int tid = threadIdx.x;
     const int w = tid >> 6;
-    running[tid] = offsets[(size_t)tid * numTiles + blockIdx.x];
+    const int lane = __lane_id();
     const size_t base = (size_t)blockIdx.x * kTile;
+    const uint32_t cnt = (uint32_t)min((size_t)kTile, n - base);
+    for (int j = lane; j < 256; j += 64) wcnt[w][j] = 0u;
     K kr[kChunks];
     uint32_t vr[kChunks];
 #pragma unroll
     for (int c = 0; c < kChunks; c++) {
-        const size_t i = base + (size_t)c * kThreads + tid;
-        if (i < n) {
-            kr[c] = kin[i];
-            vr[c] = vin[i];
-        } else {
-            kr[c] = 0;
-            vr[c] = 0;
-        }
+        const uint32_t li = (uint32_t)(w * kPerWave + c * 64 + lane);
+        const bool valid = li < cnt;
+        kr[c] = valid ? kin[base + li] : (K)0;
+        vr[c] = valid ? vin[base + li] : 0u;
     }
     const uint64_t lt = lanemask_lt();
+    uint32_t rk[kChunks];
 #pragma unroll
     for (int c = 0; c < kChunks; c++) {
-        const size_t i = base + (size_t)c * kThreads + tid;
-        const bool valid = i < n;
+        const bool valid = (uint32_t)(w * kPerWave + c * 64 + lane) < cnt;
         const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
-        for (int j = __lane_id(); j < 256; j += 64) wcount[w][j] = 0;
-        __syncthreads();
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -263,46 +270,70 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const K* __restrict__ kin,
             peers &= bit ? m : ~m;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) wcount[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = running[d] + rank;
-            for (int k = 0; k < w; k++) pos += wcount[k][d];
-            kout[pos] = kr[c];
-            vout[pos] = vr[c];
+        // every lane of the digit reads the wave's count before its first
+        // lane stores the new one: LDS serves one wave's accesses in order
+        const uint32_t before = wcnt[w][d];
+        rk[c] = before + rank;
+        if (valid && rank == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; k++) {
+            const uint32_t v = wcnt[k][tid];
+            wcnt[k][tid] = acc;
+            acc += v;
         }
-        __syncthreads();
-        uint32_t add = 0;
-        for (int k = 0; k < kThreads / 64; k++) add += wcount[k][tid];
-        running[tid] += add;
-        __syncthreads();  // wcount is cleared again at the top of the next chunk
+        uint32_t excl;
+        (void)block_exclusive_scan(acc, excl, scanTmp);
+        lstart[tid] = excl;
+        gbase[tid] = offsets[(size_t)tid * numTiles + blockIdx.x] - excl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+        const bool valid = (uint32_t)(w * kPerWave + c * 64 + lane) < cnt;
+        if (valid) {
+            const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
+            const uint32_t p = lstart[d] + wcnt[w][d] + rk[c];
+            sk[p] = kr[c];
+            sv[p] = vr[c];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < cnt; j += kThreads) {
+        const K k = sk[j];
+        const uint32_t o = gbase[(uint32_t)((k >> shift) & 0xff)] + j;
+        kout[o] = k;
+        vout[o] = sv[j];
     }
 }
 
-// ---------------------------------------------------------------- 4/7 gathers
-__global__ __launch_bounds__(kThreads) void k_gather_pos(const float4* __restrict__ pos,
-                                                         const uint32_t* __restrict__ idx, size_t n,
-                                                         float4* __restrict__ out) {
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i < n) out[i] = pos[idx[i]];
-}
-
+// ---------------------------------------------------------------- 4 gather
 // packGather colour conversion: uint(clamp(c, 0, 1) * 255) per channel (truncating)
-__global__ __launch_bounds__(kThreads) void k_gather_final(const float4* __restrict__ pos,
-                                                           const float4* __restrict__ col,
-                                                           const uint32_t* __restrict__ idx, size_t n,
-                                                           float4* __restrict__ outPos,
-                                                           uint32_t* __restrict__ outCol) {
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = idx[i];
-    outPos[i] = pos[s];
-    const float4 c = col[s];
+__device__ inline uint32_t pack_color(float4 c) {
     const uint32_t r = f2u(fminf(fmaxf(c.x, 0.0f), 1.0f) * 255.0f);
     const uint32_t g = f2u(fminf(fmaxf(c.y, 0.0f), 1.0f) * 255.0f);
     const uint32_t b = f2u(fminf(fmaxf(c.z, 0.0f), 1.0f) * 255.0f);
     const uint32_t a = f2u(fminf(fmaxf(c.w, 0.0f), 1.0f) * 255.0f);
-    outCol[i] = r | (g << 8) | (b << 16) | (a << 24);
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+// The one gather from the caller's order: positions and RGBA8 colours into
+// Morton order (every later stage reads inside a batch), levels reset to the
+// finest.
+__global__ __launch_bounds__(kThreads) void k_gather_morton(const float4* __restrict__ pos,
+                                                            const float4* __restrict__ col,
+                                                            const uint32_t* __restrict__ idx, size_t n,
+                                                            uint8_t maxLevel, float4* __restrict__ sp,
+                                                            uint32_t* __restrict__ spc, uint8_t* __restrict__ lv) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = idx[i];
+    sp[i] = pos[s];
+    spc[i] = pack_color(col[s]);
+    lv[i] = maxLevel;
 }
 
 // ---------------------------------------------------------------- 5 LOD
@@ -335,11 +366,6 @@ __device__ inline void lod_cell(const uint32_t* bounds, const LodGrid& g, float4
     cz = min((uint32_t)z, g.dz - 1u);
 }
 
-__global__ __launch_bounds__(kThreads) void k_lod_init(uint8_t* __restrict__ lv, size_t n, uint8_t maxLevel) {
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i < n) lv[i] = maxLevel;
-}
-
 __global__ __launch_bounds__(kThreads) void k_lod_claim(const float4* __restrict__ sp, const uint8_t* __restrict__ lv,
                                                         size_t n, uint8_t maxLevel, const uint32_t* __restrict__ bounds,
                                                         LodGrid g, uint32_t* __restrict__ grid) {
@@ -357,54 +383,128 @@ __global__ __launch_bounds__(kThreads) void k_lod_claim(const float4* __restrict
     if (part && !dup) atomicMin(grid + lin, (uint32_t)i);
 }
 
-__global__ __launch_bounds__(kThreads) void k_lod_assign(const float4* __restrict__ sp, uint8_t* __restrict__ lv,
-                                                         size_t n, uint8_t maxLevel, uint8_t level,
-                                                         const uint32_t* __restrict__ bounds, LodGrid g,
-                                                         const uint32_t* __restrict__ grid) {
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n || lv[i] != maxLevel) return;
+__device__ inline uint64_t lod_lin(const uint32_t* bounds, const LodGrid& g, float4 p) {
     uint32_t cx, cy, cz;
-    lod_cell(bounds, g, sp[i], cx, cy, cz);
-    const uint64_t lin = (uint64_t)cx + (uint64_t)g.dx * ((uint64_t)cy + (uint64_t)g.dy * cz);
-    if (grid[lin] == (uint32_t)i) lv[i] = level;
+    lod_cell(bounds, g, p, cx, cy, cz);
+    return (uint64_t)cx + (uint64_t)g.dx * ((uint64_t)cy + (uint64_t)g.dy * cz);
 }
 
-// ---------------------------------------------------------------- 6 bucketing
-// One workgroup per batch; stable counting scatter by level (3-bit digit) in
-// index order, cum counts out.
-__global__ __launch_bounds__(kThreads) void k_bucket(const uint8_t* __restrict__ lvIn, const uint32_t* __restrict__ idxIn,
-                                                     size_t n, uint32_t ppb, uint32_t* __restrict__ idxOut,
-                                                     uint8_t* __restrict__ lvOut, uint32_t* __restrict__ cumOut) {
+// Level `level` takes the points that won their cell in gridL (the lowest
+// Morton index of the cell's unclaimed points); the rest offer their index
+// to their cell of the next level in gridN.
+__global__ __launch_bounds__(kThreads) void k_lod_step(const float4* __restrict__ sp, uint8_t* __restrict__ lv,
+                                                       size_t n, uint8_t maxLevel, uint8_t level,
+                                                       const uint32_t* __restrict__ bounds, LodGrid gl,
+                                                       const uint32_t* __restrict__ gridL, LodGrid gn,
+                                                       uint32_t* __restrict__ gridN) {
+    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    bool part = i < n && lv[i] == maxLevel;
+    uint64_t lin = ~0ull;
+    if (part) {
+        const float4 p = sp[i];
+        if (gridL[lod_lin(bounds, gl, p)] == (uint32_t)i) {
+            lv[i] = level;
+            part = false;
+        } else {
+            lin = lod_lin(bounds, gn, p);
+        }
+    }
+    const uint64_t prev = __shfl_up(lin, 1);
+    const bool dup = __lane_id() > 0 && prev == lin;
+    if (part && !dup) atomicMin(gridN + lin, (uint32_t)i);
+}
+
+// ---------------------------------------------------------------- 6-9 per batch
+// One workgroup per batch (Morton positions [b * ppb, b * ppb + ppb)):
+//   - the last LOD level takes its cell winners (when LOD runs);
+//   - level histogram → cum counts, and the batch AABB (the batch holds the
+//     same points in Morton and in final order, so min/max are the same);
+//   - the RasterBatch record;
+//   - a stable counting scatter by level in Morton order (3-bit digit, wave
+//     ballots, per-wave counts in LDS), writing each point's final slot: level,
+//     three 10-bit position words quantised against the batch AABB, RGBA8
+//     colour, and (when asked) its source index.
+__global__ __launch_bounds__(kThreads) void k_finalize(const float4* __restrict__ sp, const uint32_t* __restrict__ spc,
+                                                       uint8_t* __restrict__ lv, const uint32_t* __restrict__ srcIdx,
+                                                       size_t n, uint32_t ppb, uint32_t slotBase, uint8_t maxLevel,
+                                                       int lastLevel, const uint32_t* __restrict__ bounds, LodGrid g,
+                                                       const uint32_t* __restrict__ grid,
+                                                       scr_raster_batch* __restrict__ batches,
+                                                       uint8_t* __restrict__ levelsOut, uint32_t* __restrict__ colorsOut,
+                                                       uint32_t* __restrict__ low, uint32_t* __restrict__ med,
+                                                       uint32_t* __restrict__ high, uint32_t* __restrict__ permOut) {
+    constexpr int kWaves = kThreads / 64;
     __shared__ uint32_t cnt[8];
     __shared__ uint32_t running[8];
-    __shared__ uint32_t wcount[kThreads / 64][8];
+    __shared__ uint32_t wcount[kWaves][8];
+    __shared__ float smn[kWaves][3], smx[kWaves][3];
+    __shared__ float bmn[3], bsz[3];
     const int tid = threadIdx.x;
     const int w = tid >> 6;
     const size_t start = (size_t)blockIdx.x * ppb;
     const size_t end = min(start + (size_t)ppb, n);
     if (tid < 8) cnt[tid] = 0;
     __syncthreads();
-    for (size_t i = start + tid; i < end; i += kThreads) atomicAdd(&cnt[lvIn[i] & 7u], 1u);
+    float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+    float mx[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    for (size_t i = start + tid; i < end; i += kThreads) {
+        const float4 p = sp[i];
+        uint8_t l = lv[i];
+        if (lastLevel >= 0 && l == maxLevel && grid[lod_lin(bounds, g, p)] == (uint32_t)i) {
+            l = (uint8_t)lastLevel;
+            lv[i] = l;   // re-read below by this same thread
+        }
+        atomicAdd(&cnt[l & 7u], 1u);
+        mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
+        mx[0] = fmaxf(mx[0], p.x); mx[1] = fmaxf(mx[1], p.y); mx[2] = fmaxf(mx[2], p.z);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int a = 0; a < 3; a++) {
+            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
+            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
+        }
+    if (__lane_id() == 0)
+        for (int a = 0; a < 3; a++) {
+            smn[w][a] = mn[a];
+            smx[w][a] = mx[a];
+        }
     __syncthreads();
     if (tid == 0) {
-        uint32_t acc = 0;
+        for (int a = 0; a < 3; a++) {
+            for (int k = 1; k < kWaves; k++) {
+                mn[a] = fminf(mn[a], smn[k][a]);
+                mx[a] = fmaxf(mx[a], smx[k][a]);
+            }
+            bmn[a] = mn[a];
+            bsz[a] = fmaxf(mx[a] - mn[a], 0.000001f);
+        }
+        uint32_t acc = 0, c[8];
         for (int l = 0; l < 8; l++) {
             running[l] = acc;
             acc += cnt[l];
+            c[l] = acc;
         }
-        const uint32_t c0 = cnt[0], c1 = c0 + cnt[1], c2 = c1 + cnt[2], c3 = c2 + cnt[3], c4 = c3 + cnt[4],
-                       c5 = c4 + cnt[5], c6 = c5 + cnt[6], c7 = c6 + cnt[7];
-        cumOut[blockIdx.x * 4 + 0] = c0 | (c1 << 16);
-        cumOut[blockIdx.x * 4 + 1] = c2 | (c3 << 16);
-        cumOut[blockIdx.x * 4 + 2] = c4 | (c5 << 16);
-        cumOut[blockIdx.x * 4 + 3] = c6 | (c7 << 16);
+        scr_raster_batch b;
+        b.state = 1;
+        b.minX = mn[0]; b.minY = mn[1]; b.minZ = mn[2];
+        b.maxX = mx[0]; b.maxY = mx[1]; b.maxZ = mx[2];
+        b.numPoints = (uint32_t)(end - start);
+        b.firstPoint = (slotBase + blockIdx.x) * ppb;
+        b.fileIndex = 0;
+        b.lodCum01 = c[0] | (c[1] << 16);
+        b.lodCum23 = c[2] | (c[3] << 16);
+        b.lodCum45 = c[4] | (c[5] << 16);
+        b.lodCum67 = c[6] | (c[7] << 16);
+        b.padding7 = 0;
+        b.padding8 = 0;
+        batches[blockIdx.x] = b;
     }
     __syncthreads();
     const uint64_t lt = lanemask_lt();
     for (size_t c = start; c < end; c += kThreads) {
         const size_t i = c + tid;
         const bool valid = i < end;
-        const uint8_t lvl = valid ? lvIn[i] : 0;
+        const uint8_t lvl = valid ? lv[i] : 0;
         const uint32_t d = lvl & 7u;
         if (__lane_id() < 8) wcount[w][__lane_id()] = 0;
         __syncthreads();
@@ -421,91 +521,30 @@ __global__ __launch_bounds__(kThreads) void k_bucket(const uint8_t* __restrict__
         if (valid) {
             uint32_t pos = running[d] + rank;
             for (int k = 0; k < w; k++) pos += wcount[k][d];
-            idxOut[start + pos] = idxIn[i];
-            lvOut[start + pos] = lvl;
+            const size_t o = start + pos;
+            levelsOut[o] = lvl;
+            colorsOut[o] = spc[i];
+            if (permOut) permOut[o] = srcIdx[i];
+            // quantize (p - min) / size, IEEE division, as the CPU pack
+            const float4 p = sp[i];
+            const float v[3] = {p.x, p.y, p.z};
+            uint32_t q[3];
+            for (int a = 0; a < 3; a++) {
+                const float nrm = fminf(fmaxf((v[a] - bmn[a]) / bsz[a], 0.0f), 0.99999994f);
+                q[a] = f2u(nrm * 1073741824.0f);  // Float(2^30 - 1) rounds to 2^30
+            }
+            low[o] = ((q[0] >> 20) & 1023u) | (((q[1] >> 20) & 1023u) << 10) | (((q[2] >> 20) & 1023u) << 20);
+            med[o] = ((q[0] >> 10) & 1023u) | (((q[1] >> 10) & 1023u) << 10) | (((q[2] >> 10) & 1023u) << 20);
+            high[o] = (q[0] & 1023u) | ((q[1] & 1023u) << 10) | ((q[2] & 1023u) << 20);
         }
         __syncthreads();
         if (tid < 8) {
             uint32_t add = 0;
-            for (int k = 0; k < kThreads / 64; k++) add += wcount[k][tid];
+            for (int k = 0; k < kWaves; k++) add += wcount[k][tid];
             running[tid] += add;
         }
         __syncthreads();
     }
-}
-
-// ---------------------------------------------------------------- 8 batch AABB
-__global__ __launch_bounds__(kThreads) void k_batch_aabb(const float4* __restrict__ sp, size_t n, uint32_t ppb,
-                                                         uint32_t slotBase, const uint32_t* __restrict__ cum,
-                                                         scr_raster_batch* __restrict__ batches) {
-    __shared__ float smn[4][3], smx[4][3];
-    const size_t start = (size_t)blockIdx.x * ppb;
-    const size_t end = min(start + (size_t)ppb, n);
-    float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
-    float mx[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
-    for (size_t i = start + threadIdx.x; i < end; i += kThreads) {
-        const float4 p = sp[i];
-        mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
-        mx[0] = fmaxf(mx[0], p.x); mx[1] = fmaxf(mx[1], p.y); mx[2] = fmaxf(mx[2], p.z);
-    }
-    for (int off = 32; off > 0; off >>= 1)
-        for (int a = 0; a < 3; a++) {
-            mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
-            mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
-        }
-    const int w = threadIdx.x >> 6;
-    if (__lane_id() == 0)
-        for (int a = 0; a < 3; a++) {
-            smn[w][a] = mn[a];
-            smx[w][a] = mx[a];
-        }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < kThreads / 64; k++)
-            for (int a = 0; a < 3; a++) {
-                mn[a] = fminf(smn[0][a], smn[k][a]);
-                smn[0][a] = mn[a];
-                mx[a] = fmaxf(smx[0][a], smx[k][a]);
-                smx[0][a] = mx[a];
-            }
-        scr_raster_batch b;
-        b.state = 1;
-        b.minX = smn[0][0]; b.minY = smn[0][1]; b.minZ = smn[0][2];
-        b.maxX = smx[0][0]; b.maxY = smx[0][1]; b.maxZ = smx[0][2];
-        b.numPoints = (uint32_t)(end - start);
-        b.firstPoint = (slotBase + blockIdx.x) * ppb;
-        b.fileIndex = 0;
-        b.lodCum01 = cum[blockIdx.x * 4 + 0];
-        b.lodCum23 = cum[blockIdx.x * 4 + 1];
-        b.lodCum45 = cum[blockIdx.x * 4 + 2];
-        b.lodCum67 = cum[blockIdx.x * 4 + 3];
-        b.padding7 = 0;
-        b.padding8 = 0;
-        batches[blockIdx.x] = b;
-    }
-}
-
-// ---------------------------------------------------------------- 9 quantize
-__global__ __launch_bounds__(kThreads) void k_quantize(const float4* __restrict__ sp, size_t n, uint32_t ppb,
-                                                       const scr_raster_batch* __restrict__ batches,
-                                                       uint32_t* __restrict__ low, uint32_t* __restrict__ med,
-                                                       uint32_t* __restrict__ high) {
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const scr_raster_batch& b = batches[i / ppb];
-    const float4 p = sp[i];
-    const float mn[3] = {b.minX, b.minY, b.minZ};
-    const float mx[3] = {b.maxX, b.maxY, b.maxZ};
-    const float v[3] = {p.x, p.y, p.z};
-    uint32_t q[3];
-    for (int a = 0; a < 3; a++) {
-        const float size = fmaxf(mx[a] - mn[a], 0.000001f);
-        const float nrm = fminf(fmaxf((v[a] - mn[a]) / size, 0.0f), 0.99999994f);
-        q[a] = f2u(nrm * 1073741824.0f);  // Float(2^30 - 1) rounds to 2^30
-    }
-    low[i] = ((q[0] >> 20) & 1023u) | (((q[1] >> 20) & 1023u) << 10) | (((q[2] >> 20) & 1023u) << 20);
-    med[i] = ((q[0] >> 10) & 1023u) | (((q[1] >> 10) & 1023u) << 10) | (((q[2] >> 10) & 1023u) << 20);
-    high[i] = (q[0] & 1023u) | ((q[1] & 1023u) << 10) | ((q[2] & 1023u) << 20);
 }
 
 unsigned blocks(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
@@ -526,24 +565,44 @@ struct Arena {
 }  // namespace
 
 
-// Scratch bytes of a pack of n points into nb batches with a dense LOD grid
-// of gridCells cells (the six bounds words are not included).
-static size_t pack_scratch_bytes(size_t n, bool wide, size_t nb, size_t gridCells) {
+// Scratch bytes of a pack of n points with LOD grids of grid0 and grid1
+// cells (the six bounds words are not included).
+static size_t pack_scratch_bytes(size_t n, bool wide, size_t grid0, size_t grid1) {
     const size_t numTiles = (n + kTile - 1) / kTile;
     const size_t scanN = 256 * numTiles;
     const size_t k = wide ? 8 : 4;
     size_t b = 0;
     b += 2 * n * k + 2 * n * 4;                 // keys, payload ping-pong
-    b += n * 16;                                // sorted positions
+    b += n * 16 + n * 4;                        // Morton-order positions, colours
     b += n;                                     // Morton-order levels
     b += 2 * scanN * 4 + (scanN / kScanSeg + 2) * 4;  // tile hist, offsets, scan sums
     b += 8 * 256 * 4 + 64;                      // digit histograms
-    b += nb * 16;                               // cum counts
-    b += gridCells * 4;
+    b += (grid0 + grid1) * 4;
     b += 32 * 256;                              // alignment slack
     return b;
 }
 
+// The dense grid of one LOD level: the device cell index's arithmetic on the
+// bounds words hb (float_flip'ed min xyz, max xyz).
+static LodGrid lod_grid(const uint32_t hb[6], int coarse, int level) {
+    LodGrid g;
+    g.voxelScale = std::pow(0.5f, (float)level);
+    g.coarse = (float)coarse;
+    float mn[3], ext[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = float_unflip(hb[a]);
+        ext[a] = std::fmax(float_unflip(hb[3 + a]) - mn[a], 0.000001f);
+    }
+    const float longest = std::fmax(ext[0], std::fmax(ext[1], ext[2]));
+    const float voxel = (longest / g.coarse) * g.voxelScale;
+    const float inv = 1.0f / std::fmax(voxel, 0.000001f);
+    g.dx = (uint32_t)f2u(std::floor(ext[0] * inv)) + 1u;
+    g.dy = (uint32_t)f2u(std::floor(ext[1] * inv)) + 1u;
+    g.dz = (uint32_t)f2u(std::floor(ext[2] * inv)) + 1u;
+    return g;
+}
+
+static size_t lod_cells(const LodGrid& g) { return (size_t)g.dx * g.dy * g.dz; }
 
 template <typename K>
 static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*& vout, size_t n, uint32_t* ghist,
@@ -611,29 +670,23 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     // bounds first (the LOD grid size depends on them), in the context's
     // bounds words (allocated with the context)
     uint32_t* dBounds = ctx->packBounds;
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, dBounds);
-    hipLaunchKernelGGL(k_bounds, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
-                       dim3(kThreads), 0, s, pos, n, dBounds);
+    {
+        PassScope ps(ctx, "pack_bounds");
+        hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, dBounds);
+        hipLaunchKernelGGL(k_bounds, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
+                           dim3(kThreads), 0, s, pos, n, dBounds);
+    }
     uint32_t hb[6];
     e = hipMemcpyAsync(hb, dBounds, sizeof hb, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, "pack bounds");
-    // dense LOD grid dims at the finest claimed level (cells per axis grow by
-    // 2 per level), within the gate's bound
-    uint64_t gridCells = 0;
-    if (lodOn) {
-        float mn[3], ext[3];
-        for (int a = 0; a < 3; a++) {
-            mn[a] = float_unflip(hb[a]);
-            ext[a] = std::fmax(float_unflip(hb[3 + a]) - mn[a], 0.000001f);
-        }
-        const float longest = std::fmax(ext[0], std::fmax(ext[1], ext[2]));
-        const float voxel = (longest / (float)coarse) * std::pow(0.5f, (float)(lodLevels - 2));
-        const float inv = 1.0f / std::fmax(voxel, 0.000001f);
-        gridCells = 1;
-        for (int a = 0; a < 3; a++) gridCells *= (uint64_t)f2u(std::floor(ext[a] * inv)) + 1u;
-    }
-    const size_t need = pack_scratch_bytes(n, wide, nb, (size_t)gridCells);
+    // dense LOD grids: level l claims in grid l % 2 (the finest claimed level
+    // has the most cells; the gate bounds it)
+    size_t gridCells[2] = {0, 0};
+    if (lodOn)
+        for (int level = 0; level < lodLevels - 1; level++)
+            gridCells[level & 1] = std::max(gridCells[level & 1], lod_cells(lod_grid(hb, coarse, level)));
+    const size_t need = pack_scratch_bytes(n, wide, gridCells[0], gridCells[1]);
     if (ctx->scratchBytes < need) {
         if (ctx->scratch) (void)hipFree(ctx->scratch);
         ctx->scratch = nullptr;
@@ -650,74 +703,78 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     uint32_t* idxA = ar.take<uint32_t>(n);
     uint32_t* idxB = ar.take<uint32_t>(n);
     float4* sp = ar.take<float4>(n);
+    uint32_t* spc = ar.take<uint32_t>(n);
     uint8_t* lvM = ar.take<uint8_t>(n);
     uint32_t* thist = ar.take<uint32_t>(scanN);
     uint32_t* offs = ar.take<uint32_t>(scanN);
     uint32_t* sums = ar.take<uint32_t>(scanN / kScanSeg + 2);
     uint32_t* ghist = ar.take<uint32_t>(8 * 256);
-    uint32_t* cum = ar.take<uint32_t>(nb * 4);
-    uint32_t* grid = ar.take<uint32_t>((size_t)gridCells);
+    uint32_t* grid[2] = {ar.take<uint32_t>(gridCells[0]), ar.take<uint32_t>(gridCells[1])};
 
-    // 2 Morton keys
+    // 2 Morton keys, 3 sort
     uint32_t* sortedIdx = nullptr;
     int rc = SCR_OK;
     if (wide) {
-        hipLaunchKernelGGL(k_morton<uint64_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
-                           (uint64_t*)keysA, idxA);
+        {
+            PassScope ps(ctx, "pack_morton");
+            hipLaunchKernelGGL(k_morton<uint64_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
+                               (uint64_t*)keysA, idxA);
+        }
         uint64_t *kin = (uint64_t*)keysA, *kout = (uint64_t*)keysB;
         uint32_t *vin = idxA, *vout = idxB;
+        PassScope ps(ctx, "pack_sort");
         rc = radix_sort<uint64_t>(ctx, kin, vin, kout, vout, n, ghist, thist, offs, sums, nullptr);
         sortedIdx = vin;
     } else {
-        hipLaunchKernelGGL(k_morton<uint32_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
-                           (uint32_t*)keysA, idxA);
+        {
+            PassScope ps(ctx, "pack_morton");
+            hipLaunchKernelGGL(k_morton<uint32_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
+                               (uint32_t*)keysA, idxA);
+        }
         uint32_t *kin = (uint32_t*)keysA, *kout = (uint32_t*)keysB;
         uint32_t *vin = idxA, *vout = idxB;
+        PassScope ps(ctx, "pack_sort");
         rc = radix_sort<uint32_t>(ctx, kin, vin, kout, vout, n, ghist, thist, offs, sums, nullptr);
         sortedIdx = vin;
     }
     if (rc != SCR_OK) return rc;
     uint32_t* permOut = sortedIdx == idxA ? idxB : idxA;
 
-    // 4 gather Morton-order positions
-    hipLaunchKernelGGL(k_gather_pos, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, sortedIdx, n, sp);
-    // 5 LOD levels
-    hipLaunchKernelGGL(k_lod_init, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, lvM, n, maxLevel);
-    for (int level = 0; lodOn && level < lodLevels - 1; level++) {
-        LodGrid g;
-        g.voxelScale = std::pow(0.5f, (float)level);
-        g.coarse = (float)coarse;
-        // dims of this level's grid (same arithmetic as the device cell index)
-        float mn[3], ext[3];
-        for (int a = 0; a < 3; a++) {
-            mn[a] = float_unflip(hb[a]);
-            ext[a] = std::fmax(float_unflip(hb[3 + a]) - mn[a], 0.000001f);
-        }
-        const float longest = std::fmax(ext[0], std::fmax(ext[1], ext[2]));
-        const float voxel = (longest / g.coarse) * g.voxelScale;
-        const float inv = 1.0f / std::fmax(voxel, 0.000001f);
-        g.dx = (uint32_t)f2u(std::floor(ext[0] * inv)) + 1u;
-        g.dy = (uint32_t)f2u(std::floor(ext[1] * inv)) + 1u;
-        g.dz = (uint32_t)f2u(std::floor(ext[2] * inv)) + 1u;
-        const size_t cells = (size_t)g.dx * g.dy * g.dz;
-        e = launch_fill_u32(s, grid, cells, 0xffffffffu);
-        if (e != hipSuccess) break;
-        hipLaunchKernelGGL(k_lod_claim, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel, dBounds,
-                           g, grid);
-        hipLaunchKernelGGL(k_lod_assign, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel,
-                           (uint8_t)level, dBounds, g, grid);
+    // 4 positions and colours into Morton order
+    {
+        PassScope ps(ctx, "pack_gather");
+        hipLaunchKernelGGL(k_gather_morton, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, col, sortedIdx, n,
+                           maxLevel, sp, spc, lvM);
     }
-    // 6 bucketing → permOut (source indices in final order), final levels
-    hipLaunchKernelGGL(k_bucket, dim3((unsigned)nb), dim3(kThreads), 0, s, lvM, sortedIdx, n, (uint32_t)ppb, permOut,
-                       dst.levels, cum);
-    // 7 final gather (positions into sp, colours into the pool)
-    hipLaunchKernelGGL(k_gather_final, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, col, permOut, n, sp,
-                       dst.colors);
-    // 8 batch AABB + metadata, 9 quantize
-    hipLaunchKernelGGL(k_batch_aabb, dim3((unsigned)nb), dim3(kThreads), 0, s, sp, n, (uint32_t)ppb, dst.slotBase,
-                       cum, dst.batches);
-    hipLaunchKernelGGL(k_quantize, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, n, (uint32_t)ppb, dst.batches,
-                       dst.low, dst.med, dst.high);
+    // 5 LOD levels 0 .. L-3 (the last claimed level, L-2, is taken inside the finalize)
+    int lastLevel = -1;
+    LodGrid lastGrid{};
+    if (lodOn) {
+        PassScope ps(ctx, "pack_lod");
+        LodGrid g = lod_grid(hb, coarse, 0);
+        e = launch_fill_u32(s, grid[0], lod_cells(g), 0xffffffffu);
+        if (e != hipSuccess) return hip_error(ctx, e, "pack lod clear");
+        hipLaunchKernelGGL(k_lod_claim, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel, dBounds,
+                           g, grid[0]);
+        for (int level = 0; level + 1 < lodLevels - 1; level++) {
+            const LodGrid gn = lod_grid(hb, coarse, level + 1);
+            e = launch_fill_u32(s, grid[(level + 1) & 1], lod_cells(gn), 0xffffffffu);
+            if (e != hipSuccess) return hip_error(ctx, e, "pack lod clear");
+            hipLaunchKernelGGL(k_lod_step, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel,
+                               (uint8_t)level, dBounds, g, grid[level & 1], gn, grid[(level + 1) & 1]);
+            g = gn;
+        }
+        lastLevel = lodLevels - 2;
+        lastGrid = g;
+    }
+    // 6-9 per batch: last level, cum counts, AABB, record, bucketed final writes
+    {
+        PassScope ps(ctx, "pack_finalize");
+        hipLaunchKernelGGL(k_finalize, dim3((unsigned)nb), dim3(kThreads), 0, s, sp, spc, lvM, sortedIdx, n,
+                           (uint32_t)ppb, dst.slotBase, maxLevel, lastLevel, dBounds, lastGrid,
+                           lastLevel >= 0 ? grid[lastLevel & 1] : nullptr, dst.batches, dst.levels, dst.colors,
+                           dst.low, dst.med, dst.high, hostSourceIndices ? permOut : nullptr);
+    }
     e = hipGetLastError();
     if (e == hipSuccess && hostSourceIndices)
         e = hipMemcpyAsync(hostSourceIndices, permOut, n * 4, hipMemcpyDeviceToHost, s);

@@ -3,17 +3,22 @@
 // GPUPacker.encodePack (GPUPacker.swift:204-392).
 //
 // Stages (all on the context stream):
-//   1 bounds       block min/max → order-preserving u32 atomics   GPUPacker/Shaders.metal:65-105
+//   1 bounds       block min/max → order-preserving u32 atomics (a few per block)  GPUPacker/Shaders.metal:65-105
 //   2 morton       30-bit (10 b/axis, the reference) or 63-bit (21 b/axis) keys, identity payload :109-126
 //   3 radix sort   stable LSD, 8-bit digits: one all-digit histogram pass decides which
 //                  passes are trivial; per pass tile histograms → device-wide exclusive
-//                  scan (digit-major) → stable scatter ranked with wave64 ballots   :134-215
-//   4 gather       Morton-ordered positions                                       :219-235
-//   5 LOD levels   dense voxel grid, atomicMin "first claimer", wave run de-dup   :317-371
-//   6 bucketing    per batch stable 8-way counting sort by level, cum counts      :383-414
-//   7 gather       final order: positions + truncating RGBA8 colours             :219-235
-//   8 batch AABB   one workgroup per batch, full RasterBatch                      :243-284
-//   9 quantize     (p - min) / size (IEEE division, as the CPU pack), 3 x 10-bit words :288-309
+//                  scan (digit-major) → stable scatter: ranked in LDS per tile, staged in
+//                  tile-sorted order, each digit's run written as consecutive addresses;
+//                  64-bit keys of large inputs in one kernel per pass with decoupled
+//                  look-back instead of the histogram and scan kernels          :134-215
+//   4 gather       the one gather from the caller's order: positions and RGBA8
+//                  colours into Morton order                                      :219-235
+//   5 LOD levels   dense voxel grids, atomicMin "first claimer", wave run de-dup; one
+//                  kernel per level takes that level's winners and claims the next :317-371
+//   6-9 finalize   one workgroup per batch: last level's winners, cum counts, AABB,
+//                  the RasterBatch record, stable 8-way counting scatter by level,
+//                  quantize (p - min) / size (IEEE division, as the CPU pack) into three
+//                  10-bit words, colours and source indices in final order  :243-309, 383-414
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -75,11 +80,23 @@ __global__ __launch_bounds__(kThreads) void k_bounds(const float4* __restrict__ 
             mn[a] = fminf(mn[a], __shfl_xor(mn[a], off));
             mx[a] = fmaxf(mx[a], __shfl_xor(mx[a], off));
         }
+    // the block's waves combine in LDS; one wave lane per word updates the
+    // six device words (a few atomics per block, not per wave)
+    __shared__ float red[kThreads / 64][6];
+    const int w = threadIdx.x >> 6;
     if (__lane_id() == 0)
         for (int a = 0; a < 3; a++) {
-            atomicMin(bounds + a, float_flip(mn[a]));
-            atomicMax(bounds + 3 + a, float_flip(mx[a]));
+            red[w][a] = mn[a];
+            red[w][3 + a] = mx[a];
         }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        float v = red[0][a];
+        for (int k = 1; k < kThreads / 64; k++) v = a < 3 ? fminf(v, red[k][a]) : fmaxf(v, red[k][a]);
+        if (a < 3) atomicMin(bounds + a, float_flip(v));
+        else atomicMax(bounds + a, float_flip(v));
+    }
 }
 
 __global__ void k_bounds_init(uint32_t* bounds) {
@@ -310,6 +327,145 @@ __global__ __launch_bounds__(kThreads) void k_scatter(const K* __restrict__ kin,
     }
 }
 
+// One-pass scatter with decoupled look-back.  Each workgroup takes the next
+// tile number from a counter (so every lower tile's workgroup has started),
+// ranks its tile as k_scatter does, publishes its per-digit counts
+// (kAggregate), then one thread per digit walks back over the lower tiles'
+// words, adding their counts until it meets a tile that has published its
+// inclusive prefix (kPrefix), and publishes its own.  A tile's publication
+// never waits on anything, so every walk ends; a walk that polls more than
+// kLookbackPolls times sets *err and stops (the pack then fails).
+constexpr uint64_t kAggregate = 1ull << 62;
+constexpr uint64_t kPrefix = 2ull << 62;
+constexpr uint64_t kCountMask = (1ull << 62) - 1;
+constexpr int kLookbackPolls = 1 << 20;
+
+__global__ __launch_bounds__(256) void k_digit_starts(const uint32_t* __restrict__ ghist, int P,
+                                                      uint32_t* __restrict__ dstart) {
+    __shared__ uint32_t lds[4];
+    for (int p = 0; p < P; p++) {
+        uint32_t excl;
+        (void)block_exclusive_scan(ghist[p * 256 + threadIdx.x], excl, lds);
+        dstart[p * 256 + threadIdx.x] = excl;
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kThreads) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
+                                                       int shift, const uint32_t* __restrict__ dstart,
+                                                       uint64_t* __restrict__ status, uint32_t* __restrict__ tileCounter,
+                                                       uint32_t* __restrict__ err) {
+    constexpr int kWaves = kThreads / 64;
+    constexpr int kPerWave = kTile / kWaves;
+    __shared__ K sk[kTile];
+    __shared__ uint32_t sv[kTile];
+    __shared__ uint32_t wcnt[kWaves][256];
+    __shared__ uint32_t lstart[256];
+    __shared__ uint32_t gbase[256];
+    __shared__ uint32_t scanTmp[kWaves];
+    __shared__ uint32_t tileShared;
+    const int tid = threadIdx.x;
+    const int w = tid >> 6;
+    const int lane = __lane_id();
+    if (tid == 0) tileShared = atomicAdd(tileCounter, 1u);
+    for (int j = lane; j < 256; j += 64) wcnt[w][j] = 0u;
+    __syncthreads();
+    const uint32_t tile = tileShared;
+    const size_t base = (size_t)tile * kTile;
+    const uint32_t cnt = (uint32_t)min((size_t)kTile, n - base);
+    K kr[kChunks];
+    uint32_t vr[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+        const uint32_t li = (uint32_t)(w * kPerWave + c * 64 + lane);
+        const bool valid = li < cnt;
+        kr[c] = valid ? kin[base + li] : (K)0;
+        vr[c] = valid ? vin[base + li] : 0u;
+    }
+    const uint64_t lt = lanemask_lt();
+    uint32_t rk[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+        const bool valid = (uint32_t)(w * kPerWave + c * 64 + lane) < cnt;
+        const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        const uint32_t before = wcnt[w][d];
+        rk[c] = before + rank;
+        if (valid && rank == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; k++) {
+            const uint32_t v = wcnt[k][tid];
+            wcnt[k][tid] = acc;
+            acc += v;
+        }
+        // publish this tile's count of digit tid, then look back for the
+        // count of the digit in all lower tiles
+        uint64_t* mine = status + (size_t)tile * 256 + tid;
+        uint64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(mine, kPrefix | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(mine, kAggregate | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
+            int polls = 0;
+            while (true) {
+                const uint64_t v = __hip_atomic_load(status + (size_t)j * 256 + tid, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if ((v & ~kCountMask) == 0) {
+                    if (++polls > kLookbackPolls) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v & kCountMask;
+                if ((v & ~kCountMask) == kPrefix || j == 0) break;
+                j--;
+            }
+            __hip_atomic_store(mine, kPrefix | (excl + acc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint32_t lex;
+        (void)block_exclusive_scan(acc, lex, scanTmp);
+        lstart[tid] = lex;
+        gbase[tid] = dstart[tid] + (uint32_t)excl - lex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+        const bool valid = (uint32_t)(w * kPerWave + c * 64 + lane) < cnt;
+        if (valid) {
+            const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
+            const uint32_t p = lstart[d] + wcnt[w][d] + rk[c];
+            sk[p] = kr[c];
+            sv[p] = vr[c];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < cnt; j += kThreads) {
+        const K k = sk[j];
+        const uint32_t o = gbase[(uint32_t)((k >> shift) & 0xff)] + j;
+        // o < n holds whenever every look-back completed; after a failed one
+        // (err set, the pack fails) no write leaves the arrays
+        if (o < n) {
+            kout[o] = k;
+            vout[o] = sv[j];
+        }
+    }
+}
+
 // ---------------------------------------------------------------- 4 gather
 // packGather colour conversion: uint(clamp(c, 0, 1) * 255) per channel (truncating)
 __device__ inline uint32_t pack_color(float4 c) {
@@ -366,27 +522,38 @@ __device__ inline void lod_cell(const uint32_t* bounds, const LodGrid& g, float4
     cz = min((uint32_t)z, g.dz - 1u);
 }
 
-__global__ __launch_bounds__(kThreads) void k_lod_claim(const float4* __restrict__ sp, const uint8_t* __restrict__ lv,
-                                                        size_t n, uint8_t maxLevel, const uint32_t* __restrict__ bounds,
-                                                        LodGrid g, uint32_t* __restrict__ grid) {
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    const bool part = i < n && lv[i] == maxLevel;
-    uint64_t lin = ~0ull;
-    if (part) {
-        uint32_t cx, cy, cz;
-        lod_cell(bounds, g, sp[i], cx, cy, cz);
-        lin = (uint64_t)cx + (uint64_t)g.dx * ((uint64_t)cy + (uint64_t)g.dy * cz);
-    }
-    // the previous point (i - 1, same wave) in the same cell holds a smaller index
-    const uint64_t prev = __shfl_up(lin, 1);
-    const bool dup = __lane_id() > 0 && prev == lin;
-    if (part && !dup) atomicMin(grid + lin, (uint32_t)i);
-}
-
 __device__ inline uint64_t lod_lin(const uint32_t* bounds, const LodGrid& g, float4 p) {
     uint32_t cx, cy, cz;
     lod_cell(bounds, g, p, cx, cy, cz);
     return (uint64_t)cx + (uint64_t)g.dx * ((uint64_t)cy + (uint64_t)g.dy * cz);
+}
+
+constexpr int kLodPts = 4;   // points per thread: index blockIdx * 1024 + k * 256 + tid
+
+// Level 0 claims: every point offers its index to its cell's atomicMin; the
+// previous point of the wave (one index lower) in the same cell holds a
+// smaller index, so only the first of a run of equal cells offers.
+__global__ __launch_bounds__(kThreads) void k_lod_claim(const float4* __restrict__ sp, const uint8_t* __restrict__ lv,
+                                                        size_t n, uint8_t maxLevel, const uint32_t* __restrict__ bounds,
+                                                        LodGrid g, uint32_t* __restrict__ grid) {
+    const size_t b0 = (size_t)blockIdx.x * (kThreads * kLodPts) + threadIdx.x;
+    float4 p[kLodPts];
+    uint8_t l[kLodPts];
+#pragma unroll
+    for (int k = 0; k < kLodPts; k++) {
+        const size_t i = b0 + (size_t)k * kThreads;
+        l[k] = i < n ? lv[i] : (uint8_t)0xff;
+        p[k] = i < n ? sp[i] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int k = 0; k < kLodPts; k++) {
+        const size_t i = b0 + (size_t)k * kThreads;
+        const bool part = i < n && l[k] == maxLevel;
+        const uint64_t lin = part ? lod_lin(bounds, g, p[k]) : ~0ull;
+        const uint64_t prev = __shfl_up(lin, 1);
+        const bool dup = __lane_id() > 0 && prev == lin;
+        if (part && !dup) atomicMin(grid + lin, (uint32_t)i);
+    }
 }
 
 // Level `level` takes the points that won their cell in gridL (the lowest
@@ -397,21 +564,34 @@ __global__ __launch_bounds__(kThreads) void k_lod_step(const float4* __restrict_
                                                        const uint32_t* __restrict__ bounds, LodGrid gl,
                                                        const uint32_t* __restrict__ gridL, LodGrid gn,
                                                        uint32_t* __restrict__ gridN) {
-    const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
-    bool part = i < n && lv[i] == maxLevel;
-    uint64_t lin = ~0ull;
-    if (part) {
-        const float4 p = sp[i];
-        if (gridL[lod_lin(bounds, gl, p)] == (uint32_t)i) {
+    const size_t b0 = (size_t)blockIdx.x * (kThreads * kLodPts) + threadIdx.x;
+    float4 p[kLodPts];
+    uint8_t l[kLodPts];
+    uint32_t won[kLodPts];
+#pragma unroll
+    for (int k = 0; k < kLodPts; k++) {
+        const size_t i = b0 + (size_t)k * kThreads;
+        l[k] = i < n ? lv[i] : (uint8_t)0xff;
+        p[k] = i < n ? sp[i] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int k = 0; k < kLodPts; k++) {
+        const size_t i = b0 + (size_t)k * kThreads;
+        won[k] = (i < n && l[k] == maxLevel) ? gridL[lod_lin(bounds, gl, p[k])] : 0xffffffffu;
+    }
+#pragma unroll
+    for (int k = 0; k < kLodPts; k++) {
+        const size_t i = b0 + (size_t)k * kThreads;
+        bool part = i < n && l[k] == maxLevel;
+        if (part && won[k] == (uint32_t)i) {
             lv[i] = level;
             part = false;
-        } else {
-            lin = lod_lin(bounds, gn, p);
         }
+        const uint64_t lin = part ? lod_lin(bounds, gn, p[k]) : ~0ull;
+        const uint64_t prev = __shfl_up(lin, 1);
+        const bool dup = __lane_id() > 0 && prev == lin;
+        if (part && !dup) atomicMin(gridN + lin, (uint32_t)i);
     }
-    const uint64_t prev = __shfl_up(lin, 1);
-    const bool dup = __lane_id() > 0 && prev == lin;
-    if (part && !dup) atomicMin(gridN + lin, (uint32_t)i);
 }
 
 // ---------------------------------------------------------------- 6-9 per batch
@@ -576,6 +756,7 @@ static size_t pack_scratch_bytes(size_t n, bool wide, size_t grid0, size_t grid1
     b += n * 16 + n * 4;                        // Morton-order positions, colours
     b += n;                                     // Morton-order levels
     b += 2 * scanN * 4 + (scanN / kScanSeg + 2) * 4;  // tile hist, offsets, scan sums
+    b += scanN * 8 + 8 * 256 * 4 + 16 * 4;      // look-back words, digit starts, counters
     b += 8 * 256 * 4 + 64;                      // digit histograms
     b += (grid0 + grid1) * 4;
     b += 32 * 256;                              // alignment slack
@@ -604,20 +785,39 @@ static LodGrid lod_grid(const uint32_t hb[6], int coarse, int level) {
 
 static size_t lod_cells(const LodGrid& g) { return (size_t)g.dx * g.dy * g.dz; }
 
+// Working memory of one sort.
+struct SortScratch {
+    uint32_t* ghist;    // all-digit histogram, 8 x 256
+    uint32_t* thist;    // tile histograms, digit-major (split passes)
+    uint32_t* offs;     // their exclusive scan
+    uint32_t* sums;     // scan block sums
+    uint64_t* status;   // look-back words, one per (tile, digit)
+    uint32_t* dstart;   // per pass exclusive digit starts
+    uint32_t* counters; // per pass tile counters [0, 8), look-back failure flag [8]
+};
+
+// 64-bit keys of at least this many points sort with the one-kernel
+// look-back pass; shorter keys and smaller inputs with the split
+// histogram / scan / scatter pass (measured faster there).
+constexpr size_t kLookbackMinPoints = size_t(1) << 24;
+
 template <typename K>
-static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*& vout, size_t n, uint32_t* ghist,
-                      uint32_t* thist, uint32_t* offs, uint32_t* sums, int* passesRun) {
+static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*& vout, size_t n,
+                      const SortScratch& w, int* passesRun) {
     hipStream_t s = ctx->stream;
     constexpr int P = sizeof(K);
+    const bool lookback = P == 8 && n >= kLookbackMinPoints;
     const uint32_t numTiles = (uint32_t)((n + kTile - 1) / kTile);
     const size_t scanN = (size_t)256 * numTiles;
     const uint32_t nseg = (uint32_t)((scanN + kScanSeg - 1) / kScanSeg);
-    hipError_t e = launch_fill_u32(s, ghist, P * 256, 0u);
+    hipError_t e = launch_fill_u32(s, w.ghist, P * 256, 0u);
+    if (e == hipSuccess && lookback) e = launch_fill_u32(s, w.counters, 16, 0u);
     if (e != hipSuccess) return hip_error(ctx, e, "radix hist clear");
     hipLaunchKernelGGL(k_all_digit_hist<K>, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
-                       dim3(kThreads), 0, s, kin, n, ghist);
+                       dim3(kThreads), 0, s, kin, n, w.ghist);
+    if (lookback) hipLaunchKernelGGL(k_digit_starts, dim3(1), dim3(256), 0, s, w.ghist, P, w.dstart);
     uint32_t h[8 * 256];
-    e = hipMemcpyAsync(h, ghist, (size_t)P * 256 * 4, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(h, w.ghist, (size_t)P * 256 * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_error(ctx, e, "radix hist");
     int run = 0;
@@ -627,18 +827,32 @@ static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*
             if (h[(size_t)p * 256 + d] == n) trivial = true;
         if (trivial) continue;  // every key has the same digit: the pass is the identity
         const int shift = 8 * p;
-        hipLaunchKernelGGL(k_tile_hist<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, n, shift, numTiles, thist);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(kThreads), 0, s, thist, scanN, sums);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, s, sums, nseg);
-        hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(kThreads), 0, s, thist, scanN, sums, offs);
-        hipLaunchKernelGGL(k_scatter<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift,
-                           numTiles, offs);
+        if (lookback) {
+            e = launch_fill_u32(s, (uint32_t*)w.status, scanN * 2, 0u);
+            if (e != hipSuccess) return hip_error(ctx, e, "radix look-back clear");
+            hipLaunchKernelGGL(k_onesweep<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift,
+                               w.dstart + p * 256, w.status, w.counters + p, w.counters + 8);
+        } else {
+            hipLaunchKernelGGL(k_tile_hist<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, n, shift, numTiles, w.thist);
+            hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(kThreads), 0, s, w.thist, scanN, w.sums);
+            hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kThreads), 0, s, w.sums, nseg);
+            hipLaunchKernelGGL(k_scan_apply, dim3(nseg), dim3(kThreads), 0, s, w.thist, scanN, w.sums, w.offs);
+            hipLaunchKernelGGL(k_scatter<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift,
+                               numTiles, w.offs);
+        }
         std::swap(kin, kout);
         std::swap(vin, vout);
         run++;
     }
     e = hipGetLastError();
     if (e != hipSuccess) return hip_error(ctx, e, "radix sort");
+    if (lookback) {
+        uint32_t flag = 0;
+        e = hipMemcpyAsync(&flag, w.counters + 8, 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_error(ctx, e, "radix sort");
+        if (flag) return set_error(ctx, SCR_ERR_HIP, "radix sort: a tile's look-back did not complete");
+    }
     if (passesRun) *passesRun = run;
     return SCR_OK;
 }
@@ -673,7 +887,7 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     {
         PassScope ps(ctx, "pack_bounds");
         hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, s, dBounds);
-        hipLaunchKernelGGL(k_bounds, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 4)),
+        hipLaunchKernelGGL(k_bounds, dim3(std::min<unsigned>(blocks(n, kThreads), (unsigned)ctx->numCUs * 8)),
                            dim3(kThreads), 0, s, pos, n, dBounds);
     }
     uint32_t hb[6];
@@ -705,10 +919,14 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     float4* sp = ar.take<float4>(n);
     uint32_t* spc = ar.take<uint32_t>(n);
     uint8_t* lvM = ar.take<uint8_t>(n);
-    uint32_t* thist = ar.take<uint32_t>(scanN);
-    uint32_t* offs = ar.take<uint32_t>(scanN);
-    uint32_t* sums = ar.take<uint32_t>(scanN / kScanSeg + 2);
-    uint32_t* ghist = ar.take<uint32_t>(8 * 256);
+    SortScratch sw;
+    sw.thist = ar.take<uint32_t>(scanN);
+    sw.offs = ar.take<uint32_t>(scanN);
+    sw.sums = ar.take<uint32_t>(scanN / kScanSeg + 2);
+    sw.ghist = ar.take<uint32_t>(8 * 256);
+    sw.status = ar.take<uint64_t>(scanN);
+    sw.dstart = ar.take<uint32_t>(8 * 256);
+    sw.counters = ar.take<uint32_t>(16);
     uint32_t* grid[2] = {ar.take<uint32_t>(gridCells[0]), ar.take<uint32_t>(gridCells[1])};
 
     // 2 Morton keys, 3 sort
@@ -723,7 +941,7 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
         uint64_t *kin = (uint64_t*)keysA, *kout = (uint64_t*)keysB;
         uint32_t *vin = idxA, *vout = idxB;
         PassScope ps(ctx, "pack_sort");
-        rc = radix_sort<uint64_t>(ctx, kin, vin, kout, vout, n, ghist, thist, offs, sums, nullptr);
+        rc = radix_sort<uint64_t>(ctx, kin, vin, kout, vout, n, sw, nullptr);
         sortedIdx = vin;
     } else {
         {
@@ -734,7 +952,7 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
         uint32_t *kin = (uint32_t*)keysA, *kout = (uint32_t*)keysB;
         uint32_t *vin = idxA, *vout = idxB;
         PassScope ps(ctx, "pack_sort");
-        rc = radix_sort<uint32_t>(ctx, kin, vin, kout, vout, n, ghist, thist, offs, sums, nullptr);
+        rc = radix_sort<uint32_t>(ctx, kin, vin, kout, vout, n, sw, nullptr);
         sortedIdx = vin;
     }
     if (rc != SCR_OK) return rc;
@@ -754,13 +972,13 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
         LodGrid g = lod_grid(hb, coarse, 0);
         e = launch_fill_u32(s, grid[0], lod_cells(g), 0xffffffffu);
         if (e != hipSuccess) return hip_error(ctx, e, "pack lod clear");
-        hipLaunchKernelGGL(k_lod_claim, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel, dBounds,
+        hipLaunchKernelGGL(k_lod_claim, dim3(blocks(n, kThreads * kLodPts)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel, dBounds,
                            g, grid[0]);
         for (int level = 0; level + 1 < lodLevels - 1; level++) {
             const LodGrid gn = lod_grid(hb, coarse, level + 1);
             e = launch_fill_u32(s, grid[(level + 1) & 1], lod_cells(gn), 0xffffffffu);
             if (e != hipSuccess) return hip_error(ctx, e, "pack lod clear");
-            hipLaunchKernelGGL(k_lod_step, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel,
+            hipLaunchKernelGGL(k_lod_step, dim3(blocks(n, kThreads * kLodPts)), dim3(kThreads), 0, s, sp, lvM, n, maxLevel,
                                (uint8_t)level, dBounds, g, grid[level & 1], gn, grid[(level + 1) & 1]);
             g = gn;
         }

@@ -40,6 +40,8 @@ import numpy as np
 REPO = Path(__file__).resolve().parent
 PKG_DIR = REPO / "satin-computerasteriser_amd"
 HBM_PEAK_GBS = 8000.0
+# the secondary measurements (bench.py --sections)
+SECTIONS = ("clod", "nearest", "pack", "blend", "holefill", "ply", "flythrough", "diff", "multiview")
 
 
 def load_pkg():
@@ -736,10 +738,22 @@ def main() -> int:
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-mode measurement and fit")
+    ap.add_argument("--sections", default="all",
+                    help="comma-separated secondary measurements to run (" + ",".join(SECTIONS) + "), or all")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (the known-size copy kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    sections = set(SECTIONS) if args.sections == "all" else {x for x in args.sections.split(",") if x}
+    unknown = sections - set(SECTIONS)
+    if unknown:
+        ap.error(f"unknown sections {sorted(unknown)}; choose from {SECTIONS}")
+    if args.no_extra:
+        sections = set()
+    if args.no_multiview:
+        sections.discard("multiview")
+    if args.no_diff:
+        sections.discard("diff")
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the library has no "
               "multi-GPU path", file=sys.stderr)
@@ -748,7 +762,7 @@ def main() -> int:
         print("bench.py runs as a single process on one GPU", file=sys.stderr)
         return 2
 
-    if not (args.no_extra or args.no_diff):
+    if "diff" in sections:
         # torch (for the fit) must own the process's HIP runtime: import it
         # before the library creates its context
         try:
@@ -764,7 +778,7 @@ def main() -> int:
     cloud = scr.ComputeRasteriserPointCloud(ctx, 1, scr.DEFAULT_POINTS_PER_BATCH)
     pack_100m = {}
     # the 63-bit pack last: it is the headline workload's cloud
-    for bits in ((63,) if args.no_extra else (30, 63)):
+    for bits in ((30, 63) if "pack" in sections else (63,)):
         packer = scr.GPUPacker(ctx, keyBits=bits)
         ctx.sync()
         runs = []
@@ -816,12 +830,13 @@ def main() -> int:
              "pass_ms": {k: round(v, 4) for k, v in prof.items()},
              "processed_points": processed, "visible_batches": int(len(vis)),
              "depth_algorithmic_bytes": depth_bytes}
-    if not args.no_extra:
-        cb_bytes, _ = algorithmic_bytes(vis, per_point_extra=4)
-        extra["color_roofline_frac"] = round(cb_bytes / (prof.get("color", float("nan")) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    cb_bytes, _ = algorithmic_bytes(vis, per_point_extra=4)
+    extra["color_roofline_frac"] = round(cb_bytes / (prof.get("color", float("nan")) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if sections:
         extra["atomic_roofline"] = {"k_depth": atomic_roofline(scr, ctx, "k_depth", depth_ms, add=False),
                                     "k_color": atomic_roofline(scr, ctx, "k_color", prof.get("color", float("nan")),
                                                                add=True)}
+    if "clod" in sections:
         # CLOD on at the same pose
         ras.configuration = scr.ComputeRasteriserConfiguration(enableCLOD=True)
         ms_c = timed_frames(ctx, ras, cam, args.steps, 1)
@@ -829,30 +844,38 @@ def main() -> int:
         _, proc_c = algorithmic_bytes(vis_c)
         extra["hqs_clod_on"] = {"ms_per_frame": round(ms_c, 4), "processed_points": proc_c,
                                 "Gpts_per_s_of_cloud": round(N / (ms_c * 1e-3) / 1e9, 3)}
+    if "nearest" in sections:
         # nearest-point mode, CLOD off
         ras.configuration = scr.ComputeRasteriserConfiguration(mode=scr.MODE_NEAREST_POINT, enableCLOD=False)
         ms_n = timed_frames(ctx, ras, cam, args.steps, 1)
         extra["nearest"] = {"ms_per_frame": round(ms_n, 4), "Gpts_per_s": round(processed / (ms_n * 1e-3) / 1e9, 3)}
+    if "pack" in sections:
         log("packer: 140k-point nodes, 10 per call, 8 per frame, 13.6M scene (30/63-bit)")
         extra["pack"] = pack_bench(scr, ctx)
         log(f"pack {extra['pack']}")
-        log("weighted-blend modes (defocus, motion blur), hole fill, PLY input")
+    if "blend" in sections:
+        log("weighted-blend modes (defocus, motion blur)")
         extra["blend_modes"] = blend_bench(scr, ctx, ras, cloud, cam, processed, max(3, args.steps // 10), W, H)
+        log(f"blend {extra['blend_modes']}")
+    if "holefill" in sections:
         ras.configuration = cfg
         extra["hole_fill_1080p"] = hole_fill_bench(scr, ctx, ras, cam, W, H)
+        log(f"hole fill {extra['hole_fill_1080p']}")
+    if "ply" in sections:
         extra["ply"] = ply_bench(scr, ctx)
-        log(f"blend {extra['blend_modes']} hole fill {extra['hole_fill_1080p']} ply {extra['ply']}")
-        ras.configuration = cfg
+        log(f"ply {extra['ply']}")
+    ras.configuration = cfg
+    if "flythrough" in sections:
         log("CLOD fly-through, 120 frames at 2560x1440")
         extra["flythrough_120_1440p"] = flythrough_bench(scr, ctx, cloud)
-        if not args.no_diff:
-            log("differentiable HQS (5M points, 32 views 512x512) and Adam fit")
-            extra["diff_hqs"] = diff_bench(scr, ctx, max(2, args.steps // 5))
-            log(f"diff {extra['diff_hqs']}")
-        if not args.no_multiview:
-            log("batched multi-view (64 views, 4 clouds)")
-            extra["multiview_64"] = multiview_bench(scr, ctx, max(3, args.steps // 4), 1)
-            log(f"multi-view {extra['multiview_64']}")
+    if "diff" in sections:
+        log("differentiable HQS (5M points, 32 views 512x512) and Adam fit")
+        extra["diff_hqs"] = diff_bench(scr, ctx, max(2, args.steps // 5))
+        log(f"diff {extra['diff_hqs']}")
+    if "multiview" in sections:
+        log("batched multi-view (64 views, 4 clouds)")
+        extra["multiview_64"] = multiview_bench(scr, ctx, max(3, args.steps // 4), 1)
+        log(f"multi-view {extra['multiview_64']}")
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -357,14 +357,31 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
     float dx[PPT], dy[PPT], dz[PPT];      // drawn (decoded + displacement)
     bool keep[PPT];
 #pragma unroll
+    for (int j = 0; j < PPT; j++) keep[j] = (uint32_t)j < n;
+    if (!bc.keepAll) {
+        // LOD / dither test: a point is kept when its dither value lies below
+        // t = (lodThreshold - level) + 0.5.  Dither values lie in [0, 1], so
+        // a wave whose points all have t > 1 keeps them all without hashing
+        // (one uniform branch per chunk; the active prefix ends at the
+        // threshold's level, so most waves hold only lower levels).
+        float t[PPT];
+        bool hash = false;
+#pragma unroll
+        for (int j = 0; j < PPT; j++) {
+            t[j] = (bc.lodThreshold - (float)((r.lv4[j / 4] >> (8 * (j % 4))) & 7u)) + 0.5f;
+            hash = hash || !(t[j] > 1.0f);
+        }
+        if (__ballot(hash) != 0ull) {
+#pragma unroll
+            for (int j = 0; j < PPT; j++) {
+                const float dither = (p.lodDither != 0) ? hash_unit(base + (uint32_t)j) : 0.5f;
+                keep[j] = keep[j] && !(dither >= t[j]);
+            }
+        }
+    }
+#pragma unroll
     for (int j = 0; j < PPT; j++) {
         const uint32_t pi = base + (uint32_t)j;
-        keep[j] = (uint32_t)j < n;
-        if (!bc.keepAll) {
-            const float pointLevel = (float)((r.lv4[j / 4] >> (8 * (j % 4))) & 7u);
-            const float dither = (p.lodDither != 0) ? hash_unit(pi) : 0.5f;
-            keep[j] = keep[j] && !(dither >= (bc.lodThreshold - pointLevel) + 0.5f);
-        }
         uint32_t qx, qy, qz;
         const uint32_t l = r.lo[j];
         if (kLevel == 0) {

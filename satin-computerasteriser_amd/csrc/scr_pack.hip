@@ -9,10 +9,11 @@
 //                  passes are trivial; per pass tile histograms → device-wide exclusive
 //                  scan (digit-major) → stable scatter: ranked in LDS per tile, staged in
 //                  tile-sorted order, each digit's run written as consecutive addresses;
-//                  64-bit keys of large inputs in one kernel per pass with decoupled
-//                  look-back instead of the histogram and scan kernels          :134-215
-//   4 gather       the one gather from the caller's order: positions and RGBA8
-//                  colours into Morton order                                      :219-235
+//                  large inputs in one kernel per pass with decoupled look-back
+//                  instead of the histogram and scan kernels                     :134-215
+//   4 gather       the one gather from the caller's order: 16-byte point records
+//                  (position + RGBA8 colour, written with the keys) into Morton
+//                  order                                                           :219-235
 //   5 LOD levels   dense voxel grids, atomicMin "first claimer", wave run de-dup; one
 //                  kernel per level takes that level's winners and claims the next :317-371
 //   6-9 finalize   one workgroup per batch: last level's winners, cum counts, AABB,
@@ -105,10 +106,23 @@ __global__ void k_bounds_init(uint32_t* bounds) {
 }
 
 // ---------------------------------------------------------------- 2 morton
+// packGather colour conversion: uint(clamp(c, 0, 1) * 255) per channel (truncating)
+__device__ inline uint32_t pack_color(float4 c) {
+    const uint32_t r = f2u(fminf(fmaxf(c.x, 0.0f), 1.0f) * 255.0f);
+    const uint32_t g = f2u(fminf(fmaxf(c.y, 0.0f), 1.0f) * 255.0f);
+    const uint32_t b = f2u(fminf(fmaxf(c.z, 0.0f), 1.0f) * 255.0f);
+    const uint32_t a = f2u(fminf(fmaxf(c.w, 0.0f), 1.0f) * 255.0f);
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+// Keys, the identity payload, and each point's 16-byte record (x, y, z and
+// its RGBA8 colour in the w bits) in the caller's order: the gather after the
+// sort then fetches one record per point instead of a position and a colour.
 template <typename K>
-__global__ __launch_bounds__(kThreads) void k_morton(const float4* __restrict__ pos, size_t n,
-                                                     const uint32_t* __restrict__ bounds, K* __restrict__ keys,
-                                                     uint32_t* __restrict__ idx) {
+__global__ __launch_bounds__(kThreads) void k_morton(const float4* __restrict__ pos, const float4* __restrict__ col,
+                                                     size_t n, const uint32_t* __restrict__ bounds,
+                                                     K* __restrict__ keys, uint32_t* __restrict__ idx,
+                                                     float4* __restrict__ rec) {
     const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     constexpr bool wide = sizeof(K) == 8;
@@ -128,6 +142,7 @@ __global__ __launch_bounds__(kThreads) void k_morton(const float4* __restrict__ 
     else
         keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
     idx[i] = (uint32_t)i;
+    rec[i] = make_float4(p.x, p.y, p.z, __uint_as_float(pack_color(col[i])));
 }
 
 // ---------------------------------------------------------------- 3 radix sort
@@ -164,8 +179,10 @@ __global__ __launch_bounds__(kThreads) void k_tile_hist(const K* __restrict__ ke
     thist[(size_t)threadIdx.x * numTiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Block-wide exclusive scan of one value per thread; returns the block total.
-__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl, uint32_t* lds /*>= 4*/) {
+// Block-wide exclusive scan of one value per thread over NT threads; returns
+// the block total.
+template <int NT = kThreads>
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl, uint32_t* lds /*>= NT / 64*/) {
     uint32_t incl = v;
     for (int off = 1; off < 64; off <<= 1) {
         const uint32_t t = __shfl_up(incl, off);
@@ -175,7 +192,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t& excl, uint
     if (__lane_id() == 63) lds[w] = incl;
     __syncthreads();
     uint32_t waveBase = 0, total = 0;
-    for (int k = 0; k < kThreads / 64; k++) {
+    for (int k = 0; k < NT / 64; k++) {
         const uint32_t s = lds[k];
         if (k < w) waveBase += s;
         total += s;
@@ -350,21 +367,44 @@ __global__ __launch_bounds__(256) void k_digit_starts(const uint32_t* __restrict
     }
 }
 
+// Shape of the look-back pass: NT threads of KPT keys each, NT * KPT keys per
+// tile, and the number of lower tiles whose words one look-back step reads at
+// once (measured on 100M-point packs: 30-bit keys walk one tile per step,
+// 63-bit keys four).  The tile is staged in tile-sorted order in LDS; 64-bit
+// keys and their payload are staged one after the other in the same words
+// (two rounds), which keeps the workgroup at 42 KB of LDS.
 template <typename K>
-__global__ __launch_bounds__(kThreads) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
-                                                       K* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
-                                                       int shift, const uint32_t* __restrict__ dstart,
-                                                       uint64_t* __restrict__ status, uint32_t* __restrict__ tileCounter,
-                                                       uint32_t* __restrict__ err) {
-    constexpr int kWaves = kThreads / 64;
-    constexpr int kPerWave = kTile / kWaves;
-    __shared__ K sk[kTile];
-    __shared__ uint32_t sv[kTile];
+struct SweepShape;
+template <>
+struct SweepShape<uint32_t> {
+    static constexpr int kNT = 512, kKPT = 8, kWindow = 1;
+};
+template <>
+struct SweepShape<uint64_t> {
+    static constexpr int kNT = 512, kKPT = 8, kWindow = 4;
+};
+template <typename K>
+constexpr int sweep_tile() { return SweepShape<K>::kNT * SweepShape<K>::kKPT; }
+
+template <typename K, int NT, int KPT, int kWindow>
+__global__ __launch_bounds__(NT, 1024 / NT) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                 K* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
+                                                 int shift, const uint32_t* __restrict__ dstart,
+                                                 uint64_t* __restrict__ status, uint32_t* __restrict__ tileCounter,
+                                                 uint32_t* __restrict__ err) {
+    constexpr int kWaves = NT / 64;
+    constexpr int kTileKeys = NT * KPT;
+    constexpr int kPerWave = kTileKeys / kWaves;
+    constexpr bool kTwoRounds = sizeof(K) > 4;
+    static_assert(NT >= 256, "one thread per digit");
+    __shared__ K sk[kTileKeys];
+    __shared__ uint32_t svOwn[kTwoRounds ? 1 : kTileKeys];
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t lstart[256];
     __shared__ uint32_t gbase[256];
     __shared__ uint32_t scanTmp[kWaves];
     __shared__ uint32_t tileShared;
+    uint32_t* sv = kTwoRounds ? reinterpret_cast<uint32_t*>(sk) : svOwn;
     const int tid = threadIdx.x;
     const int w = tid >> 6;
     const int lane = __lane_id();
@@ -372,21 +412,22 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const K* __restrict__ kin
     for (int j = lane; j < 256; j += 64) wcnt[w][j] = 0u;
     __syncthreads();
     const uint32_t tile = tileShared;
-    const size_t base = (size_t)tile * kTile;
-    const uint32_t cnt = (uint32_t)min((size_t)kTile, n - base);
-    K kr[kChunks];
-    uint32_t vr[kChunks];
+    const size_t base = (size_t)tile * kTileKeys;
+    const uint32_t cnt = (uint32_t)min((size_t)kTileKeys, n - base);
+    K kr[KPT];
+    uint32_t vr[KPT];
 #pragma unroll
-    for (int c = 0; c < kChunks; c++) {
+    for (int c = 0; c < KPT; c++) {
         const uint32_t li = (uint32_t)(w * kPerWave + c * 64 + lane);
         const bool valid = li < cnt;
         kr[c] = valid ? kin[base + li] : (K)0;
         vr[c] = valid ? vin[base + li] : 0u;
     }
+    // rank among the wave's earlier keys of the same digit
     const uint64_t lt = lanemask_lt();
-    uint32_t rk[kChunks];
+    uint32_t rk[KPT];
 #pragma unroll
-    for (int c = 0; c < kChunks; c++) {
+    for (int c = 0; c < KPT; c++) {
         const bool valid = (uint32_t)(w * kPerWave + c * 64 + lane) < cnt;
         const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
         uint64_t peers = __ballot(valid);
@@ -402,8 +443,9 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const K* __restrict__ kin
         if (valid && rank == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    {
-        uint32_t acc = 0;
+    uint32_t acc = 0;
+    uint64_t excl = 0;
+    if (tid < 256) {
 #pragma unroll
         for (int k = 0; k < kWaves; k++) {
             const uint32_t v = wcnt[k][tid];
@@ -413,82 +455,109 @@ __global__ __launch_bounds__(kThreads) void k_onesweep(const K* __restrict__ kin
         // publish this tile's count of digit tid, then look back for the
         // count of the digit in all lower tiles
         uint64_t* mine = status + (size_t)tile * 256 + tid;
-        uint64_t excl = 0;
         if (tile == 0) {
             __hip_atomic_store(mine, kPrefix | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(mine, kAggregate | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // the words of kWindow lower tiles are read at once (one memory
+            // round trip per window), then consumed in order: counts are added
+            // up to the first inclusive prefix; a word not yet published stops
+            // the window and is polled again
             int64_t j = (int64_t)tile - 1;
             int polls = 0;
-            while (true) {
-                const uint64_t v = __hip_atomic_load(status + (size_t)j * 256 + tid, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                if ((v & ~kCountMask) == 0) {
+            bool done = false;
+            while (!done) {
+                uint64_t v[kWindow];
+#pragma unroll
+                for (int k = 0; k < kWindow; k++)
+                    v[k] = j - k >= 0 ? __hip_atomic_load(status + (size_t)(j - k) * 256 + tid, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : kPrefix;   // below tile 0: nothing to add
+                int used = kWindow;
+#pragma unroll
+                for (int k = 0; k < kWindow; k++) {
+                    if (done || used < kWindow) continue;
+                    if ((v[k] & ~kCountMask) == 0) {
+                        used = k;
+                        continue;
+                    }
+                    excl += v[k] & kCountMask;
+                    if ((v[k] & ~kCountMask) == kPrefix) done = true;
+                }
+                if (done) break;
+                j -= used;
+                if (used < kWindow) {
                     if (++polls > kLookbackPolls) {
                         atomicOr(err, 1u);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
                 }
-                excl += v & kCountMask;
-                if ((v & ~kCountMask) == kPrefix || j == 0) break;
-                j--;
             }
             __hip_atomic_store(mine, kPrefix | (excl + acc), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        uint32_t lex;
-        (void)block_exclusive_scan(acc, lex, scanTmp);
+    }
+    uint32_t lex;
+    (void)block_exclusive_scan<NT>(acc, lex, scanTmp);
+    if (tid < 256) {
         lstart[tid] = lex;
         gbase[tid] = dstart[tid] + (uint32_t)excl - lex;
     }
     __syncthreads();
+    // stage the keys (and 32-bit payloads) in tile-sorted order
 #pragma unroll
-    for (int c = 0; c < kChunks; c++) {
+    for (int c = 0; c < KPT; c++) {
         const bool valid = (uint32_t)(w * kPerWave + c * 64 + lane) < cnt;
+        const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
+        rk[c] += lstart[d] + wcnt[w][d];   // the key's place in the tile
         if (valid) {
-            const uint32_t d = (uint32_t)((kr[c] >> shift) & 0xff);
-            const uint32_t p = lstart[d] + wcnt[w][d] + rk[c];
-            sk[p] = kr[c];
-            sv[p] = vr[c];
+            sk[rk[c]] = kr[c];
+            if (!kTwoRounds) sv[rk[c]] = vr[c];
         }
     }
     __syncthreads();
-    for (uint32_t j = tid; j < cnt; j += kThreads) {
-        const K k = sk[j];
-        const uint32_t o = gbase[(uint32_t)((k >> shift) & 0xff)] + j;
-        // o < n holds whenever every look-back completed; after a failed one
-        // (err set, the pack fails) no write leaves the arrays
-        if (o < n) {
-            kout[o] = k;
-            vout[o] = sv[j];
+    // thread t writes staged keys t, t + NT, ...: each digit's run leaves as
+    // consecutive addresses.  o < n holds whenever every look-back completed;
+    // after a failed one (err set, the pack fails) no write leaves the arrays.
+    uint32_t o[KPT];
+#pragma unroll
+    for (int c = 0; c < KPT; c++) {
+        const uint32_t j = (uint32_t)(c * NT + tid);
+        o[c] = 0xffffffffu;
+        if (j < cnt) {
+            const K k = sk[j];
+            o[c] = gbase[(uint32_t)((k >> shift) & 0xff)] + j;
+            if (o[c] < n) {
+                kout[o[c]] = k;
+                if (!kTwoRounds) vout[o[c]] = sv[j];
+            }
+        }
+    }
+    if (kTwoRounds) {
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < KPT; c++)
+            if ((uint32_t)(w * kPerWave + c * 64 + lane) < cnt) sv[rk[c]] = vr[c];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < KPT; c++) {
+            const uint32_t j = (uint32_t)(c * NT + tid);
+            if (j < cnt && o[c] < n) vout[o[c]] = sv[j];
         }
     }
 }
 
 // ---------------------------------------------------------------- 4 gather
-// packGather colour conversion: uint(clamp(c, 0, 1) * 255) per channel (truncating)
-__device__ inline uint32_t pack_color(float4 c) {
-    const uint32_t r = f2u(fminf(fmaxf(c.x, 0.0f), 1.0f) * 255.0f);
-    const uint32_t g = f2u(fminf(fmaxf(c.y, 0.0f), 1.0f) * 255.0f);
-    const uint32_t b = f2u(fminf(fmaxf(c.z, 0.0f), 1.0f) * 255.0f);
-    const uint32_t a = f2u(fminf(fmaxf(c.w, 0.0f), 1.0f) * 255.0f);
-    return r | (g << 8) | (b << 16) | (a << 24);
-}
-
-// The one gather from the caller's order: positions and RGBA8 colours into
-// Morton order (every later stage reads inside a batch), levels reset to the
-// finest.
-__global__ __launch_bounds__(kThreads) void k_gather_morton(const float4* __restrict__ pos,
-                                                            const float4* __restrict__ col,
+// The one gather from the caller's order: the points' records (position and
+// RGBA8 colour) into Morton order (every later stage reads inside a batch),
+// levels reset to the finest.
+__global__ __launch_bounds__(kThreads) void k_gather_morton(const float4* __restrict__ rec,
                                                             const uint32_t* __restrict__ idx, size_t n,
                                                             uint8_t maxLevel, float4* __restrict__ sp,
-                                                            uint32_t* __restrict__ spc, uint8_t* __restrict__ lv) {
+                                                            uint8_t* __restrict__ lv) {
     const size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    const uint32_t s = idx[i];
-    sp[i] = pos[s];
-    spc[i] = pack_color(col[s]);
+    sp[i] = rec[idx[i]];
     lv[i] = maxLevel;
 }
 
@@ -604,8 +673,7 @@ __global__ __launch_bounds__(kThreads) void k_lod_step(const float4* __restrict_
 //     ballots, per-wave counts in LDS), writing each point's final slot: level,
 //     three 10-bit position words quantised against the batch AABB, RGBA8
 //     colour, and (when asked) its source index.
-__global__ __launch_bounds__(kThreads) void k_finalize(const float4* __restrict__ sp, const uint32_t* __restrict__ spc,
-                                                       uint8_t* __restrict__ lv, const uint32_t* __restrict__ srcIdx,
+__global__ __launch_bounds__(kThreads) void k_finalize(const float4* __restrict__ sp, uint8_t* __restrict__ lv, const uint32_t* __restrict__ srcIdx,
                                                        size_t n, uint32_t ppb, uint32_t slotBase, uint8_t maxLevel,
                                                        int lastLevel, const uint32_t* __restrict__ bounds, LodGrid g,
                                                        const uint32_t* __restrict__ grid,
@@ -702,11 +770,12 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const float4* __restrict_
             uint32_t pos = running[d] + rank;
             for (int k = 0; k < w; k++) pos += wcount[k][d];
             const size_t o = start + pos;
+            // the record: position, RGBA8 colour in the w bits
+            const float4 p = sp[i];
             levelsOut[o] = lvl;
-            colorsOut[o] = spc[i];
+            colorsOut[o] = __float_as_uint(p.w);
             if (permOut) permOut[o] = srcIdx[i];
             // quantize (p - min) / size, IEEE division, as the CPU pack
-            const float4 p = sp[i];
             const float v[3] = {p.x, p.y, p.z};
             uint32_t q[3];
             for (int a = 0; a < 3; a++) {
@@ -753,10 +822,12 @@ static size_t pack_scratch_bytes(size_t n, bool wide, size_t grid0, size_t grid1
     const size_t k = wide ? 8 : 4;
     size_t b = 0;
     b += 2 * n * k + 2 * n * 4;                 // keys, payload ping-pong
-    b += n * 16 + n * 4;                        // Morton-order positions, colours
+    b += 2 * n * 16;                            // point records in caller and in Morton order
     b += n;                                     // Morton-order levels
     b += 2 * scanN * 4 + (scanN / kScanSeg + 2) * 4;  // tile hist, offsets, scan sums
-    b += scanN * 8 + 8 * 256 * 4 + 16 * 4;      // look-back words, digit starts, counters
+    const size_t sweepTile = (size_t)std::min(sweep_tile<uint32_t>(), sweep_tile<uint64_t>());
+    b += std::max(scanN, 256 * ((n + sweepTile - 1) / sweepTile)) * 8;   // look-back words
+    b += 8 * 256 * 4 + 16 * 4;                  // digit starts, counters
     b += 8 * 256 * 4 + 64;                      // digit histograms
     b += (grid0 + grid1) * 4;
     b += 32 * 256;                              // alignment slack
@@ -796,20 +867,23 @@ struct SortScratch {
     uint32_t* counters; // per pass tile counters [0, 8), look-back failure flag [8]
 };
 
-// 64-bit keys of at least this many points sort with the one-kernel
-// look-back pass; shorter keys and smaller inputs with the split
-// histogram / scan / scatter pass (measured faster there).
-constexpr size_t kLookbackMinPoints = size_t(1) << 24;
+// Inputs of at least this many points sort with the one-kernel look-back
+// pass; smaller ones with the split histogram / scan / scatter pass
+// (measured faster there).
+template <typename K>
+constexpr size_t lookback_min_points() { return size_t(1) << 24; }
 
 template <typename K>
 static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*& vout, size_t n,
                       const SortScratch& w, int* passesRun) {
     hipStream_t s = ctx->stream;
     constexpr int P = sizeof(K);
-    const bool lookback = P == 8 && n >= kLookbackMinPoints;
+    const bool lookback = n >= lookback_min_points<K>();
     const uint32_t numTiles = (uint32_t)((n + kTile - 1) / kTile);
     const size_t scanN = (size_t)256 * numTiles;
     const uint32_t nseg = (uint32_t)((scanN + kScanSeg - 1) / kScanSeg);
+    constexpr int kSweepNT = SweepShape<K>::kNT, kSweepKPT = SweepShape<K>::kKPT, kSweepWin = SweepShape<K>::kWindow;
+    const uint32_t sweepTiles = (uint32_t)((n + sweep_tile<K>() - 1) / sweep_tile<K>());
     hipError_t e = launch_fill_u32(s, w.ghist, P * 256, 0u);
     if (e == hipSuccess && lookback) e = launch_fill_u32(s, w.counters, 16, 0u);
     if (e != hipSuccess) return hip_error(ctx, e, "radix hist clear");
@@ -828,10 +902,10 @@ static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*
         if (trivial) continue;  // every key has the same digit: the pass is the identity
         const int shift = 8 * p;
         if (lookback) {
-            e = launch_fill_u32(s, (uint32_t*)w.status, scanN * 2, 0u);
+            e = launch_fill_u32(s, (uint32_t*)w.status, (size_t)sweepTiles * 256 * 2, 0u);
             if (e != hipSuccess) return hip_error(ctx, e, "radix look-back clear");
-            hipLaunchKernelGGL(k_onesweep<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, vin, kout, vout, n, shift,
-                               w.dstart + p * 256, w.status, w.counters + p, w.counters + 8);
+            hipLaunchKernelGGL((k_onesweep<K, kSweepNT, kSweepKPT, kSweepWin>), dim3(sweepTiles), dim3(kSweepNT), 0, s, kin, vin,
+                               kout, vout, n, shift, w.dstart + p * 256, w.status, w.counters + p, w.counters + 8);
         } else {
             hipLaunchKernelGGL(k_tile_hist<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, n, shift, numTiles, w.thist);
             hipLaunchKernelGGL(k_scan_reduce, dim3(nseg), dim3(kThreads), 0, s, w.thist, scanN, w.sums);
@@ -916,15 +990,18 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     void* keysB = ar.take<char>(n * (wide ? 8 : 4));
     uint32_t* idxA = ar.take<uint32_t>(n);
     uint32_t* idxB = ar.take<uint32_t>(n);
+    float4* rec = ar.take<float4>(n);
     float4* sp = ar.take<float4>(n);
-    uint32_t* spc = ar.take<uint32_t>(n);
     uint8_t* lvM = ar.take<uint8_t>(n);
     SortScratch sw;
     sw.thist = ar.take<uint32_t>(scanN);
     sw.offs = ar.take<uint32_t>(scanN);
     sw.sums = ar.take<uint32_t>(scanN / kScanSeg + 2);
     sw.ghist = ar.take<uint32_t>(8 * 256);
-    sw.status = ar.take<uint64_t>(scanN);
+    {
+        const size_t sweepTile = (size_t)std::min(sweep_tile<uint32_t>(), sweep_tile<uint64_t>());
+        sw.status = ar.take<uint64_t>(std::max(scanN, 256 * ((n + sweepTile - 1) / sweepTile)));
+    }
     sw.dstart = ar.take<uint32_t>(8 * 256);
     sw.counters = ar.take<uint32_t>(16);
     uint32_t* grid[2] = {ar.take<uint32_t>(gridCells[0]), ar.take<uint32_t>(gridCells[1])};
@@ -935,8 +1012,8 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     if (wide) {
         {
             PassScope ps(ctx, "pack_morton");
-            hipLaunchKernelGGL(k_morton<uint64_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
-                               (uint64_t*)keysA, idxA);
+            hipLaunchKernelGGL(k_morton<uint64_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, col, n,
+                               dBounds, (uint64_t*)keysA, idxA, rec);
         }
         uint64_t *kin = (uint64_t*)keysA, *kout = (uint64_t*)keysB;
         uint32_t *vin = idxA, *vout = idxB;
@@ -946,8 +1023,8 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     } else {
         {
             PassScope ps(ctx, "pack_morton");
-            hipLaunchKernelGGL(k_morton<uint32_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, n, dBounds,
-                               (uint32_t*)keysA, idxA);
+            hipLaunchKernelGGL(k_morton<uint32_t>, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, col, n,
+                               dBounds, (uint32_t*)keysA, idxA, rec);
         }
         uint32_t *kin = (uint32_t*)keysA, *kout = (uint32_t*)keysB;
         uint32_t *vin = idxA, *vout = idxB;
@@ -961,8 +1038,8 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     // 4 positions and colours into Morton order
     {
         PassScope ps(ctx, "pack_gather");
-        hipLaunchKernelGGL(k_gather_morton, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, pos, col, sortedIdx, n,
-                           maxLevel, sp, spc, lvM);
+        hipLaunchKernelGGL(k_gather_morton, dim3(blocks(n, kThreads)), dim3(kThreads), 0, s, rec, sortedIdx, n,
+                           maxLevel, sp, lvM);
     }
     // 5 LOD levels 0 .. L-3 (the last claimed level, L-2, is taken inside the finalize)
     int lastLevel = -1;
@@ -988,7 +1065,7 @@ int pack_points(scr_ctx* ctx, const float4* pos, const float4* col, size_t n, co
     // 6-9 per batch: last level, cum counts, AABB, record, bucketed final writes
     {
         PassScope ps(ctx, "pack_finalize");
-        hipLaunchKernelGGL(k_finalize, dim3((unsigned)nb), dim3(kThreads), 0, s, sp, spc, lvM, sortedIdx, n,
+        hipLaunchKernelGGL(k_finalize, dim3((unsigned)nb), dim3(kThreads), 0, s, sp, lvM, sortedIdx, n,
                            (uint32_t)ppb, dst.slotBase, maxLevel, lastLevel, dBounds, lastGrid,
                            lastLevel >= 0 ? grid[lastLevel & 1] : nullptr, dst.batches, dst.levels, dst.colors,
                            dst.low, dst.med, dst.high, hostSourceIndices ? permOut : nullptr);

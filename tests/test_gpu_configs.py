@@ -157,13 +157,15 @@ def test_config4_64_views_full_size(ctx):
 
 
 def test_config2_20m_gpu_pack_equals_cpu_pack(ctx):
-    """GPUPacker on the 20M-point scene (63-bit keys, 4 LOD levels, 64
-    coarse divisions) against the CPU pack: permutation, batch table, every
-    position word, colours and levels; 30-bit keys (the reference's) on a
-    2M-point prefix."""
+    """GPUPacker on the 20M-point scene (4 LOD levels, 64 coarse divisions)
+    against the CPU pack: permutation, batch table, every position word,
+    colours and levels, with 63-bit keys and with 30-bit keys (the
+    reference's); both take the one-kernel look-back sort at this size, and
+    the 30-bit keys of a 2M-point prefix the split histogram / scan / scatter
+    sort."""
     from test_gpu_pool_pack import _assert_pack_equal
     pos, col = scr.gen_scene(ctx, 20_000_000, on_device=False, seed=0x5C0002, halfExtent=1000.0, numBoxes=2000)
-    for bits, n in ((63, 20_000_000), (30, 2_000_000)):
+    for bits, n in ((63, 20_000_000), (30, 20_000_000), (30, 2_000_000)):
         ref = orc.pack(pos[:n, :3], col[:n], keyBits=bits)
         cloud = scr.ComputeRasteriserPointCloud(ctx, 1, scr.DEFAULT_POINTS_PER_BATCH)
         src = scr.GPUPacker(ctx, keyBits=bits).pack(cloud, pos[:n], col[:n], sourceIndices=True)

@@ -92,6 +92,7 @@ def lib() -> C.CDLL:
             "orc_uint_to_depth": (f, [u32]), "orc_f2u": (u32, [f]), "orc_f2i": (C.c_int32, [f]),
             "orc_mat4_mul": (None, [vp, vp, vp]), "orc_mat4_vec": (None, [vp, vp, vp]),
             "orc_transform_point": (None, [vp, vp, vp]),
+            "orc_project_q": (None, [vp, vp, vp, vp, vp]),
             "orc_decode_point": (None, [u32, vp, vp, vp, vp, C.c_int, C.c_int, vp]),
             "orc_morton30": (u32, [u32, u32, u32]), "orc_morton63": (C.c_uint64, [u32, u32, u32]),
             "orc_inside_footprint": (C.c_int, [C.c_int, C.c_int, C.c_int]),
@@ -172,6 +173,18 @@ def transform_point(m, p) -> np.ndarray:
     p = _f32(p).reshape(-1)
     out = np.zeros(4, np.float32)
     lib().orc_transform_point(m.ctypes.data, p.ctypes.data, out.ctypes.data)
+    return out
+
+
+def project_q(m, mn, step, qf) -> np.ndarray:
+    """The render passes' projection of an undisplaced point from its
+    quantised coordinates qf (floats) under transform m, batch minimum mn and
+    decode step per axis (contract C1)."""
+    m = _f32(m).reshape(-1)
+    args = [_f32(x).reshape(-1) for x in (mn, step, qf)]
+    out = np.zeros(4, np.float32)
+    lib().orc_project_q(m.ctypes.data, args[0].ctypes.data, args[1].ctypes.data, args[2].ctypes.data,
+                        out.ctypes.data)
     return out
 
 

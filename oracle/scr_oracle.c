@@ -238,16 +238,16 @@ int orc_inside_footprint(int ox, int oy, int radius) {
     return (x * x + y * y) <= r * r;
 }
 
-/* decodePoint — Common.metal:73-117 (fused: q * s + min as one fmaf, the
- * render passes' contract C1) and NearestPointCPUReference.swift:87-117
- * (fused = 0: Swift rounds the product and the sum separately). */
-void orc_decode_point(uint32_t pi, const scr_raster_batch* b, const uint32_t* low,
-                      const uint32_t* med, const uint32_t* high, int level, int fused, float out[3]) {
+/* The quantised coordinates of point pi at the batch precision (Common.metal:
+ * 73-117: 30, 20 or 10 bits from three, two or one 10-bit words) and the
+ * decode step per axis, max(size, 1e-6) * 2^-bits (a power-of-two scaling:
+ * exact). */
+static void orc_point_q(uint32_t pi, const scr_raster_batch* b, const uint32_t* low, const uint32_t* med,
+                        const uint32_t* high, int level, uint32_t q[3], float step[3]) {
     const float mn[3] = {b->minX, b->minY, b->minZ};
     const float mx[3] = {b->maxX, b->maxY, b->maxZ};
     float size[3];
     for (int i = 0; i < 3; i++) size[i] = fmaxf(mx[i] - mn[i], 0.000001f);
-    uint32_t q[3];
     float steps;
     const uint32_t lo = low[pi];
     if (level == 0) {
@@ -269,8 +269,39 @@ void orc_decode_point(uint32_t pi, const scr_raster_batch* b, const uint32_t* lo
         steps = 1024.0f;
     }
     const float invSteps = 1.0f / steps; /* power of two: exact */
+    for (int a = 0; a < 3; a++) step[a] = size[a] * invSteps;
+}
+
+/* decodePoint — Common.metal:73-117 (fused: q * s + min as one fmaf, the
+ * render passes' contract C1) and NearestPointCPUReference.swift:87-117
+ * (fused = 0: Swift rounds the product and the sum separately). */
+void orc_decode_point(uint32_t pi, const scr_raster_batch* b, const uint32_t* low,
+                      const uint32_t* med, const uint32_t* high, int level, int fused, float out[3]) {
+    const float mn[3] = {b->minX, b->minY, b->minZ};
+    uint32_t q[3];
+    float step[3];
+    orc_point_q(pi, b, low, med, high, level, q, step);
     for (int a = 0; a < 3; a++)
-        out[a] = fused ? fmaf((float)q[a], size[a] * invSteps, mn[a]) : (float)q[a] * (size[a] * invSteps) + mn[a];
+        out[a] = fused ? fmaf((float)q[a], step[a], mn[a]) : (float)q[a] * step[a] + mn[a];
+}
+
+/* The render passes' projection of an undisplaced point (contract C1):
+ * decodePoint (Common.metal:73-117) and the file transform (DepthPass/
+ * Shaders.metal:75-78) folded into one affine map of the quantised
+ * coordinates per batch, A[c][r] = T[c][r] * step_c (one rounding each),
+ * B = T (min, 1) (orc_transform_point), clip_r = fma(A[2][r], qz,
+ * fma(A[1][r], qy, fma(A[0][r], qx, B_r))). */
+void orc_project_q(const float* T, const float mn[3], const float step[3], const float qf[3], float out[4]) {
+    float B[4];
+    orc_transform_point(T, mn, B);
+#if ORC_MUTANT == 19
+    for (int r = 0; r < 4; r++) B[r] -= T[12 + r];
+#endif
+    for (int r = 0; r < 4; r++) {
+        float A[3];
+        for (int c = 0; c < 3; c++) A[c] = T[c * 4 + r] * step[ORC_MUTANT == 20 && c == 1 ? 0 : c];
+        out[r] = fmaf(A[2], qf[2], fmaf(A[1], qf[1], fmaf(A[0], qf[0], B[r])));
+    }
 }
 
 /* makePlane / intersectsFrustum — Common.metal:119-153 */
@@ -453,7 +484,16 @@ static int orc_front(const orc_cloud* c, const scr_raster_batch* b, const scr_ra
         if (ORC_MUTANT != 12 && (isnan(fr->point[0]) || isnan(fr->point[1]) || isnan(fr->point[2]))) return 0;
     }
     float clip[4];
-    orc_transform_point(f->transform, fr->point, clip);
+    if (cfg->applyDisplacement != 0 && !nearestZ) {
+        orc_transform_point(f->transform, fr->point, clip);
+    } else {
+        const float mn[3] = {b->minX, b->minY, b->minZ};
+        uint32_t q[3];
+        float step[3];
+        orc_point_q(pointIndex, b, c->xyzLow, c->xyzMed, c->xyzHigh, vb->level, q, step);
+        const float qf[3] = {(float)q[0], (float)q[1], (float)q[2]};
+        orc_project_q(f->transform, mn, step, qf, clip);
+    }
     if (clip[3] <= 0.0f) return 0;
     const float rw = orc_rcp(clip[3]);
     const float nx = clip[0] * rw, ny = clip[1] * rw, nz = clip[2] * rw;

@@ -57,6 +57,8 @@ int orc_footprint_radius(const float p[3], const scr_raster_file* file, const fl
 int orc_inside_footprint(int ox, int oy, int radius);
 void orc_decode_point(uint32_t pointIndex, const scr_raster_batch* b, const uint32_t* low,
                       const uint32_t* med, const uint32_t* high, int level, int fused, float out[3]);
+void orc_project_q(const float* T, const float mn[3], const float step[3], const float qf[3],
+                   float out4[4]); /* batch projection of quantised coordinates (C1) */
 int orc_intersects_frustum(const float* m, const float mn[3], const float mx[3]);
 float orc_pixel_size_on_screen(const scr_raster_batch* b, const scr_raster_file* f,
                                const float* view, const float* proj, int width, int height);

@@ -197,6 +197,7 @@ struct BatchCtx {
     int bx0, bx1, by0, by1;
     float zmax;
     uint32_t first, active, batchIndex;   // active: end of the drawn range
+    BatchAffine aff;    // file.transform folded with the batch decode (project_q)
     float T[16];        // file.transform (held in registers: the atomics below
                         // would otherwise force reloads through memory)
     const float* W;     // file.world (only read for per-point sizes)
@@ -226,6 +227,7 @@ __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParam
     const scr_raster_file* f = c.files + b.fileIndex;
 #pragma unroll
     for (int i = 0; i < 16; i++) bc.T[i] = gload(f->transform + i);
+    bc.aff = batch_affine(bc.T, bc.sx, bc.sy, bc.sz, bc.mnx, bc.mny, bc.mnz);
     bc.W = f->world;
     bc.rconst = bb.rconst;
     bc.fastRcp = (bb.flags & kBoundsFastRcp) != 0u;
@@ -340,10 +342,11 @@ __device__ __forceinline__ void chunk_tests(const PassParams& p, const BatchCtx&
 }
 
 // Runs the front end of every point pass on the fetched points: dither/LOD
-// skip, decodePoint (Common.metal:73-117), displacement (kDisp: position +
-// displacement, a NaN component drops the point; DepthPass/Shaders.metal:
-// 69-74, ColorPass/Shaders.metal:72-78), projection, clip and viewport tests,
-// y flip, footprint radius.
+// skip, decodePoint (Common.metal:73-117) and projection (project_q for
+// undisplaced points), displacement (kDisp: position + displacement, a NaN
+// component drops the point, then the transform; DepthPass/Shaders.metal:
+// 69-74, ColorPass/Shaders.metal:72-78), clip and viewport tests, y flip,
+// footprint radius.
 template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kUseInside = true, bool kDisp = false>
 __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                     const ChunkRaw<PPT>& r, ChunkFrags<PPT>& f) {
@@ -415,7 +418,10 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
             dy[j] = wy[j];
             dz[j] = wz[j];
         }
-        clip[j] = mul_point(bc.T, dx[j], dy[j], dz[j]);
+        // undisplaced points: the batch projection of the quantised
+        // coordinates; displaced ones: decode + displacement, then the
+        // transform
+        clip[j] = kDisp ? mul_point(bc.T, dx[j], dy[j], dz[j]) : project_q(bc.aff, (float)qx, (float)qy, (float)qz);
     }
     // 1 / w, one wave-uniform branch per chunk
     float rw[PPT];

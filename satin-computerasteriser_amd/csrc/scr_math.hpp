@@ -4,8 +4,10 @@
 //   * binary32 RNE, source operation order of the Metal passes; no implicit
 //     contraction (every .hip file is built with -ffp-contract=off), and
 //     explicit fmaf exactly where the per-point front end multiplies then
-//     adds: decodePoint's q * s + min, the point transform (mul_point) and
-//     the viewport's ndc * 0.5 + 0.5;
+//     adds: decodePoint's q * s + min, the point transform (mul_point), the
+//     batch projection of quantised coordinates (project_q: decode and
+//     transform folded into one per-batch affine map) and the viewport's
+//     ndc * 0.5 + 0.5;
 //   * a / b in the Metal passes is a * (1.0f / b) with the correctly rounded
 //     reciprocal (Metal fast-math reciprocal division made exact);
 //   * sqrt correctly rounded; log2 = scr_log2f (double-precision, + - * / only);
@@ -182,6 +184,45 @@ __host__ __device__ inline f4 mul_point(const float* m, float x, float y, float 
     return r;
 }
 
+// The draw passes' projection of an undisplaced point: decodePoint (q * s +
+// min) followed by the file transform T, folded per batch into one affine map
+// of the quantised coordinates,
+//     A[c][r] = T[c][r] * s_c        (one rounding per entry)
+//     B       = mul_point(T, min)    (the transformed batch minimum)
+//     clip_r  = fma(A[2][r], qz, fma(A[1][r], qy, fma(A[0][r], qx, B_r)))
+// with q converted to float as decodePoint converts it.  Three fused
+// multiply-adds per clip coordinate instead of the decode's three and the
+// transform's four.  (Metal's fast math leaves the rounding of decode and
+// transform unspecified; this fixes one order.)
+struct BatchAffine {
+    float a[12];  // column-major like T: a[c * 4 + r], c = x, y, z
+    float b[4];
+};
+__host__ __device__ inline BatchAffine batch_affine(const float* T, float sx, float sy, float sz, float mnx, float mny,
+                                                    float mnz) {
+    BatchAffine m;
+    const f4 o = mul_point(T, mnx, mny, mnz);
+    m.b[0] = o.x;
+    m.b[1] = o.y;
+    m.b[2] = o.z;
+    m.b[3] = o.w;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        m.a[r] = T[r] * sx;
+        m.a[4 + r] = T[4 + r] * sy;
+        m.a[8 + r] = T[8 + r] * sz;
+    }
+    return m;
+}
+__host__ __device__ inline f4 project_q(const BatchAffine& m, float qx, float qy, float qz) {
+    f4 r;
+    r.x = fmaf(m.a[8], qz, fmaf(m.a[4], qy, fmaf(m.a[0], qx, m.b[0])));
+    r.y = fmaf(m.a[9], qz, fmaf(m.a[5], qy, fmaf(m.a[1], qx, m.b[1])));
+    r.z = fmaf(m.a[10], qz, fmaf(m.a[6], qy, fmaf(m.a[2], qx, m.b[2])));
+    r.w = fmaf(m.a[11], qz, fmaf(m.a[7], qy, fmaf(m.a[3], qx, m.b[3])));
+    return r;
+}
+
 __host__ __device__ inline void mat_mul(const float* a, const float* b, float* out) {
     float r[16];
     for (int c = 0; c < 4; c++) {
@@ -320,21 +361,24 @@ __host__ inline int const_radius_for(float lo) { return radius_of_size(lo, lo, l
 // ---------------------------------------------------------------- batch bounds
 //
 // What the cull pass proves about every point of a visible batch from the
-// eight corners of its box, for the draw passes to rely on.  The decoded
-// positions fma(q, size * 2^-bits, min) lie in the float box [min, c1] with
-// c1 = min + size (fma is monotone in q < 2^bits and rounding is monotone).
-// The point transform is affine, so the exact clip coordinates of a point
-// lie between the corner values, and since w > 0 on the whole box (all
-// corners), so do x / w, y / w and z / w.  The float clip coordinates the
-// passes compute (mul_point: four roundings of partial sums bounded by
-// A = |m0 x| + |m4 y| + |m8 z| + |m12|, largest at a corner) are within
-// e = 2^-20 A of the exact ones (the rounding error is below 2^-22 A), so the
-// float ratio n = x_f * rcp(w_f) satisfies
+// eight corners of its box, for the draw passes to rely on.  A point's
+// quantised coordinates (q converted to float: q_f <= 2^bits) stand for the
+// position P = min + q_f * s, s = size * 2^-bits exactly, which lies in the
+// box [min, min + size] (evaluated exactly, in double).  The transform is
+// affine, so the exact clip coordinates of P lie between the corner values,
+// and since w > 0 on the whole box (all corners), so do x / w, y / w and
+// z / w.  The float clip coordinates the passes compute (project_q: one
+// rounding per A entry, four in B = mul_point(T, min), three in the fma
+// chain) are within 7 * 2^-24 A_r of the exact ones, with
+// A_r = sum_c |T[c][r]| (|min_c| + size_c) + |T[3][r]|, which bounds every
+// partial sum (for displaced points nothing is proven); so with the
+// allowance e = 2^-20 A the float ratio n = x_f * rcp(w_f) satisfies
 //     |n - x / w| <= (e_x + N e_w) / (w_min - e_w) = D,  plus |n| 2^-22.9
 // for the two roundings of rcp and the product (N = the largest |x / w| of
 // the corners).  The viewport (fma(n, 0.5, 0.5) * W) adds at most 2^-23 W.
-// Corner values are evaluated in double: each product of two floats is exact
-// there and the sums' rounding (2^-52 A) is far inside the allowance.
+// Corner values are evaluated in double: a product of a float and a corner
+// coordinate (a sum of two floats) is exact to 2^-52 relative, and the sums'
+// rounding (a few 2^-52 A) is far inside the allowance.
 struct BatchBounds {
     int32_t x0, x1, y0, y1;  // kProven: every fragment centre's column ix and unflipped row iy lie in [x0, x1] x [y0, y1]
     float zmax;              // kProven: every fragment's ndc z is <= zmax
@@ -346,18 +390,18 @@ constexpr uint32_t kBoundsProven = 1u;   // the rectangle and zmax hold
 constexpr uint32_t kBoundsInside = 2u;   // no point fails the w, NDC or viewport tests (nearest's ndc z > 0 included)
 constexpr uint32_t kBoundsFastRcp = 4u;  // every point's clip w lies in [2^-125, 2^125] (rcp_fast is exact)
 
-// One corner through the first or second half of m in double: clip value
-// and magnitude A per row.
+// One corner through m in double: clip value per row (the magnitude bound A
+// per row comes from the whole box, batch_magnitudes).
 struct CornerTerms {
     double c[4], a[4];
 };
-__host__ __device__ inline CornerTerms corner_terms(const float* m, float x, float y, float z) {
+__host__ __device__ inline CornerTerms corner_terms(const float* m, double x, double y, double z, const double* a) {
     CornerTerms t;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const double tx = (double)m[i] * (double)x, ty = (double)m[4 + i] * (double)y, tz = (double)m[8 + i] * (double)z;
+        const double tx = (double)m[i] * x, ty = (double)m[4 + i] * y, tz = (double)m[8 + i] * z;
         t.c[i] = ((tx + ty) + tz) + (double)m[12 + i];
-        t.a[i] = ((fabs(tx) + fabs(ty)) + fabs(tz)) + fabs((double)m[12 + i]);
+        t.a[i] = a[i];
     }
     return t;
 }
@@ -394,15 +438,31 @@ __host__ __device__ inline CornerExtremes combine(const CornerExtremes& a, const
     return e;
 }
 
-// The float box of the batch's decoded positions: min and min + size (the
-// corner the decode reaches with q = 2^bits; size * 2^-bits * 2^bits = size).
-__host__ __device__ inline void batch_box(const scr_raster_batch& b, float lo[3], float hi[3]) {
-    lo[0] = b.minX;
-    lo[1] = b.minY;
-    lo[2] = b.minZ;
-    hi[0] = b.minX + fmaxf(b.maxX - b.minX, 0.000001f);
-    hi[1] = b.minY + fmaxf(b.maxY - b.minY, 0.000001f);
-    hi[2] = b.minZ + fmaxf(b.maxZ - b.minZ, 0.000001f);
+// The box of the batch's positions, exactly (a sum of two floats in double):
+// min and min + size, the corner q = 2^bits reaches (size * 2^-bits * 2^bits
+// = size); and the magnitude bound A_r = sum_c |m[c][r]| (|min_c| + size_c) +
+// |m[3][r]| of every partial sum the projection forms inside it.
+__host__ __device__ inline void batch_box(const scr_raster_batch& b, double lo[3], double hi[3]) {
+    const float mn[3] = {b.minX, b.minY, b.minZ};
+    const float size[3] = {fmaxf(b.maxX - b.minX, 0.000001f), fmaxf(b.maxY - b.minY, 0.000001f),
+                           fmaxf(b.maxZ - b.minZ, 0.000001f)};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        lo[c] = (double)mn[c];
+        hi[c] = (double)mn[c] + (double)size[c];
+    }
+}
+__host__ __device__ inline void batch_magnitudes(const float* m, const double lo[3], const double hi[3], double a[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        double s = fabs((double)m[12 + r]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) s += fabs((double)m[c * 4 + r]) * (fabs(lo[c]) + (hi[c] - lo[c]));
+        // the factor covers the rounding of this double sum; the constant
+        // (2^-80, an allowance of 2^-100) covers products of A entries that
+        // leave the normal float range
+        a[r] = s * (1.0 + 0x1p-40) + 0x1p-80;
+    }
 }
 
 // Bounds from the corners' extremes, for a width x height viewport.
@@ -455,12 +515,13 @@ __host__ __device__ inline BatchBounds finish_bounds(const CornerExtremes& e, in
 // thread (the cull pass).
 __host__ __device__ inline BatchBounds batch_bounds(const float* m, const scr_raster_batch& b, int width, int height,
                                                     int rconst) {
-    float lo[3], hi[3];
+    double lo[3], hi[3], a[4];
     batch_box(b, lo, hi);
-    CornerExtremes e = corner_extremes(corner_terms(m, lo[0], lo[1], lo[2]));
+    batch_magnitudes(m, lo, hi, a);
+    CornerExtremes e = corner_extremes(corner_terms(m, lo[0], lo[1], lo[2], a));
     for (int k = 1; k < 8; k++)
         e = combine(e, corner_extremes(corner_terms(m, (k & 1) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1],
-                                                    (k & 4) ? hi[2] : lo[2])));
+                                                    (k & 4) ? hi[2] : lo[2], a)));
     return finish_bounds(e, width, height, rconst);
 }
 

@@ -1,8 +1,9 @@
 // Host-side property check of the per-batch bounds the cull pass computes
 // (batch_bounds): for random and adversarial batches, cameras and viewports,
 // every point that decodes inside a batch is run through the draw passes'
-// exact per-point arithmetic (decode fma, mul_point, correctly rounded
-// reciprocal, viewport fma, truncating conversion), and
+// exact per-point arithmetic (the batch projection of its quantised
+// coordinates, project_q, correctly rounded reciprocal, viewport fma,
+// truncating conversion), and
 //   * with kBoundsProven, every fragment's column / unflipped row lies in the
 //     rectangle and its ndc z is <= zmax;
 //   * with kBoundsInside, every point passes the clip-w, NDC (nearest's
@@ -75,8 +76,8 @@ static Frag front(const float* T, const scr_raster_batch& b, int level, uint32_t
     const float sx = fmaxf(b.maxX - b.minX, 0.000001f) * inv;
     const float sy = fmaxf(b.maxY - b.minY, 0.000001f) * inv;
     const float sz = fmaxf(b.maxZ - b.minZ, 0.000001f) * inv;
-    const float x = fmaf((float)qx, sx, b.minX), y = fmaf((float)qy, sy, b.minY), z = fmaf((float)qz, sz, b.minZ);
-    const f4 clip = mul_point(T, x, y, z);
+    const BatchAffine m = batch_affine(T, sx, sy, sz, b.minX, b.minY, b.minZ);
+    const f4 clip = project_q(m, (float)qx, (float)qy, (float)qz);
     const float rw = rcp(clip.w);
     const float nx = clip.x * rw, ny = clip.y * rw, nz = clip.z * rw;
     Frag f;
@@ -94,13 +95,13 @@ static Frag front(const float* T, const scr_raster_batch& b, int level, uint32_t
 // Bounds with the rounding allowance removed (amax = 0): the plain corner
 // extremes, which the per-point rounding can leave.
 static BatchBounds bounds_without_allowance(const float* m, const scr_raster_batch& b, int W, int H) {
-    float lo[3], hi[3];
+    double lo[3], hi[3];
+    const double a[4] = {0.0, 0.0, 0.0, 0.0};
     batch_box(b, lo, hi);
-    CornerExtremes e = corner_extremes(corner_terms(m, lo[0], lo[1], lo[2]));
+    CornerExtremes e = corner_extremes(corner_terms(m, lo[0], lo[1], lo[2], a));
     for (int k = 1; k < 8; k++)
         e = combine(e, corner_extremes(corner_terms(m, (k & 1) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1],
-                                                    (k & 4) ? hi[2] : lo[2])));
-    for (double& a : e.amax) a = 0.0;
+                                                    (k & 4) ? hi[2] : lo[2], a)));
     return finish_bounds(e, W, H, -1);
 }
 

@@ -29,6 +29,8 @@ MUTANTS = {
     12: ("no NaN cull of displaced points", pins.pin_displacement),
     13: ("the motion-blur sweep from the displaced point instead of position + previous displacement",
          pins.pin_displacement),
+    19: ("the batch projection's origin without the transform's translation", pins.pin_batch_projection),
+    20: ("the batch projection's y column scaled by the x step", pins.pin_batch_projection),
 }
 
 # planted in the composite's numpy restatement (oracle.COMPOSITE_MUTANT = k)

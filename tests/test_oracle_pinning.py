@@ -239,6 +239,42 @@ def test_point_transform_is_fused_in_source_order():
     assert differ > 200
 
 
+def pin_batch_projection():
+    """The draw passes' projection of an undisplaced point: decodePoint
+    (Common.metal:73-117) and the file transform folded per batch into
+    A[c][r] = T[c][r] * step_c (rounded once), B = T (min, 1) (transform_point,
+    pinned above), clip_r = fma(A[2][r], qz, fma(A[1][r], qy, fma(A[0][r], qx,
+    B_r))) with each fma rounded once (exact rationals); and, where every
+    product and sum is exact (small integer T, power-of-two steps, lattice
+    minimum), it equals T (min + q * step, 1) exactly."""
+    rng = np.random.default_rng(0xAFF1)
+    for _ in range(600):
+        T = (rng.uniform(0.5, 2.0, 16) * rng.choice([-1.0, 1.0], 16)).astype(np.float32)
+        mn = rng.uniform(-3000.0, 3000.0, 3).astype(np.float32)
+        bits = int(rng.choice([10, 20, 30]))
+        step = (rng.uniform(1.0, 500.0, 3).astype(np.float32) * np.float32(2.0 ** -bits)).astype(np.float32)
+        qf = np.float32(rng.integers(0, 1 << bits, 3))
+        got = orc.project_q(T, mn, step, qf)
+        B = orc.transform_point(T, mn)
+        for r in range(4):
+            a = B[r]
+            for c in range(3):
+                a = _fma(np.float32(T[c * 4 + r] * step[c]), qf[c], a)
+            assert got[r] == a, (T, mn, step, qf, r)
+    for _ in range(300):
+        T = rng.integers(-8, 9, 16).astype(np.float32)
+        mn = (rng.integers(-64, 65, 3) / 4.0).astype(np.float32)
+        step = np.float32(2.0) ** rng.integers(-10, -2, 3).astype(np.float32)
+        qf = np.float32(rng.integers(0, 1024, 3))
+        p = mn.astype(np.float64) + qf.astype(np.float64) * step.astype(np.float64)
+        want = T.astype(np.float64).reshape(4, 4).T @ np.append(p, 1.0)
+        assert np.array_equal(orc.project_q(T, mn, step, qf).astype(np.float64), want), (T, mn, step, qf)
+
+
+def test_batch_projection_is_the_fused_affine_map():
+    pin_batch_projection()
+
+
 def test_decode_point_is_fused_and_the_swift_reference_is_not():
     # decodePoint of the passes: fma(float(q), size * 2^-bits, min); the
     # NearestPointCPUReference restatement rounds the product first

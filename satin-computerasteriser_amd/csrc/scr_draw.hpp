@@ -139,11 +139,22 @@ __device__ __forceinline__ float pixel_size_on_screen(const scr_raster_batch& b,
     return sqrtf(dx * dx + dy * dy);
 }
 
+// The decode step per axis of a batch at a drawn precision level
+// (Common.metal:73-117): max(size, 1e-6) * 2^-bits, 30 bits at level 0, 20 at
+// level 1, 10 below.
+__device__ __forceinline__ void decode_steps(const scr_raster_batch& b, int level, float step[3]) {
+    const float inv = level == 0 ? (1.0f / 1073741824.0f) : (level == 1 ? (1.0f / 1048576.0f) : (1.0f / 1024.0f));
+    step[0] = fmaxf(b.maxX - b.minX, 0.000001f) * inv;
+    step[1] = fmaxf(b.maxY - b.minY, 0.000001f) * inv;
+    step[2] = fmaxf(b.maxZ - b.minZ, 0.000001f) * inv;
+}
+
 // The decision of cullUpdate (Cull/Shaders.metal:19-83) for slot `gid`;
-// true when the batch is drawn, with its VisibleBatch record in `vb` and what
-// its box proves about its points (batch_bounds) in `bb`.
+// true when the batch is drawn, with its VisibleBatch record in `vb` and its
+// draw record (what its box proves about its points, batch_bounds, and its
+// draw context) in `dr`.
 __device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p, int gid, scr_visible_batch& vb,
-                                          BatchBounds& bb) {
+                                          DrawRecord& dr) {
     const scr_raster_batch b = c.batches[gid];
     if (b.state == 0) return false;  // residency gate, Cull/Shaders.metal:34-36
     if ((int)b.fileIndex >= c.nFiles) return false;
@@ -173,12 +184,19 @@ __device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p
     vb.activePoints = active;
     if (p.applyDisplacement != 0) {
         // displaced points leave the box: nothing about them follows from it
-        bb = BatchBounds{};
-        bb.rconst = p.constRadius;
-        bb.zmax = __builtin_inff();
+        dr.bb = BatchBounds{};
+        dr.bb.rconst = p.constRadius;
+        dr.bb.zmax = __builtin_inff();
     } else {
-        bb = batch_bounds(f->transform, b, p.width, p.height, batch_const_radius(p, f->world, b));
+        dr.bb = batch_bounds(f->transform, b, p.width, p.height, batch_const_radius(p, f->world, b));
     }
+    dr.first = b.firstPoint;
+    dr.fileIndex = b.fileIndex;
+    dr.mn[0] = b.minX;
+    dr.mn[1] = b.minY;
+    dr.mn[2] = b.minZ;
+    decode_steps(b, level, dr.step);
+    dr.aff = batch_affine(f->transform, dr.step[0], dr.step[1], dr.step[2], b.minX, b.minY, b.minZ);
     return true;
 }
 
@@ -198,17 +216,20 @@ struct BatchCtx {
     float zmax;
     uint32_t first, active, batchIndex;   // active: end of the drawn range
     BatchAffine aff;    // file.transform folded with the batch decode (project_q)
-    float T[16];        // file.transform (held in registers: the atomics below
-                        // would otherwise force reloads through memory)
+    float T[16];        // file.transform, displaced points only (held in
+                        // registers: the atomics below would otherwise force
+                        // reloads through memory)
     const float* W;     // file.world (only read for per-point sizes)
 };
 
 // `hi` bounds the local point indices drawn (at most the active prefix).
-// `bbp` points at the batch's bounds from the cull pass.
+// `drp` points at the batch's draw record from the cull pass: the context
+// comes from it and the visible record in independent loads (the batch
+// record and its file are not read; the file transform only for displaced
+// points).
 __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
-                                              const BatchBounds* bbp, uint32_t hi) {
-    const scr_raster_batch b = gload_rec(c.batches + vb.batchIndex);
-    const BatchBounds bb = gload_rec(bbp);
+                                              const DrawRecord* drp, uint32_t hi) {
+    const DrawRecord dr = gload_rec(drp);
     BatchCtx bc;
     bc.level = vb.level;
     bc.lodThreshold = vb.lodThreshold;
@@ -216,19 +237,21 @@ __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParam
     bc.keepAll = ((vb.lodThreshold - 7.0f) + 0.5f) > 1.0f;
     bc.active = min(vb.activePoints, hi);
     bc.batchIndex = vb.batchIndex;
-    bc.first = b.firstPoint;
-    const float inv = vb.level == 0 ? (1.0f / 1073741824.0f) : (vb.level == 1 ? (1.0f / 1048576.0f) : (1.0f / 1024.0f));
-    bc.mnx = b.minX;
-    bc.mny = b.minY;
-    bc.mnz = b.minZ;
-    bc.sx = fmaxf(b.maxX - b.minX, 0.000001f) * inv;
-    bc.sy = fmaxf(b.maxY - b.minY, 0.000001f) * inv;
-    bc.sz = fmaxf(b.maxZ - b.minZ, 0.000001f) * inv;
-    const scr_raster_file* f = c.files + b.fileIndex;
+    bc.first = dr.first;
+    bc.mnx = dr.mn[0];
+    bc.mny = dr.mn[1];
+    bc.mnz = dr.mn[2];
+    bc.sx = dr.step[0];
+    bc.sy = dr.step[1];
+    bc.sz = dr.step[2];
+    bc.aff = dr.aff;
+    const scr_raster_file* f = c.files + dr.fileIndex;
+    if (p.applyDisplacement != 0) {
 #pragma unroll
-    for (int i = 0; i < 16; i++) bc.T[i] = gload(f->transform + i);
-    bc.aff = batch_affine(bc.T, bc.sx, bc.sy, bc.sz, bc.mnx, bc.mny, bc.mnz);
+        for (int i = 0; i < 16; i++) bc.T[i] = gload(f->transform + i);
+    }
     bc.W = f->world;
+    const BatchBounds& bb = dr.bb;
     bc.rconst = bb.rconst;
     bc.fastRcp = (bb.flags & kBoundsFastRcp) != 0u;
     bc.proven = (bb.flags & kBoundsProven) != 0u;
@@ -1212,7 +1235,7 @@ __device__ __forceinline__ void nearest_chunks(const CloudDev& c, const PassPara
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile, bool kDisp = false>
 __device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
-                                            const BatchBounds* bb, uint32_t lo, uint32_t hi, uint32_t* __restrict__ depth,
+                                            const DrawRecord* bb, uint32_t lo, uint32_t hi, uint32_t* __restrict__ depth,
                                             uint32_t* lds, int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
     if (!kR0 && bc.rconst == 0) depth_chunks<true, kWave, NT, PPT, kTile, kDisp>(c, p, bc, lo, depth, lds, red);
@@ -1221,7 +1244,7 @@ __device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams&
 
 template <bool kR0, bool kWave, int NT, int PPT, bool kDisp = false>
 __device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
-                                            const BatchBounds* bb, uint32_t lo, uint32_t hi,
+                                            const DrawRecord* bb, uint32_t lo, uint32_t hi,
                                             const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                             uint32_t* blue, uint32_t* count, ColorLds L) {
     const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
@@ -1231,7 +1254,7 @@ __device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams&
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kNearestTile>
 __device__ __forceinline__ void nearest_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
-                                              const BatchBounds* bb, uint32_t lo, uint32_t hi,
+                                              const DrawRecord* bb, uint32_t lo, uint32_t hi,
                                               unsigned long long* __restrict__ packed, unsigned long long* lds,
                                               int* red) {
     const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);

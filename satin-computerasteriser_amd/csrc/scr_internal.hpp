@@ -48,7 +48,7 @@ struct CloudDev {
     const float4* disp;            // displacement (xyz), nullptr: zero stand-in
     const float4* prevDisp;        // previous frame's displacement (motion blur), nullptr: zero stand-in
     scr_visible_batch* visible;    // cull output
-    scr::BatchBounds* bounds;      // cull output, one per visible entry (what its box proves)
+    scr::DrawRecord* bounds;       // cull output, one per visible entry (what its box proves, its draw context)
     uint32_t* counter;             // visible count
     int nSlots;
     int nFiles;
@@ -87,7 +87,7 @@ struct scr_cloud {
     // camera drew (ComputeRasteriserPointCloud.swift:96-103), owned
     std::map<uint64_t, float4*> prevDisp;
     scr_visible_batch* dVisible = nullptr;
-    scr::BatchBounds* dBounds = nullptr;
+    scr::DrawRecord* dBounds = nullptr;
     uint32_t* dCounter = nullptr;
 
     size_t poolPoints() const { return (size_t)maxResidentBatches * (size_t)pointsPerBatch; }

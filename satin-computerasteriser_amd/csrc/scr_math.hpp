@@ -390,6 +390,19 @@ constexpr uint32_t kBoundsProven = 1u;   // the rectangle and zmax hold
 constexpr uint32_t kBoundsInside = 2u;   // no point fails the w, NDC or viewport tests (nearest's ndc z > 0 included)
 constexpr uint32_t kBoundsFastRcp = 4u;  // every point's clip w lies in [2^-125, 2^125] (rcp_fast is exact)
 
+// What the cull pass leaves for the draws of one visible batch: the bounds
+// its box proves and everything the draws would otherwise fetch through the
+// batch record and its file (a chain of dependent loads per batch): the
+// first point, the file, the batch minimum, the decode step at the drawn
+// precision and the batch projection.  Eight 16-byte words.
+struct DrawRecord {
+    BatchBounds bb;
+    uint32_t first, fileIndex;
+    float mn[3], step[3];
+    BatchAffine aff;
+};
+static_assert(sizeof(DrawRecord) == 128, "eight 16-byte words");
+
 // One corner through m in double: clip value per row (the magnitude bound A
 // per row comes from the whole box, batch_magnitudes).
 struct CornerTerms {

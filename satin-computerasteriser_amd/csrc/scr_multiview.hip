@@ -72,13 +72,19 @@ __global__ __launch_bounds__(256) void k_mv_cull(const MVView* __restrict__ view
     MVItem it;
     it.view = blockIdx.y;
     it.pad[0] = it.pad[1] = it.pad[2] = 0u;
-    if (!cull_slot(V.c, V.p, slot, it.vb, it.bounds)) {
-        it.vb.batchIndex = (uint32_t)slot;
-        it.vb.level = 0;
-        it.vb.lodThreshold = 0.0f;
-        it.vb.activePoints = 0u;
+    MVItem* dst = items + (V.itemBase + (uint32_t)slot * V.itemStride);
+    if (cull_slot(V.c, V.p, slot, it.vb, it.bounds)) {
+        *dst = it;
+        return;
     }
-    items[V.itemBase + (uint32_t)slot * V.itemStride] = it;
+    // culled: the header only (the draws skip the item without reading its record)
+    it.vb.batchIndex = (uint32_t)slot;
+    it.vb.level = 0;
+    it.vb.lodThreshold = 0.0f;
+    it.vb.activePoints = 0u;
+    dst->view = it.view;
+    dst->pad[0] = dst->pad[1] = dst->pad[2] = 0u;
+    dst->vb = it.vb;
 }
 
 __global__ __launch_bounds__(256) void k_mv_resolve(const MVView* __restrict__ views) {
@@ -105,7 +111,7 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_color(const MVView* __
                                                                 const float4* __restrict__ params, uint32_t* heads) {
     __shared__ DiffLds L;
     drain<kDiffThreads, kDiffPPT>(views, items, nUnits, 1, heads + 3,
-                                  [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo,
+                                  [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo,
                                       uint32_t hi) {
                                       const BatchCtx bc = batch_ctx(V.c, V.p, vb, bb, hi);
                                       const size_t s = (size_t)V.stride;

@@ -31,10 +31,10 @@ struct MVView {
 struct MVItem {
     uint32_t view;
     uint32_t pad[3];
-    scr_visible_batch vb;           // activePoints == 0: culled
-    BatchBounds bounds;             // what the batch box proves under this view (cull)
+    scr_visible_batch vb;           // activePoints == 0: culled (the record is then not written)
+    DrawRecord bounds;              // what the batch box proves under this view, its draw context (cull)
 };
-static_assert(sizeof(MVItem) == 64, "MVItem is four 16-byte words");
+static_assert(sizeof(MVItem) == 160, "MVItem is ten 16-byte words");
 
 constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
 
@@ -54,15 +54,17 @@ __device__ inline void drain(const MVView* __restrict__ views, const MVItem* __r
         __syncthreads();
         if (u >= nUnits) break;
         const uint32_t item = u / (uint32_t)splits, part = u - item * (uint32_t)splits;
-        const MVItem it = items[item];
-        const uint32_t active = it.vb.activePoints;
+        // the header only: a culled item's record is not read (nor written)
+        const uint32_t view = items[item].view;
+        const scr_visible_batch vb = items[item].vb;
+        const uint32_t active = vb.activePoints;
         if (active == 0u) continue;
         const uint32_t chunks = (active + kChunk - 1) / kChunk;
         const uint32_t per = (chunks + (uint32_t)splits - 1) / (uint32_t)splits;
         const uint32_t lo = part * per * kChunk;
         if (lo >= active) continue;
         const uint32_t hi = min(active, lo + per * kChunk);
-        body(views[__builtin_amdgcn_readfirstlane(it.view)], it.vb, &items[item].bounds, lo, hi);
+        body(views[__builtin_amdgcn_readfirstlane(view)], vb, &items[item].bounds, lo, hi);
     }
 }
 
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* 
     __shared__ uint32_t lds[kMVDepthTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
-                   [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo, uint32_t hi) {
+                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) {
                        depth_batch<kR0, kWave, NT, PPT, kMVDepthTile>(V.c, V.p, vb, bb, lo, hi, V.planes, lds, red);
                    });
 }
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(c
     __shared__ unsigned long long ldsBC[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
-                   [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo, uint32_t hi) {
+                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) {
                        const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
                        const size_t s = (size_t)V.stride;
                        color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, bb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView
     __shared__ unsigned long long lds[kMVNearestTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
-                   [&](const MVView& V, const scr_visible_batch& vb, const BatchBounds* bb, uint32_t lo, uint32_t hi) {
+                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) {
                        nearest_batch<kR0, kWave, NT, PPT, kMVNearestTile>(V.c, V.p, vb, bb, lo, hi, V.nearest, lds, red);
                    });
 }

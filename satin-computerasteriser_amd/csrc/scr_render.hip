@@ -27,11 +27,11 @@ __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= c.nSlots) return;
     scr_visible_batch vb;
-    BatchBounds bb;
-    if (!cull_slot(c, p, gid, vb, bb)) return;
+    DrawRecord dr;
+    if (!cull_slot(c, p, gid, vb, dr)) return;
     const uint32_t slot = atomicAdd(c.counter, 1u);
     c.visible[slot] = vb;
-    c.bounds[slot] = bb;
+    c.bounds[slot] = dr;
 }
 
 // The draw kernels are persistent grids: workgroup b takes visible batches

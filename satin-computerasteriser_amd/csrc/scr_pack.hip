@@ -368,26 +368,28 @@ __global__ __launch_bounds__(256) void k_digit_starts(const uint32_t* __restrict
 }
 
 // Shape of the look-back pass: NT threads of KPT keys each, NT * KPT keys per
-// tile, and the number of lower tiles whose words one look-back step reads at
-// once (measured on 100M-point packs: 30-bit keys walk one tile per step,
-// 63-bit keys four).  The tile is staged in tile-sorted order in LDS; 64-bit
-// keys and their payload are staged one after the other in the same words
-// (two rounds), which keeps the workgroup at 42 KB of LDS.
+// tile, the number of lower tiles whose words one look-back step reads at
+// once, and the waves per SIMD the kernel is compiled for.  The tile is
+// staged in tile-sorted order in LDS; 64-bit keys and their payload are
+// staged one after the other in the same words (two rounds).  Measured on
+// 100M-point packs: 8,192-key tiles (each tile walks back over about half as
+// many lower tiles as with 4,096 keys); 30-bit keys with 512 threads (72 KB of
+// LDS, two workgroups per CU), 63-bit keys with 1,024 (82 KB, one per CU).
 template <typename K>
 struct SweepShape;
 template <>
 struct SweepShape<uint32_t> {
-    static constexpr int kNT = 512, kKPT = 8, kWindow = 1;
+    static constexpr int kNT = 512, kKPT = 16, kWindow = 4, kWaves = 4;
 };
 template <>
 struct SweepShape<uint64_t> {
-    static constexpr int kNT = 512, kKPT = 8, kWindow = 4;
+    static constexpr int kNT = 1024, kKPT = 8, kWindow = 4, kWaves = 4;
 };
 template <typename K>
 constexpr int sweep_tile() { return SweepShape<K>::kNT * SweepShape<K>::kKPT; }
 
-template <typename K, int NT, int KPT, int kWindow>
-__global__ __launch_bounds__(NT, 1024 / NT) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+template <typename K, int NT, int KPT, int kWindow, int kWavesPerSimd>
+__global__ __launch_bounds__(NT, kWavesPerSimd) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                  K* __restrict__ kout, uint32_t* __restrict__ vout, size_t n,
                                                  int shift, const uint32_t* __restrict__ dstart,
                                                  uint64_t* __restrict__ status, uint32_t* __restrict__ tileCounter,
@@ -869,9 +871,10 @@ struct SortScratch {
 
 // Inputs of at least this many points sort with the one-kernel look-back
 // pass; smaller ones with the split histogram / scan / scatter pass
-// (measured faster there).
+// (measured faster there: 30-bit keys below 2^24 points, 63-bit keys below
+// 2^20).
 template <typename K>
-constexpr size_t lookback_min_points() { return size_t(1) << 24; }
+constexpr size_t lookback_min_points() { return sizeof(K) == 8 ? size_t(1) << 20 : size_t(1) << 24; }
 
 template <typename K>
 static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*& vout, size_t n,
@@ -882,7 +885,8 @@ static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*
     const uint32_t numTiles = (uint32_t)((n + kTile - 1) / kTile);
     const size_t scanN = (size_t)256 * numTiles;
     const uint32_t nseg = (uint32_t)((scanN + kScanSeg - 1) / kScanSeg);
-    constexpr int kSweepNT = SweepShape<K>::kNT, kSweepKPT = SweepShape<K>::kKPT, kSweepWin = SweepShape<K>::kWindow;
+    constexpr int kSweepNT = SweepShape<K>::kNT, kSweepKPT = SweepShape<K>::kKPT, kSweepWin = SweepShape<K>::kWindow,
+                  kSweepWaves = SweepShape<K>::kWaves;
     const uint32_t sweepTiles = (uint32_t)((n + sweep_tile<K>() - 1) / sweep_tile<K>());
     hipError_t e = launch_fill_u32(s, w.ghist, P * 256, 0u);
     if (e == hipSuccess && lookback) e = launch_fill_u32(s, w.counters, 16, 0u);
@@ -904,7 +908,7 @@ static int radix_sort(scr_ctx* ctx, K*& kin, uint32_t*& vin, K*& kout, uint32_t*
         if (lookback) {
             e = launch_fill_u32(s, (uint32_t*)w.status, (size_t)sweepTiles * 256 * 2, 0u);
             if (e != hipSuccess) return hip_error(ctx, e, "radix look-back clear");
-            hipLaunchKernelGGL((k_onesweep<K, kSweepNT, kSweepKPT, kSweepWin>), dim3(sweepTiles), dim3(kSweepNT), 0, s, kin, vin,
+            hipLaunchKernelGGL((k_onesweep<K, kSweepNT, kSweepKPT, kSweepWin, kSweepWaves>), dim3(sweepTiles), dim3(kSweepNT), 0, s, kin, vin,
                                kout, vout, n, shift, w.dstart + p * 256, w.status, w.counters + p, w.counters + 8);
         } else {
             hipLaunchKernelGGL(k_tile_hist<K>, dim3(numTiles), dim3(kThreads), 0, s, kin, n, shift, numTiles, w.thist);

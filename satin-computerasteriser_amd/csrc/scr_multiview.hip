@@ -39,13 +39,24 @@ using namespace draw;
 using namespace mv;
 
 
-enum ClearMode { kClearHQS = 0, kClearNearest = 1, kClearDiff = 2 };
+// kClearHQSDepth / kClearHQSSums: the depth plane (and the queue heads) and
+// the four sum planes of an HQS render separately, so that the sums are
+// cleared on a side stream while the cull and depth passes run.
+enum ClearMode { kClearHQS = 0, kClearNearest = 1, kClearDiff = 2, kClearHQSDepth = 3, kClearHQSSums = 4 };
 
 __global__ __launch_bounds__(256) void k_mv_clear(const MVView* __restrict__ views, int mode,
                                                   uint32_t* __restrict__ heads) {
     const MVView& V = views[blockIdx.y];
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < kQueues) heads[threadIdx.x] = 0u;
+    if (mode != kClearHQSSums && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < kQueues) heads[threadIdx.x] = 0u;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (mode == kClearHQSDepth || mode == kClearHQSSums) {
+        // planes are `stride` words apart, stride a multiple of 64: uint4 stores
+        const size_t n = (mode == kClearHQSDepth ? 1 : 4) * (size_t)V.stride;
+        uint32_t* base = V.planes + (mode == kClearHQSDepth ? 0 : (size_t)V.stride);
+        const size_t j = i * 4;
+        if (j < n) *reinterpret_cast<uint4*>(base + j) = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
     if (mode == kClearNearest) {
         const size_t n = (size_t)V.p.width * (size_t)V.p.height;
         const size_t j = i * 2;
@@ -265,6 +276,10 @@ struct scr_multiview {
     int stageNext = 0;
     // per-kernel persistent grid (resident workgroups) cache
     std::vector<std::pair<scr::DrawKernel, int>> residency;
+    // side stream (default priority) for the sum planes' clear, and the events
+    // that fork it from and join it to the context stream
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     // captured launch sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -303,6 +318,11 @@ void release(scr_multiview* mv) {
         mv->hStage[k] = nullptr;
         mv->staged[k] = nullptr;
     }
+    if (mv->fork) (void)hipEventDestroy(mv->fork);
+    if (mv->join) (void)hipEventDestroy(mv->join);
+    if (mv->side) (void)hipStreamDestroy(mv->side);
+    mv->fork = mv->join = nullptr;
+    mv->side = nullptr;
     mv->dParams = nullptr;
     mv->dItems = nullptr;
     mv->dHeads = nullptr;
@@ -340,7 +360,24 @@ int enqueue(scr_multiview* mv, int mode, bool r0, uint32_t nItems, int timedPhas
     int maxSlots = 0;
     for (scr_cloud* c : mv->clouds) maxSlots = std::max(maxSlots, c->maxResidentBatches);
     const unsigned nv = (unsigned)mv->nViews;
-    {
+    // HQS: the sum planes are cleared on the side stream, overlapping the
+    // cull and depth passes (joined before the colour pass); with pass
+    // profiling on, in order on the context stream
+    const bool overlap = !nearest && !ctx->profiling;
+    bool joined = !overlap;
+    if (overlap) {
+        const size_t stride = mv->maxWords / 5;
+        scr::PassScope ps(ctx, "mv_clear");
+        hipLaunchKernelGGL(scr::k_mv_clear, dim3(std::max(1u, scr::grid_for(stride, 256 * 4u)), nv), dim3(256), 0, s,
+                           views, (int)scr::kClearHQSDepth, mv->dHeads);
+        SCR_HIP(ctx, hipGetLastError());
+        SCR_HIP(ctx, hipEventRecord(mv->fork, s));
+        SCR_HIP(ctx, hipStreamWaitEvent(mv->side, mv->fork, 0));
+        hipLaunchKernelGGL(scr::k_mv_clear, dim3(std::max(1u, scr::grid_for(4 * stride, 256 * 4u)), nv), dim3(256), 0,
+                           mv->side, views, (int)scr::kClearHQSSums, mv->dHeads);
+        SCR_HIP(ctx, hipGetLastError());
+        SCR_HIP(ctx, hipEventRecord(mv->join, mv->side));
+    } else {
         scr::PassScope ps(ctx, "mv_clear");
         const size_t perThread = nearest ? 2 : 4;
         const size_t n = nearest ? mv->maxPixels : mv->maxWords;
@@ -368,6 +405,10 @@ int enqueue(scr_multiview* mv, int mode, bool r0, uint32_t nItems, int timedPhas
         unsigned grid = 1;
         int rc = resident_grid(mv, kern, var->threads, nUnits, &grid);
         if (rc != SCR_OK) return rc;
+        if (ph == scr::kColor && !joined) {
+            SCR_HIP(ctx, hipStreamWaitEvent(s, mv->join, 0));
+            joined = true;
+        }
         if (timedPhase == (int)ph) SCR_HIP(ctx, hipEventRecord(t0, s));
         {
             scr::PassScope ps(ctx, ph == scr::kDepth ? "mv_depth" : (ph == scr::kColor ? "mv_color" : "mv_nearest"));
@@ -377,6 +418,7 @@ int enqueue(scr_multiview* mv, int mode, bool r0, uint32_t nItems, int timedPhas
         }
         if (timedPhase == (int)ph) SCR_HIP(ctx, hipEventRecord(t1, s));
     }
+    if (!joined) SCR_HIP(ctx, hipStreamWaitEvent(s, mv->join, 0));
     {
         scr::PassScope ps(ctx, nearest ? "mv_nearest_resolve" : "mv_resolve");
         if (nearest)
@@ -571,6 +613,9 @@ int scr_multiview_create(scr_ctx* ctx, scr_cloud* const* clouds, int nClouds, sc
         e = hipHostMalloc((void**)&mv->hStage[k], mv->paramsBytes);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&mv->staged[k], hipEventDisableTiming);
     }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&mv->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&mv->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&mv->join, hipEventDisableTiming);
     if (e != hipSuccess) {
         release(mv);
         delete mv;

@@ -1024,6 +1024,41 @@ int scr_frame_composite(scr_frame* frame, const scr_composite_options* opt, uint
 
 // ------------------------------------------------------------------ render
 
+namespace {
+
+// The frame's clear of `words` u32 at `planes`; when the clouds' composed
+// file tables fit the launch (at most kSetupClouds clouds and kSetupFiles
+// files), the same launch writes them (in place of upload_files) and zeroes
+// the clouds' visible counters (in place of the cull's reset), and *inlined
+// is set.
+int frame_setup(scr_ctx* ctx, uint32_t* planes, size_t words, scr_cloud* const* clouds, int nClouds,
+                const float* vp, const float* prevVP, const char* pass, bool* inlined) {
+    scr::FrameSetup su{};
+    su.planes = planes;
+    su.words = words;
+    size_t files = 0;
+    for (int i = 0; i < nClouds; i++) files += clouds[i]->files.size();
+    *inlined = nClouds <= scr::kSetupClouds && files <= (size_t)scr::kSetupFiles;
+    if (*inlined) {
+        int nf = 0;
+        for (int i = 0; i < nClouds; i++) {
+            scr_cloud* c = clouds[i];
+            su.counter[i] = c->dCounter;
+            su.dst[i] = c->dFiles;
+            su.first[i] = nf;
+            su.count[i] = (int)c->files.size();
+            scr::compose_files(c, c->model, vp, nullptr, prevVP, su.files + nf);
+            nf += su.count[i];
+        }
+        su.nClouds = nClouds;
+    }
+    scr::PassScope ps(ctx, pass);
+    SCR_HIP(ctx, scr::launch_frame_setup(ctx->stream, su));
+    return SCR_OK;
+}
+
+}  // namespace
+
 int scr_render(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam, scr_cloud* const* clouds, int nClouds,
                scr_frame* frame) {
     return scr_render_camera(ctx, cfg, cam, 0, clouds, nClouds, frame);
@@ -1054,18 +1089,21 @@ int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
         scr_cloud* c = nullptr;
         for (int i = 0; i < nClouds && !c; i++)
             if (clouds[i]->residentBatchCount > 0) c = clouds[i];
-        {
-            scr::PassScope ps(ctx, "clear_winner");
-            SCR_HIP(ctx, scr::launch_fill_u64(s, frame->dNearest, frame->pixels(), 0ull));
-        }
+        scr_cloud* const one[1] = {c};
+        bool inlined = false;
+        rc = frame_setup(ctx, reinterpret_cast<uint32_t*>(frame->dNearest), 2 * frame->pixels(), one, c ? 1 : 0, vp,
+                         prevVP, "clear_winner", &inlined);
+        if (rc != SCR_OK) return rc;
         const uint32_t* colors = nullptr;
         if (c) {
-            rc = upload_files(c, c->model, vp, nullptr, prevVP);
-            if (rc != SCR_OK) return rc;
+            if (!inlined) {
+                rc = upload_files(c, c->model, vp, nullptr, prevVP);
+                if (rc != SCR_OK) return rc;
+            }
             const CloudDev d = c->dev();
             {
                 scr::PassScope ps(ctx, "cull");
-                SCR_HIP(ctx, scr::launch_cull(s, d, p));
+                SCR_HIP(ctx, scr::launch_cull(s, d, p, !inlined));
             }
             {
                 scr::PassScope ps(ctx, "nearest");
@@ -1082,16 +1120,18 @@ int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
     }
 
     // encodeHighQualityAverage (:322-456)
-    {
-        scr::PassScope ps(ctx, "clear");
-        SCR_HIP(ctx, scr::launch_fill_u32(s, frame->dPlanes, 5 * frame->planeStride(), 0u));
-    }
     std::vector<scr_cloud*> resident;
     for (int i = 0; i < nClouds; i++)
         if (clouds[i]->residentBatchCount > 0) resident.push_back(clouds[i]);
-    for (scr_cloud* c : resident) {
-        rc = upload_files(c, c->model, vp, nullptr, prevVP);
-        if (rc != SCR_OK) return rc;
+    bool inlined = false;
+    rc = frame_setup(ctx, frame->dPlanes, 5 * frame->planeStride(), resident.data(), (int)resident.size(), vp, prevVP,
+                     "clear", &inlined);
+    if (rc != SCR_OK) return rc;
+    if (!inlined) {
+        for (scr_cloud* c : resident) {
+            rc = upload_files(c, c->model, vp, nullptr, prevVP);
+            if (rc != SCR_OK) return rc;
+        }
     }
     // each cloud's device view; the previous displacements of this camera
     // (the current ones until a motion-blurred render of it has kept a copy,
@@ -1105,7 +1145,7 @@ int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
     }
     for (const CloudDev& d : devs) {
         scr::PassScope ps(ctx, "cull");
-        SCR_HIP(ctx, scr::launch_cull(s, d, p));
+        SCR_HIP(ctx, scr::launch_cull(s, d, p, !inlined));
     }
     // translucent defocus and motion blur take the weighted-blend passes
     // (ComputeRasteriser.swift:233-246)

@@ -164,7 +164,25 @@ void compose_files(const scr_cloud* c, const float* model, const float* vp, cons
 // Kernel launchers (scr_render.hip).  All enqueue on ctx->stream.
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
-hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p);
+// resetCounter false: the visible counter was zeroed by the frame set-up.
+hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p, bool resetCounter = true);
+// A frame's set-up in one launch: zeroes `words` u32 of the accumulation
+// planes (16-byte aligned) and, for each of nClouds clouds, zeroes its
+// visible counter and writes its composed file table, which the launch
+// carries in its arguments (files[first .. first + count) to dst).
+constexpr int kSetupClouds = 4;
+constexpr int kSetupFiles = 8;
+struct FrameSetup {
+    uint32_t* planes = nullptr;
+    size_t words = 0;
+    int nClouds = 0;
+    uint32_t* counter[kSetupClouds] = {};
+    scr_raster_file* dst[kSetupClouds] = {};
+    int first[kSetupClouds] = {};
+    int count[kSetupClouds] = {};
+    scr_raster_file files[kSetupFiles];
+};
+hipError_t launch_frame_setup(hipStream_t s, const FrameSetup& su);
 // Translucent-defocus / motion-blur settings of the coverage colour pass.
 struct CoverageParams {
     int coverageMode;   // applyTint && tintAlphaIsCoverage

@@ -8,10 +8,15 @@
 //                     atomic max of depth<<32 | ~pointIndex
 //                                                       NearestDepth :14-91, NearestIndex :14-94
 //   k_nearest_resolve nearestResolveUpdate              NearestResolve/Shaders.metal:8-38
+//   k_frame_setup     the frame's clear (accumulation or winner planes), the
+//                     visible counters' reset and the composed file tables
+//                     (updateFiles) in one launch
 //
 // HIP has no indirect dispatch: the draw kernels are persistent grids sized to
 // the machine; each workgroup walks the visible list the cull kernel wrote
 // (count read on the device), so no host round trip separates cull and draw.
+#include <algorithm>
+
 #include "scr_draw.hpp"
 
 namespace scr {
@@ -102,6 +107,27 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, size
     }
 }
 
+// launch_frame_setup: four 16-byte stores per thread (256-thread blocks
+// cover 4,096 words each); block 0 also writes the tail words, the counters
+// and the file tables.
+__global__ __launch_bounds__(256) void k_frame_setup(FrameSetup su) {
+    const size_t nv = su.words / 4;
+    uint4* p4 = reinterpret_cast<uint4*>(su.planes);
+    const size_t b = (size_t)blockIdx.x * 1024 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (b + (size_t)k * 256 < nv) p4[b + (size_t)k * 256] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x < (unsigned)(su.words & 3)) su.planes[nv * 4 + threadIdx.x] = 0u;
+    constexpr int kWords = (int)(sizeof(scr_raster_file) / 4);
+    for (int c = 0; c < su.nClouds; c++) {
+        if (threadIdx.x == 0) *su.counter[c] = 0u;
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(su.files + su.first[c]);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(su.dst[c]);
+        for (int i = (int)threadIdx.x; i < su.count[c] * kWords; i += (int)blockDim.x) dst[i] = src[i];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* __restrict__ p, size_t n,
                                                   unsigned long long v) {
     const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
@@ -135,6 +161,20 @@ unsigned draw_grid(int numCUs, int nSlots) {
 }
 
 }  // namespace
+
+hipError_t launch_frame_setup(hipStream_t s, const FrameSetup& su) {
+    static_assert(sizeof(FrameSetup) <= 4096, "the set-up travels in the kernel arguments");
+    if (su.nClouds < 0 || su.nClouds > kSetupClouds) return hipErrorInvalidValue;
+    int files = 0;
+    for (int c = 0; c < su.nClouds; c++) {
+        if (su.first[c] < 0 || su.count[c] < 0 || su.first[c] + su.count[c] > kSetupFiles) return hipErrorInvalidValue;
+        files += su.count[c];
+    }
+    if (files > kSetupFiles || (((uintptr_t)su.planes) & 15u) != 0) return hipErrorInvalidValue;
+    const unsigned grid = std::max(1u, grid_for(su.words / 4, 1024));
+    hipLaunchKernelGGL(k_frame_setup, dim3(grid), dim3(256), 0, s, su);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
     if (n == 0) return hipSuccess;
@@ -564,9 +604,11 @@ hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p) {
-    hipError_t e = launch_fill_u32(s, c.counter, 1, 0u);
-    if (e != hipSuccess) return e;
+hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p, bool resetCounter) {
+    if (resetCounter) {
+        hipError_t e = launch_fill_u32(s, c.counter, 1, 0u);
+        if (e != hipSuccess) return e;
+    }
     if (c.nSlots <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cull, dim3(grid_for((size_t)c.nSlots, 256)), dim3(256), 0, s, c, p);
     return hipGetLastError();

@@ -486,12 +486,12 @@ int scr_cloud_create(scr_ctx* ctx, int maxResidentBatches, int pointsPerBatch, c
     (void)hipSetDevice(ctx->device);
     hipError_t e = alloc_pool(c);
     if (e == hipSuccess) e = dalloc(c->dFiles, c->files.size());
-    if (e == hipSuccess) e = dalloc(c->dCounter, 1);
+    if (e == hipSuccess) e = dalloc(c->dCounter, 2);
     if (e == hipSuccess)
         e = hipHostMalloc((void**)&c->hFiles, scr_cloud::kFilesRing * c->files.size() * sizeof(scr_raster_file));
     for (int k = 0; k < scr_cloud::kFilesRing && e == hipSuccess; k++)
         e = hipEventCreateWithFlags(&c->filesCopied[k], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipMemsetAsync(c->dCounter, 0, sizeof(uint32_t), ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->dCounter, 0, 2 * sizeof(uint32_t), ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess)
         e = hipMemcpy(c->dFiles, c->files.data(), c->files.size() * sizeof(scr_raster_file), hipMemcpyHostToDevice);
@@ -739,7 +739,7 @@ int scr_cloud_read_visible(scr_cloud* c, scr_visible_batch* host, int capacity, 
     if (!c || !count) return SCR_ERR_INVALID_ARGUMENT;
     uint32_t n = 0;
     SCR_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
-    SCR_HIP(c->ctx, hipMemcpy(&n, c->dCounter, 4, hipMemcpyDeviceToHost));
+    SCR_HIP(c->ctx, hipMemcpy(&n, c->dCounter + c->counterWord, 4, hipMemcpyDeviceToHost));
     *count = (int)n;
     const int k = std::min((int)n, capacity);
     if (k > 0 && host) SCR_HIP(c->ctx, hipMemcpy(host, c->dVisible, (size_t)k * sizeof(scr_visible_batch), hipMemcpyDeviceToHost));
@@ -1026,34 +1026,42 @@ int scr_frame_composite(scr_frame* frame, const scr_composite_options* opt, uint
 
 namespace {
 
-// The frame's clear of `words` u32 at `planes`; when the clouds' composed
-// file tables fit the launch (at most kSetupClouds clouds and kSetupFiles
-// files), the same launch writes them (in place of upload_files) and zeroes
-// the clouds' visible counters (in place of the cull's reset), and *inlined
-// is set.
-int frame_setup(scr_ctx* ctx, uint32_t* planes, size_t words, scr_cloud* const* clouds, int nClouds,
-                const float* vp, const float* prevVP, const char* pass, bool* inlined) {
+// The frame's front: the clear of `words` u32 at `planes` and, when the
+// clouds' composed file tables fit the launch (at most kSetupClouds clouds
+// and kSetupFiles files), their tables (in place of upload_files) and their
+// culls in the same launch; *fused tells which.
+int frame_front(scr_ctx* ctx, uint32_t* planes, size_t words, scr_cloud* const* clouds, int nClouds,
+                const float* vp, const float* prevVP, const scr::PassParams& p, const char* clearPass, bool* fused) {
     scr::FrameSetup su{};
+    scr::CullSet cs{};
     su.planes = planes;
     su.words = words;
     size_t files = 0;
     for (int i = 0; i < nClouds; i++) files += clouds[i]->files.size();
-    *inlined = nClouds <= scr::kSetupClouds && files <= (size_t)scr::kSetupFiles;
-    if (*inlined) {
-        int nf = 0;
+    *fused = nClouds > 0 && nClouds <= scr::kSetupClouds && files <= (size_t)scr::kSetupFiles;
+    if (*fused) {
+        int nf = 0, blocks = 0;
         for (int i = 0; i < nClouds; i++) {
             scr_cloud* c = clouds[i];
-            su.counter[i] = c->dCounter;
+            CloudDev d = c->dev();
+            su.zero[i] = d.counter;                           // the word the draws read until now
+            d.counter = c->dCounter + (c->counterWord ^ 1);   // left zero by the previous front (or never used)
+            cs.c[i] = d;
+            cs.blockStart[i] = blocks;
+            blocks += (c->maxResidentBatches + 255) / 256;
             su.dst[i] = c->dFiles;
             su.first[i] = nf;
             su.count[i] = (int)c->files.size();
             scr::compose_files(c, c->model, vp, nullptr, prevVP, su.files + nf);
             nf += su.count[i];
         }
-        su.nClouds = nClouds;
+        cs.blockStart[nClouds] = blocks;
+        su.nClouds = cs.nClouds = nClouds;
     }
-    scr::PassScope ps(ctx, pass);
-    SCR_HIP(ctx, scr::launch_frame_setup(ctx->stream, su));
+    scr::PassScope ps(ctx, *fused ? "clear_cull" : clearPass);
+    SCR_HIP(ctx, scr::launch_frame_front(ctx->stream, su, cs, p));
+    if (*fused)
+        for (int i = 0; i < nClouds; i++) clouds[i]->counterWord ^= 1;
     return SCR_OK;
 }
 
@@ -1090,20 +1098,18 @@ int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
         for (int i = 0; i < nClouds && !c; i++)
             if (clouds[i]->residentBatchCount > 0) c = clouds[i];
         scr_cloud* const one[1] = {c};
-        bool inlined = false;
-        rc = frame_setup(ctx, reinterpret_cast<uint32_t*>(frame->dNearest), 2 * frame->pixels(), one, c ? 1 : 0, vp,
-                         prevVP, "clear_winner", &inlined);
+        bool fused = false;
+        rc = frame_front(ctx, reinterpret_cast<uint32_t*>(frame->dNearest), 2 * frame->pixels(), one, c ? 1 : 0, vp,
+                         prevVP, p, "clear_winner", &fused);
         if (rc != SCR_OK) return rc;
         const uint32_t* colors = nullptr;
         if (c) {
-            if (!inlined) {
+            const CloudDev d = c->dev();
+            if (!fused) {
                 rc = upload_files(c, c->model, vp, nullptr, prevVP);
                 if (rc != SCR_OK) return rc;
-            }
-            const CloudDev d = c->dev();
-            {
                 scr::PassScope ps(ctx, "cull");
-                SCR_HIP(ctx, scr::launch_cull(s, d, p, !inlined));
+                SCR_HIP(ctx, scr::launch_cull(s, d, p));
             }
             {
                 scr::PassScope ps(ctx, "nearest");
@@ -1123,11 +1129,11 @@ int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
     std::vector<scr_cloud*> resident;
     for (int i = 0; i < nClouds; i++)
         if (clouds[i]->residentBatchCount > 0) resident.push_back(clouds[i]);
-    bool inlined = false;
-    rc = frame_setup(ctx, frame->dPlanes, 5 * frame->planeStride(), resident.data(), (int)resident.size(), vp, prevVP,
-                     "clear", &inlined);
+    bool fused = false;
+    rc = frame_front(ctx, frame->dPlanes, 5 * frame->planeStride(), resident.data(), (int)resident.size(), vp, prevVP,
+                     p, "clear", &fused);
     if (rc != SCR_OK) return rc;
-    if (!inlined) {
+    if (!fused) {
         for (scr_cloud* c : resident) {
             rc = upload_files(c, c->model, vp, nullptr, prevVP);
             if (rc != SCR_OK) return rc;
@@ -1143,9 +1149,11 @@ int scr_render_camera(scr_ctx* ctx, const scr_config* cfg, const scr_camera* cam
         if (it != c->prevDisp.end()) d.prevDisp = it->second;
         devs.push_back(d);
     }
-    for (const CloudDev& d : devs) {
-        scr::PassScope ps(ctx, "cull");
-        SCR_HIP(ctx, scr::launch_cull(s, d, p, !inlined));
+    if (!fused) {
+        for (const CloudDev& d : devs) {
+            scr::PassScope ps(ctx, "cull");
+            SCR_HIP(ctx, scr::launch_cull(s, d, p));
+        }
     }
     // translucent defocus and motion blur take the weighted-blend passes
     // (ComputeRasteriser.swift:233-246)

@@ -88,7 +88,11 @@ struct scr_cloud {
     std::map<uint64_t, float4*> prevDisp;
     scr_visible_batch* dVisible = nullptr;
     scr::DrawRecord* dBounds = nullptr;
+    // two visible-count words: the draws read dCounter[counterWord]; a cull
+    // fused into the frame's front counts into the other word (left zero)
+    // and zeroes this one for the next
     uint32_t* dCounter = nullptr;
+    int counterWord = 0;
 
     size_t poolPoints() const { return (size_t)maxResidentBatches * (size_t)pointsPerBatch; }
     CloudDev dev() const {
@@ -105,7 +109,7 @@ struct scr_cloud {
         c.prevDisp = dDisp;
         c.visible = dVisible;
         c.bounds = dBounds;
-        c.counter = dCounter;
+        c.counter = dCounter + counterWord;
         c.nSlots = maxResidentBatches;
         c.nFiles = (int)files.size();
         return c;
@@ -164,25 +168,32 @@ void compose_files(const scr_cloud* c, const float* model, const float* vp, cons
 // Kernel launchers (scr_render.hip).  All enqueue on ctx->stream.
 hipError_t launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 hipError_t launch_fill_u64(hipStream_t s, unsigned long long* p, size_t n, unsigned long long v);
-// resetCounter false: the visible counter was zeroed by the frame set-up.
-hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p, bool resetCounter = true);
-// A frame's set-up in one launch: zeroes `words` u32 of the accumulation
-// planes (16-byte aligned) and, for each of nClouds clouds, zeroes its
-// visible counter and writes its composed file table, which the launch
-// carries in its arguments (files[first .. first + count) to dst).
+hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p);
+// A frame's front in one launch: the clouds' culls and the frame's set-up.
+// The set-up zeroes `words` u32 of the accumulation planes (16-byte aligned)
+// and, for each of nClouds clouds, zeroes the counter word `zero` (the one
+// the cloud's next such cull counts into) and writes its composed file table,
+// which the launch carries in its arguments (files[first .. first + count) to
+// dst).  Cloud k's cull counts into cull.c[k].counter and reads its file
+// table from the arguments, in blocks [blockStart[k], blockStart[k + 1]).
 constexpr int kSetupClouds = 4;
 constexpr int kSetupFiles = 8;
 struct FrameSetup {
     uint32_t* planes = nullptr;
     size_t words = 0;
     int nClouds = 0;
-    uint32_t* counter[kSetupClouds] = {};
+    uint32_t* zero[kSetupClouds] = {};
     scr_raster_file* dst[kSetupClouds] = {};
     int first[kSetupClouds] = {};
     int count[kSetupClouds] = {};
     scr_raster_file files[kSetupFiles];
 };
-hipError_t launch_frame_setup(hipStream_t s, const FrameSetup& su);
+struct CullSet {
+    int nClouds = 0;
+    int blockStart[kSetupClouds + 1] = {};
+    CloudDev c[kSetupClouds] = {};
+};
+hipError_t launch_frame_front(hipStream_t s, const FrameSetup& su, const CullSet& cs, const PassParams& p);
 // Translucent-defocus / motion-blur settings of the coverage colour pass.
 struct CoverageParams {
     int coverageMode;   // applyTint && tintAlphaIsCoverage

@@ -8,9 +8,9 @@
 //                     atomic max of depth<<32 | ~pointIndex
 //                                                       NearestDepth :14-91, NearestIndex :14-94
 //   k_nearest_resolve nearestResolveUpdate              NearestResolve/Shaders.metal:8-38
-//   k_frame_setup     the frame's clear (accumulation or winner planes), the
-//                     visible counters' reset and the composed file tables
-//                     (updateFiles) in one launch
+//   k_frame_front     the frame's clear (accumulation or winner planes), the
+//                     composed file tables (updateFiles) and the clouds'
+//                     culls in one launch
 //
 // HIP has no indirect dispatch: the draw kernels are persistent grids sized to
 // the machine; each workgroup walks the visible list the cull kernel wrote
@@ -28,8 +28,7 @@ constexpr int kThreads = 256;   // draw workgroup
 constexpr int kPPT = 8;         // points per thread
 constexpr bool kWaveTiles = false;  // chunks and LDS tiles shared by the workgroup (measured faster than per-wave tiles)
 
-__global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cull_thread(const CloudDev& c, const PassParams& p, int gid) {
     if (gid >= c.nSlots) return;
     scr_visible_batch vb;
     DrawRecord dr;
@@ -37,6 +36,10 @@ __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
     const uint32_t slot = atomicAdd(c.counter, 1u);
     c.visible[slot] = vb;
     c.bounds[slot] = dr;
+}
+
+__global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
+    cull_thread(c, p, (int)(blockIdx.x * blockDim.x + threadIdx.x));
 }
 
 // The draw kernels are persistent grids: workgroup b takes visible batches
@@ -107,21 +110,32 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, size
     }
 }
 
-// launch_frame_setup: four 16-byte stores per thread (256-thread blocks
-// cover 4,096 words each); block 0 also writes the tail words, the counters
-// and the file tables.
-__global__ __launch_bounds__(256) void k_frame_setup(FrameSetup su) {
+// launch_frame_front: the cull blocks first (their chains of dependent loads
+// and double arithmetic start while the clear streams); the clear blocks store
+// four 16-byte words per thread (4,096 words per block), and the first of them
+// also writes the tail words, the counter words and the file tables.
+__global__ __launch_bounds__(256) void k_frame_front(FrameSetup su, CullSet cs, PassParams p) {
+    const int nCull = cs.blockStart[cs.nClouds];
+    if ((int)blockIdx.x < nCull) {
+        int k = 0;
+        while (k + 1 < cs.nClouds && (int)blockIdx.x >= cs.blockStart[k + 1]) k++;
+        CloudDev c = cs.c[k];
+        c.files = su.files + su.first[k];
+        cull_thread(c, p, ((int)blockIdx.x - cs.blockStart[k]) * 256 + (int)threadIdx.x);
+        return;
+    }
+    const size_t blk = blockIdx.x - (unsigned)nCull;
     const size_t nv = su.words / 4;
     uint4* p4 = reinterpret_cast<uint4*>(su.planes);
-    const size_t b = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    const size_t b = blk * 1024 + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < 4; k++)
         if (b + (size_t)k * 256 < nv) p4[b + (size_t)k * 256] = make_uint4(0u, 0u, 0u, 0u);
-    if (blockIdx.x != 0) return;
+    if (blk != 0) return;
     if (threadIdx.x < (unsigned)(su.words & 3)) su.planes[nv * 4 + threadIdx.x] = 0u;
     constexpr int kWords = (int)(sizeof(scr_raster_file) / 4);
     for (int c = 0; c < su.nClouds; c++) {
-        if (threadIdx.x == 0) *su.counter[c] = 0u;
+        if (threadIdx.x == 0) *su.zero[c] = 0u;
         const uint32_t* src = reinterpret_cast<const uint32_t*>(su.files + su.first[c]);
         uint32_t* dst = reinterpret_cast<uint32_t*>(su.dst[c]);
         for (int i = (int)threadIdx.x; i < su.count[c] * kWords; i += (int)blockDim.x) dst[i] = src[i];
@@ -162,17 +176,25 @@ unsigned draw_grid(int numCUs, int nSlots) {
 
 }  // namespace
 
-hipError_t launch_frame_setup(hipStream_t s, const FrameSetup& su) {
-    static_assert(sizeof(FrameSetup) <= 4096, "the set-up travels in the kernel arguments");
-    if (su.nClouds < 0 || su.nClouds > kSetupClouds) return hipErrorInvalidValue;
+hipError_t launch_frame_front(hipStream_t s, const FrameSetup& su, const CullSet& cs, const PassParams& p) {
+    static_assert(sizeof(FrameSetup) + sizeof(CullSet) + sizeof(PassParams) <= 4096,
+                  "the front travels in the kernel arguments");
+    if (su.nClouds < 0 || su.nClouds > kSetupClouds || cs.nClouds < 0 || cs.nClouds > su.nClouds)
+        return hipErrorInvalidValue;
     int files = 0;
     for (int c = 0; c < su.nClouds; c++) {
         if (su.first[c] < 0 || su.count[c] < 0 || su.first[c] + su.count[c] > kSetupFiles) return hipErrorInvalidValue;
         files += su.count[c];
     }
     if (files > kSetupFiles || (((uintptr_t)su.planes) & 15u) != 0) return hipErrorInvalidValue;
-    const unsigned grid = std::max(1u, grid_for(su.words / 4, 1024));
-    hipLaunchKernelGGL(k_frame_setup, dim3(grid), dim3(256), 0, s, su);
+    // cloud k's blocks cover its slots, and its table is the one it reads
+    if (cs.blockStart[0] != 0) return hipErrorInvalidValue;
+    for (int c = 0; c < cs.nClouds; c++)
+        if (cs.blockStart[c + 1] - cs.blockStart[c] != (int)grid_for((size_t)std::max(cs.c[c].nSlots, 0), 256) ||
+            cs.c[c].nFiles != su.count[c])
+            return hipErrorInvalidValue;
+    const unsigned grid = (unsigned)cs.blockStart[cs.nClouds] + std::max(1u, grid_for(su.words / 4, 1024));
+    hipLaunchKernelGGL(k_frame_front, dim3(grid), dim3(256), 0, s, su, cs, p);
     return hipGetLastError();
 }
 
@@ -604,11 +626,9 @@ hipError_t launch_atomic_rate(hipStream_t s, int numCUs, uint32_t* buf, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p, bool resetCounter) {
-    if (resetCounter) {
-        hipError_t e = launch_fill_u32(s, c.counter, 1, 0u);
-        if (e != hipSuccess) return e;
-    }
+hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p) {
+    hipError_t e = launch_fill_u32(s, c.counter, 1, 0u);
+    if (e != hipSuccess) return e;
     if (c.nSlots <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_cull, dim3(grid_for((size_t)c.nSlots, 256)), dim3(256), 0, s, c, p);
     return hipGetLastError();

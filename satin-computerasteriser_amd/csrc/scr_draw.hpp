@@ -13,6 +13,8 @@
 // same planes.
 #pragma once
 
+#include <type_traits>
+
 #include "scr_internal.hpp"
 
 namespace scr {
@@ -364,15 +366,31 @@ __device__ __forceinline__ void chunk_tests(const PassParams& p, const BatchCtx&
     }
 }
 
+// What a draw loop knows of its batch before the chunks, chosen once per
+// batch so that no per-chunk branch merges the points' flags (the compiler
+// packs merged flag arrays into bytes and unpacks them again, several VALU
+// instructions per point).  kFixRuntime: the LOD/dither and inside tests are
+// chosen per chunk.  Otherwise bits: kFixKeep, bc.keepAll (no LOD/dither
+// test; without it the batch needs the test); kFixInside, bc.inside for
+// single-pixel undisplaced points (the clip tests are skipped; without it
+// they run).
+constexpr int kFixRuntime = -1;
+constexpr int kFixKeep = 1;
+constexpr int kFixInside = 2;
+
 // Runs the front end of every point pass on the fetched points: dither/LOD
 // skip, decodePoint (Common.metal:73-117) and projection (project_q for
 // undisplaced points), displacement (kDisp: position + displacement, a NaN
 // component drops the point, then the transform; DepthPass/Shaders.metal:
 // 69-74, ColorPass/Shaders.metal:72-78), clip and viewport tests, y flip,
 // footprint radius.
-template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kUseInside = true, bool kDisp = false>
+template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kUseInside = true, bool kDisp = false,
+          int kFix = kFixRuntime>
 __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                     const ChunkRaw<PPT>& r, ChunkFrags<PPT>& f) {
+    constexpr bool kRuntime = kFix == kFixRuntime;
+    static_assert(kRuntime || !(kFix & kFixInside) || (kUseInside && kR0 && !kDisp),
+                  "inside needs single-pixel undisplaced points");
     const uint32_t base = bc.first + li0;
     const uint32_t n = li0 < bc.active ? min((uint32_t)PPT, bc.active - li0) : 0u;
     // Branch-free per point: every test of the reference folds into `ok`
@@ -384,12 +402,13 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
     bool keep[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) keep[j] = (uint32_t)j < n;
-    if (!bc.keepAll) {
+    if (kRuntime ? !bc.keepAll : !(kFix & kFixKeep)) {
         // LOD / dither test: a point is kept when its dither value lies below
         // t = (lodThreshold - level) + 0.5.  Dither values lie in [0, 1], so
         // a wave whose points all have t > 1 keeps them all without hashing
         // (one uniform branch per chunk; the active prefix ends at the
-        // threshold's level, so most waves hold only lower levels).
+        // threshold's level, so most waves hold only lower levels).  The
+        // dropped points leave the branch as one word of bits.
         float t[PPT];
         bool hash = false;
 #pragma unroll
@@ -397,13 +416,16 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
             t[j] = (bc.lodThreshold - (float)((r.lv4[j / 4] >> (8 * (j % 4))) & 7u)) + 0.5f;
             hash = hash || !(t[j] > 1.0f);
         }
+        uint32_t drop = 0u;
         if (__ballot(hash) != 0ull) {
 #pragma unroll
             for (int j = 0; j < PPT; j++) {
                 const float dither = (p.lodDither != 0) ? hash_unit(base + (uint32_t)j) : 0.5f;
-                keep[j] = keep[j] && !(dither >= t[j]);
+                drop |= (dither >= t[j]) ? 1u << j : 0u;
             }
         }
+#pragma unroll
+        for (int j = 0; j < PPT; j++) keep[j] = keep[j] && ((drop >> j) & 1u) == 0u;
     }
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
@@ -457,18 +479,45 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
     }
     // a batch whose box proves that every point passes the clip-w, NDC and
     // viewport tests (kBoundsInside) skips them: one uniform branch per chunk
-    if (kUseInside && !kDisp && kR0 && bc.inside)
+    if (!kRuntime) {
+        if constexpr (kFix > 0 && (kFix & kFixInside) != 0)
+            chunk_tests<kNearestZ, kR0, true, PPT>(p, bc, clip, rw, keep, dx, dy, dz, wx, wy, wz, f);
+        else
+            chunk_tests<kNearestZ, kR0, false, PPT>(p, bc, clip, rw, keep, dx, dy, dz, wx, wy, wz, f);
+    } else if (kUseInside && !kDisp && kR0 && bc.inside) {
         chunk_tests<kNearestZ, kR0, true, PPT>(p, bc, clip, rw, keep, dx, dy, dz, wx, wy, wz, f);
-    else
+    } else {
         chunk_tests<kNearestZ, kR0, false, PPT>(p, bc, clip, rw, keep, dx, dy, dz, wx, wy, wz, f);
+    }
 }
 
-template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kDisp = false>
+template <bool kNearestZ, bool kR0, int kLevel, int PPT, bool kDisp = false, int kFix = kFixRuntime>
 __device__ __forceinline__ void chunk_front(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t li0,
                                    bool vec, ChunkFrags<PPT>& f) {
     ChunkRaw<PPT> r;
     chunk_fetch<kLevel, PPT>(c, bc, li0, vec, r);
-    chunk_decode<kNearestZ, kR0, kLevel, PPT, true, kDisp>(c, p, bc, li0, r, f);
+    chunk_decode<kNearestZ, kR0, kLevel, PPT, true, kDisp, kFix>(c, p, bc, li0, r, f);
+}
+
+// The fix a batch's draw loop takes (single-pixel undisplaced points may skip
+// the clip tests; kUseInside false: never), and the loop run with it as a
+// compile-time constant: loop(std::integral_constant<int, fix>).
+template <bool kR0, bool kDisp, bool kUseInside = true>
+__device__ __forceinline__ int front_fix(const BatchCtx& bc) {
+    return (bc.keepAll ? kFixKeep : 0) | ((kUseInside && kR0 && !kDisp && bc.inside) ? kFixInside : 0);
+}
+template <bool kInsideOk, typename Loop>
+__device__ __forceinline__ void with_fix(int fix, const Loop& loop) {
+    if (fix & kFixInside) {
+        if constexpr (kInsideOk) {
+            if (fix & kFixKeep) loop(std::integral_constant<int, kFixKeep | kFixInside>{});
+            else loop(std::integral_constant<int, kFixInside>{});
+        }
+    } else if (fix & kFixKeep) {
+        loop(std::integral_constant<int, kFixKeep>{});
+    } else {
+        loop(std::integral_constant<int, 0>{});
+    }
 }
 
 // ---------------------------------------------------------------- chunk rectangle
@@ -728,34 +777,41 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
         G::sync();
         uint32_t* spare = ldsAll + kTile + __lane_id();
-        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
-            ChunkFrags<PPT> f;
-            chunk_front<false, kR0, kLevel, PPT, kDisp>(c, p, bc, cb + G::rank() * PPT, vec, f);
-            if (kR0) depth_frags_tile<PPT>(f, seg, p, lds, spare, depth);
-            else depth_frags<kR0, PPT>(f, seg, p, lds, depth);
-        }
+        const auto loop = [&](auto fix) {
+            for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+                ChunkFrags<PPT> f;
+                chunk_front<false, kR0, kLevel, PPT, kDisp, decltype(fix)::value>(c, p, bc, cb + G::rank() * PPT, vec,
+                                                                                  f);
+                if (kR0) depth_frags_tile<PPT>(f, seg, p, lds, spare, depth);
+                else depth_frags<kR0, PPT>(f, seg, p, lds, depth);
+            }
+        };
+        with_fix<kR0 && !kDisp>(front_fix<kR0, kDisp>(bc), loop);
         G::sync();
         depth_flush<G>(seg, p, lds, depth);
         G::sync();
         return;
     }
-    for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
-        ChunkFrags<PPT> f;
-        chunk_front<false, kR0, kLevel, PPT, kDisp>(c, p, bc, cb + G::rank() * PPT, vec, f);
-        const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
-        if (box.frags == 0) continue;
-        if (box.lds) {
-            const int area = box.w * box.h;
-            for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
-            G::sync();
+    const auto loop = [&](auto fix) {
+        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+            ChunkFrags<PPT> f;
+            chunk_front<false, kR0, kLevel, PPT, kDisp, decltype(fix)::value>(c, p, bc, cb + G::rank() * PPT, vec, f);
+            const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
+            if (box.frags == 0) continue;
+            if (box.lds) {
+                const int area = box.w * box.h;
+                for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
+                G::sync();
+            }
+            depth_frags<kR0, PPT>(f, box, p, lds, depth);
+            if (box.lds) {
+                G::sync();
+                depth_flush<G>(box, p, lds, depth);
+                G::sync();
+            }
         }
-        depth_frags<kR0, PPT>(f, box, p, lds, depth);
-        if (box.lds) {
-            G::sync();
-            depth_flush<G>(box, p, lds, depth);
-            G::sync();
-        }
-    }
+    };
+    with_fix<kR0 && !kDisp>(front_fix<kR0, kDisp>(bc), loop);
 }
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kDepthTile, bool kDisp = false>
@@ -1039,18 +1095,21 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
         // every point of the batch lies behind every accepted depth of its
         // rectangle: no fragment can pass the test, nothing to add
         if (tmin > seg.zmax) return;
-        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
-            const uint32_t li0 = cb + G::rank() * PPT;
-            ChunkRaw<PPT> raw;
-            chunk_fetch<kLevel, PPT>(c, bc, li0, vec, raw);
-            uint32_t rgb[PPT];
-            color_fetch<PPT>(c, bc, li0, vec, rgb);
-            ChunkFrags<PPT> f;
-            chunk_decode<false, kR0, kLevel, PPT, kColorInside, kDisp>(c, p, bc, li0, raw, f);
-            chunk_colors<PPT>(c, p, bc, li0, f, rgb);
-            if (kR0) color_frags_tile<PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
-            else color_frags<kR0, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
-        }
+        const auto loop = [&](auto fix) {
+            for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+                const uint32_t li0 = cb + G::rank() * PPT;
+                ChunkRaw<PPT> raw;
+                chunk_fetch<kLevel, PPT>(c, bc, li0, vec, raw);
+                uint32_t rgb[PPT];
+                color_fetch<PPT>(c, bc, li0, vec, rgb);
+                ChunkFrags<PPT> f;
+                chunk_decode<false, kR0, kLevel, PPT, kColorInside, kDisp, decltype(fix)::value>(c, p, bc, li0, raw, f);
+                chunk_colors<PPT>(c, p, bc, li0, f, rgb);
+                if (kR0) color_frags_tile<PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
+                else color_frags<kR0, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
+            }
+        };
+        with_fix<kColorInside && kR0 && !kDisp>(front_fix<kR0, kDisp, kColorInside>(bc), loop);
         G::sync();
         color_tile_flush<G>(seg, p, L, red, green, blue, count);
         G::sync();
@@ -1185,36 +1244,42 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
         G::sync();
         unsigned long long* spare = ldsAll + kTile + __lane_id();
-        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
-            const uint32_t li0 = cb + G::rank() * PPT;
-            ChunkFrags<PPT> f;
-            chunk_front<true, kR0, kLevel, PPT>(c, p, bc, li0, vec, f);
-            if (kR0) nearest_frags_tile<PPT>(f, bc.first + li0, seg, p, lds, spare, packed);
-            else nearest_frags<kR0, PPT>(f, bc.first + li0, seg, p, lds, packed);
-        }
+        const auto loop = [&](auto fix) {
+            for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+                const uint32_t li0 = cb + G::rank() * PPT;
+                ChunkFrags<PPT> f;
+                chunk_front<true, kR0, kLevel, PPT, false, decltype(fix)::value>(c, p, bc, li0, vec, f);
+                if (kR0) nearest_frags_tile<PPT>(f, bc.first + li0, seg, p, lds, spare, packed);
+                else nearest_frags<kR0, PPT>(f, bc.first + li0, seg, p, lds, packed);
+            }
+        };
+        with_fix<kR0>(front_fix<kR0, false>(bc), loop);
         G::sync();
         nearest_flush<G>(seg, p, lds, packed);
         G::sync();
         return;
     }
-    for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
-        const uint32_t li0 = cb + G::rank() * PPT;
-        ChunkFrags<PPT> f;
-        chunk_front<true, kR0, kLevel, PPT>(c, p, bc, li0, vec, f);
-        const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
-        if (box.frags == 0) continue;
-        if (box.lds) {
-            const int area = box.w * box.h;
-            for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
-            G::sync();
+    const auto loop = [&](auto fix) {
+        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+            const uint32_t li0 = cb + G::rank() * PPT;
+            ChunkFrags<PPT> f;
+            chunk_front<true, kR0, kLevel, PPT, false, decltype(fix)::value>(c, p, bc, li0, vec, f);
+            const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, red);
+            if (box.frags == 0) continue;
+            if (box.lds) {
+                const int area = box.w * box.h;
+                for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
+                G::sync();
+            }
+            nearest_frags<kR0, PPT>(f, bc.first + li0, box, p, lds, packed);
+            if (box.lds) {
+                G::sync();
+                nearest_flush<G>(box, p, lds, packed);
+                G::sync();
+            }
         }
-        nearest_frags<kR0, PPT>(f, bc.first + li0, box, p, lds, packed);
-        if (box.lds) {
-            G::sync();
-            nearest_flush<G>(box, p, lds, packed);
-            G::sync();
-        }
-    }
+    };
+    with_fix<kR0>(front_fix<kR0, false>(bc), loop);
 }
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kNearestTile>

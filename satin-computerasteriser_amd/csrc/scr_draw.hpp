@@ -535,8 +535,10 @@ struct ChunkBox {
     // every on-screen footprint pixel of the chunk lies inside the box (it is
     // the clamped union of the footprints), so no bounds test is needed
     // a box holds at most 2^24 pixels per side: one 24-bit multiply-add
+    // unsigned arithmetic: defined (and discarded by the callers) for the
+    // undefined pixel of a point without a fragment too
     __device__ inline int local(int tx, int ty) const {
-        return (int)__umul24((uint32_t)(ty - y0), (uint32_t)w) + (tx - x0);
+        return (int)(__umul24((uint32_t)ty - (uint32_t)y0, (uint32_t)w) + ((uint32_t)tx - (uint32_t)x0));
     }
     __device__ inline size_t pixel(int k, int width) const {
         const int row = w == 1 ? k : (int)__umulhi((uint32_t)k, magic);
@@ -995,13 +997,23 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
                                                  const PassParams& p, const uint32_t* __restrict__ depth,
                                                  uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count,
                                                  ColorLds L) {
+    // every fragment's tile slot (slot 0 for points without one) and its
+    // threshold, read before the first sum is added
+    int ks[PPT];
+    float thr[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) {
+        const int k = box.local(f.px[j], f.py[j]);
+        ks[j] = f.ok[j] ? k : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < PPT; j++) thr[j] = __uint_as_float(lds_load(L.depth + ks[j]));
     int runK = -1;
     unsigned long long runRG = 0ull, runBN = 0ull;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
-        const bool in = f.ok[j];
-        const int k = in ? box.local(f.px[j], f.py[j]) : 0;
-        const bool pass = in & (f.z[j] >= __uint_as_float(lds_load(L.depth + k)));
+        const int k = ks[j];
+        const bool pass = f.ok[j] & (f.z[j] >= thr[j]);
         if (!pass) continue;
         const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
         const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
@@ -1095,7 +1107,9 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
         // every point of the batch lies behind every accepted depth of its
         // rectangle: no fragment can pass the test, nothing to add
         if (tmin > seg.zmax) return;
-        const auto loop = [&](auto fix) {
+        // plain colours (no tint, no debug colours) are known for the whole
+        // batch: the fragments' flags then cross no colour-stage branch
+        const auto loop = [&](auto fix, auto plain) {
             for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
                 const uint32_t li0 = cb + G::rank() * PPT;
                 ChunkRaw<PPT> raw;
@@ -1104,12 +1118,22 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
                 color_fetch<PPT>(c, bc, li0, vec, rgb);
                 ChunkFrags<PPT> f;
                 chunk_decode<false, kR0, kLevel, PPT, kColorInside, kDisp, decltype(fix)::value>(c, p, bc, li0, raw, f);
-                chunk_colors<PPT>(c, p, bc, li0, f, rgb);
+                if constexpr (decltype(plain)::value) {
+#pragma unroll
+                    for (int j = 0; j < PPT; j++) rgb[j] &= 0x00ffffffu;
+                } else {
+                    chunk_colors<PPT>(c, p, bc, li0, f, rgb);
+                }
                 if (kR0) color_frags_tile<PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
                 else color_frags<kR0, PPT>(f, rgb, seg, p, depth, red, green, blue, count, L);
             }
         };
-        with_fix<kColorInside && kR0 && !kDisp>(front_fix<kR0, kDisp, kColorInside>(bc), loop);
+        if (p.colorizeChunks == 0 && p.colorizeOverdraw == 0 && p.applyTint == 0) {
+            with_fix<kColorInside && kR0 && !kDisp>(front_fix<kR0, kDisp, kColorInside>(bc),
+                                                    [&](auto fix) { loop(fix, std::true_type{}); });
+        } else {
+            loop(std::integral_constant<int, kFixRuntime>{}, std::false_type{});
+        }
         G::sync();
         color_tile_flush<G>(seg, p, L, red, green, blue, count);
         G::sync();

@@ -506,6 +506,9 @@ template <bool kR0, bool kDisp, bool kUseInside = true>
 __device__ __forceinline__ int front_fix(const BatchCtx& bc) {
     return (bc.keepAll ? kFixKeep : 0) | ((kUseInside && kR0 && !kDisp && bc.inside) ? kFixInside : 0);
 }
+// (The loops are lambdas marked always_inline: a draw body called from
+// several places is otherwise outlined into a function call with a stack
+// frame, measured up to twice as slow in the multi-view kernels.)
 template <bool kInsideOk, typename Loop>
 __device__ __forceinline__ void with_fix(int fix, const Loop& loop) {
     if (fix & kFixInside) {
@@ -779,7 +782,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
         G::sync();
         uint32_t* spare = ldsAll + kTile + __lane_id();
-        const auto loop = [&](auto fix) {
+        const auto loop = [&](auto fix) __attribute__((always_inline)) {
             for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
                 ChunkFrags<PPT> f;
                 chunk_front<false, kR0, kLevel, PPT, kDisp, decltype(fix)::value>(c, p, bc, cb + G::rank() * PPT, vec,
@@ -794,7 +797,7 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
         G::sync();
         return;
     }
-    const auto loop = [&](auto fix) {
+    const auto loop = [&](auto fix) __attribute__((always_inline)) {
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             ChunkFrags<PPT> f;
             chunk_front<false, kR0, kLevel, PPT, kDisp, decltype(fix)::value>(c, p, bc, cb + G::rank() * PPT, vec, f);
@@ -804,12 +807,14 @@ __device__ __forceinline__ void depth_chunks_level(const CloudDev& c, const Pass
                 const int area = box.w * box.h;
                 for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0u);
                 G::sync();
-            }
-            depth_frags<kR0, PPT>(f, box, p, lds, depth);
-            if (box.lds) {
+                // the chunk's rectangle holds every one of its fragments
+                if (kR0) depth_frags_tile<PPT>(f, box, p, lds, ldsAll + kTile + __lane_id(), depth);
+                else depth_frags<kR0, PPT>(f, box, p, lds, depth);
                 G::sync();
                 depth_flush<G>(box, p, lds, depth);
                 G::sync();
+            } else {
+                depth_frags<kR0, PPT>(f, box, p, lds, depth);
             }
         }
     };
@@ -1109,7 +1114,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
         if (tmin > seg.zmax) return;
         // plain colours (no tint, no debug colours) are known for the whole
         // batch: the fragments' flags then cross no colour-stage branch
-        const auto loop = [&](auto fix, auto plain) {
+        const auto loop = [&](auto fix, auto plain) __attribute__((always_inline)) {
             for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
                 const uint32_t li0 = cb + G::rank() * PPT;
                 ChunkRaw<PPT> raw;
@@ -1130,7 +1135,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
         };
         if (p.colorizeChunks == 0 && p.colorizeOverdraw == 0 && p.applyTint == 0) {
             with_fix<kColorInside && kR0 && !kDisp>(front_fix<kR0, kDisp, kColorInside>(bc),
-                                                    [&](auto fix) { loop(fix, std::true_type{}); });
+                                                    [&](auto fix) __attribute__((always_inline)) { loop(fix, std::true_type{}); });
         } else {
             loop(std::integral_constant<int, kFixRuntime>{}, std::false_type{});
         }
@@ -1139,27 +1144,42 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
         G::sync();
         return;
     }
-    for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
-        const uint32_t li0 = cb + G::rank() * PPT;
-        ChunkRaw<PPT> raw;
-        chunk_fetch<kLevel, PPT>(c, bc, li0, vec, raw);
-        uint32_t rgb[PPT];
-        color_fetch<PPT>(c, bc, li0, vec, rgb);
-        ChunkFrags<PPT> f;
-        chunk_decode<false, kR0, kLevel, PPT, kColorInside, kDisp>(c, p, bc, li0, raw, f);
-        chunk_colors<PPT>(c, p, bc, li0, f, rgb);
-        const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
-        if (box.frags == 0) continue;
-        if (box.lds) {
-            color_stage<G>(box, p, depth, L);
-            G::sync();
+    const auto loop = [&](auto fix, auto plain) __attribute__((always_inline)) {
+        for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
+            const uint32_t li0 = cb + G::rank() * PPT;
+            ChunkRaw<PPT> raw;
+            chunk_fetch<kLevel, PPT>(c, bc, li0, vec, raw);
+            uint32_t rgb[PPT];
+            color_fetch<PPT>(c, bc, li0, vec, rgb);
+            ChunkFrags<PPT> f;
+            chunk_decode<false, kR0, kLevel, PPT, kColorInside, kDisp, decltype(fix)::value>(c, p, bc, li0, raw, f);
+            if constexpr (decltype(plain)::value) {
+#pragma unroll
+                for (int j = 0; j < PPT; j++) rgb[j] &= 0x00ffffffu;
+            } else {
+                chunk_colors<PPT>(c, p, bc, li0, f, rgb);
+            }
+            const ChunkBox box = chunk_box<kWave, NT, PPT>(f, p, kCap, L.red);
+            if (box.frags == 0) continue;
+            if (box.lds) {
+                color_stage<G>(box, p, depth, L);
+                G::sync();
+                // the chunk's rectangle holds every one of its fragments
+                if (kR0) color_frags_tile<PPT>(f, rgb, box, p, depth, red, green, blue, count, L);
+                else color_frags<kR0, PPT>(f, rgb, box, p, depth, red, green, blue, count, L);
+                G::sync();
+                color_tile_flush<G>(box, p, L, red, green, blue, count);
+                G::sync();
+            } else {
+                color_frags<kR0, PPT>(f, rgb, box, p, depth, red, green, blue, count, L);
+            }
         }
-        color_frags<kR0, PPT>(f, rgb, box, p, depth, red, green, blue, count, L);
-        if (box.lds) {
-            G::sync();
-            color_tile_flush<G>(box, p, L, red, green, blue, count);
-            G::sync();
-        }
+    };
+    if (p.colorizeChunks == 0 && p.colorizeOverdraw == 0 && p.applyTint == 0) {
+        with_fix<kColorInside && kR0 && !kDisp>(front_fix<kR0, kDisp, kColorInside>(bc),
+                                                [&](auto fix) __attribute__((always_inline)) { loop(fix, std::true_type{}); });
+    } else {
+        loop(std::integral_constant<int, kFixRuntime>{}, std::false_type{});
     }
 }
 
@@ -1268,7 +1288,7 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
         for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
         G::sync();
         unsigned long long* spare = ldsAll + kTile + __lane_id();
-        const auto loop = [&](auto fix) {
+        const auto loop = [&](auto fix) __attribute__((always_inline)) {
             for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
                 const uint32_t li0 = cb + G::rank() * PPT;
                 ChunkFrags<PPT> f;
@@ -1283,7 +1303,7 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
         G::sync();
         return;
     }
-    const auto loop = [&](auto fix) {
+    const auto loop = [&](auto fix) __attribute__((always_inline)) {
         for (uint32_t cb = first; cb < bc.active; cb += NT * PPT) {
             const uint32_t li0 = cb + G::rank() * PPT;
             ChunkFrags<PPT> f;
@@ -1294,12 +1314,14 @@ __device__ __forceinline__ void nearest_chunks_level(const CloudDev& c, const Pa
                 const int area = box.w * box.h;
                 for (int i = G::rank(); i < area; i += G::kSize) lds_store(lds + i, 0ull);
                 G::sync();
-            }
-            nearest_frags<kR0, PPT>(f, bc.first + li0, box, p, lds, packed);
-            if (box.lds) {
+                // the chunk's rectangle holds every one of its fragments
+                if (kR0) nearest_frags_tile<PPT>(f, bc.first + li0, box, p, lds, ldsAll + kTile + __lane_id(), packed);
+                else nearest_frags<kR0, PPT>(f, bc.first + li0, box, p, lds, packed);
                 G::sync();
                 nearest_flush<G>(box, p, lds, packed);
                 G::sync();
+            } else {
+                nearest_frags<kR0, PPT>(f, bc.first + li0, box, p, lds, packed);
             }
         }
     };

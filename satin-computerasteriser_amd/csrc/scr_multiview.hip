@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_color(const MVView* __
     __shared__ DiffLds L;
     drain<kDiffThreads, kDiffPPT>(views, items, nUnits, 1, heads + 3,
                                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo,
-                                      uint32_t hi) {
+                                      uint32_t hi) __attribute__((always_inline)) {
                                       const BatchCtx bc = batch_ctx(V.c, V.p, vb, bb, hi);
                                       const size_t s = (size_t)V.stride;
                                       if (!kR0 && bc.rconst == 0)

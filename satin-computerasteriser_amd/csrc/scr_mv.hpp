@@ -38,12 +38,14 @@ static_assert(sizeof(MVItem) == 160, "MVItem is ten 16-byte words");
 
 constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
 
-// Takes work units from heads[queue] until the table is drained.  A unit is
+// Takes work units from heads[queue] until the table is drained.  The
+// bodies passed in are marked always_inline: a large body is otherwise called
+// as a function, with a stack frame, measured up to twice as slow.  A unit is
 // (item, part); part k of S draws the k-th of S equal, chunk-aligned pieces of
 // the item's active prefix.  Every workgroup leaves the loop once the counter
 // passes nUnits, so the grid drains.
 template <int NT, int PPT, typename Body>
-__device__ inline void drain(const MVView* __restrict__ views, const MVItem* __restrict__ items, uint32_t nUnits,
+__device__ __forceinline__ void drain(const MVView* __restrict__ views, const MVItem* __restrict__ items, uint32_t nUnits,
                              int splits, uint32_t* head, Body body) {
     __shared__ uint32_t sUnit;
     constexpr uint32_t kChunk = NT * PPT;
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* 
     __shared__ uint32_t lds[kMVDepthTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
-                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) {
+                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
                        depth_batch<kR0, kWave, NT, PPT, kMVDepthTile>(V.c, V.p, vb, bb, lo, hi, V.planes, lds, red);
                    });
 }
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(c
     __shared__ unsigned long long ldsBC[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
-                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) {
+                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
                        const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
                        const size_t s = (size_t)V.stride;
                        color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, bb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView
     __shared__ unsigned long long lds[kMVNearestTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
-                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) {
+                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
                        nearest_batch<kR0, kWave, NT, PPT, kMVNearestTile>(V.c, V.p, vb, bb, lo, hi, V.nearest, lds, red);
                    });
 }

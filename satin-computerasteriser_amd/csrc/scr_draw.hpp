@@ -210,6 +210,7 @@ struct BatchCtx {
     int level;
     float lodThreshold;
     bool keepAll;       // no point of the batch can fail the LOD/dither test
+    uint32_t hashFrom;  // the lowest level whose points take the dither test (8: none)
     int rconst;         // footprint radius shared by the batch's points, or -1
     bool fastRcp;       // every point's clip w lies in [2^-125, 2^125] (rcp_fast is exact)
     bool proven;        // every fragment centre lies in [bx0, bx1] x [by0, by1] (column, unflipped row), z <= zmax
@@ -235,8 +236,14 @@ __device__ __forceinline__ BatchCtx batch_ctx(const CloudDev& c, const PassParam
     BatchCtx bc;
     bc.level = vb.level;
     bc.lodThreshold = vb.lodThreshold;
-    // dither is in [0, 1] and (thr - L) + 0.5 decreases with L <= 7
+    // dither is in [0, 1] and t(L) = (thr - L) + 0.5 does not increase with
+    // L (each rounding is monotone): points of levels L with t(L) > 1 are kept
+    // without a dither value, and those are the levels below hashFrom
     bc.keepAll = ((vb.lodThreshold - 7.0f) + 0.5f) > 1.0f;
+    bc.hashFrom = 8u;
+#pragma unroll
+    for (int l = 7; l >= 0; l--)
+        if (!(((vb.lodThreshold - (float)l) + 0.5f) > 1.0f)) bc.hashFrom = (uint32_t)l;
     bc.active = min(vb.activePoints, hi);
     bc.batchIndex = vb.batchIndex;
     bc.first = dr.first;
@@ -409,19 +416,20 @@ __device__ __forceinline__ void chunk_decode(const CloudDev& c, const PassParams
         // (one uniform branch per chunk; the active prefix ends at the
         // threshold's level, so most waves hold only lower levels).  The
         // dropped points leave the branch as one word of bits.
-        float t[PPT];
-        bool hash = false;
+        // Whether any of the thread's levels (bytes & 7) reaches hashFrom:
+        // four at once, level + 8 - hashFrom sets bit 3 of its byte exactly
+        // when level >= hashFrom (no carry leaves a byte).
+        const uint32_t bias = (8u - bc.hashFrom) * 0x01010101u;
+        uint32_t reach = 0u;
 #pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            t[j] = (bc.lodThreshold - (float)((r.lv4[j / 4] >> (8 * (j % 4))) & 7u)) + 0.5f;
-            hash = hash || !(t[j] > 1.0f);
-        }
+        for (int q = 0; q < (PPT + 3) / 4; q++) reach |= ((r.lv4[q] & 0x07070707u) + bias) & 0x08080808u;
         uint32_t drop = 0u;
-        if (__ballot(hash) != 0ull) {
+        if (__ballot(reach != 0u) != 0ull) {
 #pragma unroll
             for (int j = 0; j < PPT; j++) {
+                const float t = (bc.lodThreshold - (float)((r.lv4[j / 4] >> (8 * (j % 4))) & 7u)) + 0.5f;
                 const float dither = (p.lodDither != 0) ? hash_unit(base + (uint32_t)j) : 0.5f;
-                drop |= (dither >= t[j]) ? 1u << j : 0u;
+                drop |= (dither >= t) ? 1u << j : 0u;
             }
         }
 #pragma unroll

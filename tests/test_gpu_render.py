@@ -235,6 +235,50 @@ def test_batches_hidden_behind_a_wall(ctx):
     assert_planes_equal(gpu, ref)
 
 
+def test_visible_count_words_across_render_paths(ctx, cube64):
+    """Each cloud keeps two visible-count words: a render whose front launch
+    holds the cull counts into the word the previous front left zero, the
+    other culls (pick, a render of more clouds than the front launch carries)
+    reset and count into the current one.  Any interleaving must leave the
+    last cull's visible list and the same planes."""
+    cfg = scr.ComputeRasteriserConfiguration()
+    cam = scr.Camera.perspective((0.3, 0.2, 1.6), (1.0, 0.5, -1.0), fov=50, aspect=1.5)  # partial view
+    oc = oracle_cloud(cube64)
+    ref_vis = orc.cull(oc, cfg, cam.viewMatrix, cam.projectionMatrix, 300, 200)
+    ref = orc.render_hqs([oc], cfg, cam.viewMatrix, cam.projectionMatrix, 300, 200)
+    assert 0 < len(ref_vis) < cube64.batchCount
+    clouds = [gpu_cloud(ctx, cube64) for _ in range(5)]
+    one = scr.ComputeRasteriser(ctx, cfg)
+    one.addPointCloud(clouds[0])
+    one.resize(300, 200)
+    many = scr.ComputeRasteriser(ctx, cfg)
+    for c in clouds:
+        many.addPointCloud(c)
+    many.resize(300, 200)
+
+    def visible(c):
+        return np.sort(c.readVisible(), order="batchIndex").tobytes()
+
+    def render_one():
+        one.render(cam)
+        assert visible(clouds[0]) == ref_vis.tobytes()
+        out = {k: one.read(v) for k, v in PLANES.items()}
+        out["depthF32"] = out["depthF32"].view(np.float32)
+        assert_planes_equal(out, ref)
+
+    render_one()
+    render_one()
+    one.pickPointIndex((0.0, 0.0), clouds[0], cam)
+    assert visible(clouds[0]) == ref_vis.tobytes()
+    render_one()
+    many.render(cam)
+    for c in clouds:
+        assert visible(c) == ref_vis.tobytes()
+    render_one()
+    many.render(cam)
+    render_one()
+
+
 def test_pick_round_trips_through_source_indices(ctx):
     """PickPointIndexTests.swift:49-114."""
     pos = [(-0.8, 0, 0), (0.0, 0, 0), (0.8, 0, 0)]

@@ -38,35 +38,41 @@ static_assert(sizeof(MVItem) == 160, "MVItem is ten 16-byte words");
 
 constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
 
-// Takes work units from heads[queue] until the table is drained.  The
-// bodies passed in are marked always_inline: a large body is otherwise called
-// as a function, with a stack frame, measured up to twice as slow.  A unit is
-// (item, part); part k of S draws the k-th of S equal, chunk-aligned pieces of
-// the item's active prefix.  Every workgroup leaves the loop once the counter
-// passes nUnits, so the grid drains.
+// Takes work units from heads[queue] until the table is drained, kClaim
+// consecutive units per claim (culled items, about half of the table, then
+// cost a header read instead of a claim's round trip and two barriers each).
+// The bodies passed in are marked always_inline: a large body is otherwise
+// called as a function, with a stack frame, measured up to twice as slow.  A
+// unit is (item, part); part k of S draws the k-th of S equal, chunk-aligned
+// pieces of the item's active prefix.  Every workgroup leaves the loop once
+// the counter passes nUnits, so the grid drains.
+constexpr uint32_t kClaim = 4;
 template <int NT, int PPT, typename Body>
 __device__ __forceinline__ void drain(const MVView* __restrict__ views, const MVItem* __restrict__ items, uint32_t nUnits,
                              int splits, uint32_t* head, Body body) {
     __shared__ uint32_t sUnit;
     constexpr uint32_t kChunk = NT * PPT;
     for (;;) {
-        if (threadIdx.x == 0) sUnit = atomicAdd(head, 1u);
+        if (threadIdx.x == 0) sUnit = atomicAdd(head, kClaim);
         __syncthreads();
-        const uint32_t u = __builtin_amdgcn_readfirstlane(sUnit);
+        const uint32_t u0 = __builtin_amdgcn_readfirstlane(sUnit);
         __syncthreads();
-        if (u >= nUnits) break;
-        const uint32_t item = u / (uint32_t)splits, part = u - item * (uint32_t)splits;
-        // the header only: a culled item's record is not read (nor written)
-        const uint32_t view = items[item].view;
-        const scr_visible_batch vb = items[item].vb;
-        const uint32_t active = vb.activePoints;
-        if (active == 0u) continue;
-        const uint32_t chunks = (active + kChunk - 1) / kChunk;
-        const uint32_t per = (chunks + (uint32_t)splits - 1) / (uint32_t)splits;
-        const uint32_t lo = part * per * kChunk;
-        if (lo >= active) continue;
-        const uint32_t hi = min(active, lo + per * kChunk);
-        body(views[__builtin_amdgcn_readfirstlane(view)], vb, &items[item].bounds, lo, hi);
+        if (u0 >= nUnits) break;
+        const uint32_t uEnd = min(u0 + kClaim, nUnits);
+        for (uint32_t u = u0; u < uEnd; u++) {
+            const uint32_t item = u / (uint32_t)splits, part = u - item * (uint32_t)splits;
+            // the header only: a culled item's record is not read (nor written)
+            const uint32_t view = items[item].view;
+            const scr_visible_batch vb = items[item].vb;
+            const uint32_t active = vb.activePoints;
+            if (active == 0u) continue;
+            const uint32_t chunks = (active + kChunk - 1) / kChunk;
+            const uint32_t per = (chunks + (uint32_t)splits - 1) / (uint32_t)splits;
+            const uint32_t lo = part * per * kChunk;
+            if (lo >= active) continue;
+            const uint32_t hi = min(active, lo + per * kChunk);
+            body(views[__builtin_amdgcn_readfirstlane(view)], vb, &items[item].bounds, lo, hi);
+        }
     }
 }
 

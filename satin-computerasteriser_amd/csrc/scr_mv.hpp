@@ -39,31 +39,43 @@ static_assert(sizeof(MVItem) == 160, "MVItem is ten 16-byte words");
 constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, differentiable colour
 
 // Takes work units from heads[queue] until the table is drained, kClaim
-// consecutive units per claim (culled items, about half of the table, then
-// cost a header read instead of a claim's round trip and two barriers each).
-// The bodies passed in are marked always_inline: a large body is otherwise
-// called as a function, with a stack frame, measured up to twice as slow.  A
-// unit is (item, part); part k of S draws the k-th of S equal, chunk-aligned
-// pieces of the item's active prefix.  Every workgroup leaves the loop once
-// the counter passes nUnits, so the grid drains.
+// consecutive units per claim, whose item headers the first threads read
+// together into LDS (culled items, about half of the table, then cost
+// neither a claim's round trip nor a dependent header read).  The bodies
+// passed in are marked always_inline: a large body is otherwise called as a
+// function, with a stack frame, measured up to twice as slow.  A unit is
+// (item, part); part k of S draws the k-th of S equal, chunk-aligned pieces of
+// the item's active prefix.  Every workgroup leaves the loop once the counter
+// passes nUnits, so the grid drains.
 constexpr uint32_t kClaim = 4;
 template <int NT, int PPT, typename Body>
 __device__ __forceinline__ void drain(const MVView* __restrict__ views, const MVItem* __restrict__ items, uint32_t nUnits,
                              int splits, uint32_t* head, Body body) {
+    static_assert(NT >= (int)kClaim, "a thread per claimed header");
     __shared__ uint32_t sUnit;
+    __shared__ uint32_t sView[kClaim];
+    __shared__ scr_visible_batch sVb[kClaim];
     constexpr uint32_t kChunk = NT * PPT;
     for (;;) {
         if (threadIdx.x == 0) sUnit = atomicAdd(head, kClaim);
         __syncthreads();
         const uint32_t u0 = __builtin_amdgcn_readfirstlane(sUnit);
-        __syncthreads();
         if (u0 >= nUnits) break;
         const uint32_t uEnd = min(u0 + kClaim, nUnits);
+        if (threadIdx.x < kClaim && u0 + threadIdx.x < uEnd) {
+            const uint32_t item = (u0 + threadIdx.x) / (uint32_t)splits;
+            sView[threadIdx.x] = items[item].view;
+            sVb[threadIdx.x] = items[item].vb;
+        }
+        __syncthreads();
         for (uint32_t u = u0; u < uEnd; u++) {
             const uint32_t item = u / (uint32_t)splits, part = u - item * (uint32_t)splits;
-            // the header only: a culled item's record is not read (nor written)
-            const uint32_t view = items[item].view;
-            const scr_visible_batch vb = items[item].vb;
+            const uint32_t view = __builtin_amdgcn_readfirstlane(sView[u - u0]);
+            scr_visible_batch vb = sVb[u - u0];
+            vb.batchIndex = __builtin_amdgcn_readfirstlane(vb.batchIndex);
+            vb.level = (int32_t)__builtin_amdgcn_readfirstlane((uint32_t)vb.level);
+            vb.lodThreshold = __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(vb.lodThreshold)));
+            vb.activePoints = __builtin_amdgcn_readfirstlane(vb.activePoints);
             const uint32_t active = vb.activePoints;
             if (active == 0u) continue;
             const uint32_t chunks = (active + kChunk - 1) / kChunk;
@@ -71,8 +83,10 @@ __device__ __forceinline__ void drain(const MVView* __restrict__ views, const MV
             const uint32_t lo = part * per * kChunk;
             if (lo >= active) continue;
             const uint32_t hi = min(active, lo + per * kChunk);
-            body(views[__builtin_amdgcn_readfirstlane(view)], vb, &items[item].bounds, lo, hi);
+            body(views[view], vb, &items[item].bounds, lo, hi);
         }
+        // every wave has read the headers before the next claim rewrites them
+        __syncthreads();
     }
 }
 

@@ -570,16 +570,30 @@ struct LodGrid {
     uint32_t dx, dy, dz;
 };
 
-__device__ inline void lod_cell(const uint32_t* bounds, const LodGrid& g, float4 p, uint32_t& cx, uint32_t& cy,
-                                uint32_t& cz) {
-    float mn[3], ext[3];
+// A level's cell transform: the cloud's minimum and the reciprocal voxel
+// size, computed once per thread from the bounds words (the same arithmetic
+// for every point).
+struct LodXform {
+    float mn[3];
+    float inv;
+};
+__device__ inline LodXform lod_xform(const uint32_t* bounds, const LodGrid& g) {
+    LodXform t;
+    float ext[3];
     for (int a = 0; a < 3; a++) {
-        mn[a] = float_unflip(bounds[a]);
-        ext[a] = fmaxf(float_unflip(bounds[3 + a]) - mn[a], 0.000001f);
+        t.mn[a] = float_unflip(bounds[a]);
+        ext[a] = fmaxf(float_unflip(bounds[3 + a]) - t.mn[a], 0.000001f);
     }
     const float longest = fmaxf(ext[0], fmaxf(ext[1], ext[2]));
     const float voxel = (longest / g.coarse) * g.voxelScale;
-    const float inv = 1.0f / fmaxf(voxel, 0.000001f);
+    t.inv = 1.0f / fmaxf(voxel, 0.000001f);
+    return t;
+}
+
+__device__ inline void lod_cell(const LodXform& t, const LodGrid& g, float4 p, uint32_t& cx, uint32_t& cy,
+                                uint32_t& cz) {
+    const float* mn = t.mn;
+    const float inv = t.inv;
     int32_t x = f2i(floorf((p.x - mn[0]) * inv));
     int32_t y = f2i(floorf((p.y - mn[1]) * inv));
     int32_t z = f2i(floorf((p.z - mn[2]) * inv));
@@ -593,9 +607,9 @@ __device__ inline void lod_cell(const uint32_t* bounds, const LodGrid& g, float4
     cz = min((uint32_t)z, g.dz - 1u);
 }
 
-__device__ inline uint64_t lod_lin(const uint32_t* bounds, const LodGrid& g, float4 p) {
+__device__ inline uint64_t lod_lin(const LodXform& t, const LodGrid& g, float4 p) {
     uint32_t cx, cy, cz;
-    lod_cell(bounds, g, p, cx, cy, cz);
+    lod_cell(t, g, p, cx, cy, cz);
     return (uint64_t)cx + (uint64_t)g.dx * ((uint64_t)cy + (uint64_t)g.dy * cz);
 }
 
@@ -607,6 +621,7 @@ constexpr int kLodPts = 4;   // points per thread: index blockIdx * 1024 + k * 2
 __global__ __launch_bounds__(kThreads) void k_lod_claim(const float4* __restrict__ sp, const uint8_t* __restrict__ lv,
                                                         size_t n, uint8_t maxLevel, const uint32_t* __restrict__ bounds,
                                                         LodGrid g, uint32_t* __restrict__ grid) {
+    const LodXform t = lod_xform(bounds, g);
     const size_t b0 = (size_t)blockIdx.x * (kThreads * kLodPts) + threadIdx.x;
     float4 p[kLodPts];
     uint8_t l[kLodPts];
@@ -620,7 +635,7 @@ __global__ __launch_bounds__(kThreads) void k_lod_claim(const float4* __restrict
     for (int k = 0; k < kLodPts; k++) {
         const size_t i = b0 + (size_t)k * kThreads;
         const bool part = i < n && l[k] == maxLevel;
-        const uint64_t lin = part ? lod_lin(bounds, g, p[k]) : ~0ull;
+        const uint64_t lin = part ? lod_lin(t, g, p[k]) : ~0ull;
         const uint64_t prev = __shfl_up(lin, 1);
         const bool dup = __lane_id() > 0 && prev == lin;
         if (part && !dup) atomicMin(grid + lin, (uint32_t)i);
@@ -635,6 +650,7 @@ __global__ __launch_bounds__(kThreads) void k_lod_step(const float4* __restrict_
                                                        const uint32_t* __restrict__ bounds, LodGrid gl,
                                                        const uint32_t* __restrict__ gridL, LodGrid gn,
                                                        uint32_t* __restrict__ gridN) {
+    const LodXform tl = lod_xform(bounds, gl), tn = lod_xform(bounds, gn);
     const size_t b0 = (size_t)blockIdx.x * (kThreads * kLodPts) + threadIdx.x;
     float4 p[kLodPts];
     uint8_t l[kLodPts];
@@ -648,7 +664,7 @@ __global__ __launch_bounds__(kThreads) void k_lod_step(const float4* __restrict_
 #pragma unroll
     for (int k = 0; k < kLodPts; k++) {
         const size_t i = b0 + (size_t)k * kThreads;
-        won[k] = (i < n && l[k] == maxLevel) ? gridL[lod_lin(bounds, gl, p[k])] : 0xffffffffu;
+        won[k] = (i < n && l[k] == maxLevel) ? gridL[lod_lin(tl, gl, p[k])] : 0xffffffffu;
     }
 #pragma unroll
     for (int k = 0; k < kLodPts; k++) {
@@ -658,7 +674,7 @@ __global__ __launch_bounds__(kThreads) void k_lod_step(const float4* __restrict_
             lv[i] = level;
             part = false;
         }
-        const uint64_t lin = part ? lod_lin(bounds, gn, p[k]) : ~0ull;
+        const uint64_t lin = part ? lod_lin(tn, gn, p[k]) : ~0ull;
         const uint64_t prev = __shfl_up(lin, 1);
         const bool dup = __lane_id() > 0 && prev == lin;
         if (part && !dup) atomicMin(gridN + lin, (uint32_t)i);
@@ -693,6 +709,7 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const float4* __restrict_
     const int w = tid >> 6;
     const size_t start = (size_t)blockIdx.x * ppb;
     const size_t end = min(start + (size_t)ppb, n);
+    const LodXform tf = lastLevel >= 0 ? lod_xform(bounds, g) : LodXform{};
     if (tid < 8) cnt[tid] = 0;
     __syncthreads();
     float mn[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
@@ -700,7 +717,7 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const float4* __restrict_
     for (size_t i = start + tid; i < end; i += kThreads) {
         const float4 p = sp[i];
         uint8_t l = lv[i];
-        if (lastLevel >= 0 && l == maxLevel && grid[lod_lin(bounds, g, p)] == (uint32_t)i) {
+        if (lastLevel >= 0 && l == maxLevel && grid[lod_lin(tf, g, p)] == (uint32_t)i) {
             l = (uint8_t)lastLevel;
             lv[i] = l;   // re-read below by this same thread
         }

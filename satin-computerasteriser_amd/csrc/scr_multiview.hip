@@ -121,7 +121,7 @@ __global__ __launch_bounds__(kDiffThreads) void k_mv_diff_color(const MVView* __
                                                                 const MVItem* __restrict__ items, uint32_t nUnits,
                                                                 const float4* __restrict__ params, uint32_t* heads) {
     __shared__ DiffLds L;
-    drain<kDiffThreads, kDiffPPT>(views, items, nUnits, 1, heads + 3,
+    drain<kDiffThreads, kDiffPPT>(views, items, nUnits, 1, 1u, heads + 3,
                                   [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo,
                                       uint32_t hi) __attribute__((always_inline)) {
                                       const BatchCtx bc = batch_ctx(V.c, V.p, vb, bb, hi);
@@ -413,7 +413,7 @@ int enqueue(scr_multiview* mv, int mode, bool r0, uint32_t nItems, int timedPhas
         {
             scr::PassScope ps(ctx, ph == scr::kDepth ? "mv_depth" : (ph == scr::kColor ? "mv_color" : "mv_nearest"));
             hipLaunchKernelGGL(kern, dim3(grid), dim3((unsigned)var->threads), 0, s, views, mv->dItems, nUnits,
-                               sh.splits, mv->dHeads);
+                               sh.splits, scr::mv::claim_for(nUnits, grid), mv->dHeads);
             SCR_HIP(ctx, hipGetLastError());
         }
         if (timedPhase == (int)ph) SCR_HIP(ctx, hipEventRecord(t1, s));
@@ -757,7 +757,7 @@ int scr_multiview_diff_render(scr_multiview* mv, const scr_config* cfg, const sc
         if (rc != SCR_OK) return rc;
         scr::PassScope ps(ctx, "mv_depth");
         hipLaunchKernelGGL(kern, dim3(grid), dim3((unsigned)var->threads), 0, s, views, mv->dItems, nUnits, sh.splits,
-                           mv->dHeads);
+                           scr::mv::claim_for(nUnits, grid), mv->dHeads);
         SCR_HIP(ctx, hipGetLastError());
     }
     {

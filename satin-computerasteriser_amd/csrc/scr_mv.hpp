@@ -40,29 +40,36 @@ constexpr int kQueues = 4;          // queue heads: depth, colour, nearest, diff
 
 // Takes work units from heads[queue] until the table is drained, kClaim
 // consecutive units per claim, whose item headers the first threads read
-// together into LDS (culled items, about half of the table, then cost
-// neither a claim's round trip nor a dependent header read).  The bodies
+// together into LDS (culled items, about half of the render workload's
+// table, then cost neither a claim's round trip nor a dependent header read;
+// a short table takes one unit per claim, which keeps the pass's tail even).  The bodies
 // passed in are marked always_inline: a large body is otherwise called as a
 // function, with a stack frame, measured up to twice as slow.  A unit is
 // (item, part); part k of S draws the k-th of S equal, chunk-aligned pieces of
 // the item's active prefix.  Every workgroup leaves the loop once the counter
 // passes nUnits, so the grid drains.
-constexpr uint32_t kClaim = 4;
+constexpr uint32_t kMaxClaim = 4;
+// The claim the host passes: kMaxClaim when the table holds at least 32 units
+// per workgroup (the pass's tail then stays fine-grained), else 1.
+__host__ __device__ inline uint32_t claim_for(uint32_t nUnits, uint32_t grid) {
+    return nUnits >= 32u * grid ? kMaxClaim : 1u;
+}
 template <int NT, int PPT, typename Body>
 __device__ __forceinline__ void drain(const MVView* __restrict__ views, const MVItem* __restrict__ items, uint32_t nUnits,
-                             int splits, uint32_t* head, Body body) {
-    static_assert(NT >= (int)kClaim, "a thread per claimed header");
+                             int splits, uint32_t claim, uint32_t* head, Body body) {
+    static_assert(NT >= (int)kMaxClaim, "a thread per claimed header");
     __shared__ uint32_t sUnit;
-    __shared__ uint32_t sView[kClaim];
-    __shared__ scr_visible_batch sVb[kClaim];
+    __shared__ uint32_t sView[kMaxClaim];
+    __shared__ scr_visible_batch sVb[kMaxClaim];
     constexpr uint32_t kChunk = NT * PPT;
+    claim = min(max(claim, 1u), kMaxClaim);
     for (;;) {
-        if (threadIdx.x == 0) sUnit = atomicAdd(head, kClaim);
+        if (threadIdx.x == 0) sUnit = atomicAdd(head, claim);
         __syncthreads();
         const uint32_t u0 = __builtin_amdgcn_readfirstlane(sUnit);
         if (u0 >= nUnits) break;
-        const uint32_t uEnd = min(u0 + kClaim, nUnits);
-        if (threadIdx.x < kClaim && u0 + threadIdx.x < uEnd) {
+        const uint32_t uEnd = min(u0 + claim, nUnits);
+        if (threadIdx.x < claim && u0 + threadIdx.x < uEnd) {
             const uint32_t item = (u0 + threadIdx.x) / (uint32_t)splits;
             sView[threadIdx.x] = items[item].view;
             sVb[threadIdx.x] = items[item].vb;
@@ -92,10 +99,10 @@ __device__ __forceinline__ void drain(const MVView* __restrict__ views, const MV
 
 template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* __restrict__ views, const MVItem* __restrict__ items,
-                                                 uint32_t nUnits, int splits, uint32_t* heads) {
+                                                 uint32_t nUnits, int splits, uint32_t claim, uint32_t* heads) {
     __shared__ uint32_t lds[kMVDepthTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
-    drain<NT, PPT>(views, items, nUnits, splits, heads + 0,
+    drain<NT, PPT>(views, items, nUnits, splits, claim, heads + 0,
                    [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
                        depth_batch<kR0, kWave, NT, PPT, kMVDepthTile>(V.c, V.p, vb, bb, lo, hi, V.planes, lds, red);
                    });
@@ -103,12 +110,12 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* 
 
 template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
-                                                 uint32_t nUnits, int splits, uint32_t* heads) {
+                                                 uint32_t nUnits, int splits, uint32_t claim, uint32_t* heads) {
     __shared__ uint32_t ldsDepth[kColorTile];
     __shared__ unsigned long long ldsRG[kColorTile];
     __shared__ unsigned long long ldsBC[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
-    drain<NT, PPT>(views, items, nUnits, splits, heads + 1,
+    drain<NT, PPT>(views, items, nUnits, splits, claim, heads + 1,
                    [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
                        const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
                        const size_t s = (size_t)V.stride;
@@ -120,16 +127,16 @@ __global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(c
 template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_nearest(const MVView* __restrict__ views,
                                                    const MVItem* __restrict__ items, uint32_t nUnits, int splits,
-                                                   uint32_t* heads) {
+                                                   uint32_t claim, uint32_t* heads) {
     __shared__ unsigned long long lds[kMVNearestTile + kSpare];
     __shared__ int red[5 * kMaxWaves];
-    drain<NT, PPT>(views, items, nUnits, splits, heads + 2,
+    drain<NT, PPT>(views, items, nUnits, splits, claim, heads + 2,
                    [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
                        nearest_batch<kR0, kWave, NT, PPT, kMVNearestTile>(V.c, V.p, vb, bb, lo, hi, V.nearest, lds, red);
                    });
 }
 
-using DrawKernel = void (*)(const MVView*, const MVItem*, uint32_t, int, uint32_t*);
+using DrawKernel = void (*)(const MVView*, const MVItem*, uint32_t, int, uint32_t, uint32_t*);
 
 // Draw shapes (threads x points per thread) the autotuner chooses from.
 struct Shape {

@@ -1048,7 +1048,7 @@ int frame_front(scr_ctx* ctx, uint32_t* planes, size_t words, scr_cloud* const* 
             d.counter = c->dCounter + (c->counterWord ^ 1);   // left zero by the previous front (or never used)
             cs.c[i] = d;
             cs.blockStart[i] = blocks;
-            blocks += (c->maxResidentBatches + 255) / 256;
+            blocks += (c->maxResidentBatches + scr::kCullSlotsPerBlock - 1) / scr::kCullSlotsPerBlock;
             su.dst[i] = c->dFiles;
             su.first[i] = nf;
             su.count[i] = (int)c->files.size();

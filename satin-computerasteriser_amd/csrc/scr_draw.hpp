@@ -155,8 +155,20 @@ __device__ __forceinline__ void decode_steps(const scr_raster_batch& b, int leve
 // true when the batch is drawn, with its VisibleBatch record in `vb` and its
 // draw record (what its box proves about its points, batch_bounds, and its
 // draw context) in `dr`.
+// The bounds part of the decision, which depends on nothing else of it: what
+// the box of slot gid's batch proves under its file transform (for a resident
+// batch with a valid file; undisplaced draws only read it).
+__device__ __forceinline__ BatchBounds cull_bounds(const CloudDev& c, const PassParams& p, int gid) {
+    const scr_raster_batch b = c.batches[gid];
+    if (b.state == 0 || (int)b.fileIndex >= c.nFiles || p.applyDisplacement != 0) return BatchBounds{};
+    const scr_raster_file* f = c.files + b.fileIndex;
+    return batch_bounds(f->transform, b, p.width, p.height, batch_const_radius(p, f->world, b));
+}
+
+// given: the slot's cull_bounds, computed elsewhere (by another wave of the
+// workgroup), or nullptr to compute them here.
 __device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p, int gid, scr_visible_batch& vb,
-                                          DrawRecord& dr) {
+                                          DrawRecord& dr, const BatchBounds* given = nullptr) {
     const scr_raster_batch b = c.batches[gid];
     if (b.state == 0) return false;  // residency gate, Cull/Shaders.metal:34-36
     if ((int)b.fileIndex >= c.nFiles) return false;
@@ -189,6 +201,8 @@ __device__ __forceinline__ bool cull_slot(const CloudDev& c, const PassParams& p
         dr.bb = BatchBounds{};
         dr.bb.rconst = p.constRadius;
         dr.bb.zmax = __builtin_inff();
+    } else if (given) {
+        dr.bb = *given;
     } else {
         dr.bb = batch_bounds(f->transform, b, p.width, p.height, batch_const_radius(p, f->world, b));
     }

@@ -178,6 +178,7 @@ hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p);
 // table from the arguments, in blocks [blockStart[k], blockStart[k + 1]).
 constexpr int kSetupClouds = 4;
 constexpr int kSetupFiles = 8;
+constexpr int kCullSlotsPerBlock = 128;   // a cull block of 256 threads: two waves per slot group
 struct FrameSetup {
     uint32_t* planes = nullptr;
     size_t words = 0;

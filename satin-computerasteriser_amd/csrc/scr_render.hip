@@ -28,13 +28,30 @@ constexpr int kThreads = 256;   // draw workgroup
 constexpr int kPPT = 8;         // points per thread
 constexpr bool kWaveTiles = false;  // chunks and LDS tiles shared by the workgroup (measured faster than per-wave tiles)
 
-// Appends the slot's visible record when it is drawn: one counter update per
-// wave (all lanes reach the ballot), each visible lane taking the place its
-// rank among the wave's visible lanes gives it.
-__device__ __forceinline__ void cull_thread(const CloudDev& c, const PassParams& p, int gid) {
+// The cull in 256-thread blocks of kCullSlotsPerBlock slots: waves 2 and 3
+// prove each slot's bounds (cull_bounds, the double arithmetic) while waves 0
+// and 1 take the rest of its decision, then read the bounds from LDS.  A
+// visible slot is appended with one counter update per wave (all lanes reach
+// the ballot), each visible lane taking the place its rank among the wave's
+// visible lanes gives it.  `blk` is the block's index among the cull blocks.
+__device__ __forceinline__ void cull_block(const CloudDev& c, const PassParams& p, int blk) {
+    __shared__ BatchBounds sBounds[kCullSlotsPerBlock];
+    const int t = (int)threadIdx.x;
+    const bool proves = t >= kCullSlotsPerBlock;
+    const int gid = blk * kCullSlotsPerBlock + (proves ? t - kCullSlotsPerBlock : t);
     scr_visible_batch vb;
     DrawRecord dr;
-    const bool vis = gid < c.nSlots && cull_slot(c, p, gid, vb, dr);
+    bool vis = false;
+    if (proves) {
+        if (gid < c.nSlots) sBounds[t - kCullSlotsPerBlock] = cull_bounds(c, p, gid);
+    } else {
+        // the rest of the decision meanwhile; the bounds are filled in below
+        const BatchBounds later{};
+        vis = gid < c.nSlots && cull_slot(c, p, gid, vb, dr, &later);
+    }
+    __syncthreads();
+    if (proves) return;
+    if (vis && p.applyDisplacement == 0) dr.bb = sBounds[t];
     const uint64_t m = __ballot(vis);
     if (m == 0ull) return;
     const int leader = __ffsll((unsigned long long)m) - 1;
@@ -48,7 +65,7 @@ __device__ __forceinline__ void cull_thread(const CloudDev& c, const PassParams&
 }
 
 __global__ __launch_bounds__(256) void k_cull(CloudDev c, PassParams p) {
-    cull_thread(c, p, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+    cull_block(c, p, (int)blockIdx.x);
 }
 
 // The draw kernels are persistent grids: workgroup b takes visible batches
@@ -130,7 +147,7 @@ __global__ __launch_bounds__(256) void k_frame_front(FrameSetup su, CullSet cs, 
         while (k + 1 < cs.nClouds && (int)blockIdx.x >= cs.blockStart[k + 1]) k++;
         CloudDev c = cs.c[k];
         c.files = su.files + su.first[k];
-        cull_thread(c, p, ((int)blockIdx.x - cs.blockStart[k]) * 256 + (int)threadIdx.x);
+        cull_block(c, p, (int)blockIdx.x - cs.blockStart[k]);
         return;
     }
     const size_t blk = blockIdx.x - (unsigned)nCull;
@@ -199,7 +216,8 @@ hipError_t launch_frame_front(hipStream_t s, const FrameSetup& su, const CullSet
     // cloud k's blocks cover its slots, and its table is the one it reads
     if (cs.blockStart[0] != 0) return hipErrorInvalidValue;
     for (int c = 0; c < cs.nClouds; c++)
-        if (cs.blockStart[c + 1] - cs.blockStart[c] != (int)grid_for((size_t)std::max(cs.c[c].nSlots, 0), 256) ||
+        if (cs.blockStart[c + 1] - cs.blockStart[c] !=
+                (int)grid_for((size_t)std::max(cs.c[c].nSlots, 0), kCullSlotsPerBlock) ||
             cs.c[c].nFiles != su.count[c])
             return hipErrorInvalidValue;
     const unsigned grid = (unsigned)cs.blockStart[cs.nClouds] + std::max(1u, grid_for(su.words / 4, 1024));
@@ -639,7 +657,7 @@ hipError_t launch_cull(hipStream_t s, const CloudDev& c, const PassParams& p) {
     hipError_t e = launch_fill_u32(s, c.counter, 1, 0u);
     if (e != hipSuccess) return e;
     if (c.nSlots <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_cull, dim3(grid_for((size_t)c.nSlots, 256)), dim3(256), 0, s, c, p);
+    hipLaunchKernelGGL(k_cull, dim3(grid_for((size_t)c.nSlots, kCullSlotsPerBlock)), dim3(256), 0, s, c, p);
     return hipGetLastError();
 }
 

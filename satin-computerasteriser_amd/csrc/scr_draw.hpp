@@ -1082,21 +1082,6 @@ __device__ __forceinline__ float color_stage(const ChunkBox& box, const PassPara
     return tmin;
 }
 
-// Smallest v over the group, in every lane; ends with a barrier.
-template <bool kWave, int NT>
-__device__ __forceinline__ float group_minf(float v, int* red) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
-    if (kWave) return v;
-    constexpr int kWaves = NT / 64;
-    if (__lane_id() == 0) lds_store(red + (threadIdx.x >> 6), __float_as_int(v));
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kWaves; k++) v = fminf(v, __int_as_float(lds_load(red + k)));
-    __syncthreads();
-    return v;
-}
-
 // Adds a tile's sums to the planes.  A tile holds one batch, so a pixel's
 // sums stay below 2^32 (at most 65,535 points per batch, each adding at most
 // 255 once), and the packed halves never carry into each other.
@@ -1129,8 +1114,18 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
     const bool vec = vec_ok<PPT>(bc);
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
     if (seg.lds) {
-        const float tmin = group_minf<kWave, NT>(color_stage<G>(seg, p, depth, L), L.red);
+        // the tile's smallest threshold: one barrier makes both the staged
+        // tile and the waves' minima visible (the reduction words are next
+        // written by the following batch, after this one's flush barriers)
+        float tmin = color_stage<G>(seg, p, depth, L);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) tmin = fminf(tmin, __shfl_xor(tmin, o));
+        if (!kWave && __lane_id() == 0) lds_store(L.red + (threadIdx.x >> 6), __float_as_int(tmin));
         G::sync();
+        if (!kWave) {
+#pragma unroll
+            for (int k = 0; k < NT / 64; k++) tmin = fminf(tmin, __int_as_float(lds_load(L.red + k)));
+        }
         // every point of the batch lies behind every accepted depth of its
         // rectangle: no fragment can pass the test, nothing to add
         if (tmin > seg.zmax) return;

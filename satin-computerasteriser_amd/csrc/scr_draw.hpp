@@ -23,7 +23,8 @@ namespace draw {
 // Draw workgroups hold NT threads (a multiple of 64) that own PPT points each.
 constexpr int kMaxWaves = 8;        // NT <= 512
 constexpr int kDepthTile = 9984;    // u32 per pixel: 39 KB; with the spare and reduction words four workgroups per CU
-constexpr int kColorTile = 2016;    // depth u32 + (r | g << 32) u64 + (b | count << 32) u64: 39.4 KB, four workgroups per CU
+constexpr int kColorTile = 2528;    // threshold u32 + (r | g << 24 | count << 48) u64 + b u32: 39.5 KB, four per CU
+constexpr int kBlendTile = 2016;    // two u64 sums per pixel (weighted-blend colour): 31.5 KB
 constexpr int kNearestTile = 4992;  // u64 per pixel: 39 KB, likewise four per CU
 // The multi-view draw kernels keep smaller tiles: their 128-thread shapes
 // stay at 4 waves per SIMD within a CU's LDS (measured faster there).
@@ -864,19 +865,18 @@ constexpr bool kColorInside = false;
 // colorPassUpdate (opaque path) — ColorPass/Shaders.metal:30-219, one visible batch.
 struct ColorLds {
     uint32_t* depth;   // per tile pixel: color_threshold of its depth, as float bits
-    unsigned long long* rg;
-    unsigned long long* bn;
+    unsigned long long* rgn;  // per tile pixel: red | green << 24 | count << 48
+    uint32_t* b;              // per tile pixel: blue
     int* red;
 };
 
 // Built from arguments rather than a constant initializer: a constant
 // aggregate of LDS addresses would become a global the back end cannot emit.
-__device__ __forceinline__ ColorLds color_lds(uint32_t* depth, unsigned long long* rg, unsigned long long* bn,
-                                              int* red) {
+__device__ __forceinline__ ColorLds color_lds(uint32_t* depth, unsigned long long* rgn, uint32_t* b, int* red) {
     ColorLds L;
     L.depth = depth;
-    L.rg = rg;
-    L.bn = bn;
+    L.rgn = rgn;
+    L.b = b;
     L.red = red;
     return L;
 }
@@ -889,18 +889,25 @@ __device__ __forceinline__ float color_threshold(uint32_t d, const PassParams& p
     return d != 0u ? uint_to_depth(d) * p.depthKeep : __builtin_inff();
 }
 
-// Adds a run's packed sums (r | g << 32, b | count << 32) to LDS tile slot k
-// or to global pixel k.
-__device__ __forceinline__ void color_flush(bool lds, int k, unsigned long long rg, unsigned long long bn, ColorLds L,
+// A fragment's or a run's sums: red | green << 24 | count << 48 and blue.
+// Every tile and run holds points of one batch (at most 65,535, each adding
+// at most 255 to a pixel once), so red, green and blue stay below 2^24 and
+// the count below 2^16: the packed fields never carry into each other.
+__device__ __forceinline__ unsigned long long color_rgn(uint32_t rgb) {
+    return (unsigned long long)(rgb & 0xffu) | ((unsigned long long)((rgb >> 8) & 0xffu) << 24) | (1ull << 48);
+}
+
+// Adds a run's sums to LDS tile slot k or to global pixel k.
+__device__ __forceinline__ void color_flush(bool lds, int k, unsigned long long rgn, uint32_t b, ColorLds L,
                                    uint32_t* red, uint32_t* green, uint32_t* blue, uint32_t* count) {
     if (lds) {
-        lds_add(L.rg + k, rg);
-        lds_add(L.bn + k, bn);
+        lds_add(L.rgn + k, rgn);
+        lds_add(L.b + k, b);
     } else {
-        gadd(red + k, (uint32_t)rg);
-        gadd(green + k, (uint32_t)(rg >> 32));
-        gadd(blue + k, (uint32_t)bn);
-        gadd(count + k, (uint32_t)(bn >> 32));
+        gadd(red + k, (uint32_t)rgn & 0xffffffu);
+        gadd(green + k, (uint32_t)(rgn >> 24) & 0xffffffu);
+        gadd(blue + k, b);
+        gadd(count + k, (uint32_t)(rgn >> 48));
     }
 }
 
@@ -968,26 +975,26 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
                                    const PassParams& p, const uint32_t* __restrict__ depth, uint32_t* red,
                                    uint32_t* green, uint32_t* blue, uint32_t* count, ColorLds L) {
     int runK = -1;
-    unsigned long long runRG = 0ull, runBN = 0ull;
+    unsigned long long runRGN = 0ull;
+    uint32_t runB = 0u;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         if (!f.ok[j]) continue;
-        const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
-        const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
-        const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
+        const unsigned long long rgn = color_rgn(rgb[j]);
+        const uint32_t b8 = (rgb[j] >> 16) & 0xffu;
         if (kR0) {
             const int k = box.lds ? box.local(f.px[j], f.py[j]) : f.py[j] * p.width + f.px[j];
             const float thr = box.lds ? __uint_as_float(lds_load(L.depth + k)) : color_threshold(gload(depth + k), p);
             if (!(f.z[j] >= thr)) continue;
             if (k == runK) {
-                runRG += rg;
-                runBN += bn;
+                runRGN += rgn;
+                runB += b8;
                 continue;
             }
-            if (runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
+            if (runK >= 0) color_flush(box.lds, runK, runRGN, runB, L, red, green, blue, count);
             runK = k;
-            runRG = rg;
-            runBN = bn;
+            runRGN = rgn;
+            runB = b8;
             continue;
         }
         const int rad = f.r[j];
@@ -1003,17 +1010,17 @@ __device__ __forceinline__ void color_frags(const ChunkFrags<PPT>& f, const uint
                     inTile ? __uint_as_float(lds_load(L.depth + k)) : color_threshold(gload(depth + pix), p);
                 if (!(f.z[j] >= thr)) continue;
                 if (inTile) {
-                    lds_add(L.rg + k, rg);
-                    lds_add(L.bn + k, bn);
+                    lds_add(L.rgn + k, rgn);
+                    lds_add(L.b + k, b8);
                 } else {
-                    gadd(red + pix, r8);
-                    gadd(green + pix, g8);
+                    gadd(red + pix, rgb[j] & 0xffu);
+                    gadd(green + pix, (rgb[j] >> 8) & 0xffu);
                     gadd(blue + pix, b8);
                     gadd(count + pix, 1u);
                 }
             }
     }
-    if (kR0 && runK >= 0) color_flush(box.lds, runK, runRG, runBN, L, red, green, blue, count);
+    if (kR0 && runK >= 0) color_flush(box.lds, runK, runRGN, runB, L, red, green, blue, count);
 }
 
 // color_frags for single-pixel points meeting in a batch tile (the proven
@@ -1036,31 +1043,31 @@ __device__ __forceinline__ void color_frags_tile(const ChunkFrags<PPT>& f, const
 #pragma unroll
     for (int j = 0; j < PPT; j++) thr[j] = __uint_as_float(lds_load(L.depth + ks[j]));
     int runK = -1;
-    unsigned long long runRG = 0ull, runBN = 0ull;
+    unsigned long long runRGN = 0ull;
+    uint32_t runB = 0u;
 #pragma unroll
     for (int j = 0; j < PPT; j++) {
         const int k = ks[j];
         const bool pass = f.ok[j] & (f.z[j] >= thr[j]);
         if (!pass) continue;
-        const uint32_t r8 = rgb[j] & 0xffu, g8 = (rgb[j] >> 8) & 0xffu, b8 = (rgb[j] >> 16) & 0xffu;
-        const unsigned long long rg = (unsigned long long)r8 | ((unsigned long long)g8 << 32);
-        const unsigned long long bn = (unsigned long long)b8 | (1ull << 32);
+        const unsigned long long rgn = color_rgn(rgb[j]);
+        const uint32_t b8 = (rgb[j] >> 16) & 0xffu;
         if (k == runK) {
-            runRG += rg;
-            runBN += bn;
+            runRGN += rgn;
+            runB += b8;
             continue;
         }
         if (runK >= 0) {
-            lds_add(L.rg + runK, runRG);
-            lds_add(L.bn + runK, runBN);
+            lds_add(L.rgn + runK, runRGN);
+            lds_add(L.b + runK, runB);
         }
         runK = k;
-        runRG = rg;
-        runBN = bn;
+        runRGN = rgn;
+        runB = b8;
     }
     if (runK >= 0) {
-        lds_add(L.rg + runK, runRG);
-        lds_add(L.bn + runK, runBN);
+        lds_add(L.rgn + runK, runRGN);
+        lds_add(L.b + runK, runB);
     }
 }
 
@@ -1075,29 +1082,26 @@ __device__ __forceinline__ float color_stage(const ChunkBox& box, const PassPara
     for (int i = G::rank(); i < area; i += G::kSize) {
         const float thr = color_threshold(gload(depth + box.pixel(i, p.width)), p);
         lds_store(L.depth + i, __float_as_uint(thr));
-        lds_store(L.rg + i, 0ull);
-        lds_store(L.bn + i, 0ull);
+        lds_store(L.rgn + i, 0ull);
+        lds_store(L.b + i, 0u);
         tmin = fminf(tmin, thr);
     }
     return tmin;
 }
 
-// Adds a tile's sums to the planes.  A tile holds one batch, so a pixel's
-// sums stay below 2^32 (at most 65,535 points per batch, each adding at most
-// 255 once), and the packed halves never carry into each other.
+// Adds a tile's sums (one batch's, packed as color_rgn describes) to the planes.
 template <typename G>
 __device__ __forceinline__ void color_tile_flush(const ChunkBox& box, const PassParams& p, ColorLds L, uint32_t* red,
                                         uint32_t* green, uint32_t* blue, uint32_t* count) {
     const int area = box.w * box.h;
     for (int i = G::rank(); i < area; i += G::kSize) {
-        const unsigned long long bn = lds_load(L.bn + i);
-        const uint32_t n = (uint32_t)(bn >> 32);
+        const unsigned long long rgn = lds_load(L.rgn + i);
+        const uint32_t n = (uint32_t)(rgn >> 48);
         if (n == 0u) continue;
-        const unsigned long long rg = lds_load(L.rg + i);
         const size_t pix = box.pixel(i, p.width);
-        gadd(red + pix, (uint32_t)rg);
-        gadd(green + pix, (uint32_t)(rg >> 32));
-        gadd(blue + pix, (uint32_t)bn);
+        gadd(red + pix, (uint32_t)rgn & 0xffffffu);
+        gadd(green + pix, (uint32_t)(rgn >> 24) & 0xffffffu);
+        gadd(blue + pix, lds_load(L.b + i));
         gadd(count + pix, n);
     }
 }
@@ -1110,7 +1114,7 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
     using G = Group<kWave, NT>;
     constexpr int kCap = kColorTile / G::kCount;
     const int slice = G::index() * kCap;
-    const ColorLds L = color_lds(LAll.depth + slice, LAll.rg + slice, LAll.bn + slice, LAll.red);
+    const ColorLds L = color_lds(LAll.depth + slice, LAll.rgn + slice, LAll.b + slice, LAll.red);
     const bool vec = vec_ok<PPT>(bc);
     const uint32_t first = lo + (uint32_t)(G::index() * G::kSize * PPT);
     if (seg.lds) {

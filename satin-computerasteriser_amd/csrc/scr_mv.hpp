@@ -112,12 +112,12 @@ template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
                                                  uint32_t nUnits, int splits, uint32_t claim, uint32_t* heads) {
     __shared__ uint32_t ldsDepth[kColorTile];
-    __shared__ unsigned long long ldsRG[kColorTile];
-    __shared__ unsigned long long ldsBC[kColorTile];
+    __shared__ unsigned long long ldsRGN[kColorTile];
+    __shared__ uint32_t ldsB[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, claim, heads + 1,
                    [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
-                       const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
+                       const ColorLds L = color_lds(ldsDepth, ldsRGN, ldsB, redbuf);
                        const size_t s = (size_t)V.stride;
                        color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, bb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
                                                  V.planes + 3 * s, V.planes + 4 * s, L);

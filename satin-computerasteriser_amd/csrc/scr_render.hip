@@ -88,10 +88,10 @@ __global__ __launch_bounds__(kThreads, kR0 ? 4 : 3) void k_color(CloudDev c, Pas
                                                     uint32_t* red, uint32_t* green, uint32_t* blue,
                                                     uint32_t* count) {
     __shared__ uint32_t ldsDepth[kColorTile];
-    __shared__ unsigned long long ldsRG[kColorTile];
-    __shared__ unsigned long long ldsBC[kColorTile];
+    __shared__ unsigned long long ldsRGN[kColorTile];
+    __shared__ uint32_t ldsB[kColorTile];
     __shared__ int redbuf[5 * kMaxWaves];
-    const ColorLds L = color_lds(ldsDepth, ldsRG, ldsBC, redbuf);
+    const ColorLds L = color_lds(ldsDepth, ldsRGN, ldsB, redbuf);
     const uint32_t nvis = *c.counter;
     for (uint32_t v = blockIdx.x; v < nvis; v += gridDim.x)
         color_batch<kR0, kWaveTiles, kThreads, kPPT, kDisp>(c, p, c.visible[v], c.bounds + v, 0u, 0xffffffffu, depth,
@@ -381,8 +381,8 @@ __device__ __forceinline__ void coverage_tile_clear(const ChunkBox& box, unsigne
 __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p, CoverageParams q,
                                                         uint32_t* __restrict__ red, uint32_t* __restrict__ green,
                                                         uint32_t* __restrict__ blue, uint32_t* __restrict__ count) {
-    __shared__ unsigned long long tileRG[kColorTile];
-    __shared__ unsigned long long tileBA[kColorTile];
+    __shared__ unsigned long long tileRG[kBlendTile];
+    __shared__ unsigned long long tileBA[kBlendTile];
     __shared__ int redbuf[5 * kMaxWaves];
     // a sweep reaches at most maxSpread + 1 pixels from the point's pixel
     const int reach = q.motionBlur > 0.0f ? ((q.maxSpread >= 0.0f && q.maxSpread < 4096.0f)
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
         const scr_visible_batch vb = c.visible[v];
         const BatchCtx bc = batch_ctx(c, p, vb, c.bounds + v, 0xffffffffu);
         const scr_raster_file* file = c.files + c.batches[vb.batchIndex].fileIndex;
-        const ChunkBox seg = segment_box(p, bc, bc.active, kColorTile, reach);
+        const ChunkBox seg = segment_box(p, bc, bc.active, kBlendTile, reach);
         if (seg.lds) {
             coverage_tile_clear(seg, tileRG, tileBA);
             __syncthreads();
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_color_coverage(CloudDev c, PassParams p
             // it fits (as the opaque colour pass does)
             ChunkBox box = seg;
             if (!seg.lds) {
-                box = chunk_box<false, 256, kCovPPT>(f, p, kColorTile, redbuf);
+                box = chunk_box<false, 256, kCovPPT>(f, p, kBlendTile, redbuf);
                 if (box.lds) {
                     coverage_tile_clear(box, tileRG, tileBA);
                     __syncthreads();

@@ -103,3 +103,42 @@ def test_patches_multiview_match_single_view(ctx, patches, mode):
             bad = np.argwhere(got != w[name])
             assert bad.size == 0, f"view {v} plane {name}: {len(bad)} pixels differ, first {bad[:3].tolist()}"
     mv.close()
+
+
+def test_full_batch_on_one_pixel_sums_do_not_carry(ctx):
+    """One batch of 65,535 white points (the most a batch holds) inside one
+    pixel: each channel sums to 65,535 x 255 = 16,711,425 there, just under
+    2^24, and the count to 65,535, just under 2^16 -- the largest values the
+    colour tile's packed fields (red | green << 24 | count << 48, blue) hold.
+    Single view against the oracle and a two-view multi-view launch against
+    the single view, bit for bit."""
+    n = 65535  # pointsPerBatch is at most 65,535
+    units_per_px = 2.0 * 3.0 * np.tan(np.radians(30.0)) / H
+    # half a pixel off the screen centre (a pixel corner) in x and y: a pixel
+    # centre whichever way the axes point; the points spread over 0.2 pixel
+    c = 0.5 * units_per_px
+    rng = np.random.default_rng(0xFEED)
+    pos = np.stack([rng.uniform(c - 0.1 * units_per_px, c + 0.1 * units_per_px, n),
+                    rng.uniform(c - 0.1 * units_per_px, c + 0.1 * units_per_px, n), np.zeros(n)], 1).astype(np.float32)
+    packed = orc.pack(pos, np.ones((n, 4), np.float32), pointsPerBatch=n)
+    assert packed.batchCount == 1
+    cam = _camera()
+    cfg = scr.ComputeRasteriserConfiguration(enableCLOD=False)
+    ref = orc.render_hqs([oracle_cloud(packed)], cfg, cam.viewMatrix, cam.projectionMatrix, W, H)
+    assert np.count_nonzero(ref["count"]) == 1 and ref["count"].max() == n
+    for ch in ("red", "green", "blue"):
+        assert ref[ch].max() == n * 255
+    cloud = gpu_cloud(ctx, packed)
+    gpu = render_gpu_hqs(ctx, [cloud], cfg, cam, W, H)
+    assert_planes_equal(gpu, ref)
+    ras = scr.ComputeRasteriser(ctx, cfg)
+    ras.addPointCloud(cloud)
+    ras.resize(W, H)
+    ras.render(cam)
+    want = {name: ras.read(p) for name, p in PLANES.items()}
+    mv = scr.MultiViewRasteriser(ctx, [cloud], [(W, H), (W, H)], [0, 0], cfg)
+    mv.render([cam, _camera()], useGraph=False)
+    for v in range(2):
+        for name, p in PLANES.items():
+            assert np.array_equal(mv.read(v, p), want[name]), (v, name)
+    mv.close()

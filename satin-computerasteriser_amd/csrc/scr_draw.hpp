@@ -1106,13 +1106,13 @@ __device__ __forceinline__ void color_tile_flush(const ChunkBox& box, const Pass
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, int kLevel, bool kDisp = false>
+template <bool kR0, bool kWave, int NT, int PPT, int kLevel, bool kDisp = false, int kTile = kColorTile>
 __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                     uint32_t* blue, uint32_t* count, ColorLds LAll,
         const ChunkBox& seg) {
     using G = Group<kWave, NT>;
-    constexpr int kCap = kColorTile / G::kCount;
+    constexpr int kCap = kTile / G::kCount;
     const int slice = G::index() * kCap;
     const ColorLds L = color_lds(LAll.depth + slice, LAll.rgn + slice, LAll.b + slice, LAll.red);
     const bool vec = vec_ok<PPT>(bc);
@@ -1204,18 +1204,18 @@ __device__ __forceinline__ void color_chunks_level(const CloudDev& c, const Pass
     }
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, bool kDisp = false>
+template <bool kR0, bool kWave, int NT, int PPT, bool kDisp = false, int kTile = kColorTile>
 __device__ __forceinline__ void color_chunks(const CloudDev& c, const PassParams& p, const BatchCtx& bc, uint32_t lo,
                                     const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                     uint32_t* blue, uint32_t* count, ColorLds LAll) {
     using G = Group<kWave, NT>;
-    constexpr int kCap = kColorTile / G::kCount;
+    constexpr int kCap = kTile / G::kCount;
     const ChunkBox seg = segment_box(p, bc, bc.active > lo ? (bc.active - lo) / G::kCount : 0u, kCap);
     if (seg.none) return;  // no point of the batch reaches the screen
     // one straight-line copy of the body per position precision
-    if (bc.level == 0) color_chunks_level<kR0, kWave, NT, PPT, 0, kDisp>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
-    else if (bc.level == 1) color_chunks_level<kR0, kWave, NT, PPT, 1, kDisp>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
-    else color_chunks_level<kR0, kWave, NT, PPT, 2, kDisp>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    if (bc.level == 0) color_chunks_level<kR0, kWave, NT, PPT, 0, kDisp, kTile>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    else if (bc.level == 1) color_chunks_level<kR0, kWave, NT, PPT, 1, kDisp, kTile>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
+    else color_chunks_level<kR0, kWave, NT, PPT, 2, kDisp, kTile>(c, p, bc, lo, depth, red, green, blue, count, LAll, seg);
 }
 
 // ---------------------------------------------------------------- nearest pass body
@@ -1374,14 +1374,14 @@ __device__ __forceinline__ void depth_batch(const CloudDev& c, const PassParams&
     else depth_chunks<kR0, kWave, NT, PPT, kTile, kDisp>(c, p, bc, lo, depth, lds, red);
 }
 
-template <bool kR0, bool kWave, int NT, int PPT, bool kDisp = false>
+template <bool kR0, bool kWave, int NT, int PPT, bool kDisp = false, int kTile = kColorTile>
 __device__ __forceinline__ void color_batch(const CloudDev& c, const PassParams& p, const scr_visible_batch& vb,
                                             const DrawRecord* bb, uint32_t lo, uint32_t hi,
                                             const uint32_t* __restrict__ depth, uint32_t* red, uint32_t* green,
                                             uint32_t* blue, uint32_t* count, ColorLds L) {
     const BatchCtx bc = batch_ctx(c, p, vb, bb, hi);
-    if (!kR0 && bc.rconst == 0) color_chunks<true, kWave, NT, PPT, kDisp>(c, p, bc, lo, depth, red, green, blue, count, L);
-    else color_chunks<kR0, kWave, NT, PPT, kDisp>(c, p, bc, lo, depth, red, green, blue, count, L);
+    if (!kR0 && bc.rconst == 0) color_chunks<true, kWave, NT, PPT, kDisp, kTile>(c, p, bc, lo, depth, red, green, blue, count, L);
+    else color_chunks<kR0, kWave, NT, PPT, kDisp, kTile>(c, p, bc, lo, depth, red, green, blue, count, L);
 }
 
 template <bool kR0, bool kWave, int NT, int PPT, int kTile = kNearestTile>

@@ -108,18 +108,23 @@ __global__ __launch_bounds__(NT, draw_waves(PPT)) void k_mv_depth(const MVView* 
                    });
 }
 
+// The multi-view colour tile per workgroup width: the most pixels (16 bytes
+// each) that keep four waves per SIMD within a CU's 160 KB of LDS.
+__host__ __device__ constexpr int mv_color_tile(int nt) { return nt <= 128 ? 1256 : (nt <= 256 ? kColorTile : 5088); }
+
 template <bool kR0, bool kWave, int NT, int PPT>
 __global__ __launch_bounds__(NT, PPT <= 8 ? (kR0 ? 4 : 3) : 2) void k_mv_color(const MVView* __restrict__ views, const MVItem* __restrict__ items,
                                                  uint32_t nUnits, int splits, uint32_t claim, uint32_t* heads) {
-    __shared__ uint32_t ldsDepth[kColorTile];
-    __shared__ unsigned long long ldsRGN[kColorTile];
-    __shared__ uint32_t ldsB[kColorTile];
+    constexpr int kTile = mv_color_tile(NT);
+    __shared__ uint32_t ldsDepth[kTile];
+    __shared__ unsigned long long ldsRGN[kTile];
+    __shared__ uint32_t ldsB[kTile];
     __shared__ int redbuf[5 * kMaxWaves];
     drain<NT, PPT>(views, items, nUnits, splits, claim, heads + 1,
                    [&](const MVView& V, const scr_visible_batch& vb, const DrawRecord* bb, uint32_t lo, uint32_t hi) __attribute__((always_inline)) {
                        const ColorLds L = color_lds(ldsDepth, ldsRGN, ldsB, redbuf);
                        const size_t s = (size_t)V.stride;
-                       color_batch<kR0, kWave, NT, PPT>(V.c, V.p, vb, bb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
+                       color_batch<kR0, kWave, NT, PPT, false, kTile>(V.c, V.p, vb, bb, lo, hi, V.planes, V.planes + s, V.planes + 2 * s,
                                                  V.planes + 3 * s, V.planes + 4 * s, L);
                    });
 }

@@ -1,7 +1,8 @@
 """Batches whose screen rectangle sits on either side of each LDS tile
 capacity: single-view depth 9,984 pixels and nearest 4,992, multi-view depth
-4,096 and nearest 2,048, translucent-defocus depth 4,096, colour 2,528,
-translucent-defocus colour 2,016.  A
+4,096, nearest 2,048 and colour 2,528 (at the default 256 threads),
+translucent-defocus depth 4,096, colour 2,528, translucent-defocus colour
+2,016.  A
 rectangle within a kernel's capacity takes the one-tile-per-batch path, a
 larger one the per-chunk tiles, so one patch size can take different paths in
 the single-view, multi-view and defocus kernels; every one must equal the
